@@ -1,0 +1,220 @@
+"""Oracle driver: writes a SPHERA v10 input deck for a DamBreak3D case and runs
+the flang-built reference harness on it.  Test infrastructure only.
+
+Deck syntax follows the v10 readers (src/Pre_processing/reading_inp/
+ReadInput.f90:102-194 and one reader per section); the section set and the
+values not describing the case (DB-SPH, bed load, monitors) mirror the v10
+3D tutorial deck input/tutorials/release_v_10_0_0/42_dam_break_3D_demo, with
+outputs throttled.  Integers are written without a decimal point (flang's
+list-directed read rejects "0." for an integer).
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REF_DIR = HERE / "_ref"
+HARNESS = REF_DIR / "harness3d.x"
+REFERENCE = REF_DIR / "sphera3d.x"
+
+
+def _f(x):
+    return repr(float(x))
+
+
+def deck_text(case, itmax=1000000000, tmax=1000.0):
+    s = case.scalars()
+    v = case.vertices()
+    lines = ["10.0.0", "##### TITLE #####", case.name, "##### END TITLE #####",
+             "##### DOMAIN #####", "semi", f"{_f(case.dx)} {_f(case.trunc)}", ".false.",
+             "##### END DOMAIN #####", "##### VERTICES #####"]
+    for i, p in enumerate(v, start=1):
+        lines.append(f"{i} {_f(p[0])} {_f(p[1])} {_f(p[2])}")
+    lines += ["##### END VERTICES #####", "##### FACES #####"]
+    for i, r in enumerate(case.faces(), start=1):
+        lines.append(f"{i} {r[0]} {r[1]} {r[2]} {r[3]} -1 -1 {r[4]}")
+    lines += ["##### END FACES #####", "##### BOUNDARIES #####",
+              "Boundary", "1", "fixed", "1 .false.", _f(case.slip_coeff), "111111", "1 8",
+              "Reservoir", "2", "perimeter", "1", "uniform 009EA8",
+              "std 0. 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[2])} 0.", "1 0 .false.",
+              "##### END BOUNDARIES #####",
+              "##### DBSPH #####", "1. .false. 0.5 0 .false.", ".false. .true.", "0 0",
+              "0 .false.", "0 0 6", "##### END DBSPH #####",
+              "##### BED LOAD TRANSPORT #####", "0 1", "1 0. 0.",
+              "##### end BED LOAD TRANSPORT #####",
+              "##### medium #####", "liquid", "1", f"{_f(case.rho0)} {_f(case.bulk)}",
+              f"{_f(case.alfa_mon)} 0.", _f(case.mu), "##### end medium #####",
+              "##### RUN PARAMETERS #####", f"{_f(tmax)} {int(itmax)}",
+              f"{_f(case.cfl)} {_f(case.vsc_coeff)} 1 1 {_f(case.pesodt)} .false.",
+              f"{_f(case.teta_p)} {_f(case.teta_v)} A",
+              f"1.1 {_f(case.coefnmaxpartj)} 0",
+              ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.",
+              f"1000000 {int(case.maxcloseboundfaces)} 100 100000000", "0",
+              "##### end RUN PARAMETERS #####",
+              "##### general physical properties #####",
+              f"{_f(case.grav[0])} {_f(case.grav[1])} {_f(case.grav[2])}", _f(case.p_ref),
+              "##### end general physical properties #####",
+              "##### restart #####", "##### end restart #####",
+              "##### output regulation #####", "results time 100000.",
+              "restart time 100000.", "print partial 100000000", "control time 100000.",
+              "level time 100000. medium 1", "depth dt_out 100000.",
+              "##### end output regulation #####",
+              "##### draw options #####", "vtkconverter any 100000.",
+              "##### end draw options #####",
+              "##### control points #####", "##### end control points #####",
+              "##### control lines #####", "##### end control lines #####",
+              "##### section flow rate #####", "0 999. 1", "##### end section flow rate #####",
+              "##### substations #####", "0 999.", "##### end substations #####"]
+    del s
+    return "\n".join(lines) + "\n"
+
+
+def write_case_dir(case, root, itmax=1000000000):
+    """Lay out a run directory the reference can start from: it concatenates
+    ./input/*/*.inp into <case>.inp (src/Main_algorithm/sphera.f90:222)."""
+    root = Path(root)
+    (root / "input" / "00_deck").mkdir(parents=True, exist_ok=True)
+    (root / "input" / "00_deck" / "deck.inp").write_text(deck_text(case, itmax=itmax))
+    (root / "output").mkdir(exist_ok=True)
+    (root / "restart").mkdir(exist_ok=True)
+    return root
+
+
+def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
+                workdir=None, timeout=3600):
+    """Run the reference harness; returns the path of its dump file."""
+    if not HARNESS.exists():
+        raise FileNotFoundError(f"{HARNESS} not built (make -f oracle/Makefile.ref)")
+    wd = Path(workdir or tempfile.mkdtemp(prefix="sph_ref_")).resolve()
+    write_case_dir(case, wd)
+    env = dict(os.environ)
+    env["SPH_HARNESS_OUT"] = str(wd / "dump.bin")
+    env["SPH_HARNESS_STEPS"] = str(int(nsteps))
+    env["SPH_HARNESS_EVERY"] = str(int(every))
+    env["SPH_HARNESS_LISTS"] = "1" if lists else "0"
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    if ic_coord is not None:
+        icf = wd / "ic.bin"
+        with open(icf, "wb") as fh:
+            fh.write(struct.pack("<i", len(ic_coord)))
+            fh.write(np.ascontiguousarray(ic_coord, dtype="<f8").tobytes())
+        env["SPH_HARNESS_IC"] = str(icf)
+    res = subprocess.run([str(HARNESS), case.name], cwd=wd, env=env, capture_output=True,
+                         text=True, timeout=timeout)
+    if res.returncode != 0 or not (wd / "dump.bin").exists():
+        raise RuntimeError(f"reference harness failed ({res.returncode}):\n{res.stdout[-3000:]}"
+                           f"\n{res.stderr[-3000:]}")
+    return wd / "dump.bin"
+
+
+class _Reader:
+    def __init__(self, buf):
+        self.b = buf
+        self.o = 0
+
+    def arr(self, dt, n):
+        a = np.frombuffer(self.b, dtype=dt, count=n, offset=self.o)
+        self.o += a.nbytes
+        return a.copy()
+
+    def i4(self, n=1):
+        a = self.arr("<i4", n)
+        return int(a[0]) if n == 1 else a
+
+    def f8(self, n=1):
+        a = self.arr("<f8", n)
+        return float(a[0]) if n == 1 else a
+
+    def raw(self, n):
+        s = self.b[self.o:self.o + n]
+        self.o += n
+        return s
+
+    def eof(self):
+        return self.o >= len(self.b)
+
+
+def read_dump(path):
+    """Parse the harness stream (layout: time_step_loop_dump.f90 write_header,
+    write_step).  Arrays are in the reference's pg order; neighbour ids and face
+    ids are the reference's 1-based indices."""
+    r = _Reader(Path(path).read_bytes())
+    if r.i4() != 1213417555:
+        raise ValueError("not a harness dump")
+    hdr = {}
+    ver, nag, nmaxpartj, nmax = r.i4(4)
+    ncd = r.i4(3)
+    hdr.update(version=ver, nag=nag, nmaxpartj=nmaxpartj, nmax=nmax, ncd=ncd)
+    hdr["extr_lo"] = r.f8(3)
+    hdr["extr_hi"] = r.f8(3)
+    hdr["dcd"] = r.f8(3)
+    (hdr["h"], hdr["dx"], hdr["coefke"], hdr["coefkacl"], hdr["prif"]) = r.f8(5)
+    hdr["grav"] = r.f8(3)
+    hdr["eta"], hdr["eta2"], hdr["pvolume"] = r.f8(3)
+    (hdr["den0"], hdr["eps"], hdr["celerita"], hdr["alfamon"], hdr["kin_visc"],
+     hdr["mumx"]) = r.f8(6)
+    hdr["cfl"], hdr["vsc_coeff"], hdr["pesodt"], hdr["tetap"], hdr["tetav"] = r.f8(5)
+    hdr["dt_alfa_mon"], hdr["maxclose"], numfacce, hdr["numbedges"] = r.i4(4)
+    faces = []
+    for _ in range(numfacce):
+        nf = r.i4()
+        if nf == 0:
+            continue
+        nodes, st = r.i4(2)
+        tipo = r.raw(4).decode()
+        vals = r.f8(1 + 9 + 3 + 3)
+        nd = []
+        for _ in range(6):
+            name = r.i4()
+            gx = r.f8(3)
+            lx = r.f8(3)
+            nd.append((name, gx, lx))
+        mode = r.i4()
+        coeff = r.f8()
+        lam = r.i4()
+        faces.append(dict(nf=nf, nodes=nodes, stretch=st, tipo=tipo, area=vals[0],
+                          T=vals[1:10].reshape(3, 3, order="F"), velocity=vals[10:13],
+                          tratto_velocity=vals[13:16], node=nd, slip_mode=mode,
+                          slip_coeff=coeff, laminar_check=lam))
+    hdr["faces"] = faces
+    hdr["gcbf_count"] = r.i4(nmax)
+    hdr["gcbf_start"] = r.i4(nmax)
+    nv = r.i4()
+    hdr["gcbf_vector"] = r.i4(nv) if nv > 0 else np.zeros(0, np.int32)
+    hdr["kerneltab"] = r.f8(41 * 5).reshape(41, 5, order="F")
+    hdr["ktdelta"] = r.f8()
+    hdr["bitab"] = r.f8(256 * 7).reshape(256, 7, order="F")
+    steps = []
+    while not r.eof():
+        if r.i4() != 1347704147:
+            raise ValueError("bad step tag")
+        it, n = r.i4(2)
+        t, dt, dtavg = r.f8(3)
+        st = dict(it=it, nag=n, time=t, dt=dt, dt_average=dtavg)
+        for k in ("coord", "vel", "var", "acc"):
+            st[k] = r.f8(3 * n).reshape(n, 3)
+        for k in ("dens", "pres", "dden", "vpres", "mass"):
+            st[k] = r.f8(n)
+        for k in ("cella", "kodvel", "koddens", "laminar_flag"):
+            st[k] = r.i4(n)
+        lists = r.i4()
+        if lists:
+            cnt = r.i4(n)
+            tot = int(cnt.sum())
+            st["nbr_count"] = cnt
+            st["nbr"] = r.i4(tot) if tot else np.zeros(0, np.int32)
+            ncb = r.i4(n)
+            st["ncbf"] = ncb
+            nb = int(ncb.sum())
+            rec = np.dtype([("face", "<i4"), ("loc", "<f8", 3), ("bi", "<f8", 8),
+                            ("ig", "<f8", 9)])
+            st["btab"] = np.frombuffer(r.raw(nb * rec.itemsize), dtype=rec).copy()
+        steps.append(st)
+    return hdr, steps

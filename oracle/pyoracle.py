@@ -1,0 +1,199 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/sph_oracle.cpp).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_build" / "libsph_oracle.so"
+_lib = None
+
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int32)
+
+
+class OrParams(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nmaxpartj", C.c_int32), ("maxclose", C.c_int32),
+                ("ncd", C.c_int32 * 3), ("extr_lo", C.c_double * 3), ("dcd", C.c_double * 3),
+                ("dx", C.c_double), ("h", C.c_double), ("coefke", C.c_double),
+                ("coefkacl", C.c_double), ("eta", C.c_double), ("eta2", C.c_double),
+                ("grav", C.c_double * 3), ("prif", C.c_double), ("den0", C.c_double),
+                ("eps", C.c_double), ("celerita", C.c_double), ("alfamon", C.c_double),
+                ("kin_visc", C.c_double), ("mumx", C.c_double), ("cfl", C.c_double),
+                ("vsc_coeff", C.c_double), ("pesodt", C.c_double), ("tetap", C.c_double),
+                ("tetav", C.c_double), ("semi", C.c_int32)]
+
+
+class OrFaces(C.Structure):
+    _fields_ = [("nf", C.c_int32), ("T", PD), ("LX", PD), ("GX", PD), ("area", PD),
+                ("velocity", PD), ("tvel", PD), ("slip_coeff", PD), ("nodes", PI),
+                ("btype", PI), ("slip_mode", PI), ("lam_check", PI), ("gcbf_count", PI),
+                ("gcbf_start", PI), ("gcbf_vec", PI), ("kerneltab", PD), ("ktdelta", C.c_double),
+                ("bitab", PD)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} missing: make -f oracle/Makefile")
+        L = C.CDLL(str(LIB_PATH))
+        L.or_create.restype = C.c_void_p
+        L.or_create.argtypes = [C.POINTER(OrParams), C.POINTER(OrFaces), PD, PD, PD, PD, PD]
+        for name in ("or_destroy",):
+            getattr(L, name).argtypes = [C.c_void_p]
+            getattr(L, name).restype = None
+        for name in ("or_nag", "or_ordgrid", "or_calcvarlength", "or_boundary_tab", "or_dt",
+                     "or_momentum", "or_vsmooth", "or_leapfrog_traj", "or_continuity",
+                     "or_leapfrog_cont", "or_eos", "or_psmooth", "or_preloop", "or_overflow"):
+            getattr(L, name).argtypes = [C.c_void_p]
+            getattr(L, name).restype = C.c_int
+        L.or_leapfrog_mom.argtypes = [C.c_void_p, C.c_double]
+        L.or_step.argtypes = [C.c_void_p, C.c_int]
+        L.or_pair_count.argtypes = [C.c_void_p]
+        L.or_pair_count.restype = C.c_int64
+        L.or_set_state.argtypes = [C.c_void_p, C.c_int, PD, PD, PD, PD, PD, PD, PI, C.c_double,
+                                   C.c_double, C.c_double, C.c_int]
+        L.or_get_state.argtypes = [C.c_void_p] + [PD] * 8
+        L.or_get_scalars.argtypes = [C.c_void_p, PD]
+        L.or_get_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
+        L.or_get_btab.argtypes = [C.c_void_p, PI, PI, PD, PD, PD]
+        _lib = L
+    return _lib
+
+
+def _pd(a):
+    return a.ctypes.data_as(PD) if a is not None else None
+
+
+def _pi(a):
+    return a.ctypes.data_as(PI) if a is not None else None
+
+
+def params_from_tables(tb, n, semi=True):
+    s, g, case = tb.scalars, tb.grid, tb.case
+    p = OrParams()
+    p.n = n
+    p.nmaxpartj = s["nmaxpartj"]
+    p.maxclose = case.maxcloseboundfaces
+    p.ncd[:] = [int(v) for v in g.ncd]
+    p.extr_lo[:] = [float(v) for v in g.extr_lo]
+    p.dcd[:] = [float(v) for v in g.dcd]
+    p.dx, p.h, p.coefke, p.coefkacl = case.dx, s["h"], s["coefke"], s["coefkacl"]
+    p.eta, p.eta2 = s["eta"], s["eta2"]
+    p.grav[:] = list(case.grav)
+    p.prif = case.p_ref
+    p.den0, p.eps, p.celerita = case.rho0, case.bulk, s["celerita"]
+    p.alfamon, p.kin_visc, p.mumx = case.alfa_mon, s["kin_visc"], 3.4e38
+    p.cfl, p.vsc_coeff, p.pesodt = case.cfl, case.vsc_coeff, case.pesodt
+    p.tetap, p.tetav = case.teta_p, case.teta_v
+    p.semi = 1 if semi else 0
+    return p
+
+
+class Oracle:
+    """One FP64 oracle context over a built case (sphera_amd.case.build)."""
+
+    def __init__(self, tb, coord=None, vel=None, dens=None, pres=None, mass=None, semi=True):
+        L = lib()
+        P = tb.particles
+        self.coord = np.ascontiguousarray(P["coord"] if coord is None else coord, np.float64)
+        n = len(self.coord)
+        vel = np.ascontiguousarray(P["vel"] if vel is None else vel, np.float64)
+        dens = np.ascontiguousarray(P["dens"] if dens is None else dens, np.float64)
+        pres = np.ascontiguousarray(P["pres"] if pres is None else pres, np.float64)
+        mass = np.ascontiguousarray(P["mass"] if mass is None else mass, np.float64)
+        self.tb = tb
+        self.p = params_from_tables(tb, n, semi)
+        F = tb.faces
+        self._keep = dict(
+            T=np.ascontiguousarray(F.T.transpose(0, 2, 1).reshape(-1)),  # column-major per face
+            LX=np.ascontiguousarray(F.LX.reshape(-1)), GX=np.ascontiguousarray(F.GX.reshape(-1)),
+            area=np.ascontiguousarray(F.area), vel=np.ascontiguousarray(F.velocity.reshape(-1)),
+            tvel=np.ascontiguousarray(F.tratto_velocity.reshape(-1)),
+            slip=np.ascontiguousarray(F.slip_coeff), nodes=np.ascontiguousarray(F.nodes, np.int32),
+            btype=np.ascontiguousarray(F.btype, np.int32),
+            mode=np.ascontiguousarray(F.slip_mode, np.int32),
+            lam=np.ascontiguousarray(F.laminar_check, np.int32),
+            gc=np.ascontiguousarray(tb.gcbf_count, np.int32),
+            gs=np.ascontiguousarray(tb.gcbf_start, np.int32),
+            gv=np.ascontiguousarray(tb.gcbf_vector if len(tb.gcbf_vector) else np.zeros(1, np.int32),
+                                    np.int32),
+            kt=np.ascontiguousarray(tb.kerneltab.reshape(-1)),
+            bit=np.ascontiguousarray(tb.bitab.reshape(-1)))
+        k = self._keep
+        f = OrFaces(len(F.face_id), _pd(k["T"]), _pd(k["LX"]), _pd(k["GX"]), _pd(k["area"]),
+                    _pd(k["vel"]), _pd(k["tvel"]), _pd(k["slip"]), _pi(k["nodes"]), _pi(k["btype"]),
+                    _pi(k["mode"]), _pi(k["lam"]), _pi(k["gc"]), _pi(k["gs"]), _pi(k["gv"]),
+                    _pd(k["kt"]), tb.ktdelta, _pd(k["bit"]))
+        self.ctx = L.or_create(C.byref(self.p), C.byref(f), _pd(self.coord), _pd(vel), _pd(dens),
+                               _pd(pres), _pd(mass))
+
+    def __del__(self):
+        if getattr(self, "ctx", None) and _lib is not None:
+            _lib.or_destroy(self.ctx)
+            self.ctx = None
+
+    @property
+    def n(self):
+        return lib().or_nag(self.ctx)
+
+    def call(self, name, *args):
+        rc = getattr(lib(), "or_" + name)(self.ctx, *args)
+        if rc:
+            raise RuntimeError(f"oracle or_{name} returned {rc}")
+        return rc
+
+    def preloop(self):
+        return self.call("preloop")
+
+    def step(self, k=1):
+        return self.call("step", int(k))
+
+    def set_state(self, st):
+        n = int(st["nag"])
+        a = {k: np.ascontiguousarray(st[k], np.float64) for k in ("coord", "vel", "var", "acc",
+                                                                   "dens", "pres")}
+        lam = np.ascontiguousarray(st["laminar_flag"], np.int32)
+        lib().or_set_state(self.ctx, n, _pd(a["coord"]), _pd(a["vel"]), _pd(a["var"]),
+                           _pd(a["acc"]), _pd(a["dens"]), _pd(a["pres"]), _pi(lam), st["dt"],
+                           st["dt_average"], st["time"], int(st["it"]))
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("coord", "vel", "var", "acc")}
+        out.update({k: np.zeros(n) for k in ("dens", "pres", "dden", "vpres")})
+        lib().or_get_state(self.ctx, *[_pd(out[k]) for k in ("coord", "vel", "var", "acc", "dens",
+                                                              "pres", "dden", "vpres")])
+        sc = np.zeros(4)
+        lib().or_get_scalars(self.ctx, _pd(sc))
+        out.update(dt=sc[0], dt_average=sc[1], time=sc[2], it=int(sc[3]), nag=n)
+        return out
+
+    def lists(self):
+        n = self.n
+        stride = int(self.p.nmaxpartj)
+        cnt = np.zeros(n, np.int32)
+        ids = np.zeros((n, stride), np.int32)
+        if lib().or_get_lists(self.ctx, _pi(cnt), _pi(ids), stride):
+            raise RuntimeError("lists not built")
+        return cnt, ids
+
+    def btab(self):
+        n = self.n
+        mc = int(self.p.maxclose)
+        nc = np.zeros(n, np.int32)
+        face = np.zeros((n, mc), np.int32)
+        loc = np.zeros((n, mc, 3))
+        bi = np.zeros((n, mc, 8))
+        ig = np.zeros((n, mc, 9))
+        lib().or_get_btab(self.ctx, _pi(nc), _pi(face), _pd(loc), _pd(bi), _pd(ig))
+        return nc, face, loc, bi, ig
+
+    def pair_count(self):
+        return int(lib().or_pair_count(self.ctx))

@@ -1,0 +1,226 @@
+! Parity harness for the SPHERA v10 reference (test infrastructure, not product).
+!
+! Project-written replacement for the reference's "time_step_loop"
+! (src/Main_algorithm/time_step_loop.f90:25-510).  It is linked, in place of
+! that unit, with the reference objects compiled by oracle/Makefile.ref, so
+! Gest_Input/Gest_Trans (src/Main_algorithm/Gest_Trans.f90:769) set everything
+! up from a deck exactly as the reference does, and this unit then runs the
+! same per-step call sequence for the branch the dam-break decks use
+! (Domain%tipo=="semi", time_split=1 leapfrog, 3D, no bodies, no KTGF) and
+! writes the FP64 state the hot path produces to a binary stream file.
+!
+! Environment:
+!   SPH_HARNESS_OUT    output file (required)
+!   SPH_HARNESS_STEPS  number of time steps to run (default input_any_t%itmax)
+!   SPH_HARNESS_EVERY  dump every K steps (default 1); step 0 is the state
+!                      after the pre-loop (first lists and boundary table)
+!   SPH_HARNESS_IC     optional file: int32 n, then n*3 float64 coordinates
+!                      (x,y,z per particle, pg order).  They replace the
+!                      generated positions before the pre-loop (jittered ICs).
+!   SPH_HARNESS_LISTS  0 to omit neighbour lists / boundary tables from dumps
+!
+! File layout (little endian, no record markers): see write_header/write_step.
+subroutine time_step_loop
+use I_O_file_module
+use Static_allocation_module
+use Hybrid_allocation_module
+use Dynamic_allocation_module
+use Memory_I_O_interface_module
+implicit none
+logical :: BC_zmax_flag
+integer(4) :: it,npi,nsteps,every,u,ios,n_ic,i_zone,Ncbf_Max,lists
+double precision :: dt_previous_step,dtvel
+double precision,allocatable :: xyz(:)
+character(len=512) :: fout,aux
+
+dtvel = 0.d0
+on_going_time_step = -999
+EpOrdGrid = 0
+SourceFace = 0
+it_eff = it_start
+BC_zmax_flag = .false.
+if ((Domain%tipo/="semi").or.(Domain%time_split/=1)) then
+   write(*,*) "harness: only tipo=semi with leapfrog (time_split=1) is handled"
+   stop 2
+endif
+call get_environment_variable("SPH_HARNESS_OUT",fout)
+if (len_trim(fout)==0) then
+   write(*,*) "harness: SPH_HARNESS_OUT not set"
+   stop 2
+endif
+nsteps = input_any_t%itmax
+call get_environment_variable("SPH_HARNESS_STEPS",aux)
+if (len_trim(aux)>0) read(aux,*) nsteps
+every = 1
+call get_environment_variable("SPH_HARNESS_EVERY",aux)
+if (len_trim(aux)>0) read(aux,*) every
+lists = 1
+call get_environment_variable("SPH_HARNESS_LISTS",aux)
+if (len_trim(aux)>0) read(aux,*) lists
+! Optional replacement of the particle positions (same pg order)
+call get_environment_variable("SPH_HARNESS_IC",aux)
+if (len_trim(aux)>0) then
+   open(newunit=u,file=trim(aux),access="stream",form="unformatted",          &
+      status="old",iostat=ios)
+   if (ios/=0) stop 3
+   read(u) n_ic
+   if (n_ic/=nag) then
+      write(*,*) "harness: IC file has",n_ic," particles, deck has",nag
+      stop 3
+   endif
+   allocate(xyz(3*nag))
+   read(u) xyz
+   close(u)
+   do npi=1,nag
+      pg(npi)%coord(1:3) = xyz(3*npi-2:3*npi)
+      pg(npi)%CoordOld(1:3) = pg(npi)%coord(1:3)
+   enddo
+   deallocate(xyz)
+   call OrdGrid1
+endif
+open(newunit=u,file=trim(fout),access="stream",form="unformatted",            &
+   status="replace")
+call write_header(u)
+! Pre-loop (time_step_loop.f90:82-173 for the semi branch)
+call liquid_particle_ID_array
+call inlet_sections
+call BC_zmax_t0
+call CalcVarLength
+call ComputeBoundaryDataTab_3D
+if (Domain%NormFix) call NormFix
+if (it_start==0) call time_step_duration
+it = it_start
+call write_step(u,it,lists)
+do while (it<nsteps)
+   it = it + 1
+   on_going_time_step = it
+   it_eff = it
+   dt_previous_step = dt
+   if (nag>0) call time_step_duration
+   simulation_time = simulation_time + dt
+   call fluid_particle_imposed_kinematics
+   call RHS_momentum_equation
+   call Leapfrog_momentum(dt_previous_step,dtvel)
+   if (input_any_t%TetaV>1.d-9) call velocity_smoothing(BC_zmax_flag)
+   call velocity_smoothing_2(BC_zmax_flag)
+   call Leapfrog_trajectories
+   if (NumOpenFaces>0) call CancelOutgoneParticles_3D
+   if (SourceFace/=0) then
+      call inlet_sections
+      call GenerateSourceParticles
+   endif
+   call BC_zmax_anyt
+   call OrdGrid1
+   if (Domain%NormFix) call NormFix
+   call CalcVarLength
+   do i_zone=1,NPartZone
+      if (Partz(i_zone)%tipo=="zmax") then
+         BC_zmax_flag = .true.
+         exit
+      endif
+   enddo
+   if (BC_zmax_flag) then
+      call liquid_particle_ID_array
+      call velocity_smoothing(BC_zmax_flag)
+      call velocity_smoothing_2(BC_zmax_flag)
+   endif
+   BC_zmax_flag = .false.
+   call ComputeBoundaryDataTab_3D
+   call liquid_particle_ID_array
+   Ncbf_Max = 0
+!$omp parallel do default(none) shared(nag,pg,Domain,Ncbf_Max) private(npi)
+   do npi=1,nag
+      pg(npi)%dden = 0.d0
+      pg(npi)%dden_ALE12 = 0.d0
+      pg(npi)%dden_ALE12_frozen = 0.d0
+      pg(npi)%koddens = 0
+      if (pg(npi)%state=="flu") call SASPH_continuity(npi,Ncbf_Max)
+      call Continuity_Equation(npi)
+   enddo
+!$omp end parallel do
+   if (Ncbf_Max>input_any_t%MAXCLOSEBOUNDFACES) then
+      write(*,*) "harness: MAXCLOSEBOUNDFACES exceeded",Ncbf_Max
+      stop 4
+   endif
+   call Leapfrog_continuity
+   call EoS_fluid_loop
+   if (input_any_t%TetaP>zero) call PressureSmoothing_3D
+   if (mod(it,every)==0) call write_step(u,it,lists)
+   if (simulation_time>=input_any_t%tmax) exit
+enddo
+close(u)
+return
+
+contains
+
+subroutine write_header(u)
+integer(4),intent(in) :: u
+integer(4) :: kf,nf,st,n,ic
+write(u) 1213417555                                  ! 'SPHH'
+write(u) 1, nag, NMAXPARTJ, Grid%nmax, Grid%ncd(1:3)
+write(u) Grid%extr(1:3,1), Grid%extr(1:3,2), Grid%dcd(1:3)
+write(u) Domain%h, Domain%dx, Domain%coefke, Domain%coefkacl, Domain%prif,  &
+   Domain%grav(1:3), eta, eta2, Domain%PVolume
+write(u) Med(1)%den0, Med(1)%eps, Med(1)%celerita, Med(1)%alfaMon,          &
+   Med(1)%kin_visc, Med(1)%mumx
+write(u) input_any_t%CFL, input_any_t%vsc_coeff, pesodt, input_any_t%TetaP, &
+   input_any_t%TetaV
+ic = 0
+if (dt_alfa_Mon) ic = 1
+write(u) ic, input_any_t%MAXCLOSEBOUNDFACES, NumFacce, NumBEdges
+do kf=1,NumFacce
+   nf = BFaceList(kf)
+   write(u) nf
+   if (nf==0) cycle
+   st = BoundaryFace(nf)%stretch
+   write(u) BoundaryFace(nf)%nodes, st, Tratto(st)%tipo
+   write(u) BoundaryFace(nf)%area, BoundaryFace(nf)%T(1:3,1:3),              &
+      BoundaryFace(nf)%velocity(1:3), Tratto(st)%velocity(1:3)
+   do n=1,MAXFACENODES
+      write(u) BoundaryFace(nf)%Node(n)%name, BoundaryFace(nf)%Node(n)%GX(1:3),&
+         BoundaryFace(nf)%Node(n)%LX(1:3)
+   enddo
+   ic = 0
+   if (Tratto(st)%laminar_no_slip_check) ic = 1
+   write(u) Partz(Tratto(st)%zone)%slip_coefficient_mode,                   &
+      Partz(Tratto(st)%zone)%BC_shear_stress_input, ic
+enddo
+write(u) GCBFPointers(1:Grid%nmax,1), GCBFPointers(1:Grid%nmax,2)
+write(u) size(GCBFVector)
+write(u) GCBFVector(:)
+write(u) kerneltab(0:ktrows,0:ktcols), ktdelta
+write(u) BoundIntegralTab(1:BITrows,1:BITcols)
+end subroutine write_header
+
+subroutine write_step(u,it,lists)
+integer(4),intent(in) :: u,it,lists
+integer(4) :: i,k,nc,ib
+write(u) 1347704147, it, nag                         ! 'STEP'
+write(u) simulation_time, dt, dt_average
+write(u) (pg(i)%coord(1:3),i=1,nag)
+write(u) (pg(i)%vel(1:3),i=1,nag)
+write(u) (pg(i)%var(1:3),i=1,nag)
+write(u) (pg(i)%acc(1:3),i=1,nag)
+write(u) (pg(i)%dens,i=1,nag), (pg(i)%pres,i=1,nag), (pg(i)%dden,i=1,nag),  &
+   (pg(i)%vpres,i=1,nag), (pg(i)%mass,i=1,nag)
+write(u) (pg(i)%cella,i=1,nag), (pg(i)%kodvel,i=1,nag),                     &
+   (pg(i)%koddens,i=1,nag), (pg(i)%laminar_flag,i=1,nag)
+write(u) lists
+if (lists==0) return
+write(u) nPartIntorno(1:nag)
+do i=1,nag
+   write(u) PartIntorno((i-1)*NMAXPARTJ+1:(i-1)*NMAXPARTJ+nPartIntorno(i))
+enddo
+write(u) BoundaryDataPointer(1,1:nag)
+do i=1,nag
+   nc = BoundaryDataPointer(1,i)
+   do k=1,nc
+      ib = BoundaryDataPointer(3,i) + k - 1
+      write(u) BoundaryDataTab(ib)%CloBoNum, BoundaryDataTab(ib)%LocXYZ(1:3),  &
+         BoundaryDataTab(ib)%BoundaryIntegral(1:8),                          &
+         BoundaryDataTab(ib)%IntGiWrRdV(1:3,1:3)
+   enddo
+enddo
+end subroutine write_step
+
+end subroutine time_step_loop

@@ -1,0 +1,1027 @@
+// CPU FP64 restatement of the SPHERA v10 hot path (oracle; test infrastructure).
+// Build: see oracle/Makefile (g++ -O2 -fopenmp -ffp-contract=off, no -march, so
+// no FMA contraction, like the flang reference build).
+//
+// Particle order, loop order and the left-to-right evaluation of every FP
+// expression follow the cited reference lines.  Indices are 0-based here; the
+// reference's 1-based cell ids are kept as such (0 = outside the grid).
+#include "sph_oracle.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+const double GI = (double)9.80665f;  // Static_allocation_module.f90:75 (a default-real literal)
+const double MinTotUnit = 0.95;      // PressureSmoothing_3D.f90:33
+
+struct Ctx {
+    or_params p;
+    // faces
+    int nf = 0;
+    std::vector<double> T, LX, GX, area, fvel, tvel, slip_coeff;
+    std::vector<int> nodes, btype, slip_mode, lam_check;
+    std::vector<int> gcbf_count, gcbf_start, gcbf_vec;
+    double kt[41][5];
+    double ktdelta = 0.0;
+    double bit[256][7];
+    // derived
+    double squareh, doubleh, square_doubleh;
+    int nmax;
+    // state, pg order
+    int n;
+    std::vector<double> coord, vel, var, acc, dens, pres, dden, vpres, mass;
+    std::vector<int> cella, kodvel, koddens, laminar;
+    std::vector<int> id;  // not used by the physics; tracks swap-with-last removals
+    // lists
+    std::vector<int> icont, npartord, nnb, nbr;
+    std::vector<double> rag, pk;
+    // boundary table
+    std::vector<int> bn, bface;
+    std::vector<double> bloc, bbi, big;
+    double dt = 0.0, dt_average = 0.0, time = 0.0;
+    int it = 0;
+    int on_going = -999;
+    int overflow = 0;
+    bool lists_built = false;
+};
+
+inline double Tm(const Ctx& c, int f, int r, int col) { return c.T[9 * f + col * 3 + r]; }
+
+// CellNumber.f90:53-59 (1-based i,j,k)
+inline int cell_number(const Ctx& c, int i, int j, int k) {
+    const int ni = c.p.ncd[0], nj = c.p.ncd[1], nk = c.p.ncd[2];
+    if (i < 1 || i > ni || j < 1 || j > nj || k < 1 || k > nk) return 0;
+    return ((k - 1) * nj + (j - 1)) * ni + i;
+}
+
+// ParticleCellNumber.f90:52-62
+inline int particle_cell(const Ctx& c, const double* x) {
+    const double xp = x[0] - c.p.extr_lo[0];
+    const double yp = x[1] - c.p.extr_lo[1];
+    const double zp = x[2] - c.p.extr_lo[2];
+    const int i = (int)std::ceil(xp / c.p.dcd[0]);
+    const int k = (int)std::ceil(zp / c.p.dcd[2]);
+    const int j = (int)std::ceil(yp / c.p.dcd[1]);
+    return cell_number(c, i, j, k);
+}
+
+// CellIndices.f90:51-63
+inline void cell_indices(const Ctx& c, int nc, int& i, int& j, int& k) {
+    const int ni = c.p.ncd[0], nj = c.p.ncd[1];
+    const int nij = ni * nj;
+    k = (nc - 1) / nij + 1;
+    const int ncij = nc - nij * (k - 1);
+    j = (ncij - 1) / ni + 1;
+    i = ncij - ni * (j - 1);
+}
+
+// LocalNormalCoordinates.f90:55-87 (PX(1:2) used) + IsPointInternal.f90:54-72
+inline bool point_internal(const Ctx& c, int f, double px0, double px1) {
+    const int nodes = c.nodes[f];
+    const int fkod = nodes - 2;
+    static const int iseg[3] = {1, 2, 0};
+    static const int mainod[2][3] = {{0, 1, 2}, {0, 2, 3}};
+    const double DueArea = (double)(3 - fkod) * c.area[f];
+    const double Usu = 1.0 / DueArea;
+    double csi[3];
+    for (int ii = 0; ii < 2; ++ii) {
+        const int jj = iseg[ii];
+        const int kk = iseg[jj];
+        const double* lj = &c.LX[8 * f + 2 * mainod[fkod - 1][jj]];
+        const double* lk = &c.LX[8 * f + 2 * mainod[fkod - 1][kk]];
+        const double xj = lj[0], yj = lj[1], xk = lk[0], yk = lk[1];
+        const double AA = xj * yk - xk * yj;
+        const double BB = yj - yk;
+        const double CC = xk - xj;
+        csi[ii] = (AA + BB * px0 + CC * px1) * Usu;
+    }
+    csi[2] = 1.0 - (csi[0] + csi[1]);
+    if (fkod == 1) {
+        for (int ii = 0; ii < 3; ++ii)
+            if (csi[ii] < 0.0 || csi[ii] > 1.0) return false;
+        return true;
+    }
+    if (fkod == 2) {
+        for (int ii = 0; ii < 2; ++ii)
+            if (csi[ii] < 0.0 || csi[ii] > 1.0) return false;
+        return true;
+    }
+    return false;
+}
+
+// InterpolateTable.f90:58-82 (columns 1..4)
+inline void interpolate_table(const Ctx& c, double xval, double out[4]) {
+    const double deltaX = c.ktdelta;
+    const double xval0 = c.kt[0][0];
+    const int nr0 = (int)((xval - xval0) / deltaX);
+    if (nr0 <= 0) {
+        for (int ic = 0; ic < 4; ++ic) out[ic] = c.kt[0][ic + 1];
+    } else if (nr0 >= 40) {
+        for (int ic = 0; ic < 4; ++ic) out[ic] = c.kt[40][ic + 1];
+    } else {
+        const int nr1 = nr0 + 1;
+        const double csi = (xval - c.kt[nr0][0]) / deltaX;
+        for (int ic = 0; ic < 4; ++ic)
+            out[ic] = c.kt[nr0][ic + 1] + csi * (c.kt[nr1][ic + 1] - c.kt[nr0][ic + 1]);
+    }
+}
+
+// ComputeBoundaryVolumeIntegrals_P0.f90:76-155
+void volume_integrals(const Ctx& c, int f, const double locx[3], double bi[8], double ig[9]) {
+    for (int q = 0; q < 8; ++q) bi[q] = 0.0;
+    for (int q = 0; q < 9; ++q) ig[q] = 0.0;
+    double IntWdV = 0.0, IntdWrm1dV = 0.0, IntGWZrm1dV = 0.0;
+    double IntGWdV[3] = {0.0, 0.0, 0.0};
+    double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double J0 = 0.0, J1 = 0.0, J2 = 0.0, J3 = 0.0;
+    const double zpmin = 0.05 * c.p.h;
+    double LocPi[3] = {locx[0], locx[1], locx[2]};
+    if (LocPi[2] < 0.0) return;
+    if (LocPi[2] < zpmin) LocPi[2] = zpmin;
+    double robpre = 2.1;
+    for (int ipk = 0; ipk < 256; ++ipk) {
+        double LocPj[3];
+        for (int d = 0; d < 3; ++d) LocPj[d] = LocPi[d] + c.bit[ipk][d];
+        if (LocPj[2] <= 0.0) {
+            const double ts = LocPi[2] / (LocPi[2] - LocPj[2]);
+            double PX[3];
+            for (int d = 0; d < 3; ++d) PX[d] = LocPi[d] + (LocPj[d] - LocPi[d]) * ts;
+            PX[2] = 0.0;
+            if (point_internal(c, f, PX[0], PX[1])) {
+                double dist = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    const double dd = PX[d] - LocPi[d];
+                    dist = dist + dd * dd;
+                }
+                dist = std::sqrt(dist);
+                const double da = c.bit[ipk][3];
+                const double rob = dist / c.p.h;
+                if (std::fabs(rob - robpre) > 1.0e-3) {
+                    double iv[4];
+                    interpolate_table(c, rob, iv);
+                    J0 = iv[0] * da;
+                    J1 = iv[1] * da;
+                    J2 = iv[2] * da;
+                    J3 = iv[3] * da;
+                    robpre = rob;
+                }
+                const double R[3] = {c.bit[ipk][4], c.bit[ipk][5], c.bit[ipk][6]};
+                const double RZ2 = R[2] * R[2];
+                IntWdV = IntWdV + J0;
+                IntdWrm1dV = IntdWrm1dV + J1;
+                IntGWZrm1dV = IntGWZrm1dV + RZ2 * J1;
+                for (int sd = 0; sd < 3; ++sd) {
+                    IntGWdV[sd] = IntGWdV[sd] + R[sd] * J2;
+                    for (int sj = sd; sj < 3; ++sj) G[sd][sj] = G[sd][sj] + R[sd] * R[sj] * J3;
+                }
+            }
+        }
+    }
+    G[1][0] = G[0][1];
+    G[2][0] = G[0][2];
+    G[2][1] = G[1][2];
+    if (LocPi[2] == zpmin) IntWdV = point_internal(c, f, LocPi[0], LocPi[1]) ? 0.5 : 0.0;
+    bi[0] = 0.0;
+    bi[1] = IntWdV;
+    bi[2] = IntdWrm1dV;
+    bi[3] = IntGWdV[0];
+    bi[4] = IntGWdV[1];
+    bi[5] = IntGWdV[2];
+    bi[6] = IntGWZrm1dV;
+    bi[7] = 0.0;
+    // IntGiWrRdV(r,c) column-major, as the reference stores it
+    for (int col = 0; col < 3; ++col)
+        for (int r = 0; r < 3; ++r) ig[col * 3 + r] = G[r][col];
+}
+
+// FindCloseBoundaryFaces3D.f90:78-188 + ComputeBoundaryDataTab_3D.f90:62-106
+void boundary_tab_particle(Ctx& c, int npi) {
+    c.bn[npi] = 0;
+    if (c.cella[npi] <= 0) return;
+    const double* x = &c.coord[3 * npi];
+    int nc = particle_cell(c, x);
+    if (nc <= 0) return;
+    int ic, jc, kc;
+    cell_indices(c, nc, ic, jc, kc);
+    const int maxc = c.p.maxclose;
+    int* clob = &c.bface[(size_t)npi * maxc];
+    double* locx = &c.bloc[(size_t)npi * maxc * 3];
+    int ncbf = 0;
+    for (int i = ic - 1; i <= ic + 1; ++i)
+        for (int j = jc - 1; j <= jc + 1; ++j)
+            for (int k = kc - 1; k <= kc + 1; ++k) {
+                const int cn = cell_number(c, i, j, k);
+                if (cn == 0) continue;
+                const int nfc = c.gcbf_count[cn - 1];
+                if (nfc <= 0) continue;
+                const int ini = c.gcbf_start[cn - 1] - 1;
+                for (int fl = ini; fl < ini + nfc; ++fl) {
+                    const int f = c.gcbf_vec[fl];
+                    double pinmin = 0.0;
+                    const double pinmax = c.doubleh;
+                    const int bt = c.btype[f];
+                    if (bt == 3) pinmin = -c.doubleh;
+                    bool there = false;
+                    for (int q = 0; q < ncbf; ++q)
+                        if (clob[q] == f) there = true;
+                    if (there) continue;
+                    const int nn = c.nodes[f];
+                    const double* ref = &c.GX[12 * f + 3 * (nn - 1)];
+                    double PX[3];
+                    for (int sdi = 0; sdi < 3; ++sdi) {
+                        double s = 0.0;
+                        for (int sdj = 0; sdj < 3; ++sdj) s = s + Tm(c, f, sdj, sdi) * (x[sdj] - ref[sdj]);
+                        PX[sdi] = s;
+                    }
+                    const double pin = PX[2];
+                    if (pin >= pinmin && pin < pinmax) {
+                        const bool needs_in = (bt == 3 || bt == 4 || bt == 5);
+                        if (!needs_in || point_internal(c, f, PX[0], PX[1])) {
+                            ncbf = ncbf + 1;
+                            if (ncbf > maxc) { c.bn[npi] = -1; return; }
+                            clob[ncbf - 1] = f;
+                            locx[3 * (ncbf - 1) + 2] = pin;
+                        }
+                    }
+                }
+            }
+    for (int q = 0; q < ncbf; ++q) {
+        const int f = clob[q];
+        const int nn = c.nodes[f];
+        const double* ref = &c.GX[12 * f + 3 * (nn - 1)];
+        for (int sdi = 0; sdi < 2; ++sdi) {
+            double s = 0.0;
+            for (int sdj = 0; sdj < 3; ++sdj) s = s + Tm(c, f, sdj, sdi) * (x[sdj] - ref[sdj]);
+            locx[3 * q + sdi] = s;
+        }
+    }
+    c.bn[npi] = ncbf;
+    for (int q = 0; q < ncbf; ++q)
+        volume_integrals(c, clob[q], &locx[3 * q], &c.bbi[((size_t)npi * maxc + q) * 8],
+                         &c.big[((size_t)npi * maxc + q) * 9]);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* or_create(const or_params* p, const or_faces* f, const double* coord, const double* vel,
+                const double* dens, const double* pres, const double* mass) {
+    Ctx* c = new Ctx();
+    c->p = *p;
+    c->n = p->n;
+    const int n = p->n;
+    c->squareh = p->h * p->h;
+    c->doubleh = 2.0 * p->h;
+    c->square_doubleh = c->doubleh * c->doubleh;
+    c->nmax = p->ncd[0] * p->ncd[1] * p->ncd[2];
+    if (f) {
+        c->nf = f->nf;
+        const int nf = f->nf;
+        c->T.assign(f->T, f->T + 9 * nf);
+        c->LX.assign(f->LX, f->LX + 8 * nf);
+        c->GX.assign(f->GX, f->GX + 12 * nf);
+        c->area.assign(f->area, f->area + nf);
+        c->fvel.assign(f->velocity, f->velocity + 3 * nf);
+        c->tvel.assign(f->tvel, f->tvel + 3 * nf);
+        c->slip_coeff.assign(f->slip_coeff, f->slip_coeff + nf);
+        c->nodes.assign(f->nodes, f->nodes + nf);
+        c->btype.assign(f->btype, f->btype + nf);
+        c->slip_mode.assign(f->slip_mode, f->slip_mode + nf);
+        c->lam_check.assign(f->lam_check, f->lam_check + nf);
+        c->gcbf_count.assign(f->gcbf_count, f->gcbf_count + c->nmax);
+        c->gcbf_start.assign(f->gcbf_start, f->gcbf_start + c->nmax);
+        int nv = 0;
+        for (int i = 0; i < c->nmax; ++i) nv += c->gcbf_count[i];
+        c->gcbf_vec.assign(f->gcbf_vec, f->gcbf_vec + nv);
+        std::memcpy(c->kt, f->kerneltab, sizeof(c->kt));
+        c->ktdelta = f->ktdelta;
+        std::memcpy(c->bit, f->bitab, sizeof(c->bit));
+    } else {
+        c->gcbf_count.assign(c->nmax, 0);
+        c->gcbf_start.assign(c->nmax, 0);
+    }
+    c->coord.assign(coord, coord + 3 * n);
+    c->vel.assign(vel, vel + 3 * n);
+    c->var.assign(vel, vel + 3 * n);
+    c->acc.assign(3 * n, 0.0);
+    c->dens.assign(dens, dens + n);
+    c->pres.assign(pres, pres + n);
+    c->mass.assign(mass, mass + n);
+    c->dden.assign(n, 0.0);
+    c->vpres.assign(n, 0.0);
+    c->cella.assign(n, 0);
+    c->kodvel.assign(n, 0);
+    c->koddens.assign(n, 0);
+    c->laminar.assign(n, 0);
+    c->id.resize(n);
+    for (int i = 0; i < n; ++i) {
+        c->id[i] = i;
+        c->cella[i] = particle_cell(*c, &c->coord[3 * i]);  // SetParticleParameters.f90:129
+    }
+    or_ordgrid(c);
+    return c;
+}
+
+void or_destroy(void* ctx) { delete (Ctx*)ctx; }
+int or_nag(void* ctx) { return ((Ctx*)ctx)->n; }
+
+void or_set_state(void* ctx, int n, const double* coord, const double* vel, const double* var,
+                  const double* acc, const double* dens, const double* pres, const int32_t* laminar,
+                  double dt, double dt_average, double time, int it) {
+    Ctx& c = *(Ctx*)ctx;
+    c.n = n;
+    c.coord.assign(coord, coord + 3 * n);
+    c.vel.assign(vel, vel + 3 * n);
+    c.var.assign(var, var + 3 * n);
+    c.acc.assign(acc, acc + 3 * n);
+    c.dens.assign(dens, dens + n);
+    c.pres.assign(pres, pres + n);
+    c.laminar.assign(laminar, laminar + n);
+    c.dt = dt;
+    c.dt_average = dt_average;
+    c.time = time;
+    c.it = it;
+    c.on_going = it;
+    c.cella.resize(n);
+    for (int i = 0; i < n; ++i) c.cella[i] = particle_cell(c, &c.coord[3 * i]);
+    or_ordgrid(ctx);
+}
+
+void or_get_state(void* ctx, double* coord, double* vel, double* var, double* acc, double* dens,
+                  double* pres, double* dden, double* vpres) {
+    Ctx& c = *(Ctx*)ctx;
+    const size_t n = c.n;
+    if (coord) std::memcpy(coord, c.coord.data(), 24 * n);
+    if (vel) std::memcpy(vel, c.vel.data(), 24 * n);
+    if (var) std::memcpy(var, c.var.data(), 24 * n);
+    if (acc) std::memcpy(acc, c.acc.data(), 24 * n);
+    if (dens) std::memcpy(dens, c.dens.data(), 8 * n);
+    if (pres) std::memcpy(pres, c.pres.data(), 8 * n);
+    if (dden) std::memcpy(dden, c.dden.data(), 8 * n);
+    if (vpres) std::memcpy(vpres, c.vpres.data(), 8 * n);
+}
+
+void or_get_scalars(void* ctx, double* out) {
+    Ctx& c = *(Ctx*)ctx;
+    out[0] = c.dt;
+    out[1] = c.dt_average;
+    out[2] = c.time;
+    out[3] = c.it;
+}
+
+int or_get_lists(void* ctx, int32_t* count, int32_t* ids, int32_t stride) {
+    Ctx& c = *(Ctx*)ctx;
+    if (!c.lists_built) return 1;
+    const int mj = c.p.nmaxpartj;
+    for (int i = 0; i < c.n; ++i) {
+        count[i] = c.nnb[i];
+        if (ids)
+            for (int k = 0; k < c.nnb[i] && k < stride; ++k) ids[(size_t)i * stride + k] = c.nbr[(size_t)i * mj + k];
+    }
+    return 0;
+}
+
+int or_get_btab(void* ctx, int32_t* ncbf, int32_t* face, double* loc, double* bi, double* ig) {
+    Ctx& c = *(Ctx*)ctx;
+    const int mc = c.p.maxclose;
+    for (int i = 0; i < c.n; ++i) {
+        ncbf[i] = c.bn[i];
+        for (int q = 0; q < c.bn[i]; ++q) {
+            const size_t s = (size_t)i * mc + q;
+            if (face) face[s] = c.bface[s];
+            if (loc) std::memcpy(&loc[3 * s], &c.bloc[3 * s], 24);
+            if (bi) std::memcpy(&bi[8 * s], &c.bbi[8 * s], 64);
+            if (ig) std::memcpy(&ig[9 * s], &c.big[9 * s], 72);
+        }
+    }
+    return 0;
+}
+
+// OrdGrid1.f90:58-105 (fluid part), with the swap-with-last removal of
+// particles outside the grid (:62-83)
+int or_ordgrid(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    const int nmax = c.nmax;
+    c.icont.assign(nmax + 2, 0);
+    std::vector<int> inc(nmax + 1, 0);
+    int npi = 0;
+    auto move_last = [&](int dst, int src) {
+        for (int d = 0; d < 3; ++d) {
+            c.coord[3 * dst + d] = c.coord[3 * src + d];
+            c.vel[3 * dst + d] = c.vel[3 * src + d];
+            c.var[3 * dst + d] = c.var[3 * src + d];
+            c.acc[3 * dst + d] = c.acc[3 * src + d];
+        }
+        c.dens[dst] = c.dens[src];
+        c.pres[dst] = c.pres[src];
+        c.mass[dst] = c.mass[src];
+        c.dden[dst] = c.dden[src];
+        c.vpres[dst] = c.vpres[src];
+        c.cella[dst] = c.cella[src];
+        c.kodvel[dst] = c.kodvel[src];
+        c.koddens[dst] = c.koddens[src];
+        c.laminar[dst] = c.laminar[src];
+        c.id[dst] = c.id[src];
+    };
+    while (c.n > 0 && npi < c.n) {
+        npi = npi + 1;
+        const int i = npi - 1;
+        const int ncel = particle_cell(c, &c.coord[3 * i]);
+        if (c.cella[i] <= 0 || ncel <= 0 || ncel > nmax) {
+            move_last(i, c.n - 1);
+            c.n = c.n - 1;
+            npi = npi - 1;
+        } else {
+            c.icont[ncel] += 1;
+            c.cella[i] = ncel;
+            c.icont[nmax + 1] += 1;
+        }
+    }
+    c.coord.resize(3 * c.n);
+    c.vel.resize(3 * c.n);
+    c.var.resize(3 * c.n);
+    c.acc.resize(3 * c.n);
+    c.dens.resize(c.n);
+    c.pres.resize(c.n);
+    c.mass.resize(c.n);
+    c.dden.resize(c.n);
+    c.vpres.resize(c.n);
+    c.cella.resize(c.n);
+    c.kodvel.resize(c.n);
+    c.koddens.resize(c.n);
+    c.laminar.resize(c.n);
+    c.id.resize(c.n);
+    for (int i = nmax; i >= 1; --i) c.icont[i] = c.icont[i + 1] - c.icont[i];
+    for (int i = 1; i <= nmax + 1; ++i) c.icont[i] = c.icont[i] + 1;
+    c.npartord.assign(c.n + 1, 0);
+    for (int i = 0; i < c.n; ++i) {
+        const int ncel = c.cella[i];
+        if (ncel == 0) continue;
+        c.npartord[c.icont[ncel] + inc[ncel]] = i;  // 1-based slot, 0-based particle
+        inc[ncel] += 1;
+    }
+    c.lists_built = false;
+    return 0;
+}
+
+// CalcVarLength.f90:111-269 (fluid-fluid part: lists, rag, PartKernel)
+int or_calcvarlength(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    const int n = c.n;
+    const int mj = c.p.nmaxpartj;
+    c.nnb.assign(n, 0);
+    if ((int)c.nbr.size() < n * mj) {
+        c.nbr.resize((size_t)n * mj);
+        c.rag.resize((size_t)n * mj * 3);
+        c.pk.resize((size_t)n * mj * 4);
+    }
+    const double ke_coef = c.p.coefke / c.p.h;
+    const double kacl_coef = c.p.coefkacl / c.p.h;
+    int overflow = 0;
+#pragma omp parallel for schedule(static) reduction(+ : overflow)
+    for (int npi = 0; npi < n; ++npi) {
+        const int nceli = c.cella[npi];
+        if (nceli == 0) continue;
+        int ig, jg, kg;
+        cell_indices(c, nceli, ig, jg, kg);
+        const double* xi = &c.coord[3 * npi];
+        int cnt = 0;
+        bool ov = false;
+        for (int jr = jg - 1; jr <= jg + 1; ++jr)
+            for (int ir = ig - 1; ir <= ig + 1; ++ir)
+                for (int kr = kg - 1; kr <= kg + 1; ++kr) {
+                    const int ncelj = cell_number(c, ir, jr, kr);
+                    if (ncelj == 0) continue;
+                    for (int mm = c.icont[ncelj]; mm <= c.icont[ncelj + 1] - 1; ++mm) {
+                        if (cnt >= mj) { ov = true; continue; }
+                        const int npj = c.npartord[mm];
+                        const double* xj = &c.coord[3 * npj];
+                        const double r0 = xi[0] - xj[0];
+                        const double r1 = xi[1] - xj[1];
+                        const double r2 = xi[2] - xj[2];
+                        double rij = r0 * r0 + r1 * r1 + r2 * r2;
+                        if (rij > c.square_doubleh) continue;
+                        cnt = cnt + 1;
+                        const size_t s = (size_t)npi * mj + (cnt - 1);
+                        c.nbr[s] = npj;
+                        const double rij2 = rij;
+                        rij = std::sqrt(rij);
+                        const double q = rij / c.p.h;
+                        const double q2 = rij2 / c.squareh;
+                        const int iq = (int)q;
+                        const double denom = 1.0 / (rij + c.p.eta);
+                        c.rag[3 * s] = r0;
+                        c.rag[3 * s + 1] = r1;
+                        c.rag[3 * s + 2] = r2;
+                        double* K = &c.pk[4 * s];
+                        K[0] = K[1] = K[2] = K[3] = 0.0;
+                        if (iq >= 2) continue;
+                        double gm = -2.0 * q + 1.5 * q2;
+                        double gw = -12.0 - 3.0 * q2 + 12.0 * q;
+                        double wu = 0.666666667 + q2 * (q * 0.5 - 1.0);
+                        if (iq > 0) {
+                            gm = -gm + q2 - 2.0;
+                            wu = (2.0 - q) * (2.0 - q) * (2.0 - q) * 0.166666666667;
+                        }
+                        gm = gm * ke_coef;
+                        gw = gw * kacl_coef;
+                        K[0] = gm * denom;
+                        K[1] = K[0] / (rij2 + c.p.eta2);
+                        K[2] = gw * denom;
+                        K[3] = wu * c.p.coefke;
+                    }
+                }
+        c.nnb[npi] = cnt;
+        if (ov) overflow += 1;
+    }
+    c.overflow = overflow;
+    c.lists_built = true;
+    return 0;
+}
+
+int or_boundary_tab(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    const int n = c.n;
+    const size_t mc = c.p.maxclose;
+    c.bn.assign(n, 0);
+    if (c.bface.size() < n * mc) {
+        c.bface.resize(n * mc);
+        c.bloc.resize(n * mc * 3);
+        c.bbi.resize(n * mc * 8);
+        c.big.resize(n * mc * 9);
+    }
+    if (!c.p.semi || c.nf == 0) return 0;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad)
+    for (int npi = 0; npi < n; ++npi) {
+        boundary_tab_particle(c, npi);
+        if (c.bn[npi] < 0) bad += 1;
+    }
+    return bad ? 86 : 0;  // diagnostic(8,6/7): too many close faces
+}
+
+// time_step_duration.f90:59-123 (liquid medium, dt_alfa_Mon=.false.)
+int or_dt(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    double dtmin = 3.4e38, dt_vis = 3.4e38;
+    if (c.n == 0) {
+        c.dt = 0.0;
+    } else {
+        if (c.p.kin_visc > 1.0e-24) {
+            dt_vis = c.p.vsc_coeff * c.squareh / (0.5 * c.p.kin_visc);
+            dtmin = std::fmin(dtmin, dt_vis);
+        }
+        for (int i = 0; i < c.n; ++i) {
+            const double* v = &c.vel[3 * i];
+            const double U = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const double dt_cfl = c.p.cfl * 2.0 * c.p.h / (c.p.celerita + U);
+            dtmin = std::fmin(dtmin, std::fmin(dt_cfl, dt_vis));
+        }
+        if (c.on_going == 0)
+            c.dt = dtmin;
+        else
+            c.dt = (1.0 - c.p.pesodt) * dtmin + c.p.pesodt * c.dt_average;
+    }
+    if (c.on_going == 0)
+        c.dt_average = c.dt;
+    else
+        c.dt_average = (c.dt_average * (double)(c.on_going - 1) + c.dt) / (double)c.on_going;
+    return 0;
+}
+
+// RHS_momentum_equation.f90:72-179 -> AddBoundaryContributions_to_ME3D.f90:
+// 118-336, inter_EqMoto.f90:108-283, viscomon.f90:54-71, viscomorris.f90:
+// 60-71, AddElasticBoundaryReaction_3D.f90:67-98
+int or_momentum(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    const int n = c.n;
+    const int mj = c.p.nmaxpartj;
+    const size_t mc = c.p.maxclose;
+    const double h = c.p.h;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+    for (int npi = 0; npi < n; ++npi) {
+        double tp[3] = {0, 0, 0}, td[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
+        c.kodvel[npi] = 0;
+        const int ncbf = c.p.semi ? c.bn[npi] : 0;
+        const double* vi = &c.var[3 * npi];
+        // --- SA-SPH wall terms (called before the inner sums) ---
+        if (ncbf > 0) {
+            double ALEt[3] = {0, 0, 0}, gradpt[3] = {0, 0, 0}, VM[3] = {0, 0, 0}, VS[3] = {0, 0, 0};
+            for (int q = 0; q < ncbf; ++q) {
+                const size_t s = (size_t)npi * mc + q;
+                const int f = c.bface[s];
+                const int bt = c.btype[f];
+                if (bt == 2) continue;
+                if (bt != 0 && bt != 1) { bad += 1; break; }
+                const double* L = &c.bloc[3 * s];
+                if (!(L[2] > 0.0)) continue;
+                const double* BI = &c.bbi[8 * s];
+                const double* IG = &c.big[9 * s];
+                double dvij[3], dvn = 0.0;
+                for (int sd = 0; sd < 3; ++sd) {
+                    dvij[sd] = 2.0 * (vi[sd] - c.fvel[3 * f + sd]);
+                    dvn = dvn + Tm(c, f, sd, 2) * dvij[sd];
+                }
+                double gl[3];
+                for (int sd = 0; sd < 3; ++sd) {
+                    double a = 0.0;
+                    for (int sj = 0; sj < 3; ++sj) a = a + Tm(c, f, sj, sd) * c.p.grav[sj];
+                    gl[sd] = a;
+                }
+                double gloc[3], gglo[3];
+                for (int r = 0; r < 3; ++r) {
+                    double a = 0.0;
+                    for (int k = 0; k < 3; ++k) a = a + IG[k * 3 + r] * gl[k];
+                    gloc[r] = a;
+                }
+                for (int r = 0; r < 3; ++r) {
+                    double a = 0.0;
+                    for (int k = 0; k < 3; ++k) a = a + Tm(c, f, r, k) * gloc[k];
+                    gglo[r] = a;
+                }
+                for (int d = 0; d < 3; ++d) gradpt[d] = gradpt[d] + gglo[d];
+                double aloc[3], aglo[3];
+                for (int d = 0; d < 3; ++d) aloc[d] = 2.0 * c.pres[npi] / c.dens[npi] * BI[3 + d];
+                for (int r = 0; r < 3; ++r) {
+                    double a = 0.0;
+                    for (int k = 0; k < 3; ++k) a = a + Tm(c, f, r, k) * aloc[k];
+                    aglo[r] = a;
+                }
+                for (int d = 0; d < 3; ++d) ALEt[d] = ALEt[d] + aglo[d];
+                const double IntGWZ = BI[6];
+                const double IntdW = BI[2];
+                if (c.p.alfamon > 0.0) {
+                    const double Monvisc = c.p.alfamon * c.p.celerita * h;
+                    const double Mmult = -Monvisc * dvn * IntGWZ;
+                    for (int d = 0; d < 3; ++d) VM[d] = VM[d] + Mmult * Tm(c, f, d, 2);
+                }
+                double slip = 0.0, cinv = 0.0;
+                if (c.slip_mode[f] == 1) {
+                    slip = c.slip_coeff[f];
+                    cinv = c.p.kin_visc;
+                }
+                if (cinv > 0.0) {
+                    if (c.laminar[npi] == 1 || c.lam_check[f] == 0) {
+                        const double cm = cinv * IntdW * slip;
+                        for (int d = 0; d < 3; ++d) {
+                            const double dm = 2.0 * (vi[d] - c.fvel[3 * f + d]);
+                            VS[d] = VS[d] + cm * dm;
+                        }
+                    }
+                }
+            }
+            for (int d = 0; d < 3; ++d) {
+                tp[d] = tp[d] + ALEt[d] + gradpt[d];
+                td[d] = td[d] - VM[d];
+                tv[d] = tv[d] + VS[d];
+            }
+        }
+        // --- inner sums ---
+        const int nn = c.lists_built ? c.nnb[npi] : 0;
+        const double rhoi = c.dens[npi], pi = c.pres[npi];
+        const double* vel_i = &c.vel[3 * npi];
+        double tpa[3] = {0, 0, 0}, ale[3] = {0, 0, 0}, rvs[3] = {0, 0, 0};
+        for (int k = 0; k < nn; ++k) {
+            const size_t s = (size_t)npi * mj + k;
+            const int npj = c.nbr[s];
+            if (npj == npi) continue;
+            const double rhoj = c.dens[npj], amassj = c.mass[npj], pj = c.pres[npj];
+            const double* vj = &c.vel[3 * npj];
+            const double dv[3] = {vj[0] - vel_i[0], vj[1] - vel_i[1], vj[2] - vel_i[2]};
+            const double* R = &c.rag[3 * s];
+            const double* K = &c.pk[4 * s];
+            double alpha = (pj - pi) / (rhoi * rhoj);
+            for (int d = 0; d < 3; ++d) {
+                const double ap = -amassj * alpha * R[d] * K[2];
+                tpa[d] = tpa[d] + ap;
+            }
+            alpha = (pi * (rhoj / rhoi + 1.0) + pj * (rhoi / rhoj - 1.0)) / (rhoi * rhoj);
+            for (int d = 0; d < 3; ++d) ale[d] = ale[d] - amassj * alpha * R[d] * K[2];
+            // viscomon
+            const double vrij = -dv[0] * R[0] - dv[1] * R[1] - dv[2] * R[2];
+            const double rhot = rhoi + rhoj;
+            const double celt = c.p.celerita + c.p.celerita;
+            const double TermMon = c.p.alfamon * celt * h / rhot;
+            for (int d = 0; d < 3; ++d) {
+                const double ra = amassj * TermMon * vrij * R[d] * K[1];
+                td[d] = td[d] + (ra + 0.0);
+            }
+            // viscomorris
+            if (c.p.kin_visc > 1.0e-12 && c.p.kin_visc > 1.0e-12) {
+                const double nui = c.p.kin_visc, nuj = c.p.kin_visc;
+                const double rtil = nui * rhoi + nuj * rhoj + 1.0e-3;
+                const double atil = 4.0 * (nui * nuj);
+                const double fac = amassj * atil / rtil;
+                const double rr = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+                for (int d = 0; d < 3; ++d) {
+                    const double rv = fac * (-dv[d] * K[1] * rr);
+                    tv[d] = tv[d] + rv;
+                    rvs[d] = rvs[d] + rv;
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d) tp[d] = tp[d] + tpa[d] + ale[d];
+        int lam = 0;
+        if (c.p.kin_visc > 0.0) {
+            const double apg = std::sqrt(tp[0] * tp[0] + tp[1] * tp[1] + tp[2] * tp[2]) + GI;
+            const double amo = std::sqrt(rvs[0] * rvs[0] + rvs[1] * rvs[1] + rvs[2] * rvs[2]);
+            double w = 0.0;
+            if (apg != 0.0) w = amo / apg * 100.0;
+            lam = (w > 5.0 || apg == 0.0) ? 1 : 0;
+        }
+        c.laminar[npi] = lam;
+        double* a = &c.acc[3 * npi];
+        if (ncbf > 0) {
+            // AddElasticBoundaryReaction_3D (faces part; no convex edges)
+            double br[3] = {0, 0, 0};
+            const double zimin = 0.25 * c.p.dx;
+            const double celer02 = c.p.eps / c.p.den0;
+            for (int q = 0; q < ncbf; ++q) {
+                const size_t s = (size_t)npi * mc + q;
+                const int f = c.bface[s];
+                const int bt = c.btype[f];
+                if (bt != 0 && bt != 1) continue;
+                const double* L = &c.bloc[3 * s];
+                const double zi = L[2];
+                if (zi < zimin && point_internal(c, f, L[0], L[1])) {
+                    double vin = 0.0;
+                    for (int sd = 0; sd < 3; ++sd) vin = vin + vi[sd] * Tm(c, f, sd, 2);
+                    if (vin < 0.0) {
+                        const double nr = -1.0 * celer02 * std::log((h + zi - zimin) / h) / h;
+                        for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * Tm(c, f, d, 2);
+                    }
+                }
+            }
+            for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + c.p.grav[d] + br[d];
+        } else {
+            for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + c.p.grav[d];
+        }
+    }
+    return bad ? 93 : 0;
+}
+
+// Leapfrog_momentum.f90:66-86
+int or_leapfrog_mom(void* ctx, double dt_previous) {
+    Ctx& c = *(Ctx*)ctx;
+    const double dtvel = 0.5 * (c.dt + dt_previous);
+    for (int i = 0; i < c.n; ++i)
+        for (int d = 0; d < 3; ++d) c.vel[3 * i + d] = c.vel[3 * i + d] + dtvel * c.acc[3 * i + d];
+    return 0;
+}
+
+// velocity_smoothing.f90:74-133, velocity_smoothing_SA_SPH_3D.f90:64-122,
+// velocity_smoothing_2.f90:59-103
+int or_vsmooth(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    const int n = c.n;
+    const int mj = c.p.nmaxpartj;
+    const size_t mc = c.p.maxclose;
+    std::vector<double> nv(3 * (size_t)n);
+    if (c.p.tetav > 1.0e-9) {
+#pragma omp parallel for schedule(static)
+        for (int npi = 0; npi < n; ++npi) {
+            double v[3] = {0.0, 0.0, 0.0};
+            const double* vel_i = &c.vel[3 * npi];
+            for (int k = 0; k < c.nnb[npi]; ++k) {
+                const size_t s = (size_t)npi * mj + k;
+                const int npj = c.nbr[s];
+                const double pes = c.mass[npj] * c.pk[4 * s + 3] / c.dens[npj];
+                for (int d = 0; d < 3; ++d) v[d] = v[d] + (c.vel[3 * npj + d] - vel_i[d]) * pes;
+            }
+            const int ncbf = c.p.semi ? c.bn[npi] : 0;
+            for (int q = 0; q < ncbf; ++q) {
+                const size_t s = (size_t)npi * mc + q;
+                const int f = c.bface[s];
+                const int bt = c.btype[f];
+                if (bt == 3 || bt == 4 || bt == 5) {
+                    v[0] = v[1] = v[2] = 0.0;
+                    break;
+                }
+                double DVG[3], DVL[3];
+                for (int d = 0; d < 3; ++d) DVG[d] = 2.0 * (c.tvel[3 * f + d] - vel_i[d]);
+                for (int i = 0; i < 3; ++i) {
+                    double a = 0.0;
+                    for (int j = 0; j < 3; ++j) a = a + Tm(c, f, j, i) * DVG[j];
+                    DVL[i] = a;
+                }
+                const double IntWdV = c.bbi[8 * s + 1];
+                if (bt == 0 || bt == 1) {
+                    double slip = 0.0;
+                    if (c.slip_mode[f] == 1) slip = c.slip_coeff[f];
+                    double BCL[3];
+                    BCL[0] = DVL[0] * IntWdV * slip;
+                    BCL[1] = DVL[1] * IntWdV * slip;
+                    BCL[2] = DVL[2] * IntWdV;
+                    for (int i = 0; i < 3; ++i) {
+                        double a = 0.0;
+                        for (int j = 0; j < 3; ++j) a = a + Tm(c, f, i, j) * BCL[j];
+                        v[i] = v[i] + a;
+                    }
+                }
+            }
+            for (int d = 0; d < 3; ++d) nv[3 * npi + d] = v[d];
+        }
+        const double tv1 = c.p.tetav * c.p.celerita * c.dt / c.p.h;
+        for (int i = 0; i < n; ++i) {
+            if (c.kodvel[i] == 0) {
+                for (int d = 0; d < 3; ++d) {
+                    c.var[3 * i + d] = c.vel[3 * i + d] + tv1 * nv[3 * i + d];
+                    c.vel[3 * i + d] = c.var[3 * i + d];
+                }
+            } else {
+                for (int d = 0; d < 3; ++d) c.var[3 * i + d] = c.vel[3 * i + d];
+            }
+        }
+    } else {
+        c.var = c.vel;
+    }
+    return 0;
+}
+
+// Leapfrog_trajectories.f90:48-76
+int or_leapfrog_traj(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    for (int i = 0; i < c.n; ++i) {
+        if (c.cella[i] <= 0) continue;
+        for (int d = 0; d < 3; ++d) c.coord[3 * i + d] = c.coord[3 * i + d] + c.dt * c.vel[3 * i + d];
+    }
+    return 0;
+}
+
+// time_step_loop.f90:338-360: SASPH_continuity.f90:89-169 with
+// AddBoundaryContribution_to_CE3D.f90:82-169, then Continuity_Equation.f90:97-249
+int or_continuity(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    const int n = c.n;
+    const int mj = c.p.nmaxpartj;
+    const size_t mc = c.p.maxclose;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+    for (int npi = 0; npi < n; ++npi) {
+        double dd = 0.0;
+        c.koddens[npi] = 0;
+        const double* vi = &c.var[3 * npi];
+        const int ncbf = c.p.semi ? c.bn[npi] : 0;
+        if (ncbf > 0) {
+            double gu = 0.0, gv = 0.0, gw = 0.0;  // grad_u_SA(1), grad_v_SA(2), grad_w_SA(3)
+            for (int q = 0; q < ncbf; ++q) {
+                const size_t s = (size_t)npi * mc + q;
+                const int f = c.bface[s];
+                const int bt = c.btype[f];
+                if (bt == 0 || bt == 1) {
+                    if (c.bloc[3 * s + 2] > 0.0) {
+                        double a2[3], nrm[3];
+                        for (int d = 0; d < 3; ++d) {
+                            a2[d] = 2.0 * (c.fvel[3 * f + d] - vi[d]);
+                            nrm[d] = Tm(c, f, d, 2);
+                        }
+                        const double dp = a2[0] * nrm[0] + a2[1] * nrm[1] + a2[2] * nrm[2];
+                        double dvel[3];
+                        for (int d = 0; d < 3; ++d) dvel[d] = dp * nrm[d];
+                        const double* BI = &c.bbi[8 * s];
+                        double av[3];
+                        for (int r = 0; r < 3; ++r) {
+                            double a = 0.0;
+                            for (int k = 0; k < 3; ++k) a = a + Tm(c, f, r, k) * BI[3 + k];
+                            av[r] = a;
+                        }
+                        gu = gu - dvel[0] * av[0];
+                        gv = gv - dvel[1] * av[1];
+                        gw = gw - dvel[2] * av[2];
+                    }
+                } else if (bt == 4 || bt == 5 || bt == 3) {
+                    bad += 1;
+                    break;
+                }
+            }
+            if (c.koddens[npi] == 0) dd = dd - c.dens[npi] * (gu + gv + gw);
+        }
+        const double rhoi = c.dens[npi];
+        double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int k = 0; k < c.nnb[npi]; ++k) {
+            const size_t s = (size_t)npi * mj + k;
+            const int npj = c.nbr[s];
+            const double rhoj = c.dens[npj], amassj = c.mass[npj];
+            const double* vj = &c.var[3 * npj];
+            const double dv0 = vj[0] - vi[0], dv1 = vj[1] - vi[1], dv2 = vj[2] - vi[2];
+            const double* R = &c.rag[3 * s];
+            const double K1 = c.pk[4 * s];
+            a1 = a1 + rhoi * (amassj / rhoj) * K1 * dv0 * R[0];
+            a2 = a2 + rhoi * (amassj / rhoj) * K1 * dv1 * R[1];
+            a3 = a3 + rhoi * (amassj / rhoj) * K1 * dv2 * R[2];
+        }
+        c.dden[npi] = dd - (a1 + a2 + a3);
+    }
+    return bad ? 94 : 0;
+}
+
+// Leapfrog_continuity.f90:51-60
+int or_leapfrog_cont(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    for (int i = 0; i < c.n; ++i) c.dens[i] = c.dens[i] + c.dt * c.dden[i];
+    return 0;
+}
+
+// EoS_fluid_loop.f90:59-70, EoS_barotropic_linear.f90:49-55
+int or_eos(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    if (c.on_going <= 0) return 0;
+    for (int i = 0; i < c.n; ++i) c.pres[i] = c.p.eps / c.p.den0 * (c.dens[i] - c.p.den0) + c.p.prif;
+    return 0;
+}
+
+// PressureSmoothing_3D.f90:80-161
+int or_psmooth(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    const int n = c.n;
+    const int mj = c.p.nmaxpartj;
+    const size_t mc = c.p.maxclose;
+    if (!(c.p.tetap > 0.0)) return 0;
+#pragma omp parallel for schedule(static)
+    for (int npi = 0; npi < n; ++npi) {
+        if (c.koddens[npi] != 0) continue;
+        double diffp = 0.0;
+        const int nsp = c.nnb[npi];
+        if (nsp > 0) {
+            const double p0i = c.p.prif, pi = c.pres[npi];
+            double somp = 0.0, app = 0.0;
+            for (int k = 0; k < nsp; ++k) {
+                const size_t s = (size_t)npi * mj + k;
+                const int npj = c.nbr[s];
+                const double pes = c.mass[npj] * c.pk[4 * s + 3] / c.dens[npj];
+                app = app + pes;
+                somp = somp + (c.pres[npj] - pi) * pes;
+            }
+            const int ncbf = c.p.semi ? c.bn[npi] : 0;
+            for (int q = 0; q < ncbf; ++q) {
+                const size_t s = (size_t)npi * mc + q;
+                if (c.bloc[3 * s + 2] > 0.0) app = app + c.bbi[8 * s + 1];
+            }
+            if (app < MinTotUnit)
+                diffp = somp + (p0i - pi) * (1.0 - app);
+            else
+                diffp = somp / app;
+        }
+        c.vpres[npi] = diffp;
+    }
+    const double tp1 = c.p.tetap * c.p.celerita * c.dt / c.p.h;
+    for (int i = 0; i < n; ++i) {
+        if (c.koddens[i] != 0) continue;
+        c.pres[i] = c.pres[i] + tp1 * c.vpres[i];
+        c.dens[i] = c.p.den0 * (1.0 + (c.pres[i] - c.p.prif) / c.p.eps);
+    }
+    return 0;
+}
+
+// time_step_loop.f90:82-173 (semi branch)
+int or_preloop(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    c.on_going = -999;
+    int rc = or_calcvarlength(ctx);
+    if (!rc) rc = or_boundary_tab(ctx);
+    if (!rc && c.it == 0) rc = or_dt(ctx);
+    return rc;
+}
+
+// time_step_loop.f90:175-489 (time_split=1, semi, no open/inlet/zmax zones)
+int or_step(void* ctx, int nsteps) {
+    Ctx& c = *(Ctx*)ctx;
+    for (int s = 0; s < nsteps; ++s) {
+        c.it = c.it + 1;
+        c.on_going = c.it;
+        const double dt_prev = c.dt;
+        if (c.n > 0) or_dt(ctx);
+        c.time = c.time + c.dt;
+        int rc = or_momentum(ctx);
+        if (rc) return rc;
+        or_leapfrog_mom(ctx, dt_prev);
+        or_vsmooth(ctx);
+        or_leapfrog_traj(ctx);
+        or_ordgrid(ctx);
+        or_calcvarlength(ctx);
+        rc = or_boundary_tab(ctx);
+        if (rc) return rc;
+        rc = or_continuity(ctx);
+        if (rc) return rc;
+        or_leapfrog_cont(ctx);
+        or_eos(ctx);
+        or_psmooth(ctx);
+    }
+    return 0;
+}
+
+int64_t or_pair_count(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    int64_t s = 0;
+    for (int i = 0; i < c.n; ++i) s += c.nnb[i] - 1;
+    return s;
+}
+
+int or_overflow(void* ctx) { return ((Ctx*)ctx)->overflow; }
+
+}  // extern "C"

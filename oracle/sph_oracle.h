@@ -1,0 +1,81 @@
+/* CPU FP64 restatement of the SPHERA v10 particle-interaction hot path.
+ *
+ * TEST INFRASTRUCTURE, NOT PRODUCT.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load this library, and only as the checker (or
+ * as the timed CPU baseline).  The device library never links or calls it.
+ *
+ * Every routine follows one reference unit, in the reference's particle order
+ * (pg order), with the reference's loop and summation order, so that on the
+ * same inputs it reproduces the flang-built reference bit for bit (checked in
+ * tests/test_oracle_vs_reference.py against oracle/_ref/harness3d.x).
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_params {
+    int32_t n;              /* nag */
+    int32_t nmaxpartj;      /* NMAXPARTJ */
+    int32_t maxclose;       /* MAXCLOSEBOUNDFACES */
+    int32_t ncd[3];
+    double extr_lo[3], dcd[3];
+    double dx, h, coefke, coefkacl, eta, eta2;
+    double grav[3], prif;
+    double den0, eps, celerita, alfamon, kin_visc, mumx;
+    double cfl, vsc_coeff, pesodt, tetap, tetav;
+    int32_t semi;           /* 1: SA-SPH walls active */
+} or_params;
+
+/* nf faces: T[9*f] column-major (T(r,c) at [c*3+r]), LX[8*f] (4 nodes x 2),
+ * GX[12*f] (4 nodes x 3), btype (0 fixe,1 tapi,2 open,3 sour,4 velo,5 flow),
+ * velocity = BoundaryFace%velocity, tvel = Tratto%velocity. gcbf_start is
+ * 1-based into gcbf_vec; gcbf_vec holds 0-based face indices. */
+typedef struct or_faces {
+    int32_t nf;
+    const double *T, *LX, *GX, *area, *velocity, *tvel, *slip_coeff;
+    const int32_t *nodes, *btype, *slip_mode, *lam_check;
+    const int32_t *gcbf_count, *gcbf_start, *gcbf_vec;
+    const double *kerneltab;  /* 41 x 5, row-major [row*5+col] */
+    double ktdelta;
+    const double *bitab;      /* 256 x 7, row-major */
+} or_faces;
+
+void* or_create(const or_params* p, const or_faces* f, const double* coord, const double* vel,
+                const double* dens, const double* pres, const double* mass);
+void or_destroy(void* ctx);
+int or_nag(void* ctx);
+
+/* state exchange (pg order, xyz interleaved) */
+void or_set_state(void* ctx, int n, const double* coord, const double* vel, const double* var,
+                  const double* acc, const double* dens, const double* pres, const int32_t* laminar,
+                  double dt, double dt_average, double time, int it);
+void or_get_state(void* ctx, double* coord, double* vel, double* var, double* acc, double* dens,
+                  double* pres, double* dden, double* vpres);
+void or_get_scalars(void* ctx, double* out4); /* dt, dt_average, time, it */
+int or_get_lists(void* ctx, int32_t* count, int32_t* ids, int32_t stride); /* ids 0-based */
+int or_get_btab(void* ctx, int32_t* ncbf, int32_t* face, double* loc, double* bi, double* ig);
+
+/* phases (time_step_loop.f90 order); each returns 0 or a diagnostic code */
+int or_ordgrid(void* ctx);          /* OrdGrid1 */
+int or_calcvarlength(void* ctx);    /* CalcVarLength (fluid part) */
+int or_boundary_tab(void* ctx);     /* ComputeBoundaryDataTab_3D */
+int or_dt(void* ctx);               /* time_step_duration */
+int or_momentum(void* ctx);         /* RHS_momentum_equation */
+int or_leapfrog_mom(void* ctx, double dt_previous);
+int or_vsmooth(void* ctx);          /* velocity_smoothing + velocity_smoothing_2 */
+int or_leapfrog_traj(void* ctx);
+int or_continuity(void* ctx);       /* SASPH_continuity + Continuity_Equation loop */
+int or_leapfrog_cont(void* ctx);
+int or_eos(void* ctx);
+int or_psmooth(void* ctx);          /* PressureSmoothing_3D */
+int or_preloop(void* ctx);          /* lists, boundary table, first dt */
+int or_step(void* ctx, int nsteps); /* whole leapfrog time steps */
+int64_t or_pair_count(void* ctx);   /* sum_i (nPartIntorno(i)-1) of the current lists */
+int or_overflow(void* ctx);         /* particles whose list was truncated */
+
+#ifdef __cplusplus
+}
+#endif

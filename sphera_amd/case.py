@@ -1,0 +1,448 @@
+"""Host-side set-up of a 3D dam-break solve: geometry, grid, initial particles,
+SA-SPH frontier tables.
+
+This is the part of the reference's pre-processing the device path needs in
+order to run without the Fortran front end.  Every function restates one
+reference unit (cited per function) so that a case built here is the case the
+reference builds from the equivalent v10 deck (``oracle/decks.py`` writes that
+deck from the same :class:`DamBreak3D`).  The tables are plain numpy arrays in
+the reference's own layout and units; they are handed to the C ABI unchanged.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Static_allocation_module.f90 constants
+PIGRECO = 3.14159265358979
+KERNELCONST3D = 1.0 / PIGRECO
+FI_STEPS = 8
+TETA_STEPS = 4 * FI_STEPS
+BITROWS = FI_STEPS * TETA_STEPS
+INT_KERNELTABLE = 40
+MAX_POSITIVE = 3.4e38
+
+# boundary type codes carried to the device (Tratto%tipo)
+BT_FIXE, BT_TAPI, BT_OPEN, BT_SOUR, BT_VELO, BT_FLOW, BT_PERI = 0, 1, 2, 3, 4, 5, 6
+_BT_NAMES = {"fixe": BT_FIXE, "tapi": BT_TAPI, "open": BT_OPEN, "sour": BT_SOUR,
+             "velo": BT_VELO, "flow": BT_FLOW, "peri": BT_PERI}
+
+
+@dataclass
+class DamBreak3D:
+    """A rectangular tank (open top, SA-SPH "fixed" walls) with a water column
+    in the corner x=y=z=0, hydrostatic at rest.  Units SI."""
+    tank: tuple = (1.6, 0.6, 0.6)
+    column: tuple = (0.4, 0.6, 0.3)
+    dx: float = 0.005
+    trunc: float = 1.3            # h/dx
+    rho0: float = 1000.0
+    bulk: float | None = None     # K; None -> rho0*(10*sqrt(g*H))^2 (linear EoS)
+    alfa_mon: float = 0.02
+    mu: float = 1.0e-3            # dynamic viscosity as the deck gives it
+    grav: tuple = (0.0, 0.0, -9.81)
+    p_ref: float = 0.0
+    cfl: float = 0.05
+    vsc_coeff: float = 0.05
+    pesodt: float = 0.0
+    teta_p: float = 0.2
+    teta_v: float = 0.2
+    coefnmaxpartj: float = 2.0
+    maxcloseboundfaces: int = 100
+    slip_coeff: float = 0.0       # slip_coefficient_mode=1 input value
+    name: str = "dam_break_3D"
+
+    def __post_init__(self):
+        if self.bulk is None:
+            g = math.sqrt(sum(c * c for c in self.grav))
+            c0 = 10.0 * math.sqrt(g * self.column[2])
+            self.bulk = self.rho0 * c0 * c0
+
+    # -- derived scalars (ReadInput.f90:198-205, Gest_Input.f90:266-289) ----
+    @property
+    def h(self):
+        return self.dx * self.trunc
+
+    def scalars(self):
+        h = self.h
+        s = dict(h=h, doubleh=2.0 * h, squareh=h * h, square_doubleh=(2.0 * h) * (2.0 * h),
+                 eta=1.0e-3 * h, eta2=1.0e-2 * (h * h))
+        h3 = h * h * h
+        s["coefke"] = 0.477464 / h3
+        s["coefkacl"] = 0.074603 / h3
+        s["nmaxpartj"] = int(self.coefnmaxpartj * (h * 4.0 / self.dx) ** 3) + 1
+        s["celerita"] = math.sqrt(self.bulk / self.rho0)
+        s["kin_visc"] = self.mu / self.rho0
+        s["pvolume"] = self.dx * self.dx * self.dx
+        return s
+
+    # -- geometry -----------------------------------------------------------
+    def vertices(self):
+        """Vertex table (1-based ids): 1-8 tank corners, 9-16 column corners.
+        Corner order per box: bottom (x0y0, x1y0, x1y1, x0y1), then top."""
+        def box(x1, y1, z1):
+            return [(0.0, 0.0, 0.0), (x1, 0.0, 0.0), (x1, y1, 0.0), (0.0, y1, 0.0),
+                    (0.0, 0.0, z1), (x1, 0.0, z1), (x1, y1, z1), (0.0, y1, z1)]
+        return np.array(box(*self.tank) + box(*self.column), dtype=np.float64)
+
+    def faces(self):
+        """Face table rows (n1,n2,n3,n4,zone).  Node order makes the normal
+        U1 x U2 (DefineLocalSystemVersors.f90) point into the fluid."""
+        def box_faces(o, top):
+            f = [(o + 1, o + 5, o + 6, o + 2), (o + 4, o + 3, o + 7, o + 8),
+                 (o + 1, o + 4, o + 8, o + 5), (o + 2, o + 6, o + 7, o + 3),
+                 (o + 1, o + 2, o + 3, o + 4)]
+            if top:
+                f.append((o + 7, o + 6, o + 5, o + 8))
+            return f
+        rows = [fc + (1,) for fc in box_faces(0, False)]
+        rows += [fc + (2,) for fc in box_faces(8, True)]
+        return rows
+
+    def zone_types(self):
+        return {1: "fixe", 2: "peri"}
+
+
+# ---------------------------------------------------------------------------
+# Grid: CreaGrid.f90:58-74
+# ---------------------------------------------------------------------------
+@dataclass
+class Grid:
+    extr_lo: np.ndarray
+    extr_hi: np.ndarray
+    ncd: np.ndarray
+    dcd: np.ndarray
+
+    @property
+    def nmax(self):
+        return int(self.ncd[0] * self.ncd[1] * self.ncd[2])
+
+
+def make_grid(domain_lo, domain_hi, dx, doubleh):
+    epsi = 0.01 * dx
+    lo = np.array([c - doubleh - epsi for c in domain_lo])
+    hi = np.array([c + doubleh + epsi for c in domain_hi])
+    dextr = hi - lo
+    ncd = np.array([int(np.rint(d / doubleh)) if abs(d / doubleh - math.floor(d / doubleh)) != 0.5
+                    else int(math.floor(d / doubleh + 0.5)) for d in dextr], dtype=np.int32)
+    dcd = dextr / ncd
+    return Grid(lo, hi, ncd, dcd)
+
+
+def cell_number(g: Grid, coord):
+    """ParticleCellNumber.f90:52-62 + CellNumber.f90:53-59, vectorised.
+    Returns 1-based cell id, 0 outside the grid."""
+    c = np.atleast_2d(coord)
+    idx = np.ceil((c - g.extr_lo) / g.dcd).astype(np.int64)
+    ni, nj, nk = (int(v) for v in g.ncd)
+    ok = ((idx[:, 0] >= 1) & (idx[:, 0] <= ni) & (idx[:, 1] >= 1) & (idx[:, 1] <= nj)
+          & (idx[:, 2] >= 1) & (idx[:, 2] <= nk))
+    cid = ((idx[:, 2] - 1) * nj + (idx[:, 1] - 1)) * ni + idx[:, 0]
+    return np.where(ok, cid, 0).astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Faces: DefineLocalSystemVersors.f90:55-133, CompleteBoundaries3D.f90
+# ---------------------------------------------------------------------------
+@dataclass
+class Faces:
+    """SA-SPH frontier faces in BFaceList order (faces of non-peri zones)."""
+    face_id: np.ndarray      # reference face index (1-based)
+    nodes: np.ndarray        # 3 or 4
+    btype: np.ndarray        # BT_* code of the face's zone
+    zone: np.ndarray
+    area: np.ndarray
+    T: np.ndarray            # (nf,3,3) T[:, :, 2] is the normal
+    GX: np.ndarray           # (nf,4,3) node global coords
+    LX: np.ndarray           # (nf,4,2) node local coords
+    velocity: np.ndarray     # (nf,3) BoundaryFace%velocity
+    tratto_velocity: np.ndarray  # (nf,3) Tratto%velocity
+    slip_mode: np.ndarray
+    slip_coeff: np.ndarray
+    laminar_check: np.ndarray
+
+
+def _local_system(gx):
+    nn = len(gx)
+    sidek = nn - 2
+    nidx = {1: (0, 1, 2), 2: (0, 2, 3)}[sidek]
+    u1 = gx[nidx[0]] - gx[nidx[2]]
+    u2 = gx[nidx[1]] - gx[nidx[2]]
+    u1len = math.sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2])
+    rr = u1 / u1len
+    w = np.array([u1[1] * u2[2] - u1[2] * u2[1], u1[2] * u2[0] - u1[0] * u2[2],
+                  u1[0] * u2[1] - u1[1] * u2[0]])
+    wlen = math.sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2])
+    area = wlen / float(3 - sidek)
+    nrm = w / wlen
+    ss = np.array([nrm[1] * rr[2] - nrm[2] * rr[1], nrm[2] * rr[0] - nrm[0] * rr[2],
+                   nrm[0] * rr[1] - nrm[1] * rr[0]])
+    T = np.stack([rr, ss, nrm], axis=1)
+    ref = gx[nn - 1]
+    lx = np.zeros((4, 2))
+    for n in range(nn - 1):
+        for i in range(2):
+            acc = 0.0
+            for j in range(3):
+                acc = acc + T[j, i] * (gx[n][j] - ref[j])
+            lx[n, i] = acc
+    return T, lx, area
+
+
+def make_faces(case: DamBreak3D) -> Faces:
+    vert = case.vertices()
+    rows = case.faces()
+    ztype = case.zone_types()
+    recs = []
+    # CompleteBoundaries3D: faces ordered by zone, then by face index
+    for zone in sorted(ztype):
+        for fi, row in enumerate(rows, start=1):
+            if row[4] != zone or ztype[zone] == "peri":
+                continue
+            gx = np.array([vert[n - 1] for n in row[:4]])
+            T, lx, area = _local_system(gx)
+            recs.append((fi, 4, _BT_NAMES[ztype[zone]], zone, area, T, gx, lx))
+    nf = len(recs)
+    return Faces(
+        face_id=np.array([r[0] for r in recs], np.int32),
+        nodes=np.array([r[1] for r in recs], np.int32),
+        btype=np.array([r[2] for r in recs], np.int32),
+        zone=np.array([r[3] for r in recs], np.int32),
+        area=np.array([r[4] for r in recs]),
+        T=np.array([r[5] for r in recs]).reshape(nf, 3, 3),
+        GX=np.array([r[6] for r in recs]).reshape(nf, 4, 3),
+        LX=np.array([r[7] for r in recs]).reshape(nf, 4, 2),
+        velocity=np.zeros((nf, 3)), tratto_velocity=np.zeros((nf, 3)),
+        slip_mode=np.ones(nf, np.int32), slip_coeff=np.full(nf, case.slip_coeff),
+        laminar_check=np.zeros(nf, np.int32))
+
+
+def grid_face_incidence(g: Grid, faces: Faces, maxclose: int):
+    """GridCellBoundaryFacesIntersections3D.f90:76-200 restated with numpy over
+    cells.  Returns (count[nmax], start[nmax] 1-based into vector, vector of
+    indices into ``faces`` (0-based))."""
+    ni, nj, nk = (int(v) for v in g.ncd)
+    nmax = g.nmax
+    cid = np.arange(1, nmax + 1)
+    k = (cid - 1) // (ni * nj) + 1
+    r = cid - ni * nj * (k - 1)
+    j = (r - 1) // ni + 1
+    i = r - ni * (j - 1)
+    lo = np.stack([g.extr_lo[0] + g.dcd[0] * (i - 1), g.extr_lo[1] + g.dcd[1] * (j - 1),
+                   g.extr_lo[2] + g.dcd[2] * (k - 1)], axis=1)
+    hi = np.stack([g.extr_lo[0] + g.dcd[0] * i, g.extr_lo[1] + g.dcd[1] * j,
+                   g.extr_lo[2] + g.dcd[2] * k], axis=1)
+    hits = np.zeros((nmax, len(faces.face_id)), bool)
+    for f in range(len(faces.face_id)):
+        nn = int(faces.nodes[f])
+        gx = faces.GX[f, :nn]
+        found = np.ones(nmax, bool)
+        for sd in range(3):
+            v = gx[:, sd][None, :]
+            below = (v < lo[:, sd:sd + 1]).sum(1)
+            eqlo = (v == lo[:, sd:sd + 1]).sum(1)
+            eqhi = (v == hi[:, sd:sd + 1]).sum(1)
+            above = (v > hi[:, sd:sd + 1]).sum(1)
+            # values in (lo,hi) are neither; the reference's elseif chain puts
+            # v==lo into -1 and v==hi into +1 only when not below lo
+            ex = (((below + eqlo) == nn) & (eqlo != nn)) | (((above + eqhi) == nn) & (eqhi != nn))
+            found &= ~ex
+        # second phase: the 8 cell corners against the face plane
+        ref = faces.GX[f, nn - 1]
+        nrm = faces.T[f, :, 2]
+        neg = np.zeros(nmax, np.int32)
+        zer = np.zeros(nmax, np.int32)
+        pos = np.zeros(nmax, np.int32)
+        for ii in range(2):
+            for jj in range(2):
+                for kk in range(2):
+                    corner = np.stack([(lo if ii == 0 else hi)[:, 0], (lo if jj == 0 else hi)[:, 1],
+                                       (lo if kk == 0 else hi)[:, 2]], axis=1)
+                    z = np.zeros(nmax)
+                    for sd in range(3):
+                        z = z + nrm[sd] * (corner[:, sd] - ref[sd])
+                    neg += z < 0.0
+                    zer += z == 0.0
+                    pos += z > 0.0
+        ex2 = (((neg + zer) == 8) & (zer != 4)) | (((pos + zer) == 8) & (zer != 4))
+        hits[:, f] = found & ~ex2
+    count = hits.sum(1).astype(np.int32)
+    if count.max(initial=0) > maxclose:
+        raise ValueError("too many faces crossing a grid cell (MAXCLOSEBOUNDFACES)")
+    start = np.zeros(nmax, np.int32)
+    nz = count > 0
+    start[nz] = (np.cumsum(count) - count)[nz] + 1
+    cells, fidx = np.nonzero(hits)
+    return count, start, fidx.astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+# SA-SPH IC tables: ComputeKernelTable.f90, IWro2dro.f90, JdWsRn.f90,
+# ComputeBoundaryIntegralTab.f90
+# ---------------------------------------------------------------------------
+def iwro2dro(ro):
+    a1, a2, a3, a4 = 0.333333333333333, 0.3, 0.125, 0.25
+    b1, b2, b3 = 0.0625, 0.025, 4.16666666666667e-03
+    if 0.0 <= ro < 1.0:
+        ro2 = ro * ro
+        ro3 = ro2 * ro
+        v = (a1 - a2 * ro2 + a3 * ro3) * ro3
+        return KERNELCONST3D * (a4 - v)
+    if 1.0 <= ro < 2.0:
+        ro2 = ro * ro
+        d = 2.0 - ro
+        d2 = d * d
+        d4 = d2 * d2
+        v = -(b1 * ro2 + b2 * d * ro + b3 * d2) * d4
+        return KERNELCONST3D * (-v)
+    return 0.0
+
+
+def jdwsrn(ro, n):
+    """Beta-spline branch (kernel=1) of JdWsRn.f90."""
+    KS3D = 0.31830989
+    ro2 = ro * ro
+    ro3 = ro2 * ro
+    ro4 = ro2 * ro2
+    ro5 = ro4 * ro
+    v = 0.0
+    if n == 1:
+        if 0.0 <= ro < 1.0:
+            v = -0.75 + (1.0 - 0.5625 * ro) * ro3
+        elif 1.0 <= ro < 2.0:
+            v = -1.0 + (1.5 - 1.0 * ro + 0.1875 * ro2) * ro2
+    elif n == 2:
+        if 0.0 <= ro < 1.0:
+            v = -0.7 + (0.75 - 0.45 * ro) * ro4
+        elif 1.0 <= ro < 2.0:
+            v = -0.8 + (1.0 - 0.75 * ro + 0.15 * ro2) * ro3
+    elif n == 3:
+        if 0.0 <= ro < 1.0:
+            v = -0.75 + (0.6 - 0.375 * ro) * ro5
+        elif 1.0 <= ro < 2.0:
+            v = -0.8 + (0.75 - 0.6 * ro + 0.125 * ro2) * ro4
+    return v * KS3D
+
+
+def kernel_table(h):
+    ktdelta = 2.0 / INT_KERNELTABLE
+    squareh = h * h
+    unosuh = 1.0 / h
+    unosusquareh = 1.0 / squareh
+    tab = np.zeros((INT_KERNELTABLE + 1, 5))
+    for nr in range(INT_KERNELTABLE + 1):
+        rob = ktdelta * nr
+        tab[nr, 0] = rob
+        tab[nr, 1] = iwro2dro(rob)
+        tab[nr, 2] = jdwsrn(rob, 1) * unosusquareh
+        tab[nr, 3] = jdwsrn(rob, 2) * unosuh
+        tab[nr, 4] = jdwsrn(rob, 3)
+    return tab, ktdelta
+
+
+def boundary_integral_tab(doubleh):
+    fi_int = PIGRECO / 2.0
+    teta_int = PIGRECO + PIGRECO
+    dfi = fi_int / FI_STEPS
+    dfimz = 0.5 * dfi
+    duesen = 2.0 * math.sin(dfimz)
+    dteta = teta_int / TETA_STEPS
+    tab = np.zeros((BITROWS, 7))
+    p = 0
+    fi = -0.5 * dfi
+    for _ in range(FI_STEPS):
+        fi = fi + dfi
+        teta = -0.5 * dteta
+        zp = -doubleh * math.cos(fi)
+        for _ in range(TETA_STEPS):
+            teta = teta + dteta
+            xp = doubleh * math.sin(fi) * math.cos(teta)
+            yp = doubleh * math.sin(fi) * math.sin(teta)
+            da = dteta * duesen * math.sin(fi)
+            tab[p] = (xp, yp, zp, da, xp / doubleh, yp / doubleh, zp / doubleh)
+            p += 1
+    return tab
+
+
+# ---------------------------------------------------------------------------
+# Initial particles: GeneratePart.f90:230-280, SetParticles.f90:86-152,
+# SetParticleParameters.f90:54-131, SubCalcPreIdro.f90:60-196 ("qp" branch)
+# ---------------------------------------------------------------------------
+def _axis_points(xmin, npps, dx):
+    pts = []
+    p = xmin - dx * 0.5
+    for _ in range(npps):
+        p = p + dx
+        pts.append(p)
+    return np.array(pts)
+
+
+def fortran_nint(x):
+    return int(math.floor(x + 0.5)) if x >= 0 else -int(math.floor(-x + 0.5))
+
+
+def initial_particles(case: DamBreak3D):
+    """Lattice of the column zone, hydrostatic pressure, inverse linear EoS.
+    Particle order is the reference's: x outer, y, z inner."""
+    s = case.scalars()
+    vert = case.vertices()
+    xmin_all = vert.min(0)
+    col = vert[8:16]
+    zmin, zmax = col.min(0), col.max(0)
+    npps, xres = [], []
+    for i in range(3):
+        nbefore = fortran_nint((zmin[i] - xmin_all[i]) / case.dx)
+        xr = xmin_all[i] + nbefore * case.dx
+        xres.append(xr)
+        npps.append(fortran_nint((zmax[i] - xr) / case.dx))
+    ax = [_axis_points(xres[i], npps[i], case.dx) for i in range(3)]
+    X, Y, Z = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
+    coord = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    inside = np.all((coord > zmin) & (coord < zmax), axis=1)
+    coord = np.ascontiguousarray(coord[inside])
+    n = len(coord)
+    g = case.grav
+    gmod = math.sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2])
+    coshor = -g[2] / gmod
+    senhor = g[0] / gmod
+    zq = float(case.column[2])
+    affond = (zq - coord[:, 2]) * coshor + coord[:, 0] * senhor
+    pres = (affond * case.rho0 * gmod) + case.p_ref
+    dens = case.rho0 * (1.0 + (pres - case.p_ref) / case.bulk)
+    mass = np.full(n, s["pvolume"] * case.rho0)
+    return dict(coord=coord, vel=np.zeros((n, 3)), dens=dens, pres=pres, mass=mass)
+
+
+def domain_box(case: DamBreak3D):
+    """Domain%coord: bounding box of all vertices (the grid spans it)."""
+    v = case.vertices()
+    return v.min(0), v.max(0)
+
+
+@dataclass
+class CaseTables:
+    case: DamBreak3D
+    scalars: dict
+    grid: Grid
+    faces: Faces
+    gcbf_count: np.ndarray
+    gcbf_start: np.ndarray
+    gcbf_vector: np.ndarray
+    kerneltab: np.ndarray
+    ktdelta: float
+    bitab: np.ndarray
+    particles: dict = field(default_factory=dict)
+
+
+def build(case: DamBreak3D) -> CaseTables:
+    s = case.scalars()
+    lo, hi = domain_box(case)
+    g = make_grid(lo, hi, case.dx, s["doubleh"])
+    f = make_faces(case)
+    cnt, st, vec = grid_face_incidence(g, f, case.maxcloseboundfaces)
+    kt, ktd = kernel_table(s["h"])
+    bit = boundary_integral_tab(s["doubleh"])
+    parts = initial_particles(case)
+    return CaseTables(case, s, g, f, cnt, st, vec, kt, ktd, bit, parts)
