@@ -1,0 +1,131 @@
+/* sphera_hip.h — C ABI of the MI355X (gfx950) particle-interaction hot path of
+ * SPHERA v10.  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * The reference has no FFI: its hot path is a set of argument-less Fortran
+ * subroutines acting on module globals, called in a fixed order by
+ * time_step_loop (src/Main_algorithm/time_step_loop.f90:175-489).  Each
+ * per-phase entry point below replaces one of those subroutines (cited), with
+ * the same meaning, so that a Fortran ISO_C_BINDING bridge can call them in
+ * the reference's own order; sph_step runs that whole order on the device.
+ *
+ * Conventions
+ *  - Return value: 0 on success, otherwise a status code; codes 10..99 map to
+ *    the reference's diagnostic(arg1,arg2) pairs as 10*arg1+arg2 where one
+ *    exists (e.g. 86 "too many close boundary faces"), codes >= 100 are
+ *    device/runtime errors.  sph_last_error() gives the message.  Nothing
+ *    exits the process (the reference stops in diagnostic.f90:317).
+ *  - Host arrays are borrowed for the duration of the call; xyz vectors are
+ *    interleaved (x0,y0,z0,x1,...), particle order is the caller's pg order.
+ *  - Device memory is owned by the context.  Calls on one context are
+ *    sequential and not thread-safe; separate contexts are independent.
+ */
+#ifndef SPHERA_HIP_H
+#define SPHERA_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SPH_FP64 = 0, SPH_FP32_MIXED = 1 };
+
+/* Scalars of Domain, Grid, Med(1) and input_any_t the hot path reads
+ * (src/Modules/Hybrid_allocation_module.f90:159-168,1120-1131;
+ *  src/Modules/Static_allocation_module.f90:226-239). */
+typedef struct sph_params {
+    int32_t precision;   /* SPH_FP64 | SPH_FP32_MIXED */
+    int32_t capacity;    /* maximum particle count held by the context */
+    int32_t nmaxpartj;   /* NMAXPARTJ: neighbour-list cap (lists truncate past it) */
+    int32_t maxclose;    /* MAXCLOSEBOUNDFACES */
+    int32_t ncd[3];      /* Grid%ncd */
+    double extr_lo[3];   /* Grid%extr(:,1) */
+    double dcd[3];       /* Grid%dcd */
+    double dx, h;        /* Domain%dx, Domain%h */
+    double coefke, coefkacl, eta, eta2;
+    double grav[3], prif;                       /* Domain%grav, Domain%prif */
+    double den0, eps, celerita, alfamon, kin_visc; /* Med(1) */
+    double cfl, vsc_coeff, pesodt, tetap, tetav;   /* input_any_t */
+    int32_t semi;        /* 1: SA-SPH frontiers active (Domain%tipo=="semi") */
+    int32_t reserved;
+} sph_params;
+
+/* SA-SPH frontier faces in BFaceList order and the grid-cell/face incidence
+ * (BoundaryFace, Tratto, Partz, GCBFPointers, GCBFVector, kerneltab,
+ * BoundIntegralTab).  T is column-major per face (T(r,c) at [9f+3c+r]);
+ * LX: 4 nodes x (x,y); GX: 4 nodes x (x,y,z); btype: 0 fixe, 1 tapi, 2 open,
+ * 3 sour, 4 velo, 5 flow.  gcbf_start is 1-based into gcbf_vec, which holds
+ * 0-based face indices. */
+typedef struct sph_faces {
+    int32_t nf;
+    const double *T, *LX, *GX, *area, *velocity, *tvel, *slip_coeff;
+    const int32_t *nodes, *btype, *slip_mode, *lam_check;
+    const int32_t *gcbf_count, *gcbf_start, *gcbf_vec;
+    const double *kerneltab; /* 41 x 5 row-major */
+    double ktdelta;
+    const double *bitab;     /* 256 x 7 row-major */
+} sph_faces;
+
+typedef struct sph_ctx sph_ctx;
+
+/* Context: allocates device memory for `capacity` particles and copies the
+ * frontier tables.  Replaces the allocations of Gest_Trans.f90:130-211 and
+ * the SA-SPH set-up of Gest_Trans.f90:692-701 (tables computed by the
+ * caller). */
+int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out);
+int sph_ctx_destroy(sph_ctx* ctx);
+
+/* Particle state in pg order: coord, vel (xyz interleaved), dens, pres, mass.
+ * Initialises ids 0..n-1 and orders the particles on the grid (OrdGrid1). */
+int sph_state_upload(sph_ctx* ctx, int32_t n, const double* coord, const double* vel,
+                     const double* dens, const double* pres, const double* mass);
+/* Current particle count (nag; particles leaving the grid are removed). */
+int sph_particle_count(sph_ctx* ctx, int32_t* n);
+/* Download in pg (id) order.  Any pointer may be NULL.  var equals vel at
+ * every point where the reference reads it.  ids: the pg index of each
+ * returned row (rows are sorted by it). */
+int sph_state_download(sph_ctx* ctx, double* coord, double* vel, double* acc, double* dens,
+                       double* pres, double* dden, double* vpres, int32_t* ids);
+/* dt, dt_average, simulation_time, it, pair count of the current lists */
+int sph_get_scalars(sph_ctx* ctx, double out[5]);
+/* Running total of pair interactions, sum over every list built so far of
+ * sum_i (nPartIntorno(i) - 1) (the unit of the pair-interactions/s metric). */
+int sph_pair_counter(sph_ctx* ctx, unsigned long long* total);
+/* Debug: neighbour lists in pg order, entries are pg indices (0-based), in the
+ * reference's list order.  stride >= nmaxpartj. */
+int sph_get_neighbour_lists(sph_ctx* ctx, int32_t* count, int32_t* ids, int32_t stride);
+
+/* ---- phases, in time_step_loop order ------------------------------------ */
+int sph_dt(sph_ctx* ctx);              /* time_step_duration.f90:59-123 */
+int sph_momentum(sph_ctx* ctx);        /* RHS_momentum_equation.f90:72-198 */
+int sph_leapfrog_mom(sph_ctx* ctx);    /* Leapfrog_momentum.f90:66-111 */
+int sph_velocity_smoothing(sph_ctx* ctx); /* velocity_smoothing.f90:74-133 + _2.f90:59-103 */
+int sph_leapfrog_traj(sph_ctx* ctx);   /* Leapfrog_trajectories.f90:48-76 */
+int sph_sort(sph_ctx* ctx);            /* OrdGrid1.f90:58-105 */
+int sph_neighbours(sph_ctx* ctx);      /* CalcVarLength.f90:111-541 (fluid part) */
+int sph_boundary_tab(sph_ctx* ctx);    /* ComputeBoundaryDataTab_3D.f90:62-106 */
+int sph_continuity(sph_ctx* ctx);      /* time_step_loop.f90:338-360 */
+int sph_leapfrog_cont(sph_ctx* ctx);   /* Leapfrog_continuity.f90:51-88 */
+int sph_eos(sph_ctx* ctx);             /* EoS_fluid_loop.f90:59-70 */
+int sph_pressure_smoothing(sph_ctx* ctx); /* PressureSmoothing_3D.f90:80-161 */
+
+/* Pre-loop of time_step_loop.f90:82-173: lists, boundary table, first dt. */
+int sph_preloop(sph_ctx* ctx);
+/* n whole time steps (time_step_loop.f90:175-489, leapfrog, semi), all on the
+ * device; checks the device status flags once at the end. */
+int sph_step(sph_ctx* ctx, int32_t nsteps);
+/* Same, enqueued on the context stream without the final synchronisation
+ * and flag check (for overlap and graph capture by the caller). */
+int sph_step_async(sph_ctx* ctx, int32_t nsteps);
+int sph_sync(sph_ctx* ctx);
+
+/* Per-kernel timing of the last sph_step: name/avg-ms pairs, for the bench. */
+int sph_set_timing(sph_ctx* ctx, int32_t on);
+int sph_get_timing(sph_ctx* ctx, char* names, int32_t names_len, double* ms, int32_t* calls,
+                   int32_t max_kernels);
+
+const char* sph_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,757 @@
+// Host orchestration and C ABI (include/sphera_hip.h) of the device hot path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/sphera_hip.h"
+#include "sph_kernels.hip"
+
+using namespace sph;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPC(expr)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return fail(100 + (int)e_, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+enum Phase {
+    PH_EMPTY = 0, PH_SORTED,  // state uploaded and ordered
+    PH_READY,                 // lists, boundary table, dt available (step boundary)
+    PH_DT, PH_MOM, PH_VS, PH_SORT, PH_NB, PH_BT, PH_CONT, PH_PS
+};
+
+struct Timer {
+    std::string name;
+    double ms = 0.0;
+    int calls = 0;
+};
+
+}  // namespace
+
+struct sph_ctx {
+    DP P{};
+    int precision = SPH_FP64;
+    hipStream_t stream = nullptr;
+    // faces / tables
+    DFace* d_faces = nullptr;
+    int *d_gcount = nullptr, *d_gstart = nullptr, *d_gvec = nullptr;
+    double *d_kt = nullptr, *d_bit = nullptr;
+    // state
+    double4 *xm[2] = {}, *vel[3] = {}, *acc[2] = {};
+    double2* rp[3] = {};
+    int* lam[2] = {};
+    float4 *xmf = nullptr, *velf[3] = {};
+    float2* rpf[3] = {};
+    double *dden = nullptr, *vpres = nullptr;
+    int *cell = nullptr, *key = nullptr, *slot = nullptr, *cnt = nullptr, *start = nullptr,
+        *perm = nullptr, *bsum = nullptr;
+    uint32_t* nbr = nullptr;
+    int* nnb = nullptr;
+    int* bn = nullptr;
+    BTab* bt = nullptr;
+    double* scal = nullptr;
+    int* iscal = nullptr;
+    unsigned long long* dtmin = nullptr;
+    unsigned long long* pairs = nullptr;  // running pair-interaction count
+    Phase phase = PH_EMPTY;
+    bool timing = false;
+    std::vector<Timer> timers;
+    std::vector<hipEvent_t> ev;
+    int n_upload = 0;
+
+    ~sph_ctx();
+    int alloc();
+};
+
+namespace {
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+}
+
+__global__ void k_fill_int(int* p, int v, int m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) p[i] = v;
+}
+
+inline int nblk(int n, int b = 256) { return (n + b - 1) / b; }
+
+struct KTime {
+    sph_ctx* c;
+    int idx;
+    KTime(sph_ctx* cc, const char* name) : c(cc), idx(-1) {
+        if (!c->timing) return;
+        for (size_t t = 0; t < c->timers.size(); ++t)
+            if (c->timers[t].name == name) idx = (int)t;
+        if (idx < 0) {
+            c->timers.push_back(Timer{name});
+            idx = (int)c->timers.size() - 1;
+        }
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        c->ev.push_back(a);
+        c->ev.push_back(b);
+        hipEventRecord(a, c->stream);
+    }
+    ~KTime() {
+        if (!c->timing || idx < 0) return;
+        hipEventRecord(c->ev.back(), c->stream);
+        evidx.push_back(idx);
+    }
+    static thread_local std::vector<int> evidx;
+};
+thread_local std::vector<int> KTime::evidx;
+
+int collect_timing(sph_ctx* c) {
+    if (!c->timing) return 0;
+    HIPC(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < KTime::evidx.size(); ++k) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]);
+        c->timers[KTime::evidx[k]].ms += ms;
+        c->timers[KTime::evidx[k]].calls += 1;
+    }
+    for (auto e : c->ev) hipEventDestroy(e);
+    c->ev.clear();
+    KTime::evidx.clear();
+    return 0;
+}
+
+// ---- device steps --------------------------------------------------------
+int run_sort(sph_ctx* c) {
+    const DP& P = c->P;
+    const int m = P.nmax + 1;
+    const int cap = P.cap;
+    {
+        KTime t(c, "sort_count");
+        k_fill_int<<<nblk(m), 256, 0, c->stream>>>(c->cnt, 0, m);
+        k_cell_count<<<nblk(cap), 256, 0, c->stream>>>(P, c->xm[1], c->key, c->slot, c->cnt, c->iscal);
+    }
+    {
+        KTime t(c, "sort_scan");
+        const int nb = (m + 1023) / 1024;
+        k_scan_local<<<nb, 256, 0, c->stream>>>(c->cnt, c->start, c->bsum, m);
+        k_scan_sums<<<1, 1024, 0, c->stream>>>(c->bsum, nb, nullptr);
+        k_scan_add<<<nblk(m), 256, 0, c->stream>>>(c->start, c->bsum, m);
+    }
+    {
+        KTime t(c, "sort_scatter");
+        k_scatter<<<nblk(cap), 256, 0, c->stream>>>(c->key, c->slot, c->start, c->perm, c->iscal);
+        k_cell_sort<<<nblk(P.nmax), 256, 0, c->stream>>>(P, c->start, c->perm, c->vel[2]);
+    }
+    {
+        KTime t(c, "sort_gather");
+        k_gather<<<nblk(cap), 256, 0, c->stream>>>(P, c->perm, c->key, c->xm[1], c->vel[2], c->rp[0],
+                                                   c->lam[0], c->acc[1], c->xm[0], c->vel[0], c->rp[1],
+                                                   c->lam[1], c->acc[0], c->cell, c->xmf, c->velf[0],
+                                                   c->rpf[1], c->start, c->iscal);
+    }
+    return 0;
+}
+
+int run_neighbours(sph_ctx* c) {
+    KTime t(c, "neighbours");
+    k_neighbours<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->xm[0], c->cell, c->start, c->nbr,
+                                                         c->nnb, c->iscal, c->pairs);
+    return 0;
+}
+
+int run_btab(sph_ctx* c) {
+    if (!c->P.semi || c->P.nf == 0) {
+        k_fill_int<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->bn, 0, c->P.cap);
+        return 0;
+    }
+    KTime t(c, "boundary_tab");
+    k_boundary_tab<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_gcount, c->d_gstart,
+                                                           c->d_gvec, c->d_kt, c->d_bit, c->xm[0], c->bn,
+                                                           c->bt, c->iscal);
+    return 0;
+}
+
+int run_dt(sph_ctx* c, int advance, bool reduce) {
+    KTime t(c, "dt");
+    if (reduce)
+        k_dtmin<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->vel[0], c->dtmin, c->iscal);
+    k_dt_begin<<<1, 64, 0, c->stream>>>(c->P, c->scal, c->dtmin, c->iscal, advance);
+    return 0;
+}
+
+int run_momentum(sph_ctx* c) {
+    KTime t(c, "momentum");
+    const int g = nblk(c->P.cap);
+    if (c->P.f32)
+        k_momentum<true><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+                                                   c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1],
+                                                   c->lam[0], c->nbr, c->nnb, c->acc[1], c->vel[1],
+                                                   c->velf[1], c->scal, c->iscal);
+    else
+        k_momentum<false><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+                                                    c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1],
+                                                    c->lam[0], c->nbr, c->nnb, c->acc[1], c->vel[1],
+                                                    c->velf[1], c->scal, c->iscal);
+    return 0;
+}
+
+int run_vsmooth(sph_ctx* c) {
+    KTime t(c, "velocity_smoothing");
+    const int g = nblk(c->P.cap);
+    if (c->P.f32)
+        k_vsmooth<true><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
+                                                  c->rp[0], c->xmf, c->velf[1], c->rpf[0], c->nbr, c->nnb,
+                                                  c->vel[2], c->velf[2], c->xm[1], c->scal, c->dtmin,
+                                                  c->iscal, 1);
+    else
+        k_vsmooth<false><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
+                                                   c->rp[0], c->xmf, c->velf[1], c->rpf[0], c->nbr, c->nnb,
+                                                   c->vel[2], c->velf[2], c->xm[1], c->scal, c->dtmin,
+                                                   c->iscal, 1);
+    return 0;
+}
+
+int run_continuity(sph_ctx* c) {
+    KTime t(c, "continuity");
+    const int g = nblk(c->P.cap);
+    if (c->P.f32)
+        k_continuity<true><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+                                                     c->rp[1], c->xmf, c->velf[0], c->rpf[1], c->nbr,
+                                                     c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal,
+                                                     c->iscal);
+    else
+        k_continuity<false><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+                                                      c->rp[1], c->xmf, c->velf[0], c->rpf[1], c->nbr,
+                                                      c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal,
+                                                      c->iscal);
+    return 0;
+}
+
+int run_psmooth(sph_ctx* c) {
+    KTime t(c, "pressure_smoothing");
+    const int g = nblk(c->P.cap);
+    if (c->P.f32)
+        k_psmooth<true><<<g, 256, 0, c->stream>>>(c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf, c->rpf[2],
+                                                  c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal,
+                                                  c->iscal);
+    else
+        k_psmooth<false><<<g, 256, 0, c->stream>>>(c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
+                                                   c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0],
+                                                   c->scal, c->iscal);
+    return 0;
+}
+
+int check_flags(sph_ctx* c) {
+    int is[IS_COUNT];
+    HIPC(hipMemcpyAsync(is, c->iscal, sizeof(is), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    if (is[IS_UNSUPPORTED])
+        return fail(95, "particle close to a boundary type the device path does not implement "
+                        "(sour/velo/flow)");
+    if (is[IS_BT_OVERFLOW])
+        return fail(86, "a particle has more close boundary faces than the device table holds (" +
+                            std::to_string(SPH_MB) + ")");
+    return 0;
+}
+
+int expect(sph_ctx* c, Phase need, const char* what) {
+    if (c->phase != need)
+        return fail(120, std::string(what) + " called out of the time_step_loop order");
+    return 0;
+}
+
+}  // namespace
+
+sph_ctx::~sph_ctx() {
+    auto F = [](void* p) {
+        if (p) hipFree(p);
+    };
+    F(d_faces); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit);
+    for (auto p : xm) F(p);
+    for (auto p : vel) F(p);
+    for (auto p : acc) F(p);
+    for (auto p : rp) F(p);
+    for (auto p : lam) F(p);
+    F(xmf);
+    for (auto p : velf) F(p);
+    for (auto p : rpf) F(p);
+    F(dden); F(vpres); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(bsum);
+    F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
+    for (auto e : ev) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+}
+
+int sph_ctx::alloc() {
+    const size_t N = P.cap;
+    const size_t m = P.nmax + 1;
+    for (auto& p : xm) HIPC(dalloc(&p, N));
+    for (auto& p : vel) HIPC(dalloc(&p, N));
+    for (auto& p : acc) HIPC(dalloc(&p, N));
+    for (auto& p : rp) HIPC(dalloc(&p, N));
+    for (auto& p : lam) HIPC(dalloc(&p, N));
+    HIPC(dalloc(&xmf, P.f32 ? N : 1));
+    for (auto& p : velf) HIPC(dalloc(&p, P.f32 ? N : 1));
+    for (auto& p : rpf) HIPC(dalloc(&p, P.f32 ? N : 1));
+    HIPC(dalloc(&dden, N));
+    HIPC(dalloc(&vpres, N));
+    HIPC(dalloc(&cell, N));
+    HIPC(dalloc(&key, N));
+    HIPC(dalloc(&slot, N));
+    HIPC(dalloc(&perm, N));
+    HIPC(dalloc(&cnt, m));
+    HIPC(dalloc(&start, m + 1));
+    HIPC(dalloc(&bsum, (m + 1023) / 1024 + 1));
+    HIPC(dalloc(&nbr, N * (size_t)P.nmaxpartj));
+    HIPC(dalloc(&nnb, N));
+    HIPC(dalloc(&bn, N));
+    HIPC(dalloc(&bt, P.semi ? N * SPH_MB : 1));
+    HIPC(dalloc(&scal, SC_COUNT));
+    HIPC(dalloc(&iscal, IS_COUNT));
+    HIPC(dalloc(&dtmin, 1));
+    HIPC(dalloc(&pairs, 1));
+    HIPC(hipMemset(pairs, 0, sizeof(unsigned long long)));
+    HIPC(hipMemset(acc[0], 0, sizeof(double4) * N));
+    HIPC(hipMemset(acc[1], 0, sizeof(double4) * N));
+    HIPC(hipMemset(lam[0], 0, sizeof(int) * N));
+    HIPC(hipMemset(lam[1], 0, sizeof(int) * N));
+    HIPC(hipMemset(dden, 0, sizeof(double) * N));
+    HIPC(hipMemset(vpres, 0, sizeof(double) * N));
+    HIPC(hipMemset(nnb, 0, sizeof(int) * N));
+    HIPC(hipMemset(bn, 0, sizeof(int) * N));
+    HIPC(hipMemset(scal, 0, sizeof(double) * SC_COUNT));
+    HIPC(hipMemset(iscal, 0, sizeof(int) * IS_COUNT));
+    return 0;
+}
+
+extern "C" {
+
+const char* sph_last_error(void) { return g_err.c_str(); }
+
+int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
+    if (!p || !out) return fail(1, "null argument");
+    if (p->capacity <= 0 || p->capacity > (int)NB_MASK) return fail(2, "capacity out of range");
+    if (p->nmaxpartj <= 0) return fail(2, "nmaxpartj must be positive");
+    for (int d = 0; d < 3; ++d)
+        if (p->ncd[d] <= 0 || !(p->dcd[d] > 0.0)) return fail(2, "invalid grid");
+    if (p->precision != SPH_FP64 && p->precision != SPH_FP32_MIXED) return fail(2, "invalid precision");
+    if (p->semi && (!f || f->nf <= 0)) return fail(2, "SA-SPH requested without frontier faces");
+    const long long nmax = (long long)p->ncd[0] * p->ncd[1] * p->ncd[2];
+    if (nmax >= (1LL << 30)) return fail(2, "grid too large");
+    sph_ctx* c = new sph_ctx();
+    DP& P = c->P;
+    c->precision = p->precision;
+    P.cap = p->capacity;
+    P.nmaxpartj = p->nmaxpartj;
+    P.nmax = (int)nmax;
+    P.semi = p->semi ? 1 : 0;
+    P.f32 = p->precision == SPH_FP32_MIXED;
+    P.nf = f ? f->nf : 0;
+    for (int d = 0; d < 3; ++d) {
+        P.ncd[d] = p->ncd[d];
+        P.extr_lo[d] = p->extr_lo[d];
+        P.dcd[d] = p->dcd[d];
+        P.grav[d] = p->grav[d];
+    }
+    P.h = p->h;
+    P.dx = p->dx;
+    P.coefke = p->coefke;
+    P.coefkacl = p->coefkacl;
+    P.eta = p->eta;
+    P.eta2 = p->eta2;
+    P.squareh = p->h * p->h;
+    P.doubleh = 2.0 * p->h;
+    P.square_doubleh = P.doubleh * P.doubleh;
+    P.ke_coef = p->coefke / p->h;
+    P.kacl_coef = p->coefkacl / p->h;
+    P.ktdelta = f ? f->ktdelta : 0.05;
+    P.GI = (double)9.80665f;  // Static_allocation_module.f90:75, a default-real literal
+    P.prif = p->prif;
+    P.den0 = p->den0;
+    P.eps = p->eps;
+    P.celerita = p->celerita;
+    P.alfamon = p->alfamon;
+    P.kin_visc = p->kin_visc;
+    P.cfl = p->cfl;
+    P.vsc_coeff = p->vsc_coeff;
+    P.pesodt = p->pesodt;
+    P.tetap = p->tetap;
+    P.tetav = p->tetav;
+    P.f_h = (float)P.h;
+    P.f_unosuh = (float)(1.0 / P.h);
+    P.f_unosusquareh = (float)(1.0 / P.squareh);
+    P.f_eta = (float)P.eta;
+    P.f_eta2 = (float)P.eta2;
+    P.f_ke_coef = (float)P.ke_coef;
+    P.f_kacl_coef = (float)P.kacl_coef;
+    P.f_coefke = (float)P.coefke;
+    for (int d = 0; d < 3; ++d) P.f_dcd[d] = (float)P.dcd[d];
+    P.f_alfamon = (float)P.alfamon;
+    P.f_celerita = (float)P.celerita;
+    P.f_kin_visc = (float)P.kin_visc;
+    int rc;
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return fail(101, "hipStreamCreate failed");
+    }
+    if ((rc = c->alloc())) {
+        delete c;
+        return rc;
+    }
+    if (f && f->nf > 0) {
+        std::vector<DFace> hf(f->nf);
+        for (int q = 0; q < f->nf; ++q) {
+            DFace& F = hf[q];
+            std::memcpy(F.T, f->T + 9 * q, sizeof(F.T));
+            std::memcpy(F.LX, f->LX + 8 * q, sizeof(F.LX));
+            std::memcpy(F.GX, f->GX + 12 * q, sizeof(F.GX));
+            F.area = f->area[q];
+            F.slip = f->slip_coeff[q];
+            for (int d = 0; d < 3; ++d) {
+                F.vel[d] = f->velocity[3 * q + d];
+                F.tvel[d] = f->tvel[3 * q + d];
+            }
+            F.nodes = f->nodes[q];
+            F.btype = f->btype[q];
+            F.mode = f->slip_mode[q];
+            F.lam = f->lam_check[q];
+            if (F.nodes != 3 && F.nodes != 4) {
+                delete c;
+                return fail(2, "SA-SPH faces must be triangles or quadrilaterals");
+            }
+        }
+        long long nv = 0;
+        for (int i = 0; i < P.nmax; ++i) nv += f->gcbf_count[i];
+        bool ok = hipMalloc(&c->d_faces, sizeof(DFace) * f->nf) == hipSuccess &&
+                  hipMalloc(&c->d_gcount, sizeof(int) * P.nmax) == hipSuccess &&
+                  hipMalloc(&c->d_gstart, sizeof(int) * P.nmax) == hipSuccess &&
+                  hipMalloc(&c->d_gvec, sizeof(int) * (nv ? nv : 1)) == hipSuccess &&
+                  hipMalloc(&c->d_kt, sizeof(double) * 41 * 5) == hipSuccess &&
+                  hipMalloc(&c->d_bit, sizeof(double) * 256 * 7) == hipSuccess;
+        ok = ok && hipMemcpy(c->d_faces, hf.data(), sizeof(DFace) * f->nf, hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && hipMemcpy(c->d_gcount, f->gcbf_count, sizeof(int) * P.nmax, hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && hipMemcpy(c->d_gstart, f->gcbf_start, sizeof(int) * P.nmax, hipMemcpyHostToDevice) == hipSuccess;
+        if (nv) ok = ok && hipMemcpy(c->d_gvec, f->gcbf_vec, sizeof(int) * nv, hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && hipMemcpy(c->d_kt, f->kerneltab, sizeof(double) * 41 * 5, hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && hipMemcpy(c->d_bit, f->bitab, sizeof(double) * 256 * 7, hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) {
+            delete c;
+            return fail(102, "device allocation/copy of frontier tables failed");
+        }
+        for (long long q = 0; q < nv; ++q)
+            if (f->gcbf_vec[q] < 0 || f->gcbf_vec[q] >= f->nf) {
+                delete c;
+                return fail(2, "gcbf_vec entry out of range");
+            }
+    }
+    *out = c;
+    return 0;
+}
+
+int sph_ctx_destroy(sph_ctx* c) {
+    if (c) {
+        hipStreamSynchronize(c->stream);
+        delete c;
+    }
+    return 0;
+}
+
+int sph_state_upload(sph_ctx* c, int32_t n, const double* coord, const double* vel,
+                     const double* dens, const double* pres, const double* mass) {
+    if (!c || !coord || !vel || !dens || !pres || !mass) return fail(1, "null argument");
+    if (n < 0 || n > c->P.cap) return fail(3, "particle count exceeds the context capacity");
+    std::vector<double4> hx(n), hv(n);
+    std::vector<double2> hr(n);
+    for (int i = 0; i < n; ++i) {
+        hx[i] = make_double4(coord[3 * i], coord[3 * i + 1], coord[3 * i + 2], mass[i]);
+        hv[i] = make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], (double)i);
+        hr[i] = make_double2(dens[i], pres[i]);
+    }
+    int is[IS_COUNT] = {0};
+    is[IS_N] = n;
+    is[IS_ONGOING] = -999;
+    double sc[SC_COUNT] = {0};
+    HIPC(hipMemcpyAsync(c->xm[1], hx.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->vel[2], hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->rp[0], hr.data(), sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemsetAsync(c->lam[0], 0, sizeof(int) * c->P.cap, c->stream));
+    HIPC(hipMemsetAsync(c->acc[1], 0, sizeof(double4) * c->P.cap, c->stream));
+    HIPC(hipMemcpyAsync(c->iscal, is, sizeof(is), hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->scal, sc, sizeof(sc), hipMemcpyHostToDevice, c->stream));
+    int rc = run_sort(c);
+    if (rc) return rc;
+    // the sort leaves (rho,p) in the buffer the continuity pass reads; the
+    // state at rest keeps it in the buffer the momentum pass reads
+    HIPC(hipMemcpyAsync(c->rp[0], c->rp[1], sizeof(double2) * c->P.cap, hipMemcpyDeviceToDevice, c->stream));
+    if (c->P.f32)
+        HIPC(hipMemcpyAsync(c->rpf[0], c->rpf[1], sizeof(float2) * c->P.cap, hipMemcpyDeviceToDevice,
+                            c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    c->n_upload = n;
+    c->phase = PH_SORTED;
+    return 0;
+}
+
+int sph_particle_count(sph_ctx* c, int32_t* n) {
+    if (!c || !n) return fail(1, "null argument");
+    HIPC(hipMemcpy(n, c->iscal + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sph_state_download(sph_ctx* c, double* coord, double* vel, double* acc, double* dens,
+                       double* pres, double* dden, double* vpres, int32_t* ids) {
+    if (!c) return fail(1, "null argument");
+    if (c->phase != PH_SORTED && c->phase != PH_READY && c->phase != PH_PS)
+        return fail(121, "state download is only defined at a time-step boundary");
+    HIPC(hipStreamSynchronize(c->stream));
+    int n;
+    HIPC(hipMemcpy(&n, c->iscal + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double4> hx(n), hv(n), ha(n);
+    std::vector<double2> hr(n);
+    std::vector<double> hd(n), hp(n);
+    HIPC(hipMemcpy(hx.data(), c->xm[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hv.data(), c->vel[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(ha.data(), c->acc[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hr.data(), c->rp[0], sizeof(double2) * n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hd.data(), c->dden, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hp.data(), c->vpres, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return hv[a].w < hv[b].w; });
+    for (int r = 0; r < n; ++r) {
+        const int s = order[r];
+        if (coord) { coord[3 * r] = hx[s].x; coord[3 * r + 1] = hx[s].y; coord[3 * r + 2] = hx[s].z; }
+        if (vel) { vel[3 * r] = hv[s].x; vel[3 * r + 1] = hv[s].y; vel[3 * r + 2] = hv[s].z; }
+        if (acc) { acc[3 * r] = ha[s].x; acc[3 * r + 1] = ha[s].y; acc[3 * r + 2] = ha[s].z; }
+        if (dens) dens[r] = hr[s].x;
+        if (pres) pres[r] = hr[s].y;
+        if (dden) dden[r] = hd[s];
+        if (vpres) vpres[r] = hp[s];
+        if (ids) ids[r] = (int32_t)hv[s].w;
+    }
+    return 0;
+}
+
+int sph_get_scalars(sph_ctx* c, double out[5]) {
+    if (!c || !out) return fail(1, "null argument");
+    HIPC(hipStreamSynchronize(c->stream));
+    double sc[SC_COUNT];
+    int is[IS_COUNT];
+    HIPC(hipMemcpy(sc, c->scal, sizeof(sc), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(is, c->iscal, sizeof(is), hipMemcpyDeviceToHost));
+    out[0] = sc[SC_DT];
+    out[1] = sc[SC_DTAVG];
+    out[2] = sc[SC_TIME];
+    out[3] = is[IS_IT];
+    const int n = is[IS_N];
+    std::vector<int> nn(n);
+    HIPC(hipMemcpy(nn.data(), c->nnb, sizeof(int) * n, hipMemcpyDeviceToHost));
+    long long pc = 0;
+    for (int i = 0; i < n; ++i) pc += nn[i] - 1;
+    out[4] = (double)pc;
+    return 0;
+}
+
+int sph_pair_counter(sph_ctx* c, unsigned long long* total) {
+    if (!c || !total) return fail(1, "null argument");
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpy(total, c->pairs, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sph_get_neighbour_lists(sph_ctx* c, int32_t* count, int32_t* ids, int32_t stride) {
+    if (!c || !count) return fail(1, "null argument");
+    if (stride < c->P.nmaxpartj && ids) return fail(3, "stride smaller than nmaxpartj");
+    HIPC(hipStreamSynchronize(c->stream));
+    int n;
+    HIPC(hipMemcpy(&n, c->iscal + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    const size_t N = c->P.cap;
+    std::vector<int> nn(n);
+    std::vector<double4> hv(n);
+    HIPC(hipMemcpy(nn.data(), c->nnb, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hv.data(), c->vel[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
+    int kmax = 0;
+    for (int i = 0; i < n; ++i) kmax = std::max(kmax, nn[i]);
+    std::vector<uint32_t> nb((size_t)kmax * N);
+    if (kmax) HIPC(hipMemcpy(nb.data(), c->nbr, sizeof(uint32_t) * kmax * N, hipMemcpyDeviceToHost));
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return hv[a].w < hv[b].w; });
+    for (int r = 0; r < n; ++r) {
+        const int s = order[r];
+        count[r] = nn[s];
+        if (ids)
+            for (int k = 0; k < nn[s]; ++k)
+                ids[(size_t)r * stride + k] = (int32_t)hv[nb[(size_t)k * N + s] & NB_MASK].w;
+    }
+    return 0;
+}
+
+int sph_dt(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (int rc = expect(c, PH_READY, "sph_dt")) return rc;
+    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8, c->stream));
+    run_dt(c, 1, true);
+    c->phase = PH_DT;
+    return 0;
+}
+
+int sph_momentum(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (int rc = expect(c, PH_DT, "sph_momentum")) return rc;
+    run_momentum(c);
+    c->phase = PH_MOM;
+    return 0;
+}
+int sph_leapfrog_mom(sph_ctx* c) {  // fused into sph_momentum
+    if (!c) return fail(1, "null argument");
+    return expect(c, PH_MOM, "sph_leapfrog_mom");
+}
+int sph_velocity_smoothing(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (int rc = expect(c, PH_MOM, "sph_velocity_smoothing")) return rc;
+    run_vsmooth(c);
+    c->phase = PH_VS;
+    return 0;
+}
+int sph_leapfrog_traj(sph_ctx* c) {  // fused into sph_velocity_smoothing
+    if (!c) return fail(1, "null argument");
+    return expect(c, PH_VS, "sph_leapfrog_traj");
+}
+int sph_sort(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (int rc = expect(c, PH_VS, "sph_sort")) return rc;
+    run_sort(c);
+    c->phase = PH_SORT;
+    return 0;
+}
+int sph_neighbours(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (c->phase != PH_SORT && c->phase != PH_SORTED)
+        return fail(120, "sph_neighbours called out of the time_step_loop order");
+    run_neighbours(c);
+    c->phase = c->phase == PH_SORTED ? PH_SORTED : PH_NB;
+    return 0;
+}
+int sph_boundary_tab(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (c->phase != PH_NB && c->phase != PH_SORTED)
+        return fail(120, "sph_boundary_tab called out of the time_step_loop order");
+    run_btab(c);
+    c->phase = c->phase == PH_SORTED ? PH_SORTED : PH_BT;
+    return 0;
+}
+int sph_continuity(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (int rc = expect(c, PH_BT, "sph_continuity")) return rc;
+    run_continuity(c);
+    c->phase = PH_CONT;
+    return 0;
+}
+int sph_leapfrog_cont(sph_ctx* c) {  // fused into sph_continuity
+    if (!c) return fail(1, "null argument");
+    return expect(c, PH_CONT, "sph_leapfrog_cont");
+}
+int sph_eos(sph_ctx* c) {  // fused into sph_continuity
+    if (!c) return fail(1, "null argument");
+    return expect(c, PH_CONT, "sph_eos");
+}
+int sph_pressure_smoothing(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (int rc = expect(c, PH_CONT, "sph_pressure_smoothing")) return rc;
+    run_psmooth(c);
+    c->phase = PH_READY;
+    HIPC(hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    return check_flags(c);
+}
+
+int sph_preloop(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    if (c->phase != PH_SORTED) return fail(120, "sph_preloop needs a freshly uploaded state");
+    run_neighbours(c);
+    run_btab(c);
+    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8, c->stream));
+    run_dt(c, 0, true);
+    // the first step's time_step_duration sees the same velocities again
+    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8, c->stream));
+    k_dtmin<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->vel[0], c->dtmin, c->iscal);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    c->phase = PH_READY;
+    return check_flags(c);
+}
+
+int sph_step_async(sph_ctx* c, int32_t nsteps) {
+    if (!c) return fail(1, "null argument");
+    if (c->phase != PH_READY) return fail(120, "sph_step needs the state at a time-step boundary");
+    for (int s = 0; s < nsteps; ++s) {
+        run_dt(c, 1, false);
+        run_momentum(c);
+        run_vsmooth(c);
+        run_sort(c);
+        run_neighbours(c);
+        run_btab(c);
+        run_continuity(c);
+        run_psmooth(c);
+    }
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int sph_sync(sph_ctx* c) {
+    if (!c) return fail(1, "null argument");
+    HIPC(hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    return check_flags(c);
+}
+
+int sph_step(sph_ctx* c, int32_t nsteps) {
+    int rc = sph_step_async(c, nsteps);
+    if (rc) return rc;
+    return sph_sync(c);
+}
+
+int sph_set_timing(sph_ctx* c, int32_t on) {
+    if (!c) return fail(1, "null argument");
+    c->timing = on != 0;
+    c->timers.clear();
+    return 0;
+}
+
+int sph_get_timing(sph_ctx* c, char* names, int32_t names_len, double* ms, int32_t* calls,
+                   int32_t max_kernels) {
+    if (!c) return fail(1, "null argument");
+    std::string all;
+    int k = 0;
+    for (auto& t : c->timers) {
+        if (k >= max_kernels) break;
+        all += t.name + ";";
+        ms[k] = t.ms;
+        calls[k] = t.calls;
+        ++k;
+    }
+    if (names && names_len > 0) {
+        std::strncpy(names, all.c_str(), names_len - 1);
+        names[names_len - 1] = 0;
+    }
+    return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// Device-side structures shared by the kernels and the host orchestration.
+#pragma once
+#include <stdint.h>
+
+#define SPH_MB 8  // boundary-table slots per particle (close SA-SPH faces)
+
+namespace sph {
+
+constexpr int NB_SHIFT = 27;
+constexpr uint32_t NB_MASK = (1u << NB_SHIFT) - 1u;
+
+// indices into the device scalar blocks
+enum { SC_DT = 0, SC_DTPREV, SC_DTAVG, SC_TIME, SC_DTVEL, SC_TV1, SC_TP1, SC_COUNT };
+enum {
+    IS_N = 0, IS_IT, IS_ONGOING, IS_NB_OVERFLOW, IS_BT_OVERFLOW, IS_REMOVED, IS_UNSUPPORTED,
+    IS_NPREV, IS_COUNT
+};
+
+// per-context constants, passed by value to every kernel
+struct DP {
+    int cap, nmaxpartj, nmax, semi, f32, nf;
+    int ncd[3];
+    double extr_lo[3], dcd[3];
+    double h, dx, coefke, coefkacl, eta, eta2, squareh, square_doubleh, doubleh;
+    double ke_coef, kacl_coef, ktdelta, GI;
+    double grav[3], prif, den0, eps, celerita, alfamon, kin_visc;
+    double cfl, vsc_coeff, pesodt, tetap, tetav;
+    // FP32-mixed copies
+    float f_h, f_unosuh, f_unosusquareh, f_eta, f_eta2, f_ke_coef, f_kacl_coef, f_coefke;
+    float f_dcd[3], f_alfamon, f_celerita, f_kin_visc;
+};
+
+// one SA-SPH frontier face (BoundaryFace + its Tratto/Partz data)
+struct DFace {
+    double T[9];    // column-major, T[6..8] = normal
+    double LX[8];   // node local coordinates (x,y) x 4
+    double GX[12];  // node global coordinates x 4
+    double area, slip;
+    double vel[3], tvel[3];
+    int nodes, btype, mode, lam;
+};
+
+// one BoundaryDataTab record (Hybrid_allocation_module.f90:796-805)
+struct BTab {
+    int face;
+    int pad;
+    double loc[3];
+    double bi[8];
+    double ig[9];
+};
+
+}  // namespace sph

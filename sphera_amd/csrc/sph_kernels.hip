@@ -1,0 +1,1103 @@
+// MI355X (gfx950) kernels of the SPHERA v10 particle-interaction hot path.
+//
+// Layout in HBM (one context = one solve, capacity N particles), all arrays in
+// cell-sorted order (cells ascending, particle id ascending inside a cell):
+//   xm   double4 (x, y, z, mass)          x2 buffers (trajectory writes the other)
+//   vel  double4 (vx, vy, vz, id)         x3 buffers (momentum -> smoothing -> sort)
+//   rp   double2 (rho, p)                 x3 buffers (sort -> continuity -> smoothing)
+//   lam  int32 laminar flag               x2 buffers
+//   acc  double4, dden, vpres             outputs of the passes
+//   FP32-mixed mirrors: xmf float4 (cell-relative x,y,z, mass), velf float4,
+//   rpf float2, written by the same kernels that write the FP64 fields.
+//   Neighbour list: nbr[k*N + i] (column-major, coalesced over i), the sorted
+//   index of the k-th neighbour in bits 0..26 and its stencil cell code
+//   (dx+1)+3(dy+1)+9(dz+1) in bits 27..31; count in nnb[i].
+//
+// FP64 kernels are compiled with -ffp-contract=off and evaluate every
+// expression in the order of the cited reference line, so sums over a list in
+// the reference's order reproduce the reference's values.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sph_device.h"
+
+namespace sph {
+
+// ----------------------------------------------------------------------------
+// grid helpers: CellNumber.f90:53-59, ParticleCellNumber.f90:52-62,
+// CellIndices.f90:51-63 (1-based cell ids, 0 = outside)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ int d_cell_number(const DP& P, int i, int j, int k) {
+    if (i < 1 || i > P.ncd[0] || j < 1 || j > P.ncd[1] || k < 1 || k > P.ncd[2]) return 0;
+    return ((k - 1) * P.ncd[1] + (j - 1)) * P.ncd[0] + i;
+}
+__device__ __forceinline__ int d_particle_cell(const DP& P, double x, double y, double z) {
+    const double xp = x - P.extr_lo[0];
+    const double yp = y - P.extr_lo[1];
+    const double zp = z - P.extr_lo[2];
+    const int i = (int)ceil(xp / P.dcd[0]);
+    const int k = (int)ceil(zp / P.dcd[2]);
+    const int j = (int)ceil(yp / P.dcd[1]);
+    return d_cell_number(P, i, j, k);
+}
+__device__ __forceinline__ void d_cell_indices(const DP& P, int nc, int& i, int& j, int& k) {
+    const int nij = P.ncd[0] * P.ncd[1];
+    k = (nc - 1) / nij + 1;
+    const int ncij = nc - nij * (k - 1);
+    j = (ncij - 1) / P.ncd[0] + 1;
+    i = ncij - P.ncd[0] * (j - 1);
+}
+
+// 8 XCDs each with their own L2: consecutive workgroups are dispatched round
+// robin over the XCDs, so remap them to give every XCD one contiguous range
+// of cell-sorted particles (their neighbour data then share that XCD's L2).
+__device__ __forceinline__ int xcd_block(int nblocks) {
+    const int b = blockIdx.x;
+    const int per = (nblocks + 7) >> 3;
+    const int xcd = b & 7;
+    const int r = b >> 3;
+    const int full = nblocks - (per - 1) * 8;  // XCDs that get `per` blocks
+    int base;
+    if (xcd < full) base = xcd * per;
+    else base = full * per + (xcd - full) * (per - 1);
+    return base + r;
+}
+
+// ----------------------------------------------------------------------------
+// kernel function values: CalcVarLength.f90:243-265
+// ----------------------------------------------------------------------------
+struct K64 { double K1, K2, K3, W; };
+
+__device__ __forceinline__ void kvals64(const DP& P, double rij, bool need12, bool need3,
+                                        bool needW, K64& K) {
+    const double rij2 = rij;
+    const double r = sqrt(rij);
+    const double q = r / P.h;
+    const double q2 = rij2 / P.squareh;
+    const int iq = (int)q;
+    K.K1 = K.K2 = K.K3 = K.W = 0.0;
+    if (iq >= 2) return;
+    const double denom = 1.0 / (r + P.eta);
+    if (need12) {
+        double gm = -2.0 * q + 1.5 * q2;
+        if (iq > 0) gm = -gm + q2 - 2.0;
+        gm = gm * P.ke_coef;
+        K.K1 = gm * denom;
+        K.K2 = K.K1 / (rij2 + P.eta2);
+    }
+    if (need3) {
+        double gw = -12.0 - 3.0 * q2 + 12.0 * q;
+        gw = gw * P.kacl_coef;
+        K.K3 = gw * denom;
+    }
+    if (needW) {
+        double wu = 0.666666667 + q2 * (q * 0.5 - 1.0);
+        if (iq > 0) wu = (2.0 - q) * (2.0 - q) * (2.0 - q) * 0.166666666667;
+        K.W = wu * P.coefke;
+    }
+}
+
+struct K32 { float K1, K2, K3, W; };
+
+// FP32 form: same piecewise cubic spline, single-precision hardware sqrt/rcp
+__device__ __forceinline__ void kvals32(const DP& P, float rij, bool need12, bool need3,
+                                        bool needW, K32& K) {
+    const float r = __builtin_amdgcn_sqrtf(rij);
+    const float q = r * P.f_unosuh;
+    const float q2 = rij * P.f_unosusquareh;
+    K.K1 = K.K2 = K.K3 = K.W = 0.0f;
+    if (q >= 2.0f) return;
+    const bool outer = q >= 1.0f;
+    const float denom = __builtin_amdgcn_rcpf(r + P.f_eta);
+    if (need12) {
+        float gm = -2.0f * q + 1.5f * q2;
+        if (outer) gm = -gm + q2 - 2.0f;
+        K.K1 = gm * P.f_ke_coef * denom;
+        K.K2 = K.K1 * __builtin_amdgcn_rcpf(rij + P.f_eta2);
+    }
+    if (need3) K.K3 = (-12.0f - 3.0f * q2 + 12.0f * q) * P.f_kacl_coef * denom;
+    if (needW) {
+        const float t = 2.0f - q;
+        const float wu = outer ? t * t * t * 0.166666666667f : 0.666666667f + q2 * (q * 0.5f - 1.0f);
+        K.W = wu * P.f_coefke;
+    }
+}
+
+// cell-relative FP32 separation x_i - x_j from the stencil code of j
+__device__ __forceinline__ void rag32(const DP& P, float4 xi, float4 xj, uint32_t code, float& r0,
+                                      float& r1, float& r2) {
+    const int c = (int)(code >> NB_SHIFT);
+    const int dx = c % 3 - 1, dy = (c / 3) % 3 - 1, dz = c / 9 - 1;
+    r0 = (xi.x - xj.x) - (float)dx * P.f_dcd[0];
+    r1 = (xi.y - xj.y) - (float)dy * P.f_dcd[1];
+    r2 = (xi.z - xj.z) - (float)dz * P.f_dcd[2];
+}
+
+// ----------------------------------------------------------------------------
+// SA-SPH geometry: LocalNormalCoordinates.f90:55-87 + IsPointInternal.f90
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ bool d_point_internal(const DFace& F, double px0, double px1) {
+    const int fkod = F.nodes - 2;
+    const double DueArea = (double)(3 - fkod) * F.area;
+    const double Usu = 1.0 / DueArea;
+    // (jj,kk) node pairs of LocalNormalCoordinates for ii=1,2
+    const int a0 = fkod == 1 ? 1 : 2, b0 = fkod == 1 ? 2 : 3;  // ii=1
+    const int a1 = fkod == 1 ? 2 : 3, b1 = 0;                   // ii=2
+    double csi[3];
+    {
+        const double xj = F.LX[2 * a0], yj = F.LX[2 * a0 + 1];
+        const double xk = F.LX[2 * b0], yk = F.LX[2 * b0 + 1];
+        const double AA = xj * yk - xk * yj, BB = yj - yk, CC = xk - xj;
+        csi[0] = (AA + BB * px0 + CC * px1) * Usu;
+    }
+    {
+        const double xj = F.LX[2 * a1], yj = F.LX[2 * a1 + 1];
+        const double xk = F.LX[2 * b1], yk = F.LX[2 * b1 + 1];
+        const double AA = xj * yk - xk * yj, BB = yj - yk, CC = xk - xj;
+        csi[1] = (AA + BB * px0 + CC * px1) * Usu;
+    }
+    csi[2] = 1.0 - (csi[0] + csi[1]);
+    if (fkod == 1) {
+        for (int ii = 0; ii < 3; ++ii)
+            if (csi[ii] < 0.0 || csi[ii] > 1.0) return false;
+        return true;
+    }
+    if (fkod == 2) {
+        for (int ii = 0; ii < 2; ++ii)
+            if (csi[ii] < 0.0 || csi[ii] > 1.0) return false;
+        return true;
+    }
+    return false;
+}
+
+// ----------------------------------------------------------------------------
+// sort: OrdGrid1.f90:58-105 as a counting sort on the device
+// ----------------------------------------------------------------------------
+__global__ void k_cell_count(DP P, const double4* __restrict__ xm, int* __restrict__ key,
+                             int* __restrict__ slot, int* __restrict__ cnt, int* iscal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = iscal[IS_N];
+    if (i >= n) return;
+    const double4 x = xm[i];
+    int c = d_particle_cell(P, x.x, x.y, x.z);
+    int k;
+    if (c <= 0) {  // out of the grid: removed (OrdGrid1.f90:71-82)
+        k = P.nmax;
+        atomicAdd(&iscal[IS_REMOVED], 1);
+    } else {
+        k = c - 1;
+    }
+    key[i] = k;
+    slot[i] = atomicAdd(&cnt[k], 1);
+}
+
+// wave64 inclusive scan with DPP-free shuffles
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// block-level exclusive scan of 4*256 elements; writes per-block totals
+__global__ __launch_bounds__(256) void k_scan_local(const int* __restrict__ in, int* __restrict__ out,
+                                                    int* __restrict__ bsum, int m) {
+    __shared__ int wsum[4];
+    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    int v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = (base + q < m) ? in[base + q] : 0;
+    const int tsum = v[0] + v[1] + v[2] + v[3];
+    const int incl = wave_incl_scan(tsum);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wid; ++w) woff += wsum[w];
+    int run = woff + incl - tsum;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (base + q < m) out[base + q] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = woff + incl;
+}
+
+// exclusive scan of the block totals by one workgroup (nb <= 1024*64)
+__global__ __launch_bounds__(1024) void k_scan_sums(int* __restrict__ bsum, int nb, int* total) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        const int idx = base + threadIdx.x;
+        const int v = idx < nb ? bsum[idx] : 0;
+        const int incl = wave_incl_scan(v);
+        const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < wid; ++w) woff += wsum[w];
+        const int c0 = carry;
+        if (idx < nb) bsum[idx] = c0 + woff + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c0 + woff + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] += bsum[i >> 10];
+}
+
+__global__ void k_scatter(const int* __restrict__ key, const int* __restrict__ slot,
+                          const int* __restrict__ start, int* __restrict__ perm, const int* iscal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= iscal[IS_N]) return;
+    perm[start[key[i]] + slot[i]] = i;
+}
+
+// ascending particle id inside every cell (the order OrdGrid1's 4th loop
+// produces, and hence the reference's neighbour-list order)
+__global__ void k_cell_sort(DP P, const int* __restrict__ start, int* __restrict__ perm,
+                            const double4* __restrict__ vel) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P.nmax) return;
+    const int b = start[c], e = start[c + 1];
+    for (int a = b + 1; a < e; ++a) {
+        const int pv = perm[a];
+        const double idv = vel[pv].w;
+        int q = a - 1;
+        while (q >= b && vel[perm[q]].w > idv) {
+            perm[q + 1] = perm[q];
+            --q;
+        }
+        perm[q + 1] = pv;
+    }
+}
+
+__global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restrict__ key,
+                         const double4* __restrict__ xm_s, const double4* __restrict__ vel_s,
+                         const double2* __restrict__ rp_s, const int* __restrict__ lam_s,
+                         const double4* __restrict__ acc_s, double4* __restrict__ xm_d,
+                         double4* __restrict__ vel_d, double2* __restrict__ rp_d,
+                         int* __restrict__ lam_d, double4* __restrict__ acc_d,
+                         int* __restrict__ cell_d, float4* __restrict__ xmf, float4* __restrict__ velf,
+                         float2* __restrict__ rpf, const int* __restrict__ start, int* iscal) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nin = start[P.nmax];  // particles still inside the grid
+    if (p == 0) {
+        iscal[IS_NPREV] = iscal[IS_N];
+        iscal[IS_N] = nin;
+    }
+    if (p >= nin) return;
+    const int s = perm[p];
+    const double4 x = xm_s[s];
+    const double4 v = vel_s[s];
+    const double2 r = rp_s[s];
+    xm_d[p] = x;
+    vel_d[p] = v;
+    rp_d[p] = r;
+    lam_d[p] = lam_s[s];
+    acc_d[p] = acc_s[s];
+    const int c = key[s] + 1;
+    cell_d[p] = c;
+    if (P.f32) {
+        int ci, cj, ck;
+        d_cell_indices(P, c, ci, cj, ck);
+        const double ox = P.extr_lo[0] + P.dcd[0] * (double)(ci - 1);
+        const double oy = P.extr_lo[1] + P.dcd[1] * (double)(cj - 1);
+        const double oz = P.extr_lo[2] + P.dcd[2] * (double)(ck - 1);
+        xmf[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
+        velf[p] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.0f);
+        rpf[p] = make_float2((float)r.x, (float)r.y);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// neighbour lists: CalcVarLength.f90:133-242 (stencil y -> x -> z, ascending
+// particle id in a cell, FP64 r^2 <= (2h)^2, self included, truncation past
+// NMAXPARTJ)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_neighbours(DP P, const double4* __restrict__ xm,
+                                                    const int* __restrict__ cell,
+                                                    const int* __restrict__ start,
+                                                    uint32_t* __restrict__ nbr,
+                                                    int* __restrict__ nnb, int* iscal,
+                                                    unsigned long long* pairs) {
+    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int n = iscal[IS_N];
+    int cnt = 1;  // lanes past the end contribute 0 pairs below
+    if (i < n) {
+        int ig, jg, kg;
+        d_cell_indices(P, cell[i], ig, jg, kg);
+        const double4 xi = xm[i];
+        const int mj = P.nmaxpartj;
+        const size_t N = P.cap;
+        cnt = 0;
+        bool ov = false;
+        for (int dy = -1; dy <= 1 && !ov; ++dy)
+            for (int dx = -1; dx <= 1 && !ov; ++dx)
+                for (int dz = -1; dz <= 1 && !ov; ++dz) {
+                    const int cn = d_cell_number(P, ig + dx, jg + dy, kg + dz);
+                    if (cn == 0) continue;
+                    const int b = start[cn - 1], e = start[cn];
+                    const uint32_t code = (uint32_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) << NB_SHIFT;
+                    for (int m = b; m < e; ++m) {
+                        const double4 xj = xm[m];
+                        const double r0 = xi.x - xj.x;
+                        const double r1 = xi.y - xj.y;
+                        const double r2 = xi.z - xj.z;
+                        const double rij = r0 * r0 + r1 * r1 + r2 * r2;
+                        if (rij > P.square_doubleh) continue;
+                        if (cnt >= mj) { ov = true; break; }
+                        nbr[(size_t)cnt * N + i] = (uint32_t)m | code;
+                        ++cnt;
+                    }
+                }
+        nnb[i] = cnt;
+        if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
+    }
+    // pair-interaction counter: sum_i (nPartIntorno(i) - 1), self excluded
+    unsigned long long v = (unsigned long long)(cnt - 1);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(pairs, v);
+}
+
+// ----------------------------------------------------------------------------
+// boundary table: ComputeBoundaryDataTab_3D.f90:62-106,
+// FindCloseBoundaryFaces3D.f90:78-188, ComputeBoundaryVolumeIntegrals_P0.f90:
+// 76-155, InterpolateTable.f90:58-82 (FP64, both precision modes)
+// ----------------------------------------------------------------------------
+__device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __restrict__ kt,
+                                   const double* __restrict__ bit, const double locx[3],
+                                   double bi[8], double ig[9]) {
+    double IntWdV = 0.0, IntdWrm1dV = 0.0, IntGWZrm1dV = 0.0;
+    double G0 = 0.0, G1 = 0.0, G2 = 0.0;
+    double M00 = 0.0, M01 = 0.0, M02 = 0.0, M11 = 0.0, M12 = 0.0, M22 = 0.0;
+    double J0 = 0.0, J1 = 0.0, J2 = 0.0, J3 = 0.0;
+    for (int q = 0; q < 8; ++q) bi[q] = 0.0;
+    for (int q = 0; q < 9; ++q) ig[q] = 0.0;
+    const double zpmin = 0.05 * P.h;
+    double L0 = locx[0], L1 = locx[1], L2 = locx[2];
+    if (L2 < 0.0) return;
+    if (L2 < zpmin) L2 = zpmin;
+    double robpre = 2.1;
+    for (int ipk = 0; ipk < 256; ++ipk) {
+        const double* b = &bit[ipk * 7];
+        const double Pj0 = L0 + b[0], Pj1 = L1 + b[1], Pj2 = L2 + b[2];
+        if (!(Pj2 <= 0.0)) continue;
+        const double ts = L2 / (L2 - Pj2);
+        const double X0 = L0 + (Pj0 - L0) * ts;
+        const double X1 = L1 + (Pj1 - L1) * ts;
+        if (!d_point_internal(F, X0, X1)) continue;
+        double dist = 0.0;
+        {
+            const double d0 = X0 - L0, d1 = X1 - L1, d2 = 0.0 - L2;
+            dist = dist + d0 * d0;
+            dist = dist + d1 * d1;
+            dist = dist + d2 * d2;
+        }
+        dist = sqrt(dist);
+        const double da = b[3];
+        const double rob = dist / P.h;
+        if (fabs(rob - robpre) > 1.0e-3) {
+            const double deltaX = P.ktdelta;
+            const int nr0 = (int)((rob - kt[0]) / deltaX);
+            double iv0, iv1, iv2, iv3;
+            if (nr0 <= 0) {
+                iv0 = kt[1]; iv1 = kt[2]; iv2 = kt[3]; iv3 = kt[4];
+            } else if (nr0 >= 40) {
+                iv0 = kt[40 * 5 + 1]; iv1 = kt[40 * 5 + 2]; iv2 = kt[40 * 5 + 3]; iv3 = kt[40 * 5 + 4];
+            } else {
+                const double* r0 = &kt[nr0 * 5];
+                const double* r1 = &kt[(nr0 + 1) * 5];
+                const double csi = (rob - r0[0]) / deltaX;
+                iv0 = r0[1] + csi * (r1[1] - r0[1]);
+                iv1 = r0[2] + csi * (r1[2] - r0[2]);
+                iv2 = r0[3] + csi * (r1[3] - r0[3]);
+                iv3 = r0[4] + csi * (r1[4] - r0[4]);
+            }
+            J0 = iv0 * da;
+            J1 = iv1 * da;
+            J2 = iv2 * da;
+            J3 = iv3 * da;
+            robpre = rob;
+        }
+        const double R0 = b[4], R1 = b[5], R2 = b[6];
+        const double RZ2 = R2 * R2;
+        IntWdV = IntWdV + J0;
+        IntdWrm1dV = IntdWrm1dV + J1;
+        IntGWZrm1dV = IntGWZrm1dV + RZ2 * J1;
+        G0 = G0 + R0 * J2;
+        M00 = M00 + R0 * R0 * J3;
+        M01 = M01 + R0 * R1 * J3;
+        M02 = M02 + R0 * R2 * J3;
+        G1 = G1 + R1 * J2;
+        M11 = M11 + R1 * R1 * J3;
+        M12 = M12 + R1 * R2 * J3;
+        G2 = G2 + R2 * J2;
+        M22 = M22 + R2 * R2 * J3;
+    }
+    if (L2 == zpmin) IntWdV = d_point_internal(F, L0, L1) ? 0.5 : 0.0;
+    bi[1] = IntWdV;
+    bi[2] = IntdWrm1dV;
+    bi[3] = G0;
+    bi[4] = G1;
+    bi[5] = G2;
+    bi[6] = IntGWZrm1dV;
+    // IntGiWrRdV column-major
+    ig[0] = M00; ig[1] = M01; ig[2] = M02;
+    ig[3] = M01; ig[4] = M11; ig[5] = M12;
+    ig[6] = M02; ig[7] = M12; ig[8] = M22;
+}
+
+__global__ __launch_bounds__(256) void k_boundary_tab(DP P, const DFace* __restrict__ faces,
+                                                      const int* __restrict__ gcount,
+                                                      const int* __restrict__ gstart,
+                                                      const int* __restrict__ gvec,
+                                                      const double* __restrict__ kt_g,
+                                                      const double* __restrict__ bit_g,
+                                                      const double4* __restrict__ xm,
+                                                      int* __restrict__ bn, BTab* __restrict__ bt,
+                                                      int* iscal) {
+    __shared__ double kt[41 * 5];
+    __shared__ double bit[256 * 7];
+    for (int t = threadIdx.x; t < 41 * 5; t += blockDim.x) kt[t] = kt_g[t];
+    for (int t = threadIdx.x; t < 256 * 7; t += blockDim.x) bit[t] = bit_g[t];
+    __syncthreads();
+    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int n = iscal[IS_N];
+    if (i >= n) return;
+    const double4 x = xm[i];
+    const double xc[3] = {x.x, x.y, x.z};
+    const int nc = d_particle_cell(P, x.x, x.y, x.z);
+    int ncbf = 0;
+    int clob[SPH_MB];
+    double lz[SPH_MB];
+    if (nc > 0) {
+        int ic, jc, kc;
+        d_cell_indices(P, nc, ic, jc, kc);
+        for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int dk = -1; dk <= 1; ++dk) {
+                    const int cn = d_cell_number(P, ic + di, jc + dj, kc + dk);
+                    if (cn == 0) continue;
+                    const int nfc = gcount[cn - 1];
+                    if (nfc <= 0) continue;
+                    const int ini = gstart[cn - 1] - 1;
+                    for (int fl = ini; fl < ini + nfc; ++fl) {
+                        const int f = gvec[fl];
+                        bool there = false;
+                        for (int q = 0; q < ncbf; ++q) there |= (clob[q] == f);
+                        if (there) continue;
+                        const DFace& F = faces[f];
+                        const double pinmin = (F.btype == 3) ? -P.doubleh : 0.0;
+                        const double* ref = &F.GX[3 * (F.nodes - 1)];
+                        double PX[3];
+                        for (int sdi = 0; sdi < 3; ++sdi) {
+                            double s = 0.0;
+                            for (int sdj = 0; sdj < 3; ++sdj) s = s + F.T[3 * sdi + sdj] * (xc[sdj] - ref[sdj]);
+                            PX[sdi] = s;
+                        }
+                        const double pin = PX[2];
+                        if (pin >= pinmin && pin < P.doubleh) {
+                            const bool need_in = (F.btype == 3 || F.btype == 4 || F.btype == 5);
+                            if (!need_in || d_point_internal(F, PX[0], PX[1])) {
+                                if (ncbf >= SPH_MB) {
+                                    atomicAdd(&iscal[IS_BT_OVERFLOW], 1);
+                                    continue;
+                                }
+                                clob[ncbf] = f;
+                                lz[ncbf] = pin;
+                                ++ncbf;
+                            }
+                        }
+                    }
+                }
+    }
+    bn[i] = ncbf;
+    for (int q = 0; q < ncbf; ++q) {
+        const DFace& F = faces[clob[q]];
+        const double* ref = &F.GX[3 * (F.nodes - 1)];
+        double loc[3];
+        for (int sdi = 0; sdi < 2; ++sdi) {
+            double s = 0.0;
+            for (int sdj = 0; sdj < 3; ++sdj) s = s + F.T[3 * sdi + sdj] * (xc[sdj] - ref[sdj]);
+            loc[sdi] = s;
+        }
+        loc[2] = lz[q];
+        BTab rec;
+        rec.face = clob[q];
+        rec.loc[0] = loc[0];
+        rec.loc[1] = loc[1];
+        rec.loc[2] = loc[2];
+        d_volume_integrals(P, F, kt, bit, loc, rec.bi, rec.ig);
+        bt[(size_t)q * P.cap + i] = rec;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// dt: time_step_duration.f90:74-116.  Positive doubles order like their bit
+// patterns, so the minimum over particles is an exact atomicMin on uint64.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double d_dt_cfl(const DP& P, double vx, double vy, double vz) {
+    const double U = sqrt(vx * vx + vy * vy + vz * vz);
+    return P.cfl * 2.0 * P.h / (P.celerita + U);
+}
+
+__device__ __forceinline__ void block_min_to(unsigned long long v, unsigned long long* dst) {
+    __shared__ unsigned long long red[16];
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(v, o, 64);
+        v = t < v ? t : v;
+    }
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] < m ? red[w] : m;
+        atomicMin(dst, m);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dtmin(DP P, const double4* __restrict__ vel,
+                                               unsigned long long* dtmin, const int* iscal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = 0x7FF0000000000000ull;
+    if (i < iscal[IS_N]) {
+        const double4 u = vel[i];
+        v = (unsigned long long)__double_as_longlong(d_dt_cfl(P, u.x, u.y, u.z));
+    }
+    block_min_to(v, dtmin);
+}
+
+// step prologue (time_step_loop.f90:180-186): dt_previous_step, dt, time
+__global__ void k_dt_begin(DP P, double* scal, unsigned long long* dtmin, int* iscal, int advance) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (advance) {
+        iscal[IS_IT] += 1;
+        iscal[IS_ONGOING] = iscal[IS_IT];
+        scal[SC_DTPREV] = scal[SC_DT];
+    }
+    const int og = iscal[IS_ONGOING];
+    double dtmin_v = 3.4e38;
+    double dt_vis = 3.4e38;
+    if (P.kin_visc > 1.0e-24) {
+        dt_vis = P.vsc_coeff * P.squareh / (0.5 * P.kin_visc);
+        dtmin_v = fmin(dtmin_v, dt_vis);
+    }
+    const double mparts = __longlong_as_double((long long)*dtmin);
+    if (iscal[IS_N] == 0) {
+        scal[SC_DT] = 0.0;
+    } else {
+        dtmin_v = fmin(dtmin_v, fmin(mparts, dt_vis));
+        if (og == 0) scal[SC_DT] = dtmin_v;
+        else scal[SC_DT] = (1.0 - P.pesodt) * dtmin_v + P.pesodt * scal[SC_DTAVG];
+    }
+    const double dt = scal[SC_DT];
+    if (og == 0) scal[SC_DTAVG] = dt;
+    else scal[SC_DTAVG] = (scal[SC_DTAVG] * (double)(og - 1) + dt) / (double)og;
+    if (advance) scal[SC_TIME] = scal[SC_TIME] + dt;
+    scal[SC_DTVEL] = 0.5 * (dt + scal[SC_DTPREV]);
+    scal[SC_TV1] = P.tetav * P.celerita * dt / P.h;
+    scal[SC_TP1] = P.tetap * P.celerita * dt / P.h;
+    *dtmin = 0x7FF0000000000000ull;
+}
+
+// ----------------------------------------------------------------------------
+// momentum: RHS_momentum_equation.f90:72-179 with
+// AddBoundaryContributions_to_ME3D.f90:118-336 (FP64), inter_EqMoto.f90:
+// 108-283, viscomon.f90:54-71, viscomorris.f90:60-71,
+// AddElasticBoundaryReaction_3D.f90:67-98; fused Leapfrog_momentum.f90:66-86
+// ----------------------------------------------------------------------------
+template <bool F32>
+__global__ __launch_bounds__(256) void k_momentum(
+    DP P, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
+    const double4* __restrict__ xm, const double4* __restrict__ vel, const double2* __restrict__ rp,
+    const float4* __restrict__ xmf, const float4* __restrict__ velf, const float2* __restrict__ rpf,
+    const int* __restrict__ lam_in, int* __restrict__ lam_out, const uint32_t* __restrict__ nbr,
+    const int* __restrict__ nnb, double4* __restrict__ acc_out, double4* __restrict__ vel_out,
+    float4* __restrict__ velf_out, const double* scal, int* iscal) {
+    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int n = iscal[IS_N];
+    if (i >= n) return;
+    const double h = P.h;
+    const double4 xi = xm[i];
+    const double4 vi = vel[i];   // var == vel here
+    const double2 rpi = rp[i];
+    double tp[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0}, tv[3] = {0.0, 0.0, 0.0};
+    const int ncbf = P.semi ? bn[i] : 0;
+    const double v_i[3] = {vi.x, vi.y, vi.z};
+    if (ncbf > 0) {
+        double ALEt[3] = {0, 0, 0}, gradpt[3] = {0, 0, 0}, VM[3] = {0, 0, 0}, VS[3] = {0, 0, 0};
+        const int lam_prev = lam_in[i];
+        for (int q = 0; q < ncbf; ++q) {
+            const BTab& B = bt[(size_t)q * P.cap + i];
+            const DFace& F = faces[B.face];
+            if (F.btype == 2) continue;
+            if (F.btype != 0 && F.btype != 1) {
+                atomicAdd(&iscal[IS_UNSUPPORTED], 1);
+                break;
+            }
+            if (!(B.loc[2] > 0.0)) continue;
+            double dvij[3], dvn = 0.0;
+            for (int sd = 0; sd < 3; ++sd) {
+                dvij[sd] = 2.0 * (v_i[sd] - F.vel[sd]);
+                dvn = dvn + F.T[6 + sd] * dvij[sd];
+            }
+            double gl[3];
+            for (int sd = 0; sd < 3; ++sd) {
+                double a = 0.0;
+                for (int sj = 0; sj < 3; ++sj) a = a + F.T[3 * sd + sj] * P.grav[sj];
+                gl[sd] = a;
+            }
+            double gloc[3];
+            for (int r = 0; r < 3; ++r) {
+                double a = 0.0;
+                for (int k = 0; k < 3; ++k) a = a + B.ig[3 * k + r] * gl[k];
+                gloc[r] = a;
+            }
+            for (int r = 0; r < 3; ++r) {
+                double a = 0.0;
+                for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * gloc[k];
+                gradpt[r] = gradpt[r] + a;
+            }
+            double aloc[3];
+            for (int d = 0; d < 3; ++d) aloc[d] = 2.0 * rpi.y / rpi.x * B.bi[3 + d];
+            for (int r = 0; r < 3; ++r) {
+                double a = 0.0;
+                for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * aloc[k];
+                ALEt[r] = ALEt[r] + a;
+            }
+            if (P.alfamon > 0.0) {
+                const double Monvisc = P.alfamon * P.celerita * h;
+                const double Mmult = -Monvisc * dvn * B.bi[6];
+                for (int d = 0; d < 3; ++d) VM[d] = VM[d] + Mmult * F.T[6 + d];
+            }
+            double slip = 0.0, cinv = 0.0;
+            if (F.mode == 1) {
+                slip = F.slip;
+                cinv = P.kin_visc;
+            }
+            if (cinv > 0.0 && (lam_prev == 1 || F.lam == 0)) {
+                const double cm = cinv * B.bi[2] * slip;
+                for (int d = 0; d < 3; ++d) VS[d] = VS[d] + cm * (2.0 * (v_i[d] - F.vel[d]));
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            tp[d] = tp[d] + ALEt[d] + gradpt[d];
+            td[d] = td[d] - VM[d];
+            tv[d] = tv[d] + VS[d];
+        }
+    }
+    const int nn = nnb[i];
+    const size_t N = P.cap;
+    double rvs[3];
+    if constexpr (!F32) {
+        const double rhoi = rpi.x, pi = rpi.y;
+        double tpa[3] = {0, 0, 0}, ale[3] = {0, 0, 0};
+        rvs[0] = rvs[1] = rvs[2] = 0.0;
+        const double celt = P.celerita + P.celerita;
+        const double nui = P.kin_visc, nuj = P.kin_visc;
+        const bool morris = nui > 1.0e-12 && nuj > 1.0e-12;
+        for (int k = 0; k < nn; ++k) {
+            const int j = (int)(nbr[(size_t)k * N + i] & NB_MASK);
+            if (j == i) continue;
+            const double4 xj = xm[j];
+            const double4 vj = vel[j];
+            const double2 rpj = rp[j];
+            const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
+            const double rij = R0 * R0 + R1 * R1 + R2 * R2;
+            K64 K;
+            kvals64(P, rij, true, true, false, K);
+            const double rhoj = rpj.x, pj = rpj.y, amassj = xj.w;
+            const double dv0 = vj.x - vi.x, dv1 = vj.y - vi.y, dv2 = vj.z - vi.z;
+            double alpha = (pj - pi) / (rhoi * rhoj);
+            tpa[0] = tpa[0] + -amassj * alpha * R0 * K.K3;
+            tpa[1] = tpa[1] + -amassj * alpha * R1 * K.K3;
+            tpa[2] = tpa[2] + -amassj * alpha * R2 * K.K3;
+            alpha = (pi * (rhoj / rhoi + 1.0) + pj * (rhoi / rhoj - 1.0)) / (rhoi * rhoj);
+            ale[0] = ale[0] - amassj * alpha * R0 * K.K3;
+            ale[1] = ale[1] - amassj * alpha * R1 * K.K3;
+            ale[2] = ale[2] - amassj * alpha * R2 * K.K3;
+            const double vrij = -dv0 * R0 - dv1 * R1 - dv2 * R2;
+            const double TermMon = P.alfamon * celt * h / (rhoi + rhoj);
+            td[0] = td[0] + (amassj * TermMon * vrij * R0 * K.K2 + 0.0);
+            td[1] = td[1] + (amassj * TermMon * vrij * R1 * K.K2 + 0.0);
+            td[2] = td[2] + (amassj * TermMon * vrij * R2 * K.K2 + 0.0);
+            if (morris) {
+                const double rtil = nui * rhoi + nuj * rhoj + 1.0e-3;
+                const double atil = 4.0 * (nui * nuj);
+                const double fac = amassj * atil / rtil;
+                const double rr = R0 * R0 + R1 * R1 + R2 * R2;
+                const double w0 = fac * (-dv0 * K.K2 * rr);
+                const double w1 = fac * (-dv1 * K.K2 * rr);
+                const double w2 = fac * (-dv2 * K.K2 * rr);
+                tv[0] = tv[0] + w0; tv[1] = tv[1] + w1; tv[2] = tv[2] + w2;
+                rvs[0] = rvs[0] + w0; rvs[1] = rvs[1] + w1; rvs[2] = rvs[2] + w2;
+            }
+        }
+        for (int d = 0; d < 3; ++d) tp[d] = tp[d] + tpa[d] + ale[d];
+    } else {
+        // FP32 pair sums on cell-relative coordinates: the pressure term in
+        // its symmetric form -m_j (p_i/rho_i^2 + p_j/rho_j^2) r K3, which is
+        // the sum of the reference's difference term and ALE1 complement.
+        const float4 xf = xmf[i];
+        const float4 vf = velf[i];
+        const float2 rpfi = rpf[i];
+        const float rhoi = rpfi.x, pi = rpfi.y;
+        const float pri = pi / (rhoi * rhoi);
+        float s_p0 = 0.f, s_p1 = 0.f, s_p2 = 0.f, s_d0 = 0.f, s_d1 = 0.f, s_d2 = 0.f;
+        float s_v0 = 0.f, s_v1 = 0.f, s_v2 = 0.f;
+        const float mon = P.f_alfamon * (P.f_celerita + P.f_celerita) * P.f_h;
+        const float nu = P.f_kin_visc;
+        const float atil = 4.0f * (nu * nu);
+        const bool morris = P.kin_visc > 1.0e-12;
+        for (int k = 0; k < nn; ++k) {
+            const uint32_t e = nbr[(size_t)k * N + i];
+            const int j = (int)(e & NB_MASK);
+            if (j == i) continue;
+            const float4 xj = xmf[j];
+            const float4 vj = velf[j];
+            const float2 rpj = rpf[j];
+            float R0, R1, R2;
+            rag32(P, xf, xj, e, R0, R1, R2);
+            const float rij = R0 * R0 + R1 * R1 + R2 * R2;
+            K32 K;
+            kvals32(P, rij, true, true, false, K);
+            const float rhoj = rpj.x, pj = rpj.y, mj = xj.w;
+            const float dv0 = vj.x - vf.x, dv1 = vj.y - vf.y, dv2 = vj.z - vf.z;
+            const float cp = -mj * (pri + pj / (rhoj * rhoj)) * K.K3;
+            s_p0 += cp * R0; s_p1 += cp * R1; s_p2 += cp * R2;
+            const float vrij = -(dv0 * R0 + dv1 * R1 + dv2 * R2);
+            const float cd = mj * mon * __builtin_amdgcn_rcpf(rhoi + rhoj) * vrij * K.K2;
+            s_d0 += cd * R0; s_d1 += cd * R1; s_d2 += cd * R2;
+            if (morris) {
+                const float fac = mj * atil * __builtin_amdgcn_rcpf(nu * rhoi + nu * rhoj + 1.0e-3f);
+                const float cv = -fac * K.K2 * rij;
+                s_v0 += cv * dv0; s_v1 += cv * dv1; s_v2 += cv * dv2;
+            }
+        }
+        tp[0] = tp[0] + (double)s_p0; tp[1] = tp[1] + (double)s_p1; tp[2] = tp[2] + (double)s_p2;
+        td[0] = td[0] + (double)s_d0; td[1] = td[1] + (double)s_d1; td[2] = td[2] + (double)s_d2;
+        tv[0] = tv[0] + (double)s_v0; tv[1] = tv[1] + (double)s_v1; tv[2] = tv[2] + (double)s_v2;
+        rvs[0] = s_v0; rvs[1] = s_v1; rvs[2] = s_v2;
+    }
+    int lam = 0;
+    if (P.kin_visc > 0.0) {
+        const double apg = sqrt(tp[0] * tp[0] + tp[1] * tp[1] + tp[2] * tp[2]) + P.GI;
+        const double amo = sqrt(rvs[0] * rvs[0] + rvs[1] * rvs[1] + rvs[2] * rvs[2]);
+        double w = 0.0;
+        if (apg != 0.0) w = amo / apg * 100.0;
+        lam = (w > 5.0 || apg == 0.0) ? 1 : 0;
+    }
+    lam_out[i] = lam;
+    double a[3];
+    if (ncbf > 0) {
+        double br[3] = {0.0, 0.0, 0.0};
+        const double zimin = 0.25 * P.dx;
+        const double celer02 = P.eps / P.den0;
+        for (int q = 0; q < ncbf; ++q) {
+            const BTab& B = bt[(size_t)q * P.cap + i];
+            const DFace& F = faces[B.face];
+            if (F.btype != 0 && F.btype != 1) continue;
+            const double zi = B.loc[2];
+            if (zi < zimin && d_point_internal(F, B.loc[0], B.loc[1])) {
+                double vin = 0.0;
+                for (int sd = 0; sd < 3; ++sd) vin = vin + v_i[sd] * F.T[6 + sd];
+                if (vin < 0.0) {
+                    const double nr = -1.0 * celer02 * log((h + zi - zimin) / h) / h;
+                    for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * F.T[6 + d];
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + P.grav[d] + br[d];
+    } else {
+        for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + P.grav[d];
+    }
+    acc_out[i] = make_double4(a[0], a[1], a[2], 0.0);
+    // Leapfrog_momentum: vel = vel + dtvel*acc (kodvel == 0)
+    const double dtvel = scal[SC_DTVEL];
+    const double4 vn = make_double4(vi.x + dtvel * a[0], vi.y + dtvel * a[1], vi.z + dtvel * a[2], vi.w);
+    vel_out[i] = vn;
+    if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, 0.0f);
+}
+
+// ----------------------------------------------------------------------------
+// velocity smoothing: velocity_smoothing.f90:74-133,
+// velocity_smoothing_SA_SPH_3D.f90:64-122 (FP64), velocity_smoothing_2.f90:
+// 59-103; fused Leapfrog_trajectories.f90:48-76 and the particle part of the
+// next time_step_duration
+// ----------------------------------------------------------------------------
+template <bool F32>
+__global__ __launch_bounds__(256) void k_vsmooth(
+    DP P, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
+    const double4* __restrict__ xm, const double4* __restrict__ vel1, const double2* __restrict__ rp,
+    const float4* __restrict__ xmf, const float4* __restrict__ velf1, const float2* __restrict__ rpf,
+    const uint32_t* __restrict__ nbr, const int* __restrict__ nnb, double4* __restrict__ vel_out,
+    float4* __restrict__ velf_out, double4* __restrict__ xm_out, const double* scal,
+    unsigned long long* dtmin, int* iscal, int do_traj) {
+    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int n = iscal[IS_N];
+    unsigned long long dtbits = 0x7FF0000000000000ull;
+    if (i < n) {
+        const double4 xi = xm[i];
+        const double4 vi = vel1[i];
+        double v[3] = {0.0, 0.0, 0.0};
+        if (P.tetav > 1.0e-9) {
+            const int nn = nnb[i];
+            const size_t N = P.cap;
+            if constexpr (!F32) {
+                for (int k = 0; k < nn; ++k) {
+                    const int j = (int)(nbr[(size_t)k * N + i] & NB_MASK);
+                    const double4 xj = xm[j];
+                    const double4 vj = vel1[j];
+                    const double rhoj = rp[j].x;
+                    const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
+                    const double rij = R0 * R0 + R1 * R1 + R2 * R2;
+                    K64 K;
+                    kvals64(P, rij, false, false, true, K);
+                    const double pes = xj.w * K.W / rhoj;
+                    v[0] = v[0] + (vj.x - vi.x) * pes;
+                    v[1] = v[1] + (vj.y - vi.y) * pes;
+                    v[2] = v[2] + (vj.z - vi.z) * pes;
+                }
+            } else {
+                const float4 xf = xmf[i];
+                const float4 vf = velf1[i];
+                float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+                for (int k = 0; k < nn; ++k) {
+                    const uint32_t e = nbr[(size_t)k * N + i];
+                    const int j = (int)(e & NB_MASK);
+                    const float4 xj = xmf[j];
+                    const float4 vj = velf1[j];
+                    const float rhoj = rpf[j].x;
+                    float R0, R1, R2;
+                    rag32(P, xf, xj, e, R0, R1, R2);
+                    K32 K;
+                    kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                    const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(rhoj);
+                    s0 += (vj.x - vf.x) * pes;
+                    s1 += (vj.y - vf.y) * pes;
+                    s2 += (vj.z - vf.z) * pes;
+                }
+                v[0] = s0; v[1] = s1; v[2] = s2;
+            }
+            const int ncbf = P.semi ? bn[i] : 0;
+            for (int q = 0; q < ncbf; ++q) {
+                const BTab& B = bt[(size_t)q * P.cap + i];
+                const DFace& F = faces[B.face];
+                if (F.btype == 3 || F.btype == 4 || F.btype == 5) {
+                    v[0] = v[1] = v[2] = 0.0;
+                    break;
+                }
+                if (F.btype != 0 && F.btype != 1) continue;
+                const double DVG[3] = {2.0 * (F.tvel[0] - vi.x), 2.0 * (F.tvel[1] - vi.y),
+                                       2.0 * (F.tvel[2] - vi.z)};
+                double DVL[3];
+                for (int ii = 0; ii < 3; ++ii) {
+                    double a = 0.0;
+                    for (int jj = 0; jj < 3; ++jj) a = a + F.T[3 * ii + jj] * DVG[jj];
+                    DVL[ii] = a;
+                }
+                const double IntWdV = B.bi[1];
+                const double slip = F.mode == 1 ? F.slip : 0.0;
+                const double BCL[3] = {DVL[0] * IntWdV * slip, DVL[1] * IntWdV * slip, DVL[2] * IntWdV};
+                for (int ii = 0; ii < 3; ++ii) {
+                    double a = 0.0;
+                    for (int jj = 0; jj < 3; ++jj) a = a + F.T[3 * jj + ii] * BCL[jj];
+                    v[ii] = v[ii] + a;
+                }
+            }
+        }
+        double4 vn = vi;
+        if (P.tetav > 1.0e-9) {
+            const double tv1 = scal[SC_TV1];
+            vn = make_double4(vi.x + tv1 * v[0], vi.y + tv1 * v[1], vi.z + tv1 * v[2], vi.w);
+        }
+        vel_out[i] = vn;
+        if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, 0.0f);
+        if (do_traj) {
+            const double dt = scal[SC_DT];
+            xm_out[i] = make_double4(xi.x + dt * vn.x, xi.y + dt * vn.y, xi.z + dt * vn.z, xi.w);
+        }
+        dtbits = (unsigned long long)__double_as_longlong(d_dt_cfl(P, vn.x, vn.y, vn.z));
+    }
+    if (do_traj) block_min_to(dtbits, dtmin);
+}
+
+// ----------------------------------------------------------------------------
+// continuity: SASPH_continuity.f90:89-169 with AddBoundaryContribution_to_CE3D
+// .f90:82-169 (FP64), Continuity_Equation.f90:97-235; fused
+// Leapfrog_continuity.f90:51-60 and EoS_fluid_loop.f90:59-70
+// ----------------------------------------------------------------------------
+template <bool F32>
+__global__ __launch_bounds__(256) void k_continuity(
+    DP P, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
+    const double4* __restrict__ xm, const double4* __restrict__ vel, const double2* __restrict__ rp,
+    const float4* __restrict__ xmf, const float4* __restrict__ velf, const float2* __restrict__ rpf,
+    const uint32_t* __restrict__ nbr, const int* __restrict__ nnb, double* __restrict__ dden_out,
+    double2* __restrict__ rp_out, float2* __restrict__ rpf_out, const double* scal, int* iscal) {
+    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int n = iscal[IS_N];
+    if (i >= n) return;
+    const double4 xi = xm[i];
+    const double4 vi = vel[i];
+    const double rhoi = rp[i].x;
+    double dd = 0.0;
+    const int ncbf = P.semi ? bn[i] : 0;
+    if (ncbf > 0) {
+        double gu = 0.0, gv = 0.0, gw = 0.0;
+        const double v_i[3] = {vi.x, vi.y, vi.z};
+        for (int q = 0; q < ncbf; ++q) {
+            const BTab& B = bt[(size_t)q * P.cap + i];
+            const DFace& F = faces[B.face];
+            if (F.btype == 0 || F.btype == 1) {
+                if (B.loc[2] > 0.0) {
+                    double a2[3];
+                    for (int d = 0; d < 3; ++d) a2[d] = 2.0 * (F.vel[d] - v_i[d]);
+                    const double dp = a2[0] * F.T[6] + a2[1] * F.T[7] + a2[2] * F.T[8];
+                    double av[3];
+                    for (int r = 0; r < 3; ++r) {
+                        double a = 0.0;
+                        for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * B.bi[3 + k];
+                        av[r] = a;
+                    }
+                    gu = gu - dp * F.T[6] * av[0];
+                    gv = gv - dp * F.T[7] * av[1];
+                    gw = gw - dp * F.T[8] * av[2];
+                }
+            } else if (F.btype >= 3) {
+                atomicAdd(&iscal[IS_UNSUPPORTED], 1);
+                break;
+            }
+        }
+        dd = dd - rhoi * (gu + gv + gw);
+    }
+    const int nn = nnb[i];
+    const size_t N = P.cap;
+    double inner;
+    if constexpr (!F32) {
+        double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int k = 0; k < nn; ++k) {
+            const int j = (int)(nbr[(size_t)k * N + i] & NB_MASK);
+            const double4 xj = xm[j];
+            const double4 vj = vel[j];
+            const double rhoj = rp[j].x;
+            const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
+            const double rij = R0 * R0 + R1 * R1 + R2 * R2;
+            K64 K;
+            kvals64(P, rij, true, false, false, K);
+            const double amassj = xj.w;
+            a1 = a1 + rhoi * (amassj / rhoj) * K.K1 * (vj.x - vi.x) * R0;
+            a2 = a2 + rhoi * (amassj / rhoj) * K.K1 * (vj.y - vi.y) * R1;
+            a3 = a3 + rhoi * (amassj / rhoj) * K.K1 * (vj.z - vi.z) * R2;
+        }
+        inner = a1 + a2 + a3;
+    } else {
+        const float4 xf = xmf[i];
+        const float4 vf = velf[i];
+        float s = 0.f;
+        for (int k = 0; k < nn; ++k) {
+            const uint32_t e = nbr[(size_t)k * N + i];
+            const int j = (int)(e & NB_MASK);
+            const float4 xj = xmf[j];
+            const float4 vj = velf[j];
+            const float rhoj = rpf[j].x;
+            float R0, R1, R2;
+            rag32(P, xf, xj, e, R0, R1, R2);
+            K32 K;
+            kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, true, false, false, K);
+            s += xj.w * __builtin_amdgcn_rcpf(rhoj) * K.K1 *
+                 ((vj.x - vf.x) * R0 + (vj.y - vf.y) * R1 + (vj.z - vf.z) * R2);
+        }
+        inner = rhoi * (double)s;
+    }
+    const double dden = dd - inner;
+    dden_out[i] = dden;
+    const double rho1 = rhoi + scal[SC_DT] * dden;
+    const double p1 = (iscal[IS_ONGOING] > 0) ? P.eps / P.den0 * (rho1 - P.den0) + P.prif : rp[i].y;
+    rp_out[i] = make_double2(rho1, p1);
+    if (F32) rpf_out[i] = make_float2((float)rho1, (float)p1);
+}
+
+// ----------------------------------------------------------------------------
+// pressure smoothing: PressureSmoothing_3D.f90:80-161 (both loops)
+// ----------------------------------------------------------------------------
+template <bool F32>
+__global__ __launch_bounds__(256) void k_psmooth(
+    DP P, const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
+    const double2* __restrict__ rp, const float4* __restrict__ xmf, const float2* __restrict__ rpf,
+    const uint32_t* __restrict__ nbr, const int* __restrict__ nnb, double* __restrict__ vpres_out,
+    double2* __restrict__ rp_out, float2* __restrict__ rpf_out, const double* scal, int* iscal) {
+    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    const int n = iscal[IS_N];
+    if (i >= n) return;
+    const double4 xi = xm[i];
+    const double2 rpi = rp[i];
+    const int nsp = nnb[i];
+    double diffp = 0.0;
+    if (nsp > 0) {
+        const double p0i = P.prif, pi = rpi.y;
+        double somp, app;
+        const size_t N = P.cap;
+        if constexpr (!F32) {
+            somp = 0.0;
+            app = 0.0;
+            for (int k = 0; k < nsp; ++k) {
+                const int j = (int)(nbr[(size_t)k * N + i] & NB_MASK);
+                const double4 xj = xm[j];
+                const double2 rpj = rp[j];
+                const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
+                K64 K;
+                kvals64(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                const double pes = xj.w * K.W / rpj.x;
+                app = app + pes;
+                somp = somp + (rpj.y - pi) * pes;
+            }
+        } else {
+            const float4 xf = xmf[i];
+            const float pif = rpf[i].y;
+            float s_a = 0.f, s_p = 0.f;
+            for (int k = 0; k < nsp; ++k) {
+                const uint32_t e = nbr[(size_t)k * N + i];
+                const int j = (int)(e & NB_MASK);
+                const float4 xj = xmf[j];
+                const float2 rpj = rpf[j];
+                float R0, R1, R2;
+                rag32(P, xf, xj, e, R0, R1, R2);
+                K32 K;
+                kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(rpj.x);
+                s_a += pes;
+                s_p += (rpj.y - pif) * pes;
+            }
+            somp = s_p;
+            app = s_a;
+        }
+        const int ncbf = P.semi ? bn[i] : 0;
+        for (int q = 0; q < ncbf; ++q) {
+            const BTab& B = bt[(size_t)q * P.cap + i];
+            if (B.loc[2] > 0.0) app = app + B.bi[1];
+        }
+        if (app < 0.95) diffp = somp + (p0i - pi) * (1.0 - app);
+        else diffp = somp / app;
+    }
+    vpres_out[i] = diffp;
+    double p2 = rpi.y, rho2 = rpi.x;
+    if (P.tetap > 0.0) {
+        p2 = rpi.y + scal[SC_TP1] * diffp;
+        rho2 = P.den0 * (1.0 + (p2 - P.prif) / P.eps);
+    }
+    rp_out[i] = make_double2(rho2, p2);
+    if (F32) rpf_out[i] = make_float2((float)rho2, (float)p2);
+}
+
+}  // namespace sph

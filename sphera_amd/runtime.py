@@ -1,0 +1,251 @@
+"""ctypes binding of the device library (include/sphera_hip.h).
+
+This is the host-side mirror of the reference's hot-path interface: one method
+per time_step_loop phase (same names as the Fortran units they replace), plus
+the state exchange a Fortran ISO_C_BINDING bridge would do.  There is no CPU
+fallback: if libsphera_hip.so is missing or no GPU is present, every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_build" / "libsphera_hip.so"
+
+FP64, FP32_MIXED = 0, 1
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int32)
+
+EXPORTS = [
+    "sph_ctx_create", "sph_ctx_destroy", "sph_state_upload", "sph_particle_count",
+    "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_get_neighbour_lists", "sph_dt", "sph_momentum",
+    "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
+    "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
+    "sph_preloop", "sph_step", "sph_step_async", "sph_sync", "sph_set_timing", "sph_get_timing",
+    "sph_last_error",
+]
+
+PHASES = ["dt", "momentum", "leapfrog_mom", "velocity_smoothing", "leapfrog_traj", "sort",
+          "neighbours", "boundary_tab", "continuity", "leapfrog_cont", "eos", "pressure_smoothing"]
+
+
+class SphParams(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("capacity", C.c_int32), ("nmaxpartj", C.c_int32),
+                ("maxclose", C.c_int32), ("ncd", C.c_int32 * 3), ("extr_lo", C.c_double * 3),
+                ("dcd", C.c_double * 3), ("dx", C.c_double), ("h", C.c_double),
+                ("coefke", C.c_double), ("coefkacl", C.c_double), ("eta", C.c_double),
+                ("eta2", C.c_double), ("grav", C.c_double * 3), ("prif", C.c_double),
+                ("den0", C.c_double), ("eps", C.c_double), ("celerita", C.c_double),
+                ("alfamon", C.c_double), ("kin_visc", C.c_double), ("cfl", C.c_double),
+                ("vsc_coeff", C.c_double), ("pesodt", C.c_double), ("tetap", C.c_double),
+                ("tetav", C.c_double), ("semi", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SphFaces(C.Structure):
+    _fields_ = [("nf", C.c_int32), ("T", PD), ("LX", PD), ("GX", PD), ("area", PD),
+                ("velocity", PD), ("tvel", PD), ("slip_coeff", PD), ("nodes", PI),
+                ("btype", PI), ("slip_mode", PI), ("lam_check", PI), ("gcbf_count", PI),
+                ("gcbf_start", PI), ("gcbf_vec", PI), ("kerneltab", PD), ("ktdelta", C.c_double),
+                ("bitab", PD)]
+
+
+_lib = None
+
+
+def load():
+    """Load libsphera_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(str(LIB_PATH))
+    L.sph_ctx_create.argtypes = [C.POINTER(SphParams), C.POINTER(SphFaces), C.POINTER(C.c_void_p)]
+    L.sph_ctx_destroy.argtypes = [C.c_void_p]
+    L.sph_state_upload.argtypes = [C.c_void_p, C.c_int32, PD, PD, PD, PD, PD]
+    L.sph_particle_count.argtypes = [C.c_void_p, PI]
+    L.sph_state_download.argtypes = [C.c_void_p, PD, PD, PD, PD, PD, PD, PD, PI]
+    L.sph_get_scalars.argtypes = [C.c_void_p, PD]
+    L.sph_get_neighbour_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
+    L.sph_pair_counter.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    for ph in PHASES + ["preloop", "sync"]:
+        getattr(L, "sph_" + ph).argtypes = [C.c_void_p]
+    L.sph_step.argtypes = [C.c_void_p, C.c_int32]
+    L.sph_step_async.argtypes = [C.c_void_p, C.c_int32]
+    L.sph_set_timing.argtypes = [C.c_void_p, C.c_int32]
+    L.sph_get_timing.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, PD, PI, C.c_int32]
+    L.sph_last_error.restype = C.c_char_p
+    for name in EXPORTS:
+        if name != "sph_last_error":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+class SphError(RuntimeError):
+    def __init__(self, code, where, msg):
+        super().__init__(f"{where} failed with status {code}: {msg}")
+        self.code = code
+
+
+def _pd(a):
+    return a.ctypes.data_as(PD)
+
+
+def _pi(a):
+    return a.ctypes.data_as(PI)
+
+
+def faces_struct(tb):
+    """Pack CaseTables frontier data in the C layout; returns (struct, keepalive)."""
+    F = tb.faces
+    k = dict(
+        T=np.ascontiguousarray(F.T.transpose(0, 2, 1).reshape(-1)),
+        LX=np.ascontiguousarray(F.LX.reshape(-1)), GX=np.ascontiguousarray(F.GX.reshape(-1)),
+        area=np.ascontiguousarray(F.area, np.float64),
+        vel=np.ascontiguousarray(F.velocity.reshape(-1)),
+        tvel=np.ascontiguousarray(F.tratto_velocity.reshape(-1)),
+        slip=np.ascontiguousarray(F.slip_coeff, np.float64),
+        nodes=np.ascontiguousarray(F.nodes, np.int32), btype=np.ascontiguousarray(F.btype, np.int32),
+        mode=np.ascontiguousarray(F.slip_mode, np.int32),
+        lam=np.ascontiguousarray(F.laminar_check, np.int32),
+        gc=np.ascontiguousarray(tb.gcbf_count, np.int32),
+        gs=np.ascontiguousarray(tb.gcbf_start, np.int32),
+        gv=np.ascontiguousarray(tb.gcbf_vector if len(tb.gcbf_vector) else np.zeros(1), np.int32),
+        kt=np.ascontiguousarray(tb.kerneltab.reshape(-1)), bit=np.ascontiguousarray(tb.bitab.reshape(-1)))
+    s = SphFaces(len(F.face_id), _pd(k["T"]), _pd(k["LX"]), _pd(k["GX"]), _pd(k["area"]),
+                 _pd(k["vel"]), _pd(k["tvel"]), _pd(k["slip"]), _pi(k["nodes"]), _pi(k["btype"]),
+                 _pi(k["mode"]), _pi(k["lam"]), _pi(k["gc"]), _pi(k["gs"]), _pi(k["gv"]),
+                 _pd(k["kt"]), tb.ktdelta, _pd(k["bit"]))
+    return s, k
+
+
+def params_struct(tb, precision=FP64, capacity=None, semi=True):
+    s, g, case = tb.scalars, tb.grid, tb.case
+    p = SphParams()
+    p.precision = precision
+    p.capacity = int(capacity or len(tb.particles["coord"]))
+    p.nmaxpartj = s["nmaxpartj"]
+    p.maxclose = case.maxcloseboundfaces
+    p.ncd[:] = [int(v) for v in g.ncd]
+    p.extr_lo[:] = [float(v) for v in g.extr_lo]
+    p.dcd[:] = [float(v) for v in g.dcd]
+    p.dx, p.h, p.coefke, p.coefkacl = case.dx, s["h"], s["coefke"], s["coefkacl"]
+    p.eta, p.eta2 = s["eta"], s["eta2"]
+    p.grav[:] = list(case.grav)
+    p.prif = case.p_ref
+    p.den0, p.eps, p.celerita = case.rho0, case.bulk, s["celerita"]
+    p.alfamon, p.kin_visc = case.alfa_mon, s["kin_visc"]
+    p.cfl, p.vsc_coeff, p.pesodt = case.cfl, case.vsc_coeff, case.pesodt
+    p.tetap, p.tetav = case.teta_p, case.teta_v
+    p.semi = 1 if semi else 0
+    return p
+
+
+class SphContext:
+    """One device solve (one C-ABI context)."""
+
+    def __init__(self, tb, precision=FP64, capacity=None, semi=True):
+        self.L = load()
+        self.tb = tb
+        self.precision = precision
+        self.params = params_struct(tb, precision, capacity, semi)
+        fs, self._keep = faces_struct(tb)
+        h = C.c_void_p()
+        self._check(self.L.sph_ctx_create(C.byref(self.params), C.byref(fs), C.byref(h)),
+                    "sph_ctx_create")
+        self.h = h
+
+    def _check(self, rc, where):
+        if rc:
+            raise SphError(rc, where, self.L.sph_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.sph_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state ------------------------------------------------------------------
+    def upload(self, coord=None, vel=None, dens=None, pres=None, mass=None):
+        P = self.tb.particles
+        a = [np.ascontiguousarray(P[k] if v is None else v, np.float64)
+             for k, v in (("coord", coord), ("vel", vel), ("dens", dens), ("pres", pres),
+                          ("mass", mass))]
+        self._check(self.L.sph_state_upload(self.h, len(a[0]), *[_pd(x) for x in a]),
+                    "sph_state_upload")
+
+    @property
+    def n(self):
+        v = np.zeros(1, np.int32)
+        self._check(self.L.sph_particle_count(self.h, _pi(v)), "sph_particle_count")
+        return int(v[0])
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("coord", "vel", "acc")}
+        out.update({k: np.zeros(n) for k in ("dens", "pres", "dden", "vpres")})
+        ids = np.zeros(n, np.int32)
+        self._check(self.L.sph_state_download(self.h, *[_pd(out[k]) for k in (
+            "coord", "vel", "acc", "dens", "pres", "dden", "vpres")], _pi(ids)), "sph_state_download")
+        out["ids"] = ids
+        out["var"] = out["vel"]
+        sc = self.scalars()
+        out.update(sc)
+        out["nag"] = n
+        return out
+
+    def scalars(self):
+        sc = np.zeros(5)
+        self._check(self.L.sph_get_scalars(self.h, _pd(sc)), "sph_get_scalars")
+        return dict(dt=sc[0], dt_average=sc[1], time=sc[2], it=int(sc[3]), pairs=int(sc[4]))
+
+    def pair_counter(self):
+        v = C.c_uint64(0)
+        self._check(self.L.sph_pair_counter(self.h, C.byref(v)), "sph_pair_counter")
+        return int(v.value)
+
+    def lists(self):
+        n = self.n
+        stride = int(self.params.nmaxpartj)
+        cnt = np.zeros(n, np.int32)
+        ids = np.zeros((n, stride), np.int32)
+        self._check(self.L.sph_get_neighbour_lists(self.h, _pi(cnt), _pi(ids), stride),
+                    "sph_get_neighbour_lists")
+        return cnt, ids
+
+    # phases -----------------------------------------------------------------
+    def phase(self, name):
+        return self._check(getattr(self.L, "sph_" + name)(self.h), "sph_" + name)
+
+    def preloop(self):
+        return self.phase("preloop")
+
+    def step(self, k=1):
+        return self._check(self.L.sph_step(self.h, int(k)), "sph_step")
+
+    def step_async(self, k=1):
+        return self._check(self.L.sph_step_async(self.h, int(k)), "sph_step_async")
+
+    def sync(self):
+        return self.phase("sync")
+
+    def set_timing(self, on=True):
+        self._check(self.L.sph_set_timing(self.h, 1 if on else 0), "sph_set_timing")
+
+    def timing(self):
+        names = C.create_string_buffer(4096)
+        ms = np.zeros(64)
+        calls = np.zeros(64, np.int32)
+        k = self.L.sph_get_timing(self.h, names, 4096, _pd(ms), _pi(calls), 64)
+        nm = names.value.decode().split(";")[:k]
+        return {nm[i]: (float(ms[i]), int(calls[i])) for i in range(k)}
