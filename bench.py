@@ -51,9 +51,9 @@ def make_case(args):
     return case, tb, coord
 
 
-def run_device(tb, coord, precision, steps, warmup):
+def run_device(tb, coord, precision, steps, warmup, tune=None):
     from sphera_amd import runtime as rt
-    d = rt.SphContext(tb, precision=precision)
+    d = rt.SphContext(tb, precision=precision, tune=tune)
     d.upload(coord=coord)
     d.preloop()
     if warmup:
@@ -67,8 +67,10 @@ def run_device(tb, coord, precision, steps, warmup):
     timing = d.timing()
     n = d.n
     sc = d.scalars()
+    launch = d.launch()
     d.close()
-    return dict(seconds=t1 - t0, pairs=pairs, timing=timing, n=n, it=sc["it"], dt=sc["dt"])
+    return dict(seconds=t1 - t0, pairs=pairs, timing=timing, n=n, it=sc["it"], dt=sc["dt"],
+                launch=launch)
 
 
 def cpu_baseline(tb, coord, steps):
@@ -99,6 +101,7 @@ def main():
     ap.add_argument("--dx", type=float, default=0.005)
     ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("SPH_CPU_STEPS", "5")))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU decomposition "
@@ -108,9 +111,16 @@ def main():
     case, tb, coord = make_case(args)
     log(f"[bench] case n={len(coord)} grid={tuple(tb.grid.ncd)} steps={args.steps} warmup={args.warmup}")
     res = {}
+    tunes = {}
     for name, prec in (("fp32_mixed", rt.FP32_MIXED), ("fp64", rt.FP64)):
+        tune = None
+        if not args.no_autotune:
+            from sphera_amd.autotune import autotune
+            tune, _ = autotune(tb, coord, prec, log=log)
+            log(f"[bench] {name}: autotuned launch {tune}")
+        tunes[name] = tune
         log(f"[bench] running {name}")
-        res[name] = run_device(tb, coord, prec, args.steps, args.warmup)
+        res[name] = run_device(tb, coord, prec, args.steps, args.warmup, tune)
         r = res[name]
         log(f"[bench] {name}: {r['seconds'] / args.steps * 1e3:.3f} ms/step, "
             f"{r['pairs'] / r['seconds'] / 1e9:.3f} G pairs/s")
@@ -150,7 +160,8 @@ def main():
                    "grid_cells": int(tb.grid.nmax)},
         "fp64": {"value": res["fp64"]["pairs"] / res["fp64"]["seconds"],
                  "ms_per_step": res["fp64"]["seconds"] / args.steps * 1e3,
-                 "kernel_ms": kern64},
+                 "kernel_ms": kern64, "launch": res["fp64"]["launch"]},
+        "launch": main_r["launch"],
         "mixed_over_fp64": (value / (res["fp64"]["pairs"] / res["fp64"]["seconds"])),
         "pair_kernels_only_value": main_r["pairs"] / pair_time if pair_time > 0 else None,
         "kernel_ms": kern,

@@ -46,7 +46,10 @@ typedef struct sph_params {
     double den0, eps, celerita, alfamon, kin_visc; /* Med(1) */
     double cfl, vsc_coeff, pesodt, tetap, tetav;   /* input_any_t */
     int32_t semi;        /* 1: SA-SPH frontiers active (Domain%tipo=="semi") */
-    int32_t reserved;
+    /* launch tunable of the particle kernels (0 = default; the autotuner in
+     * sphera_amd/autotune.py picks it per case) */
+    int32_t wg_size;     /* threads per workgroup: 64, 128, 192 or 256 */
+    int32_t reserved[4];
 } sph_params;
 
 /* SA-SPH frontier faces in BFaceList order and the grid-cell/face incidence
@@ -118,6 +121,9 @@ int sph_step(sph_ctx* ctx, int32_t nsteps);
 int sph_step_async(sph_ctx* ctx, int32_t nsteps);
 int sph_sync(sph_ctx* ctx);
 
+/* Launch shape in use: wg_size, list entries prefetched per step of the
+ * FP32 list walks, grid of the particle kernels. */
+int sph_get_launch(sph_ctx* ctx, int32_t out[3]);
 /* Per-kernel timing of the last sph_step: name/avg-ms pairs, for the bench. */
 int sph_set_timing(sph_ctx* ctx, int32_t on);
 int sph_get_timing(sph_ctx* ctx, char* names, int32_t names_len, double* ms, int32_t* calls,

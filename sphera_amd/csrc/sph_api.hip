@@ -60,7 +60,9 @@ struct sph_ctx {
     float2* rpf[3] = {};
     double *dden = nullptr, *vpres = nullptr;
     int *cell = nullptr, *key = nullptr, *slot = nullptr, *cnt = nullptr, *start = nullptr,
-        *perm = nullptr, *bsum = nullptr;
+        *perm = nullptr, *pid = nullptr, *bsum = nullptr;
+    int *work = nullptr, *nwork = nullptr;  // (particle, face) pairs of the boundary table
+    int wg = 256, grid = 1, bt_grid = 1024;  // launch shape of the particle kernels
     uint32_t* nbr = nullptr;
     int* nnb = nullptr;
     int* bn = nullptr;
@@ -136,6 +138,14 @@ int collect_timing(sph_ctx* c) {
 }
 
 // ---- device steps --------------------------------------------------------
+void scan_ints(sph_ctx* c, const int* in, int* out, int* bs, int m, int* total) {
+    const int nb = (m + 1023) / 1024;
+    k_scan_local<<<nb, 256, 0, c->stream>>>(in, out, bs, m);
+    k_scan_sums<<<1, 1024, 0, c->stream>>>(bs, nb, total);
+    k_scan_add<<<nblk(m), 256, 0, c->stream>>>(out, bs, m);
+}
+
+// cell sort + block descriptors (OrdGrid1)
 int run_sort(sph_ctx* c) {
     const DP& P = c->P;
     const int m = P.nmax + 1;
@@ -147,15 +157,13 @@ int run_sort(sph_ctx* c) {
     }
     {
         KTime t(c, "sort_scan");
-        const int nb = (m + 1023) / 1024;
-        k_scan_local<<<nb, 256, 0, c->stream>>>(c->cnt, c->start, c->bsum, m);
-        k_scan_sums<<<1, 1024, 0, c->stream>>>(c->bsum, nb, nullptr);
-        k_scan_add<<<nblk(m), 256, 0, c->stream>>>(c->start, c->bsum, m);
+        scan_ints(c, c->cnt, c->start, c->bsum, m, nullptr);
     }
     {
         KTime t(c, "sort_scatter");
-        k_scatter<<<nblk(cap), 256, 0, c->stream>>>(c->key, c->slot, c->start, c->perm, c->iscal);
-        k_cell_sort<<<nblk(P.nmax), 256, 0, c->stream>>>(P, c->start, c->perm, c->vel[2]);
+        k_scatter<<<nblk(cap), 256, 0, c->stream>>>(c->key, c->slot, c->start, c->vel[2], c->perm, c->pid,
+                                                    c->iscal);
+        k_cell_sort<<<(P.nmax + 255) / 256, 256, 0, c->stream>>>(P, c->start, c->perm, c->pid);
     }
     {
         KTime t(c, "sort_gather");
@@ -169,8 +177,8 @@ int run_sort(sph_ctx* c) {
 
 int run_neighbours(sph_ctx* c) {
     KTime t(c, "neighbours");
-    k_neighbours<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->xm[0], c->cell, c->start, c->nbr,
-                                                         c->nnb, c->iscal, c->pairs);
+    k_neighbours<<<c->grid, c->wg, 0, c->stream>>>(c->P, c->xm[0], c->xmf, c->cell, c->start, c->nbr,
+                                                  c->nnb, c->iscal, c->pairs);
     return 0;
 }
 
@@ -180,9 +188,12 @@ int run_btab(sph_ctx* c) {
         return 0;
     }
     KTime t(c, "boundary_tab");
-    k_boundary_tab<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_gcount, c->d_gstart,
-                                                           c->d_gvec, c->d_kt, c->d_bit, c->xm[0], c->bn,
-                                                           c->bt, c->iscal);
+    k_fill_int<<<1, 64, 0, c->stream>>>(c->nwork, 0, 1);
+    k_boundary_find<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_gcount, c->d_gstart,
+                                                            c->d_gvec, c->xm[0], c->bn, c->bt, c->work,
+                                                            c->nwork, c->iscal);
+    k_boundary_int<<<c->bt_grid, 256, 0, c->stream>>>(c->P, c->d_faces, c->d_kt, c->d_bit, c->bt, c->work,
+                                                   c->nwork);
     return 0;
 }
 
@@ -196,63 +207,57 @@ int run_dt(sph_ctx* c, int advance, bool reduce) {
 
 int run_momentum(sph_ctx* c) {
     KTime t(c, "momentum");
-    const int g = nblk(c->P.cap);
     if (c->P.f32)
-        k_momentum<true><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-                                                   c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1],
-                                                   c->lam[0], c->nbr, c->nnb, c->acc[1], c->vel[1],
-                                                   c->velf[1], c->scal, c->iscal);
+        k_momentum<true><<<c->grid, c->wg, 0, c->stream>>>(
+            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+            c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
+            c->vel[1], c->velf[1], c->scal, c->iscal);
     else
-        k_momentum<false><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-                                                    c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1],
-                                                    c->lam[0], c->nbr, c->nnb, c->acc[1], c->vel[1],
-                                                    c->velf[1], c->scal, c->iscal);
+        k_momentum<false><<<c->grid, c->wg, 0, c->stream>>>(
+            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+            c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
+            c->vel[1], c->velf[1], c->scal, c->iscal);
     return 0;
 }
 
 int run_vsmooth(sph_ctx* c) {
     KTime t(c, "velocity_smoothing");
-    const int g = nblk(c->P.cap);
     if (c->P.f32)
-        k_vsmooth<true><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
-                                                  c->rp[0], c->xmf, c->velf[1], c->rpf[0], c->nbr, c->nnb,
-                                                  c->vel[2], c->velf[2], c->xm[1], c->scal, c->dtmin,
-                                                  c->iscal, 1);
+        k_vsmooth<true><<<c->grid, c->wg, 0, c->stream>>>(
+            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
+            c->rp[0], c->xmf, c->velf[1], c->nbr, c->nnb, c->vel[2], c->velf[2], c->xm[1], c->scal,
+            c->dtmin, c->iscal);
     else
-        k_vsmooth<false><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
-                                                   c->rp[0], c->xmf, c->velf[1], c->rpf[0], c->nbr, c->nnb,
-                                                   c->vel[2], c->velf[2], c->xm[1], c->scal, c->dtmin,
-                                                   c->iscal, 1);
+        k_vsmooth<false><<<c->grid, c->wg, 0, c->stream>>>(
+            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
+            c->rp[0], c->xmf, c->velf[1], c->nbr, c->nnb, c->vel[2], c->velf[2], c->xm[1], c->scal,
+            c->dtmin, c->iscal);
     return 0;
 }
 
 int run_continuity(sph_ctx* c) {
     KTime t(c, "continuity");
-    const int g = nblk(c->P.cap);
     if (c->P.f32)
-        k_continuity<true><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-                                                     c->rp[1], c->xmf, c->velf[0], c->rpf[1], c->nbr,
-                                                     c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal,
-                                                     c->iscal);
+        k_continuity<true><<<c->grid, c->wg, 0, c->stream>>>(
+            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+            c->rp[1], c->xmf, c->velf[0], c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
     else
-        k_continuity<false><<<g, 256, 0, c->stream>>>(c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-                                                      c->rp[1], c->xmf, c->velf[0], c->rpf[1], c->nbr,
-                                                      c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal,
-                                                      c->iscal);
+        k_continuity<false><<<c->grid, c->wg, 0, c->stream>>>(
+            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+            c->rp[1], c->xmf, c->velf[0], c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
     return 0;
 }
 
 int run_psmooth(sph_ctx* c) {
     KTime t(c, "pressure_smoothing");
-    const int g = nblk(c->P.cap);
     if (c->P.f32)
-        k_psmooth<true><<<g, 256, 0, c->stream>>>(c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf, c->rpf[2],
-                                                  c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal,
-                                                  c->iscal);
+        k_psmooth<true><<<c->grid, c->wg, 0, c->stream>>>(
+            c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
+            c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal);
     else
-        k_psmooth<false><<<g, 256, 0, c->stream>>>(c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
-                                                   c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0],
-                                                   c->scal, c->iscal);
+        k_psmooth<false><<<c->grid, c->wg, 0, c->stream>>>(
+            c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
+            c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal);
     return 0;
 }
 
@@ -290,7 +295,8 @@ sph_ctx::~sph_ctx() {
     F(xmf);
     for (auto p : velf) F(p);
     for (auto p : rpf) F(p);
-    F(dden); F(vpres); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(bsum);
+    F(dden); F(vpres); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
+    F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     for (auto e : ev) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
@@ -304,7 +310,7 @@ int sph_ctx::alloc() {
     for (auto& p : acc) HIPC(dalloc(&p, N));
     for (auto& p : rp) HIPC(dalloc(&p, N));
     for (auto& p : lam) HIPC(dalloc(&p, N));
-    HIPC(dalloc(&xmf, P.f32 ? N : 1));
+    HIPC(dalloc(&xmf, N));
     for (auto& p : velf) HIPC(dalloc(&p, P.f32 ? N : 1));
     for (auto& p : rpf) HIPC(dalloc(&p, P.f32 ? N : 1));
     HIPC(dalloc(&dden, N));
@@ -313,6 +319,9 @@ int sph_ctx::alloc() {
     HIPC(dalloc(&key, N));
     HIPC(dalloc(&slot, N));
     HIPC(dalloc(&perm, N));
+    HIPC(dalloc(&pid, N));
+    HIPC(dalloc(&work, P.semi ? N * SPH_MB : 1));
+    HIPC(dalloc(&nwork, 1));
     HIPC(dalloc(&cnt, m));
     HIPC(dalloc(&start, m + 1));
     HIPC(dalloc(&bsum, (m + 1023) / 1024 + 1));
@@ -407,6 +416,19 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         delete c;
         return fail(101, "hipStreamCreate failed");
+    }
+    // launch shape of the particle kernels (one lane per particle)
+    {
+        int dev = 0, ncu = 256;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        c->wg = p->wg_size > 0 ? p->wg_size : 256;
+        if (c->wg < 64 || c->wg > 256 || c->wg % 64) {
+            delete c;
+            return fail(2, "wg_size must be 64, 128, 192 or 256");
+        }
+        c->grid = (P.cap + c->wg - 1) / c->wg;
+        c->bt_grid = 4 * ncu;
     }
     if ((rc = c->alloc())) {
         delete c;
@@ -726,6 +748,14 @@ int sph_step(sph_ctx* c, int32_t nsteps) {
     int rc = sph_step_async(c, nsteps);
     if (rc) return rc;
     return sph_sync(c);
+}
+
+int sph_get_launch(sph_ctx* c, int32_t out[3]) {
+    if (!c || !out) return fail(1, "null argument");
+    out[0] = c->wg;
+    out[1] = 8;
+    out[2] = c->grid;
+    return 0;
 }
 
 int sph_set_timing(sph_ctx* c, int32_t on) {

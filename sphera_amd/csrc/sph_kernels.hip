@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "sph_device.h"
 
 namespace sph {
@@ -105,8 +107,9 @@ __device__ __forceinline__ void kvals32(const DP& P, float rij, bool need12, boo
     const float r = __builtin_amdgcn_sqrtf(rij);
     const float q = r * P.f_unosuh;
     const float q2 = rij * P.f_unosusquareh;
+    // every listed pair has r <= 2h, where both branches of the spline and its
+    // derivatives vanish at q = 2, so no cut-off test is needed
     K.K1 = K.K2 = K.K3 = K.W = 0.0f;
-    if (q >= 2.0f) return;
     const bool outer = q >= 1.0f;
     const float denom = __builtin_amdgcn_rcpf(r + P.f_eta);
     if (need12) {
@@ -123,14 +126,26 @@ __device__ __forceinline__ void kvals32(const DP& P, float rij, bool need12, boo
     }
 }
 
+// The 27 stencil-cell offsets (dx,dy,dz) * dcd in FP32, indexed by the
+// stencil code of a list entry; each workgroup keeps a copy in LDS.
+__shared__ float4 g_soff[27];
+
+__device__ __forceinline__ void stencil_offsets(const DP& P) {
+    if (threadIdx.x < 27) {
+        const int c = threadIdx.x;
+        g_soff[c] = make_float4((float)(c % 3 - 1) * P.f_dcd[0], (float)((c / 3) % 3 - 1) * P.f_dcd[1],
+                                (float)(c / 9 - 1) * P.f_dcd[2], 0.0f);
+    }
+    __syncthreads();
+}
+
 // cell-relative FP32 separation x_i - x_j from the stencil code of j
-__device__ __forceinline__ void rag32(const DP& P, float4 xi, float4 xj, uint32_t code, float& r0,
-                                      float& r1, float& r2) {
-    const int c = (int)(code >> NB_SHIFT);
-    const int dx = c % 3 - 1, dy = (c / 3) % 3 - 1, dz = c / 9 - 1;
-    r0 = (xi.x - xj.x) - (float)dx * P.f_dcd[0];
-    r1 = (xi.y - xj.y) - (float)dy * P.f_dcd[1];
-    r2 = (xi.z - xj.z) - (float)dz * P.f_dcd[2];
+__device__ __forceinline__ void rag32(float4 xi, float4 xj, uint32_t code, float& r0, float& r1,
+                                      float& r2) {
+    const float4 o = g_soff[code >> NB_SHIFT];
+    r0 = (xi.x - xj.x) - o.x;
+    r1 = (xi.y - xj.y) - o.y;
+    r2 = (xi.z - xj.z) - o.z;
 }
 
 // ----------------------------------------------------------------------------
@@ -171,27 +186,57 @@ __device__ __forceinline__ bool d_point_internal(const DFace& F, double px0, dou
 }
 
 // ----------------------------------------------------------------------------
-// sort: OrdGrid1.f90:58-105 as a counting sort on the device
+// sort: OrdGrid1.f90:58-105 as a device counting sort.  The particles arrive
+// almost in cell order (they moved less than a cell since the last sort), so
+// a workgroup's 256 particles fall in a narrow window of cells: it counts
+// them in an LDS histogram and issues one global atomic per occupied cell.
 // ----------------------------------------------------------------------------
-__global__ void k_cell_count(DP P, const double4* __restrict__ xm, int* __restrict__ key,
-                             int* __restrict__ slot, int* __restrict__ cnt, int* iscal) {
+constexpr int HIST_W = 512;
+
+__global__ __launch_bounds__(256) void k_cell_count(DP P, const double4* __restrict__ xm,
+                                                    int* __restrict__ key, int* __restrict__ slot,
+                                                    int* __restrict__ cnt, int* iscal) {
+    __shared__ int hist[HIST_W];
+    __shared__ int base[HIST_W];
+    __shared__ int cmin;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = iscal[IS_N];
-    if (i >= n) return;
-    const double4 x = xm[i];
-    int c = d_particle_cell(P, x.x, x.y, x.z);
-    int k;
-    if (c <= 0) {  // out of the grid: removed (OrdGrid1.f90:71-82)
-        k = P.nmax;
-        atomicAdd(&iscal[IS_REMOVED], 1);
-    } else {
-        k = c - 1;
+    for (int b = threadIdx.x; b < HIST_W; b += blockDim.x) hist[b] = 0;
+    if (threadIdx.x == 0) cmin = 0x7fffffff;
+    __syncthreads();
+    int k = -1;
+    if (i < n) {
+        const double4 x = xm[i];
+        const int c = d_particle_cell(P, x.x, x.y, x.z);
+        if (c <= 0) {  // out of the grid: removed (OrdGrid1.f90:71-82)
+            k = P.nmax;
+            atomicAdd(&iscal[IS_REMOVED], 1);
+        } else {
+            k = c - 1;
+            atomicMin(&cmin, k);
+        }
+        key[i] = k;
     }
-    key[i] = k;
-    slot[i] = atomicAdd(&cnt[k], 1);
+    __syncthreads();
+    const int c0 = cmin;
+    int loc = -1, rank = 0;
+    if (k >= 0) {
+        const int d = k - c0;
+        if (k < P.nmax && d < HIST_W) {
+            loc = d;
+            rank = atomicAdd(&hist[d], 1);
+        } else {
+            slot[i] = atomicAdd(&cnt[k], 1);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < HIST_W; b += blockDim.x)
+        if (hist[b] > 0) base[b] = atomicAdd(&cnt[c0 + b], hist[b]);
+    __syncthreads();
+    if (loc >= 0) slot[i] = base[loc] + rank;
 }
 
-// wave64 inclusive scan with DPP-free shuffles
+// wave64 inclusive scan
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -226,7 +271,7 @@ __global__ __launch_bounds__(256) void k_scan_local(const int* __restrict__ in, 
     if (threadIdx.x == 255) bsum[blockIdx.x] = woff + incl;
 }
 
-// exclusive scan of the block totals by one workgroup (nb <= 1024*64)
+// exclusive scan of the block totals by one workgroup
 __global__ __launch_bounds__(1024) void k_scan_sums(int* __restrict__ bsum, int nb, int* total) {
     __shared__ int wsum[16];
     __shared__ int carry;
@@ -256,31 +301,69 @@ __global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, 
 }
 
 __global__ void k_scatter(const int* __restrict__ key, const int* __restrict__ slot,
-                          const int* __restrict__ start, int* __restrict__ perm, const int* iscal) {
+                          const int* __restrict__ start, const double4* __restrict__ vel,
+                          int* __restrict__ perm, int* __restrict__ pid, const int* iscal) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= iscal[IS_N]) return;
-    perm[start[key[i]] + slot[i]] = i;
+    const int d = start[key[i]] + slot[i];
+    perm[d] = i;
+    pid[d] = (int)vel[i].w;
 }
 
-// ascending particle id inside every cell (the order OrdGrid1's 4th loop
-// produces, and hence the reference's neighbour-list order)
-__global__ void k_cell_sort(DP P, const int* __restrict__ start, int* __restrict__ perm,
-                            const double4* __restrict__ vel) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= P.nmax) return;
-    const int b = start[c], e = start[c + 1];
-    for (int a = b + 1; a < e; ++a) {
-        const int pv = perm[a];
-        const double idv = vel[pv].w;
-        int q = a - 1;
-        while (q >= b && vel[perm[q]].w > idv) {
-            perm[q + 1] = perm[q];
-            --q;
+// ascending particle id inside every cell: the order OrdGrid1's 4th loop
+// produces (NPartOrd), hence the reference's neighbour-list order.  A
+// workgroup sorts the segments of 256 consecutive cells in LDS.
+constexpr int CSORT_CAP = 6144;
+
+__global__ __launch_bounds__(256) void k_cell_sort(DP P, const int* __restrict__ start,
+                                                   int* __restrict__ perm, int* __restrict__ pid) {
+    __shared__ int sid[CSORT_CAP];
+    __shared__ int sperm[CSORT_CAP];
+    const int c0 = blockIdx.x * 256;
+    const int c1 = min(c0 + 256, P.nmax);
+    const int g0 = start[c0], g1 = start[c1];
+    const int len = g1 - g0;
+    const int c = c0 + threadIdx.x;
+    if (len <= CSORT_CAP) {
+        for (int t = threadIdx.x; t < len; t += blockDim.x) {
+            sid[t] = pid[g0 + t];
+            sperm[t] = perm[g0 + t];
         }
-        perm[q + 1] = pv;
+        __syncthreads();
+        if (c < c1) {
+            const int b = start[c] - g0, e = start[c + 1] - g0;
+            for (int a = b + 1; a < e; ++a) {
+                const int kv = sid[a], pv = sperm[a];
+                int q = a - 1;
+                while (q >= b && sid[q] > kv) {
+                    sid[q + 1] = sid[q];
+                    sperm[q + 1] = sperm[q];
+                    --q;
+                }
+                sid[q + 1] = kv;
+                sperm[q + 1] = pv;
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < len; t += blockDim.x) perm[g0 + t] = sperm[t];
+    } else if (c < c1) {  // dense window: sort in place in global memory
+        const int b = start[c], e = start[c + 1];
+        for (int a = b + 1; a < e; ++a) {
+            const int kv = pid[a], pv = perm[a];
+            int q = a - 1;
+            while (q >= b && pid[q] > kv) {
+                pid[q + 1] = pid[q];
+                perm[q + 1] = perm[q];
+                --q;
+            }
+            pid[q + 1] = kv;
+            perm[q + 1] = pv;
+        }
     }
 }
 
+// permutes the state into the new order and writes the FP32 cell-relative
+// positions (used by the neighbour test in both precisions)
 __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restrict__ key,
                          const double4* __restrict__ xm_s, const double4* __restrict__ vel_s,
                          const double2* __restrict__ rp_s, const int* __restrict__ lam_s,
@@ -307,60 +390,103 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
     acc_d[p] = acc_s[s];
     const int c = key[s] + 1;
     cell_d[p] = c;
+    int ci, cj, ck;
+    d_cell_indices(P, c, ci, cj, ck);
+    const double ox = P.extr_lo[0] + P.dcd[0] * (double)(ci - 1);
+    const double oy = P.extr_lo[1] + P.dcd[1] * (double)(cj - 1);
+    const double oz = P.extr_lo[2] + P.dcd[2] * (double)(ck - 1);
+    xmf[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
     if (P.f32) {
-        int ci, cj, ck;
-        d_cell_indices(P, c, ci, cj, ck);
-        const double ox = P.extr_lo[0] + P.dcd[0] * (double)(ci - 1);
-        const double oy = P.extr_lo[1] + P.dcd[1] * (double)(cj - 1);
-        const double oz = P.extr_lo[2] + P.dcd[2] * (double)(ck - 1);
-        xmf[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
-        velf[p] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.0f);
+        velf[p] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)r.x);
         rpf[p] = make_float2((float)r.x, (float)r.y);
     }
 }
 
 // ----------------------------------------------------------------------------
-// neighbour lists: CalcVarLength.f90:133-242 (stencil y -> x -> z, ascending
-// particle id in a cell, FP64 r^2 <= (2h)^2, self included, truncation past
-// NMAXPARTJ)
+// neighbour lists: CalcVarLength.f90:133-242 — stencil y -> x -> z, ascending
+// particle id in a cell, r^2 <= (2h)^2 in FP64, self included, truncation past
+// NMAXPARTJ.  Candidates are tested on FP32 cell-relative positions;
+// a candidate whose FP32 r^2 lies within a relative 4e-5 of (2h)^2 is decided
+// by the reference's FP64 expression on the FP64 positions, so the lists are
+// exactly the reference's.  (Each relative coordinate is rounded once from
+// |x - corner| <= dcd, so the FP32 separation is within ~5e-7 dcd per axis and
+// the FP32 r^2 within ~1.5e-6 relative of (2h)^2 near the cut-off.)
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_neighbours(DP P, const double4* __restrict__ xm,
+                                                    const float4* __restrict__ xmf,
                                                     const int* __restrict__ cell,
                                                     const int* __restrict__ start,
-                                                    uint32_t* __restrict__ nbr,
-                                                    int* __restrict__ nnb, int* iscal,
-                                                    unsigned long long* pairs) {
-    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+                                                    uint32_t* __restrict__ nbr, int* __restrict__ nnb,
+                                                    int* iscal, unsigned long long* pairs) {
+    stencil_offsets(P);
+    const int p = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
     const int n = iscal[IS_N];
-    int cnt = 1;  // lanes past the end contribute 0 pairs below
-    if (i < n) {
-        int ig, jg, kg;
-        d_cell_indices(P, cell[i], ig, jg, kg);
-        const double4 xi = xm[i];
+    const float sq = (float)P.square_doubleh;
+    const float lo = sq * (1.0f - 4.0e-5f), hi = sq * (1.0f + 4.0e-5f);
+    int cnt = 1;  // lanes past the end contribute 0 pairs
+    if (p < n) {
+        const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
         const int mj = P.nmaxpartj;
         const size_t N = P.cap;
+        int ic, jc, kc;
+        d_cell_indices(P, cell[p], ic, jc, kc);
+        const float4 xi = xmf[p];
+        double4 xi64;
+        bool have64 = false;
         cnt = 0;
         bool ov = false;
-        for (int dy = -1; dy <= 1 && !ov; ++dy)
-            for (int dx = -1; dx <= 1 && !ov; ++dx)
+        for (int dy = -1; dy <= 1 && !ov; ++dy) {
+            const int jj = jc + dy;
+            if (jj < 1 || jj > nj) continue;
+            for (int dx = -1; dx <= 1 && !ov; ++dx) {
+                const int ii = ic + dx;
+                if (ii < 1 || ii > ni) continue;
                 for (int dz = -1; dz <= 1 && !ov; ++dz) {
-                    const int cn = d_cell_number(P, ig + dx, jg + dy, kg + dz);
-                    if (cn == 0) continue;
+                    const int kk = kc + dz;
+                    if (kk < 1 || kk > nk) continue;
+                    const int cn = ((kk - 1) * nj + (jj - 1)) * ni + ii;
                     const int b = start[cn - 1], e = start[cn];
-                    const uint32_t code = (uint32_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) << NB_SHIFT;
-                    for (int m = b; m < e; ++m) {
-                        const double4 xj = xm[m];
-                        const double r0 = xi.x - xj.x;
-                        const double r1 = xi.y - xj.y;
-                        const double r2 = xi.z - xj.z;
-                        const double rij = r0 * r0 + r1 * r1 + r2 * r2;
-                        if (rij > P.square_doubleh) continue;
-                        if (cnt >= mj) { ov = true; break; }
-                        nbr[(size_t)cnt * N + i] = (uint32_t)m | code;
-                        ++cnt;
+                    const int code = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                    const float4 o = g_soff[code];
+                    const uint32_t ctag = (uint32_t)code << NB_SHIFT;
+                    for (int m0 = b; m0 < e && !ov; m0 += 4) {
+                        float4 xq[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (m0 + u < e) xq[u] = xmf[m0 + u];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int m = m0 + u;
+                            if (m >= e) break;
+                            const float r0 = (xi.x - xq[u].x) - o.x;
+                            const float r1 = (xi.y - xq[u].y) - o.y;
+                            const float r2 = (xi.z - xq[u].z) - o.z;
+                            const float r2f = r0 * r0 + r1 * r1 + r2 * r2;
+                            if (r2f > hi) continue;
+                            if (r2f >= lo) {  // guard band: the reference's FP64 test
+                                if (!have64) {
+                                    xi64 = xm[p];
+                                    have64 = true;
+                                }
+                                const double4 xj64 = xm[m];
+                                const double d0 = xi64.x - xj64.x;
+                                const double d1 = xi64.y - xj64.y;
+                                const double d2 = xi64.z - xj64.z;
+                                const double rij = d0 * d0 + d1 * d1 + d2 * d2;
+                                if (rij > P.square_doubleh) continue;
+                            }
+                            if (cnt >= mj) {
+                                ov = true;
+                                break;
+                            }
+                            nbr[(size_t)cnt * N + p] = (uint32_t)m | ctag;
+                            ++cnt;
+                        }
                     }
                 }
-        nnb[i] = cnt;
+            }
+        }
+        nnb[p] = cnt;
         if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
     }
     // pair-interaction counter: sum_i (nPartIntorno(i) - 1), self excluded
@@ -370,9 +496,7 @@ __global__ __launch_bounds__(256) void k_neighbours(DP P, const double4* __restr
 }
 
 // ----------------------------------------------------------------------------
-// boundary table: ComputeBoundaryDataTab_3D.f90:62-106,
-// FindCloseBoundaryFaces3D.f90:78-188, ComputeBoundaryVolumeIntegrals_P0.f90:
-// 76-155, InterpolateTable.f90:58-82 (FP64, both precision modes)
+// SA-SPH volume integrals (FP64)
 // ----------------------------------------------------------------------------
 __device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __restrict__ kt,
                                    const double* __restrict__ bit, const double locx[3],
@@ -457,23 +581,22 @@ __device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __
     ig[6] = M02; ig[7] = M12; ig[8] = M22;
 }
 
-__global__ __launch_bounds__(256) void k_boundary_tab(DP P, const DFace* __restrict__ faces,
-                                                      const int* __restrict__ gcount,
-                                                      const int* __restrict__ gstart,
-                                                      const int* __restrict__ gvec,
-                                                      const double* __restrict__ kt_g,
-                                                      const double* __restrict__ bit_g,
-                                                      const double4* __restrict__ xm,
-                                                      int* __restrict__ bn, BTab* __restrict__ bt,
-                                                      int* iscal) {
-    __shared__ double kt[41 * 5];
-    __shared__ double bit[256 * 7];
-    for (int t = threadIdx.x; t < 41 * 5; t += blockDim.x) kt[t] = kt_g[t];
-    for (int t = threadIdx.x; t < 256 * 7; t += blockDim.x) bit[t] = bit_g[t];
-    __syncthreads();
-    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int n = iscal[IS_N];
-    if (i >= n) return;
+// ----------------------------------------------------------------------------
+// boundary table: ComputeBoundaryDataTab_3D.f90:62-106 in two kernels.
+// k_boundary_find: FindCloseBoundaryFaces3D.f90:78-188 per particle, appends
+// every (particle, close face) pair to a compact work list; k_boundary_int:
+// ComputeBoundaryVolumeIntegrals_P0.f90:76-155 for one pair per thread, so the
+// 256-direction quadrature keeps every lane busy.  FP64 in both precisions.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_boundary_find(DP P, const DFace* __restrict__ faces,
+                                                       const int* __restrict__ gcount,
+                                                       const int* __restrict__ gstart,
+                                                       const int* __restrict__ gvec,
+                                                       const double4* __restrict__ xm,
+                                                       int* __restrict__ bn, BTab* __restrict__ bt,
+                                                       int* __restrict__ work, int* nwork, int* iscal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= iscal[IS_N]) return;
     const double4 x = xm[i];
     const double xc[3] = {x.x, x.y, x.z};
     const int nc = d_particle_cell(P, x.x, x.y, x.z);
@@ -522,23 +645,44 @@ __global__ __launch_bounds__(256) void k_boundary_tab(DP P, const DFace* __restr
                 }
     }
     bn[i] = ncbf;
+    if (ncbf == 0) return;
+    const int w0 = atomicAdd(nwork, ncbf);
     for (int q = 0; q < ncbf; ++q) {
         const DFace& F = faces[clob[q]];
         const double* ref = &F.GX[3 * (F.nodes - 1)];
-        double loc[3];
+        BTab& rec = bt[(size_t)q * P.cap + i];
         for (int sdi = 0; sdi < 2; ++sdi) {
             double s = 0.0;
             for (int sdj = 0; sdj < 3; ++sdj) s = s + F.T[3 * sdi + sdj] * (xc[sdj] - ref[sdj]);
-            loc[sdi] = s;
+            rec.loc[sdi] = s;
         }
-        loc[2] = lz[q];
-        BTab rec;
+        rec.loc[2] = lz[q];
         rec.face = clob[q];
-        rec.loc[0] = loc[0];
-        rec.loc[1] = loc[1];
-        rec.loc[2] = loc[2];
-        d_volume_integrals(P, F, kt, bit, loc, rec.bi, rec.ig);
-        bt[(size_t)q * P.cap + i] = rec;
+        work[w0 + q] = i | (q << NB_SHIFT);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_boundary_int(DP P, const DFace* __restrict__ faces,
+                                                      const double* __restrict__ kt_g,
+                                                      const double* __restrict__ bit_g,
+                                                      BTab* __restrict__ bt,
+                                                      const int* __restrict__ work,
+                                                      const int* nwork) {
+    __shared__ double kt[41 * 5];
+    __shared__ double bit[256 * 7];
+    for (int t = threadIdx.x; t < 41 * 5; t += blockDim.x) kt[t] = kt_g[t];
+    for (int t = threadIdx.x; t < 256 * 7; t += blockDim.x) bit[t] = bit_g[t];
+    __syncthreads();
+    const int nw = *nwork;
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) {
+        const int e = work[w];
+        const int i = e & (int)NB_MASK, q = (int)((uint32_t)e >> NB_SHIFT);
+        BTab& rec = bt[(size_t)q * P.cap + i];
+        const double loc[3] = {rec.loc[0], rec.loc[1], rec.loc[2]};
+        double bi[8], ig[9];
+        d_volume_integrals(P, faces[rec.face], kt, bit, loc, bi, ig);
+        for (int t = 0; t < 8; ++t) rec.bi[t] = bi[t];
+        for (int t = 0; t < 9; ++t) rec.ig[t] = ig[t];
     }
 }
 
@@ -612,325 +756,427 @@ __global__ void k_dt_begin(DP P, double* scal, unsigned long long* dtmin, int* i
 }
 
 // ----------------------------------------------------------------------------
-// momentum: RHS_momentum_equation.f90:72-179 with
-// AddBoundaryContributions_to_ME3D.f90:118-336 (FP64), inter_EqMoto.f90:
-// 108-283, viscomon.f90:54-71, viscomorris.f90:60-71,
-// AddElasticBoundaryReaction_3D.f90:67-98; fused Leapfrog_momentum.f90:66-86
+// The four pair passes.  A persistent workgroup takes a cell block, stages the
+// block's 9 neighbour rows into LDS (coalesced, contiguous in sorted order)
+// and gives each home particle one lane that walks its neighbour list in the
+// reference's order, reading neighbour records from LDS.  Blocks whose rows
+// exceed the LDS tile read their neighbours from global memory instead.
+// FP64: the reference's expressions, operation for operation.  FP32-mixed:
+// FP32 pair math on cell-relative separations, FP32 sums per particle, FP64
+// wall terms, FP64 integration.
 // ----------------------------------------------------------------------------
 template <bool F32>
-__global__ __launch_bounds__(256) void k_momentum(
-    DP P, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
-    const double4* __restrict__ xm, const double4* __restrict__ vel, const double2* __restrict__ rp,
-    const float4* __restrict__ xmf, const float4* __restrict__ velf, const float2* __restrict__ rpf,
-    const int* __restrict__ lam_in, int* __restrict__ lam_out, const uint32_t* __restrict__ nbr,
-    const int* __restrict__ nnb, double4* __restrict__ acc_out, double4* __restrict__ vel_out,
-    float4* __restrict__ velf_out, const double* scal, int* iscal) {
-    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int n = iscal[IS_N];
-    if (i >= n) return;
+struct Rec;
+template <>
+struct Rec<false> {
+    using X = double4;
+    using V = double4;
+    using RP = double2;
+};
+template <>
+struct Rec<true> {
+    using X = float4;
+    using V = float4;
+    using RP = float2;
+};
+
+
+constexpr uint32_t NB_NONE = 0xFFFFFFFFu;
+
+// Walks particle i's neighbour list in the reference's order, CH entries at a
+// time: the CH list entries and then the CH neighbour records are all loaded
+// before any is used, so their latencies overlap; body() then consumes them
+// strictly in list order (the FP64 sums keep the reference's order).
+template <int CH, class R, class Fetch, class Body>
+__device__ __forceinline__ void walk_list(const uint32_t* __restrict__ nbr, size_t N, int i, int nn,
+                                          Fetch&& fetch, Body&& body) {
+    for (int k0 = 0; k0 < nn; k0 += CH) {
+        uint32_t ev[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) ev[u] = (k0 + u < nn) ? nbr[(size_t)(k0 + u) * N + i] : NB_NONE;
+        R rq[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+            if (ev[u] != NB_NONE) rq[u] = fetch(ev[u]);
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+            if (ev[u] != NB_NONE) body(ev[u], rq[u]);
+    }
+}
+
+template <class X, class V, class RP>
+struct NRec {
+    X x;
+    V v;
+    RP r;
+};
+template <class X, class RP>
+struct NRec2 {
+    X x;
+    RP r;
+};
+
+// SA-SPH terms of the momentum equation (AddBoundaryContributions_to_ME3D.f90:
+// 118-336), FP64; tp/td/tv start at zero
+__device__ __forceinline__ void sa_momentum(const DP& P, const DFace* __restrict__ faces,
+                                            const BTab* __restrict__ bt, int i, int ncbf,
+                                            const double v_i[3], double2 rpi, int lam_prev,
+                                            double tp[3], double td[3], double tv[3], int* iscal) {
     const double h = P.h;
-    const double4 xi = xm[i];
-    const double4 vi = vel[i];   // var == vel here
-    const double2 rpi = rp[i];
-    double tp[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0}, tv[3] = {0.0, 0.0, 0.0};
-    const int ncbf = P.semi ? bn[i] : 0;
-    const double v_i[3] = {vi.x, vi.y, vi.z};
-    if (ncbf > 0) {
-        double ALEt[3] = {0, 0, 0}, gradpt[3] = {0, 0, 0}, VM[3] = {0, 0, 0}, VS[3] = {0, 0, 0};
-        const int lam_prev = lam_in[i];
-        for (int q = 0; q < ncbf; ++q) {
-            const BTab& B = bt[(size_t)q * P.cap + i];
-            const DFace& F = faces[B.face];
-            if (F.btype == 2) continue;
-            if (F.btype != 0 && F.btype != 1) {
-                atomicAdd(&iscal[IS_UNSUPPORTED], 1);
-                break;
-            }
-            if (!(B.loc[2] > 0.0)) continue;
-            double dvij[3], dvn = 0.0;
-            for (int sd = 0; sd < 3; ++sd) {
-                dvij[sd] = 2.0 * (v_i[sd] - F.vel[sd]);
-                dvn = dvn + F.T[6 + sd] * dvij[sd];
-            }
-            double gl[3];
-            for (int sd = 0; sd < 3; ++sd) {
-                double a = 0.0;
-                for (int sj = 0; sj < 3; ++sj) a = a + F.T[3 * sd + sj] * P.grav[sj];
-                gl[sd] = a;
-            }
-            double gloc[3];
-            for (int r = 0; r < 3; ++r) {
-                double a = 0.0;
-                for (int k = 0; k < 3; ++k) a = a + B.ig[3 * k + r] * gl[k];
-                gloc[r] = a;
-            }
-            for (int r = 0; r < 3; ++r) {
-                double a = 0.0;
-                for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * gloc[k];
-                gradpt[r] = gradpt[r] + a;
-            }
-            double aloc[3];
-            for (int d = 0; d < 3; ++d) aloc[d] = 2.0 * rpi.y / rpi.x * B.bi[3 + d];
-            for (int r = 0; r < 3; ++r) {
-                double a = 0.0;
-                for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * aloc[k];
-                ALEt[r] = ALEt[r] + a;
-            }
-            if (P.alfamon > 0.0) {
-                const double Monvisc = P.alfamon * P.celerita * h;
-                const double Mmult = -Monvisc * dvn * B.bi[6];
-                for (int d = 0; d < 3; ++d) VM[d] = VM[d] + Mmult * F.T[6 + d];
-            }
-            double slip = 0.0, cinv = 0.0;
-            if (F.mode == 1) {
-                slip = F.slip;
-                cinv = P.kin_visc;
-            }
-            if (cinv > 0.0 && (lam_prev == 1 || F.lam == 0)) {
-                const double cm = cinv * B.bi[2] * slip;
-                for (int d = 0; d < 3; ++d) VS[d] = VS[d] + cm * (2.0 * (v_i[d] - F.vel[d]));
-            }
+    double ALEt[3] = {0, 0, 0}, gradpt[3] = {0, 0, 0}, VM[3] = {0, 0, 0}, VS[3] = {0, 0, 0};
+    for (int q = 0; q < ncbf; ++q) {
+        const BTab& B = bt[(size_t)q * P.cap + i];
+        const DFace& F = faces[B.face];
+        if (F.btype == 2) continue;
+        if (F.btype != 0 && F.btype != 1) {
+            atomicAdd(&iscal[IS_UNSUPPORTED], 1);
+            break;
         }
-        for (int d = 0; d < 3; ++d) {
-            tp[d] = tp[d] + ALEt[d] + gradpt[d];
-            td[d] = td[d] - VM[d];
-            tv[d] = tv[d] + VS[d];
+        if (!(B.loc[2] > 0.0)) continue;
+        double dvij[3], dvn = 0.0;
+        for (int sd = 0; sd < 3; ++sd) {
+            dvij[sd] = 2.0 * (v_i[sd] - F.vel[sd]);
+            dvn = dvn + F.T[6 + sd] * dvij[sd];
+        }
+        double gl[3];
+        for (int sd = 0; sd < 3; ++sd) {
+            double a = 0.0;
+            for (int sj = 0; sj < 3; ++sj) a = a + F.T[3 * sd + sj] * P.grav[sj];
+            gl[sd] = a;
+        }
+        double gloc[3];
+        for (int r = 0; r < 3; ++r) {
+            double a = 0.0;
+            for (int k = 0; k < 3; ++k) a = a + B.ig[3 * k + r] * gl[k];
+            gloc[r] = a;
+        }
+        for (int r = 0; r < 3; ++r) {
+            double a = 0.0;
+            for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * gloc[k];
+            gradpt[r] = gradpt[r] + a;
+        }
+        double aloc[3];
+        for (int d = 0; d < 3; ++d) aloc[d] = 2.0 * rpi.y / rpi.x * B.bi[3 + d];
+        for (int r = 0; r < 3; ++r) {
+            double a = 0.0;
+            for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * aloc[k];
+            ALEt[r] = ALEt[r] + a;
+        }
+        if (P.alfamon > 0.0) {
+            const double Monvisc = P.alfamon * P.celerita * h;
+            const double Mmult = -Monvisc * dvn * B.bi[6];
+            for (int d = 0; d < 3; ++d) VM[d] = VM[d] + Mmult * F.T[6 + d];
+        }
+        double slip = 0.0, cinv = 0.0;
+        if (F.mode == 1) {
+            slip = F.slip;
+            cinv = P.kin_visc;
+        }
+        if (cinv > 0.0 && (lam_prev == 1 || F.lam == 0)) {
+            const double cm = cinv * B.bi[2] * slip;
+            for (int d = 0; d < 3; ++d) VS[d] = VS[d] + cm * (2.0 * (v_i[d] - F.vel[d]));
         }
     }
-    const int nn = nnb[i];
-    const size_t N = P.cap;
-    double rvs[3];
-    if constexpr (!F32) {
-        const double rhoi = rpi.x, pi = rpi.y;
-        double tpa[3] = {0, 0, 0}, ale[3] = {0, 0, 0};
-        rvs[0] = rvs[1] = rvs[2] = 0.0;
-        const double celt = P.celerita + P.celerita;
-        const double nui = P.kin_visc, nuj = P.kin_visc;
-        const bool morris = nui > 1.0e-12 && nuj > 1.0e-12;
-        for (int k = 0; k < nn; ++k) {
-            const int j = (int)(nbr[(size_t)k * N + i] & NB_MASK);
-            if (j == i) continue;
-            const double4 xj = xm[j];
-            const double4 vj = vel[j];
-            const double2 rpj = rp[j];
-            const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
-            const double rij = R0 * R0 + R1 * R1 + R2 * R2;
-            K64 K;
-            kvals64(P, rij, true, true, false, K);
-            const double rhoj = rpj.x, pj = rpj.y, amassj = xj.w;
-            const double dv0 = vj.x - vi.x, dv1 = vj.y - vi.y, dv2 = vj.z - vi.z;
-            double alpha = (pj - pi) / (rhoi * rhoj);
-            tpa[0] = tpa[0] + -amassj * alpha * R0 * K.K3;
-            tpa[1] = tpa[1] + -amassj * alpha * R1 * K.K3;
-            tpa[2] = tpa[2] + -amassj * alpha * R2 * K.K3;
-            alpha = (pi * (rhoj / rhoi + 1.0) + pj * (rhoi / rhoj - 1.0)) / (rhoi * rhoj);
-            ale[0] = ale[0] - amassj * alpha * R0 * K.K3;
-            ale[1] = ale[1] - amassj * alpha * R1 * K.K3;
-            ale[2] = ale[2] - amassj * alpha * R2 * K.K3;
-            const double vrij = -dv0 * R0 - dv1 * R1 - dv2 * R2;
-            const double TermMon = P.alfamon * celt * h / (rhoi + rhoj);
-            td[0] = td[0] + (amassj * TermMon * vrij * R0 * K.K2 + 0.0);
-            td[1] = td[1] + (amassj * TermMon * vrij * R1 * K.K2 + 0.0);
-            td[2] = td[2] + (amassj * TermMon * vrij * R2 * K.K2 + 0.0);
-            if (morris) {
-                const double rtil = nui * rhoi + nuj * rhoj + 1.0e-3;
-                const double atil = 4.0 * (nui * nuj);
-                const double fac = amassj * atil / rtil;
-                const double rr = R0 * R0 + R1 * R1 + R2 * R2;
-                const double w0 = fac * (-dv0 * K.K2 * rr);
-                const double w1 = fac * (-dv1 * K.K2 * rr);
-                const double w2 = fac * (-dv2 * K.K2 * rr);
-                tv[0] = tv[0] + w0; tv[1] = tv[1] + w1; tv[2] = tv[2] + w2;
-                rvs[0] = rvs[0] + w0; rvs[1] = rvs[1] + w1; rvs[2] = rvs[2] + w2;
-            }
-        }
-        for (int d = 0; d < 3; ++d) tp[d] = tp[d] + tpa[d] + ale[d];
-    } else {
-        // FP32 pair sums on cell-relative coordinates: the pressure term in
-        // its symmetric form -m_j (p_i/rho_i^2 + p_j/rho_j^2) r K3, which is
-        // the sum of the reference's difference term and ALE1 complement.
-        const float4 xf = xmf[i];
-        const float4 vf = velf[i];
-        const float2 rpfi = rpf[i];
-        const float rhoi = rpfi.x, pi = rpfi.y;
-        const float pri = pi / (rhoi * rhoi);
-        float s_p0 = 0.f, s_p1 = 0.f, s_p2 = 0.f, s_d0 = 0.f, s_d1 = 0.f, s_d2 = 0.f;
-        float s_v0 = 0.f, s_v1 = 0.f, s_v2 = 0.f;
-        const float mon = P.f_alfamon * (P.f_celerita + P.f_celerita) * P.f_h;
-        const float nu = P.f_kin_visc;
-        const float atil = 4.0f * (nu * nu);
-        const bool morris = P.kin_visc > 1.0e-12;
-        for (int k = 0; k < nn; ++k) {
-            const uint32_t e = nbr[(size_t)k * N + i];
-            const int j = (int)(e & NB_MASK);
-            if (j == i) continue;
-            const float4 xj = xmf[j];
-            const float4 vj = velf[j];
-            const float2 rpj = rpf[j];
-            float R0, R1, R2;
-            rag32(P, xf, xj, e, R0, R1, R2);
-            const float rij = R0 * R0 + R1 * R1 + R2 * R2;
-            K32 K;
-            kvals32(P, rij, true, true, false, K);
-            const float rhoj = rpj.x, pj = rpj.y, mj = xj.w;
-            const float dv0 = vj.x - vf.x, dv1 = vj.y - vf.y, dv2 = vj.z - vf.z;
-            const float cp = -mj * (pri + pj / (rhoj * rhoj)) * K.K3;
-            s_p0 += cp * R0; s_p1 += cp * R1; s_p2 += cp * R2;
-            const float vrij = -(dv0 * R0 + dv1 * R1 + dv2 * R2);
-            const float cd = mj * mon * __builtin_amdgcn_rcpf(rhoi + rhoj) * vrij * K.K2;
-            s_d0 += cd * R0; s_d1 += cd * R1; s_d2 += cd * R2;
-            if (morris) {
-                const float fac = mj * atil * __builtin_amdgcn_rcpf(nu * rhoi + nu * rhoj + 1.0e-3f);
-                const float cv = -fac * K.K2 * rij;
-                s_v0 += cv * dv0; s_v1 += cv * dv1; s_v2 += cv * dv2;
-            }
-        }
-        tp[0] = tp[0] + (double)s_p0; tp[1] = tp[1] + (double)s_p1; tp[2] = tp[2] + (double)s_p2;
-        td[0] = td[0] + (double)s_d0; td[1] = td[1] + (double)s_d1; td[2] = td[2] + (double)s_d2;
-        tv[0] = tv[0] + (double)s_v0; tv[1] = tv[1] + (double)s_v1; tv[2] = tv[2] + (double)s_v2;
-        rvs[0] = s_v0; rvs[1] = s_v1; rvs[2] = s_v2;
+    for (int d = 0; d < 3; ++d) {
+        tp[d] = tp[d] + ALEt[d] + gradpt[d];
+        td[d] = td[d] - VM[d];
+        tv[d] = tv[d] + VS[d];
     }
-    int lam = 0;
-    if (P.kin_visc > 0.0) {
-        const double apg = sqrt(tp[0] * tp[0] + tp[1] * tp[1] + tp[2] * tp[2]) + P.GI;
-        const double amo = sqrt(rvs[0] * rvs[0] + rvs[1] * rvs[1] + rvs[2] * rvs[2]);
-        double w = 0.0;
-        if (apg != 0.0) w = amo / apg * 100.0;
-        lam = (w > 5.0 || apg == 0.0) ? 1 : 0;
-    }
-    lam_out[i] = lam;
-    double a[3];
-    if (ncbf > 0) {
-        double br[3] = {0.0, 0.0, 0.0};
-        const double zimin = 0.25 * P.dx;
-        const double celer02 = P.eps / P.den0;
-        for (int q = 0; q < ncbf; ++q) {
-            const BTab& B = bt[(size_t)q * P.cap + i];
-            const DFace& F = faces[B.face];
-            if (F.btype != 0 && F.btype != 1) continue;
-            const double zi = B.loc[2];
-            if (zi < zimin && d_point_internal(F, B.loc[0], B.loc[1])) {
-                double vin = 0.0;
-                for (int sd = 0; sd < 3; ++sd) vin = vin + v_i[sd] * F.T[6 + sd];
-                if (vin < 0.0) {
-                    const double nr = -1.0 * celer02 * log((h + zi - zimin) / h) / h;
-                    for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * F.T[6 + d];
-                }
-            }
-        }
-        for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + P.grav[d] + br[d];
-    } else {
-        for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + P.grav[d];
-    }
-    acc_out[i] = make_double4(a[0], a[1], a[2], 0.0);
-    // Leapfrog_momentum: vel = vel + dtvel*acc (kodvel == 0)
-    const double dtvel = scal[SC_DTVEL];
-    const double4 vn = make_double4(vi.x + dtvel * a[0], vi.y + dtvel * a[1], vi.z + dtvel * a[2], vi.w);
-    vel_out[i] = vn;
-    if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, 0.0f);
 }
 
 // ----------------------------------------------------------------------------
-// velocity smoothing: velocity_smoothing.f90:74-133,
-// velocity_smoothing_SA_SPH_3D.f90:64-122 (FP64), velocity_smoothing_2.f90:
-// 59-103; fused Leapfrog_trajectories.f90:48-76 and the particle part of the
-// next time_step_duration
+// momentum: RHS_momentum_equation.f90:72-179, inter_EqMoto.f90:108-283,
+// viscomon.f90:54-71, viscomorris.f90:60-71, AddElasticBoundaryReaction_3D.f90:
+// 67-98; fused Leapfrog_momentum.f90:66-86
 // ----------------------------------------------------------------------------
 template <bool F32>
-__global__ __launch_bounds__(256) void k_vsmooth(
-    DP P, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
-    const double4* __restrict__ xm, const double4* __restrict__ vel1, const double2* __restrict__ rp,
-    const float4* __restrict__ xmf, const float4* __restrict__ velf1, const float2* __restrict__ rpf,
-    const uint32_t* __restrict__ nbr, const int* __restrict__ nnb, double4* __restrict__ vel_out,
-    float4* __restrict__ velf_out, double4* __restrict__ xm_out, const double* scal,
-    unsigned long long* dtmin, int* iscal, int do_traj) {
-    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int n = iscal[IS_N];
-    unsigned long long dtbits = 0x7FF0000000000000ull;
-    if (i < n) {
-        const double4 xi = xm[i];
-        const double4 vi = vel1[i];
-        double v[3] = {0.0, 0.0, 0.0};
-        if (P.tetav > 1.0e-9) {
+__global__ __launch_bounds__(256) void k_momentum(
+    DP P, const DFace* __restrict__ faces,
+    const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
+    const double4* __restrict__ vel, const double2* __restrict__ rp, const float4* __restrict__ xmf,
+    const float4* __restrict__ velf, const float2* __restrict__ rpf, const int* __restrict__ lam_in,
+    int* __restrict__ lam_out, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
+    double4* __restrict__ acc_out, double4* __restrict__ vel_out, float4* __restrict__ velf_out,
+    const double* scal, int* iscal) {
+    using X = typename Rec<F32>::X;
+    using V = typename Rec<F32>::V;
+    using RPT = typename Rec<F32>::RP;
+    stencil_offsets(P);
+    const X* gx;
+    const V* gv;
+    const RPT* gr;
+    if constexpr (F32) {
+        gx = xmf; gv = velf; gr = rpf;
+    } else {
+        gx = xm; gv = vel; gr = rp;
+    }
+    const double h = P.h;
+    const double dtvel = scal[SC_DTVEL];
+    const size_t N = P.cap;
+    {
+        const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+        if (i < iscal[IS_N]) {
+            const double4 vi = vel[i];  // var == vel here
+            const double2 rpi = rp[i];
+            double tp[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0}, tv[3] = {0.0, 0.0, 0.0};
+            const int ncbf = P.semi ? bn[i] : 0;
+            const double v_i[3] = {vi.x, vi.y, vi.z};
+            if (ncbf > 0) sa_momentum(P, faces, bt, i, ncbf, v_i, rpi, lam_in[i], tp, td, tv, iscal);
             const int nn = nnb[i];
-            const size_t N = P.cap;
+            double rvs[3];
             if constexpr (!F32) {
-                for (int k = 0; k < nn; ++k) {
-                    const int j = (int)(nbr[(size_t)k * N + i] & NB_MASK);
-                    const double4 xj = xm[j];
-                    const double4 vj = vel1[j];
-                    const double rhoj = rp[j].x;
+                const double4 xi = xm[i];
+                const double rhoi = rpi.x, pi = rpi.y;
+                double tpa[3] = {0, 0, 0}, ale[3] = {0, 0, 0};
+                rvs[0] = rvs[1] = rvs[2] = 0.0;
+                const double celt = P.celerita + P.celerita;
+                const double nui = P.kin_visc, nuj = P.kin_visc;
+                const bool morris = nui > 1.0e-12 && nuj > 1.0e-12;
+                using RM = NRec<double4, double4, double2>;
+                auto fetch = [&](uint32_t e) {
+                    const int j = (int)(e & NB_MASK);
+                    return RM{gx[j], gv[j], gr[j]};
+                };
+                walk_list<4, RM>(nbr, N, i, nn, fetch, [&](uint32_t e, const RM& q) {
+                    if ((int)(e & NB_MASK) == i) return;
+                    const double4 xj = q.x;
+                    const double4 vj = q.v;
+                    const double2 rpj = q.r;
                     const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
                     const double rij = R0 * R0 + R1 * R1 + R2 * R2;
                     K64 K;
-                    kvals64(P, rij, false, false, true, K);
-                    const double pes = xj.w * K.W / rhoj;
-                    v[0] = v[0] + (vj.x - vi.x) * pes;
-                    v[1] = v[1] + (vj.y - vi.y) * pes;
-                    v[2] = v[2] + (vj.z - vi.z) * pes;
-                }
+                    kvals64(P, rij, true, true, false, K);
+                    const double rhoj = rpj.x, pj = rpj.y, amassj = xj.w;
+                    const double dv0 = vj.x - vi.x, dv1 = vj.y - vi.y, dv2 = vj.z - vi.z;
+                    double alpha = (pj - pi) / (rhoi * rhoj);
+                    tpa[0] = tpa[0] + -amassj * alpha * R0 * K.K3;
+                    tpa[1] = tpa[1] + -amassj * alpha * R1 * K.K3;
+                    tpa[2] = tpa[2] + -amassj * alpha * R2 * K.K3;
+                    alpha = (pi * (rhoj / rhoi + 1.0) + pj * (rhoi / rhoj - 1.0)) / (rhoi * rhoj);
+                    ale[0] = ale[0] - amassj * alpha * R0 * K.K3;
+                    ale[1] = ale[1] - amassj * alpha * R1 * K.K3;
+                    ale[2] = ale[2] - amassj * alpha * R2 * K.K3;
+                    const double vrij = -dv0 * R0 - dv1 * R1 - dv2 * R2;
+                    const double TermMon = P.alfamon * celt * h / (rhoi + rhoj);
+                    td[0] = td[0] + (amassj * TermMon * vrij * R0 * K.K2 + 0.0);
+                    td[1] = td[1] + (amassj * TermMon * vrij * R1 * K.K2 + 0.0);
+                    td[2] = td[2] + (amassj * TermMon * vrij * R2 * K.K2 + 0.0);
+                    if (morris) {
+                        const double rtil = nui * rhoi + nuj * rhoj + 1.0e-3;
+                        const double atil = 4.0 * (nui * nuj);
+                        const double fac = amassj * atil / rtil;
+                        const double rr = R0 * R0 + R1 * R1 + R2 * R2;
+                        const double w0 = fac * (-dv0 * K.K2 * rr);
+                        const double w1 = fac * (-dv1 * K.K2 * rr);
+                        const double w2 = fac * (-dv2 * K.K2 * rr);
+                        tv[0] = tv[0] + w0; tv[1] = tv[1] + w1; tv[2] = tv[2] + w2;
+                        rvs[0] = rvs[0] + w0; rvs[1] = rvs[1] + w1; rvs[2] = rvs[2] + w2;
+                    }
+                });
+                for (int d = 0; d < 3; ++d) tp[d] = tp[d] + tpa[d] + ale[d];
             } else {
+                // symmetric form -m_j (p_i/rho_i^2 + p_j/rho_j^2) r K3 of the
+                // reference's difference term plus its ALE1 complement
                 const float4 xf = xmf[i];
-                const float4 vf = velf1[i];
-                float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-                for (int k = 0; k < nn; ++k) {
-                    const uint32_t e = nbr[(size_t)k * N + i];
+                const float4 vf = velf[i];
+                const float2 rpfi = rpf[i];
+                const float rhoi = rpfi.x, pi = rpfi.y;
+                const float pri = pi * __builtin_amdgcn_rcpf(rhoi * rhoi);
+                float s_p0 = 0.f, s_p1 = 0.f, s_p2 = 0.f, s_d0 = 0.f, s_d1 = 0.f, s_d2 = 0.f;
+                float s_v0 = 0.f, s_v1 = 0.f, s_v2 = 0.f;
+                const float mon = P.f_alfamon * (P.f_celerita + P.f_celerita) * P.f_h;
+                const float nu = P.f_kin_visc;
+                const float atil = 4.0f * (nu * nu);
+                const bool morris = P.kin_visc > 1.0e-12;
+                using RM = NRec<float4, float4, float2>;
+                auto fetch = [&](uint32_t e) {
                     const int j = (int)(e & NB_MASK);
-                    const float4 xj = xmf[j];
-                    const float4 vj = velf1[j];
-                    const float rhoj = rpf[j].x;
+                    return RM{gx[j], gv[j], gr[j]};
+                };
+                walk_list<8, RM>(nbr, N, i, nn, fetch, [&](uint32_t e, const RM& q) {
+                    if ((int)(e & NB_MASK) == i) return;
+                    const float4 xj = q.x;
+                    const float4 vj = q.v;
+                    const float2 rpj = q.r;
                     float R0, R1, R2;
-                    rag32(P, xf, xj, e, R0, R1, R2);
+                    rag32(xf, xj, e, R0, R1, R2);
+                    const float rij = R0 * R0 + R1 * R1 + R2 * R2;
                     K32 K;
-                    kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
-                    const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(rhoj);
-                    s0 += (vj.x - vf.x) * pes;
-                    s1 += (vj.y - vf.y) * pes;
-                    s2 += (vj.z - vf.z) * pes;
-                }
-                v[0] = s0; v[1] = s1; v[2] = s2;
+                    kvals32(P, rij, true, true, false, K);
+                    const float rhoj = rpj.x, pj = rpj.y, mj = xj.w;
+                    const float dv0 = vj.x - vf.x, dv1 = vj.y - vf.y, dv2 = vj.z - vf.z;
+                    const float cp = -mj * (pri + pj * __builtin_amdgcn_rcpf(rhoj * rhoj)) * K.K3;
+                    s_p0 += cp * R0; s_p1 += cp * R1; s_p2 += cp * R2;
+                    const float vrij = -(dv0 * R0 + dv1 * R1 + dv2 * R2);
+                    const float cd = mj * mon * __builtin_amdgcn_rcpf(rhoi + rhoj) * vrij * K.K2;
+                    s_d0 += cd * R0; s_d1 += cd * R1; s_d2 += cd * R2;
+                    if (morris) {
+                        const float fac = mj * atil * __builtin_amdgcn_rcpf(nu * rhoi + nu * rhoj + 1.0e-3f);
+                        const float cv = -fac * K.K2 * rij;
+                        s_v0 += cv * dv0; s_v1 += cv * dv1; s_v2 += cv * dv2;
+                    }
+                });
+                tp[0] = tp[0] + (double)s_p0; tp[1] = tp[1] + (double)s_p1; tp[2] = tp[2] + (double)s_p2;
+                td[0] = td[0] + (double)s_d0; td[1] = td[1] + (double)s_d1; td[2] = td[2] + (double)s_d2;
+                tv[0] = tv[0] + (double)s_v0; tv[1] = tv[1] + (double)s_v1; tv[2] = tv[2] + (double)s_v2;
+                rvs[0] = s_v0; rvs[1] = s_v1; rvs[2] = s_v2;
             }
-            const int ncbf = P.semi ? bn[i] : 0;
-            for (int q = 0; q < ncbf; ++q) {
-                const BTab& B = bt[(size_t)q * P.cap + i];
-                const DFace& F = faces[B.face];
-                if (F.btype == 3 || F.btype == 4 || F.btype == 5) {
-                    v[0] = v[1] = v[2] = 0.0;
-                    break;
-                }
-                if (F.btype != 0 && F.btype != 1) continue;
-                const double DVG[3] = {2.0 * (F.tvel[0] - vi.x), 2.0 * (F.tvel[1] - vi.y),
-                                       2.0 * (F.tvel[2] - vi.z)};
-                double DVL[3];
-                for (int ii = 0; ii < 3; ++ii) {
-                    double a = 0.0;
-                    for (int jj = 0; jj < 3; ++jj) a = a + F.T[3 * ii + jj] * DVG[jj];
-                    DVL[ii] = a;
-                }
-                const double IntWdV = B.bi[1];
-                const double slip = F.mode == 1 ? F.slip : 0.0;
-                const double BCL[3] = {DVL[0] * IntWdV * slip, DVL[1] * IntWdV * slip, DVL[2] * IntWdV};
-                for (int ii = 0; ii < 3; ++ii) {
-                    double a = 0.0;
-                    for (int jj = 0; jj < 3; ++jj) a = a + F.T[3 * jj + ii] * BCL[jj];
-                    v[ii] = v[ii] + a;
-                }
+            int lam = 0;
+            if (P.kin_visc > 0.0) {
+                const double apg = sqrt(tp[0] * tp[0] + tp[1] * tp[1] + tp[2] * tp[2]) + P.GI;
+                const double amo = sqrt(rvs[0] * rvs[0] + rvs[1] * rvs[1] + rvs[2] * rvs[2]);
+                double w = 0.0;
+                if (apg != 0.0) w = amo / apg * 100.0;
+                lam = (w > 5.0 || apg == 0.0) ? 1 : 0;
             }
+            lam_out[i] = lam;
+            double a[3];
+            if (ncbf > 0) {
+                double br[3] = {0.0, 0.0, 0.0};
+                const double zimin = 0.25 * P.dx;
+                const double celer02 = P.eps / P.den0;
+                for (int q = 0; q < ncbf; ++q) {
+                    const BTab& B = bt[(size_t)q * P.cap + i];
+                    const DFace& F = faces[B.face];
+                    if (F.btype != 0 && F.btype != 1) continue;
+                    const double zi = B.loc[2];
+                    if (zi < zimin && d_point_internal(F, B.loc[0], B.loc[1])) {
+                        double vin = 0.0;
+                        for (int sd = 0; sd < 3; ++sd) vin = vin + v_i[sd] * F.T[6 + sd];
+                        if (vin < 0.0) {
+                            const double nr = -1.0 * celer02 * log((h + zi - zimin) / h) / h;
+                            for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * F.T[6 + d];
+                        }
+                    }
+                }
+                for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + P.grav[d] + br[d];
+            } else {
+                for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + P.grav[d];
+            }
+            acc_out[i] = make_double4(a[0], a[1], a[2], 0.0);
+            // Leapfrog_momentum: vel = vel + dtvel*acc (kodvel == 0)
+            const double4 vn =
+                make_double4(vi.x + dtvel * a[0], vi.y + dtvel * a[1], vi.z + dtvel * a[2], vi.w);
+            vel_out[i] = vn;
+            if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, (float)rpi.x);
         }
-        double4 vn = vi;
-        if (P.tetav > 1.0e-9) {
-            const double tv1 = scal[SC_TV1];
-            vn = make_double4(vi.x + tv1 * v[0], vi.y + tv1 * v[1], vi.z + tv1 * v[2], vi.w);
-        }
-        vel_out[i] = vn;
-        if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, 0.0f);
-        if (do_traj) {
-            const double dt = scal[SC_DT];
-            xm_out[i] = make_double4(xi.x + dt * vn.x, xi.y + dt * vn.y, xi.z + dt * vn.z, xi.w);
-        }
-        dtbits = (unsigned long long)__double_as_longlong(d_dt_cfl(P, vn.x, vn.y, vn.z));
+        __syncthreads();
     }
-    if (do_traj) block_min_to(dtbits, dtmin);
+}
+
+// ----------------------------------------------------------------------------
+// velocity smoothing: velocity_smoothing.f90:74-133, velocity_smoothing_SA_SPH
+// _3D.f90:64-122 (FP64), velocity_smoothing_2.f90:59-103; fused
+// Leapfrog_trajectories.f90:48-76 and the particle part of the next
+// time_step_duration.  Neighbour density travels in velf.w (FP32).
+// ----------------------------------------------------------------------------
+template <bool F32>
+__global__ __launch_bounds__(256) void k_vsmooth(
+    DP P, const DFace* __restrict__ faces,
+    const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
+    const double4* __restrict__ vel1, const double2* __restrict__ rp, const float4* __restrict__ xmf,
+    const float4* __restrict__ velf1, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
+    double4* __restrict__ vel_out, float4* __restrict__ velf_out, double4* __restrict__ xm_out,
+    const double* scal, unsigned long long* dtmin, int* iscal) {
+    using X = typename Rec<F32>::X;
+    using V = typename Rec<F32>::V;
+    using RT = typename std::conditional<F32, float, double>::type;
+    stencil_offsets(P);
+    const size_t N = P.cap;
+    const bool smooth = P.tetav > 1.0e-9;
+    const double tv1 = scal[SC_TV1];
+    const double dt = scal[SC_DT];
+    unsigned long long dtbits = 0x7FF0000000000000ull;
+    {
+        const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+        if (i < iscal[IS_N]) {
+            const double4 xi = xm[i];
+            const double4 vi = vel1[i];
+            double v[3] = {0.0, 0.0, 0.0};
+            if (smooth) {
+                const int nn = nnb[i];
+                if constexpr (!F32) {
+                    using RV = NRec<double4, double4, double>;
+                    auto fetch = [&](uint32_t e) {
+                        const int j = (int)(e & NB_MASK);
+                            return RV{xm[j], vel1[j], rp[j].x};
+                    };
+                    walk_list<4, RV>(nbr, N, i, nn, fetch, [&](uint32_t e, const RV& q) {
+                        const double4 xj = q.x;
+                        const double4 vj = q.v;
+                        const double rhoj = q.r;
+                        const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
+                        const double rij = R0 * R0 + R1 * R1 + R2 * R2;
+                        K64 K;
+                        kvals64(P, rij, false, false, true, K);
+                        const double pes = xj.w * K.W / rhoj;
+                        v[0] = v[0] + (vj.x - vi.x) * pes;
+                        v[1] = v[1] + (vj.y - vi.y) * pes;
+                        v[2] = v[2] + (vj.z - vi.z) * pes;
+                                        });
+                } else {
+                    const float4 xf = xmf[i];
+                    const float4 vf = velf1[i];
+                    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+                    using RV = NRec2<float4, float4>;
+                    auto fetch = [&](uint32_t e) {
+                        const int j = (int)(e & NB_MASK);
+                            return RV{xmf[j], velf1[j]};
+                    };
+                    walk_list<8, RV>(nbr, N, i, nn, fetch, [&](uint32_t e, const RV& q) {
+                        const float4 xj = q.x;
+                        const float4 vj = q.r;
+                        float R0, R1, R2;
+                        rag32(xf, xj, e, R0, R1, R2);
+                        K32 K;
+                        kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                        const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(vj.w);
+                        s0 += (vj.x - vf.x) * pes;
+                        s1 += (vj.y - vf.y) * pes;
+                        s2 += (vj.z - vf.z) * pes;
+                                        });
+                    v[0] = s0; v[1] = s1; v[2] = s2;
+                }
+                const int ncbf = P.semi ? bn[i] : 0;
+                for (int q = 0; q < ncbf; ++q) {
+                    const BTab& B = bt[(size_t)q * P.cap + i];
+                    const DFace& F = faces[B.face];
+                    if (F.btype == 3 || F.btype == 4 || F.btype == 5) {
+                        v[0] = v[1] = v[2] = 0.0;
+                        break;
+                    }
+                    if (F.btype != 0 && F.btype != 1) continue;
+                    const double DVG[3] = {2.0 * (F.tvel[0] - vi.x), 2.0 * (F.tvel[1] - vi.y),
+                                           2.0 * (F.tvel[2] - vi.z)};
+                    double DVL[3];
+                    for (int ii = 0; ii < 3; ++ii) {
+                        double a = 0.0;
+                        for (int jj = 0; jj < 3; ++jj) a = a + F.T[3 * ii + jj] * DVG[jj];
+                        DVL[ii] = a;
+                    }
+                    const double IntWdV = B.bi[1];
+                    const double slip = F.mode == 1 ? F.slip : 0.0;
+                    const double BCL[3] = {DVL[0] * IntWdV * slip, DVL[1] * IntWdV * slip,
+                                           DVL[2] * IntWdV};
+                    for (int ii = 0; ii < 3; ++ii) {
+                        double a = 0.0;
+                        for (int jj = 0; jj < 3; ++jj) a = a + F.T[3 * jj + ii] * BCL[jj];
+                        v[ii] = v[ii] + a;
+                    }
+                }
+            }
+            double4 vn = vi;
+            if (smooth) vn = make_double4(vi.x + tv1 * v[0], vi.y + tv1 * v[1], vi.z + tv1 * v[2], vi.w);
+            vel_out[i] = vn;
+            if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, (float)rp[i].x);
+            xm_out[i] = make_double4(xi.x + dt * vn.x, xi.y + dt * vn.y, xi.z + dt * vn.z, xi.w);
+            const unsigned long long b =
+                (unsigned long long)__double_as_longlong(d_dt_cfl(P, vn.x, vn.y, vn.z));
+            dtbits = b < dtbits ? b : dtbits;
+        }
+        __syncthreads();
+    }
+    block_min_to(dtbits, dtmin);
 }
 
 // ----------------------------------------------------------------------------
@@ -940,92 +1186,109 @@ __global__ __launch_bounds__(256) void k_vsmooth(
 // ----------------------------------------------------------------------------
 template <bool F32>
 __global__ __launch_bounds__(256) void k_continuity(
-    DP P, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
-    const double4* __restrict__ xm, const double4* __restrict__ vel, const double2* __restrict__ rp,
-    const float4* __restrict__ xmf, const float4* __restrict__ velf, const float2* __restrict__ rpf,
-    const uint32_t* __restrict__ nbr, const int* __restrict__ nnb, double* __restrict__ dden_out,
-    double2* __restrict__ rp_out, float2* __restrict__ rpf_out, const double* scal, int* iscal) {
-    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int n = iscal[IS_N];
-    if (i >= n) return;
-    const double4 xi = xm[i];
-    const double4 vi = vel[i];
-    const double rhoi = rp[i].x;
-    double dd = 0.0;
-    const int ncbf = P.semi ? bn[i] : 0;
-    if (ncbf > 0) {
-        double gu = 0.0, gv = 0.0, gw = 0.0;
-        const double v_i[3] = {vi.x, vi.y, vi.z};
-        for (int q = 0; q < ncbf; ++q) {
-            const BTab& B = bt[(size_t)q * P.cap + i];
-            const DFace& F = faces[B.face];
-            if (F.btype == 0 || F.btype == 1) {
-                if (B.loc[2] > 0.0) {
-                    double a2[3];
-                    for (int d = 0; d < 3; ++d) a2[d] = 2.0 * (F.vel[d] - v_i[d]);
-                    const double dp = a2[0] * F.T[6] + a2[1] * F.T[7] + a2[2] * F.T[8];
-                    double av[3];
-                    for (int r = 0; r < 3; ++r) {
-                        double a = 0.0;
-                        for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * B.bi[3 + k];
-                        av[r] = a;
-                    }
-                    gu = gu - dp * F.T[6] * av[0];
-                    gv = gv - dp * F.T[7] * av[1];
-                    gw = gw - dp * F.T[8] * av[2];
-                }
-            } else if (F.btype >= 3) {
-                atomicAdd(&iscal[IS_UNSUPPORTED], 1);
-                break;
-            }
-        }
-        dd = dd - rhoi * (gu + gv + gw);
-    }
-    const int nn = nnb[i];
+    DP P, const DFace* __restrict__ faces,
+    const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
+    const double4* __restrict__ vel, const double2* __restrict__ rp, const float4* __restrict__ xmf,
+    const float4* __restrict__ velf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
+    double* __restrict__ dden_out, double2* __restrict__ rp_out, float2* __restrict__ rpf_out,
+    const double* scal, int* iscal) {
+    using X = typename Rec<F32>::X;
+    using V = typename Rec<F32>::V;
+    using RT = typename std::conditional<F32, float, double>::type;
+    stencil_offsets(P);
     const size_t N = P.cap;
-    double inner;
-    if constexpr (!F32) {
-        double a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        for (int k = 0; k < nn; ++k) {
-            const int j = (int)(nbr[(size_t)k * N + i] & NB_MASK);
-            const double4 xj = xm[j];
-            const double4 vj = vel[j];
-            const double rhoj = rp[j].x;
-            const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
-            const double rij = R0 * R0 + R1 * R1 + R2 * R2;
-            K64 K;
-            kvals64(P, rij, true, false, false, K);
-            const double amassj = xj.w;
-            a1 = a1 + rhoi * (amassj / rhoj) * K.K1 * (vj.x - vi.x) * R0;
-            a2 = a2 + rhoi * (amassj / rhoj) * K.K1 * (vj.y - vi.y) * R1;
-            a3 = a3 + rhoi * (amassj / rhoj) * K.K1 * (vj.z - vi.z) * R2;
+    const double dt = scal[SC_DT];
+    const bool eos = iscal[IS_ONGOING] > 0;
+    {
+        const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+        if (i < iscal[IS_N]) {
+            const double4 vi = vel[i];
+            const double2 rpi = rp[i];
+            const double rhoi = rpi.x;
+            double dd = 0.0;
+            const int ncbf = P.semi ? bn[i] : 0;
+            if (ncbf > 0) {
+                double gu = 0.0, gv = 0.0, gw = 0.0;
+                const double v_i[3] = {vi.x, vi.y, vi.z};
+                for (int q = 0; q < ncbf; ++q) {
+                    const BTab& B = bt[(size_t)q * P.cap + i];
+                    const DFace& F = faces[B.face];
+                    if (F.btype == 0 || F.btype == 1) {
+                        if (B.loc[2] > 0.0) {
+                            double a2[3];
+                            for (int d = 0; d < 3; ++d) a2[d] = 2.0 * (F.vel[d] - v_i[d]);
+                            const double dp = a2[0] * F.T[6] + a2[1] * F.T[7] + a2[2] * F.T[8];
+                            double av[3];
+                            for (int r = 0; r < 3; ++r) {
+                                double a = 0.0;
+                                for (int k = 0; k < 3; ++k) a = a + F.T[3 * k + r] * B.bi[3 + k];
+                                av[r] = a;
+                            }
+                            gu = gu - dp * F.T[6] * av[0];
+                            gv = gv - dp * F.T[7] * av[1];
+                            gw = gw - dp * F.T[8] * av[2];
+                        }
+                    } else if (F.btype >= 3) {
+                        atomicAdd(&iscal[IS_UNSUPPORTED], 1);
+                        break;
+                    }
+                }
+                dd = dd - rhoi * (gu + gv + gw);
+            }
+            const int nn = nnb[i];
+            double inner;
+            if constexpr (!F32) {
+                const double4 xi = xm[i];
+                double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+                using RC = NRec<double4, double4, double>;
+                auto fetch = [&](uint32_t e) {
+                    const int j = (int)(e & NB_MASK);
+                    return RC{xm[j], vel[j], rp[j].x};
+                };
+                walk_list<4, RC>(nbr, N, i, nn, fetch, [&](uint32_t e, const RC& q) {
+                    const double4 xj = q.x;
+                    const double4 vj = q.v;
+                    const double rhoj = q.r;
+                    const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
+                    const double rij = R0 * R0 + R1 * R1 + R2 * R2;
+                    K64 K;
+                    kvals64(P, rij, true, false, false, K);
+                    const double amassj = xj.w;
+                    a1 = a1 + rhoi * (amassj / rhoj) * K.K1 * (vj.x - vi.x) * R0;
+                    a2 = a2 + rhoi * (amassj / rhoj) * K.K1 * (vj.y - vi.y) * R1;
+                    a3 = a3 + rhoi * (amassj / rhoj) * K.K1 * (vj.z - vi.z) * R2;
+                                });
+                inner = a1 + a2 + a3;
+            } else {
+                const float4 xf = xmf[i];
+                const float4 vf = velf[i];
+                float s = 0.f;
+                using RC = NRec2<float4, float4>;
+                auto fetch = [&](uint32_t e) {
+                    const int j = (int)(e & NB_MASK);
+                    return RC{xmf[j], velf[j]};
+                };
+                walk_list<8, RC>(nbr, N, i, nn, fetch, [&](uint32_t e, const RC& q) {
+                    const float4 xj = q.x;
+                    const float4 vj = q.r;
+                    float R0, R1, R2;
+                    rag32(xf, xj, e, R0, R1, R2);
+                    K32 K;
+                    kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, true, false, false, K);
+                    s += xj.w * __builtin_amdgcn_rcpf(vj.w) * K.K1 *
+                         ((vj.x - vf.x) * R0 + (vj.y - vf.y) * R1 + (vj.z - vf.z) * R2);
+                                });
+                inner = rhoi * (double)s;
+            }
+            const double dden = dd - inner;
+            dden_out[i] = dden;
+            const double rho1 = rhoi + dt * dden;
+            const double p1 = eos ? P.eps / P.den0 * (rho1 - P.den0) + P.prif : rpi.y;
+            rp_out[i] = make_double2(rho1, p1);
+            if (F32) rpf_out[i] = make_float2((float)rho1, (float)p1);
         }
-        inner = a1 + a2 + a3;
-    } else {
-        const float4 xf = xmf[i];
-        const float4 vf = velf[i];
-        float s = 0.f;
-        for (int k = 0; k < nn; ++k) {
-            const uint32_t e = nbr[(size_t)k * N + i];
-            const int j = (int)(e & NB_MASK);
-            const float4 xj = xmf[j];
-            const float4 vj = velf[j];
-            const float rhoj = rpf[j].x;
-            float R0, R1, R2;
-            rag32(P, xf, xj, e, R0, R1, R2);
-            K32 K;
-            kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, true, false, false, K);
-            s += xj.w * __builtin_amdgcn_rcpf(rhoj) * K.K1 *
-                 ((vj.x - vf.x) * R0 + (vj.y - vf.y) * R1 + (vj.z - vf.z) * R2);
-        }
-        inner = rhoi * (double)s;
+        __syncthreads();
     }
-    const double dden = dd - inner;
-    dden_out[i] = dden;
-    const double rho1 = rhoi + scal[SC_DT] * dden;
-    const double p1 = (iscal[IS_ONGOING] > 0) ? P.eps / P.den0 * (rho1 - P.den0) + P.prif : rp[i].y;
-    rp_out[i] = make_double2(rho1, p1);
-    if (F32) rpf_out[i] = make_float2((float)rho1, (float)p1);
 }
 
 // ----------------------------------------------------------------------------
@@ -1033,71 +1296,86 @@ __global__ __launch_bounds__(256) void k_continuity(
 // ----------------------------------------------------------------------------
 template <bool F32>
 __global__ __launch_bounds__(256) void k_psmooth(
-    DP P, const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
-    const double2* __restrict__ rp, const float4* __restrict__ xmf, const float2* __restrict__ rpf,
-    const uint32_t* __restrict__ nbr, const int* __restrict__ nnb, double* __restrict__ vpres_out,
-    double2* __restrict__ rp_out, float2* __restrict__ rpf_out, const double* scal, int* iscal) {
-    const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int n = iscal[IS_N];
-    if (i >= n) return;
-    const double4 xi = xm[i];
-    const double2 rpi = rp[i];
-    const int nsp = nnb[i];
-    double diffp = 0.0;
-    if (nsp > 0) {
-        const double p0i = P.prif, pi = rpi.y;
-        double somp, app;
-        const size_t N = P.cap;
-        if constexpr (!F32) {
-            somp = 0.0;
-            app = 0.0;
-            for (int k = 0; k < nsp; ++k) {
-                const int j = (int)(nbr[(size_t)k * N + i] & NB_MASK);
-                const double4 xj = xm[j];
-                const double2 rpj = rp[j];
-                const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
-                K64 K;
-                kvals64(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
-                const double pes = xj.w * K.W / rpj.x;
-                app = app + pes;
-                somp = somp + (rpj.y - pi) * pes;
+    DP P, const int* __restrict__ bn, const BTab* __restrict__ bt,
+    const double4* __restrict__ xm, const double2* __restrict__ rp, const float4* __restrict__ xmf,
+    const float2* __restrict__ rpf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
+    double* __restrict__ vpres_out, double2* __restrict__ rp_out, float2* __restrict__ rpf_out,
+    const double* scal, const int* iscal) {
+    using X = typename Rec<F32>::X;
+    using RPT = typename Rec<F32>::RP;
+    stencil_offsets(P);
+    const size_t N = P.cap;
+    const double tp1 = scal[SC_TP1];
+    {
+        const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+        if (i < iscal[IS_N]) {
+            const double2 rpi = rp[i];
+            const int nsp = nnb[i];
+            double diffp = 0.0;
+            if (nsp > 0) {
+                const double p0i = P.prif, pi = rpi.y;
+                double somp, app;
+                if constexpr (!F32) {
+                    const double4 xi = xm[i];
+                    somp = 0.0;
+                    app = 0.0;
+                    using RP2 = NRec2<double4, double2>;
+                    auto fetch = [&](uint32_t e) {
+                        const int j = (int)(e & NB_MASK);
+                            return RP2{xm[j], rp[j]};
+                    };
+                    walk_list<4, RP2>(nbr, N, i, nsp, fetch, [&](uint32_t e, const RP2& q) {
+                        const double4 xj = q.x;
+                        const double2 rpj = q.r;
+                        const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
+                        K64 K;
+                        kvals64(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                        const double pes = xj.w * K.W / rpj.x;
+                        app = app + pes;
+                        somp = somp + (rpj.y - pi) * pes;
+                                        });
+                } else {
+                    const float4 xf = xmf[i];
+                    const float pif = rpf[i].y;
+                    float s_a = 0.f, s_p = 0.f;
+                    using RP2 = NRec2<float4, float2>;
+                    auto fetch = [&](uint32_t e) {
+                        const int j = (int)(e & NB_MASK);
+                            return RP2{xmf[j], rpf[j]};
+                    };
+                    walk_list<8, RP2>(nbr, N, i, nsp, fetch, [&](uint32_t e, const RP2& q) {
+                        const float4 xj = q.x;
+                        const float2 rpj = q.r;
+                        float R0, R1, R2;
+                        rag32(xf, xj, e, R0, R1, R2);
+                        K32 K;
+                        kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                        const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(rpj.x);
+                        s_a += pes;
+                        s_p += (rpj.y - pif) * pes;
+                                        });
+                    somp = s_p;
+                    app = s_a;
+                }
+                const int ncbf = P.semi ? bn[i] : 0;
+                for (int q = 0; q < ncbf; ++q) {
+                    const BTab& B = bt[(size_t)q * P.cap + i];
+                    if (B.loc[2] > 0.0) app = app + B.bi[1];
+                }
+                if (app < 0.95) diffp = somp + (p0i - pi) * (1.0 - app);
+                else diffp = somp / app;
             }
-        } else {
-            const float4 xf = xmf[i];
-            const float pif = rpf[i].y;
-            float s_a = 0.f, s_p = 0.f;
-            for (int k = 0; k < nsp; ++k) {
-                const uint32_t e = nbr[(size_t)k * N + i];
-                const int j = (int)(e & NB_MASK);
-                const float4 xj = xmf[j];
-                const float2 rpj = rpf[j];
-                float R0, R1, R2;
-                rag32(P, xf, xj, e, R0, R1, R2);
-                K32 K;
-                kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
-                const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(rpj.x);
-                s_a += pes;
-                s_p += (rpj.y - pif) * pes;
+            vpres_out[i] = diffp;
+            double p2 = rpi.y, rho2 = rpi.x;
+            if (P.tetap > 0.0) {
+                p2 = rpi.y + tp1 * diffp;
+                rho2 = P.den0 * (1.0 + (p2 - P.prif) / P.eps);
             }
-            somp = s_p;
-            app = s_a;
+            rp_out[i] = make_double2(rho2, p2);
+            if (F32) rpf_out[i] = make_float2((float)rho2, (float)p2);
         }
-        const int ncbf = P.semi ? bn[i] : 0;
-        for (int q = 0; q < ncbf; ++q) {
-            const BTab& B = bt[(size_t)q * P.cap + i];
-            if (B.loc[2] > 0.0) app = app + B.bi[1];
-        }
-        if (app < 0.95) diffp = somp + (p0i - pi) * (1.0 - app);
-        else diffp = somp / app;
+        __syncthreads();
     }
-    vpres_out[i] = diffp;
-    double p2 = rpi.y, rho2 = rpi.x;
-    if (P.tetap > 0.0) {
-        p2 = rpi.y + scal[SC_TP1] * diffp;
-        rho2 = P.den0 * (1.0 + (p2 - P.prif) / P.eps);
-    }
-    rp_out[i] = make_double2(rho2, p2);
-    if (F32) rpf_out[i] = make_float2((float)rho2, (float)p2);
 }
 
 }  // namespace sph

@@ -24,7 +24,8 @@ EXPORTS = [
     "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_get_neighbour_lists", "sph_dt", "sph_momentum",
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
     "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
-    "sph_preloop", "sph_step", "sph_step_async", "sph_sync", "sph_set_timing", "sph_get_timing",
+    "sph_preloop", "sph_step", "sph_step_async", "sph_sync", "sph_get_launch", "sph_set_timing",
+    "sph_get_timing",
     "sph_last_error",
 ]
 
@@ -41,7 +42,8 @@ class SphParams(C.Structure):
                 ("den0", C.c_double), ("eps", C.c_double), ("celerita", C.c_double),
                 ("alfamon", C.c_double), ("kin_visc", C.c_double), ("cfl", C.c_double),
                 ("vsc_coeff", C.c_double), ("pesodt", C.c_double), ("tetap", C.c_double),
-                ("tetav", C.c_double), ("semi", C.c_int32), ("reserved", C.c_int32)]
+                ("tetav", C.c_double), ("semi", C.c_int32), ("wg_size", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class SphFaces(C.Structure):
@@ -76,6 +78,7 @@ def load():
     L.sph_step.argtypes = [C.c_void_p, C.c_int32]
     L.sph_step_async.argtypes = [C.c_void_p, C.c_int32]
     L.sph_set_timing.argtypes = [C.c_void_p, C.c_int32]
+    L.sph_get_launch.argtypes = [C.c_void_p, PI]
     L.sph_get_timing.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, PD, PI, C.c_int32]
     L.sph_last_error.restype = C.c_char_p
     for name in EXPORTS:
@@ -123,7 +126,7 @@ def faces_struct(tb):
     return s, k
 
 
-def params_struct(tb, precision=FP64, capacity=None, semi=True):
+def params_struct(tb, precision=FP64, capacity=None, semi=True, tune=None):
     s, g, case = tb.scalars, tb.grid, tb.case
     p = SphParams()
     p.precision = precision
@@ -142,17 +145,19 @@ def params_struct(tb, precision=FP64, capacity=None, semi=True):
     p.cfl, p.vsc_coeff, p.pesodt = case.cfl, case.vsc_coeff, case.pesodt
     p.tetap, p.tetav = case.teta_p, case.teta_v
     p.semi = 1 if semi else 0
+    for k, v in (tune or {}).items():
+        setattr(p, k, int(v))
     return p
 
 
 class SphContext:
     """One device solve (one C-ABI context)."""
 
-    def __init__(self, tb, precision=FP64, capacity=None, semi=True):
+    def __init__(self, tb, precision=FP64, capacity=None, semi=True, tune=None):
         self.L = load()
         self.tb = tb
         self.precision = precision
-        self.params = params_struct(tb, precision, capacity, semi)
+        self.params = params_struct(tb, precision, capacity, semi, tune)
         fs, self._keep = faces_struct(tb)
         h = C.c_void_p()
         self._check(self.L.sph_ctx_create(C.byref(self.params), C.byref(fs), C.byref(h)),
@@ -238,6 +243,11 @@ class SphContext:
 
     def sync(self):
         return self.phase("sync")
+
+    def launch(self):
+        v = np.zeros(3, np.int32)
+        self._check(self.L.sph_get_launch(self.h, _pi(v)), "sph_get_launch")
+        return dict(wg_size=int(v[0]), chunk=int(v[1]), grid=int(v[2]))
 
     def set_timing(self, on=True):
         self._check(self.L.sph_set_timing(self.h, 1 if on else 0), "sph_set_timing")

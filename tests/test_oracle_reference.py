@@ -1,0 +1,121 @@
+"""The oracle (oracle/sph_oracle.cpp) against the reference itself.
+
+1. Against the committed golden fixtures written by tests/golden/make_golden.py
+   from the flang build of the reference: bit-identical neighbour lists (in
+   the reference's order), SA-SPH boundary table and fields.
+2. Where oracle/_ref/harness3d.x is built (this container), against a live
+   reference run: longer horizons, jittered input.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle
+from sphera_amd.case import DamBreak3D, build
+
+GOLD = Path(__file__).resolve().parent / "golden"
+FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
+HARNESS = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "harness3d.x"
+
+
+def _tiny():
+    return DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
+
+
+def _small():
+    return DamBreak3D(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), dx=0.01, name="small")
+
+
+def _check_lists(o, counts, flat):
+    cnt, ids = o.lists()
+    assert np.array_equal(cnt, counts)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    for i in range(len(cnt)):
+        assert np.array_equal(ids[i, :cnt[i]] + 1, flat[off[i]:off[i + 1]]), i
+
+
+def _check_btab(o, tb, ncbf, face, loc, bi, ig):
+    nc, f, l, b, g = o.btab()
+    assert np.array_equal(nc, ncbf)
+    k = 0
+    for i in range(len(nc)):
+        for q in range(nc[i]):
+            assert tb.faces.face_id[f[i, q]] == face[k]
+            assert np.array_equal(l[i, q], loc[k])
+            assert np.array_equal(b[i, q], bi[k])
+            assert np.array_equal(g[i, q], ig[k])
+            k += 1
+
+
+def test_golden_tiny_bit_identical():
+    g = np.load(GOLD / "ref_tiny.npz")
+    tb = build(_tiny())
+    o = Oracle(tb)
+    o.preloop()
+    st = o.state()
+    assert st["dt"] == g["s0_dt"] and st["dt_average"] == g["s0_dt_average"]
+    _check_lists(o, g["s0_nbr_count"], g["s0_nbr"])
+    _check_btab(o, tb, g["s0_ncbf"], g["s0_bt_face"], g["s0_bt_loc"], g["s0_bt_bi"], g["s0_bt_ig"])
+    o.step(5)
+    st = o.state()
+    for f in FIELDS:
+        assert np.array_equal(st[f], g["s5_" + f]), f
+    assert st["dt"] == g["s5_dt"] and st["time"] == g["s5_time"]
+    _check_lists(o, g["s5_nbr_count"], g["s5_nbr"])
+    _check_btab(o, tb, g["s5_ncbf"], g["s5_bt_face"], g["s5_bt_loc"], g["s5_bt_bi"], g["s5_bt_ig"])
+
+
+def test_golden_small_jitter_bit_identical():
+    g = np.load(GOLD / "ref_small_jitter.npz")
+    tb = build(_small())
+    o = Oracle(tb, coord=g["ic_coord"])
+    o.preloop()
+    o.step(10)
+    st = o.state()
+    for f in FIELDS:
+        assert np.array_equal(st[f], g["s10_" + f]), f
+    assert np.array_equal(o.state()["coord"], g["s10_coord"])
+
+
+def test_state_restart_from_reference_step():
+    """Set the oracle to the reference's step-0 state, run 5 steps, compare."""
+    g = np.load(GOLD / "ref_tiny.npz")
+    tb = build(_tiny())
+    o = Oracle(tb)
+    st = {k: g["s0_" + k] for k in ("coord", "vel", "acc", "dens", "pres", "laminar_flag", "dt",
+                                      "dt_average", "time", "it")}
+    st["var"] = g["s0_vel"]
+    st["nag"] = len(st["coord"])
+    st["it"] = int(st["it"])
+    o.set_state(st)
+    o.call("calcvarlength")
+    o.call("boundary_tab")
+    o.step(5)
+    s = o.state()
+    for f in ("coord", "vel", "dens", "pres"):
+        assert np.array_equal(s[f], g["s5_" + f]), f
+
+
+@pytest.mark.skipif(not HARNESS.exists(), reason="reference not built here")
+@pytest.mark.parametrize("jit", [False, True])
+def test_live_reference_run(jit, tmp_path):
+    from oracle.decks import read_dump, run_harness
+    case = DamBreak3D(tank=(0.5, 0.2, 0.3), column=(0.15, 0.2, 0.15), dx=0.01, name="live")
+    tb = build(case)
+    ic = None
+    if jit:
+        rng = np.random.default_rng(7)
+        ic = tb.particles["coord"] + rng.uniform(-0.01, 0.01, tb.particles["coord"].shape) * case.dx
+    hdr, steps = read_dump(run_harness(case, nsteps=60, every=30, lists=False, ic_coord=ic,
+                                       workdir=tmp_path))
+    o = Oracle(tb, coord=ic)
+    o.preloop()
+    prev = 0
+    for r in steps[1:]:
+        o.step(r["it"] - prev)
+        prev = r["it"]
+        st = o.state()
+        assert st["dt"] == r["dt"]
+        for f in FIELDS:
+            assert np.array_equal(st[f], r[f]), (f, r["it"])
