@@ -21,6 +21,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 REF_DIR = HERE / "_ref"
 HARNESS = REF_DIR / "harness3d.x"
+HARNESS_HIP = REF_DIR / "harness3d_hip.x"
 REFERENCE = REF_DIR / "sphera3d.x"
 
 
@@ -87,10 +88,12 @@ def write_case_dir(case, root, itmax=1000000000):
 
 
 def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
-                workdir=None, timeout=3600):
-    """Run the reference harness; returns the path of its dump file."""
-    if not HARNESS.exists():
-        raise FileNotFoundError(f"{HARNESS} not built (make -f oracle/Makefile.ref)")
+                workdir=None, timeout=3600, exe=None, extra_env=None):
+    """Run the reference harness (or ``exe``, e.g. the device drop-in harness
+    HARNESS_HIP); returns the path of its dump file."""
+    exe = Path(exe or HARNESS)
+    if not exe.exists():
+        raise FileNotFoundError(f"{exe} not built (make -f oracle/Makefile.ref)")
     wd = Path(workdir or tempfile.mkdtemp(prefix="sph_ref_")).resolve()
     write_case_dir(case, wd)
     env = dict(os.environ)
@@ -100,13 +103,14 @@ def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
     env["SPH_HARNESS_LISTS"] = "1" if lists else "0"
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
+    env.update(extra_env or {})
     if ic_coord is not None:
         icf = wd / "ic.bin"
         with open(icf, "wb") as fh:
             fh.write(struct.pack("<i", len(ic_coord)))
             fh.write(np.ascontiguousarray(ic_coord, dtype="<f8").tobytes())
         env["SPH_HARNESS_IC"] = str(icf)
-    res = subprocess.run([str(HARNESS), case.name], cwd=wd, env=env, capture_output=True,
+    res = subprocess.run([str(exe), case.name], cwd=wd, env=env, capture_output=True,
                          text=True, timeout=timeout)
     if res.returncode != 0 or not (wd / "dump.bin").exists():
         raise RuntimeError(f"reference harness failed ({res.returncode}):\n{res.stdout[-3000:]}"

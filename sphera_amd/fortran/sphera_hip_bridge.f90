@@ -1,0 +1,266 @@
+! Bridge between SPHERA v10's module globals and the C ABI of the MI355X hot
+! path (include/sphera_hip.h, interfaces in sphera_hip_c.f90).
+!
+! It is compiled against the reference's own modules and replaces, inside
+! time_step_loop (src/Main_algorithm/time_step_loop.f90:175-489), the calls of
+! the hot-path subroutines by device calls:
+!
+!   sph_hip_open(prec,ctx)  packs Domain, Grid, Med(1), input_any_t and the
+!                           SA-SPH frontier tables (BoundaryFace, BFaceList,
+!                           Tratto, Partz, GCBFPointers, GCBFVector, kerneltab,
+!                           BoundIntegralTab) and uploads pg(1:nag)
+!   sph_hip_pull(ctx)       device state -> pg(1:nag), nag, dt, dt_average,
+!                           simulation_time (at output steps)
+!   sph_hip_check(rc,where) stops like diagnostic() on a non-zero status
+!
+! Scope the device implements (checked by sph_hip_open): Domain%tipo="semi",
+! leapfrog (time_split=1), one medium, no NormFix, no open/source faces, no
+! dt_alfa_Mon, fluid particles only.
+module sphera_hip_bridge
+use iso_c_binding
+use sphera_hip_c
+use Static_allocation_module
+use Hybrid_allocation_module
+use Dynamic_allocation_module
+implicit none
+private
+public :: sph_hip_open, sph_hip_pull, sph_hip_check, sph_hip_close
+
+! Host copies of the frontier tables in the C layout; they only need to live
+! for the sph_ctx_create call, but are kept so a caller may inspect them.
+real(c_double),allocatable,target,save :: fT(:), fLX(:), fGX(:), farea(:), fvel(:)
+real(c_double),allocatable,target,save :: ftvel(:), fslip(:), fkt(:), fbit(:)
+integer(c_int32_t),allocatable,target,save :: fnodes(:), fbtype(:), fmode(:), flam(:)
+integer(c_int32_t),allocatable,target,save :: gcount(:), gstart(:), gvec(:)
+
+contains
+
+subroutine sph_hip_check(rc,where)
+integer(c_int),intent(in) :: rc
+character(len=*),intent(in) :: where
+if (rc==0) return
+write(*,'(a,a,a,i5,a,a)') "sphera_hip: ",where," failed, status ",rc,": ",         &
+   sph_error_message()
+stop 1
+end subroutine sph_hip_check
+
+integer(c_int32_t) function btype_code(tipo)
+character(len=4),intent(in) :: tipo
+select case (tipo)
+   case ("fixe")
+      btype_code = 0
+   case ("tapi")
+      btype_code = 1
+   case ("open")
+      btype_code = 2
+   case ("sour")
+      btype_code = 3
+   case ("velo")
+      btype_code = 4
+   case ("flow")
+      btype_code = 5
+   case default
+      btype_code = 6
+end select
+end function btype_code
+
+subroutine pack_params(prec,p)
+integer(c_int32_t),intent(in) :: prec
+type(sph_params_t),intent(out) :: p
+p%precision = prec
+p%capacity = max(nag,1)
+p%nmaxpartj = NMAXPARTJ
+p%maxclose = input_any_t%MAXCLOSEBOUNDFACES
+p%ncd(1:3) = Grid%ncd(1:3)
+p%extr_lo(1:3) = Grid%extr(1:3,1)
+p%dcd(1:3) = Grid%dcd(1:3)
+p%dx = Domain%dx
+p%h = Domain%h
+p%coefke = Domain%coefke
+p%coefkacl = Domain%coefkacl
+p%eta = eta
+p%eta2 = eta2
+p%grav(1:3) = Domain%grav(1:3)
+p%prif = Domain%prif
+p%den0 = Med(1)%den0
+p%eps = Med(1)%eps
+p%celerita = Med(1)%celerita
+p%alfamon = Med(1)%alfaMon
+p%kin_visc = Med(1)%kin_visc
+p%cfl = input_any_t%CFL
+p%vsc_coeff = input_any_t%vsc_coeff
+p%pesodt = pesodt
+p%tetap = input_any_t%TetaP
+p%tetav = input_any_t%TetaV
+p%semi = 0
+if (Domain%tipo=="semi") p%semi = 1
+p%wg_size = 0
+p%reserved(:) = 0
+end subroutine pack_params
+
+! Frontier faces in BFaceList order; GCBFVector entries (BoundaryFace
+! indices) become 0-based positions in that order.
+subroutine pack_faces(f)
+type(sph_faces_t),intent(out) :: f
+integer(4) :: kf,nf,st,k,n,c,r,nv
+integer(4),allocatable :: pos(:)
+allocate(pos(size(BoundaryFace)))
+pos(:) = -1
+k = 0
+do kf=1,NumFacce
+   nf = BFaceList(kf)
+   if (nf==0) cycle
+   pos(nf) = k
+   k = k + 1
+enddo
+n = k
+if (allocated(fT)) call free_tables
+allocate(fT(9*max(n,1)),fLX(8*max(n,1)),fGX(12*max(n,1)),farea(max(n,1)))
+allocate(fvel(3*max(n,1)),ftvel(3*max(n,1)),fslip(max(n,1)))
+allocate(fnodes(max(n,1)),fbtype(max(n,1)),fmode(max(n,1)),flam(max(n,1)))
+do kf=1,NumFacce
+   nf = BFaceList(kf)
+   if (nf==0) cycle
+   k = pos(nf)
+   st = BoundaryFace(nf)%stretch
+   do c=1,3
+      do r=1,3
+         fT(9*k+3*(c-1)+r) = BoundaryFace(nf)%T(r,c)
+      enddo
+   enddo
+   do c=1,4
+      fLX(8*k+2*(c-1)+1:8*k+2*(c-1)+2) = BoundaryFace(nf)%Node(c)%LX(1:2)
+      fGX(12*k+3*(c-1)+1:12*k+3*(c-1)+3) = BoundaryFace(nf)%Node(c)%GX(1:3)
+   enddo
+   farea(k+1) = BoundaryFace(nf)%area
+   fvel(3*k+1:3*k+3) = BoundaryFace(nf)%velocity(1:3)
+   ftvel(3*k+1:3*k+3) = Tratto(st)%velocity(1:3)
+   fnodes(k+1) = BoundaryFace(nf)%nodes
+   fbtype(k+1) = btype_code(Tratto(st)%tipo)
+   fmode(k+1) = Partz(Tratto(st)%zone)%slip_coefficient_mode
+   fslip(k+1) = Partz(Tratto(st)%zone)%BC_shear_stress_input
+   flam(k+1) = 0
+   if (Tratto(st)%laminar_no_slip_check) flam(k+1) = 1
+enddo
+allocate(gcount(Grid%nmax),gstart(Grid%nmax))
+gcount(:) = GCBFPointers(1:Grid%nmax,1)
+gstart(:) = GCBFPointers(1:Grid%nmax,2)
+nv = size(GCBFVector)
+allocate(gvec(max(nv,1)))
+gvec(:) = 0
+do k=1,nv
+   if (GCBFVector(k)>0) gvec(k) = pos(GCBFVector(k))
+enddo
+allocate(fkt(41*5),fbit(256*7))
+do r=0,40
+   do c=0,4
+      fkt(5*r+c+1) = kerneltab(r,c)
+   enddo
+enddo
+do r=1,256
+   do c=1,7
+      fbit(7*(r-1)+c) = BoundIntegralTab(r,c)
+   enddo
+enddo
+f%nf = n
+f%T = c_loc(fT)
+f%LX = c_loc(fLX)
+f%GX = c_loc(fGX)
+f%area = c_loc(farea)
+f%velocity = c_loc(fvel)
+f%tvel = c_loc(ftvel)
+f%slip_coeff = c_loc(fslip)
+f%nodes = c_loc(fnodes)
+f%btype = c_loc(fbtype)
+f%slip_mode = c_loc(fmode)
+f%lam_check = c_loc(flam)
+f%gcbf_count = c_loc(gcount)
+f%gcbf_start = c_loc(gstart)
+f%gcbf_vec = c_loc(gvec)
+f%kerneltab = c_loc(fkt)
+f%ktdelta = ktdelta
+f%bitab = c_loc(fbit)
+deallocate(pos)
+end subroutine pack_faces
+
+subroutine free_tables
+deallocate(fT,fLX,fGX,farea,fvel,ftvel,fslip,fnodes,fbtype,fmode,flam)
+if (allocated(gcount)) deallocate(gcount,gstart,gvec)
+if (allocated(fkt)) deallocate(fkt,fbit)
+end subroutine free_tables
+
+! Creates the device context for the current case and uploads pg(1:nag).
+subroutine sph_hip_open(prec,ctx)
+integer(c_int32_t),intent(in) :: prec
+type(c_ptr),intent(out) :: ctx
+type(sph_params_t) :: p
+type(sph_faces_t) :: f
+real(c_double),allocatable :: xyz(:),uvw(:),rho(:),pp(:),m(:)
+integer(4) :: i
+if ((Domain%tipo/="semi").or.(Domain%time_split/=1).or.(NMedium/=1).or.           &
+   Domain%NormFix.or.(NumOpenFaces>0).or.(SourceFace/=0).or.dt_alfa_Mon) then
+   write(*,*) "sphera_hip: case outside the device scope (semi, leapfrog, one ",   &
+      "medium, fixed/tapis frontiers)"
+   stop 1
+endif
+call pack_params(prec,p)
+call pack_faces(f)
+call sph_hip_check(sph_ctx_create(p,f,ctx),"sph_ctx_create")
+allocate(xyz(3*max(nag,1)),uvw(3*max(nag,1)),rho(max(nag,1)),pp(max(nag,1)),m(max(nag,1)))
+do i=1,nag
+   xyz(3*i-2:3*i) = pg(i)%coord(1:3)
+   uvw(3*i-2:3*i) = pg(i)%vel(1:3)
+   rho(i) = pg(i)%dens
+   pp(i) = pg(i)%pres
+   m(i) = pg(i)%mass
+enddo
+call sph_hip_check(sph_state_upload(ctx,int(nag,c_int32_t),xyz,uvw,rho,pp,m),          &
+   "sph_state_upload")
+deallocate(xyz,uvw,rho,pp,m)
+end subroutine sph_hip_open
+
+! Device state -> pg(1:nag).  Particles that left the grid are removed on the
+! device; the remaining rows come back in pg order and fill pg(1:n).
+subroutine sph_hip_pull(ctx)
+type(c_ptr),intent(in) :: ctx
+real(c_double),allocatable :: xyz(:),uvw(:),a(:),rho(:),pp(:),dd(:),vp(:)
+integer(c_int32_t),allocatable :: ids(:)
+integer(c_int32_t) :: n
+real(c_double) :: sc(5)
+integer(4) :: i,k
+call sph_hip_check(sph_particle_count(ctx,n),"sph_particle_count")
+allocate(xyz(3*max(n,1)),uvw(3*max(n,1)),a(3*max(n,1)),rho(max(n,1)),pp(max(n,1)))
+allocate(dd(max(n,1)),vp(max(n,1)),ids(max(n,1)))
+call sph_hip_check(sph_state_download(ctx,xyz,uvw,a,rho,pp,dd,vp,ids),                &
+   "sph_state_download")
+do k=1,n
+   i = ids(k) + 1
+   if (i/=k) pg(k) = pg(i)
+   pg(k)%coord(1:3) = xyz(3*k-2:3*k)
+   pg(k)%vel(1:3) = uvw(3*k-2:3*k)
+   pg(k)%var(1:3) = uvw(3*k-2:3*k)
+   pg(k)%acc(1:3) = a(3*k-2:3*k)
+   pg(k)%dens = rho(k)
+   pg(k)%pres = pp(k)
+   pg(k)%dden = dd(k)
+   pg(k)%vpres = vp(k)
+enddo
+do k=n+1,nag
+   pg(k) = PgZero
+enddo
+nag = n
+call sph_hip_check(sph_get_scalars(ctx,sc),"sph_get_scalars")
+dt = sc(1)
+dt_average = sc(2)
+simulation_time = sc(3)
+deallocate(xyz,uvw,a,rho,pp,dd,vp,ids)
+end subroutine sph_hip_pull
+
+subroutine sph_hip_close(ctx)
+type(c_ptr),intent(inout) :: ctx
+call sph_hip_check(sph_ctx_destroy(ctx),"sph_ctx_destroy")
+ctx = c_null_ptr
+if (allocated(fT)) call free_tables
+end subroutine sph_hip_close
+
+end module sphera_hip_bridge

@@ -30,3 +30,18 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
     assert set(runtime.EXPORTS) <= set(declared())
+
+
+def test_dropin_harness_fails_loudly_without_device(tmp_path):
+    """The reference program linked with the Fortran bridge stops with the C
+    ABI's status and message when no device is there (no CPU fallback)."""
+    import torch
+    from oracle.decks import HARNESS_HIP, run_harness
+    from sphera_amd.case import DamBreak3D
+    if not HARNESS_HIP.exists():
+        pytest.skip("drop-in harness not built")
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a device is present")
+    case = DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
+    with pytest.raises(RuntimeError, match="sph_ctx_create failed, status +1"):
+        run_harness(case, 2, lists=False, workdir=tmp_path, exe=HARNESS_HIP, timeout=60)

@@ -1,0 +1,69 @@
+"""The reference program itself driving the device through the Fortran
+bridge (sphera_amd/fortran/): oracle/_ref/harness3d_hip.x is the flang build
+of the reference with time_step_loop replaced by a unit that calls the C ABI
+via ISO_C_BINDING.  Its FP64 output must equal the reference's own output
+(the golden fixtures written from the flang build) bit for bit; its
+FP32-mixed output must stay within the per-field tolerances of
+test_gpu_parity.py.  Skipped where the harness was not built (it needs the
+reference sources at build time)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle.decks import HARNESS_HIP, read_dump, run_harness
+from sphera_amd.case import DamBreak3D
+from tests.test_gpu_parity import FP32_TOL
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not HARNESS_HIP.exists(), reason="drop-in harness not built")]
+
+FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
+
+
+def _tiny():
+    return DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
+
+
+def _small():
+    return DamBreak3D(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), dx=0.01, name="small")
+
+
+def _run(case, nsteps, every, tmp_path, precision=0, phases=0, ic=None):
+    print(f"[bridge] {case.name} steps={nsteps} precision={precision} phases={phases}", flush=True)
+    hdr, steps = read_dump(run_harness(case, nsteps=nsteps, every=every, lists=False,
+                                       ic_coord=ic, workdir=tmp_path, exe=HARNESS_HIP,
+                                       timeout=120,
+                                       extra_env={"SPH_HIP_PRECISION": str(precision),
+                                                  "SPH_HIP_PHASES": str(phases)}))
+    return hdr, {r["it"]: r for r in steps}
+
+
+@pytest.mark.parametrize("phases", [0, 1])
+def test_bridge_fp64_equals_reference_tiny(phases, tmp_path):
+    g = np.load(GOLD / "ref_tiny.npz")
+    _, st = _run(_tiny(), 5, 5, tmp_path, phases=phases)
+    assert st[0]["dt"] == g["s0_dt"]
+    for f in FIELDS:
+        assert np.array_equal(st[0][f], g["s0_" + f]), ("s0", f)
+        assert np.array_equal(st[5][f], g["s5_" + f]), ("s5", f)
+    assert st[5]["dt"] == g["s5_dt"] and st[5]["time"] == g["s5_time"]
+
+
+def test_bridge_fp64_equals_reference_small_jitter(tmp_path):
+    g = np.load(GOLD / "ref_small_jitter.npz")
+    _, st = _run(_small(), 10, 10, tmp_path, ic=g["ic_coord"])
+    for f in FIELDS:
+        assert np.array_equal(st[10][f], g["s10_" + f]), f
+    assert st[10]["dt"] == g["s10_dt"]
+
+
+def test_bridge_fp32_mixed_within_tolerance(tmp_path):
+    g = np.load(GOLD / "ref_small_jitter.npz")
+    _, st = _run(_small(), 10, 10, tmp_path, precision=1, ic=g["ic_coord"])
+    for f in FIELDS:
+        ref = g["s10_" + f]
+        err = np.abs(st[10][f] - ref).max() / max(np.abs(ref).max(), 1e-300)
+        assert err < FP32_TOL[f], (f, err)

@@ -1,5 +1,5 @@
 """Device hot path vs the FP64 oracle (which is itself pinned bit for bit to the
-flang-built reference, tests/test_oracle_vs_reference.py).
+flang-built reference, tests/test_oracle_reference.py).
 
 Bars: neighbour lists identical (as ordered lists, hence also as sorted sets);
 plain FP64 fields identical to the oracle (max |diff| == 0 is asserted where
