@@ -15,6 +15,7 @@ is timed beside it.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -35,6 +36,49 @@ REC_BYTES = {  # pass: (S fp64, out fp64, S fp32, out fp32)
     "momentum": (84, 24, 44, 12),
     "velocity_smoothing": (68, 24, 36, 12),
 }
+# neighbour search: each cell stages the FP32 positions (float4, 16 B) of its 27
+# neighbour cells once; output 4 B per list entry (self included) + 4 B count
+NB_STAGE_BYTES = 16
+
+
+def alg_bytes(kernel, n, pairs_per_step, fp32=True):
+    """Algorithmic bytes of one launch of `kernel` on n particles."""
+    if kernel in REC_BYTES:
+        S, out = REC_BYTES[kernel][2:] if fp32 else REC_BYTES[kernel][:2]
+        return n * (27 * S + out)
+    if kernel == "neighbours":
+        return n * 27 * NB_STAGE_BYTES + 4 * (pairs_per_step + n) + 4 * n
+    return None
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_damb3d_576k.md"
+
+
+def source_hash():
+    """Hash of the device sources; a measured traffic figure is used only for
+    the sources it was measured on."""
+    h = hashlib.sha1()
+    for f in ("sph_device.h", "sph_kernels.hip", "sph_api.hip"):
+        h.update((ROOT / "sphera_amd" / "csrc" / f).read_bytes())
+    return h.hexdigest()[:12]
+
+
+def traffic_bytes(kernel, n, fp32):
+    """HBM bytes per launch of `kernel` measured by the two rocprofv3 counter
+    runs (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known size) recorded in
+    TRAFFIC_FILE for this workload and these sources, else None."""
+    prec = "fp32_mixed" if fp32 else "fp64"
+    try:
+        lines = TRAFFIC_FILE.read_text().splitlines()
+    except OSError:
+        return None
+    src = source_hash()
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if (len(cells) == 7 and cells[0] == kernel and cells[1] == prec and cells[2] == str(n)
+                and cells[6] == src):
+            return int(cells[5])
+    return None
 
 
 def log(*a):
@@ -73,24 +117,42 @@ def run_device(tb, coord, precision, steps, warmup, tune=None):
                 launch=launch)
 
 
-def cpu_baseline(tb, coord, steps):
+def cpu_baseline(tb, coord, max_steps, budget_s):
+    """The FP64 OpenMP restatement on the host cores, on a bounded sample: whole
+    time steps of the same case until `budget_s` seconds or `max_steps`."""
     from oracle.pyoracle import Oracle
-    threads = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     o = Oracle(tb, coord=coord)
     o.preloop()
     o.step(1)
-    # Σ over steps of the list pairs built in each step
+    # sum over steps of the list pairs built in each step
     pairs = 0
+    steps = 0
     t0 = time.perf_counter()
-    for _ in range(steps):
+    while steps < max_steps and time.perf_counter() - t0 < budget_s:
         o.step(1)
         pairs += o.pair_count()
+        steps += 1
     t1 = time.perf_counter()
     return dict(value=pairs / (t1 - t0), unit="pair-interactions/s", cores=threads,
                 kind="port",
                 sample=f"oracle FP64 OpenMP restatement, full {len(coord)}-particle case, "
                        f"{steps} whole time steps after 1 untimed ({(t1 - t0):.1f} s)")
+
+
+def copy_probe(mib):
+    """A plain device copy of known size (mib MiB read, mib MiB written), run
+    before the benchmark so that rocprofv3 counter runs of this command can
+    scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
 
 
 def main():
@@ -99,22 +161,35 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.005)
-    ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("SPH_CPU_STEPS", "5")))
+    ap.add_argument("--cpu-steps", type=int, default=int(os.environ.get("SPH_CPU_STEPS", "200")),
+                    help="at most this many CPU baseline steps")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="CPU baseline sample length (whole steps until this many seconds)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--wg", type=int, default=0,
+                    help="fixed workgroup size of the particle kernels (skips the autotuner)")
+    ap.add_argument("--only", choices=("fp32_mixed", "fp64"), default=None,
+                    help="time one precision only (profiling runs)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run a device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU decomposition "
               "(every run uses one MI355X); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     from sphera_amd import runtime as rt
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
     case, tb, coord = make_case(args)
     log(f"[bench] case n={len(coord)} grid={tuple(tb.grid.ncd)} steps={args.steps} warmup={args.warmup}")
     res = {}
     tunes = {}
     for name, prec in (("fp32_mixed", rt.FP32_MIXED), ("fp64", rt.FP64)):
-        tune = None
-        if not args.no_autotune:
+        if args.only and name != args.only:
+            continue
+        tune = dict(wg_size=args.wg) if args.wg else None
+        if not args.no_autotune and not args.wg:
             from sphera_amd.autotune import autotune
             tune, _ = autotune(tb, coord, prec, log=log)
             log(f"[bench] {name}: autotuned launch {tune}")
@@ -124,24 +199,37 @@ def main():
         r = res[name]
         log(f"[bench] {name}: {r['seconds'] / args.steps * 1e3:.3f} ms/step, "
             f"{r['pairs'] / r['seconds'] / 1e9:.3f} G pairs/s")
-    main_r = res["fp32_mixed"]
+    main_name = args.only or "fp32_mixed"
+    main_r = res[main_name]
+    fp32 = main_name == "fp32_mixed"
     value = main_r["pairs"] / main_r["seconds"]
     n = main_r["n"]
-    # dominant kernel of the FP32-mixed step and its roofline
-    pair_k = {k: v for k, v in main_r["timing"].items() if k in REC_BYTES}
-    dom = max(pair_k, key=lambda k: pair_k[k][0])
-    ms_tot, calls = pair_k[dom]
-    avg_s = ms_tot / calls / 1e3
-    S, out = REC_BYTES[dom][2], REC_BYTES[dom][3]
-    bytes_alg = n * (27 * S + out)
-    achieved = bytes_alg / avg_s / 1e9
+    pairs_per_step = main_r["pairs"] / args.steps
+    # dominant kernel of the step (by time, among the kernels with a byte
+    # model) and its roofline; every such kernel is listed under "kernels"
+    per_k = {}
+    for k, (ms_tot, calls) in main_r["timing"].items():
+        b = alg_bytes(k, n, pairs_per_step, fp32)
+        if b is None or calls == 0:
+            continue
+        avg_s = ms_tot / calls / 1e3
+        per_k[k] = dict(avg_launch_ms=avg_s * 1e3, bytes_alg_per_launch=int(b),
+                        achieved_gbs=b / avg_s / 1e9, frac=b / avg_s / 1e9 / HBM_PEAK_GBS)
+    dom = max(per_k, key=lambda k: per_k[k]["avg_launch_ms"])
     kern = {k: round(v[0] / v[1], 4) for k, v in main_r["timing"].items()}
-    kern64 = {k: round(v[0] / v[1], 4) for k, v in res["fp64"]["timing"].items()}
     pair_time = sum(v[0] for k, v in main_r["timing"].items() if k in REC_BYTES) / 1e3
+    fp64 = None
+    if "fp64" in res and main_name != "fp64":
+        r64 = res["fp64"]
+        fp64 = {"value": r64["pairs"] / r64["seconds"],
+                "ms_per_step": r64["seconds"] / args.steps * 1e3,
+                "kernel_ms": {k: round(v[0] / v[1], 4) for k, v in r64["timing"].items()},
+                "launch": r64["launch"]}
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        log(f"[bench] cpu baseline ({args.cpu_steps} steps)")
-        cpu = cpu_baseline(tb, coord, args.cpu_steps)
+        log(f"[bench] cpu baseline (up to {args.cpu_steps} steps / {args.cpu_seconds} s)")
+        cpu = cpu_baseline(tb, coord, args.cpu_steps, args.cpu_seconds)
+        log(f"[bench] cpu baseline: {cpu['value'] / 1e6:.2f} M pairs/s ({cpu['sample']})")
     out_line = {
         "metric": "pair-interactions/s mixed vs FP64 (3D dam break); p50/p99 ensemble latency",
         "value": value,
@@ -153,22 +241,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-mixed (fp64 positions/integration)",
+        "dtype": "fp32-mixed (fp64 positions/integration)" if fp32 else "f64",
         "data": "synthetic: jittered lattice dam break, seed 20260101",
         "config": {"workload": "3D dam break, tank 1.6x0.6x0.6 m, column 0.4x0.6x0.3 m, "
                                f"dx={case.dx} m, h/dx=1.3, SA-SPH walls", "particles": n,
                    "grid_cells": int(tb.grid.nmax)},
-        "fp64": {"value": res["fp64"]["pairs"] / res["fp64"]["seconds"],
-                 "ms_per_step": res["fp64"]["seconds"] / args.steps * 1e3,
-                 "kernel_ms": kern64, "launch": res["fp64"]["launch"]},
+        "fp64": fp64,
         "launch": main_r["launch"],
-        "mixed_over_fp64": (value / (res["fp64"]["pairs"] / res["fp64"]["seconds"])),
+        "mixed_over_fp64": value / fp64["value"] if fp64 else None,
         "pair_kernels_only_value": main_r["pairs"] / pair_time if pair_time > 0 else None,
         "kernel_ms": kern,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_alg_per_launch": bytes_alg,
-                     "avg_launch_ms": avg_s * 1e3},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": per_k[dom]["achieved_gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": per_k[dom]["frac"],
+                     "traffic": traffic_bytes(dom, n, fp32),
+                     "bytes_alg_per_launch": per_k[dom]["bytes_alg_per_launch"],
+                     "avg_launch_ms": per_k[dom]["avg_launch_ms"]},
+        "kernels": per_k,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out_line), flush=True)

@@ -48,7 +48,7 @@ typedef struct sph_params {
     int32_t semi;        /* 1: SA-SPH frontiers active (Domain%tipo=="semi") */
     /* launch tunable of the particle kernels (0 = default; the autotuner in
      * sphera_amd/autotune.py picks it per case) */
-    int32_t wg_size;     /* threads per workgroup: 64, 128, 192 or 256 */
+    int32_t wg_size;     /* threads per workgroup: 64 (default), 128, 192 or 256 */
     int32_t reserved[4];
 } sph_params;
 

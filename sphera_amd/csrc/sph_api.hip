@@ -62,6 +62,8 @@ struct sph_ctx {
     int *cell = nullptr, *key = nullptr, *slot = nullptr, *cnt = nullptr, *start = nullptr,
         *perm = nullptr, *pid = nullptr, *bsum = nullptr;
     int *work = nullptr, *nwork = nullptr;  // (particle, face) pairs of the boundary table
+    int* rowblk = nullptr;                  // first neighbour-search block of each cell row
+    int nb_grid = 8;                        // persistent workgroups of the neighbour search
     int wg = 256, grid = 1, bt_grid = 1024;  // launch shape of the particle kernels
     uint32_t* nbr = nullptr;
     int* nnb = nullptr;
@@ -177,8 +179,9 @@ int run_sort(sph_ctx* c) {
 
 int run_neighbours(sph_ctx* c) {
     KTime t(c, "neighbours");
-    k_neighbours<<<c->grid, c->wg, 0, c->stream>>>(c->P, c->xm[0], c->xmf, c->cell, c->start, c->nbr,
-                                                  c->nnb, c->iscal, c->pairs);
+    k_nb_rowblocks<<<1, 1024, 0, c->stream>>>(c->P, c->start, c->rowblk);
+    k_neighbours<<<c->nb_grid, NB_WG, 0, c->stream>>>(c->P, c->xm[0], c->xmf, c->cell, c->start,
+                                                     c->rowblk, c->nbr, c->nnb, c->iscal, c->pairs);
     return 0;
 }
 
@@ -295,7 +298,7 @@ sph_ctx::~sph_ctx() {
     F(xmf);
     for (auto p : velf) F(p);
     for (auto p : rpf) F(p);
-    F(dden); F(vpres); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
+    F(dden); F(vpres); F(rowblk); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     for (auto e : ev) hipEventDestroy(e);
@@ -325,6 +328,7 @@ int sph_ctx::alloc() {
     HIPC(dalloc(&cnt, m));
     HIPC(dalloc(&start, m + 1));
     HIPC(dalloc(&bsum, (m + 1023) / 1024 + 1));
+    HIPC(dalloc(&rowblk, (size_t)P.ncd[1] * P.ncd[2] + 1));
     HIPC(dalloc(&nbr, N * (size_t)P.nmaxpartj));
     HIPC(dalloc(&nnb, N));
     HIPC(dalloc(&bn, N));
@@ -422,13 +426,15 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
         int dev = 0, ncu = 256;
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        c->wg = p->wg_size > 0 ? p->wg_size : 256;
+        c->wg = p->wg_size > 0 ? p->wg_size : 64;
         if (c->wg < 64 || c->wg > 256 || c->wg % 64) {
             delete c;
             return fail(2, "wg_size must be 64, 128, 192 or 256");
         }
         c->grid = (P.cap + c->wg - 1) / c->wg;
         c->bt_grid = 4 * ncu;
+        // three neighbour-search workgroups fit a CU (LDS), a multiple of 8 XCDs
+        c->nb_grid = (3 * ncu + 7) / 8 * 8;
     }
     if ((rc = c->alloc())) {
         delete c;

@@ -412,85 +412,258 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
 // |x - corner| <= dcd, so the FP32 separation is within ~5e-7 dcd per axis and
 // the FP32 r^2 within ~1.5e-6 relative of (2h)^2 near the cut-off.)
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_neighbours(DP P, const double4* __restrict__ xm,
-                                                    const float4* __restrict__ xmf,
-                                                    const int* __restrict__ cell,
-                                                    const int* __restrict__ start,
-                                                    uint32_t* __restrict__ nbr, int* __restrict__ nnb,
-                                                    int* iscal, unsigned long long* pairs) {
-    stencil_offsets(P);
-    const int p = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int n = iscal[IS_N];
-    const float sq = (float)P.square_doubleh;
-    const float lo = sq * (1.0f - 4.0e-5f), hi = sq * (1.0f + 4.0e-5f);
-    int cnt = 1;  // lanes past the end contribute 0 pairs
-    if (p < n) {
-        const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
-        const int mj = P.nmaxpartj;
-        const size_t N = P.cap;
-        int ic, jc, kc;
-        d_cell_indices(P, cell[p], ic, jc, kc);
-        const float4 xi = xmf[p];
-        double4 xi64;
-        bool have64 = false;
-        cnt = 0;
-        bool ov = false;
-        for (int dy = -1; dy <= 1 && !ov; ++dy) {
-            const int jj = jc + dy;
-            if (jj < 1 || jj > nj) continue;
-            for (int dx = -1; dx <= 1 && !ov; ++dx) {
-                const int ii = ic + dx;
-                if (ii < 1 || ii > ni) continue;
-                for (int dz = -1; dz <= 1 && !ov; ++dz) {
-                    const int kk = kc + dz;
-                    if (kk < 1 || kk > nk) continue;
-                    const int cn = ((kk - 1) * nj + (jj - 1)) * ni + ii;
-                    const int b = start[cn - 1], e = start[cn];
-                    const int code = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-                    const float4 o = g_soff[code];
-                    const uint32_t ctag = (uint32_t)code << NB_SHIFT;
-                    for (int m0 = b; m0 < e && !ov; m0 += 4) {
-                        float4 xq[4];
+// One candidate of the neighbour test; returns false once the list is full.
+// xj: FP32 cell-relative position of the candidate, m: its sorted index.
+__device__ __forceinline__ bool nb_candidate(const DP& P, int p, const double4* __restrict__ xm,
+                                             float4 xi, float4 xj, float4 o, int m, uint32_t ctag,
+                                             float lo, float hi, double4& xi64, bool& have64,
+                                             uint32_t* __restrict__ nbr, size_t N, int mj, int& cnt) {
+    const float r0 = (xi.x - xj.x) - o.x;
+    const float r1 = (xi.y - xj.y) - o.y;
+    const float r2 = (xi.z - xj.z) - o.z;
+    const float r2f = r0 * r0 + r1 * r1 + r2 * r2;
+    if (r2f > hi) return true;
+    if (r2f >= lo) {  // guard band: the reference's FP64 test
+        if (!have64) {
+            xi64 = xm[p];
+            have64 = true;
+        }
+        const double4 xj64 = xm[m];
+        const double d0 = xi64.x - xj64.x;
+        const double d1 = xi64.y - xj64.y;
+        const double d2 = xi64.z - xj64.z;
+        const double rij = d0 * d0 + d1 * d1 + d2 * d2;
+        if (rij > P.square_doubleh) return true;
+    }
+    if (cnt >= mj) return false;
+    nbr[(size_t)cnt * N + p] = (uint32_t)m | ctag;
+    ++cnt;
+    return true;
+}
+
+// List of particle p from global memory (candidates gathered through L1/L2).
+__device__ int nb_particle_global(const DP& P, int p, const double4* __restrict__ xm,
+                                  const float4* __restrict__ xmf, const int* __restrict__ cell,
+                                  const int* __restrict__ start, uint32_t* __restrict__ nbr,
+                                  float lo, float hi, bool& ov) {
+    const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
+    const int mj = P.nmaxpartj;
+    const size_t N = P.cap;
+    int ic, jc, kc;
+    d_cell_indices(P, cell[p], ic, jc, kc);
+    const float4 xi = xmf[p];
+    double4 xi64;
+    bool have64 = false;
+    int cnt = 0;
+    ov = false;
+    for (int dy = -1; dy <= 1 && !ov; ++dy) {
+        const int jj = jc + dy;
+        if (jj < 1 || jj > nj) continue;
+        for (int dx = -1; dx <= 1 && !ov; ++dx) {
+            const int ii = ic + dx;
+            if (ii < 1 || ii > ni) continue;
+            for (int dz = -1; dz <= 1 && !ov; ++dz) {
+                const int kk = kc + dz;
+                if (kk < 1 || kk > nk) continue;
+                const int cn = ((kk - 1) * nj + (jj - 1)) * ni + ii;
+                const int b = start[cn - 1], e = start[cn];
+                const int code = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                const float4 o = g_soff[code];
+                const uint32_t ctag = (uint32_t)code << NB_SHIFT;
+                for (int m0 = b; m0 < e && !ov; m0 += 4) {
+                    float4 xq[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            if (m0 + u < e) xq[u] = xmf[m0 + u];
+                    for (int u = 0; u < 4; ++u)
+                        if (m0 + u < e) xq[u] = xmf[m0 + u];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int m = m0 + u;
-                            if (m >= e) break;
-                            const float r0 = (xi.x - xq[u].x) - o.x;
-                            const float r1 = (xi.y - xq[u].y) - o.y;
-                            const float r2 = (xi.z - xq[u].z) - o.z;
-                            const float r2f = r0 * r0 + r1 * r1 + r2 * r2;
-                            if (r2f > hi) continue;
-                            if (r2f >= lo) {  // guard band: the reference's FP64 test
-                                if (!have64) {
-                                    xi64 = xm[p];
-                                    have64 = true;
-                                }
-                                const double4 xj64 = xm[m];
-                                const double d0 = xi64.x - xj64.x;
-                                const double d1 = xi64.y - xj64.y;
-                                const double d2 = xi64.z - xj64.z;
-                                const double rij = d0 * d0 + d1 * d1 + d2 * d2;
-                                if (rij > P.square_doubleh) continue;
-                            }
-                            if (cnt >= mj) {
-                                ov = true;
-                                break;
-                            }
-                            nbr[(size_t)cnt * N + p] = (uint32_t)m | ctag;
-                            ++cnt;
+                    for (int u = 0; u < 4; ++u) {
+                        if (m0 + u >= e) break;
+                        if (!nb_candidate(P, p, xm, xi, xq[u], o, m0 + u, ctag, lo, hi, xi64, have64,
+                                          nbr, N, mj, cnt)) {
+                            ov = true;
+                            break;
                         }
                     }
                 }
             }
         }
-        nnb[p] = cnt;
-        if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
+    }
+    return cnt;
+}
+
+// Tiled form.  The particles of one row of cells (fixed j,k; cells i are
+// contiguous in the sorted order, hence in memory) are cut into blocks of at
+// most NB_WG.  For a block over cells ia..ib the FP32 positions of the 9
+// neighbouring row segments (rows j+-1, k+-1 over cells ia-1..ib+1) are staged
+// in LDS once, and every lane walks its 27 stencil cells in the reference's
+// order out of LDS.  A block over more than NB_MAXC cells or with more than
+// NB_TILE candidates takes the global path above.  Persistent workgroups take
+// the blocks of one contiguous range per XCD.
+constexpr int NB_WG = 256;
+constexpr int NB_MAXC = 48;
+constexpr int NB_TILE = 3072;
+
+// rowblk[r] = first block of row r (exclusive scan of ceil(count_r / NB_WG)),
+// rowblk[R] = number of blocks.  One workgroup.
+__global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P, const int* __restrict__ start,
+                                                       int* __restrict__ rowblk) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int ni = P.ncd[0];
+    const int R = P.ncd[1] * P.ncd[2];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < R; base += 1024) {
+        const int r = base + threadIdx.x;
+        const int v = r < R ? (start[(r + 1) * ni] - start[r * ni] + NB_WG - 1) / NB_WG : 0;
+        const int incl = wave_incl_scan(v);
+        const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < wid; ++w) woff += wsum[w];
+        const int c0 = carry;
+        if (r < R) rowblk[r] = c0 + woff + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c0 + woff + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rowblk[R] = carry;
+}
+
+__global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __restrict__ xm,
+                                                      const float4* __restrict__ xmf,
+                                                      const int* __restrict__ cell,
+                                                      const int* __restrict__ start,
+                                                      const int* __restrict__ rowblk,
+                                                      uint32_t* __restrict__ nbr, int* __restrict__ nnb,
+                                                      int* iscal, unsigned long long* pairs) {
+    __shared__ float4 tile[NB_TILE];
+    __shared__ int bnd[9][NB_MAXC + 4];  // LDS offset of the start of cell ia-1+u of row t
+    __shared__ int seg_g[9];             // sorted index of a segment's first particle
+    __shared__ int seg_l[9];             // its LDS offset
+    __shared__ int seg_n[9];
+    __shared__ int s_row, s_ia, s_ib, s_p0, s_p1, s_global;
+    stencil_offsets(P);
+    const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
+    const int R = nj * nk;
+    const int nblocks = rowblk[R];
+    const int per_xcd = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    const int xcd = blockIdx.x & 7;
+    const int b_lo = (int)((long long)nblocks * xcd / 8), b_hi = (int)((long long)nblocks * (xcd + 1) / 8);
+    const float sq = (float)P.square_doubleh;
+    const float lo = sq * (1.0f - 4.0e-5f), hi = sq * (1.0f + 4.0e-5f);
+    const int mj = P.nmaxpartj;
+    const size_t N = P.cap;
+    unsigned long long mypairs = 0;
+    for (int blk = b_lo + (blockIdx.x >> 3); blk < b_hi; blk += per_xcd) {
+        if (threadIdx.x == 0) {
+            // row of the block: last r with rowblk[r] <= blk
+            int a = 0, b = R;  // rowblk[a] <= blk < rowblk[b]
+            while (b - a > 1) {
+                const int m = (a + b) >> 1;
+                if (rowblk[m] <= blk) a = m;
+                else b = m;
+            }
+            const int r = a;
+            const int p0 = start[r * ni] + (blk - rowblk[r]) * NB_WG;
+            const int p1 = min(p0 + NB_WG, start[(r + 1) * ni]);
+            const int ia = cell[p0] - r * ni, ib = cell[p1 - 1] - r * ni;
+            s_row = r;
+            s_ia = ia;
+            s_ib = ib;
+            s_p0 = p0;
+            s_p1 = p1;
+            s_global = ib - ia + 1 > NB_MAXC;
+        }
+        __syncthreads();
+        const int r = s_row, ia = s_ia, ib = s_ib, p0 = s_p0, p1 = s_p1;
+        const int jc = r % nj + 1, kc = r / nj + 1;
+        const int ca = max(ia - 1, 1), cb = min(ib + 1, ni);
+        if (!s_global) {
+            if (threadIdx.x < 9) {
+                const int t = threadIdx.x;
+                const int jj = jc + t / 3 - 1, kk = kc + t % 3 - 1;
+                int g0 = 0, g1 = 0;
+                if (jj >= 1 && jj <= nj && kk >= 1 && kk <= nk) {
+                    const int rb = ((kk - 1) * nj + (jj - 1)) * ni;
+                    g0 = start[rb + ca - 1];
+                    g1 = start[rb + cb];
+                }
+                seg_g[t] = g0;
+                seg_n[t] = g1 - g0;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int off = 0;
+                for (int t = 0; t < 9; ++t) {
+                    seg_l[t] = off;
+                    off += seg_n[t];
+                }
+                s_global = off > NB_TILE;
+            }
+            __syncthreads();
+        }
+        const int p = p0 + threadIdx.x;
+        if (!s_global) {
+            // cell boundaries inside the tile: u = 0 .. ib-ia+3 is the start of cell
+            // ia-1+u (cells ia-1 .. ib+1, and the end of ib+1)
+            const int nu = ib - ia + 4;
+            for (int x = threadIdx.x; x < 9 * nu; x += NB_WG) {
+                const int t = x / nu, u = x % nu;
+                const int jj = jc + t / 3 - 1, kk = kc + t % 3 - 1;
+                const int ii = ia - 1 + u;
+                int v;
+                if (jj < 1 || jj > nj || kk < 1 || kk > nk || ii < ca) v = seg_l[t];
+                else if (ii > cb) v = seg_l[t] + seg_n[t];
+                else v = seg_l[t] + start[((kk - 1) * nj + (jj - 1)) * ni + ii - 1] - seg_g[t];
+                bnd[t][u] = v;
+            }
+            // stage the candidates' FP32 positions
+            for (int t = 0; t < 9; ++t) {
+                const int g0 = seg_g[t], l0 = seg_l[t], m = seg_n[t];
+                for (int e = threadIdx.x; e < m; e += NB_WG) tile[l0 + e] = xmf[g0 + e];
+            }
+            __syncthreads();
+            if (p < p1) {
+                const int ucell = cell[p] - r * ni - ia + 1;  // u index of the own cell
+                const float4 xi = xmf[p];
+                double4 xi64;
+                bool have64 = false;
+                bool ov = false;
+                int cnt = 0;
+                for (int dy = 0; dy < 3 && !ov; ++dy)
+                    for (int dx = 0; dx < 3 && !ov; ++dx)
+                        for (int dz = 0; dz < 3 && !ov; ++dz) {
+                            const int t = 3 * dy + dz;
+                            const int b = bnd[t][ucell + dx - 1], e = bnd[t][ucell + dx];
+                            const int code = dx + 3 * dy + 9 * dz;
+                            const float4 o = g_soff[code];
+                            const uint32_t ctag = (uint32_t)code << NB_SHIFT;
+                            const int gl = seg_g[t] - seg_l[t];
+                            for (int x = b; x < e; ++x) {
+                                if (!nb_candidate(P, p, xm, xi, tile[x], o, gl + x, ctag, lo, hi, xi64,
+                                                  have64, nbr, N, mj, cnt)) {
+                                    ov = true;
+                                    break;
+                                }
+                            }
+                        }
+                nnb[p] = cnt;
+                mypairs += (unsigned long long)(cnt - 1);
+                if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
+            }
+        } else if (p < p1) {
+            bool ov;
+            const int cnt = nb_particle_global(P, p, xm, xmf, cell, start, nbr, lo, hi, ov);
+            nnb[p] = cnt;
+            mypairs += (unsigned long long)(cnt - 1);
+            if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
+        }
+        __syncthreads();  // the tile and the block descriptors are reused
     }
     // pair-interaction counter: sum_i (nPartIntorno(i) - 1), self excluded
-    unsigned long long v = (unsigned long long)(cnt - 1);
+    unsigned long long v = mypairs;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(pairs, v);
 }

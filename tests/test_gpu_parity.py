@@ -157,3 +157,36 @@ def test_empty_state(tiny_case):
     d.preloop()
     d.step(2)
     assert d.n == 0 and d.scalars()["dt"] == 0.0
+
+
+@pytest.mark.parametrize("kind", ["uniform", "sparse_rows", "dense_cluster"])
+def test_lists_match_oracle_on_clouds(kind):
+    """Neighbour lists of random clouds: the LDS-tiled search (uniform), the
+    global-gather path for blocks over many cells (sparse_rows) and for blocks
+    with more candidates than the LDS tile holds, with lists truncated at
+    NMAXPARTJ (dense_cluster)."""
+    from sphera_amd.case import DamBreak3D
+    rng = np.random.default_rng({"uniform": 1, "sparse_rows": 2, "dense_cluster": 3}[kind])
+    if kind == "uniform":
+        tb = build(DamBreak3D(tank=(0.3, 0.2, 0.2), column=(0.3, 0.2, 0.2), dx=0.01, name="cloud"))
+        coord = rng.uniform(0.002, np.array([0.298, 0.198, 0.198]), (6000, 3))
+    elif kind == "sparse_rows":
+        tb = build(DamBreak3D(tank=(2.0, 0.1, 0.1), column=(2.0, 0.1, 0.1), dx=0.01, name="cloud"))
+        coord = rng.uniform(0.002, np.array([1.998, 0.098, 0.098]), (600, 3))
+    else:
+        tb = build(DamBreak3D(tank=(0.2, 0.2, 0.2), column=(0.2, 0.2, 0.2), dx=0.01, name="cloud"))
+        coord = np.concatenate([rng.uniform(0.09, 0.11, (3000, 3)),
+                                rng.uniform(0.002, 0.198, (1000, 3))])
+    n = len(coord)
+    print(f"[gpu] cloud={kind} n={n}", flush=True)
+    kw = dict(vel=np.zeros((n, 3)), dens=np.full(n, 1000.0), pres=np.zeros(n), mass=np.full(n, 1e-3))
+    from oracle.pyoracle import Oracle
+    o = Oracle(tb, coord=coord, **kw)
+    o.preloop()
+    rt = _rt()
+    for prec in (rt.FP64, rt.FP32_MIXED):
+        d = rt.SphContext(tb, precision=prec, capacity=n)
+        d.upload(coord=coord, **kw)
+        d.preloop()
+        _lists_equal(d, o)
+        d.close()
