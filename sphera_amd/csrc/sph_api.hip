@@ -329,7 +329,7 @@ int sph_ctx::alloc() {
     HIPC(dalloc(&start, m + 1));
     HIPC(dalloc(&bsum, (m + 1023) / 1024 + 1));
     HIPC(dalloc(&rowblk, (size_t)P.ncd[1] * P.ncd[2] + 1));
-    HIPC(dalloc(&nbr, N * (size_t)P.nmaxpartj));
+    HIPC(dalloc(&nbr, N * (size_t)nb_stride(P.nmaxpartj)));
     HIPC(dalloc(&nnb, N));
     HIPC(dalloc(&bn, N));
     HIPC(dalloc(&bt, P.semi ? N * SPH_MB : 1));
@@ -615,8 +615,9 @@ int sph_get_neighbour_lists(sph_ctx* c, int32_t* count, int32_t* ids, int32_t st
     HIPC(hipMemcpy(hv.data(), c->vel[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
     int kmax = 0;
     for (int i = 0; i < n; ++i) kmax = std::max(kmax, nn[i]);
-    std::vector<uint32_t> nb((size_t)kmax * N);
-    if (kmax) HIPC(hipMemcpy(nb.data(), c->nbr, sizeof(uint32_t) * kmax * N, hipMemcpyDeviceToHost));
+    const size_t ngroups = (size_t)(kmax + NB_G - 1) / NB_G;
+    std::vector<uint32_t> nb(ngroups * N * NB_G);
+    if (kmax) HIPC(hipMemcpy(nb.data(), c->nbr, sizeof(uint32_t) * nb.size(), hipMemcpyDeviceToHost));
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int a, int b) { return hv[a].w < hv[b].w; });
@@ -625,7 +626,7 @@ int sph_get_neighbour_lists(sph_ctx* c, int32_t* count, int32_t* ids, int32_t st
         count[r] = nn[s];
         if (ids)
             for (int k = 0; k < nn[s]; ++k)
-                ids[(size_t)r * stride + k] = (int32_t)hv[nb[(size_t)k * N + s] & NB_MASK].w;
+                ids[(size_t)r * stride + k] = (int32_t)hv[nb[nb_index(N, s, k)] & NB_MASK].w;
     }
     return 0;
 }

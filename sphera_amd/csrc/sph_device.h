@@ -8,6 +8,19 @@ namespace sph {
 
 constexpr int NB_SHIFT = 27;
 constexpr uint32_t NB_MASK = (1u << NB_SHIFT) - 1u;
+// neighbour-list layout: entry k of particle p at ((k/NB_G)*N + p)*NB_G + k%NB_G,
+// i.e. groups of NB_G consecutive entries per particle, the groups of one
+// rank k/NB_G contiguous over p (32-byte vector stores and loads per lane)
+#ifndef SPH_NB_GROUP
+#define SPH_NB_GROUP 8
+#endif
+constexpr int NB_G = SPH_NB_GROUP;
+__host__ __device__ inline size_t nb_index(size_t N, int p, int k) {
+    const unsigned u = (unsigned)k;
+    return ((size_t)(u / NB_G) * N + (size_t)p) * NB_G + (size_t)(u % NB_G);
+}
+// list capacity per particle: NMAXPARTJ rounded up to whole groups
+__host__ __device__ inline int nb_stride(int nmaxpartj) { return (nmaxpartj + NB_G - 1) / NB_G * NB_G; }
 
 // indices into the device scalar blocks
 enum { SC_DT = 0, SC_DTPREV, SC_DTAVG, SC_TIME, SC_DTVEL, SC_TV1, SC_TP1, SC_COUNT };

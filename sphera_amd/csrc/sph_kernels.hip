@@ -9,8 +9,9 @@
 //   acc  double4, dden, vpres             outputs of the passes
 //   FP32-mixed mirrors: xmf float4 (cell-relative x,y,z, mass), velf float4,
 //   rpf float2, written by the same kernels that write the FP64 fields.
-//   Neighbour list: nbr[k*N + i] (column-major, coalesced over i), the sorted
-//   index of the k-th neighbour in bits 0..26 and its stencil cell code
+//   Neighbour list: entry k of particle i at nb_index(N, i, k) (groups of 8
+//   entries per particle, a group rank contiguous over i), the sorted index of
+//   the k-th neighbour in bits 0..26 and its stencil cell code
 //   (dx+1)+3(dy+1)+9(dz+1) in bits 27..31; count in nnb[i].
 //
 // FP64 kernels are compiled with -ffp-contract=off and evaluate every
@@ -412,33 +413,56 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
 // |x - corner| <= dcd, so the FP32 separation is within ~5e-7 dcd per axis and
 // the FP32 r^2 within ~1.5e-6 relative of (2h)^2 near the cut-off.)
 // ----------------------------------------------------------------------------
-// One candidate of the neighbour test; returns false once the list is full.
-// xj: FP32 cell-relative position of the candidate, m: its sorted index.
-__device__ __forceinline__ bool nb_candidate(const DP& P, int p, const double4* __restrict__ xm,
-                                             float4 xi, float4 xj, float4 o, int m, uint32_t ctag,
-                                             float lo, float hi, double4& xi64, bool& have64,
-                                             uint32_t* __restrict__ nbr, size_t N, int mj, int& cnt) {
-    const float r0 = (xi.x - xj.x) - o.x;
-    const float r1 = (xi.y - xj.y) - o.y;
-    const float r2 = (xi.z - xj.z) - o.z;
-    const float r2f = r0 * r0 + r1 * r1 + r2 * r2;
-    if (r2f > hi) return true;
-    if (r2f >= lo) {  // guard band: the reference's FP64 test
-        if (!have64) {
-            xi64 = xm[p];
-            have64 = true;
+// The neighbour test of one candidate and the list append: FP32 on
+// cell-relative positions, and the reference's FP64 expression on the FP64
+// positions inside the guard band.  xo: FP32 position of particle p relative
+// to the candidate's cell corner (its own cell-relative position minus the
+// stencil offset), xj: the candidate's cell-relative position, m: its sorted
+// index.  The FP32 r^2 (fused multiply-adds) is within ~3e-7 relative of the
+// exact one, far inside the 4e-5 band.  Every accepted candidate is counted;
+// the first NMAXPARTJ are stored (the reference keeps the first NMAXPARTJ of
+// its list, CalcVarLength.f90:200-207), so the loops have no early exit.
+__device__ __forceinline__ void nb_test(const DP& P, const double4* __restrict__ xm, double4 xi64,
+                                        float4 xo, float4 xj, int m, uint32_t ctag, float lo, float hi,
+                                        int& cnt, uint32_t* __restrict__ nbr, size_t N, int p, int mj) {
+    const float r0 = xo.x - xj.x;
+    const float r1 = xo.y - xj.y;
+    const float r2 = xo.z - xj.z;
+    const float r2f = __builtin_fmaf(r0, r0, __builtin_fmaf(r1, r1, r2 * r2));
+    if (r2f <= hi) {
+        bool acc = true;
+        if (r2f >= lo) {
+            const double4 xj64 = xm[m];
+            const double d0 = xi64.x - xj64.x;
+            const double d1 = xi64.y - xj64.y;
+            const double d2 = xi64.z - xj64.z;
+            const double rij = d0 * d0 + d1 * d1 + d2 * d2;
+            acc = !(rij > P.square_doubleh);
         }
-        const double4 xj64 = xm[m];
-        const double d0 = xi64.x - xj64.x;
-        const double d1 = xi64.y - xj64.y;
-        const double d2 = xi64.z - xj64.z;
-        const double rij = d0 * d0 + d1 * d1 + d2 * d2;
-        if (rij > P.square_doubleh) return true;
+        if (acc) {
+            if (cnt < mj) nbr[nb_index(N, p, cnt)] = (uint32_t)m | ctag;
+            ++cnt;
+        }
     }
-    if (cnt >= mj) return false;
-    nbr[(size_t)cnt * N + p] = (uint32_t)m | ctag;
-    ++cnt;
-    return true;
+}
+
+// Candidates [b,e) of one stencil cell, four loads ahead: load(x) gives the
+// FP32 position of slot x, gl + x its sorted index.
+template <class Load>
+__device__ __forceinline__ void nb_cell(const DP& P, int p, const double4* __restrict__ xm, double4 xi64,
+                                        float4 xi, float4 o, uint32_t ctag, int b, int e, int gl,
+                                        Load&& load, float lo, float hi, int& cnt,
+                                        uint32_t* __restrict__ nbr, size_t N, int mj) {
+    const float4 xo = make_float4(xi.x - o.x, xi.y - o.y, xi.z - o.z, 0.0f);
+    int x = b;
+    for (; x + 4 <= e; x += 4) {
+        const float4 q0 = load(x), q1 = load(x + 1), q2 = load(x + 2), q3 = load(x + 3);
+        nb_test(P, xm, xi64, xo, q0, gl + x, ctag, lo, hi, cnt, nbr, N, p, mj);
+        nb_test(P, xm, xi64, xo, q1, gl + x + 1, ctag, lo, hi, cnt, nbr, N, p, mj);
+        nb_test(P, xm, xi64, xo, q2, gl + x + 2, ctag, lo, hi, cnt, nbr, N, p, mj);
+        nb_test(P, xm, xi64, xo, q3, gl + x + 3, ctag, lo, hi, cnt, nbr, N, p, mj);
+    }
+    for (; x < e; ++x) nb_test(P, xm, xi64, xo, load(x), gl + x, ctag, lo, hi, cnt, nbr, N, p, mj);
 }
 
 // List of particle p from global memory (candidates gathered through L1/L2).
@@ -452,43 +476,27 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
     int ic, jc, kc;
     d_cell_indices(P, cell[p], ic, jc, kc);
     const float4 xi = xmf[p];
-    double4 xi64;
-    bool have64 = false;
+    const double4 xi64 = xm[p];
     int cnt = 0;
-    ov = false;
-    for (int dy = -1; dy <= 1 && !ov; ++dy) {
+    auto load = [&](int x) { return xmf[x]; };
+    for (int dy = -1; dy <= 1; ++dy) {
         const int jj = jc + dy;
         if (jj < 1 || jj > nj) continue;
-        for (int dx = -1; dx <= 1 && !ov; ++dx) {
+        for (int dx = -1; dx <= 1; ++dx) {
             const int ii = ic + dx;
             if (ii < 1 || ii > ni) continue;
-            for (int dz = -1; dz <= 1 && !ov; ++dz) {
+            for (int dz = -1; dz <= 1; ++dz) {
                 const int kk = kc + dz;
                 if (kk < 1 || kk > nk) continue;
                 const int cn = ((kk - 1) * nj + (jj - 1)) * ni + ii;
-                const int b = start[cn - 1], e = start[cn];
                 const int code = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-                const float4 o = g_soff[code];
-                const uint32_t ctag = (uint32_t)code << NB_SHIFT;
-                for (int m0 = b; m0 < e && !ov; m0 += 4) {
-                    float4 xq[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (m0 + u < e) xq[u] = xmf[m0 + u];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (m0 + u >= e) break;
-                        if (!nb_candidate(P, p, xm, xi, xq[u], o, m0 + u, ctag, lo, hi, xi64, have64,
-                                          nbr, N, mj, cnt)) {
-                            ov = true;
-                            break;
-                        }
-                    }
-                }
+                nb_cell(P, p, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT, start[cn - 1],
+                        start[cn], 0, load, lo, hi, cnt, nbr, N, mj);
             }
         }
     }
-    return cnt;
+    ov = cnt > mj;
+    return min(cnt, mj);
 }
 
 // Tiled form.  The particles of one row of cells (fixed j,k; cells i are
@@ -619,36 +627,47 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                 else v = seg_l[t] + start[((kk - 1) * nj + (jj - 1)) * ni + ii - 1] - seg_g[t];
                 bnd[t][u] = v;
             }
-            // stage the candidates' FP32 positions
-            for (int t = 0; t < 9; ++t) {
-                const int g0 = seg_g[t], l0 = seg_l[t], m = seg_n[t];
-                for (int e = threadIdx.x; e < m; e += NB_WG) tile[l0 + e] = xmf[g0 + e];
+            // stage the candidates' FP32 positions, four loads in flight per thread
+            {
+                const int total = seg_l[8] + seg_n[8];
+                for (int e0 = threadIdx.x; e0 < total; e0 += 4 * NB_WG) {
+                    float4 v[4];
+                    int dst[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int e = e0 + u * NB_WG;
+                        dst[u] = -1;
+                        if (e < total) {
+                            int t = 0;
+#pragma unroll
+                            for (int q = 1; q < 9; ++q) t += e >= seg_l[q];
+                            v[u] = xmf[seg_g[t] + (e - seg_l[t])];
+                            dst[u] = e;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (dst[u] >= 0) tile[dst[u]] = v[u];
+                }
             }
             __syncthreads();
             if (p < p1) {
                 const int ucell = cell[p] - r * ni - ia + 1;  // u index of the own cell
                 const float4 xi = xmf[p];
-                double4 xi64;
-                bool have64 = false;
-                bool ov = false;
-                int cnt = 0;
-                for (int dy = 0; dy < 3 && !ov; ++dy)
-                    for (int dx = 0; dx < 3 && !ov; ++dx)
-                        for (int dz = 0; dz < 3 && !ov; ++dz) {
+                const double4 xi64 = xm[p];
+                int c = 0;
+                auto load = [&](int x) { return tile[x]; };
+                for (int dy = 0; dy < 3; ++dy)
+                    for (int dx = 0; dx < 3; ++dx)
+                        for (int dz = 0; dz < 3; ++dz) {
                             const int t = 3 * dy + dz;
-                            const int b = bnd[t][ucell + dx - 1], e = bnd[t][ucell + dx];
                             const int code = dx + 3 * dy + 9 * dz;
-                            const float4 o = g_soff[code];
-                            const uint32_t ctag = (uint32_t)code << NB_SHIFT;
-                            const int gl = seg_g[t] - seg_l[t];
-                            for (int x = b; x < e; ++x) {
-                                if (!nb_candidate(P, p, xm, xi, tile[x], o, gl + x, ctag, lo, hi, xi64,
-                                                  have64, nbr, N, mj, cnt)) {
-                                    ov = true;
-                                    break;
-                                }
-                            }
+                            nb_cell(P, p, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT,
+                                    bnd[t][ucell + dx - 1], bnd[t][ucell + dx], seg_g[t] - seg_l[t], load,
+                                    lo, hi, c, nbr, N, mj);
                         }
+                const bool ov = c > mj;
+                const int cnt = min(c, mj);
                 nnb[p] = cnt;
                 mypairs += (unsigned long long)(cnt - 1);
                 if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
@@ -963,10 +982,25 @@ constexpr uint32_t NB_NONE = 0xFFFFFFFFu;
 template <int CH, class R, class Fetch, class Body>
 __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ nbr, size_t N, int i, int nn,
                                           Fetch&& fetch, Body&& body) {
+    static_assert(CH % 4 == 0, "list walks read whole 16-byte quarters of a group");
     for (int k0 = 0; k0 < nn; k0 += CH) {
         uint32_t ev[CH];
+        if constexpr (NB_G >= 4) {
 #pragma unroll
-        for (int u = 0; u < CH; ++u) ev[u] = (k0 + u < nn) ? nbr[(size_t)(k0 + u) * N + i] : NB_NONE;
+            for (int q = 0; q < CH / 4; ++q) {
+                const uint4 v = *reinterpret_cast<const uint4*>(nbr + nb_index(N, i, k0 + 4 * q));
+                ev[4 * q + 0] = v.x;
+                ev[4 * q + 1] = v.y;
+                ev[4 * q + 2] = v.z;
+                ev[4 * q + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < CH; ++u) ev[u] = nbr[nb_index(N, i, k0 + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+            if (k0 + u >= nn) ev[u] = NB_NONE;
         R rq[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u)

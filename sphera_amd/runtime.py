@@ -9,11 +9,14 @@ raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "_build" / "libsphera_hip.so"
+# the in-tree build; SPHERA_HIP_LIB names another build of the same sources
+LIB_PATH = Path(os.environ.get("SPHERA_HIP_LIB") or
+                Path(__file__).resolve().parent / "_build" / "libsphera_hip.so")
 
 FP64, FP32_MIXED = 0, 1
 PD = C.POINTER(C.c_double)
