@@ -171,6 +171,12 @@ def main():
                     help="fixed workgroup size of the particle kernels (skips the autotuner)")
     ap.add_argument("--only", choices=("fp32_mixed", "fp64"), default=None,
                     help="time one precision only (profiling runs)")
+    ap.add_argument("--ens-requests", type=int, default=64,
+                    help="ensemble: number of requests (0 skips the ensemble)")
+    ap.add_argument("--ens-steps", type=int, default=100, help="ensemble: time steps per request")
+    ap.add_argument("--ens-rate", type=float, default=200.0,
+                    help="ensemble: Poisson arrival rate, requests per second")
+    ap.add_argument("--ens-active", type=int, default=16, help="ensemble: concurrent requests")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -225,6 +231,23 @@ def main():
                 "ms_per_step": r64["seconds"] / args.steps * 1e3,
                 "kernel_ms": {k: round(v[0] / v[1], 4) for k, v in r64["timing"].items()},
                 "launch": r64["launch"]}
+    ens = None
+    if args.ens_requests > 0:
+        from sphera_amd import serve
+        reqs = serve.make_ensemble(args.ens_requests, args.ens_steps, args.ens_rate)
+        log(f"[bench] ensemble: {len(reqs)} requests x {args.ens_steps} steps, "
+            f"{min(r.n for r in reqs)}-{max(r.n for r in reqs)} particles, rate {args.ens_rate}/s")
+        prec = rt.FP32_MIXED if fp32 else rt.FP64
+        ens = serve.EnsembleServer(precision=prec, max_active=args.ens_active).run(reqs)
+        ens = {"requests": ens["requests"], "steps": args.ens_steps, "arrival_rate_per_s": args.ens_rate,
+               "max_active": args.ens_active,
+               "particles_min": min(ens["particles"]), "particles_max": max(ens["particles"]),
+               "p50_latency_s": ens["p50_latency_s"], "p99_latency_s": ens["p99_latency_s"],
+               "mean_latency_s": ens["mean_latency_s"], "p99_queue_s": ens["p99_queue_s"],
+               "makespan_s": ens["makespan_s"],
+               "pair_interactions_per_s": ens["pair_interactions_per_s"]}
+        log(f"[bench] ensemble: p50 {ens['p50_latency_s'] * 1e3:.1f} ms, p99 "
+            f"{ens['p99_latency_s'] * 1e3:.1f} ms, {ens['pair_interactions_per_s'] / 1e9:.2f} G pairs/s")
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"[bench] cpu baseline (up to {args.cpu_steps} steps / {args.cpu_seconds} s)")
@@ -257,6 +280,7 @@ def main():
                      "bytes_alg_per_launch": per_k[dom]["bytes_alg_per_launch"],
                      "avg_launch_ms": per_k[dom]["avg_launch_ms"]},
         "kernels": per_k,
+        "ensemble": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out_line), flush=True)

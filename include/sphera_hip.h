@@ -121,6 +121,19 @@ int sph_step(sph_ctx* ctx, int32_t nsteps);
 int sph_step_async(sph_ctx* ctx, int32_t nsteps);
 int sph_sync(sph_ctx* ctx);
 
+/* ---- serving ------------------------------------------------------------- *
+ * No counterpart in the reference (one case per process there).  An ensemble
+ * server keeps one context per request, each with its own stream, admits a
+ * request when sph_ctx_bytes fits the free device memory, enqueues its steps
+ * with sph_step_async and polls sph_query, so requests join and leave while
+ * others run. */
+/* Device bytes a context created with *p would hold. */
+int sph_ctx_bytes(const sph_params* p, unsigned long long* bytes);
+/* Free and total device memory (hipMemGetInfo). */
+int sph_device_memory(unsigned long long* free_bytes, unsigned long long* total_bytes);
+/* busy = 1 while work enqueued on the context's stream is running, else 0. */
+int sph_query(sph_ctx* ctx, int32_t* busy);
+
 /* Launch shape in use: wg_size, list entries prefetched per step of the
  * FP32 list walks, grid of the particle kernels. */
 int sph_get_launch(sph_ctx* ctx, int32_t out[3]);

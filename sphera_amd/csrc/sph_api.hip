@@ -81,6 +81,14 @@ struct sph_ctx {
 
     ~sph_ctx();
     int alloc();
+    // device bytes held by the context; with dry set, alloc() only counts them
+    size_t bytes = 0;
+    bool dry = false;
+    template <class T>
+    hipError_t da(T** p, size_t n) {
+        bytes += sizeof(T) * (n ? n : 1);
+        return dry ? hipSuccess : hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+    }
 };
 
 namespace {
@@ -308,35 +316,36 @@ sph_ctx::~sph_ctx() {
 int sph_ctx::alloc() {
     const size_t N = P.cap;
     const size_t m = P.nmax + 1;
-    for (auto& p : xm) HIPC(dalloc(&p, N));
-    for (auto& p : vel) HIPC(dalloc(&p, N));
-    for (auto& p : acc) HIPC(dalloc(&p, N));
-    for (auto& p : rp) HIPC(dalloc(&p, N));
-    for (auto& p : lam) HIPC(dalloc(&p, N));
-    HIPC(dalloc(&xmf, N));
-    for (auto& p : velf) HIPC(dalloc(&p, P.f32 ? N : 1));
-    for (auto& p : rpf) HIPC(dalloc(&p, P.f32 ? N : 1));
-    HIPC(dalloc(&dden, N));
-    HIPC(dalloc(&vpres, N));
-    HIPC(dalloc(&cell, N));
-    HIPC(dalloc(&key, N));
-    HIPC(dalloc(&slot, N));
-    HIPC(dalloc(&perm, N));
-    HIPC(dalloc(&pid, N));
-    HIPC(dalloc(&work, P.semi ? N * SPH_MB : 1));
-    HIPC(dalloc(&nwork, 1));
-    HIPC(dalloc(&cnt, m));
-    HIPC(dalloc(&start, m + 1));
-    HIPC(dalloc(&bsum, (m + 1023) / 1024 + 1));
-    HIPC(dalloc(&rowblk, (size_t)P.ncd[1] * P.ncd[2] + 1));
-    HIPC(dalloc(&nbr, N * (size_t)nb_stride(P.nmaxpartj)));
-    HIPC(dalloc(&nnb, N));
-    HIPC(dalloc(&bn, N));
-    HIPC(dalloc(&bt, P.semi ? N * SPH_MB : 1));
-    HIPC(dalloc(&scal, SC_COUNT));
-    HIPC(dalloc(&iscal, IS_COUNT));
-    HIPC(dalloc(&dtmin, 1));
-    HIPC(dalloc(&pairs, 1));
+    for (auto& p : xm) HIPC(da(&p, N));
+    for (auto& p : vel) HIPC(da(&p, N));
+    for (auto& p : acc) HIPC(da(&p, N));
+    for (auto& p : rp) HIPC(da(&p, N));
+    for (auto& p : lam) HIPC(da(&p, N));
+    HIPC(da(&xmf, N));
+    for (auto& p : velf) HIPC(da(&p, P.f32 ? N : 1));
+    for (auto& p : rpf) HIPC(da(&p, P.f32 ? N : 1));
+    HIPC(da(&dden, N));
+    HIPC(da(&vpres, N));
+    HIPC(da(&cell, N));
+    HIPC(da(&key, N));
+    HIPC(da(&slot, N));
+    HIPC(da(&perm, N));
+    HIPC(da(&pid, N));
+    HIPC(da(&work, P.semi ? N * SPH_MB : 1));
+    HIPC(da(&nwork, 1));
+    HIPC(da(&cnt, m));
+    HIPC(da(&start, m + 1));
+    HIPC(da(&bsum, (m + 1023) / 1024 + 1));
+    HIPC(da(&rowblk, (size_t)P.ncd[1] * P.ncd[2] + 1));
+    HIPC(da(&nbr, N * (size_t)nb_stride(P.nmaxpartj)));
+    HIPC(da(&nnb, N));
+    HIPC(da(&bn, N));
+    HIPC(da(&bt, P.semi ? N * SPH_MB : 1));
+    HIPC(da(&scal, SC_COUNT));
+    HIPC(da(&iscal, IS_COUNT));
+    HIPC(da(&dtmin, 1));
+    HIPC(da(&pairs, 1));
+    if (dry) return 0;
     HIPC(hipMemset(pairs, 0, sizeof(unsigned long long)));
     HIPC(hipMemset(acc[0], 0, sizeof(double4) * N));
     HIPC(hipMemset(acc[1], 0, sizeof(double4) * N));
@@ -354,6 +363,43 @@ int sph_ctx::alloc() {
 extern "C" {
 
 const char* sph_last_error(void) { return g_err.c_str(); }
+
+int sph_ctx_bytes(const sph_params* p, unsigned long long* bytes) {
+    if (!p || !bytes) return fail(1, "null argument");
+    if (p->capacity <= 0 || p->nmaxpartj <= 0) return fail(2, "invalid capacity or nmaxpartj");
+    sph_ctx c;
+    c.dry = true;
+    c.P.cap = p->capacity;
+    c.P.nmaxpartj = p->nmaxpartj;
+    c.P.nmax = p->ncd[0] * p->ncd[1] * p->ncd[2];
+    for (int d = 0; d < 3; ++d) c.P.ncd[d] = p->ncd[d];
+    c.P.f32 = p->precision == SPH_FP32_MIXED;
+    c.P.semi = p->semi;
+    c.alloc();
+    *bytes = c.bytes;
+    return 0;
+}
+
+int sph_device_memory(unsigned long long* free_bytes, unsigned long long* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(1, "null argument");
+    size_t f = 0, t = 0;
+    HIPC(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return 0;
+}
+
+int sph_query(sph_ctx* c, int32_t* busy) {
+    if (!c || !busy) return fail(1, "null argument");
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipErrorNotReady) {
+        *busy = 1;
+        return 0;
+    }
+    if (e != hipSuccess) return fail(100, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+    *busy = 0;
+    return 0;
+}
 
 int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
     if (!p || !out) return fail(1, "null argument");

@@ -28,7 +28,7 @@ EXPORTS = [
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
     "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
     "sph_preloop", "sph_step", "sph_step_async", "sph_sync", "sph_get_launch", "sph_set_timing",
-    "sph_get_timing",
+    "sph_get_timing", "sph_ctx_bytes", "sph_device_memory", "sph_query",
     "sph_last_error",
 ]
 
@@ -83,6 +83,9 @@ def load():
     L.sph_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.sph_get_launch.argtypes = [C.c_void_p, PI]
     L.sph_get_timing.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, PD, PI, C.c_int32]
+    L.sph_ctx_bytes.argtypes = [C.POINTER(SphParams), C.POINTER(C.c_uint64)]
+    L.sph_device_memory.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.sph_query.argtypes = [C.c_void_p, PI]
     L.sph_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "sph_last_error":
@@ -151,6 +154,27 @@ def params_struct(tb, precision=FP64, capacity=None, semi=True, tune=None):
     for k, v in (tune or {}).items():
         setattr(p, k, int(v))
     return p
+
+
+def ctx_bytes(tb, precision=FP64, capacity=None, semi=True):
+    """Device bytes a context for this case would hold."""
+    L = load()
+    v = C.c_uint64()
+    p = params_struct(tb, precision, capacity, semi)
+    rc = L.sph_ctx_bytes(C.byref(p), C.byref(v))
+    if rc:
+        raise SphError(rc, "sph_ctx_bytes", L.sph_last_error().decode())
+    return int(v.value)
+
+
+def device_memory():
+    """(free, total) device bytes."""
+    L = load()
+    f, t = C.c_uint64(), C.c_uint64()
+    rc = L.sph_device_memory(C.byref(f), C.byref(t))
+    if rc:
+        raise SphError(rc, "sph_device_memory", L.sph_last_error().decode())
+    return int(f.value), int(t.value)
 
 
 class SphContext:
@@ -246,6 +270,12 @@ class SphContext:
 
     def sync(self):
         return self.phase("sync")
+
+    def busy(self):
+        """True while work enqueued on this context's stream is running."""
+        v = np.zeros(1, np.int32)
+        self._check(self.L.sph_query(self.h, _pi(v)), "sph_query")
+        return bool(v[0])
 
     def launch(self):
         v = np.zeros(3, np.int32)

@@ -1,0 +1,175 @@
+"""Ensemble serving: many independent dam-break solves on one GPU.
+
+The reference runs one case per process (src/Main_algorithm/sphera.f90).  An
+ensemble here is a stream of requests, each a whole solve of its own case for
+a given number of time steps.  The server keeps one device context per
+admitted request, each with its own HIP stream, so the kernels of different
+requests overlap on the GPU:
+
+* admission is FIFO in arrival order, by free device memory: a request joins
+  when the bytes its context needs (sph_ctx_bytes) fit the free HBM above a
+  reserve, and at most `max_active` requests run at once;
+* continuous batching: every active request gets `chunk` more steps enqueued
+  (sph_step_async) as soon as its stream is idle (sph_query), and a finished
+  request leaves while the others keep running, so waiting requests join
+  without a batch boundary;
+* contexts of finished requests are released when memory is needed or at the
+  end (releasing device memory synchronises the device, which would stall the
+  streams of the running requests).
+
+Latency of a request = completion time - arrival time (time to solution);
+the run reports p50/p99 latency and the aggregate pair-interactions/s.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import runtime as rt
+from .case import DamBreak3D, build
+
+
+@dataclass
+class Request:
+    rid: int
+    case: DamBreak3D
+    steps: int
+    arrival: float  # seconds after the start of the run
+    keep_state: bool = False
+    tb: object = None
+    n: int = 0
+    t_admit: float = float("nan")
+    t_done: float = float("nan")
+    pairs: int = 0
+    state: dict | None = None
+    ctx: object = field(default=None, repr=False)
+    done_steps: int = 0
+
+
+def ensemble_case(seed, size_range=(0.2, 0.6), dx_range=(0.008, 0.02)):
+    """Dam break of request `seed`: tank width, depth and height ~U(size_range)
+    m, water column (width/2, depth, 0.8 height), dx ~U(dx_range) m."""
+    r = np.random.default_rng(seed)
+    W, D, H = r.uniform(*size_range, 3)
+    dx = float(r.uniform(*dx_range))
+    return DamBreak3D(tank=(W, D, H), column=(0.5 * W, D, 0.8 * H), dx=dx, name=f"req{seed}")
+
+
+def make_ensemble(count, steps, rate, seed0=0, arrival_seed=12345, **kw):
+    """`count` requests with seeds seed0.., Poisson arrivals of `rate` per second
+    (rate <= 0: all at t=0).  Case tables are built here, before the run."""
+    gaps = np.random.default_rng(arrival_seed).exponential(1.0 / rate, count) if rate > 0 else np.zeros(count)
+    t = np.cumsum(gaps) - (gaps[0] if rate > 0 else 0.0)
+    reqs = []
+    for k in range(count):
+        case = ensemble_case(seed0 + k, **kw)
+        tb = build(case)
+        reqs.append(Request(rid=seed0 + k, case=case, steps=steps, arrival=float(t[k]), tb=tb,
+                            n=len(tb.particles["coord"])))
+    return reqs
+
+
+class EnsembleServer:
+    def __init__(self, precision=rt.FP32_MIXED, max_active=16, chunk=25, reserve_frac=0.05):
+        self.precision = precision
+        self.max_active = max_active
+        self.chunk = chunk
+        self.reserve_frac = reserve_frac
+
+    def _start(self, r):
+        d = rt.SphContext(r.tb, precision=self.precision)
+        d.upload()
+        d.preloop()
+        r.ctx = d
+        r.done_steps = 0
+
+    def _finish(self, r, now):
+        d = r.ctx
+        d.sync()  # checks the device status flags of this request
+        r.pairs = d.pair_counter()
+        if r.keep_state:
+            r.state = d.state()
+        r.t_done = now()
+
+    def run(self, requests):
+        """Serves `requests`; returns a summary dict (latencies in seconds)."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = deque()
+        active, finished, parked = [], [], []
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        free, total = rt.device_memory()
+        while pending or waiting or active:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.popleft())
+            # admission
+            while waiting and len(active) < self.max_active:
+                r = waiting[0]
+                need = rt.ctx_bytes(r.tb, self.precision)
+                free, total = rt.device_memory()
+                if free - need < self.reserve_frac * total:
+                    if parked:  # release finished contexts, then retry
+                        for q in parked:
+                            q.ctx.close()
+                            q.ctx = None
+                        parked.clear()
+                        continue
+                    if not active:
+                        raise MemoryError(f"request {r.rid} needs {need} B, {free} B free")
+                    break
+                waiting.popleft()
+                r.t_admit = now()
+                self._start(r)
+                active.append(r)
+            # progress of the running requests
+            progressed = False
+            for r in list(active):
+                if r.ctx.busy():
+                    continue
+                progressed = True
+                if r.done_steps < r.steps:
+                    k = min(self.chunk, r.steps - r.done_steps)
+                    r.ctx.step_async(k)
+                    r.done_steps += k
+                else:
+                    self._finish(r, now)
+                    active.remove(r)
+                    finished.append(r)
+                    parked.append(r)
+            if not progressed:
+                time.sleep(20e-6)
+        makespan = now()
+        for r in parked:
+            r.ctx.close()
+            r.ctx = None
+        lat = np.array([r.t_done - r.arrival for r in finished])
+        wait = np.array([r.t_admit - r.arrival for r in finished])
+        pairs = sum(r.pairs for r in finished)
+        return {
+            "requests": len(finished),
+            "makespan_s": makespan,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()),
+            "p50_queue_s": float(np.percentile(wait, 50)),
+            "p99_queue_s": float(np.percentile(wait, 99)),
+            "pair_interactions": int(pairs),
+            "pair_interactions_per_s": pairs / makespan if makespan > 0 else 0.0,
+            "particles": [int(r.n) for r in finished],
+        }
+
+
+def run_alone(req, precision=rt.FP32_MIXED):
+    """One request on an otherwise idle device: its state after its steps."""
+    d = rt.SphContext(req.tb, precision=precision)
+    try:
+        d.upload()
+        d.preloop()
+        d.step(req.steps)
+        return d.state()
+    finally:
+        d.close()
