@@ -63,6 +63,7 @@ struct sph_ctx {
         *perm = nullptr, *pid = nullptr, *bsum = nullptr;
     int *work = nullptr, *nwork = nullptr;  // (particle, face) pairs of the boundary table
     int* rowblk = nullptr;                  // first neighbour-search block of each cell row
+    int *idslot = nullptr, *rmids = nullptr;  // id -> slot, ids removed by a sort
     int nb_grid = 8;                        // persistent workgroups of the neighbour search
     int wg = 256, grid = 1, bt_grid = 1024;  // launch shape of the particle kernels
     uint32_t* nbr = nullptr;
@@ -163,7 +164,10 @@ int run_sort(sph_ctx* c) {
     {
         KTime t(c, "sort_count");
         k_fill_int<<<nblk(m), 256, 0, c->stream>>>(c->cnt, 0, m);
-        k_cell_count<<<nblk(cap), 256, 0, c->stream>>>(P, c->xm[1], c->key, c->slot, c->cnt, c->iscal);
+        k_fill_int<<<1, 64, 0, c->stream>>>(c->iscal + IS_NRM, 0, 1);
+        k_cell_count<<<nblk(cap), 256, 0, c->stream>>>(P, c->xm[1], c->vel[2], c->key, c->slot, c->cnt,
+                                                       c->idslot, c->rmids, c->iscal);
+        k_renumber<<<1, 64, 0, c->stream>>>(c->rmids, c->idslot, c->vel[2], c->iscal);
     }
     {
         KTime t(c, "sort_scan");
@@ -279,6 +283,9 @@ int check_flags(sph_ctx* c) {
     if (is[IS_UNSUPPORTED])
         return fail(95, "particle close to a boundary type the device path does not implement "
                         "(sour/velo/flow)");
+    if (is[IS_RM_OVERFLOW])
+        return fail(102, "more than " + std::to_string(RM_CAP) +
+                             " particles left the grid in one step (renumbering list)");
     if (is[IS_BT_OVERFLOW])
         return fail(86, "a particle has more close boundary faces than the device table holds (" +
                             std::to_string(SPH_MB) + ")");
@@ -306,7 +313,7 @@ sph_ctx::~sph_ctx() {
     F(xmf);
     for (auto p : velf) F(p);
     for (auto p : rpf) F(p);
-    F(dden); F(vpres); F(rowblk); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
+    F(dden); F(vpres); F(rowblk); F(idslot); F(rmids); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     for (auto e : ev) hipEventDestroy(e);
@@ -337,6 +344,8 @@ int sph_ctx::alloc() {
     HIPC(da(&start, m + 1));
     HIPC(da(&bsum, (m + 1023) / 1024 + 1));
     HIPC(da(&rowblk, (size_t)P.ncd[1] * P.ncd[2] + 1));
+    HIPC(da(&idslot, N));
+    HIPC(da(&rmids, RM_CAP));
     HIPC(da(&nbr, N * (size_t)nb_stride(P.nmaxpartj)));
     HIPC(da(&nnb, N));
     HIPC(da(&bn, N));

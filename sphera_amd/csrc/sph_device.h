@@ -26,8 +26,11 @@ __host__ __device__ inline int nb_stride(int nmaxpartj) { return (nmaxpartj + NB
 enum { SC_DT = 0, SC_DTPREV, SC_DTAVG, SC_TIME, SC_DTVEL, SC_TV1, SC_TP1, SC_COUNT };
 enum {
     IS_N = 0, IS_IT, IS_ONGOING, IS_NB_OVERFLOW, IS_BT_OVERFLOW, IS_REMOVED, IS_UNSUPPORTED,
-    IS_NPREV, IS_COUNT
+    IS_NPREV, IS_NRM, IS_RM_OVERFLOW, IS_COUNT
 };
+
+// most particles one sort may remove from the grid (renumbering list)
+constexpr int RM_CAP = 4096;
 
 // per-context constants, passed by value to every kernel
 struct DP {

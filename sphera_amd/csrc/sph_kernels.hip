@@ -195,8 +195,10 @@ __device__ __forceinline__ bool d_point_internal(const DFace& F, double px0, dou
 constexpr int HIST_W = 512;
 
 __global__ __launch_bounds__(256) void k_cell_count(DP P, const double4* __restrict__ xm,
+                                                    const double4* __restrict__ vel,
                                                     int* __restrict__ key, int* __restrict__ slot,
-                                                    int* __restrict__ cnt, int* iscal) {
+                                                    int* __restrict__ cnt, int* __restrict__ idslot,
+                                                    int* __restrict__ rmids, int* iscal) {
     __shared__ int hist[HIST_W];
     __shared__ int base[HIST_W];
     __shared__ int cmin;
@@ -209,9 +211,13 @@ __global__ __launch_bounds__(256) void k_cell_count(DP P, const double4* __restr
     if (i < n) {
         const double4 x = xm[i];
         const int c = d_particle_cell(P, x.x, x.y, x.z);
+        const int id = (int)vel[i].w;
+        idslot[id] = i;
         if (c <= 0) {  // out of the grid: removed (OrdGrid1.f90:71-82)
             k = P.nmax;
             atomicAdd(&iscal[IS_REMOVED], 1);
+            const int q = atomicAdd(&iscal[IS_NRM], 1);
+            if (q < RM_CAP) rmids[q] = id;
         } else {
             k = c - 1;
             atomicMin(&cmin, k);
@@ -270,6 +276,50 @@ __global__ __launch_bounds__(256) void k_scan_local(const int* __restrict__ in, 
         run += v[q];
     }
     if (threadIdx.x == 255) bsum[blockIdx.x] = woff + incl;
+}
+
+// Ids after the removals of this sort, as OrdGrid1.f90:62-83 leaves pg: the
+// first loop walks npi upwards; a particle out of the grid is overwritten by
+// pg(nag), nag decreases and the same npi is tested again.  So, for the removed
+// ids r in ascending order while r < nag: the last particle moves to r unless
+// it is itself removed (then it is dropped and the next last is tried).  One
+// lane; removals are rare (particles leaving a closed tank).
+__global__ void k_renumber(int* __restrict__ rmids, const int* __restrict__ idslot,
+                           double4* __restrict__ vel, int* iscal) {
+    if (threadIdx.x != 0) return;
+    const int m = iscal[IS_NRM];
+    if (m == 0) return;
+    if (m > RM_CAP) {
+        iscal[IS_RM_OVERFLOW] = 1;
+        return;
+    }
+    for (int a = 1; a < m; ++a) {  // insertion sort, ascending
+        const int v = rmids[a];
+        int b = a - 1;
+        while (b >= 0 && rmids[b] > v) {
+            rmids[b + 1] = rmids[b];
+            --b;
+        }
+        rmids[b + 1] = v;
+    }
+    int nag = iscal[IS_N];
+    for (int q = 0; q < m; ++q) {
+        const int r = rmids[q];
+        if (r >= nag) break;
+        for (;;) {
+            const int last = --nag;
+            if (last == r) break;
+            int lo = 0, hi = m;  // is `last` removed too?
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (rmids[mid] < last) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < m && rmids[lo] == last) continue;
+            vel[idslot[last]].w = (double)r;
+            break;
+        }
+    }
 }
 
 // exclusive scan of the block totals by one workgroup

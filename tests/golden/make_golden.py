@@ -11,6 +11,9 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         steps; set-up tables of the header
   ref_small_jitter.npz  1440 particles, lattice + U(+-0.01 dx) jitter from
                         seed 20260101; state after 0 and 10 steps
+  ref_removal.npz       800 particles of which 6 start outside the grid (three
+                        holes, three at the tail): OrdGrid1's swap-with-last
+                        order; state after the pre-loop and after 6 steps
 """
 from __future__ import annotations
 
@@ -29,6 +32,11 @@ from sphera_amd.case import DamBreak3D, build  # noqa: E402
 TINY = dict(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
 SMALL = dict(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), dx=0.01, name="small")
 FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
+
+
+def removal_slots(n):
+    """Particles moved outside the grid in the removal fixture."""
+    return (3, 17, 200, n - 4, n - 2, n - 1)
 
 
 def jitter(coord, dx, seed=20260101, amp=0.01):
@@ -80,6 +88,19 @@ def main():
     for st in steps:
         pack_step(f"s{st['it']}_", st, out, lists=False)
     np.savez_compressed(HERE / "ref_small_jitter.npz", **out)
+    # removal order: particles placed outside the grid
+    case = DamBreak3D(**TINY)
+    tb = build(case)
+    ic = tb.particles["coord"].copy()
+    n = len(ic)
+    for k in removal_slots(n):
+        ic[k] = (50.0, 50.0, 50.0)
+    hdr, steps = read_dump(run_harness(case, nsteps=6, every=6, lists=False, ic_coord=ic))
+    out = {"ic_coord": ic}
+    for st in steps:
+        out[f"s{st['it']}_nag"] = st["nag"]
+        pack_step(f"s{st['it']}_", st, out, lists=False)
+    np.savez_compressed(HERE / "ref_removal.npz", **out)
     for f in sorted(HERE.glob("*.npz")):
         print(f.name, f.stat().st_size)
 

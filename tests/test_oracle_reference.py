@@ -119,3 +119,47 @@ def test_live_reference_run(jit, tmp_path):
         assert st["dt"] == r["dt"]
         for f in FIELDS:
             assert np.array_equal(st[f], r[f]), (f, r["it"])
+
+
+@pytest.mark.skipif(not HARNESS.exists(), reason="reference not built here")
+def test_live_reference_removal_order(tmp_path):
+    """Particles outside the grid are removed by OrdGrid1's swap-with-last
+    (OrdGrid1.f90:62-83), which renumbers pg: the oracle keeps the same order
+    (tail particles moved into the holes, removed tails dropped)."""
+    from oracle.decks import read_dump, run_harness
+    case = DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="rm")
+    tb = build(case)
+    ic = tb.particles["coord"].copy()
+    n = len(ic)
+    for k in (3, 17, 200, n - 4, n - 2, n - 1):  # holes, and removed particles at the tail
+        ic[k] = (50.0, 50.0, 50.0)
+    hdr, steps = read_dump(run_harness(case, nsteps=6, every=3, lists=False, ic_coord=ic,
+                                       workdir=tmp_path))
+    o = Oracle(tb, coord=ic)
+    o.preloop()
+    assert o.n == n - 6 == steps[0]["nag"]
+    prev = 0
+    for r in steps:
+        o.step(r["it"] - prev)
+        prev = r["it"]
+        st = o.state()
+        for f in FIELDS:
+            assert np.array_equal(st[f], r[f]), (f, r["it"])
+
+
+def test_golden_removal_order():
+    """Six particles start outside the grid (three holes, three at the tail):
+    the oracle's pg order after OrdGrid1's swap-with-last equals the
+    reference's, and so do the fields after 6 steps."""
+    g = np.load(GOLD / "ref_removal.npz")
+    tb = build(_tiny())
+    o = Oracle(tb, coord=g["ic_coord"])
+    o.preloop()
+    assert o.n == int(g["s0_nag"]) == len(g["ic_coord"]) - 6
+    st = o.state()
+    for f in FIELDS:
+        assert np.array_equal(st[f], g["s0_" + f]), ("s0", f)
+    o.step(6)
+    st = o.state()
+    for f in FIELDS:
+        assert np.array_equal(st[f], g["s6_" + f]), ("s6", f)
