@@ -95,19 +95,30 @@ def make_case(args):
     return case, tb, coord
 
 
-def run_device(tb, coord, precision, steps, warmup, tune=None):
+def run_device(tb, coord, precision, steps, warmup, tune=None, graph=True):
     from sphera_amd import runtime as rt
     d = rt.SphContext(tb, precision=precision, tune=tune)
     d.upload(coord=coord)
     d.preloop()
     if warmup:
         d.step(warmup)
-    d.set_timing(True)
+    if graph:  # capture (and instantiate) the graph outside the timed region
+        d.step_graph(1)
+        d.sync()
     p0 = d.pair_counter()  # synchronises the context stream
     t0 = time.perf_counter()
-    d.step(steps)          # enqueues every step, then synchronises
+    if graph:
+        for _ in range(steps):  # one graph launch per step
+            d.step_graph(1)
+        d.sync()
+    else:
+        d.step(steps)       # enqueues every step, then synchronises
     t1 = time.perf_counter()
     pairs = d.pair_counter() - p0
+    # per-kernel durations from HIP events on the context stream, in a run of
+    # its own so that the events do not load the timed steps above
+    d.set_timing(True)
+    d.step(max(5, min(steps, 20)))
     timing = d.timing()
     n = d.n
     sc = d.scalars()
@@ -167,8 +178,11 @@ def main():
                     help="CPU baseline sample length (whole steps until this many seconds)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel of a step instead of replaying the step's HIP graph")
     ap.add_argument("--wg", type=int, default=0,
                     help="fixed workgroup size of the particle kernels (skips the autotuner)")
+    ap.add_argument("--chunk", type=int, default=0, help="fixed FP32 list-walk chunk (4 or 8)")
     ap.add_argument("--only", choices=("fp32_mixed", "fp64"), default=None,
                     help="time one precision only (profiling runs)")
     ap.add_argument("--ens-requests", type=int, default=64,
@@ -194,14 +208,14 @@ def main():
     for name, prec in (("fp32_mixed", rt.FP32_MIXED), ("fp64", rt.FP64)):
         if args.only and name != args.only:
             continue
-        tune = dict(wg_size=args.wg) if args.wg else None
+        tune = dict(wg_size=args.wg, chunk=args.chunk) if args.wg else None
         if not args.no_autotune and not args.wg:
             from sphera_amd.autotune import autotune
             tune, _ = autotune(tb, coord, prec, log=log)
             log(f"[bench] {name}: autotuned launch {tune}")
         tunes[name] = tune
         log(f"[bench] running {name}")
-        res[name] = run_device(tb, coord, prec, args.steps, args.warmup, tune)
+        res[name] = run_device(tb, coord, prec, args.steps, args.warmup, tune, graph=not args.no_graph)
         r = res[name]
         log(f"[bench] {name}: {r['seconds'] / args.steps * 1e3:.3f} ms/step, "
             f"{r['pairs'] / r['seconds'] / 1e9:.3f} G pairs/s")
@@ -270,7 +284,7 @@ def main():
                                f"dx={case.dx} m, h/dx=1.3, SA-SPH walls", "particles": n,
                    "grid_cells": int(tb.grid.nmax)},
         "fp64": fp64,
-        "launch": main_r["launch"],
+        "launch": dict(main_r["launch"], graph=not args.no_graph),
         "mixed_over_fp64": value / fp64["value"] if fp64 else None,
         "pair_kernels_only_value": main_r["pairs"] / pair_time if pair_time > 0 else None,
         "kernel_ms": kern,

@@ -46,10 +46,11 @@ typedef struct sph_params {
     double den0, eps, celerita, alfamon, kin_visc; /* Med(1) */
     double cfl, vsc_coeff, pesodt, tetap, tetav;   /* input_any_t */
     int32_t semi;        /* 1: SA-SPH frontiers active (Domain%tipo=="semi") */
-    /* launch tunable of the particle kernels (0 = default; the autotuner in
-     * sphera_amd/autotune.py picks it per case) */
+    /* launch tunables of the particle kernels (0 = default; the autotuner in
+     * sphera_amd/autotune.py picks them per case) */
     int32_t wg_size;     /* threads per workgroup: 64 (default), 128, 192 or 256 */
-    int32_t reserved[4];
+    int32_t chunk;       /* list entries gathered per step of the FP32 walks: 4 or 8 (default) */
+    int32_t reserved[3];
 } sph_params;
 
 /* SA-SPH frontier faces in BFaceList order and the grid-cell/face incidence
@@ -119,6 +120,9 @@ int sph_step(sph_ctx* ctx, int32_t nsteps);
 /* Same, enqueued on the context stream without the final synchronisation
  * and flag check (for overlap and graph capture by the caller). */
 int sph_step_async(sph_ctx* ctx, int32_t nsteps);
+/* Same, as one HIP graph of nsteps whole steps (captured at the first call
+ * with this nsteps, replayed after): one launch instead of ~20 per step. */
+int sph_step_graph(sph_ctx* ctx, int32_t nsteps);
 int sph_sync(sph_ctx* ctx);
 
 /* ---- serving ------------------------------------------------------------- *

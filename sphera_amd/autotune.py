@@ -1,11 +1,10 @@
 """Launch-shape autotuner for the particle kernels.
 
 For one case and precision it times a few whole time steps for each
-workgroup size on a scratch context and returns the fastest as the ``tune``
-dict that ``SphContext`` / ``sph_params.wg_size`` accept.  (Staging the
-neighbour cells' particles in LDS was measured against plain L2-cached
-gathers during development and lost at every block size and tile on MI355X,
-so the only launch knob left is the workgroup size.)
+workgroup size (and, for FP32-mixed, each list-walk chunk) on a scratch
+context and returns the fastest as the ``tune`` dict that ``SphContext`` /
+``sph_params`` accept.  The chunk trades registers (occupancy) against loads
+in flight per lane.
 """
 from __future__ import annotations
 
@@ -13,7 +12,8 @@ import time
 
 
 def candidates(tb, precision):
-    return [dict(wg_size=w) for w in (64, 128, 192, 256)]
+    chunks = (4, 8) if precision == 1 else (8,)
+    return [dict(wg_size=w, chunk=c) for w in (64, 128, 256) for c in chunks]
 
 
 def time_config(tb, coord, precision, tune, steps=4, warmup=2):

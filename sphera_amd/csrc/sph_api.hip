@@ -66,6 +66,8 @@ struct sph_ctx {
     int *idslot = nullptr, *rmids = nullptr;  // id -> slot, ids removed by a sort
     int nb_grid = 8;                        // persistent workgroups of the neighbour search
     int wg = 256, grid = 1, bt_grid = 1024;  // launch shape of the particle kernels
+    int chunk = 8;                            // list entries gathered per step of the FP32 walks
+    std::vector<std::pair<int, hipGraphExec_t>> graphs;  // captured n-step sequences
     uint32_t* nbr = nullptr;
     int* nnb = nullptr;
     int* bn = nullptr;
@@ -223,12 +225,16 @@ int run_dt(sph_ctx* c, int advance, bool reduce) {
 int run_momentum(sph_ctx* c) {
     KTime t(c, "momentum");
     if (c->P.f32)
-        k_momentum<true><<<c->grid, c->wg, 0, c->stream>>>(
+        {
+            auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
             c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
-            c->vel[1], c->velf[1], c->scal, c->iscal);
+            c->vel[1], c->velf[1], c->scal, c->iscal); };
+            if (c->chunk == 4) L(k_momentum<true, 4>);
+            else L(k_momentum<true, 8>);
+        }
     else
-        k_momentum<false><<<c->grid, c->wg, 0, c->stream>>>(
+        k_momentum<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
             c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
             c->vel[1], c->velf[1], c->scal, c->iscal);
@@ -238,12 +244,16 @@ int run_momentum(sph_ctx* c) {
 int run_vsmooth(sph_ctx* c) {
     KTime t(c, "velocity_smoothing");
     if (c->P.f32)
-        k_vsmooth<true><<<c->grid, c->wg, 0, c->stream>>>(
+        {
+            auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
             c->rp[0], c->xmf, c->velf[1], c->nbr, c->nnb, c->vel[2], c->velf[2], c->xm[1], c->scal,
-            c->dtmin, c->iscal);
+            c->dtmin, c->iscal); };
+            if (c->chunk == 4) L(k_vsmooth<true, 4>);
+            else L(k_vsmooth<true, 8>);
+        }
     else
-        k_vsmooth<false><<<c->grid, c->wg, 0, c->stream>>>(
+        k_vsmooth<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
             c->rp[0], c->xmf, c->velf[1], c->nbr, c->nnb, c->vel[2], c->velf[2], c->xm[1], c->scal,
             c->dtmin, c->iscal);
@@ -253,11 +263,15 @@ int run_vsmooth(sph_ctx* c) {
 int run_continuity(sph_ctx* c) {
     KTime t(c, "continuity");
     if (c->P.f32)
-        k_continuity<true><<<c->grid, c->wg, 0, c->stream>>>(
+        {
+            auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[1], c->xmf, c->velf[0], c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
+            c->rp[1], c->xmf, c->velf[0], c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal); };
+            if (c->chunk == 4) L(k_continuity<true, 4>);
+            else L(k_continuity<true, 8>);
+        }
     else
-        k_continuity<false><<<c->grid, c->wg, 0, c->stream>>>(
+        k_continuity<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
             c->rp[1], c->xmf, c->velf[0], c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
     return 0;
@@ -266,11 +280,15 @@ int run_continuity(sph_ctx* c) {
 int run_psmooth(sph_ctx* c) {
     KTime t(c, "pressure_smoothing");
     if (c->P.f32)
-        k_psmooth<true><<<c->grid, c->wg, 0, c->stream>>>(
+        {
+            auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
-            c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal);
+            c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal); };
+            if (c->chunk == 4) L(k_psmooth<true, 4>);
+            else L(k_psmooth<true, 8>);
+        }
     else
-        k_psmooth<false><<<c->grid, c->wg, 0, c->stream>>>(
+        k_psmooth<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
             c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal);
     return 0;
@@ -317,6 +335,7 @@ sph_ctx::~sph_ctx() {
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     for (auto e : ev) hipEventDestroy(e);
+    for (auto& g : graphs) hipGraphExecDestroy(g.second);
     if (stream) hipStreamDestroy(stream);
 }
 
@@ -487,6 +506,11 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
             return fail(2, "wg_size must be 64, 128, 192 or 256");
         }
         c->grid = (P.cap + c->wg - 1) / c->wg;
+        c->chunk = p->chunk > 0 ? p->chunk : 8;
+        if (c->chunk != 4 && c->chunk != 8) {
+            delete c;
+            return fail(2, "chunk must be 4 or 8");
+        }
         c->bt_grid = 4 * ncu;
         // three neighbour-search workgroups fit a CU (LDS), a multiple of 8 XCDs
         c->nb_grid = (3 * ncu + 7) / 8 * 8;
@@ -782,9 +806,7 @@ int sph_preloop(sph_ctx* c) {
     return check_flags(c);
 }
 
-int sph_step_async(sph_ctx* c, int32_t nsteps) {
-    if (!c) return fail(1, "null argument");
-    if (c->phase != PH_READY) return fail(120, "sph_step needs the state at a time-step boundary");
+static void enqueue_steps(sph_ctx* c, int nsteps) {
     for (int s = 0; s < nsteps; ++s) {
         run_dt(c, 1, false);
         run_momentum(c);
@@ -795,7 +817,37 @@ int sph_step_async(sph_ctx* c, int32_t nsteps) {
         run_continuity(c);
         run_psmooth(c);
     }
+}
+
+int sph_step_async(sph_ctx* c, int32_t nsteps) {
+    if (!c) return fail(1, "null argument");
+    if (c->phase != PH_READY) return fail(120, "sph_step needs the state at a time-step boundary");
+    enqueue_steps(c, nsteps);
     HIPC(hipGetLastError());
+    return 0;
+}
+
+int sph_step_graph(sph_ctx* c, int32_t nsteps) {
+    if (!c) return fail(1, "null argument");
+    if (c->phase != PH_READY) return fail(120, "sph_step needs the state at a time-step boundary");
+    if (nsteps <= 0) return 0;
+    if (c->timing) return sph_step_async(c, nsteps);  // per-kernel events need eager launches
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+        if (g.first == nsteps) exec = g.second;
+    if (!exec) {
+        // every launch of a step uses the same arguments (each buffer role
+        // returns to its buffer after a step) and no memset/copy nodes
+        HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_steps(c, nsteps);
+        hipGraph_t g = nullptr;
+        HIPC(hipStreamEndCapture(c->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(100, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        c->graphs.emplace_back(nsteps, exec);
+    }
+    HIPC(hipGraphLaunch(exec, c->stream));
     return 0;
 }
 
@@ -815,7 +867,7 @@ int sph_step(sph_ctx* c, int32_t nsteps) {
 int sph_get_launch(sph_ctx* c, int32_t out[3]) {
     if (!c || !out) return fail(1, "null argument");
     out[0] = c->wg;
-    out[1] = 8;
+    out[1] = c->chunk;
     out[2] = c->grid;
     return 0;
 }

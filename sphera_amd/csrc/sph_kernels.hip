@@ -1146,7 +1146,7 @@ __device__ __forceinline__ void sa_momentum(const DP& P, const DFace* __restrict
 // viscomon.f90:54-71, viscomorris.f90:60-71, AddElasticBoundaryReaction_3D.f90:
 // 67-98; fused Leapfrog_momentum.f90:66-86
 // ----------------------------------------------------------------------------
-template <bool F32>
+template <bool F32, int CH>
 __global__ __launch_bounds__(256) void k_momentum(
     DP P, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
@@ -1250,7 +1250,7 @@ __global__ __launch_bounds__(256) void k_momentum(
                     const int j = (int)(e & NB_MASK);
                     return RM{gx[j], gv[j], gr[j]};
                 };
-                walk_list<8, RM>(nbr, N, i, nn, fetch, [&](uint32_t e, const RM& q) {
+                walk_list<CH, RM>(nbr, N, i, nn, fetch, [&](uint32_t e, const RM& q) {
                     if ((int)(e & NB_MASK) == i) return;
                     const float4 xj = q.x;
                     const float4 vj = q.v;
@@ -1327,7 +1327,7 @@ __global__ __launch_bounds__(256) void k_momentum(
 // Leapfrog_trajectories.f90:48-76 and the particle part of the next
 // time_step_duration.  Neighbour density travels in velf.w (FP32).
 // ----------------------------------------------------------------------------
-template <bool F32>
+template <bool F32, int CH>
 __global__ __launch_bounds__(256) void k_vsmooth(
     DP P, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
@@ -1380,7 +1380,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                         const int j = (int)(e & NB_MASK);
                             return RV{xmf[j], velf1[j]};
                     };
-                    walk_list<8, RV>(nbr, N, i, nn, fetch, [&](uint32_t e, const RV& q) {
+                    walk_list<CH, RV>(nbr, N, i, nn, fetch, [&](uint32_t e, const RV& q) {
                         const float4 xj = q.x;
                         const float4 vj = q.r;
                         float R0, R1, R2;
@@ -1441,7 +1441,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
 // .f90:82-169 (FP64), Continuity_Equation.f90:97-235; fused
 // Leapfrog_continuity.f90:51-60 and EoS_fluid_loop.f90:59-70
 // ----------------------------------------------------------------------------
-template <bool F32>
+template <bool F32, int CH>
 __global__ __launch_bounds__(256) void k_continuity(
     DP P, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
@@ -1525,7 +1525,7 @@ __global__ __launch_bounds__(256) void k_continuity(
                     const int j = (int)(e & NB_MASK);
                     return RC{xmf[j], velf[j]};
                 };
-                walk_list<8, RC>(nbr, N, i, nn, fetch, [&](uint32_t e, const RC& q) {
+                walk_list<CH, RC>(nbr, N, i, nn, fetch, [&](uint32_t e, const RC& q) {
                     const float4 xj = q.x;
                     const float4 vj = q.r;
                     float R0, R1, R2;
@@ -1551,7 +1551,7 @@ __global__ __launch_bounds__(256) void k_continuity(
 // ----------------------------------------------------------------------------
 // pressure smoothing: PressureSmoothing_3D.f90:80-161 (both loops)
 // ----------------------------------------------------------------------------
-template <bool F32>
+template <bool F32, int CH>
 __global__ __launch_bounds__(256) void k_psmooth(
     DP P, const int* __restrict__ bn, const BTab* __restrict__ bt,
     const double4* __restrict__ xm, const double2* __restrict__ rp, const float4* __restrict__ xmf,
@@ -1600,7 +1600,7 @@ __global__ __launch_bounds__(256) void k_psmooth(
                         const int j = (int)(e & NB_MASK);
                             return RP2{xmf[j], rpf[j]};
                     };
-                    walk_list<8, RP2>(nbr, N, i, nsp, fetch, [&](uint32_t e, const RP2& q) {
+                    walk_list<CH, RP2>(nbr, N, i, nsp, fetch, [&](uint32_t e, const RP2& q) {
                         const float4 xj = q.x;
                         const float2 rpj = q.r;
                         float R0, R1, R2;

@@ -95,6 +95,7 @@ p%tetav = input_any_t%TetaV
 p%semi = 0
 if (Domain%tipo=="semi") p%semi = 1
 p%wg_size = 0
+p%chunk = 0
 p%reserved(:) = 0
 end subroutine pack_params
 

@@ -20,7 +20,8 @@ type,bind(C) :: sph_params_t
    real(c_double) :: cfl, vsc_coeff, pesodt, tetap, tetav
    integer(c_int32_t) :: semi
    integer(c_int32_t) :: wg_size
-   integer(c_int32_t) :: reserved(4)
+   integer(c_int32_t) :: chunk
+   integer(c_int32_t) :: reserved(3)
 end type sph_params_t
 
 type,bind(C) :: sph_faces_t
@@ -143,6 +144,21 @@ interface
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr),value :: ctx
       integer(c_int32_t),value :: nsteps
+   end function
+   integer(c_int) function sph_step_graph(ctx,nsteps) bind(C,name="sph_step_graph")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr),value :: ctx
+      integer(c_int32_t),value :: nsteps
+   end function
+   integer(c_int) function sph_query(ctx,busy) bind(C,name="sph_query")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr),value :: ctx
+      integer(c_int32_t),intent(out) :: busy
+   end function
+   integer(c_int) function sph_device_memory(free_bytes,total_bytes)                  &
+      bind(C,name="sph_device_memory")
+      import :: c_int, c_long_long
+      integer(c_long_long),intent(out) :: free_bytes, total_bytes
    end function
    integer(c_int) function sph_sync(ctx) bind(C,name="sph_sync")
       import :: c_int, c_ptr

@@ -27,7 +27,7 @@ EXPORTS = [
     "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_get_neighbour_lists", "sph_dt", "sph_momentum",
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
     "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
-    "sph_preloop", "sph_step", "sph_step_async", "sph_sync", "sph_get_launch", "sph_set_timing",
+    "sph_preloop", "sph_step", "sph_step_async", "sph_step_graph", "sph_sync", "sph_get_launch", "sph_set_timing",
     "sph_get_timing", "sph_ctx_bytes", "sph_device_memory", "sph_query",
     "sph_last_error",
 ]
@@ -46,7 +46,7 @@ class SphParams(C.Structure):
                 ("alfamon", C.c_double), ("kin_visc", C.c_double), ("cfl", C.c_double),
                 ("vsc_coeff", C.c_double), ("pesodt", C.c_double), ("tetap", C.c_double),
                 ("tetav", C.c_double), ("semi", C.c_int32), ("wg_size", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("chunk", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class SphFaces(C.Structure):
@@ -80,6 +80,7 @@ def load():
         getattr(L, "sph_" + ph).argtypes = [C.c_void_p]
     L.sph_step.argtypes = [C.c_void_p, C.c_int32]
     L.sph_step_async.argtypes = [C.c_void_p, C.c_int32]
+    L.sph_step_graph.argtypes = [C.c_void_p, C.c_int32]
     L.sph_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.sph_get_launch.argtypes = [C.c_void_p, PI]
     L.sph_get_timing.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, PD, PI, C.c_int32]
@@ -267,6 +268,10 @@ class SphContext:
 
     def step_async(self, k=1):
         return self._check(self.L.sph_step_async(self.h, int(k)), "sph_step_async")
+
+    def step_graph(self, k=1):
+        """k steps as one replayed HIP graph (asynchronous; sync() to wait)."""
+        return self._check(self.L.sph_step_graph(self.h, int(k)), "sph_step_graph")
 
     def sync(self):
         return self.phase("sync")

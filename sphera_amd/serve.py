@@ -10,7 +10,9 @@ requests overlap on the GPU:
   when the bytes its context needs (sph_ctx_bytes) fit the free HBM above a
   reserve, and at most `max_active` requests run at once;
 * continuous batching: every active request gets `chunk` more steps enqueued
-  (sph_step_async) as soon as its stream is idle (sph_query), and a finished
+  (sph_step_async, or with graphs=True replays of the request's captured
+  one-step HIP graph, sph_step_graph) as soon as its stream is idle
+  (sph_query), and a finished
   request leaves while the others keep running, so waiting requests join
   without a batch boundary;
 * contexts of finished requests are released when memory is needed or at the
@@ -73,8 +75,10 @@ def make_ensemble(count, steps, rate, seed0=0, arrival_seed=12345, **kw):
 
 
 class EnsembleServer:
-    def __init__(self, precision=rt.FP32_MIXED, max_active=16, chunk=25, reserve_frac=0.05):
+    def __init__(self, precision=rt.FP32_MIXED, max_active=16, chunk=25, reserve_frac=0.05,
+                 graphs=False):
         self.precision = precision
+        self.graphs = graphs
         self.max_active = max_active
         self.chunk = chunk
         self.reserve_frac = reserve_frac
@@ -133,7 +137,11 @@ class EnsembleServer:
                 progressed = True
                 if r.done_steps < r.steps:
                     k = min(self.chunk, r.steps - r.done_steps)
-                    r.ctx.step_async(k)
+                    if self.graphs:  # the one-step graph, captured once per request
+                        for _ in range(k):
+                            r.ctx.step_graph(1)
+                    else:
+                        r.ctx.step_async(k)
                     r.done_steps += k
                 else:
                     self._finish(r, now)

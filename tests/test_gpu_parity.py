@@ -217,3 +217,26 @@ def test_removal_renumbering_matches_oracle(tiny_case):
             assert np.array_equal(ds[f], os_[f]), (k, f)
         assert np.array_equal(ds["coord"], g[f"s{k}_coord"])
     _lists_equal(d, o)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_graph_replay_matches_eager(precision, small_case):
+    """sph_step_graph replays a captured 3-step graph: after every replay
+    (with eager work on another context in between) the state equals the
+    eagerly stepped context's, bit for bit."""
+    rt = _rt()
+    prec = rt.FP64 if precision == "fp64" else rt.FP32_MIXED
+    tb = build(small_case)
+    coord = _jittered(tb)
+    a = _device(tb, prec, coord)
+    b = _device(tb, prec, coord)
+    print(f"[gpu] graph replay {precision}", flush=True)
+    for rep in range(3):
+        a.step(3)
+        b.step_graph(3)
+        b.sync()
+        sa, sb = a.state(), b.state()
+        for f in FIELDS:
+            assert np.array_equal(sa[f], sb[f]), (rep, f)
+        assert sa["dt"] == sb["dt"] and sa["it"] == sb["it"]
+    _lists_equal(a, b)

@@ -42,6 +42,9 @@ class FakeCtx:
     def step_async(self, k):
         self.pending += k
 
+    def step_graph(self, k):
+        self.pending += k
+
     def sync(self):
         pass
 
