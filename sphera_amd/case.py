@@ -436,13 +436,21 @@ class CaseTables:
     particles: dict = field(default_factory=dict)
 
 
-def build(case: DamBreak3D) -> CaseTables:
+def build(case: DamBreak3D, particles=True, incidence=True) -> CaseTables:
+    """All set-up tables of a case.  particles=False skips the lattice IC (the
+    caller uploads its own particles); incidence=False leaves the grid-cell /
+    face incidence empty (for runs without SA-SPH frontiers)."""
     s = case.scalars()
     lo, hi = domain_box(case)
     g = make_grid(lo, hi, case.dx, s["doubleh"])
     f = make_faces(case)
-    cnt, st, vec = grid_face_incidence(g, f, case.maxcloseboundfaces)
+    if incidence:
+        cnt, st, vec = grid_face_incidence(g, f, case.maxcloseboundfaces)
+    else:
+        nmax = int(g.nmax)
+        cnt, st, vec = np.zeros(nmax, np.int32), np.ones(nmax, np.int32), np.zeros(0, np.int32)
     kt, ktd = kernel_table(s["h"])
     bit = boundary_integral_tab(s["doubleh"])
-    parts = initial_particles(case)
+    parts = initial_particles(case) if particles else {
+        k: np.zeros((0, 3) if k in ("coord", "vel") else 0) for k in ("coord", "vel", "pres", "dens", "mass")}
     return CaseTables(case, s, g, f, cnt, st, vec, kt, ktd, bit, parts)
