@@ -51,6 +51,7 @@ struct sph_ctx {
     // faces / tables
     DFace* d_faces = nullptr;
     int *d_gcount = nullptr, *d_gstart = nullptr, *d_gvec = nullptr;
+    int* d_gnear = nullptr;  // per cell: frontier faces listed in its 27-cell neighbourhood
     double *d_kt = nullptr, *d_bit = nullptr;
     // state
     double4 *xm[2] = {}, *vel[3] = {}, *acc[2] = {};
@@ -206,7 +207,7 @@ int run_btab(sph_ctx* c) {
     }
     KTime t(c, "boundary_tab");
     k_fill_int<<<1, 64, 0, c->stream>>>(c->nwork, 0, 1);
-    k_boundary_find<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_gcount, c->d_gstart,
+    k_boundary_find<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_gnear, c->d_gcount, c->d_gstart,
                                                             c->d_gvec, c->xm[0], c->bn, c->bt, c->work,
                                                             c->nwork, c->iscal);
     k_boundary_int<<<c->bt_grid, 256, 0, c->stream>>>(c->P, c->d_faces, c->d_kt, c->d_bit, c->bt, c->work,
@@ -322,7 +323,7 @@ sph_ctx::~sph_ctx() {
     auto F = [](void* p) {
         if (p) hipFree(p);
     };
-    F(d_faces); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit);
+    F(d_faces); F(d_gnear); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit);
     for (auto p : xm) F(p);
     for (auto p : vel) F(p);
     for (auto p : acc) F(p);
@@ -540,6 +541,7 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
                 delete c;
                 return fail(2, "SA-SPH faces must be triangles or quadrilaterals");
             }
+            dface_point_coeffs(F);
         }
         long long nv = 0;
         for (int i = 0; i < P.nmax; ++i) nv += f->gcbf_count[i];
@@ -551,6 +553,26 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
                   hipMalloc(&c->d_bit, sizeof(double) * 256 * 7) == hipSuccess;
         ok = ok && hipMemcpy(c->d_faces, hf.data(), sizeof(DFace) * f->nf, hipMemcpyHostToDevice) == hipSuccess;
         ok = ok && hipMemcpy(c->d_gcount, f->gcbf_count, sizeof(int) * P.nmax, hipMemcpyHostToDevice) == hipSuccess;
+        {
+            // particles of cells with no face in reach skip the face search
+            std::vector<int> near(P.nmax, 0);
+            const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
+            for (int k = 0; k < nk; ++k)
+                for (int j = 0; j < nj; ++j)
+                    for (int i = 0; i < ni; ++i) {
+                        int sum = 0;
+                        for (int dk = -1; dk <= 1; ++dk)
+                            for (int dj = -1; dj <= 1; ++dj)
+                                for (int di = -1; di <= 1; ++di) {
+                                    const int ii = i + di, jj = j + dj, kk = k + dk;
+                                    if (ii < 0 || ii >= ni || jj < 0 || jj >= nj || kk < 0 || kk >= nk) continue;
+                                    sum += f->gcbf_count[((size_t)kk * nj + jj) * ni + ii];
+                                }
+                        near[((size_t)k * nj + j) * ni + i] = sum;
+                    }
+            ok = ok && hipMalloc(&c->d_gnear, sizeof(int) * P.nmax) == hipSuccess &&
+                 hipMemcpy(c->d_gnear, near.data(), sizeof(int) * P.nmax, hipMemcpyHostToDevice) == hipSuccess;
+        }
         ok = ok && hipMemcpy(c->d_gstart, f->gcbf_start, sizeof(int) * P.nmax, hipMemcpyHostToDevice) == hipSuccess;
         if (nv) ok = ok && hipMemcpy(c->d_gvec, f->gcbf_vec, sizeof(int) * nv, hipMemcpyHostToDevice) == hipSuccess;
         ok = ok && hipMemcpy(c->d_kt, f->kerneltab, sizeof(double) * 41 * 5, hipMemcpyHostToDevice) == hipSuccess;

@@ -53,8 +53,28 @@ struct DFace {
     double GX[12];  // node global coordinates x 4
     double area, slip;
     double vel[3], tvel[3];
+    // IsPointInternal's per-face constants (1/DueArea and the AA, BB, CC of
+    // its two edges), evaluated once with the same expressions
+    double pu, pA[2], pB[2], pC[2];
     int nodes, btype, mode, lam;
 };
+
+// LocalNormalCoordinates.f90:55-87 constants of a face: node pairs (jj,kk) of
+// the two area coordinates, DueArea = (3-fkod) area
+inline void dface_point_coeffs(DFace& F) {
+    const int fkod = F.nodes - 2;
+    const double DueArea = (double)(3 - fkod) * F.area;
+    F.pu = 1.0 / DueArea;
+    const int a[2] = {fkod == 1 ? 1 : 2, fkod == 1 ? 2 : 3};
+    const int b[2] = {fkod == 1 ? 2 : 3, 0};
+    for (int e = 0; e < 2; ++e) {
+        const double xj = F.LX[2 * a[e]], yj = F.LX[2 * a[e] + 1];
+        const double xk = F.LX[2 * b[e]], yk = F.LX[2 * b[e] + 1];
+        F.pA[e] = xj * yk - xk * yj;
+        F.pB[e] = yj - yk;
+        F.pC[e] = xk - xj;
+    }
+}
 
 // one BoundaryDataTab record (Hybrid_allocation_module.f90:796-805)
 struct BTab {

@@ -152,38 +152,26 @@ __device__ __forceinline__ void rag32(float4 xi, float4 xj, uint32_t code, float
 // ----------------------------------------------------------------------------
 // SA-SPH geometry: LocalNormalCoordinates.f90:55-87 + IsPointInternal.f90
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ bool d_point_internal(const DFace& F, double px0, double px1) {
-    const int fkod = F.nodes - 2;
-    const double DueArea = (double)(3 - fkod) * F.area;
-    const double Usu = 1.0 / DueArea;
-    // (jj,kk) node pairs of LocalNormalCoordinates for ii=1,2
-    const int a0 = fkod == 1 ? 1 : 2, b0 = fkod == 1 ? 2 : 3;  // ii=1
-    const int a1 = fkod == 1 ? 2 : 3, b1 = 0;                   // ii=2
-    double csi[3];
-    {
-        const double xj = F.LX[2 * a0], yj = F.LX[2 * a0 + 1];
-        const double xk = F.LX[2 * b0], yk = F.LX[2 * b0 + 1];
-        const double AA = xj * yk - xk * yj, BB = yj - yk, CC = xk - xj;
-        csi[0] = (AA + BB * px0 + CC * px1) * Usu;
-    }
-    {
-        const double xj = F.LX[2 * a1], yj = F.LX[2 * a1 + 1];
-        const double xk = F.LX[2 * b1], yk = F.LX[2 * b1 + 1];
-        const double AA = xj * yk - xk * yj, BB = yj - yk, CC = xk - xj;
-        csi[1] = (AA + BB * px0 + CC * px1) * Usu;
-    }
-    csi[2] = 1.0 - (csi[0] + csi[1]);
-    if (fkod == 1) {
-        for (int ii = 0; ii < 3; ++ii)
-            if (csi[ii] < 0.0 || csi[ii] > 1.0) return false;
-        return true;
-    }
-    if (fkod == 2) {
-        for (int ii = 0; ii < 2; ++ii)
-            if (csi[ii] < 0.0 || csi[ii] > 1.0) return false;
-        return true;
-    }
+// IsPointInternal.f90 on the face's precomputed constants (kept in registers
+// by the callers that test many points of one face)
+struct PIC {
+    double pu, a0, b0, c0, a1, b1, c1;
+    int fkod;
+};
+__device__ __forceinline__ PIC pic_of(const DFace& F) {
+    return PIC{F.pu, F.pA[0], F.pB[0], F.pC[0], F.pA[1], F.pB[1], F.pC[1], F.nodes - 2};
+}
+__device__ __forceinline__ bool d_point_internal(const PIC& c, double px0, double px1) {
+    const double csi0 = (c.a0 + c.b0 * px0 + c.c0 * px1) * c.pu;
+    const double csi1 = (c.a1 + c.b1 * px0 + c.c1 * px1) * c.pu;
+    const double csi2 = 1.0 - (csi0 + csi1);
+    const bool in01 = !(csi0 < 0.0 || csi0 > 1.0) && !(csi1 < 0.0 || csi1 > 1.0);
+    if (c.fkod == 1) return in01 && !(csi2 < 0.0 || csi2 > 1.0);
+    if (c.fkod == 2) return in01;
     return false;
+}
+__device__ __forceinline__ bool d_point_internal(const DFace& F, double px0, double px1) {
+    return d_point_internal(pic_of(F), px0, px1);
 }
 
 // ----------------------------------------------------------------------------
@@ -754,6 +742,7 @@ __device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __
     if (L2 < 0.0) return;
     if (L2 < zpmin) L2 = zpmin;
     double robpre = 2.1;
+    const PIC pic = pic_of(F);
     for (int ipk = 0; ipk < 256; ++ipk) {
         const double* b = &bit[ipk * 7];
         const double Pj0 = L0 + b[0], Pj1 = L1 + b[1], Pj2 = L2 + b[2];
@@ -761,7 +750,7 @@ __device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __
         const double ts = L2 / (L2 - Pj2);
         const double X0 = L0 + (Pj0 - L0) * ts;
         const double X1 = L1 + (Pj1 - L1) * ts;
-        if (!d_point_internal(F, X0, X1)) continue;
+        if (!d_point_internal(pic, X0, X1)) continue;
         double dist = 0.0;
         {
             const double d0 = X0 - L0, d1 = X1 - L1, d2 = 0.0 - L2;
@@ -831,6 +820,7 @@ __device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __
 // 256-direction quadrature keeps every lane busy.  FP64 in both precisions.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_boundary_find(DP P, const DFace* __restrict__ faces,
+                                                       const int* __restrict__ gnear,
                                                        const int* __restrict__ gcount,
                                                        const int* __restrict__ gstart,
                                                        const int* __restrict__ gvec,
@@ -842,6 +832,10 @@ __global__ __launch_bounds__(256) void k_boundary_find(DP P, const DFace* __rest
     const double4 x = xm[i];
     const double xc[3] = {x.x, x.y, x.z};
     const int nc = d_particle_cell(P, x.x, x.y, x.z);
+    if (nc <= 0 || gnear[nc - 1] == 0) {  // no frontier face listed within reach
+        bn[i] = 0;
+        return;
+    }
     int ncbf = 0;
     int clob[SPH_MB];
     double lz[SPH_MB];

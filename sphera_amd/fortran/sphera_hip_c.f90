@@ -160,6 +160,21 @@ interface
       import :: c_int, c_long_long
       integer(c_long_long),intent(out) :: free_bytes, total_bytes
    end function
+   integer(c_int) function sph_ctx_bytes(p,bytes) bind(C,name="sph_ctx_bytes")
+      import :: c_int, c_long_long, sph_params_t
+      type(sph_params_t),intent(in) :: p
+      integer(c_long_long),intent(out) :: bytes
+   end function
+   integer(c_int) function sph_get_launch(ctx,out) bind(C,name="sph_get_launch")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr),value :: ctx
+      integer(c_int32_t),intent(out) :: out(3)
+   end function
+   integer(c_int) function sph_set_timing(ctx,on) bind(C,name="sph_set_timing")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr),value :: ctx
+      integer(c_int32_t),value :: on
+   end function
    integer(c_int) function sph_sync(ctx) bind(C,name="sph_sync")
       import :: c_int, c_ptr
       type(c_ptr),value :: ctx
