@@ -91,6 +91,10 @@ int sph_state_download(sph_ctx* ctx, double* coord, double* vel, double* acc, do
                        double* pres, double* dden, double* vpres, int32_t* ids);
 /* dt, dt_average, simulation_time, it, pair count of the current lists */
 int sph_get_scalars(sph_ctx* ctx, double out[5]);
+/* Fluid global quantities (fluid_global_quantities.f90:66-72): mass, linear
+ * momentum (3), angular momentum about the origin (3), volume sum m/rho;
+ * compensated FP64 sums (exact sum of the per-particle terms, rounded once). */
+int sph_global_quantities(sph_ctx* ctx, double out[8]);
 /* Running total of pair interactions, sum over every list built so far of
  * sum_i (nPartIntorno(i) - 1) (the unit of the pair-interactions/s metric). */
 int sph_pair_counter(sph_ctx* ctx, unsigned long long* total);

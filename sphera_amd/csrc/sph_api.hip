@@ -696,6 +696,25 @@ int sph_get_scalars(sph_ctx* c, double out[5]) {
     return 0;
 }
 
+int sph_global_quantities(sph_ctx* c, double out[8]) {
+    if (!c || !out) return fail(1, "null argument");
+    if (c->phase != PH_SORTED && c->phase != PH_READY && c->phase != PH_PS)
+        return fail(121, "global quantities are taken at a time-step boundary");
+    const int nb = 2 * c->bt_grid / 4;  // two workgroups per CU
+    double* part = nullptr;
+    double* dout = nullptr;
+    HIPC(hipMalloc(&part, sizeof(double) * 2 * GQ_N * nb));
+    HIPC(hipMalloc(&dout, sizeof(double) * GQ_N));
+    k_global_partial<<<nb, 256, 0, c->stream>>>(c->xm[0], c->vel[0], c->rp[0], part, c->iscal);
+    k_global_final<<<1, 64, 0, c->stream>>>(part, nb, dout);
+    hipError_t e = hipMemcpyAsync(out, dout, sizeof(double) * GQ_N, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(part);
+    hipFree(dout);
+    if (e != hipSuccess) return fail(100, std::string("global quantities: ") + hipGetErrorString(e));
+    return 0;
+}
+
 int sph_pair_counter(sph_ctx* c, unsigned long long* total) {
     if (!c || !total) return fail(1, "null argument");
     HIPC(hipStreamSynchronize(c->stream));

@@ -923,6 +923,78 @@ __global__ __launch_bounds__(256) void k_boundary_int(DP P, const DFace* __restr
 }
 
 // ----------------------------------------------------------------------------
+// Global quantities of the fluid (fluid_global_quantities.f90:66-72: mass,
+// linear momentum, angular momentum about the origin, volume) as compensated
+// FP64 sums: every partial sum is a double-double (s + c, Knuth's TwoSum on
+// each addition), so the result is the exact sum of the terms rounded once,
+// whatever the reduction order.
+// ----------------------------------------------------------------------------
+struct DD {
+    double s, c;
+};
+__device__ __forceinline__ DD dd_add(DD x, DD y) {
+    const double s = x.s + y.s;
+    const double bp = s - x.s;
+    const double e = (x.s - (s - bp)) + (y.s - bp);
+    const double c = e + (x.c + y.c);
+    const double s2 = s + c;  // renormalise (|s| >= |c|)
+    return DD{s2, c - (s2 - s)};
+}
+__device__ __forceinline__ DD dd_shfl_xor(DD v, int o) {
+    return DD{__shfl_xor(v.s, o, 64), __shfl_xor(v.c, o, 64)};
+}
+
+constexpr int GQ_N = 8;  // mass, lin mom x3, ang mom x3, volume
+
+__global__ __launch_bounds__(256) void k_global_partial(const double4* __restrict__ xm,
+                                                        const double4* __restrict__ vel,
+                                                        const double2* __restrict__ rp,
+                                                        double* __restrict__ part, const int* iscal) {
+    __shared__ double red[4][GQ_N][2];
+    const int n = iscal[IS_N];
+    DD acc[GQ_N];
+    for (int q = 0; q < GQ_N; ++q) acc[q] = DD{0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double4 x = xm[i];
+        const double4 v = vel[i];
+        const double m = x.w;
+        // Vector_Product(coord, vel)
+        const double a0 = x.y * v.z - x.z * v.y;
+        const double a1 = x.z * v.x - x.x * v.z;
+        const double a2 = x.x * v.y - x.y * v.x;
+        const double t[GQ_N] = {m, m * v.x, m * v.y, m * v.z, m * a0, m * a1, m * a2, m / rp[i].x};
+#pragma unroll
+        for (int q = 0; q < GQ_N; ++q) acc[q] = dd_add(acc[q], DD{t[q], 0.0});
+    }
+#pragma unroll
+    for (int q = 0; q < GQ_N; ++q)
+        for (int o = 32; o > 0; o >>= 1) acc[q] = dd_add(acc[q], dd_shfl_xor(acc[q], o));
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < GQ_N; ++q) {
+            red[wid][q][0] = acc[q].s;
+            red[wid][q][1] = acc[q].c;
+        }
+    __syncthreads();
+    if (threadIdx.x < GQ_N) {
+        const int q = threadIdx.x;
+        DD r = DD{red[0][q][0], red[0][q][1]};
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = dd_add(r, DD{red[w][q][0], red[w][q][1]});
+        part[(size_t)blockIdx.x * GQ_N * 2 + 2 * q] = r.s;
+        part[(size_t)blockIdx.x * GQ_N * 2 + 2 * q + 1] = r.c;
+    }
+}
+
+__global__ void k_global_final(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
+    const int q = threadIdx.x;
+    if (q >= GQ_N) return;
+    DD r = DD{0.0, 0.0};
+    for (int b = 0; b < nblocks; ++b)
+        r = dd_add(r, DD{part[(size_t)b * GQ_N * 2 + 2 * q], part[(size_t)b * GQ_N * 2 + 2 * q + 1]});
+    out[q] = r.s + r.c;
+}
+
+// ----------------------------------------------------------------------------
 // dt: time_step_duration.f90:74-116.  Positive doubles order like their bit
 // patterns, so the minimum over particles is an exact atomicMin on uint64.
 // ----------------------------------------------------------------------------

@@ -24,7 +24,7 @@ PI = C.POINTER(C.c_int32)
 
 EXPORTS = [
     "sph_ctx_create", "sph_ctx_destroy", "sph_state_upload", "sph_particle_count",
-    "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_get_neighbour_lists", "sph_dt", "sph_momentum",
+    "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_global_quantities", "sph_get_neighbour_lists", "sph_dt", "sph_momentum",
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
     "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
     "sph_preloop", "sph_step", "sph_step_async", "sph_step_graph", "sph_sync", "sph_get_launch", "sph_set_timing",
@@ -76,6 +76,7 @@ def load():
     L.sph_get_scalars.argtypes = [C.c_void_p, PD]
     L.sph_get_neighbour_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
     L.sph_pair_counter.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    L.sph_global_quantities.argtypes = [C.c_void_p, PD]
     for ph in PHASES + ["preloop", "sync"]:
         getattr(L, "sph_" + ph).argtypes = [C.c_void_p]
     L.sph_step.argtypes = [C.c_void_p, C.c_int32]
@@ -275,6 +276,12 @@ class SphContext:
 
     def sync(self):
         return self.phase("sync")
+
+    def global_quantities(self):
+        """mass, linear momentum (3), angular momentum (3), volume."""
+        v = np.zeros(8)
+        self._check(self.L.sph_global_quantities(self.h, _pd(v)), "sph_global_quantities")
+        return dict(mass=v[0], lin_mom=v[1:4].copy(), ang_mom=v[4:7].copy(), volume=v[7])
 
     def busy(self):
         """True while work enqueued on this context's stream is running."""
