@@ -451,6 +451,21 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
 // |x - corner| <= dcd, so the FP32 separation is within ~5e-7 dcd per axis and
 // the FP32 r^2 within ~1.5e-6 relative of (2h)^2 near the cut-off.)
 // ----------------------------------------------------------------------------
+// List writer of one lane: the next entry's address advances by one inside a
+// group of NB_G entries and jumps to the particle's next group after it.
+struct NbOut {
+    uint32_t* wp;
+    int cnt;
+    size_t jump;  // N*NB_G - (NB_G-1): from the last entry of a group to the first of the next
+    __device__ __forceinline__ NbOut(uint32_t* nbr, size_t N, int p)
+        : wp(nbr + nb_index(N, p, 0)), cnt(0), jump(N * NB_G - (NB_G - 1)) {}
+    __device__ __forceinline__ void push(uint32_t e, int mj) {
+        if (cnt < mj) *wp = e;
+        ++cnt;
+        wp += ((cnt & (NB_G - 1)) == 0) ? jump : (size_t)1;
+    }
+};
+
 // The neighbour test of one candidate and the list append: FP32 on
 // cell-relative positions, and the reference's FP64 expression on the FP64
 // positions inside the guard band.  xo: FP32 position of particle p relative
@@ -462,14 +477,14 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
 // its list, CalcVarLength.f90:200-207), so the loops have no early exit.
 __device__ __forceinline__ void nb_test(const DP& P, const double4* __restrict__ xm, double4 xi64,
                                         float4 xo, float4 xj, int m, uint32_t ctag, float lo, float hi,
-                                        int& cnt, uint32_t* __restrict__ nbr, size_t N, int p, int mj) {
+                                        NbOut& w, int mj) {
     const float r0 = xo.x - xj.x;
     const float r1 = xo.y - xj.y;
     const float r2 = xo.z - xj.z;
     const float r2f = __builtin_fmaf(r0, r0, __builtin_fmaf(r1, r1, r2 * r2));
     if (r2f <= hi) {
-        bool acc = true;
-        if (r2f >= lo) {
+        bool acc = r2f < lo;
+        if (!acc) {  // guard band: the reference's FP64 test
             const double4 xj64 = xm[m];
             const double d0 = xi64.x - xj64.x;
             const double d1 = xi64.y - xj64.y;
@@ -477,30 +492,26 @@ __device__ __forceinline__ void nb_test(const DP& P, const double4* __restrict__
             const double rij = d0 * d0 + d1 * d1 + d2 * d2;
             acc = !(rij > P.square_doubleh);
         }
-        if (acc) {
-            if (cnt < mj) nbr[nb_index(N, p, cnt)] = (uint32_t)m | ctag;
-            ++cnt;
-        }
+        if (acc) w.push((uint32_t)m | ctag, mj);
     }
 }
 
 // Candidates [b,e) of one stencil cell, four loads ahead: load(x) gives the
 // FP32 position of slot x, gl + x its sorted index.
 template <class Load>
-__device__ __forceinline__ void nb_cell(const DP& P, int p, const double4* __restrict__ xm, double4 xi64,
+__device__ __forceinline__ void nb_cell(const DP& P, const double4* __restrict__ xm, double4 xi64,
                                         float4 xi, float4 o, uint32_t ctag, int b, int e, int gl,
-                                        Load&& load, float lo, float hi, int& cnt,
-                                        uint32_t* __restrict__ nbr, size_t N, int mj) {
+                                        Load&& load, float lo, float hi, NbOut& w, int mj) {
     const float4 xo = make_float4(xi.x - o.x, xi.y - o.y, xi.z - o.z, 0.0f);
     int x = b;
     for (; x + 4 <= e; x += 4) {
         const float4 q0 = load(x), q1 = load(x + 1), q2 = load(x + 2), q3 = load(x + 3);
-        nb_test(P, xm, xi64, xo, q0, gl + x, ctag, lo, hi, cnt, nbr, N, p, mj);
-        nb_test(P, xm, xi64, xo, q1, gl + x + 1, ctag, lo, hi, cnt, nbr, N, p, mj);
-        nb_test(P, xm, xi64, xo, q2, gl + x + 2, ctag, lo, hi, cnt, nbr, N, p, mj);
-        nb_test(P, xm, xi64, xo, q3, gl + x + 3, ctag, lo, hi, cnt, nbr, N, p, mj);
+        nb_test(P, xm, xi64, xo, q0, gl + x, ctag, lo, hi, w, mj);
+        nb_test(P, xm, xi64, xo, q1, gl + x + 1, ctag, lo, hi, w, mj);
+        nb_test(P, xm, xi64, xo, q2, gl + x + 2, ctag, lo, hi, w, mj);
+        nb_test(P, xm, xi64, xo, q3, gl + x + 3, ctag, lo, hi, w, mj);
     }
-    for (; x < e; ++x) nb_test(P, xm, xi64, xo, load(x), gl + x, ctag, lo, hi, cnt, nbr, N, p, mj);
+    for (; x < e; ++x) nb_test(P, xm, xi64, xo, load(x), gl + x, ctag, lo, hi, w, mj);
 }
 
 // List of particle p from global memory (candidates gathered through L1/L2).
@@ -515,7 +526,7 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
     d_cell_indices(P, cell[p], ic, jc, kc);
     const float4 xi = xmf[p];
     const double4 xi64 = xm[p];
-    int cnt = 0;
+    NbOut w(nbr, N, p);
     auto load = [&](int x) { return xmf[x]; };
     for (int dy = -1; dy <= 1; ++dy) {
         const int jj = jc + dy;
@@ -528,13 +539,13 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
                 if (kk < 1 || kk > nk) continue;
                 const int cn = ((kk - 1) * nj + (jj - 1)) * ni + ii;
                 const int code = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-                nb_cell(P, p, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT, start[cn - 1],
-                        start[cn], 0, load, lo, hi, cnt, nbr, N, mj);
+                nb_cell(P, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT, start[cn - 1],
+                        start[cn], 0, load, lo, hi, w, mj);
             }
         }
     }
-    ov = cnt > mj;
-    return min(cnt, mj);
+    ov = w.cnt > mj;
+    return min(w.cnt, mj);
 }
 
 // Tiled form.  The particles of one row of cells (fixed j,k; cells i are
@@ -693,19 +704,19 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                 const int ucell = cell[p] - r * ni - ia + 1;  // u index of the own cell
                 const float4 xi = xmf[p];
                 const double4 xi64 = xm[p];
-                int c = 0;
+                NbOut w(nbr, N, p);
                 auto load = [&](int x) { return tile[x]; };
                 for (int dy = 0; dy < 3; ++dy)
                     for (int dx = 0; dx < 3; ++dx)
                         for (int dz = 0; dz < 3; ++dz) {
                             const int t = 3 * dy + dz;
                             const int code = dx + 3 * dy + 9 * dz;
-                            nb_cell(P, p, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT,
+                            nb_cell(P, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT,
                                     bnd[t][ucell + dx - 1], bnd[t][ucell + dx], seg_g[t] - seg_l[t], load,
-                                    lo, hi, c, nbr, N, mj);
+                                    lo, hi, w, mj);
                         }
-                const bool ov = c > mj;
-                const int cnt = min(c, mj);
+                const bool ov = w.cnt > mj;
+                const int cnt = min(w.cnt, mj);
                 nnb[p] = cnt;
                 mypairs += (unsigned long long)(cnt - 1);
                 if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
