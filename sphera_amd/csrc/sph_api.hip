@@ -57,7 +57,8 @@ struct sph_ctx {
     double4 *xm[2] = {}, *vel[3] = {}, *acc[2] = {};
     double2* rp[3] = {};
     int* lam[2] = {};
-    float4 *xmf = nullptr, *velf[3] = {};
+    float4* xmf = nullptr;  // FP32 cell-relative positions (x, y, z, mass)
+    float4* rec[3] = {};  // FP32 record buffers: (position, velocity) float4 pairs per particle
     float2* rpf[3] = {};
     double *dden = nullptr, *vpres = nullptr;
     int *cell = nullptr, *key = nullptr, *slot = nullptr, *cnt = nullptr, *start = nullptr,
@@ -152,6 +153,11 @@ int collect_timing(sph_ctx* c) {
 }
 
 // ---- device steps --------------------------------------------------------
+// position / velocity halves of FP32 record buffer k
+inline F4In XF(const sph_ctx* c, int k) { return F4In{c->rec[k]}; }
+inline F4In VF(const sph_ctx* c, int k) { return F4In{c->rec[k] + 1}; }
+inline F4Out XFo(const sph_ctx* c, int k) { return F4Out{c->rec[k]}; }
+inline F4Out VFo(const sph_ctx* c, int k) { return F4Out{c->rec[k] + 1}; }
 void scan_ints(sph_ctx* c, const int* in, int* out, int* bs, int m, int* total) {
     const int nb = (m + 1023) / 1024;
     k_scan_local<<<nb, 256, 0, c->stream>>>(in, out, bs, m);
@@ -186,7 +192,7 @@ int run_sort(sph_ctx* c) {
         KTime t(c, "sort_gather");
         k_gather<<<nblk(cap), 256, 0, c->stream>>>(P, c->perm, c->key, c->xm[1], c->vel[2], c->rp[0],
                                                    c->lam[0], c->acc[1], c->xm[0], c->vel[0], c->rp[1],
-                                                   c->lam[1], c->acc[0], c->cell, c->xmf, c->velf[0],
+                                                   c->lam[1], c->acc[0], c->cell, c->xmf, XFo(c, 0), XFo(c, 1), XFo(c, 2), VFo(c, 0),
                                                    c->rpf[1], c->start, c->iscal);
     }
     return 0;
@@ -229,16 +235,16 @@ int run_momentum(sph_ctx* c) {
         {
             auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
-            c->vel[1], c->velf[1], c->scal, c->iscal); };
+            c->rp[0], XF(c, 0), VF(c, 0), c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
+            c->vel[1], VFo(c, 1), c->scal, c->iscal); };
             if (c->chunk == 4) L(k_momentum<true, 4>);
             else L(k_momentum<true, 8>);
         }
     else
         k_momentum<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[0], c->xmf, c->velf[0], c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
-            c->vel[1], c->velf[1], c->scal, c->iscal);
+            c->rp[0], XF(c, 0), VF(c, 0), c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
+            c->vel[1], VFo(c, 1), c->scal, c->iscal);
     return 0;
 }
 
@@ -248,7 +254,7 @@ int run_vsmooth(sph_ctx* c) {
         {
             auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
-            c->rp[0], c->xmf, c->velf[1], c->nbr, c->nnb, c->vel[2], c->velf[2], c->xm[1], c->scal,
+            c->rp[0], XF(c, 1), VF(c, 1), c->nbr, c->nnb, c->vel[2], VFo(c, 2), c->xm[1], c->scal,
             c->dtmin, c->iscal); };
             if (c->chunk == 4) L(k_vsmooth<true, 4>);
             else L(k_vsmooth<true, 8>);
@@ -256,7 +262,7 @@ int run_vsmooth(sph_ctx* c) {
     else
         k_vsmooth<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
-            c->rp[0], c->xmf, c->velf[1], c->nbr, c->nnb, c->vel[2], c->velf[2], c->xm[1], c->scal,
+            c->rp[0], XF(c, 1), VF(c, 1), c->nbr, c->nnb, c->vel[2], VFo(c, 2), c->xm[1], c->scal,
             c->dtmin, c->iscal);
     return 0;
 }
@@ -267,14 +273,14 @@ int run_continuity(sph_ctx* c) {
         {
             auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[1], c->xmf, c->velf[0], c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal); };
+            c->rp[1], XF(c, 0), VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal); };
             if (c->chunk == 4) L(k_continuity<true, 4>);
             else L(k_continuity<true, 8>);
         }
     else
         k_continuity<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[1], c->xmf, c->velf[0], c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
+            c->rp[1], XF(c, 0), VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
     return 0;
 }
 
@@ -330,7 +336,7 @@ sph_ctx::~sph_ctx() {
     for (auto p : rp) F(p);
     for (auto p : lam) F(p);
     F(xmf);
-    for (auto p : velf) F(p);
+    for (auto p : rec) F(p);
     for (auto p : rpf) F(p);
     F(dden); F(vpres); F(rowblk); F(idslot); F(rmids); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
     F(work); F(nwork);
@@ -349,7 +355,7 @@ int sph_ctx::alloc() {
     for (auto& p : rp) HIPC(da(&p, N));
     for (auto& p : lam) HIPC(da(&p, N));
     HIPC(da(&xmf, N));
-    for (auto& p : velf) HIPC(da(&p, P.f32 ? N : 1));
+    for (auto& r : rec) HIPC(da(&r, P.f32 ? 2 * N : 1));
     for (auto& p : rpf) HIPC(da(&p, P.f32 ? N : 1));
     HIPC(da(&dden, N));
     HIPC(da(&vpres, N));

@@ -26,6 +26,20 @@
 
 namespace sph {
 
+// FP32 particle records of the passes that read velocities are interleaved:
+// slot i of a record buffer holds the cell-relative position (x, y, z, mass)
+// at float4 2i and (vx, vy, vz, rho) at 2i+1, so a neighbour's position and
+// velocity come from one 32-byte run.  F4View is the strided view of one of
+// the two halves.  The neighbour search and the pressure smoothing read the
+// positions alone from the compact array xmf.
+template <class T>
+struct F4View {
+    T* p;
+    __host__ __device__ __forceinline__ T& operator[](size_t i) const { return p[2 * i]; }
+};
+using F4In = F4View<const float4>;
+using F4Out = F4View<float4>;
+
 // ----------------------------------------------------------------------------
 // grid helpers: CellNumber.f90:53-59, ParticleCellNumber.f90:52-62,
 // CellIndices.f90:51-63 (1-based cell ids, 0 = outside)
@@ -409,7 +423,8 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
                          const double4* __restrict__ acc_s, double4* __restrict__ xm_d,
                          double4* __restrict__ vel_d, double2* __restrict__ rp_d,
                          int* __restrict__ lam_d, double4* __restrict__ acc_d,
-                         int* __restrict__ cell_d, float4* __restrict__ xmf, float4* __restrict__ velf,
+                         int* __restrict__ cell_d, float4* __restrict__ xmf, F4Out xmf0, F4Out xmf1, F4Out xmf2,
+                         F4Out velf,
                          float2* __restrict__ rpf, const int* __restrict__ start, int* iscal) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int nin = start[P.nmax];  // particles still inside the grid
@@ -434,8 +449,12 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
     const double ox = P.extr_lo[0] + P.dcd[0] * (double)(ci - 1);
     const double oy = P.extr_lo[1] + P.dcd[1] * (double)(cj - 1);
     const double oz = P.extr_lo[2] + P.dcd[2] * (double)(ck - 1);
-    xmf[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
-    if (P.f32) {
+    const float4 xr = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
+    xmf[p] = xr;
+    if (P.f32) {  // the position half of every FP32 record buffer
+        xmf0[p] = xr;
+        xmf1[p] = xr;
+        xmf2[p] = xr;
         velf[p] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)r.x);
         rpf[p] = make_float2((float)r.x, (float)r.y);
     }
@@ -1227,23 +1246,12 @@ template <bool F32, int CH>
 __global__ __launch_bounds__(256) void k_momentum(
     DP P, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
-    const double4* __restrict__ vel, const double2* __restrict__ rp, const float4* __restrict__ xmf,
-    const float4* __restrict__ velf, const float2* __restrict__ rpf, const int* __restrict__ lam_in,
+    const double4* __restrict__ vel, const double2* __restrict__ rp, F4In xmf,
+    F4In velf, const float2* __restrict__ rpf, const int* __restrict__ lam_in,
     int* __restrict__ lam_out, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
-    double4* __restrict__ acc_out, double4* __restrict__ vel_out, float4* __restrict__ velf_out,
+    double4* __restrict__ acc_out, double4* __restrict__ vel_out, F4Out velf_out,
     const double* scal, int* iscal) {
-    using X = typename Rec<F32>::X;
-    using V = typename Rec<F32>::V;
-    using RPT = typename Rec<F32>::RP;
     stencil_offsets(P);
-    const X* gx;
-    const V* gv;
-    const RPT* gr;
-    if constexpr (F32) {
-        gx = xmf; gv = velf; gr = rpf;
-    } else {
-        gx = xm; gv = vel; gr = rp;
-    }
     const double h = P.h;
     const double dtvel = scal[SC_DTVEL];
     const size_t N = P.cap;
@@ -1269,7 +1277,7 @@ __global__ __launch_bounds__(256) void k_momentum(
                 using RM = NRec<double4, double4, double2>;
                 auto fetch = [&](uint32_t e) {
                     const int j = (int)(e & NB_MASK);
-                    return RM{gx[j], gv[j], gr[j]};
+                    return RM{xm[j], vel[j], rp[j]};
                 };
                 walk_list<4, RM>(nbr, N, i, nn, fetch, [&](uint32_t e, const RM& q) {
                     if ((int)(e & NB_MASK) == i) return;
@@ -1325,7 +1333,7 @@ __global__ __launch_bounds__(256) void k_momentum(
                 using RM = NRec<float4, float4, float2>;
                 auto fetch = [&](uint32_t e) {
                     const int j = (int)(e & NB_MASK);
-                    return RM{gx[j], gv[j], gr[j]};
+                    return RM{xmf[j], velf[j], rpf[j]};
                 };
                 walk_list<CH, RM>(nbr, N, i, nn, fetch, [&](uint32_t e, const RM& q) {
                     if ((int)(e & NB_MASK) == i) return;
@@ -1408,9 +1416,9 @@ template <bool F32, int CH>
 __global__ __launch_bounds__(256) void k_vsmooth(
     DP P, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
-    const double4* __restrict__ vel1, const double2* __restrict__ rp, const float4* __restrict__ xmf,
-    const float4* __restrict__ velf1, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
-    double4* __restrict__ vel_out, float4* __restrict__ velf_out, double4* __restrict__ xm_out,
+    const double4* __restrict__ vel1, const double2* __restrict__ rp, F4In xmf,
+    F4In velf1, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
+    double4* __restrict__ vel_out, F4Out velf_out, double4* __restrict__ xm_out,
     const double* scal, unsigned long long* dtmin, int* iscal) {
     using X = typename Rec<F32>::X;
     using V = typename Rec<F32>::V;
@@ -1522,8 +1530,8 @@ template <bool F32, int CH>
 __global__ __launch_bounds__(256) void k_continuity(
     DP P, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
-    const double4* __restrict__ vel, const double2* __restrict__ rp, const float4* __restrict__ xmf,
-    const float4* __restrict__ velf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
+    const double4* __restrict__ vel, const double2* __restrict__ rp, F4In xmf,
+    F4In velf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
     double* __restrict__ dden_out, double2* __restrict__ rp_out, float2* __restrict__ rpf_out,
     const double* scal, int* iscal) {
     using X = typename Rec<F32>::X;
