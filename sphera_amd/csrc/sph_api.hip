@@ -519,8 +519,13 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
             return fail(2, "chunk must be 4 or 8");
         }
         c->bt_grid = 4 * ncu;
-        // three neighbour-search workgroups fit a CU (LDS), a multiple of 8 XCDs
-        c->nb_grid = (3 * ncu + 7) / 8 * 8;
+        // persistent neighbour-search workgroups: as many as fit a CU (LDS
+        // tile), times the CUs, a multiple of the 8 XCDs
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_neighbours, NB_WG, 0) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 3;
+        c->nb_grid = (per_cu * ncu + 7) / 8 * 8;
     }
     if ((rc = c->alloc())) {
         delete c;

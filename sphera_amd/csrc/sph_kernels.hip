@@ -614,7 +614,8 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                                                       const int* __restrict__ rowblk,
                                                       uint32_t* __restrict__ nbr, int* __restrict__ nnb,
                                                       int* iscal, unsigned long long* pairs) {
-    __shared__ float4 tile[NB_TILE];
+    __shared__ float2 tile_xy[NB_TILE];  // staged candidates: (x, y) and z, 12 bytes each
+    __shared__ float tile_z[NB_TILE];
     __shared__ int bnd[9][NB_MAXC + 4];  // LDS offset of the start of cell ia-1+u of row t
     __shared__ int seg_g[9];             // sorted index of a segment's first particle
     __shared__ int seg_l[9];             // its LDS offset
@@ -715,7 +716,10 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
-                        if (dst[u] >= 0) tile[dst[u]] = v[u];
+                        if (dst[u] >= 0) {
+                            tile_xy[dst[u]] = make_float2(v[u].x, v[u].y);
+                            tile_z[dst[u]] = v[u].z;
+                        }
                 }
             }
             __syncthreads();
@@ -724,7 +728,10 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                 const float4 xi = xmf[p];
                 const double4 xi64 = xm[p];
                 NbOut w(nbr, N, p);
-                auto load = [&](int x) { return tile[x]; };
+                auto load = [&](int x) {
+                    const float2 a = tile_xy[x];
+                    return make_float4(a.x, a.y, tile_z[x], 0.0f);
+                };
                 for (int dy = 0; dy < 3; ++dy)
                     for (int dx = 0; dx < 3; ++dx)
                         for (int dz = 0; dz < 3; ++dz) {
