@@ -284,7 +284,7 @@ def main():
                                f"dx={case.dx} m, h/dx=1.3, SA-SPH walls", "particles": n,
                    "grid_cells": int(tb.grid.nmax)},
         "fp64": fp64,
-        "launch": dict(main_r["launch"], graph=not args.no_graph),
+        "launch": dict(main_r["launch"], graph=not args.no_graph, tune=tunes.get(main_name)),
         "mixed_over_fp64": value / fp64["value"] if fp64 else None,
         "pair_kernels_only_value": main_r["pairs"] / pair_time if pair_time > 0 else None,
         "kernel_ms": kern,

@@ -49,8 +49,10 @@ typedef struct sph_params {
     /* launch tunables of the particle kernels (0 = default; the autotuner in
      * sphera_amd/autotune.py picks them per case) */
     int32_t wg_size;     /* threads per workgroup: 64 (default), 128, 192 or 256 */
-    int32_t chunk;       /* list entries gathered per step of the FP32 walks: 4 or 8 (default) */
-    int32_t reserved[3];
+    int32_t chunk;       /* list entries gathered per step of the FP32 walks: 4 or 8 (default);
+                            momentum pass, and the others where theirs is 0 */
+    int32_t chunk_v, chunk_c, chunk_p; /* the same for velocity smoothing, continuity,
+                                          pressure smoothing (0 = chunk) */
 } sph_params;
 
 /* SA-SPH frontier faces in BFaceList order and the grid-cell/face incidence

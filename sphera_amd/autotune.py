@@ -1,22 +1,23 @@
 """Launch-shape autotuner for the particle kernels.
 
-For one case and precision it times a few whole time steps for each
-workgroup size (and, for FP32-mixed, each list-walk chunk) on a scratch
-context and returns the fastest as the ``tune`` dict that ``SphContext`` /
-``sph_params`` accept.  The chunk trades registers (occupancy) against loads
-in flight per lane.
+For one case and precision it picks the workgroup size of the particle
+kernels and, for FP32-mixed, the list-walk chunk of each pair pass (4 or 8
+entries gathered per step: fewer registers, more waves in flight, or more
+loads in flight per lane).  For each workgroup size two scratch contexts run
+a few steps with every pass at chunk 4 and at chunk 8, with per-kernel HIP
+event timing; each pass takes its faster chunk, and the workgroup size with
+the smallest resulting step wins.  The result is the ``tune`` dict that
+``SphContext`` / ``sph_params`` accept.
 """
 from __future__ import annotations
 
 import time
 
-
-def candidates(tb, precision):
-    chunks = (4, 8) if precision == 1 else (8,)
-    return [dict(wg_size=w, chunk=c) for w in (64, 128, 256) for c in chunks]
+PASSES = {"momentum": "chunk", "velocity_smoothing": "chunk_v", "continuity": "chunk_c",
+          "pressure_smoothing": "chunk_p"}
 
 
-def time_config(tb, coord, precision, tune, steps=4, warmup=2):
+def _run(tb, coord, precision, tune, steps, warmup):
     from . import runtime as rt
     d = rt.SphContext(tb, precision=precision, tune=tune)
     try:
@@ -27,23 +28,36 @@ def time_config(tb, coord, precision, tune, steps=4, warmup=2):
         t0 = time.perf_counter()
         d.step(steps)
         t1 = time.perf_counter()
-        return (t1 - t0) / steps
+        d.set_timing(True)
+        d.step(steps)
+        ms = {k: v[0] / v[1] for k, v in d.timing().items()}
+        return (t1 - t0) / steps, ms
     finally:
         d.close()
 
 
-def autotune(tb, coord, precision, steps=4, log=None):
+def autotune(tb, coord, precision, steps=4, warmup=2, log=None):
     """Returns (best tune dict, list of (tune, seconds per step))."""
     results = []
-    for cfg in candidates(tb, precision):
-        try:
-            sec = time_config(tb, coord, precision, cfg, steps=steps)
-        except Exception as e:  # a shape the device rejects is skipped
-            if log:
-                log(f"[autotune] {cfg}: rejected ({e})")
-            continue
-        results.append((cfg, sec))
+    best, best_s = None, float("inf")
+    for wg in (64, 128, 256):
+        if precision == 1:  # FP32-mixed: per-pass chunk from the kernel times
+            s8, ms8 = _run(tb, coord, precision, dict(wg_size=wg, chunk=8), steps, warmup)
+            s4, ms4 = _run(tb, coord, precision, dict(wg_size=wg, chunk=4), steps, warmup)
+            tune = dict(wg_size=wg)
+            est = s8
+            for k, field in PASSES.items():
+                c = 4 if ms4.get(k, 1e9) < ms8.get(k, 1e9) else 8
+                tune[field] = c
+                if c == 4:
+                    est -= (ms8[k] - ms4[k]) / 1e3
+            results += [(dict(wg_size=wg, chunk=8), s8), (dict(wg_size=wg, chunk=4), s4)]
+        else:
+            tune = dict(wg_size=wg, chunk=8)
+            est, _ = _run(tb, coord, precision, tune, steps, warmup)
+            results.append((tune, est))
         if log:
-            log(f"[autotune] precision={precision} {cfg} -> {sec * 1e3:.3f} ms/step")
-    best = min(results, key=lambda r: r[1])[0]
+            log(f"[autotune] precision={precision} {tune} -> {est * 1e3:.3f} ms/step")
+        if est < best_s:
+            best, best_s = tune, est
     return best, results

@@ -69,6 +69,7 @@ struct sph_ctx {
     int nb_grid = 8;                        // persistent workgroups of the neighbour search
     int wg = 256, grid = 1, bt_grid = 1024;  // launch shape of the particle kernels
     int chunk = 8;                            // list entries gathered per step of the FP32 walks
+    int chunk_v = 8, chunk_c = 8, chunk_p = 8;  // the same for smoothing, continuity, pressure smoothing
     std::vector<std::pair<int, hipGraphExec_t>> graphs;  // captured n-step sequences
     uint32_t* nbr = nullptr;
     int* nnb = nullptr;
@@ -256,7 +257,7 @@ int run_vsmooth(sph_ctx* c) {
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
             c->rp[0], XF(c, 1), VF(c, 1), c->nbr, c->nnb, c->vel[2], VFo(c, 2), c->xm[1], c->scal,
             c->dtmin, c->iscal); };
-            if (c->chunk == 4) L(k_vsmooth<true, 4>);
+            if (c->chunk_v == 4) L(k_vsmooth<true, 4>);
             else L(k_vsmooth<true, 8>);
         }
     else
@@ -274,7 +275,7 @@ int run_continuity(sph_ctx* c) {
             auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
             c->rp[1], XF(c, 0), VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal); };
-            if (c->chunk == 4) L(k_continuity<true, 4>);
+            if (c->chunk_c == 4) L(k_continuity<true, 4>);
             else L(k_continuity<true, 8>);
         }
     else
@@ -291,7 +292,7 @@ int run_psmooth(sph_ctx* c) {
             auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
             c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
             c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal); };
-            if (c->chunk == 4) L(k_psmooth<true, 4>);
+            if (c->chunk_p == 4) L(k_psmooth<true, 4>);
             else L(k_psmooth<true, 8>);
         }
     else
@@ -514,7 +515,11 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
         }
         c->grid = (P.cap + c->wg - 1) / c->wg;
         c->chunk = p->chunk > 0 ? p->chunk : 8;
-        if (c->chunk != 4 && c->chunk != 8) {
+        c->chunk_v = p->chunk_v > 0 ? p->chunk_v : c->chunk;
+        c->chunk_c = p->chunk_c > 0 ? p->chunk_c : c->chunk;
+        c->chunk_p = p->chunk_p > 0 ? p->chunk_p : c->chunk;
+        auto bad = [](int v) { return v != 4 && v != 8; };
+        if (bad(c->chunk) || bad(c->chunk_v) || bad(c->chunk_c) || bad(c->chunk_p)) {
             delete c;
             return fail(2, "chunk must be 4 or 8");
         }

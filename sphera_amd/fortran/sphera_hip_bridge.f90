@@ -96,7 +96,9 @@ p%semi = 0
 if (Domain%tipo=="semi") p%semi = 1
 p%wg_size = 0
 p%chunk = 0
-p%reserved(:) = 0
+p%chunk_v = 0
+p%chunk_c = 0
+p%chunk_p = 0
 end subroutine pack_params
 
 ! Frontier faces in BFaceList order; GCBFVector entries (BoundaryFace
