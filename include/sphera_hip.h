@@ -97,6 +97,20 @@ int sph_get_scalars(sph_ctx* ctx, double out[5]);
  * momentum (3), angular momentum about the origin (3), volume sum m/rho;
  * compensated FP64 sums (exact sum of the per-particle terms, rounded once). */
 int sph_global_quantities(sph_ctx* ctx, double out[8]);
+/* Monitoring lines of the free-surface monitor, as ReadInputControlLines.f90
+ * :97-124 stores them: icont[2*i], icont[2*i+1] = control_lines(i+1)%Icont
+ * (1-based first and last point), points[3*j..] = control_points(j+1)%coord.
+ * Copied to the device; nlines = 0 clears them.  Status 3 if a line's points
+ * are outside 1..npoints. */
+int sph_monitor_lines(sph_ctx* ctx, int32_t nlines, const int32_t* icont, int32_t npoints,
+                      const double* points);
+/* Free-surface monitor (calc_pelo.f90:82-131, cells as CalcVarp.f90:66-72):
+ * per line, pelolib[3*i..] = mean of the two particles closest to the point
+ * of the last filled cell along the line, + dx/2 on each coordinate;
+ * zlft[3*i..] = the same at the last filled cell below the first empty cell
+ * above fluid (z_lower_fluid_top).  0 where a line meets no fluid.  Taken at
+ * a time-step boundary (status 121 otherwise); blocks until copied. */
+int sph_free_surface(sph_ctx* ctx, double* pelolib, double* zlft);
 /* Running total of pair interactions, sum over every list built so far of
  * sum_i (nPartIntorno(i) - 1) (the unit of the pair-interactions/s metric). */
 int sph_pair_counter(sph_ctx* ctx, unsigned long long* total);

@@ -69,7 +69,11 @@ def deck_text(case, itmax=1000000000, tmax=1000.0):
               "##### draw options #####", "vtkconverter any 100000.",
               "##### end draw options #####",
               "##### control points #####", "##### end control points #####",
-              "##### control lines #####", "##### end control lines #####",
+              "##### control lines #####"]
+    for label, first, second, ndiv in case.control_lines:
+        lines += [str(label)[:8], " ".join(_f(v) for v in first), " ".join(_f(v) for v in second),
+                  str(int(ndiv))]
+    lines += ["##### end control lines #####",
               "##### section flow rate #####", "0 999. 1", "##### end section flow rate #####",
               "##### substations #####", "0 999.", "##### end substations #####"]
     del s
@@ -116,6 +120,19 @@ def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
         raise RuntimeError(f"reference harness failed ({res.returncode}):\n{res.stdout[-3000:]}"
                            f"\n{res.stderr[-3000:]}")
     return wd / "dump.bin"
+
+
+def read_pelo(workdir, case):
+    """The reference's free-surface monitor files <case>.plb and <case>.zlft
+    (calc_pelo.f90:132-133: time, then pelolib(3, nlines) in g14.7 records of
+    30 values).  Returns (time (k,), pelolib (k, nlines, 3), zlft (k, nlines, 3))."""
+    nl = len(case.control_lines)
+    out = []
+    for ext in ("plb", "zlft"):
+        tok = (Path(workdir) / f"{case.name}.{ext}").read_text().split("\n", 1)[1].split()
+        v = np.array([float(t) for t in tok]).reshape(-1, 1 + 3 * nl)
+        out.append(v)
+    return out[0][:, 0], out[0][:, 1:].reshape(-1, nl, 3), out[1][:, 1:].reshape(-1, nl, 3)
 
 
 class _Reader:

@@ -63,6 +63,7 @@ def lib():
         L.or_get_scalars.argtypes = [C.c_void_p, PD]
         L.or_get_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
         L.or_get_btab.argtypes = [C.c_void_p, PI, PI, PD, PD, PD]
+        L.or_free_surface.argtypes = [C.c_void_p, C.c_int, PI, C.c_int, PD, PD, PD]
         _lib = L
     return _lib
 
@@ -194,6 +195,16 @@ class Oracle:
         ig = np.zeros((n, mc, 9))
         lib().or_get_btab(self.ctx, _pi(nc), _pi(face), _pd(loc), _pd(bi), _pd(ig))
         return nc, face, loc, bi, ig
+
+    def free_surface(self, icont, points):
+        """calc_pelo on the current cell lists: (pelolib, zlft), each (nlines, 3)."""
+        icont = np.ascontiguousarray(icont, np.int32).reshape(-1, 2)
+        points = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        pel = np.zeros((len(icont), 3))
+        zl = np.zeros((len(icont), 3))
+        self.call("free_surface", len(icont), _pi(icont), len(points), _pd(points), _pd(pel),
+                  _pd(zl))
+        return pel, zl
 
     def pair_count(self):
         return int(lib().or_pair_count(self.ctx))

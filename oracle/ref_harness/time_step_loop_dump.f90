@@ -18,6 +18,10 @@
 !                      (x,y,z per particle, pg order).  They replace the
 !                      generated positions before the pre-loop (jittered ICs).
 !   SPH_HARNESS_LISTS  0 to omit neighbour lists / boundary tables from dumps
+!   SPH_HARNESS_PELO   1: at every dumped step also run the reference's
+!                      free-surface monitor (CalcVarp, then calc_pelo, as
+!                      time_step_post_processing.f90:73,111 call them), which
+!                      appends to the case's .plb and .zlft files
 !
 ! File layout: harness_dump_io.f90 (write_header/write_step).
 subroutine time_step_loop
@@ -29,7 +33,7 @@ use Memory_I_O_interface_module
 use harness_dump_io
 implicit none
 logical :: BC_zmax_flag
-integer(4) :: it,npi,nsteps,every,u,ios,n_ic,i_zone,Ncbf_Max,lists
+integer(4) :: it,npi,nsteps,every,u,ios,n_ic,i_zone,Ncbf_Max,lists,pelo
 double precision :: dt_previous_step,dtvel
 double precision,allocatable :: xyz(:)
 character(len=512) :: fout,aux
@@ -58,6 +62,9 @@ if (len_trim(aux)>0) read(aux,*) every
 lists = 1
 call get_environment_variable("SPH_HARNESS_LISTS",aux)
 if (len_trim(aux)>0) read(aux,*) lists
+pelo = 0
+call get_environment_variable("SPH_HARNESS_PELO",aux)
+if (len_trim(aux)>0) read(aux,*) pelo
 ! Optional replacement of the particle positions (same pg order)
 call get_environment_variable("SPH_HARNESS_IC",aux)
 if (len_trim(aux)>0) then
@@ -92,6 +99,10 @@ if (Domain%NormFix) call NormFix
 if (it_start==0) call time_step_duration
 it = it_start
 call write_step(u,it,lists)
+if ((pelo==1).and.(nlines>0)) then
+   call CalcVarp
+   call calc_pelo
+endif
 do while (it<nsteps)
    it = it + 1
    on_going_time_step = it
@@ -146,7 +157,13 @@ do while (it<nsteps)
    call Leapfrog_continuity
    call EoS_fluid_loop
    if (input_any_t%TetaP>zero) call PressureSmoothing_3D
-   if (mod(it,every)==0) call write_step(u,it,lists)
+   if (mod(it,every)==0) then
+      call write_step(u,it,lists)
+      if ((pelo==1).and.(nlines>0)) then
+         call CalcVarp
+         call calc_pelo
+      endif
+   endif
    if (simulation_time>=input_any_t%tmax) exit
 enddo
 close(u)

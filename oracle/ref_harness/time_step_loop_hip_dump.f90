@@ -18,6 +18,10 @@
 !   SPH_HIP_PRECISION  0 FP64 (default), 1 FP32-mixed
 !   SPH_HIP_PHASES     1: call the per-phase entry points in the reference's
 !                      order instead of sph_step
+!   SPH_HARNESS_PELO   1: at every dumped step also run the free-surface
+!                      monitor on the device (sph_hip_calc_pelo), which
+!                      appends to the case's .plb and .zlft files as the
+!                      reference's calc_pelo does
 subroutine time_step_loop
 use iso_c_binding
 use I_O_file_module
@@ -29,7 +33,7 @@ use sphera_hip_c
 use sphera_hip_bridge
 use harness_dump_io
 implicit none
-integer(4) :: it,npi,nsteps,every,u,ios,n_ic,k,phases
+integer(4) :: it,npi,nsteps,every,u,ios,n_ic,k,phases,pelo
 integer(c_int32_t) :: prec
 double precision,allocatable :: xyz(:)
 character(len=512) :: fout,aux
@@ -58,6 +62,9 @@ if (len_trim(aux)>0) read(aux,*) prec
 phases = 0
 call get_environment_variable("SPH_HIP_PHASES",aux)
 if (len_trim(aux)>0) read(aux,*) phases
+pelo = 0
+call get_environment_variable("SPH_HARNESS_PELO",aux)
+if (len_trim(aux)>0) read(aux,*) pelo
 call get_environment_variable("SPH_HARNESS_IC",aux)
 if (len_trim(aux)>0) then
    open(newunit=u,file=trim(aux),access="stream",form="unformatted",          &
@@ -82,8 +89,10 @@ call write_header(u)
 call sph_hip_open(prec,ctx)
 call sph_hip_check(sph_preloop(ctx),"sph_preloop")
 call sph_hip_pull(ctx)
+if (pelo==1) call sph_hip_monitor_lines(ctx)
 it = 0
 call write_step(u,it,0)
+if (pelo==1) call sph_hip_calc_pelo(ctx)
 do while (it<nsteps)
    k = min(every,nsteps-it)
    if (phases==1) then
@@ -96,6 +105,7 @@ do while (it<nsteps)
    it = it + k
    call sph_hip_pull(ctx)
    call write_step(u,it,0)
+   if (pelo==1) call sph_hip_calc_pelo(ctx)
 enddo
 close(u)
 call sph_hip_close(ctx)

@@ -7,6 +7,7 @@
 // reference's 1-based cell ids are kept as such (0 = outside the grid).
 #include "sph_oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1023,5 +1024,73 @@ int64_t or_pair_count(void* ctx) {
 }
 
 int or_overflow(void* ctx) { return ((Ctx*)ctx)->overflow; }
+
+// Free-surface monitor lines: calc_pelo.f90:82-131, with the point cells of
+// CalcVarp.f90:66-72 (ceiling of the offset over the cell size, CellNumber).
+// icont holds control_lines(i)%Icont(1:2) (1-based, inclusive); the loop
+// starts at Icont(1)+1 as the reference's does.  The two closest particles
+// come from two minloc passes over an NMAXPARTJ array preset to 9e12
+// (calc_pelo.f90:108-122).  Returns 3 if the cell lists are not built, 4 if
+// icont is out of range.
+int or_free_surface(void* ctx, int nlines, const int32_t* icont, int npoints, const double* pts,
+                    double* pelolib, double* zlft) {
+    Ctx& c = *(Ctx*)ctx;
+    if ((int)c.icont.size() != c.nmax + 2) return 3;
+    for (int i = 0; i < nlines; ++i)
+        if (icont[2 * i] < 1 || icont[2 * i + 1] > npoints) return 4;
+    const int nmpj = c.p.nmaxpartj;
+    std::vector<int> particle_ID;
+    std::vector<double> dis2;
+    for (int i = 0; i < 3 * nlines; ++i) pelolib[i] = zlft[i] = 0.0;
+    for (int i = 0; i < nlines; ++i) {
+        bool zlft_flag = true, fluid_below = false;
+        for (int j = icont[2 * i] + 1; j <= icont[2 * i + 1]; ++j) {
+            const double* q = &pts[3 * (j - 1)];
+            int ijk[3];
+            bool inside = true;
+            for (int d = 0; d < 3; ++d) {
+                const double f = std::ceil((q[d] - c.p.extr_lo[d]) / c.p.dcd[d]);
+                inside = inside && f >= 1.0 && f <= (double)c.p.ncd[d];
+                ijk[d] = inside ? (int)f : 0;
+            }
+            const int ncel = inside ? cell_number(c, ijk[0], ijk[1], ijk[2]) : 0;
+            if (ncel == 0) continue;
+            if (c.icont[ncel + 1] <= c.icont[ncel]) {
+                if (fluid_below) zlft_flag = false;
+                continue;
+            }
+            const int cnt = c.icont[ncel + 1] - c.icont[ncel];
+            const size_t len = (size_t)std::max(nmpj, cnt);
+            particle_ID.assign(len, -1);
+            dis2.assign(len, 9.0e12);
+            int npartcel = 0;
+            for (int mm = c.icont[ncel]; mm <= c.icont[ncel + 1] - 1; ++mm) {
+                const int jj = c.npartord[mm];
+                const double r0 = q[0] - c.coord[3 * jj];
+                const double r1 = q[1] - c.coord[3 * jj + 1];
+                const double r2 = q[2] - c.coord[3 * jj + 2];
+                particle_ID[npartcel] = jj;
+                dis2[npartcel] = r0 * r0 + r1 * r1 + r2 * r2;
+                npartcel += 1;
+            }
+            auto minloc = [&]() {
+                size_t m = 0;
+                for (size_t t = 1; t < len; ++t)
+                    if (dis2[t] < dis2[m]) m = t;
+                return m;
+            };
+            const size_t m1 = minloc();
+            dis2[m1] = 9.0e12;
+            const size_t m2 = minloc();
+            const int a = particle_ID[m1], b = particle_ID[m2];
+            for (int d = 0; d < 3; ++d) {
+                pelolib[3 * i + d] = (c.coord[3 * a + d] + c.coord[3 * b + d]) * 0.5 + c.p.dx / 2.0;
+                if (zlft_flag) zlft[3 * i + d] = pelolib[3 * i + d];
+            }
+            fluid_below = true;
+        }
+    }
+    return 0;
+}
 
 }  // extern "C"

@@ -7,7 +7,7 @@
  * Every routine follows one reference unit, in the reference's particle order
  * (pg order), with the reference's loop and summation order, so that on the
  * same inputs it reproduces the flang-built reference bit for bit (checked in
- * tests/test_oracle_vs_reference.py against oracle/_ref/harness3d.x).
+ * tests/test_oracle_reference.py against oracle/_ref/harness3d.x).
  */
 #pragma once
 #include <stdint.h>
@@ -75,6 +75,12 @@ int or_preloop(void* ctx);          /* lists, boundary table, first dt */
 int or_step(void* ctx, int nsteps); /* whole leapfrog time steps */
 int64_t or_pair_count(void* ctx);   /* sum_i (nPartIntorno(i)-1) of the current lists */
 int or_overflow(void* ctx);         /* particles whose list was truncated */
+
+/* calc_pelo: free-surface position (pelolib) and lower-fluid-top height (zlft)
+ * of each monitoring line, 3 values per line; icont: 2 per line, 1-based
+ * first and last point as control_lines(:)%Icont; pts: 3 per point */
+int or_free_surface(void* ctx, int nlines, const int32_t* icont, int npoints, const double* pts,
+                    double* pelolib, double* zlft);
 
 #ifdef __cplusplus
 }

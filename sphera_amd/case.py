@@ -53,6 +53,9 @@ class DamBreak3D:
     maxcloseboundfaces: int = 100
     slip_coeff: float = 0.0       # slip_coefficient_mode=1 input value
     name: str = "dam_break_3D"
+    # monitoring lines of the deck's "control lines" section:
+    # (label, first point (x,y,z), second point, number of points)
+    control_lines: tuple = ()
 
     def __post_init__(self):
         if self.bulk is None:
@@ -419,6 +422,26 @@ def domain_box(case: DamBreak3D):
     """Domain%coord: bounding box of all vertices (the grid spans it)."""
     v = case.vertices()
     return v.min(0), v.max(0)
+
+
+def control_line_points(lines):
+    """Points of the monitoring lines, as ReadInputControlLines.f90:97-124
+    lays them out: line i holds the 1-based points icont[i,0]..icont[i,1];
+    the step is (second - first)/(ndiv - 1) and each point adds it to the
+    previous one.  Returns (icont int32 (nlines, 2), points (npoints, 3))."""
+    icont, pts, npts = [], [], 0
+    for _label, first, second, ndiv in lines:
+        ndiv = int(ndiv)
+        v1 = np.array(first, dtype=np.float64)
+        icont.append((npts + 1, npts + ndiv))
+        npts += ndiv
+        with np.errstate(divide="ignore", invalid="ignore"):
+            v3 = (np.array(second, dtype=np.float64) - v1) / float(ndiv - 1)
+            for _ in range(ndiv):
+                pts.append(v1.copy())
+                v1 = v1 + v3
+    return (np.array(icont, dtype=np.int32).reshape(-1, 2),
+            np.array(pts, dtype=np.float64).reshape(-1, 3))
 
 
 @dataclass

@@ -79,6 +79,10 @@ struct sph_ctx {
     int* iscal = nullptr;
     unsigned long long* dtmin = nullptr;
     unsigned long long* pairs = nullptr;  // running pair-interaction count
+    // monitoring lines of the free-surface monitor
+    int mon_nlines = 0, mon_npoints = 0;
+    int* mon_icont = nullptr;
+    double *mon_pts = nullptr, *mon_out = nullptr;
     Phase phase = PH_EMPTY;
     bool timing = false;
     std::vector<Timer> timers;
@@ -342,6 +346,7 @@ sph_ctx::~sph_ctx() {
     F(dden); F(vpres); F(rowblk); F(idslot); F(rmids); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
+    F(mon_icont); F(mon_pts); F(mon_out);
     for (auto e : ev) hipEventDestroy(e);
     for (auto& g : graphs) hipGraphExecDestroy(g.second);
     if (stream) hipStreamDestroy(stream);
@@ -728,6 +733,53 @@ int sph_global_quantities(sph_ctx* c, double out[8]) {
     hipFree(part);
     hipFree(dout);
     if (e != hipSuccess) return fail(100, std::string("global quantities: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int sph_monitor_lines(sph_ctx* c, int32_t nlines, const int32_t* icont, int32_t npoints,
+                      const double* points) {
+    if (!c || nlines < 0 || npoints < 0 || (nlines && !icont) || (npoints && !points))
+        return fail(1, "null or negative argument");
+    for (int i = 0; i < nlines; ++i)
+        if (icont[2 * i] < 1 || icont[2 * i + 1] > npoints)
+            return fail(3, "monitoring line " + std::to_string(i + 1) + " points outside 1.." +
+                               std::to_string(npoints));
+    HIPC(hipStreamSynchronize(c->stream));
+    auto F = [](auto*& p) {
+        if (p) hipFree(p);
+        p = nullptr;
+    };
+    F(c->mon_icont);
+    F(c->mon_pts);
+    F(c->mon_out);
+    c->mon_nlines = c->mon_npoints = 0;
+    if (nlines == 0) return 0;
+    HIPC(dalloc(&c->mon_icont, 2 * (size_t)nlines));
+    HIPC(dalloc(&c->mon_pts, 3 * (size_t)npoints));
+    HIPC(dalloc(&c->mon_out, 6 * (size_t)nlines));
+    HIPC(hipMemcpy(c->mon_icont, icont, sizeof(int) * 2 * nlines, hipMemcpyHostToDevice));
+    if (npoints) HIPC(hipMemcpy(c->mon_pts, points, sizeof(double) * 3 * npoints, hipMemcpyHostToDevice));
+    c->mon_nlines = nlines;
+    c->mon_npoints = npoints;
+    return 0;
+}
+
+int sph_free_surface(sph_ctx* c, double* pelolib, double* zlft) {
+    if (!c || ((!pelolib || !zlft) && c->mon_nlines)) return fail(1, "null argument");
+    if (c->phase != PH_SORTED && c->phase != PH_READY && c->phase != PH_PS)
+        return fail(121, "the free surface is taken at a time-step boundary");
+    const int nl = c->mon_nlines;
+    if (nl == 0) return 0;
+    k_free_surface<<<nl, FS_WG, 0, c->stream>>>(c->P, c->xm[0], c->start, c->mon_pts, c->mon_icont, nl,
+                                                c->mon_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pelolib, c->mon_out, sizeof(double) * 3 * nl, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(zlft, c->mon_out + 3 * nl, sizeof(double) * 3 * nl, hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(100, std::string("free surface: ") + hipGetErrorString(e));
     return 0;
 }
 

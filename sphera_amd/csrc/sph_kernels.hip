@@ -1032,6 +1032,109 @@ __global__ void k_global_final(const double* __restrict__ part, int nblocks, dou
 }
 
 // ----------------------------------------------------------------------------
+// Free-surface monitor lines: calc_pelo.f90:82-131.  One workgroup per line;
+// each thread takes one point of a FS_WG-point chunk: its cell (the ceiling
+// form of CalcVarp.f90:66-72, with out-of-range indices giving cell 0 before
+// any integer conversion), then the two particles of that cell closest to
+// the point.  The reference takes them with two minloc passes over an array
+// preset to 9e12; a running (first, second) minimum with strict comparisons
+// in cell order (ascending particle index, as NPartOrd) picks the same pair,
+// because a point and a particle of the same cell are closer than 9e12.  A
+// cell holding one particle gives that particle twice.  Thread 0 then runs
+// the reference's sequential flags over the chunk: pelolib takes the last
+// filled cell, z_lower_fluid_top stops at the first empty cell above fluid;
+// cells outside the grid are skipped.
+// icont: control_lines(:)%Icont, 1-based (first, last); the reference starts
+// at Icont(1)+1, which is 0-based index Icont(1).
+// out[3*line + d] = pelolib, out[3*(nlines + line) + d] = z_lower_fluid_top.
+// ----------------------------------------------------------------------------
+constexpr int FS_WG = 256;
+
+__device__ __forceinline__ int d_point_cell(const DP& P, double x, double y, double z) {
+    const double q[3] = {x, y, z};
+    int ijk[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double f = ceil((q[d] - P.extr_lo[d]) / P.dcd[d]);
+        if (!(f >= 1.0 && f <= (double)P.ncd[d])) return 0;  // also NaN
+        ijk[d] = (int)f;
+    }
+    return d_cell_number(P, ijk[0], ijk[1], ijk[2]);
+}
+
+__global__ __launch_bounds__(FS_WG) void k_free_surface(DP P, const double4* __restrict__ xm,
+                                                       const int* __restrict__ start,
+                                                       const double* __restrict__ pts,
+                                                       const int* __restrict__ icont, int nlines,
+                                                       double* __restrict__ out) {
+    __shared__ double val[FS_WG][3];
+    __shared__ int st[FS_WG];  // 0 outside the grid, 1 empty cell, 2 filled cell
+    const int line = blockIdx.x;
+    const int j0 = icont[2 * line], j1 = icont[2 * line + 1];
+    bool zflag = true, below = false;
+    double pel[3] = {0.0, 0.0, 0.0}, zl[3] = {0.0, 0.0, 0.0};
+    const double half_dx = P.dx / 2.0;
+    for (int base = j0; base < j1; base += FS_WG) {
+        const int j = base + (int)threadIdx.x;
+        int s = 0;
+        if (j < j1) {
+            const double qx = pts[3 * j], qy = pts[3 * j + 1], qz = pts[3 * j + 2];
+            const int c = d_point_cell(P, qx, qy, qz);
+            if (c > 0) {
+                const int b = start[c - 1], e = start[c];
+                s = e > b ? 2 : 1;
+                if (s == 2) {
+                    double m1 = 0.0, m2 = 0.0;
+                    int i1 = -1, i2 = -1;
+                    for (int m = b; m < e; ++m) {
+                        const double4 x = xm[m];
+                        const double r0 = qx - x.x, r1 = qy - x.y, r2 = qz - x.z;
+                        const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+                        if (i1 < 0 || d2 < m1) {
+                            m2 = m1;
+                            i2 = i1;
+                            m1 = d2;
+                            i1 = m;
+                        } else if (i2 < 0 || d2 < m2) {
+                            m2 = d2;
+                            i2 = m;
+                        }
+                    }
+                    if (i2 < 0) i2 = i1;
+                    const double4 a = xm[i1], bb = xm[i2];
+                    val[threadIdx.x][0] = (a.x + bb.x) * 0.5 + half_dx;
+                    val[threadIdx.x][1] = (a.y + bb.y) * 0.5 + half_dx;
+                    val[threadIdx.x][2] = (a.z + bb.z) * 0.5 + half_dx;
+                }
+            }
+        }
+        st[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int nt = min(FS_WG, j1 - base);
+            for (int t = 0; t < nt; ++t) {
+                if (st[t] == 0) continue;
+                if (st[t] == 1) {
+                    if (below) zflag = false;
+                    continue;
+                }
+                for (int d = 0; d < 3; ++d) {
+                    pel[d] = val[t][d];
+                    if (zflag) zl[d] = val[t][d];
+                }
+                below = true;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) {
+            out[3 * line + d] = pel[d];
+            out[3 * (nlines + line) + d] = zl[d];
+        }
+}
+
+// ----------------------------------------------------------------------------
 // dt: time_step_duration.f90:74-116.  Positive doubles order like their bit
 // patterns, so the minimum over particles is an exact atomicMin on uint64.
 // ----------------------------------------------------------------------------

@@ -12,6 +12,9 @@
 !   sph_hip_pull(ctx)       device state -> pg(1:nag), nag, dt, dt_average,
 !                           simulation_time (at output steps)
 !   sph_hip_check(rc,where) stops like diagnostic() on a non-zero status
+!   sph_hip_monitor_lines(ctx)  control_lines / control_points -> device
+!   sph_hip_calc_pelo(ctx)  calc_pelo on the device: writes the same .plb and
+!                           .zlft records (calc_pelo.f90:132-133)
 !
 ! Scope the device implements (checked by sph_hip_open): Domain%tipo="semi",
 ! leapfrog (time_split=1), one medium, no NormFix, no open/source faces, no
@@ -25,6 +28,7 @@ use Dynamic_allocation_module
 implicit none
 private
 public :: sph_hip_open, sph_hip_pull, sph_hip_check, sph_hip_close
+public :: sph_hip_monitor_lines, sph_hip_calc_pelo
 
 ! Host copies of the frontier tables in the C layout; they only need to live
 ! for the sph_ctx_create call, but are kept so a caller may inspect them.
@@ -258,6 +262,37 @@ dt_average = sc(2)
 simulation_time = sc(3)
 deallocate(xyz,uvw,a,rho,pp,dd,vp,ids)
 end subroutine sph_hip_pull
+
+! Monitoring lines of the deck (ReadInputControlLines) -> device.
+subroutine sph_hip_monitor_lines(ctx)
+type(c_ptr),intent(in) :: ctx
+integer(c_int32_t),allocatable :: ic(:)
+real(c_double),allocatable :: xyz(:)
+integer(4) :: i
+allocate(ic(2*max(nlines,1)),xyz(3*max(npointst,1)))
+do i=1,nlines
+   ic(2*i-1:2*i) = control_lines(i)%icont(1:2)
+enddo
+do i=1,npointst
+   xyz(3*i-2:3*i) = control_points(i)%coord(1:3)
+enddo
+call sph_hip_check(sph_monitor_lines(ctx,int(nlines,c_int32_t),ic,                    &
+   int(npointst,c_int32_t),xyz),"sph_monitor_lines")
+deallocate(ic,xyz)
+end subroutine sph_hip_monitor_lines
+
+! calc_pelo (calc_pelo.f90:82-133) with the lines on the device: the time
+! series records go to the same units in the same format.  Call after
+! sph_hip_pull (simulation_time).
+subroutine sph_hip_calc_pelo(ctx)
+use I_O_file_module
+type(c_ptr),intent(in) :: ctx
+real(c_double) :: pelolib(3,max(nlines,1)),z_lower_fluid_top(3,max(nlines,1))
+if (nlines<1) return
+call sph_hip_check(sph_free_surface(ctx,pelolib,z_lower_fluid_top),"sph_free_surface")
+write(nplb,'(30g14.7)') simulation_time,pelolib(:,1:nlines)
+write(uzlft,'(30g14.7)') simulation_time,z_lower_fluid_top(:,1:nlines)
+end subroutine sph_hip_calc_pelo
 
 subroutine sph_hip_close(ctx)
 type(c_ptr),intent(inout) :: ctx

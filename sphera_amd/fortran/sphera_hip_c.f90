@@ -150,6 +150,26 @@ interface
       type(c_ptr),value :: ctx
       integer(c_int32_t),value :: nsteps
    end function
+   integer(c_int) function sph_global_quantities(ctx,out)                              &
+      bind(C,name="sph_global_quantities")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr),value :: ctx
+      real(c_double),intent(out) :: out(8)
+   end function
+   integer(c_int) function sph_monitor_lines(ctx,nlines,icont,npoints,points)          &
+      bind(C,name="sph_monitor_lines")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr),value :: ctx
+      integer(c_int32_t),value :: nlines, npoints
+      integer(c_int32_t),intent(in) :: icont(*)
+      real(c_double),intent(in) :: points(*)
+   end function
+   integer(c_int) function sph_free_surface(ctx,pelolib,zlft)                          &
+      bind(C,name="sph_free_surface")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr),value :: ctx
+      real(c_double),intent(out) :: pelolib(*), zlft(*)
+   end function
    integer(c_int) function sph_query(ctx,busy) bind(C,name="sph_query")
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr),value :: ctx

@@ -24,7 +24,8 @@ PI = C.POINTER(C.c_int32)
 
 EXPORTS = [
     "sph_ctx_create", "sph_ctx_destroy", "sph_state_upload", "sph_particle_count",
-    "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_global_quantities", "sph_get_neighbour_lists", "sph_dt", "sph_momentum",
+    "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_global_quantities",
+    "sph_monitor_lines", "sph_free_surface", "sph_get_neighbour_lists", "sph_dt", "sph_momentum",
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
     "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
     "sph_preloop", "sph_step", "sph_step_async", "sph_step_graph", "sph_sync", "sph_get_launch", "sph_set_timing",
@@ -78,6 +79,8 @@ def load():
     L.sph_get_neighbour_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
     L.sph_pair_counter.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.sph_global_quantities.argtypes = [C.c_void_p, PD]
+    L.sph_monitor_lines.argtypes = [C.c_void_p, C.c_int32, PI, C.c_int32, PD]
+    L.sph_free_surface.argtypes = [C.c_void_p, PD, PD]
     for ph in PHASES + ["preloop", "sync"]:
         getattr(L, "sph_" + ph).argtypes = [C.c_void_p]
     L.sph_step.argtypes = [C.c_void_p, C.c_int32]
@@ -283,6 +286,21 @@ class SphContext:
         v = np.zeros(8)
         self._check(self.L.sph_global_quantities(self.h, _pd(v)), "sph_global_quantities")
         return dict(mass=v[0], lin_mom=v[1:4].copy(), ang_mom=v[4:7].copy(), volume=v[7])
+
+    def monitor_lines(self, icont, points):
+        """Sets the free-surface monitoring lines: icont (nlines, 2) 1-based
+        first/last point of each line, points (npoints, 3)."""
+        self._icont = np.ascontiguousarray(icont, np.int32).reshape(-1, 2)
+        pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        self._check(self.L.sph_monitor_lines(self.h, len(self._icont), _pi(self._icont), len(pts),
+                                             _pd(pts)), "sph_monitor_lines")
+
+    def free_surface(self):
+        """calc_pelo: (pelolib, zlft), each (nlines, 3)."""
+        nl = len(getattr(self, "_icont", ()))
+        pel, zl = np.zeros((nl, 3)), np.zeros((nl, 3))
+        self._check(self.L.sph_free_surface(self.h, _pd(pel), _pd(zl)), "sph_free_surface")
+        return pel, zl
 
     def busy(self):
         """True while work enqueued on this context's stream is running."""

@@ -3,7 +3,9 @@
 SPHERA v10 reference (oracle/_ref/harness3d.x, built by oracle/Makefile.ref
 from /root/reference/src) on project-authored decks.
 
-    make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py
+    make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
+
+With names (tiny, small_jitter, removal, pelo) only those fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -14,6 +16,10 @@ Fixtures (numpy .npz, inputs and reference outputs only):
   ref_removal.npz       800 particles of which 6 start outside the grid (three
                         holes, three at the tail): OrdGrid1's swap-with-last
                         order; state after the pre-loop and after 6 steps
+  ref_pelo.npz          800 particles, jittered, with a block lifted above a
+                        gap and one lone particle; four monitoring lines (two
+                        partly outside the grid): the reference's free-surface
+                        monitor (.plb, .zlft at g14.7) after 0, 3 and 6 steps
 """
 from __future__ import annotations
 
@@ -26,12 +32,26 @@ HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent.parent
 sys.path.insert(0, str(ROOT))
 
-from oracle.decks import read_dump, run_harness  # noqa: E402
-from sphera_amd.case import DamBreak3D, build  # noqa: E402
+from oracle.decks import read_dump, read_pelo, run_harness  # noqa: E402
+from sphera_amd.case import DamBreak3D, build, control_line_points  # noqa: E402
 
 TINY = dict(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
 SMALL = dict(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), dx=0.01, name="small")
 FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
+PELO_LINES = (("v_gap", (0.015, 0.05, 0.0), (0.015, 0.05, 0.19), 20),
+              ("v_out", (0.071, 0.021, -0.05), (0.071, 0.021, 0.25), 31),
+              ("h_out", (-0.02, 0.05, 0.05), (0.32, 0.05, 0.05), 35),
+              ("diag", (0.0, 0.0, 0.0), (0.3, 0.1, 0.2), 41))
+
+
+def pelo_ic(coord, dx):
+    """Jittered lattice; particles with x < 0.03 and z > 0.06 lifted by 0.06
+    (fluid above an empty gap) and one particle moved alone to the top."""
+    ic = jitter(coord, dx, seed=20260202)
+    m = (ic[:, 0] < 0.03) & (ic[:, 2] > 0.06)
+    ic[m, 2] += 0.06
+    ic[7] = (0.0148, 0.0497, 0.1852)
+    return ic
 
 
 def removal_slots(n):
@@ -62,8 +82,8 @@ def pack_step(prefix, st, out, lists):
         out[prefix + "bt_ig"] = bt["ig"]
 
 
-def main():
-    # tiny: lists and boundary table
+def make_tiny():
+    """lists and boundary table"""
     case = DamBreak3D(**TINY)
     hdr, steps = read_dump(run_harness(case, nsteps=5, every=5))
     out = {}
@@ -79,7 +99,10 @@ def main():
     for st in steps:
         pack_step(f"s{st['it']}_", st, out, lists=True)
     np.savez_compressed(HERE / "ref_tiny.npz", **out)
-    # small, jittered: fields over 10 steps
+
+
+def make_small_jitter():
+    """small, jittered: fields over 10 steps"""
     case = DamBreak3D(**SMALL)
     tb = build(case)
     ic = jitter(tb.particles["coord"], case.dx)
@@ -88,7 +111,10 @@ def main():
     for st in steps:
         pack_step(f"s{st['it']}_", st, out, lists=False)
     np.savez_compressed(HERE / "ref_small_jitter.npz", **out)
-    # removal order: particles placed outside the grid
+
+
+def make_removal():
+    """removal order: particles placed outside the grid"""
     case = DamBreak3D(**TINY)
     tb = build(case)
     ic = tb.particles["coord"].copy()
@@ -101,6 +127,28 @@ def main():
         out[f"s{st['it']}_nag"] = st["nag"]
         pack_step(f"s{st['it']}_", st, out, lists=False)
     np.savez_compressed(HERE / "ref_removal.npz", **out)
+
+
+def make_pelo():
+    """free-surface monitor lines"""
+    case = DamBreak3D(**TINY, control_lines=PELO_LINES)
+    tb = build(case)
+    ic = pelo_ic(tb.particles["coord"], case.dx)
+    dump = run_harness(case, nsteps=6, every=3, lists=False, ic_coord=ic,
+                       extra_env={"SPH_HARNESS_PELO": "1"})
+    t, pel, zl = read_pelo(dump.parent, case)
+    icont, pts = control_line_points(PELO_LINES)
+    np.savez_compressed(HERE / "ref_pelo.npz", ic_coord=ic, icont=icont, points=pts,
+                        time=t, pelolib=pel, zlft=zl)
+
+
+MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_removal,
+          "pelo": make_pelo}
+
+
+def main():
+    for name in sys.argv[1:] or list(MAKERS):
+        MAKERS[name]()
     for f in sorted(HERE.glob("*.npz")):
         print(f.name, f.stat().st_size)
 

@@ -67,3 +67,21 @@ def test_bridge_fp32_mixed_within_tolerance(tmp_path):
         ref = g["s10_" + f]
         err = np.abs(st[10][f] - ref).max() / max(np.abs(ref).max(), 1e-300)
         assert err < FP32_TOL[f], (f, err)
+
+
+def test_bridge_free_surface_equals_reference(tmp_path):
+    """calc_pelo through the bridge (sph_hip_monitor_lines, sph_hip_calc_pelo):
+    the device run writes .plb/.zlft records that read back as the reference's
+    own (tests/golden/ref_pelo.npz, same deck and initial positions)."""
+    from oracle.decks import read_pelo
+    from tests.golden.make_golden import PELO_LINES, TINY
+    g = np.load(GOLD / "ref_pelo.npz")
+    case = DamBreak3D(**TINY, control_lines=PELO_LINES)
+    print("[bridge] free surface tiny steps=6", flush=True)
+    dump = run_harness(case, nsteps=6, every=3, lists=False, ic_coord=g["ic_coord"],
+                       workdir=tmp_path, exe=HARNESS_HIP, timeout=120,
+                       extra_env={"SPH_HIP_PRECISION": "0", "SPH_HARNESS_PELO": "1"})
+    t, pel, zl = read_pelo(dump.parent, case)
+    assert np.allclose(t, g["time"], rtol=1e-6, atol=1e-12)
+    assert np.allclose(pel, g["pelolib"], rtol=1e-6, atol=1e-12)
+    assert np.allclose(zl, g["zlft"], rtol=1e-6, atol=1e-12)
