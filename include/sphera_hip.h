@@ -117,6 +117,12 @@ int sph_pair_counter(sph_ctx* ctx, unsigned long long* total);
 /* Debug: neighbour lists in pg order, entries are pg indices (0-based), in the
  * reference's list order.  stride >= nmaxpartj. */
 int sph_get_neighbour_lists(sph_ctx* ctx, int32_t* count, int32_t* ids, int32_t stride);
+/* Debug: the SA-SPH boundary table (BoundaryDataTab, ComputeBoundaryDataTab_3D
+ * .f90:62-106) in pg order: ncbf[r] close faces of row r; for slot q < ncbf[r]
+ * at [r*stride + q]: face (0-based index into sph_faces), loc (3), bi (8:
+ * BoundaryVolumeIntegrals_P0 order), ig (9, column-major).  stride >= 8. */
+int sph_get_boundary_table(sph_ctx* ctx, int32_t* ncbf, int32_t* face, double* loc, double* bi,
+                           double* ig, int32_t stride);
 
 /* ---- phases, in time_step_loop order ------------------------------------ */
 int sph_dt(sph_ctx* ctx);              /* time_step_duration.f90:59-123 */

@@ -53,6 +53,7 @@ struct sph_ctx {
     int *d_gcount = nullptr, *d_gstart = nullptr, *d_gvec = nullptr;
     int* d_gnear = nullptr;  // per cell: frontier faces listed in its 27-cell neighbourhood
     double *d_kt = nullptr, *d_bit = nullptr;
+    BtF32* d_btf = nullptr;  // the quadrature table in the FP32-mixed layout
     // state
     double4 *xm[2] = {}, *vel[3] = {}, *acc[2] = {};
     double2* rp[3] = {};
@@ -216,13 +217,20 @@ int run_btab(sph_ctx* c) {
         k_fill_int<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->bn, 0, c->P.cap);
         return 0;
     }
-    KTime t(c, "boundary_tab");
-    k_fill_int<<<1, 64, 0, c->stream>>>(c->nwork, 0, 1);
-    k_boundary_find<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_gnear, c->d_gcount, c->d_gstart,
-                                                            c->d_gvec, c->xm[0], c->bn, c->bt, c->work,
-                                                            c->nwork, c->iscal);
-    k_boundary_int<<<c->bt_grid, 256, 0, c->stream>>>(c->P, c->d_faces, c->d_kt, c->d_bit, c->bt, c->work,
-                                                   c->nwork);
+    {
+        KTime t(c, "boundary_find");
+        k_fill_int<<<1, 64, 0, c->stream>>>(c->nwork, 0, 1);
+        k_boundary_find<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_gnear, c->d_gcount,
+                                                                c->d_gstart, c->d_gvec, c->xm[0], c->bn, c->bt,
+                                                                c->work, c->nwork, c->iscal);
+    }
+    KTime t(c, "boundary_int");
+    if (c->P.f32)  // eight lanes per pair
+        k_boundary_int_f32<<<4 * c->bt_grid, 256, 0, c->stream>>>(c->P, c->d_faces, c->d_btf, c->bt, c->work,
+                                                                  c->nwork);
+    else
+        k_boundary_int<<<c->bt_grid, 256, 0, c->stream>>>(c->P, c->d_faces, c->d_kt, c->d_bit, c->bt, c->work,
+                                                          c->nwork);
     return 0;
 }
 
@@ -334,7 +342,7 @@ sph_ctx::~sph_ctx() {
     auto F = [](void* p) {
         if (p) hipFree(p);
     };
-    F(d_faces); F(d_gnear); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit);
+    F(d_faces); F(d_gnear); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit); F(d_btf);
     for (auto p : xm) F(p);
     for (auto p : vel) F(p);
     for (auto p : acc) F(p);
@@ -598,6 +606,12 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
         if (nv) ok = ok && hipMemcpy(c->d_gvec, f->gcbf_vec, sizeof(int) * nv, hipMemcpyHostToDevice) == hipSuccess;
         ok = ok && hipMemcpy(c->d_kt, f->kerneltab, sizeof(double) * 41 * 5, hipMemcpyHostToDevice) == hipSuccess;
         ok = ok && hipMemcpy(c->d_bit, f->bitab, sizeof(double) * 256 * 7, hipMemcpyHostToDevice) == hipSuccess;
+        {
+            BtF32 T;
+            build_btf32(f->bitab, f->kerneltab, f->ktdelta, P.h, T);
+            ok = ok && hipMalloc(&c->d_btf, sizeof(BtF32)) == hipSuccess &&
+                 hipMemcpy(c->d_btf, &T, sizeof(BtF32), hipMemcpyHostToDevice) == hipSuccess;
+        }
         if (!ok) {
             delete c;
             return fail(102, "device allocation/copy of frontier tables failed");
@@ -815,6 +829,40 @@ int sph_get_neighbour_lists(sph_ctx* c, int32_t* count, int32_t* ids, int32_t st
         if (ids)
             for (int k = 0; k < nn[s]; ++k)
                 ids[(size_t)r * stride + k] = (int32_t)hv[nb[nb_index(N, s, k)] & NB_MASK].w;
+    }
+    return 0;
+}
+
+int sph_get_boundary_table(sph_ctx* c, int32_t* ncbf, int32_t* face, double* loc, double* bi,
+                           double* ig, int32_t stride) {
+    if (!c || !ncbf || !face || !loc || !bi || !ig) return fail(1, "null argument");
+    if (stride < SPH_MB) return fail(3, "stride smaller than the boundary-table slots");
+    HIPC(hipStreamSynchronize(c->stream));
+    int n;
+    HIPC(hipMemcpy(&n, c->iscal + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    const size_t N = c->P.cap;
+    std::vector<int> nc(n);
+    std::vector<double4> hv(n);
+    HIPC(hipMemcpy(nc.data(), c->bn, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hv.data(), c->vel[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
+    int qmax = 0;
+    for (int i = 0; i < n; ++i) qmax = std::max(qmax, nc[i]);
+    std::vector<BTab> hb((size_t)qmax * N);
+    if (qmax) HIPC(hipMemcpy(hb.data(), c->bt, sizeof(BTab) * hb.size(), hipMemcpyDeviceToHost));
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return hv[a].w < hv[b].w; });
+    for (int r = 0; r < n; ++r) {
+        const int s = order[r];
+        ncbf[r] = nc[s];
+        for (int q = 0; q < nc[s]; ++q) {
+            const BTab& B = hb[(size_t)q * N + s];
+            const size_t o = (size_t)r * stride + q;
+            face[o] = B.face;
+            for (int t = 0; t < 3; ++t) loc[3 * o + t] = B.loc[t];
+            for (int t = 0; t < 8; ++t) bi[8 * o + t] = B.bi[t];
+            for (int t = 0; t < 9; ++t) ig[9 * o + t] = B.ig[t];
+        }
     }
     return 0;
 }

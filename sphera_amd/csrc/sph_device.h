@@ -1,5 +1,6 @@
 // Device-side structures shared by the kernels and the host orchestration.
 #pragma once
+#include <cmath>
 #include <stdint.h>
 
 #define SPH_MB 8  // boundary-table slots per particle (close SA-SPH faces)
@@ -56,6 +57,9 @@ struct DFace {
     // IsPointInternal's per-face constants (1/DueArea and the AA, BB, CC of
     // its two edges), evaluated once with the same expressions
     double pu, pA[2], pB[2], pC[2];
+    // |grad| of the area coordinates csi0, csi1 and csi2 = 1 - csi0 - csi1
+    // over the face plane (bounds of a coordinate over a circle)
+    double pg[3];
     int nodes, btype, mode, lam;
 };
 
@@ -74,6 +78,75 @@ inline void dface_point_coeffs(DFace& F) {
         F.pB[e] = yj - yk;
         F.pC[e] = xk - xj;
     }
+    for (int e = 0; e < 2; ++e) F.pg[e] = F.pu * std::sqrt(F.pB[e] * F.pB[e] + F.pC[e] * F.pC[e]);
+    const double sb = F.pB[0] + F.pB[1], sc = F.pC[0] + F.pC[1];
+    F.pg[2] = F.pu * std::sqrt(sb * sb + sc * sc);
+}
+
+// BoundIntegralTab (ComputeBoundaryIntegralTab.f90) as the FP32-mixed
+// quadrature reads it.  The table holds BT_RINGS rings of BT_RDIRS
+// directions; within a ring every direction has the same vertical offset
+// b_z, the same weight da and the same |b_xy|.
+constexpr int BT_RINGS = 8, BT_RDIRS = 32;
+
+struct RingC {
+    double bz, nr, rho;           // b_z, -1/b_z, |b_xy|
+    float ck, da, R2;             // |b|/(-b_z h), weight, R_z
+    float s0, s1, s00, s01, s11;  // sums over the ring of R0, R1, R0^2, R0 R1, R1^2
+    int ok;                       // the ring's directions share b_z, da and |b_xy|
+    int pad;
+};
+struct BitF64 {
+    double b0, b1, b2, nr;  // offset, and -1/b2 (0 where b2 >= 0)
+};
+struct alignas(16) BtF32 {
+    RingC rc[BT_RINGS];
+    BitF64 bd[BT_RINGS * BT_RDIRS];
+    float bf[BT_RINGS * BT_RDIRS][4];  // |b|/(-b_z h), da, R0, R1
+    float br2[BT_RINGS * BT_RDIRS];    // R_z
+    float kt[41 * 5];                  // kernel table, FP32
+    float kt_inv_delta;
+    int all_ok;
+};
+
+// bit: 256 x 7 (offset x3, da, R x3), kt: 41 x 5
+inline void build_btf32(const double* bit, const double* kt, double ktdelta, double h, BtF32& T) {
+    T.all_ok = 1;
+    for (int r = 0; r < BT_RINGS; ++r) {
+        const double* b = &bit[r * BT_RDIRS * 7];
+        const double bz = b[2], da = b[3];
+        const double rho = std::sqrt(b[0] * b[0] + b[1] * b[1]);
+        int ok = bz < 0.0;
+        double s0 = 0.0, s1 = 0.0, s00 = 0.0, s01 = 0.0, s11 = 0.0;
+        for (int k = 0; k < BT_RDIRS; ++k) {
+            const double* q = &b[k * 7];
+            const double rk = std::sqrt(q[0] * q[0] + q[1] * q[1]);
+            ok &= q[2] == bz && q[3] == da && std::fabs(rk - rho) <= 1.0e-12 * rho;
+            s0 += q[4];
+            s1 += q[5];
+            s00 += q[4] * q[4];
+            s01 += q[4] * q[5];
+            s11 += q[5] * q[5];
+        }
+        const double nr = bz < 0.0 ? -1.0 / bz : 0.0;
+        const double blen = std::sqrt(rho * rho + bz * bz);
+        T.rc[r] = RingC{bz, nr, rho, (float)(blen * nr / h), (float)da, (float)b[6], (float)s0,
+                        (float)s1, (float)s00, (float)s01, (float)s11, ok, 0};
+        T.all_ok &= ok;
+    }
+    for (int t = 0; t < BT_RINGS * BT_RDIRS; ++t) {
+        const double* b = &bit[t * 7];
+        const double nr = b[2] < 0.0 ? -1.0 / b[2] : 0.0;
+        const double blen = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        T.bd[t] = BitF64{b[0], b[1], b[2], nr};
+        T.bf[t][0] = (float)(blen * nr / h);
+        T.bf[t][1] = (float)b[3];
+        T.bf[t][2] = (float)b[4];
+        T.bf[t][3] = (float)b[5];
+        T.br2[t] = (float)b[6];
+    }
+    for (int t = 0; t < 41 * 5; ++t) T.kt[t] = (float)kt[t];
+    T.kt_inv_delta = (float)(1.0 / ktdelta);
 }
 
 // one BoundaryDataTab record (Hybrid_allocation_module.f90:796-805)

@@ -854,7 +854,7 @@ __device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __
 // k_boundary_find: FindCloseBoundaryFaces3D.f90:78-188 per particle, appends
 // every (particle, close face) pair to a compact work list; k_boundary_int:
 // ComputeBoundaryVolumeIntegrals_P0.f90:76-155 for one pair per thread, so the
-// 256-direction quadrature keeps every lane busy.  FP64 in both precisions.
+// 256-direction quadrature keeps every lane busy.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_boundary_find(DP P, const DFace* __restrict__ faces,
                                                        const int* __restrict__ gnear,
@@ -956,6 +956,195 @@ __global__ __launch_bounds__(256) void k_boundary_int(DP P, const DFace* __restr
         d_volume_integrals(P, faces[rec.face], kt, bit, loc, bi, ig);
         for (int t = 0; t < 8; ++t) rec.bi[t] = bi[t];
         for (int t = 0; t < 9; ++t) rec.ig[t] = ig[t];
+    }
+}
+
+// ----------------------------------------------------------------------------
+// FP32-mixed form of the same quadrature, eight lanes per (particle, face)
+// pair, lane r taking ring r of BoundIntegralTab.  The rays of a ring meet
+// the face plane on one circle of radius ts |b_xy| around (L0, L1), all at
+// the same distance rob h = L2 |b| / (-b_z).  IsPointInternal's area
+// coordinates are affine over the plane, so over that circle each stays
+// within its centre value +- radius |grad|: a circle that clears every bound
+// (or lies beyond one) by BT_MARGIN has every direction inside (outside) the
+// face, as the FP64 per-direction test finds it; other rings run that test
+// per direction (FP64, with -1/b_z precomputed).  The reference's reuse of
+// the kernel-table values (refresh when rob moves by more than 1e-3) changes
+// state only at the first kept direction of a ring, so the lanes replay it
+// over the rings' (count, rob) in ring order; interpolation and sums are
+// FP32 and the eight ring sums are reduced across the lanes.
+// ----------------------------------------------------------------------------
+constexpr double BT_MARGIN = 1.0e-10;
+
+// 1: the circle (centre c0, c1, radius rad) lies inside the face, -1: outside,
+// 0: undecided
+__device__ __forceinline__ int ring_side(const DFace& F, const PIC& c, double c0, double c1, double rad) {
+    if (c.fkod != 1 && c.fkod != 2) return -1;
+    const double u0 = (c.a0 + c.b0 * c0 + c.c0 * c1) * c.pu;
+    const double u1 = (c.a1 + c.b1 * c0 + c.c1 * c1) * c.pu;
+    const double m0 = rad * F.pg[0] + BT_MARGIN, m1 = rad * F.pg[1] + BT_MARGIN;
+    bool in = u0 - m0 >= 0.0 && u0 + m0 <= 1.0 && u1 - m1 >= 0.0 && u1 + m1 <= 1.0;
+    bool out = u0 + m0 < 0.0 || u0 - m0 > 1.0 || u1 + m1 < 0.0 || u1 - m1 > 1.0;
+    if (c.fkod == 1) {
+        const double u2 = 1.0 - (u0 + u1), m2 = rad * F.pg[2] + BT_MARGIN;
+        in = in && u2 - m2 >= 0.0 && u2 + m2 <= 1.0;
+        out = out || u2 + m2 < 0.0 || u2 - m2 > 1.0;
+    }
+    return in ? 1 : (out ? -1 : 0);
+}
+
+// kernel-table values at rob times the weight da (InterpolateTable.f90:58-82)
+__device__ __forceinline__ void kt_values_f32(const BtF32& T, float rob, float da, float J[4]) {
+#pragma clang fp contract(fast)
+    const float* kt = T.kt;
+    const int nr0 = (int)((rob - kt[0]) * T.kt_inv_delta);
+    if (nr0 <= 0) {
+        for (int c = 0; c < 4; ++c) J[c] = kt[1 + c] * da;
+    } else if (nr0 >= 40) {
+        for (int c = 0; c < 4; ++c) J[c] = kt[40 * 5 + 1 + c] * da;
+    } else {
+        const float* r0 = &kt[nr0 * 5];
+        const float csi = (rob - r0[0]) * T.kt_inv_delta;
+        for (int c = 0; c < 4; ++c) J[c] = (r0[1 + c] + csi * (r0[6 + c] - r0[1 + c])) * da;
+    }
+}
+
+// accumulators: IntWdV, IntdWrm1dV, IntGWZrm1dV, G0..2, M00 M01 M02 M11 M12 M22;
+// n directions with common R_z and unit-vector sums s = (R0, R1, R0^2, R0R1, R1^2)
+__device__ __forceinline__ void bt_add(float a[12], const float J[4], float n, float R2, const float s[5]) {
+#pragma clang fp contract(fast)
+    a[0] += n * J[0];
+    a[1] += n * J[1];
+    a[2] += R2 * R2 * n * J[1];
+    a[3] += J[2] * s[0];
+    a[4] += J[2] * s[1];
+    a[5] += J[2] * R2 * n;
+    a[6] += J[3] * s[2];
+    a[7] += J[3] * s[3];
+    a[8] += J[3] * R2 * s[0];
+    a[9] += J[3] * s[4];
+    a[10] += J[3] * R2 * s[1];
+    a[11] += J[3] * R2 * R2 * n;
+}
+
+// a table without the ring structure: one lane, direction by direction
+__device__ void bt_pair_f32_dirs(const BtF32& T, const DFace& F, double L0, double L1, double L2,
+                                 float a[12]) {
+    const PIC pic = pic_of(F);
+    const float L2f = (float)L2;
+    float robpre = 2.1f, J[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < BT_RINGS * BT_RDIRS; ++k) {
+        const BitF64 g = T.bd[k];
+        if (!(L2 + g.b2 <= 0.0)) continue;
+        const double ts = L2 * g.nr;
+        if (!d_point_internal(pic, L0 + g.b0 * ts, L1 + g.b1 * ts)) continue;
+        const float* f = T.bf[k];
+        const float rob = L2f * f[0];
+        if (fabsf(rob - robpre) > 1.0e-3f) {
+            kt_values_f32(T, rob, f[1], J);
+            robpre = rob;
+        }
+        const float s[5] = {f[2], f[3], f[2] * f[2], f[2] * f[3], f[3] * f[3]};
+        bt_add(a, J, 1.f, T.br2[k], s);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_boundary_int_f32(DP P, const DFace* __restrict__ faces,
+                                                          const BtF32* __restrict__ tab,
+                                                          BTab* __restrict__ bt,
+                                                          const int* __restrict__ work,
+                                                          const int* nwork) {
+    __shared__ BtF32 T;
+    const int nw = *nwork;
+    const int per_block = blockDim.x / BT_RINGS;
+    if (blockIdx.x * per_block >= nw) return;  // no pair for this workgroup
+    {
+        const int4* src = reinterpret_cast<const int4*>(tab);
+        int4* dst = reinterpret_cast<int4*>(&T);
+        for (int t = threadIdx.x; t < (int)(sizeof(BtF32) / 16); t += blockDim.x) dst[t] = src[t];
+    }
+    __syncthreads();
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = gt & (BT_RINGS - 1);
+    const int stride = (gridDim.x * blockDim.x) / BT_RINGS;
+    // the eight lanes of a pair stay together: same w, same trip count
+    for (int w = gt / BT_RINGS; w < nw; w += stride) {
+        const int e = work[w];
+        const int i = e & (int)NB_MASK, q = (int)((uint32_t)e >> NB_SHIFT);
+        BTab& rec = bt[(size_t)q * P.cap + i];
+        const double L0 = rec.loc[0], L1 = rec.loc[1];
+        double L2 = rec.loc[2];
+        const DFace& F = faces[rec.face];
+        const double zpmin = 0.05 * P.h;
+        float a[12];
+        for (int t = 0; t < 12; ++t) a[t] = 0.f;
+        const bool neg = L2 < 0.0;
+        if (L2 < zpmin) L2 = zpmin;
+        if (!neg && !T.all_ok) {
+            if (r == 0) bt_pair_f32_dirs(T, F, L0, L1, L2, a);
+        } else if (!neg) {
+            const RingC R = T.rc[r];
+            const PIC pic = pic_of(F);
+            float n = 0.f, s[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            if (L2 + R.bz <= 0.0) {  // the plane test, common to the ring
+                const double ts = L2 * R.nr;
+                const int side = ring_side(F, pic, L0, L1, ts * R.rho);
+                if (side > 0) {
+                    n = (float)BT_RDIRS;
+                    s[0] = R.s0; s[1] = R.s1; s[2] = R.s00; s[3] = R.s01; s[4] = R.s11;
+                } else if (side == 0) {
+                    for (int k = r * BT_RDIRS; k < (r + 1) * BT_RDIRS; ++k) {
+                        const BitF64 g = T.bd[k];
+                        if (!d_point_internal(pic, L0 + g.b0 * ts, L1 + g.b1 * ts)) continue;
+                        const float* f = T.bf[k];
+                        n += 1.f;
+                        s[0] += f[2];
+                        s[1] += f[3];
+                        s[2] += f[2] * f[2];
+                        s[3] += f[2] * f[3];
+                        s[4] += f[3] * f[3];
+                    }
+                }
+            }
+            const float L2f = (float)L2;
+            const float rob = L2f * R.ck;
+            // the reuse rule over the rings in order: the ring whose values ring r uses
+            float robpre = 2.1f;
+            int src = -1;
+            for (int k = 0; k < BT_RINGS; ++k) {
+                const float nk = __shfl(n, k, BT_RINGS), rk = __shfl(rob, k, BT_RINGS);
+                if (nk > 0.f && fabsf(rk - robpre) > 1.0e-3f) {
+                    robpre = rk;
+                    if (k <= r) src = k;
+                }
+            }
+            if (n > 0.f && src >= 0) {
+                float J[4];
+                kt_values_f32(T, L2f * T.rc[src].ck, T.rc[src].da, J);
+                bt_add(a, J, n, R.R2, s);
+            }
+        }
+        for (int o = BT_RINGS / 2; o > 0; o >>= 1)
+            for (int t = 0; t < 12; ++t) a[t] += __shfl_xor(a[t], o, BT_RINGS);
+        // bi(1:8), then IntGiWrRdV column-major: 17 doubles, spread over the lanes
+        double v[17];
+        v[0] = 0.0;
+        v[1] = neg ? 0.0 : (L2 == zpmin ? (d_point_internal(F, L0, L1) ? 0.5 : 0.0) : (double)a[0]);
+        v[2] = a[1]; v[3] = a[3]; v[4] = a[4]; v[5] = a[5]; v[6] = a[2]; v[7] = 0.0;
+        v[8] = a[6]; v[9] = a[7]; v[10] = a[8];
+        v[11] = a[7]; v[12] = a[9]; v[13] = a[10];
+        v[14] = a[8]; v[15] = a[10]; v[16] = a[11];
+        double lo = 0.0, hi = 0.0;
+#pragma unroll
+        for (int t = 0; t < BT_RINGS; ++t)
+            if (t == r) {
+                lo = v[t];
+                hi = v[t + 8];
+            }
+        double* out = rec.bi;  // bi[8] and ig[9] are contiguous
+        out[r] = lo;
+        out[r + 8] = hi;
+        if (r == 0) out[16] = v[16];
     }
 }
 
@@ -1427,7 +1616,7 @@ __global__ __launch_bounds__(256) void k_momentum(
                 });
                 for (int d = 0; d < 3; ++d) tp[d] = tp[d] + tpa[d] + ale[d];
             } else {
-                // symmetric form -m_j (p_i/rho_i^2 + p_j/rho_j^2) r K3 of the
+                                // symmetric form -m_j (p_i/rho_i^2 + p_j/rho_j^2) r K3 of the
                 // reference's difference term plus its ALE1 complement
                 const float4 xf = xmf[i];
                 const float4 vf = velf[i];
@@ -1567,7 +1756,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                         v[2] = v[2] + (vj.z - vi.z) * pes;
                                         });
                 } else {
-                    const float4 xf = xmf[i];
+                                        const float4 xf = xmf[i];
                     const float4 vf = velf1[i];
                     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
                     using RV = NRec2<float4, float4>;
@@ -1712,7 +1901,7 @@ __global__ __launch_bounds__(256) void k_continuity(
                                 });
                 inner = a1 + a2 + a3;
             } else {
-                const float4 xf = xmf[i];
+                                const float4 xf = xmf[i];
                 const float4 vf = velf[i];
                 float s = 0.f;
                 using RC = NRec2<float4, float4>;
@@ -1787,7 +1976,7 @@ __global__ __launch_bounds__(256) void k_psmooth(
                         somp = somp + (rpj.y - pi) * pes;
                                         });
                 } else {
-                    const float4 xf = xmf[i];
+                                        const float4 xf = xmf[i];
                     const float pif = rpf[i].y;
                     float s_a = 0.f, s_p = 0.f;
                     using RP2 = NRec2<float4, float2>;

@@ -25,7 +25,7 @@ PI = C.POINTER(C.c_int32)
 EXPORTS = [
     "sph_ctx_create", "sph_ctx_destroy", "sph_state_upload", "sph_particle_count",
     "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_global_quantities",
-    "sph_monitor_lines", "sph_free_surface", "sph_get_neighbour_lists", "sph_dt", "sph_momentum",
+    "sph_monitor_lines", "sph_free_surface", "sph_get_neighbour_lists", "sph_get_boundary_table", "sph_dt", "sph_momentum",
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
     "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
     "sph_preloop", "sph_step", "sph_step_async", "sph_step_graph", "sph_sync", "sph_get_launch", "sph_set_timing",
@@ -77,6 +77,7 @@ def load():
     L.sph_state_download.argtypes = [C.c_void_p, PD, PD, PD, PD, PD, PD, PD, PI]
     L.sph_get_scalars.argtypes = [C.c_void_p, PD]
     L.sph_get_neighbour_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
+    L.sph_get_boundary_table.argtypes = [C.c_void_p, PI, PI, PD, PD, PD, C.c_int32]
     L.sph_pair_counter.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.sph_global_quantities.argtypes = [C.c_void_p, PD]
     L.sph_monitor_lines.argtypes = [C.c_void_p, C.c_int32, PI, C.c_int32, PD]
@@ -260,6 +261,18 @@ class SphContext:
         self._check(self.L.sph_get_neighbour_lists(self.h, _pi(cnt), _pi(ids), stride),
                     "sph_get_neighbour_lists")
         return cnt, ids
+
+    def btab(self, stride=8):
+        """Boundary table in pg order: ncbf, face, loc, bi, ig (oracle layout)."""
+        n = self.n
+        nc = np.zeros(n, np.int32)
+        face = np.zeros((n, stride), np.int32)
+        loc = np.zeros((n, stride, 3))
+        bi = np.zeros((n, stride, 8))
+        ig = np.zeros((n, stride, 9))
+        self._check(self.L.sph_get_boundary_table(self.h, _pi(nc), _pi(face), _pd(loc), _pd(bi),
+                                                  _pd(ig), stride), "sph_get_boundary_table")
+        return nc, face, loc, bi, ig
 
     # phases -----------------------------------------------------------------
     def phase(self, name):

@@ -261,3 +261,59 @@ def test_global_quantities_compensated(small_case):
     got = [g["mass"], *g["lin_mom"], *g["ang_mom"], g["volume"]]
     for w, h in zip(want, got):
         assert abs(w - h) <= np.spacing(abs(w)), (w, h)
+
+
+# FP32-mixed boundary table vs FP64, per component, relative to the largest
+# |value| of that component: the FP32 interpolation and sums (measured 2e-7
+# at most) and the rare flip of the reference's 1e-3 reuse rule when rob sits
+# within FP32 rounding of the threshold (one ring's table values, ~1e-4)
+BT_FP32_TOL = 3e-4
+BT_FP32_TOL_RMS = 1e-6
+
+
+def test_boundary_table():
+    """ComputeBoundaryDataTab_3D on the device: FP64 bit-identical to the
+    oracle (faces, local coordinates, the 8 integrals and the 3x3 moments), at
+    the pre-loop and after 5 steps; the FP32-mixed quadrature (eight lanes per
+    pair, ring sums) within BT_FP32_TOL of FP64 on the same positions."""
+    case = DamBreak3D(tank=(0.4, 0.2, 0.3), column=(0.12, 0.2, 0.12), dx=0.005, name="bt")
+    tb = build(case)
+    coord = _jittered(tb, seed=5, amp=0.05)
+    print(f"[gpu] boundary table n={len(coord)}", flush=True)
+    o = _oracle(tb, coord)
+    d64 = _device(tb, _rt().FP64, coord)
+    d32 = _device(tb, _rt().FP32_MIXED, coord)
+    try:
+        for k in (0, 5):
+            if k:
+                o.step(k)
+                d64.step(k)
+            a = d64.btab()
+            b = [v if v.ndim == 1 else v[:, :8] for v in o.btab()]
+            nc = b[0]
+            assert np.array_equal(a[0], nc)
+            m = np.arange(a[1].shape[1])[None, :] < nc[:, None]
+            for x, y, name in zip(a[1:], b[1:], ("face", "loc", "bi", "ig")):
+                assert np.array_equal(x[m], y[m]), (k, name)
+        a = d32.btab()
+        b = _device_btab_at_start(tb, coord)
+        assert np.array_equal(a[0], b[0])
+        m = np.arange(a[1].shape[1])[None, :] < b[0][:, None]
+        assert np.array_equal(a[1][m], b[1][m])
+        for x, y, name in zip(a[3:], b[3:], ("bi", "ig")):
+            x, y = x[m], y[m]
+            scale = np.maximum(np.abs(y).max(0), 1e-300)
+            err = np.abs(x - y) / scale
+            print(f"[gpu] fp32 {name} max {err.max():.2e} rms {np.sqrt((err ** 2).mean()):.2e}", flush=True)
+            assert err.max() < BT_FP32_TOL and np.sqrt((err ** 2).mean()) < BT_FP32_TOL_RMS, name
+    finally:
+        d64.close()
+        d32.close()
+
+
+def _device_btab_at_start(tb, coord):
+    d = _device(tb, _rt().FP64, coord)
+    try:
+        return d.btab()
+    finally:
+        d.close()
