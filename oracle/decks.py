@@ -23,6 +23,8 @@ REF_DIR = HERE / "_ref"
 HARNESS = REF_DIR / "harness3d.x"
 HARNESS_HIP = REF_DIR / "harness3d_hip.x"
 REFERENCE = REF_DIR / "sphera3d.x"
+HARNESS2D = REF_DIR / "harness2d.x"
+REFERENCE2D = REF_DIR / "sphera2d.x"
 
 
 def _f(x):
@@ -80,12 +82,65 @@ def deck_text(case, itmax=1000000000, tmax=1000.0):
     return "\n".join(lines) + "\n"
 
 
+def deck_text_2d(case, itmax=1000000000, tmax=1000.0):
+    """v10 deck of a DamBreak2D for the reference's 2D build: vertices (x, z),
+    a LINES section instead of FACES, gravity and monitor points with two
+    components, and none of the 3D-only lines (the CLC flag of
+    ReadInputDomain.f90:109-113, MAXCLOSEBOUNDFACES of
+    ReadInputRunParameters.f90:70-76)."""
+    v = case.vertices()
+    lines = ["10.0.0", "##### TITLE #####", case.name, "##### END TITLE #####",
+             "##### DOMAIN #####", "semi", f"{_f(case.dx)} {_f(case.trunc)}",
+             "##### END DOMAIN #####", "##### VERTICES #####"]
+    for i, p in enumerate(v, start=1):
+        lines.append(f"{i} {_f(p[0])} {_f(p[1])}")
+    lines += ["##### END VERTICES #####", "##### LINES #####"]
+    for lid, vids in sorted(case.lines().items()):
+        lines.append(f"{lid} " + " ".join(str(i) for i in vids))
+    lines += ["##### END LINES #####", "##### BOUNDARIES #####",
+              "Boundary", "1", "fixed", "1 .false.", _f(case.slip_coeff), "111111",
+              "Reservoir", "2", "perimeter", "1", "uniform 009EA8",
+              "std 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[1])}", "1 0 .false.",
+              "##### END BOUNDARIES #####",
+              "##### DBSPH #####", "1. .false. 0.5 0 .false.", ".false. .true.", "0 0",
+              "0 .false.", "0 0 6", "##### END DBSPH #####",
+              "##### BED LOAD TRANSPORT #####", "0 1", "1 0. 0.",
+              "##### end BED LOAD TRANSPORT #####",
+              "##### medium #####", "liquid", "1", f"{_f(case.rho0)} {_f(case.bulk)}",
+              f"{_f(case.alfa_mon)} 0.", _f(case.mu), "##### end medium #####",
+              "##### RUN PARAMETERS #####", f"{_f(tmax)} {int(itmax)}",
+              f"{_f(case.cfl)} {_f(case.vsc_coeff)} 1 1 {_f(case.pesodt)} .false.",
+              f"{_f(case.teta_p)} {_f(case.teta_v)} A",
+              f"1.1 {_f(case.coefnmaxpartj)} 0",
+              ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.", "0",
+              "##### end RUN PARAMETERS #####",
+              "##### general physical properties #####",
+              f"{_f(case.grav[0])} {_f(case.grav[2])}", _f(case.p_ref),
+              "##### end general physical properties #####",
+              "##### restart #####", "##### end restart #####",
+              "##### output regulation #####", "results time 100000.",
+              "restart time 100000.", "print partial 100000000", "control time 100000.",
+              "level time 100000. medium 1", "depth dt_out 100000.",
+              "##### end output regulation #####",
+              "##### draw options #####", "vtkconverter any 100000.",
+              "##### end draw options #####",
+              "##### control points #####", "##### end control points #####",
+              "##### control lines #####"]
+    for label, first, second, ndiv in case.control_lines:
+        lines += [str(label)[:8], " ".join(_f(v) for v in first), " ".join(_f(v) for v in second),
+                  str(int(ndiv))]
+    lines += ["##### end control lines #####"]
+    return "\n".join(lines) + "\n"
+
+
 def write_case_dir(case, root, itmax=1000000000):
     """Lay out a run directory the reference can start from: it concatenates
     ./input/*/*.inp into <case>.inp (src/Main_algorithm/sphera.f90:222)."""
     root = Path(root)
     (root / "input" / "00_deck").mkdir(parents=True, exist_ok=True)
-    (root / "input" / "00_deck" / "deck.inp").write_text(deck_text(case, itmax=itmax))
+    text = deck_text_2d(case, itmax=itmax) if getattr(case, "dim", 3) == 2 else \
+        deck_text(case, itmax=itmax)
+    (root / "input" / "00_deck" / "deck.inp").write_text(text)
     (root / "output").mkdir(exist_ok=True)
     (root / "restart").mkdir(exist_ok=True)
     return root
@@ -95,7 +150,7 @@ def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
                 workdir=None, timeout=3600, exe=None, extra_env=None):
     """Run the reference harness (or ``exe``, e.g. the device drop-in harness
     HARNESS_HIP); returns the path of its dump file."""
-    exe = Path(exe or HARNESS)
+    exe = Path(exe or (HARNESS2D if getattr(case, "dim", 3) == 2 else HARNESS))
     if not exe.exists():
         raise FileNotFoundError(f"{exe} not built (make -f oracle/Makefile.ref)")
     wd = Path(workdir or tempfile.mkdtemp(prefix="sph_ref_")).resolve()
@@ -183,6 +238,8 @@ def read_dump(path):
      hdr["mumx"]) = r.f8(6)
     hdr["cfl"], hdr["vsc_coeff"], hdr["pesodt"], hdr["tetap"], hdr["tetav"] = r.f8(5)
     hdr["dt_alfa_mon"], hdr["maxclose"], numfacce, hdr["numbedges"] = r.i4(4)
+    if ver == 2:
+        return _read_dump_2d(r, hdr, numfacce, nmaxpartj)
     faces = []
     for _ in range(numfacce):
         nf = r.i4()
@@ -234,6 +291,55 @@ def read_dump(path):
             ncb = r.i4(n)
             st["ncbf"] = ncb
             nb = int(ncb.sum())
+            rec = np.dtype([("face", "<i4"), ("loc", "<f8", 3), ("bi", "<f8", 8),
+                            ("ig", "<f8", 9)])
+            st["btab"] = np.frombuffer(r.raw(nb * rec.itemsize), dtype=rec).copy()
+        steps.append(st)
+    return hdr, steps
+
+
+def _read_dump_2d(r, hdr, nsides, nmaxpartj):
+    """2D layout (harness_dump_io.f90 under SPACE_2D): the boundary sides in
+    the header; per step Ncbs and IntNcbs per particle, then IntNcbs records."""
+    sides = []
+    for _ in range(nsides):
+        st, prev = r.i4(2)
+        vtx = r.i4(2)
+        tipo = r.raw(4).decode()
+        vals = r.f8(2 + 27 + 3 + 3 + 3 + 3)
+        mode = r.i4()
+        coeff = r.f8()
+        lam = r.i4()
+        sides.append(dict(stretch=st, previous=prev, vertex=vtx, tipo=tipo, length=vals[0],
+                          angle=vals[1], T=vals[2:11].reshape(3, 3, order="F"),
+                          R=vals[11:20].reshape(3, 3, order="F"),
+                          RN=vals[20:29].reshape(3, 3, order="F"), velocity=vals[29:32],
+                          tratto_velocity=vals[32:35], v1=vals[35:38], v2=vals[38:41],
+                          slip_mode=mode, slip_coeff=coeff, laminar_check=lam))
+    hdr["sides"] = sides
+    steps = []
+    while not r.eof():
+        if r.i4() != 1347704147:
+            raise ValueError("bad step tag")
+        it, n = r.i4(2)
+        t, dt, dtavg = r.f8(3)
+        st = dict(it=it, nag=n, time=t, dt=dt, dt_average=dtavg)
+        for k in ("coord", "vel", "var", "acc"):
+            st[k] = r.f8(3 * n).reshape(n, 3)
+        for k in ("dens", "pres", "dden", "vpres", "mass"):
+            st[k] = r.f8(n)
+        for k in ("cella", "kodvel", "koddens", "laminar_flag"):
+            st[k] = r.i4(n)
+        lists = r.i4()
+        if lists:
+            cnt = r.i4(n)
+            tot = int(cnt.sum())
+            st["nbr_count"] = cnt
+            st["nbr"] = r.i4(tot) if tot else np.zeros(0, np.int32)
+            st["ncbs"] = r.i4(n)
+            nint_ = r.i4(n)
+            st["intncbs"] = nint_
+            nb = int(nint_.sum())
             rec = np.dtype([("face", "<i4"), ("loc", "<f8", 3), ("bi", "<f8", 8),
                             ("ig", "<f8", 9)])
             st["btab"] = np.frombuffer(r.raw(nb * rec.itemsize), dtype=rec).copy()

@@ -37,6 +37,13 @@ class OrFaces(C.Structure):
                 ("bitab", PD)]
 
 
+class OrSides2D(C.Structure):
+    _fields_ = [("ns", C.c_int32), ("T", PD), ("RN", PD), ("v1", PD), ("length", PD),
+                ("angle", PD), ("velocity", PD), ("tvel", PD), ("slip_coeff", PD),
+                ("btype", PI), ("previous", PI), ("slip_mode", PI), ("lam_check", PI),
+                ("bit2d", PD)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -45,6 +52,8 @@ def lib():
         L = C.CDLL(str(LIB_PATH))
         L.or_create.restype = C.c_void_p
         L.or_create.argtypes = [C.POINTER(OrParams), C.POINTER(OrFaces), PD, PD, PD, PD, PD]
+        L.or_create2d.restype = C.c_void_p
+        L.or_create2d.argtypes = [C.POINTER(OrParams), C.POINTER(OrSides2D), PD, PD, PD, PD, PD]
         for name in ("or_destroy",):
             getattr(L, name).argtypes = [C.c_void_p]
             getattr(L, name).restype = None
@@ -81,7 +90,7 @@ def params_from_tables(tb, n, semi=True):
     p = OrParams()
     p.n = n
     p.nmaxpartj = s["nmaxpartj"]
-    p.maxclose = case.maxcloseboundfaces
+    p.maxclose = 4 if getattr(tb, "dim", 3) == 2 else case.maxcloseboundfaces
     p.ncd[:] = [int(v) for v in g.ncd]
     p.extr_lo[:] = [float(v) for v in g.extr_lo]
     p.dcd[:] = [float(v) for v in g.dcd]
@@ -111,6 +120,27 @@ class Oracle:
         mass = np.ascontiguousarray(P["mass"] if mass is None else mass, np.float64)
         self.tb = tb
         self.p = params_from_tables(tb, n, semi)
+        if getattr(tb, "dim", 3) == 2:
+            S = tb.sides
+            self._keep = dict(
+                T=np.ascontiguousarray(S.T.transpose(0, 2, 1).reshape(-1)),
+                RN=np.ascontiguousarray(S.RN.transpose(0, 2, 1).reshape(-1)),
+                v1=np.ascontiguousarray(S.v1.reshape(-1)), L=np.ascontiguousarray(S.length),
+                ang=np.ascontiguousarray(S.angle), vel=np.ascontiguousarray(S.velocity.reshape(-1)),
+                tvel=np.ascontiguousarray(S.tratto_velocity.reshape(-1)),
+                slip=np.ascontiguousarray(S.slip_coeff), bt=np.ascontiguousarray(S.btype, np.int32),
+                prev=np.ascontiguousarray(S.previous, np.int32),
+                mode=np.ascontiguousarray(S.slip_mode, np.int32),
+                lam=np.ascontiguousarray(S.laminar_check, np.int32),
+                bit=np.ascontiguousarray(tb.bit2d.reshape(-1)))
+            k = self._keep
+            sd = OrSides2D(len(S.length), _pd(k["T"]), _pd(k["RN"]), _pd(k["v1"]), _pd(k["L"]),
+                           _pd(k["ang"]), _pd(k["vel"]), _pd(k["tvel"]), _pd(k["slip"]),
+                           _pi(k["bt"]), _pi(k["prev"]), _pi(k["mode"]), _pi(k["lam"]),
+                           _pd(k["bit"]))
+            self.ctx = L.or_create2d(C.byref(self.p), C.byref(sd), _pd(self.coord), _pd(vel),
+                                     _pd(dens), _pd(pres), _pd(mass))
+            return
         F = tb.faces
         self._keep = dict(
             T=np.ascontiguousarray(F.T.transpose(0, 2, 1).reshape(-1)),  # column-major per face

@@ -17,7 +17,11 @@ subroutine write_header(u)
 integer(4),intent(in) :: u
 integer(4) :: kf,nf,st,n,ic
 write(u) 1213417555                                  ! 'SPHH'
+#ifdef SPACE_2D
+write(u) 2, nag, NMAXPARTJ, Grid%nmax, Grid%ncd(1:3)
+#else
 write(u) 1, nag, NMAXPARTJ, Grid%nmax, Grid%ncd(1:3)
+#endif
 write(u) Grid%extr(1:3,1), Grid%extr(1:3,2), Grid%dcd(1:3)
 write(u) Domain%h, Domain%dx, Domain%coefke, Domain%coefkacl, Domain%prif,  &
    Domain%grav(1:3), eta, eta2, Domain%PVolume
@@ -27,6 +31,25 @@ write(u) input_any_t%CFL, input_any_t%vsc_coeff, pesodt, input_any_t%TetaP, &
    input_any_t%TetaV
 ic = 0
 if (dt_alfa_Mon) ic = 1
+#ifdef SPACE_2D
+! 2D: the boundary sides (DefineBoundarySideGeometry2D.f90) with their
+! Tratto data and end-vertex coordinates
+write(u) ic, MAXCLOSEBOUNDSIDES, NumBSides, 0
+do kf=1,NumBSides
+   st = BoundarySide(kf)%stretch
+   write(u) st, BoundarySide(kf)%previous_side, BoundarySide(kf)%vertex(1:2), &
+      Tratto(st)%tipo
+   write(u) BoundarySide(kf)%length, BoundarySide(kf)%angle,                 &
+      BoundarySide(kf)%T(1:3,1:3), BoundarySide(kf)%R(1:3,1:3),               &
+      BoundarySide(kf)%RN(1:3,1:3), BoundarySide(kf)%velocity(1:3),           &
+      Tratto(st)%velocity(1:3), Vertice(1:3,BoundarySide(kf)%vertex(1)),      &
+      Vertice(1:3,BoundarySide(kf)%vertex(2))
+   ic = 0
+   if (Tratto(st)%laminar_no_slip_check) ic = 1
+   write(u) Partz(Tratto(st)%zone)%slip_coefficient_mode,                   &
+      Partz(Tratto(st)%zone)%BC_shear_stress_input, ic
+enddo
+#else
 write(u) ic, input_any_t%MAXCLOSEBOUNDFACES, NumFacce, NumBEdges
 do kf=1,NumFacce
    nf = BFaceList(kf)
@@ -50,6 +73,7 @@ write(u) size(GCBFVector)
 write(u) GCBFVector(:)
 write(u) kerneltab(0:ktrows,0:ktcols), ktdelta
 write(u) BoundIntegralTab(1:BITrows,1:BITcols)
+#endif
 end subroutine write_header
 
 subroutine write_step(u,it,lists)
@@ -71,9 +95,17 @@ write(u) nPartIntorno(1:nag)
 do i=1,nag
    write(u) PartIntorno((i-1)*NMAXPARTJ+1:(i-1)*NMAXPARTJ+nPartIntorno(i))
 enddo
+#ifdef SPACE_2D
+! 2D: Ncbs and IntNcbs per particle, then the IntNcbs records
+! (ComputeBoundaryDataTab_2D.f90 fills IntNcbs records)
+write(u) BoundaryDataPointer(1,1:nag), BoundaryDataPointer(2,1:nag)
+do i=1,nag
+   nc = BoundaryDataPointer(2,i)
+#else
 write(u) BoundaryDataPointer(1,1:nag)
 do i=1,nag
    nc = BoundaryDataPointer(1,i)
+#endif
    do k=1,nc
       ib = BoundaryDataPointer(3,i) + k - 1
       write(u) BoundaryDataTab(ib)%CloBoNum, BoundaryDataTab(ib)%LocXYZ(1:3),  &

@@ -6,8 +6,9 @@
 ! Gest_Input/Gest_Trans (src/Main_algorithm/Gest_Trans.f90:769) set everything
 ! up from a deck exactly as the reference does, and this unit then runs the
 ! same per-step call sequence for the branch the dam-break decks use
-! (Domain%tipo=="semi", time_split=1 leapfrog, 3D, no bodies, no KTGF) and
-! writes the FP64 state the hot path produces to a binary stream file.
+! (Domain%tipo=="semi", time_split=1 leapfrog, no bodies, no KTGF; 3D, or 2D
+! when compiled with -DSPACE_2D) and writes the FP64 state the hot path
+! produces to a binary stream file.
 !
 ! Environment:
 !   SPH_HARNESS_OUT    output file (required)
@@ -41,7 +42,11 @@ character(len=512) :: fout,aux
 dtvel = 0.d0
 on_going_time_step = -999
 EpOrdGrid = 0
+#ifdef SPACE_2D
+SourceSide = 0
+#else
 SourceFace = 0
+#endif
 it_eff = it_start
 BC_zmax_flag = .false.
 if ((Domain%tipo/="semi").or.(Domain%time_split/=1)) then
@@ -92,9 +97,15 @@ call write_header(u)
 ! Pre-loop (time_step_loop.f90:82-173 for the semi branch)
 call liquid_particle_ID_array
 call inlet_sections
+#ifdef SPACE_3D
 call BC_zmax_t0
+#endif
 call CalcVarLength
+#ifdef SPACE_2D
+call ComputeBoundaryDataTab_2D
+#else
 call ComputeBoundaryDataTab_3D
+#endif
 if (Domain%NormFix) call NormFix
 if (it_start==0) call time_step_duration
 it = it_start
@@ -113,6 +124,16 @@ do while (it<nsteps)
    call fluid_particle_imposed_kinematics
    call RHS_momentum_equation
    call Leapfrog_momentum(dt_previous_step,dtvel)
+#ifdef SPACE_2D
+   if (input_any_t%TetaV>1.d-9) call velocity_smoothing
+   call velocity_smoothing_2
+   call Leapfrog_trajectories
+   if (NumOpenSides>0) call CancelOutgoneParticles_2D
+   if (SourceSide/=0) then
+      call inlet_sections
+      call GenerateSourceParticles
+   endif
+#else
    if (input_any_t%TetaV>1.d-9) call velocity_smoothing(BC_zmax_flag)
    call velocity_smoothing_2(BC_zmax_flag)
    call Leapfrog_trajectories
@@ -122,9 +143,24 @@ do while (it<nsteps)
       call GenerateSourceParticles
    endif
    call BC_zmax_anyt
+#endif
    call OrdGrid1
    if (Domain%NormFix) call NormFix
    call CalcVarLength
+#ifdef SPACE_2D
+   call ComputeBoundaryDataTab_2D
+   call liquid_particle_ID_array
+!$omp parallel do default(none) shared(nag,pg,Domain) private(npi)
+   do npi=1,nag
+      pg(npi)%dden = 0.d0
+      pg(npi)%dden_ALE12 = 0.d0
+      pg(npi)%dden_ALE12_frozen = 0.d0
+      pg(npi)%koddens = 0
+      if (pg(npi)%state=="flu") call SASPH_continuity(npi)
+      call Continuity_Equation(npi)
+   enddo
+!$omp end parallel do
+#else
    do i_zone=1,NPartZone
       if (Partz(i_zone)%tipo=="zmax") then
          BC_zmax_flag = .true.
@@ -154,9 +190,14 @@ do while (it<nsteps)
       write(*,*) "harness: MAXCLOSEBOUNDFACES exceeded",Ncbf_Max
       stop 4
    endif
+#endif
    call Leapfrog_continuity
    call EoS_fluid_loop
+#ifdef SPACE_2D
+   if (input_any_t%TetaP>zero) call PressureSmoothing_2D
+#else
    if (input_any_t%TetaP>zero) call PressureSmoothing_3D
+#endif
    if (mod(it,every)==0) then
       call write_step(u,it,lists)
       if ((pelo==1).and.(nlines>0)) then

@@ -42,6 +42,12 @@ struct Ctx {
     // boundary table
     std::vector<int> bn, bface;
     std::vector<double> bloc, bbi, big;
+    // 2D (x-z plane): boundary sides instead of faces
+    int dim = 3;
+    int ns = 0;
+    std::vector<double> sT, sRN, sv1, slen, sang, stvel, sfvel, sslip;
+    std::vector<int> sbtype, sprev, sslipmode, slam;
+    double bit2d[22][6];
     double dt = 0.0, dt_average = 0.0, time = 0.0;
     int it = 0;
     int on_going = -999;
@@ -65,7 +71,7 @@ inline int particle_cell(const Ctx& c, const double* x) {
     const double zp = x[2] - c.p.extr_lo[2];
     const int i = (int)std::ceil(xp / c.p.dcd[0]);
     const int k = (int)std::ceil(zp / c.p.dcd[2]);
-    const int j = (int)std::ceil(yp / c.p.dcd[1]);
+    const int j = c.dim == 2 ? 1 : (int)std::ceil(yp / c.p.dcd[1]);  // :59-61 (2D: j = 1)
     return cell_number(c, i, j, k);
 }
 
@@ -265,6 +271,292 @@ void boundary_tab_particle(Ctx& c, int npi) {
                          &c.big[((size_t)npi * maxc + q) * 9]);
 }
 
+
+// ---------------------------------------------------------------------------
+// 2D (SPACE_2D build): boundary sides in the x-z plane.  Side s has T(r,c) at
+// sT[9*s + 3*c + r] (column-major, 0-based r,c), first vertex sv1[3*s..].
+// ---------------------------------------------------------------------------
+const double PIGRECO = 3.14159265358979;         // Static_allocation_module.f90:50
+const double KERNELCONST2D = 0.454728408833987;  // Static_allocation_module.f90:58
+const double DELTAX_BIT = 0.1;                   // SA_SPH_module.f90:28
+
+inline double ST(const Ctx& c, int s, int r, int col) { return c.sT[9 * s + col * 3 + r]; }
+inline double SRN(const Ctx& c, int s, int r, int col) { return c.sRN[9 * s + col * 3 + r]; }
+
+// w.f90:48-57
+double w_fn(double r, double h, double coef) {
+    const double sq = r / h;
+    double q;
+    if (sq <= 1.0) {
+        q = 0.666666667 + sq * sq * (sq * 0.5 - 1.0);
+    } else if (sq >= 2.0) {
+        q = 0.0;
+    } else {
+        const double dms = 2.0 - sq;
+        q = dms * dms * dms / 6.0;
+    }
+    return q * coef;
+}
+
+// WIntegr.f90:60-82
+double wintegr(double ri, double hh) {
+    const double ro = ri / hh;
+    if (0.0 <= ro && ro < 1.0) {
+        const double ro2 = ro * ro, ro3 = ro2 * ro;
+        const double w1 = (0.5 + -0.375 * ro2 + 0.15 * ro3) * ro2;
+        return KERNELCONST2D * (0.35 - w1);
+    } else if (1.0 <= ro && ro < 2.0) {
+        const double q1 = 2.0 - ro, q2 = q1 * q1, q4 = q2 * q2;
+        const double w2 = (-0.125 + 0.05 * q1) * q4;
+        return -KERNELCONST2D * w2;
+    }
+    return 0.0;
+}
+
+// J2Wro2.f90:55-70
+double j2wro2(double ro) {
+    const double a1 = 0.00833333333333333, a2 = 0.26666666666666667;
+    const double ro2 = ro * ro, ro3 = ro2 * ro;
+    if (0.0 <= ro && ro < 1.0) return KERNELCONST2D * (0.25 - (a1 * (40.0 - 36.0 * ro2 + 15.0 * ro3) * ro3));
+    if (1.0 <= ro && ro < 2.0)
+        return KERNELCONST2D * (a2 - (a1 * (80.0 - 90.0 * ro + 36.0 * ro2 - 5.0 * ro3) * ro3));
+    return 0.0;
+}
+
+// InterpolateBoundaryIntegrals2D.f90:61-70 (cols: 1-based columns 1..5)
+void interp2d(const Ctx& c, int ncols, const int* cols, double xx, double* func) {
+    if (xx < c.bit2d[0][0]) xx = c.bit2d[0][0];
+    const int ii = (int)((xx - c.bit2d[0][0]) / DELTAX_BIT) + 1;
+    const double xi = c.bit2d[ii - 1][0];
+    for (int nc = 0; nc < ncols; ++nc) {
+        const int jj = cols[nc];
+        const double fi = c.bit2d[ii - 1][jj], fip1 = c.bit2d[ii][jj];
+        func[nc] = fi + (fip1 - fi) * (xx - xi) / DELTAX_BIT;
+    }
+}
+
+// FindBoundaryIntersection2D.f90:59-86
+void intersection2d(const Ctx& c, double xpi, double ypi, double sidelen, double& xpmin,
+                    double& xpmax, double& interlen) {
+    const double minlen = 0.01 * c.p.h;
+    if (ypi < 0.0) ypi = 0.0;
+    const double halfchord = std::sqrt(c.square_doubleh - ypi * ypi);
+    const double XA = xpi - halfchord, XB = xpi + halfchord;
+    xpmin = 0.0;
+    if (XA > xpmin) xpmin = XA;
+    xpmax = sidelen;
+    if (XB < xpmax) xpmax = XB;
+    interlen = xpmax - xpmin;
+    if (interlen <= minlen) interlen = 0.0;
+}
+
+// ComputeSurfaceIntegral_WdS2D.f90:57-85
+double surface_wds2d(const Ctx& c, double xpi, double ypi, double xpmin, double interlen) {
+    double sum = 0.0;
+    const double dsrif = c.p.h / 2.0;
+    if (interlen > 0.0) {
+        if (ypi < 0.0) ypi = 0.0;
+        const double ypiq = ypi * ypi;
+        int ndiv = (int)(interlen / dsrif + 0.5);
+        if (ndiv < 2) ndiv = 2;
+        const double deltas = interlen / ndiv;
+        double xpip = xpmin - 0.5 * deltas;
+        for (int ipt = 1; ipt <= ndiv; ++ipt) {
+            xpip = xpip + deltas;
+            const double dxpip = xpip - xpi;
+            const double risq = dxpip * dxpip + ypiq;
+            const double ris = std::sqrt(risq);
+            sum = sum + w_fn(ris, c.p.h, c.p.coefke) * deltas;
+        }
+    }
+    return sum;
+}
+
+// BoundaryVolumeIntegrals2D.f90:60-97
+void volume_bvi2d(const Ctx& c, double xpi, double ypi, double xpmin, double xpmax, double interlen,
+                  double& IntWdV, double* IntDpWdV) {
+    const double dalfarif = PIGRECO / 10;
+    IntWdV = 0.0;
+    IntDpWdV[0] = IntDpWdV[1] = 0.0;
+    if (interlen <= 0.0) return;
+    const double yplimite = 0.05 * c.p.h;
+    if (ypi < yplimite) ypi = yplimite;
+    const double alfaA = std::atan2(xpmin - xpi, ypi), alfaB = std::atan2(xpmax - xpi, ypi);
+    const double intalfa = alfaB - alfaA;
+    int ndiv = (int)(intalfa / dalfarif + 0.5);
+    if (ndiv < 1) ndiv = 1;
+    const double dalfa = intalfa / ndiv;
+    double alfa_k = alfaA - 0.5 * dalfa;
+    for (int k = 1; k <= ndiv; ++k) {
+        alfa_k = alfa_k + dalfa;
+        const double sa = std::sin(alfa_k), ca = std::cos(alfa_k);
+        const double rb = ypi / ca;
+        const double rob = rb / c.p.h;
+        const double mult = c.p.h * j2wro2(rob) * dalfa;
+        IntDpWdV[0] = IntDpWdV[0] + sa * mult;
+        IntDpWdV[1] = IntDpWdV[1] - ca * mult;
+        IntWdV = IntWdV + wintegr(rb, c.p.h) * dalfa;
+    }
+    if (ypi == yplimite) IntWdV = 0.5;
+}
+
+// ComputeVolumeIntegral_WdV2D.f90:69-132 (called with Ncbslocal = 2)
+int volume_wdv2d(const Ctx& c, int icbs, const int* side, const double* loc, double xpmin,
+                 double xpmax, double interlen, double& V) {
+    const double dalfarif = PIGRECO / 10;
+    V = 0.0;
+    if (interlen <= 0.0) return 0;
+    const double yplimite = 0.05 * c.p.h;
+    const double xpi = loc[2 * icbs], ypi = loc[2 * icbs + 1];
+    if (ypi >= yplimite) {
+        const double alfaA = std::atan2(xpmin - xpi, ypi), alfaB = std::atan2(xpmax - xpi, ypi);
+        const double intalfa = alfaB - alfaA;
+        int ndiv = (int)(intalfa / dalfarif + 0.5);
+        if (ndiv < 2) ndiv = 2;
+        const double dalfa = intalfa / ndiv;
+        double alfa = alfaA - 0.5 * dalfa;
+        for (int ipt = 1; ipt <= ndiv; ++ipt) {
+            alfa = alfa + dalfa;
+            const double ta = std::tan(alfa);
+            const double ris = ypi * std::sqrt(1.0 + ta * ta);
+            V = V + wintegr(ris, c.p.h) * dalfa;
+        }
+    } else {
+        const int jcbs = 1 - icbs;
+        const double ypj = loc[2 * jcbs + 1];
+        if (ypj <= yplimite) {
+            double angle = 0.0;
+            if (c.sprev[side[0]] == side[1] + 1)
+                angle = c.sang[side[0]];
+            else if (c.sprev[side[1]] == side[0] + 1)
+                angle = c.sang[side[1]];
+            else
+                return 84;  // diagnostic(8,4): sides not consecutive
+            V = 0.5 * (angle + PIGRECO) / (2.0 * PIGRECO);
+        } else {
+            V = (xpmin >= xpi && xpi <= xpmax) ? 0.5 : 0.0;
+        }
+    }
+    return 0;
+}
+
+// FindCloseBoundarySides2D.f90:68-170, SelectCloseBoundarySides2D.f90:66-120,
+// ComputeBoundaryDataTab_2D.f90:71-162 for one particle.  bn = IntNcbs, or
+// -code on error (88: more than MAXCLOSEBOUNDSIDES sides, 95: a side type
+// other than fixe/tapi, 89: more than two selected sides, 84: sides not
+// consecutive).
+void boundary_tab_particle_2d(Ctx& c, int npi) {
+    const size_t mc = c.p.maxclose;
+    c.bn[npi] = 0;
+    if (c.cella[npi] <= 0) return;
+    const double* X = &c.coord[3 * npi];
+    const double px = X[0], pz = X[2];
+    const double dh = c.doubleh;
+    int ncbs = 0;
+    int clo[4];
+    double loc[8];
+    for (int isi = 0; isi < c.ns; ++isi) {
+        if (c.sbtype[isi] == 6) continue;  // peri
+        const double p1x = c.sv1[3 * isi], p1z = c.sv1[3 * isi + 2];
+        const double P1P0 = px - p1x, P1P1 = pz - p1z;
+        const double s0 = ST(c, isi, 0, 0), s1 = ST(c, isi, 2, 0);
+        const double n0 = ST(c, isi, 0, 2), n1 = ST(c, isi, 2, 2);
+        const double sidel = c.slen[isi];
+        const double xp = P1P0 * s0 + P1P1 * s1;
+        const double yp = P1P0 * n0 + P1P1 * n1;
+        const double xpmin = -dh, xpmax = sidel + dh, ypmin = -dh, ypmax = dh;
+        if ((xpmin < xp && xp < xpmax) && (ypmin < yp && yp < ypmax)) {
+            double ypmx = 0.0, ypmn = 0.0;
+            if (xp < 0.0) {
+                ypmx = std::sqrt(c.square_doubleh - xp * xp);
+                ypmn = -ypmx;
+            } else if (xp <= sidel) {
+                ypmx = ypmax;
+                ypmn = ypmin;
+            } else if (xp < xpmax) {
+                const double Lmxpq = (sidel - xp) * (sidel - xp);
+                ypmx = std::sqrt(c.square_doubleh - Lmxpq);
+                ypmn = -ypmx;
+            }
+            if (ypmn < yp && yp < ypmx) {
+                if (ncbs >= 4) { c.bn[npi] = -88; return; }
+                clo[ncbs] = isi;
+                loc[2 * ncbs] = xp;
+                loc[2 * ncbs + 1] = yp;
+                ncbs = ncbs + 1;
+            }
+        }
+    }
+    for (int q = 0; q < ncbs; ++q) {
+        const int bt = c.sbtype[clo[q]];
+        if (bt != 0 && bt != 1) { c.bn[npi] = -95; return; }
+    }
+    if (ncbs == 2) {
+        const int a = clo[0], b = clo[1];
+        if (c.sprev[a] == b + 1)
+            loc[2] = c.slen[b] - loc[2];
+        else if (c.sprev[b] == a + 1)
+            loc[0] = c.slen[a] - loc[0];
+    }
+    // SelectCloseBoundarySides2D: fixe/tapi sides with y > 0
+    int nint = 0;
+    int iside[4];
+    double iloc[8];
+    for (int q = 0; q < ncbs; ++q) {
+        if (loc[2 * q + 1] > 0.0) {
+            iside[nint] = clo[q];
+            iloc[2 * nint] = loc[2 * q];
+            iloc[2 * nint + 1] = loc[2 * q + 1];
+            nint = nint + 1;
+        }
+    }
+    if (nint == 0) return;
+    if (nint > 2) { c.bn[npi] = -89; return; }
+    for (int icbs = 0; icbs < nint; ++icbs) {
+        const size_t sl = (size_t)npi * mc + icbs;
+        c.bface[sl] = iside[icbs];
+        c.bloc[3 * sl] = iloc[2 * icbs];
+        c.bloc[3 * sl + 1] = iloc[2 * icbs + 1];
+        c.bloc[3 * sl + 2] = 0.0;
+        double* BI = &c.bbi[8 * sl];
+        const double ypi = iloc[2 * icbs + 1];
+        const double deltai = ypi / c.p.h;
+        double F[5];
+        if (nint == 2) {
+            double xpmin, xpmax, interlen;
+            intersection2d(c, iloc[2 * icbs], ypi, c.slen[iside[icbs]], xpmin, xpmax, interlen);
+            const double IntWdS = surface_wds2d(c, iloc[2 * icbs], ypi, xpmin, interlen);
+            double IntWdV, IntDp[2];
+            volume_bvi2d(c, iloc[2 * icbs], ypi, xpmin, xpmax, interlen, IntWdV, IntDp);
+            double IntWdV1;
+            const int rc = volume_wdv2d(c, icbs, iside, iloc, xpmin, xpmax, interlen, IntWdV1);
+            if (rc) { c.bn[npi] = -rc; return; }
+            const int cols[3] = {3, 4, 5};
+            interp2d(c, 3, cols, deltai, F);
+            BI[0] = IntWdS;
+            BI[1] = IntWdV;
+            BI[2] = IntWdV1;
+            BI[3] = IntDp[0];
+            BI[4] = IntDp[1];
+            BI[5] = F[0] / c.squareh;
+            BI[6] = F[1] / c.squareh;
+            BI[7] = F[2] / c.squareh;
+        } else {
+            const int cols[5] = {1, 2, 3, 4, 5};
+            interp2d(c, 5, cols, deltai, F);
+            BI[0] = F[0] / c.p.h;
+            BI[1] = F[1];
+            BI[2] = F[1];
+            BI[3] = 0.0;
+            BI[4] = 0.0;
+            BI[5] = F[2] / c.squareh;
+            BI[6] = F[3] / c.squareh;
+            BI[7] = F[4] / c.squareh;
+        }
+    }
+    c.bn[npi] = nint;
+}
+
 }  // namespace
 
 extern "C" {
@@ -323,6 +615,33 @@ void* or_create(const or_params* p, const or_faces* f, const double* coord, cons
         c->id[i] = i;
         c->cella[i] = particle_cell(*c, &c->coord[3 * i]);  // SetParticleParameters.f90:129
     }
+    or_ordgrid(c);
+    return c;
+}
+
+void* or_create2d(const or_params* p, const or_sides2d* sd, const double* coord, const double* vel,
+                  const double* dens, const double* pres, const double* mass) {
+    Ctx* c = (Ctx*)or_create(p, nullptr, coord, vel, dens, pres, mass);
+    c->dim = 2;
+    const int ns = sd ? sd->ns : 0;
+    c->ns = ns;
+    if (ns > 0) {
+        c->sT.assign(sd->T, sd->T + 9 * ns);
+        c->sRN.assign(sd->RN, sd->RN + 9 * ns);
+        c->sv1.assign(sd->v1, sd->v1 + 3 * ns);
+        c->slen.assign(sd->length, sd->length + ns);
+        c->sang.assign(sd->angle, sd->angle + ns);
+        c->sfvel.assign(sd->velocity, sd->velocity + 3 * ns);
+        c->stvel.assign(sd->tvel, sd->tvel + 3 * ns);
+        c->sslip.assign(sd->slip_coeff, sd->slip_coeff + ns);
+        c->sbtype.assign(sd->btype, sd->btype + ns);
+        c->sprev.assign(sd->previous, sd->previous + ns);
+        c->sslipmode.assign(sd->slip_mode, sd->slip_mode + ns);
+        c->slam.assign(sd->lam_check, sd->lam_check + ns);
+        std::memcpy(c->bit2d, sd->bit2d, sizeof(c->bit2d));
+    }
+    // cells again with j = 1 (ParticleCellNumber.f90:59-61), then OrdGrid1
+    for (int i = 0; i < c->n; ++i) c->cella[i] = particle_cell(*c, &c->coord[3 * i]);
     or_ordgrid(c);
     return c;
 }
@@ -555,6 +874,15 @@ int or_boundary_tab(void* ctx) {
         c.bbi.resize(n * mc * 8);
         c.big.resize(n * mc * 9);
     }
+    if (c.dim == 2) {
+        if (!c.p.semi || c.ns == 0) return 0;
+        int code = 0;
+        for (int npi = 0; npi < n; ++npi) {  // ComputeBoundaryDataTab_2D is a serial loop
+            boundary_tab_particle_2d(c, npi);
+            if (c.bn[npi] < 0 && code == 0) code = -c.bn[npi];
+        }
+        return code;
+    }
     if (!c.p.semi || c.nf == 0) return 0;
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad)
@@ -611,7 +939,72 @@ int or_momentum(void* ctx) {
         const int ncbf = c.p.semi ? c.bn[npi] : 0;
         const double* vi = &c.var[3 * npi];
         // --- SA-SPH wall terms (called before the inner sums) ---
-        if (ncbf > 0) {
+        if (ncbf > 0 && c.dim == 2) {
+            // AddBoundaryContributions_to_ME2D.f90:121-405, fixe/tapi sides
+            const double ro0 = c.p.den0, roi = c.dens[npi], pressi = c.pres[npi];
+            const double distpimin = c.p.dx;
+            double VMo[2] = {0.0, 0.0}, VSh[2] = {0.0, 0.0}, RGs[2] = {0.0, 0.0};
+            double ALEt[3] = {0.0, 0.0, 0.0};
+            const double g2[2] = {c.p.grav[0], c.p.grav[2]};
+            for (int q = 0; q < ncbf; ++q) {
+                const size_t sl = (size_t)npi * mc + q;
+                const int sd = c.bface[sl];
+                const double xpi = c.bloc[3 * sl], ypi = c.bloc[3 * sl + 1];
+                const double* BI = &c.bbi[8 * sl];
+                const double IntWdS = BI[0], IntWdV = BI[2], IntWd1s0 = BI[5], IntWd3s0 = BI[6];
+                const double ss[2] = {ST(c, sd, 0, 0), ST(c, sd, 2, 0)};
+                const double nn[2] = {ST(c, sd, 0, 2), ST(c, sd, 2, 2)};
+                double GravN = 0.0;
+                GravN = GravN + c.p.grav[0] * nn[0];
+                GravN = GravN + c.p.grav[2] * nn[1];
+                const double sidevel[2] = {c.sbtype[sd] == 1 ? c.sfvel[3 * sd] : 0.0,
+                                           c.sbtype[sd] == 1 ? c.sfvel[3 * sd + 2] : 0.0};
+                const double Dvel[2] = {2.0 * (vi[0] - sidevel[0]), 2.0 * (vi[2] - sidevel[1])};
+                const double DvelN = Dvel[0] * nn[0] + Dvel[1] * nn[1];
+                const double distpi = std::fmax(ypi, distpimin);
+                double pressib = pressi;
+                const double pressibmin = 10.0 * ro0 * distpimin;
+                if (DvelN <= 0.0) {
+                    if (pressib < pressibmin) pressib = pressibmin;
+                }
+                const double PressB = pressib - ro0 * GravN * distpi;
+                const double Qii = IntWdS * (pressib + PressB) / roi;
+                double RG[2] = {0.0, 0.0};
+                for (int i = 0; i < 2; ++i) {
+                    const int ai = i == 0 ? 0 : 2;
+                    for (int j = 0; j < 2; ++j) RG[i] = RG[i] + SRN(c, sd, ai, j == 0 ? 0 : 2) * g2[j];
+                    RG[i] = RG[i] * IntWdV;
+                }
+                RGs[0] = RGs[0] + RG[0];
+                RGs[1] = RGs[1] + RG[1];
+                ALEt[0] = ALEt[0] - nn[0] * Qii;
+                ALEt[2] = ALEt[2] - nn[1] * Qii;
+                if (xpi >= 0.0 && xpi <= c.slen[sd]) {
+                    double slip = 0.0, cinv = 0.0;
+                    if (c.sslipmode[sd] == 1) {
+                        slip = c.sslip[sd];
+                        cinv = c.p.kin_visc;
+                    }
+                    const double Monvisc = c.p.alfamon * c.p.celerita * h;
+                    const double TN = Monvisc * DvelN * IntWd3s0;
+                    const double TT[2] = {TN * nn[0], TN * nn[1]};
+                    double SVforce = 0.0;
+                    if (c.laminar[npi] == 1 || c.slam[sd] == 0) SVforce = 2.0 * cinv * slip * IntWd1s0;
+                    const double DvM[2] = {2.0 * vi[0], 2.0 * vi[2]};
+                    for (int i = 0; i < 2; ++i) {
+                        VMo[i] = VMo[i] + TT[i];
+                        VSh[i] = VSh[i] + DvM[i] * SVforce;
+                    }
+                    (void)ss;
+                }
+            }
+            tp[0] = tp[0] - RGs[0] - ALEt[0];
+            tp[2] = tp[2] - RGs[1] - ALEt[2];
+            td[0] = td[0] - VMo[0];
+            td[2] = td[2] - VMo[1];
+            tv[0] = tv[0] - VSh[0];
+            tv[2] = tv[2] - VSh[1];
+        } else if (ncbf > 0) {
             double ALEt[3] = {0, 0, 0}, gradpt[3] = {0, 0, 0}, VM[3] = {0, 0, 0}, VS[3] = {0, 0, 0};
             for (int q = 0; q < ncbf; ++q) {
                 const size_t s = (size_t)npi * mc + q;
@@ -737,7 +1130,26 @@ int or_momentum(void* ctx) {
         }
         c.laminar[npi] = lam;
         double* a = &c.acc[3 * npi];
-        if (ncbf > 0) {
+        if (ncbf > 0 && c.dim == 2) {
+            // AddElasticBoundaryReaction_2D.f90:66-99 over the IntNcbs sides
+            double br[3] = {0, 0, 0};
+            const double ypimin = 0.25 * c.p.dx;
+            const double celer02 = c.p.eps / c.p.den0;
+            for (int q = 0; q < ncbf; ++q) {
+                const size_t sl = (size_t)npi * mc + q;
+                const int sd = c.bface[sl];
+                const double xpi = c.bloc[3 * sl], ypi = c.bloc[3 * sl + 1];
+                if (ypi < ypimin && xpi > 0.0 && xpi < c.slen[sd]) {
+                    double vin = 0.0;
+                    for (int k = 0; k < 3; ++k) vin = vin + vi[k] * ST(c, sd, k, 2);
+                    if (vin < 0.0) {
+                        const double nr = -1.0 * celer02 * std::log((h + ypi - ypimin) / h) / h;
+                        for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * ST(c, sd, d, 2);
+                    }
+                }
+            }
+            for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + c.p.grav[d] + br[d];
+        } else if (ncbf > 0) {
             // AddElasticBoundaryReaction_3D (faces part; no convex edges)
             double br[3] = {0, 0, 0};
             const double zimin = 0.25 * c.p.dx;
@@ -795,7 +1207,25 @@ int or_vsmooth(void* ctx) {
                 for (int d = 0; d < 3; ++d) v[d] = v[d] + (c.vel[3 * npj + d] - vel_i[d]) * pes;
             }
             const int ncbf = c.p.semi ? c.bn[npi] : 0;
-            for (int q = 0; q < ncbf; ++q) {
+            for (int q = 0; q < ncbf && c.dim == 2; ++q) {
+                // velocity_smoothing_SA_SPH_2D.f90:64-118 (fixe/tapi sides)
+                const size_t sl = (size_t)npi * mc + q;
+                const int sd = c.bface[sl];
+                const double sss[2] = {ST(c, sd, 0, 0), ST(c, sd, 2, 0)};
+                const double nnn[2] = {ST(c, sd, 0, 2), ST(c, sd, 2, 2)};
+                const double DVG[2] = {2.0 * (c.stvel[3 * sd] - vel_i[0]),
+                                       2.0 * (c.stvel[3 * sd + 2] - vel_i[2])};
+                const double DVL0 = sss[0] * DVG[0] + sss[1] * DVG[1];
+                const double DVL1 = nnn[0] * DVG[0] + nnn[1] * DVG[1];
+                const double IntWdV = c.bbi[8 * sl + 2];
+                double slip = 0.0;
+                if (c.sslipmode[sd] == 1) slip = c.sslip[sd];
+                const double BCL0 = DVL0 * IntWdV * slip;
+                const double BCL1 = DVL1 * IntWdV;
+                v[0] = v[0] + (sss[0] * BCL0 + nnn[0] * BCL1);
+                v[2] = v[2] + (sss[1] * BCL0 + nnn[1] * BCL1);
+            }
+            for (int q = 0; q < ncbf && c.dim == 3; ++q) {
                 const size_t s = (size_t)npi * mc + q;
                 const int f = c.bface[s];
                 const int bt = c.btype[f];
@@ -868,7 +1298,30 @@ int or_continuity(void* ctx) {
         c.koddens[npi] = 0;
         const double* vi = &c.var[3 * npi];
         const int ncbf = c.p.semi ? c.bn[npi] : 0;
-        if (ncbf > 0) {
+        if (ncbf > 0 && c.dim == 2) {
+            // SASPH_continuity.f90:126-169 with AddBoundaryContribution_to_CE2D.f90:
+            // 74-150 (fixe/tapi sides): grad_u_SA(1), grad_w_SA(3)
+            double gu[3] = {0.0, 0.0, 0.0}, gw[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < ncbf; ++q) {
+                const size_t sl = (size_t)npi * mc + q;
+                const int sd = c.bface[sl];
+                const double IntWdS = c.bbi[8 * sl];
+                double dvel[2];
+                if (c.sbtype[sd] == 0) {
+                    dvel[0] = 2.0 * (-vi[0]);
+                    dvel[1] = 2.0 * (-vi[2]);
+                } else {
+                    dvel[0] = 2.0 * (c.sfvel[3 * sd] - vi[0]);
+                    dvel[1] = 2.0 * (c.sfvel[3 * sd + 2] - vi[2]);
+                }
+                for (int ii = 0; ii < 2; ++ii) {
+                    const int a = ii == 0 ? 0 : 2;
+                    gu[a] = gu[a] - dvel[0] * ST(c, sd, a, 2) * IntWdS;
+                    gw[a] = gw[a] - dvel[1] * ST(c, sd, a, 2) * IntWdS;
+                }
+            }
+            if (c.koddens[npi] == 0) dd = dd - c.dens[npi] * (gu[0] + 0.0 + gw[2]);
+        } else if (ncbf > 0) {
             double gu = 0.0, gv = 0.0, gw = 0.0;  // grad_u_SA(1), grad_v_SA(2), grad_w_SA(3)
             for (int q = 0; q < ncbf; ++q) {
                 const size_t s = (size_t)npi * mc + q;
@@ -959,6 +1412,26 @@ int or_psmooth(void* ctx) {
                 somp = somp + (c.pres[npj] - pi) * pes;
             }
             const int ncbf = c.p.semi ? c.bn[npi] : 0;
+            if (c.dim == 2) {
+                // PressureSmoothing_2D.f90:124-214, SmoothVersion "b", fixe/tapi sides
+                double VFT = 0.0;
+                const double ro0i = c.p.den0;
+                for (int q = 0; q < ncbf; ++q) {
+                    const size_t sl = (size_t)npi * mc + q;
+                    const int sd = c.bface[sl];
+                    const double sss[2] = {ST(c, sd, 0, 0), ST(c, sd, 2, 0)};
+                    const double nnn[2] = {ST(c, sd, 0, 2), ST(c, sd, 2, 2)};
+                    const double mf0 = sss[0] * c.p.grav[0] + sss[1] * c.p.grav[2];
+                    const double mf1 = nnn[0] * c.p.grav[0] + nnn[1] * c.p.grav[2];
+                    const double IntWdV = c.bbi[8 * sl + 1];
+                    VFT = VFT + IntWdV;
+                    somp = somp + ro0i * (mf0 * c.bbi[8 * sl + 3] + mf1 * c.bbi[8 * sl + 4]);
+                }
+                const double TetaP1 = c.p.tetap * c.p.celerita * c.dt / c.p.h;
+                app = app + VFT + 0.0 + 0.0 + 0.0;
+                c.vpres[npi] = pi + TetaP1 * (somp + (pi - pi) * 0.0 + (pi - pi) * 0.0 + (p0i - pi) * (1.0 - app));
+                continue;
+            }
             for (int q = 0; q < ncbf; ++q) {
                 const size_t s = (size_t)npi * mc + q;
                 if (c.bloc[3 * s + 2] > 0.0) app = app + c.bbi[8 * s + 1];
@@ -973,6 +1446,11 @@ int or_psmooth(void* ctx) {
     const double tp1 = c.p.tetap * c.p.celerita * c.dt / c.p.h;
     for (int i = 0; i < n; ++i) {
         if (c.koddens[i] != 0) continue;
+        if (c.dim == 2) {  // PressureSmoothing_2D.f90:217-231: p = vpres, rho = EoS^-1(p)
+            c.pres[i] = c.vpres[i];
+            c.dens[i] = c.p.den0 * (1.0 + (c.pres[i] - c.p.prif) / c.p.eps);
+            continue;
+        }
         c.pres[i] = c.pres[i] + tp1 * c.vpres[i];
         c.dens[i] = c.p.den0 * (1.0 + (c.pres[i] - c.p.prif) / c.p.eps);
     }

@@ -45,6 +45,22 @@ typedef struct or_faces {
 
 void* or_create(const or_params* p, const or_faces* f, const double* coord, const double* vel,
                 const double* dens, const double* pres, const double* mass);
+
+/* 2D (SPACE_2D) boundary sides in BoundarySide order: T and RN column-major
+ * 3x3 per side, v1 = first vertex (x, 0, z), btype codes as for faces
+ * (6 = peri), previous = previous_side (1-based, 0 = none), velocity =
+ * BoundarySide%velocity, tvel = Tratto%velocity, bit2d = BoundIntegralTab2D
+ * (22 x 6, row-major).  The particles lie in the x-z plane (y = 0); the
+ * grid has ncd[1] = 1; maxclose must be MAXCLOSEBOUNDSIDES (4). */
+typedef struct or_sides2d {
+    int32_t ns;
+    const double *T, *RN, *v1, *length, *angle, *velocity, *tvel, *slip_coeff;
+    const int32_t *btype, *previous, *slip_mode, *lam_check;
+    const double *bit2d;
+} or_sides2d;
+
+void* or_create2d(const or_params* p, const or_sides2d* s, const double* coord, const double* vel,
+                  const double* dens, const double* pres, const double* mass);
 void or_destroy(void* ctx);
 int or_nag(void* ctx);
 
@@ -61,7 +77,7 @@ int or_get_btab(void* ctx, int32_t* ncbf, int32_t* face, double* loc, double* bi
 /* phases (time_step_loop.f90 order); each returns 0 or a diagnostic code */
 int or_ordgrid(void* ctx);          /* OrdGrid1 */
 int or_calcvarlength(void* ctx);    /* CalcVarLength (fluid part) */
-int or_boundary_tab(void* ctx);     /* ComputeBoundaryDataTab_3D */
+int or_boundary_tab(void* ctx);     /* ComputeBoundaryDataTab_3D (2D: _2D) */
 int or_dt(void* ctx);               /* time_step_duration */
 int or_momentum(void* ctx);         /* RHS_momentum_equation */
 int or_leapfrog_mom(void* ctx, double dt_previous);
@@ -70,7 +86,7 @@ int or_leapfrog_traj(void* ctx);
 int or_continuity(void* ctx);       /* SASPH_continuity + Continuity_Equation loop */
 int or_leapfrog_cont(void* ctx);
 int or_eos(void* ctx);
-int or_psmooth(void* ctx);          /* PressureSmoothing_3D */
+int or_psmooth(void* ctx);          /* PressureSmoothing_3D (2D: _2D) */
 int or_preloop(void* ctx);          /* lists, boundary table, first dt */
 int or_step(void* ctx, int nsteps); /* whole leapfrog time steps */
 int64_t or_pair_count(void* ctx);   /* sum_i (nPartIntorno(i)-1) of the current lists */

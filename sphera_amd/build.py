@@ -47,12 +47,14 @@ def build_oracle():
 
 
 def build_reference():
-    """flang build of the reference from /root/reference (only where it is mounted)."""
+    """flang builds of the reference (3D and 2D) from /root/reference (only
+    where it is mounted)."""
     if not Path("/root/reference/src").is_dir():
         return None
     if not shutil.which("/opt/rocm/lib/llvm/bin/flang"):
         return None
     _run(["make", "-s", "-f", "oracle/Makefile.ref", "-j8"])
+    _run(["make", "-s", "-f", "oracle/Makefile.ref", "-j8", "DIM=2"])
     return ROOT / "oracle" / "_ref" / "harness3d.x"
 
 

@@ -5,7 +5,7 @@ from /root/reference/src) on project-authored decks.
 
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
-With names (tiny, small_jitter, removal, pelo) only those fixtures are written.
+With names (tiny, small_jitter, removal, pelo, 2d) only those fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -20,6 +20,11 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         gap and one lone particle; four monitoring lines (two
                         partly outside the grid): the reference's free-surface
                         monitor (.plb, .zlft at g14.7) after 0, 3 and 6 steps
+  ref_2d.npz            the 2D build (oracle/_ref/harness2d.x): 600 particles
+                        in the x-z plane, x and z jittered U(+-0.01 dx) from
+                        seed 20260303; boundary sides of the header; state,
+                        neighbour lists and 2D boundary table after the
+                        pre-loop and after 10 steps
 """
 from __future__ import annotations
 
@@ -34,6 +39,7 @@ sys.path.insert(0, str(ROOT))
 
 from oracle.decks import read_dump, read_pelo, run_harness  # noqa: E402
 from sphera_amd.case import DamBreak3D, build, control_line_points  # noqa: E402
+from sphera_amd import case2d  # noqa: E402
 
 TINY = dict(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
 SMALL = dict(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), dx=0.01, name="small")
@@ -142,8 +148,55 @@ def make_pelo():
                         time=t, pelolib=pel, zlft=zl)
 
 
+CASE2D = dict(tank=(0.6, 0.4), column=(0.3, 0.2), dx=0.01, name="t2d")
+
+
+def jitter2d(coord, dx, seed=20260303, amp=0.01):
+    """U(+-amp dx) on x and z only (the 2D particles stay at y = 0)."""
+    rng = np.random.default_rng(seed)
+    out = coord.copy()
+    j = rng.uniform(-amp, amp, (len(coord), 2)) * dx
+    out[:, 0] += j[:, 0]
+    out[:, 2] += j[:, 1]
+    return out
+
+
+def make_2d():
+    """2D build: sides, lists, 2D boundary table and fields"""
+    case = case2d.DamBreak2D(**CASE2D)
+    tb = case2d.build(case)
+    ic = jitter2d(tb.particles["coord"], case.dx)
+    hdr, steps = read_dump(run_harness(case, nsteps=10, every=10, ic_coord=ic))
+    out = {"ic_coord": ic}
+    for k in ("nag", "nmaxpartj", "ncd", "extr_lo", "dcd", "h", "coefke", "coefkacl", "eta",
+              "eta2", "celerita", "kin_visc"):
+        out["hdr_" + k] = np.asarray(hdr[k])
+    sd = hdr["sides"]
+    out["hdr_side_stretch"] = np.array([x["stretch"] for x in sd])
+    out["hdr_side_tipo"] = np.array([x["tipo"] for x in sd])
+    out["hdr_side_previous"] = np.array([x["previous"] for x in sd])
+    for k in ("length", "angle", "T", "RN", "v1", "v2"):
+        out["hdr_side_" + k] = np.array([x[k] for x in sd])
+    for st in steps:
+        pre = f"s{st['it']}_"
+        out[pre + "it"] = st["it"]
+        out[pre + "time"] = st["time"]
+        out[pre + "dt"] = st["dt"]
+        out[pre + "dt_average"] = st["dt_average"]
+        for f in FIELDS + ("var", "laminar_flag"):
+            out[pre + f] = st[f]
+        out[pre + "nbr_count"] = st["nbr_count"]
+        out[pre + "nbr"] = st["nbr"]
+        out[pre + "ncbs"] = st["ncbs"]
+        out[pre + "intncbs"] = st["intncbs"]
+        out[pre + "bt_side"] = st["btab"]["face"]
+        out[pre + "bt_loc"] = st["btab"]["loc"]
+        out[pre + "bt_bi"] = st["btab"]["bi"]
+    np.savez_compressed(HERE / "ref_2d.npz", **out)
+
+
 MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_removal,
-          "pelo": make_pelo}
+          "pelo": make_pelo, "2d": make_2d}
 
 
 def main():
