@@ -78,6 +78,28 @@ typedef struct sph_ctx sph_ctx;
  * the SA-SPH set-up of Gest_Trans.f90:692-701 (tables computed by the
  * caller). */
 int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out);
+
+/* 2D form: the reference's -DSPACE_2D build (x-z plane; particles at y = 0;
+ * the grid has ncd[1] = 1 and every particle in row j = 1, CreaGrid.f90:69-71,
+ * ParticleCellNumber.f90:59-61).  Frontiers are boundary sides in BoundarySide
+ * order (DefineBoundarySideGeometry2D.f90:60-176, peri zones included): T and
+ * RN (BoundaryMassForceMatrix2D) column-major 3x3 per side, v1 the first
+ * vertex (x, 0, z), length, angle and previous (previous_side, 1-based, 0 =
+ * none) of DefineBoundarySideRelativeAngles2D.f90, btype as for faces (6 =
+ * peri), velocity = BoundarySide%velocity, tvel = Tratto%velocity; bit2d is
+ * BoundIntegralTab2D (SA_SPH_module.f90:30-54, 22 x 6 row-major).  The phase
+ * entry points below then run the 2D units: ComputeBoundaryDataTab_2D,
+ * AddBoundaryContributions_to_ME2D, AddElasticBoundaryReaction_2D,
+ * velocity_smoothing_SA_SPH_2D, AddBoundaryContribution_to_CE2D and
+ * PressureSmoothing_2D.  Replaces Gest_Input.f90:292-293 (side geometry
+ * computed by the caller). */
+typedef struct sph_sides2d {
+    int32_t ns;
+    const double *T, *RN, *v1, *length, *angle, *velocity, *tvel, *slip_coeff;
+    const int32_t *btype, *previous, *slip_mode, *lam_check;
+    const double *bit2d;
+} sph_sides2d;
+int sph_ctx_create2d(const sph_params* p, const sph_sides2d* s, sph_ctx** out);
 int sph_ctx_destroy(sph_ctx* ctx);
 
 /* Particle state in pg order: coord, vel (xyz interleaved), dens, pres, mass.
@@ -132,11 +154,11 @@ int sph_velocity_smoothing(sph_ctx* ctx); /* velocity_smoothing.f90:74-133 + _2.
 int sph_leapfrog_traj(sph_ctx* ctx);   /* Leapfrog_trajectories.f90:48-76 */
 int sph_sort(sph_ctx* ctx);            /* OrdGrid1.f90:58-105 */
 int sph_neighbours(sph_ctx* ctx);      /* CalcVarLength.f90:111-541 (fluid part) */
-int sph_boundary_tab(sph_ctx* ctx);    /* ComputeBoundaryDataTab_3D.f90:62-106 */
+int sph_boundary_tab(sph_ctx* ctx);    /* ComputeBoundaryDataTab_3D.f90:62-106 (2D: _2D.f90:71-162) */
 int sph_continuity(sph_ctx* ctx);      /* time_step_loop.f90:338-360 */
 int sph_leapfrog_cont(sph_ctx* ctx);   /* Leapfrog_continuity.f90:51-88 */
 int sph_eos(sph_ctx* ctx);             /* EoS_fluid_loop.f90:59-70 */
-int sph_pressure_smoothing(sph_ctx* ctx); /* PressureSmoothing_3D.f90:80-161 */
+int sph_pressure_smoothing(sph_ctx* ctx); /* PressureSmoothing_3D.f90:80-161 (2D: _2D.f90:95-231) */
 
 /* Pre-loop of time_step_loop.f90:82-173: lists, boundary table, first dt. */
 int sph_preloop(sph_ctx* ctx);

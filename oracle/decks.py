@@ -24,6 +24,7 @@ HARNESS = REF_DIR / "harness3d.x"
 HARNESS_HIP = REF_DIR / "harness3d_hip.x"
 REFERENCE = REF_DIR / "sphera3d.x"
 HARNESS2D = REF_DIR / "harness2d.x"
+HARNESS2D_HIP = REF_DIR / "harness2d_hip.x"
 REFERENCE2D = REF_DIR / "sphera2d.x"
 
 

@@ -217,6 +217,12 @@ int run_btab(sph_ctx* c) {
         k_fill_int<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->bn, 0, c->P.cap);
         return 0;
     }
+    if (c->P.dim == 2) {  // ComputeBoundaryDataTab_2D
+        KTime t(c, "boundary_2d");
+        k_boundary_2d<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_bit, c->xm[0], c->cell,
+                                                            c->bn, c->bt, c->iscal);
+        return 0;
+    }
     {
         KTime t(c, "boundary_find");
         k_fill_int<<<1, 64, 0, c->stream>>>(c->nwork, 0, 1);
@@ -302,14 +308,14 @@ int run_psmooth(sph_ctx* c) {
     if (c->P.f32)
         {
             auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
+            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
             c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal); };
             if (c->chunk_p == 4) L(k_psmooth<true, 4>);
             else L(k_psmooth<true, 8>);
         }
     else
         k_psmooth<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
+            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
             c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal);
     return 0;
 }
@@ -469,6 +475,7 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
     P.semi = p->semi ? 1 : 0;
     P.f32 = p->precision == SPH_FP32_MIXED;
     P.nf = f ? f->nf : 0;
+    P.dim = 3;
     for (int d = 0; d < 3; ++d) {
         P.ncd[d] = p->ncd[d];
         P.extr_lo[d] = p->extr_lo[d];
@@ -623,6 +630,63 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
             }
     }
     *out = c;
+    return 0;
+}
+
+int sph_ctx_create2d(const sph_params* p, const sph_sides2d* sd, sph_ctx** out) {
+    if (!p || !out) return fail(1, "null argument");
+    if (p->ncd[1] != 1) return fail(2, "a 2D grid has one cell row in y (ncd[1] = 1)");
+    if (p->semi && (!sd || sd->ns <= 0)) return fail(2, "SA-SPH requested without boundary sides");
+    sph_params q = *p;
+    q.semi = 0;
+    int rc = sph_ctx_create(&q, nullptr, out);
+    if (rc) return rc;
+    sph_ctx* c = *out;
+    DP& P = c->P;
+    P.dim = 2;
+    P.semi = p->semi ? 1 : 0;
+    P.nf = sd ? sd->ns : 0;
+    if (P.nf > 0) {
+        std::vector<DFace> hs(P.nf);
+        for (int k = 0; k < P.nf; ++k) {
+            DFace& S = hs[k];
+            std::memset(&S, 0, sizeof(S));
+            std::memcpy(S.T, sd->T + 9 * k, sizeof(S.T));
+            const double* RN = sd->RN + 9 * k;  // column-major: RN(r,c) at [3c+r]
+            S.RN[0] = RN[0];
+            S.RN[1] = RN[6];
+            S.RN[2] = RN[2];
+            S.RN[3] = RN[8];
+            S.v1[0] = sd->v1[3 * k];
+            S.v1[1] = sd->v1[3 * k + 2];
+            S.len = sd->length[k];
+            S.angle = sd->angle[k];
+            S.prev = sd->previous[k];
+            S.slip = sd->slip_coeff[k];
+            for (int d = 0; d < 3; ++d) {
+                S.vel[d] = sd->velocity[3 * k + d];
+                S.tvel[d] = sd->tvel[3 * k + d];
+            }
+            S.btype = sd->btype[k];
+            S.mode = sd->slip_mode[k];
+            S.lam = sd->lam_check[k];
+            S.nodes = 2;
+            if (S.prev < 0 || S.prev > P.nf) {
+                sph_ctx_destroy(c);
+                *out = nullptr;
+                return fail(2, "previous side out of range");
+            }
+        }
+        bool ok = hipMalloc(&c->d_faces, sizeof(DFace) * P.nf) == hipSuccess &&
+                  hipMalloc(&c->d_bit, sizeof(double) * 22 * 6) == hipSuccess;
+        ok = ok && hipMemcpy(c->d_faces, hs.data(), sizeof(DFace) * P.nf, hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && hipMemcpy(c->d_bit, sd->bit2d, sizeof(double) * 22 * 6, hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) {
+            sph_ctx_destroy(c);
+            *out = nullptr;
+            return fail(102, "device allocation/copy of boundary sides failed");
+        }
+    }
     return 0;
 }
 

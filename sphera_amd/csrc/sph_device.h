@@ -45,6 +45,9 @@ struct DP {
     // FP32-mixed copies
     float f_h, f_unosuh, f_unosusquareh, f_eta, f_eta2, f_ke_coef, f_kacl_coef, f_coefke;
     float f_dcd[3], f_alfamon, f_celerita, f_kin_visc;
+    // 3, or 2 for the x-z plane form (-DSPACE_2D build of the reference): one
+    // cell row in y (every particle in row j = 1), boundary sides for faces
+    int dim;
 };
 
 // one SA-SPH frontier face (BoundaryFace + its Tratto/Partz data)
@@ -61,6 +64,15 @@ struct DFace {
     // over the face plane (bounds of a coordinate over a circle)
     double pg[3];
     int nodes, btype, mode, lam;
+    // 2D context: the array holds the boundary sides (BoundarySide of
+    // DefineBoundarySideGeometry2D.f90) and these fields: RN(1,1), RN(1,3),
+    // RN(3,1), RN(3,3) of BoundaryMassForceMatrix2D, the first vertex (x, z),
+    // length, angle to the previous side and previous_side (1-based, 0 none).
+    // T, vel, tvel, slip, btype, mode and lam keep their meaning.
+    double RN[4];
+    double v1[2];
+    double len, angle;
+    int prev;
 };
 
 // LocalNormalCoordinates.f90:55-87 constants of a face: node pairs (jj,kk) of

@@ -54,7 +54,7 @@ __device__ __forceinline__ int d_particle_cell(const DP& P, double x, double y, 
     const double zp = z - P.extr_lo[2];
     const int i = (int)ceil(xp / P.dcd[0]);
     const int k = (int)ceil(zp / P.dcd[2]);
-    const int j = (int)ceil(yp / P.dcd[1]);
+    const int j = P.dim == 2 ? 1 : (int)ceil(yp / P.dcd[1]);  // :59-61 (2D: j = 1)
     return d_cell_number(P, i, j, k);
 }
 __device__ __forceinline__ void d_cell_indices(const DP& P, int nc, int& i, int& j, int& k) {
@@ -960,6 +960,279 @@ __global__ __launch_bounds__(256) void k_boundary_int(DP P, const DFace* __restr
 }
 
 // ----------------------------------------------------------------------------
+// 2D form (x-z plane): boundary table over the boundary sides.
+// FindCloseBoundarySides2D.f90:68-170, SelectCloseBoundarySides2D.f90:66-120,
+// ComputeBoundaryDataTab_2D.f90:71-162 with FindBoundaryIntersection2D.f90:
+// 59-86, ComputeSurfaceIntegral_WdS2D.f90:57-85, BoundaryVolumeIntegrals2D
+// .f90:60-97, ComputeVolumeIntegral_WdV2D.f90:69-132 and
+// InterpolateBoundaryIntegrals2D.f90:61-70.  One lane per particle over the
+// sides (a tank has a handful); every expression in the reference's order.
+// The integrals use the device's sin, cos, tan and atan2 (the host reference
+// calls the C library's).
+// ----------------------------------------------------------------------------
+constexpr double PIGRECO_D = 3.14159265358979;
+constexpr double KERNELCONST2D_D = 0.454728408833987;
+constexpr double DELTAX_BIT_D = 0.1;
+
+// w.f90:48-57
+__device__ __forceinline__ double d_w2(double r, double h, double coef) {
+    const double sq = r / h;
+    double q;
+    if (sq <= 1.0) {
+        q = 0.666666667 + sq * sq * (sq * 0.5 - 1.0);
+    } else if (sq >= 2.0) {
+        q = 0.0;
+    } else {
+        const double dms = 2.0 - sq;
+        q = dms * dms * dms / 6.0;
+    }
+    return q * coef;
+}
+// WIntegr.f90:60-82
+__device__ __forceinline__ double d_wintegr(double ri, double hh) {
+    const double ro = ri / hh;
+    if (0.0 <= ro && ro < 1.0) {
+        const double ro2 = ro * ro, ro3 = ro2 * ro;
+        const double w1 = (0.5 + -0.375 * ro2 + 0.15 * ro3) * ro2;
+        return KERNELCONST2D_D * (0.35 - w1);
+    } else if (1.0 <= ro && ro < 2.0) {
+        const double q1 = 2.0 - ro, q2 = q1 * q1, q4 = q2 * q2;
+        return -KERNELCONST2D_D * ((-0.125 + 0.05 * q1) * q4);
+    }
+    return 0.0;
+}
+// J2Wro2.f90:55-70
+__device__ __forceinline__ double d_j2wro2(double ro) {
+    const double a1 = 0.00833333333333333, a2 = 0.26666666666666667;
+    const double ro2 = ro * ro, ro3 = ro2 * ro;
+    if (0.0 <= ro && ro < 1.0) return KERNELCONST2D_D * (0.25 - (a1 * (40.0 - 36.0 * ro2 + 15.0 * ro3) * ro3));
+    if (1.0 <= ro && ro < 2.0)
+        return KERNELCONST2D_D * (a2 - (a1 * (80.0 - 90.0 * ro + 36.0 * ro2 - 5.0 * ro3) * ro3));
+    return 0.0;
+}
+// BoundIntegralTab2D row r (0-based), column c; 22 x 6 row-major
+__device__ __forceinline__ void d_interp2d(const double* __restrict__ tab, int ncols, const int* cols,
+                                           double xx, double* func) {
+    if (xx < tab[0]) xx = tab[0];
+    // rows 1..21 hold ii (0 <= delta < 2 for a selected side); the clamp only
+    // keeps a non-finite position inside the table
+    const int ii = min(max((int)((xx - tab[0]) / DELTAX_BIT_D) + 1, 1), 21);
+    const double xi = tab[(ii - 1) * 6];
+    for (int nc = 0; nc < ncols; ++nc) {
+        const int jj = cols[nc];
+        const double fi = tab[(ii - 1) * 6 + jj], fip1 = tab[ii * 6 + jj];
+        func[nc] = fi + (fip1 - fi) * (xx - xi) / DELTAX_BIT_D;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_boundary_2d(DP P, const DFace* __restrict__ sides,
+                                                     const double* __restrict__ tab,
+                                                     const double4* __restrict__ xm,
+                                                     const int* __restrict__ cell,
+                                                     int* __restrict__ bn, BTab* __restrict__ bt,
+                                                     int* iscal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.cap) return;
+    if (i >= iscal[IS_N]) {
+        bn[i] = 0;
+        return;
+    }
+    int nint = 0;
+    const double4 X = xm[i];
+    const double px = X.x, pz = X.z;
+    const double dh = P.doubleh;
+    int ncbs = 0;
+    int clo[4];
+    double loc[8];
+    bool bad = false;
+    for (int isi = 0; isi < P.nf && !bad; ++isi) {
+        const DFace& S = sides[isi];
+        if (S.btype == 6) continue;  // peri
+        const double P1P0 = px - S.v1[0], P1P1 = pz - S.v1[1];
+        const double s0 = S.T[0], s1 = S.T[2], n0 = S.T[6], n1 = S.T[8];
+        const double sidel = S.len;
+        const double xp = P1P0 * s0 + P1P1 * s1;
+        const double yp = P1P0 * n0 + P1P1 * n1;
+        const double xpmin = -dh, xpmax = sidel + dh, ypmin = -dh, ypmax = dh;
+        if ((xpmin < xp && xp < xpmax) && (ypmin < yp && yp < ypmax)) {
+            double ypmx = 0.0, ypmn = 0.0;
+            if (xp < 0.0) {
+                ypmx = sqrt(P.square_doubleh - xp * xp);
+                ypmn = -ypmx;
+            } else if (xp <= sidel) {
+                ypmx = ypmax;
+                ypmn = ypmin;
+            } else if (xp < xpmax) {
+                const double Lmxpq = (sidel - xp) * (sidel - xp);
+                ypmx = sqrt(P.square_doubleh - Lmxpq);
+                ypmn = -ypmx;
+            }
+            if (ypmn < yp && yp < ypmx) {
+                if (ncbs >= 4) {
+                    bad = true;
+                    break;
+                }
+                clo[ncbs] = isi;
+                loc[2 * ncbs] = xp;
+                loc[2 * ncbs + 1] = yp;
+                ncbs = ncbs + 1;
+            }
+        }
+    }
+    for (int q = 0; q < ncbs && !bad; ++q) {
+        const int b = sides[clo[q]].btype;
+        if (b != 0 && b != 1) {
+            atomicAdd(&iscal[IS_UNSUPPORTED], 1);
+            bad = true;
+        }
+    }
+    if (bad) {
+        if (ncbs >= 4) atomicAdd(&iscal[IS_BT_OVERFLOW], 1);
+        bn[i] = 0;
+        return;
+    }
+    if (ncbs == 2) {
+        const int a = clo[0], b = clo[1];
+        if (sides[a].prev == b + 1)
+            loc[2] = sides[b].len - loc[2];
+        else if (sides[b].prev == a + 1)
+            loc[0] = sides[a].len - loc[0];
+    }
+    int iside[4];
+    double iloc[8];
+    for (int q = 0; q < ncbs; ++q) {
+        if (loc[2 * q + 1] > 0.0) {
+            iside[nint] = clo[q];
+            iloc[2 * nint] = loc[2 * q];
+            iloc[2 * nint + 1] = loc[2 * q + 1];
+            nint = nint + 1;
+        }
+    }
+    if (nint > 2) {
+        atomicAdd(&iscal[IS_BT_OVERFLOW], 1);
+        bn[i] = 0;
+        return;
+    }
+    const double h = P.h;
+    const double yplimite = 0.05 * h;
+    const double dalfarif = PIGRECO_D / 10;
+    for (int icbs = 0; icbs < nint; ++icbs) {
+        BTab& B = bt[(size_t)icbs * P.cap + i];
+        const double xpi = iloc[2 * icbs];
+        double ypi = iloc[2 * icbs + 1];
+        B.face = iside[icbs];
+        B.loc[0] = xpi;
+        B.loc[1] = ypi;
+        B.loc[2] = 0.0;
+        const double deltai = ypi / h;
+        double F[5];
+        if (nint == 2) {
+            // FindBoundaryIntersection2D (ypi > 0 here)
+            const double halfchord = sqrt(P.square_doubleh - ypi * ypi);
+            const double XA = xpi - halfchord, XB = xpi + halfchord;
+            double xpmin = 0.0;
+            if (XA > xpmin) xpmin = XA;
+            double xpmax = sides[iside[icbs]].len;
+            if (XB < xpmax) xpmax = XB;
+            double interlen = xpmax - xpmin;
+            if (interlen <= 0.01 * h) interlen = 0.0;
+            // ComputeSurfaceIntegral_WdS2D
+            double IntWdS = 0.0;
+            if (interlen > 0.0) {
+                const double dsrif = h / 2.0;
+                const double ypiq = ypi * ypi;
+                int ndiv = (int)(interlen / dsrif + 0.5);
+                if (ndiv < 2) ndiv = 2;
+                const double deltas = interlen / ndiv;
+                double xpip = xpmin - 0.5 * deltas;
+                for (int ipt = 1; ipt <= ndiv; ++ipt) {
+                    xpip = xpip + deltas;
+                    const double dxpip = xpip - xpi;
+                    const double ris = sqrt(dxpip * dxpip + ypiq);
+                    IntWdS = IntWdS + d_w2(ris, h, P.coefke) * deltas;
+                }
+            }
+            // BoundaryVolumeIntegrals2D
+            double IntWdV = 0.0, Dp0 = 0.0, Dp1 = 0.0;
+            if (interlen > 0.0) {
+                double yb = ypi;
+                if (yb < yplimite) yb = yplimite;
+                const double alfaA = atan2(xpmin - xpi, yb), alfaB = atan2(xpmax - xpi, yb);
+                const double intalfa = alfaB - alfaA;
+                int ndiv = (int)(intalfa / dalfarif + 0.5);
+                if (ndiv < 1) ndiv = 1;
+                const double dalfa = intalfa / ndiv;
+                double alfa_k = alfaA - 0.5 * dalfa;
+                for (int k = 1; k <= ndiv; ++k) {
+                    alfa_k = alfa_k + dalfa;
+                    const double sa = sin(alfa_k), ca = cos(alfa_k);
+                    const double rb = yb / ca;
+                    const double mult = h * d_j2wro2(rb / h) * dalfa;
+                    Dp0 = Dp0 + sa * mult;
+                    Dp1 = Dp1 - ca * mult;
+                    IntWdV = IntWdV + d_wintegr(rb, h) * dalfa;
+                }
+                if (yb == yplimite) IntWdV = 0.5;
+            }
+            // ComputeVolumeIntegral_WdV2D (Ncbslocal = 2)
+            double IntWdV1 = 0.0;
+            if (interlen > 0.0) {
+                if (ypi >= yplimite) {
+                    const double alfaA = atan2(xpmin - xpi, ypi), alfaB = atan2(xpmax - xpi, ypi);
+                    const double intalfa = alfaB - alfaA;
+                    int ndiv = (int)(intalfa / dalfarif + 0.5);
+                    if (ndiv < 2) ndiv = 2;
+                    const double dalfa = intalfa / ndiv;
+                    double alfa = alfaA - 0.5 * dalfa;
+                    for (int ipt = 1; ipt <= ndiv; ++ipt) {
+                        alfa = alfa + dalfa;
+                        const double ta = tan(alfa);
+                        const double ris = ypi * sqrt(1.0 + ta * ta);
+                        IntWdV1 = IntWdV1 + d_wintegr(ris, h) * dalfa;
+                    }
+                } else {
+                    const double ypj = iloc[2 * (1 - icbs) + 1];
+                    if (ypj <= yplimite) {
+                        double angle = 0.0;
+                        if (sides[iside[0]].prev == iside[1] + 1)
+                            angle = sides[iside[0]].angle;
+                        else if (sides[iside[1]].prev == iside[0] + 1)
+                            angle = sides[iside[1]].angle;
+                        else
+                            atomicAdd(&iscal[IS_UNSUPPORTED], 1);  // sides not consecutive
+                        IntWdV1 = 0.5 * (angle + PIGRECO_D) / (2.0 * PIGRECO_D);
+                    } else {
+                        IntWdV1 = (xpmin >= xpi && xpi <= xpmax) ? 0.5 : 0.0;
+                    }
+                }
+            }
+            const int cols[3] = {3, 4, 5};
+            d_interp2d(tab, 3, cols, deltai, F);
+            B.bi[0] = IntWdS;
+            B.bi[1] = IntWdV;
+            B.bi[2] = IntWdV1;
+            B.bi[3] = Dp0;
+            B.bi[4] = Dp1;
+            B.bi[5] = F[0] / P.squareh;
+            B.bi[6] = F[1] / P.squareh;
+            B.bi[7] = F[2] / P.squareh;
+        } else {
+            const int cols[5] = {1, 2, 3, 4, 5};
+            d_interp2d(tab, 5, cols, deltai, F);
+            B.bi[0] = F[0] / h;
+            B.bi[1] = F[1];
+            B.bi[2] = F[1];
+            B.bi[3] = 0.0;
+            B.bi[4] = 0.0;
+            B.bi[5] = F[2] / P.squareh;
+            B.bi[6] = F[3] / P.squareh;
+            B.bi[7] = F[4] / P.squareh;
+        }
+    }
+    bn[i] = nint;
+}
+
+// ----------------------------------------------------------------------------
 // FP32-mixed form of the same quadrature, eight lanes per (particle, face)
 // pair, lane r taking ring r of BoundIntegralTab.  The rays of a ring meet
 // the face plane on one circle of radius ts |b_xy| around (L0, L1), all at
@@ -1536,6 +1809,94 @@ __device__ __forceinline__ void sa_momentum(const DP& P, const DFace* __restrict
     }
 }
 
+// AddBoundaryContributions_to_ME2D.f90:121-405 for fixe/tapi sides (2D form);
+// tp/td/tv start at zero, as the reference's tpres/tdiss/tvisc do
+__device__ __forceinline__ void sa_momentum_2d(const DP& P, const DFace* __restrict__ sides,
+                                               const BTab* __restrict__ bt, int i, int ncbf,
+                                               const double v_i[3], double2 rpi, int lam_prev,
+                                               double tp[3], double td[3], double tv[3]) {
+    const double ro0 = P.den0, roi = rpi.x, pressi = rpi.y;
+    const double distpimin = P.dx;
+    double VMo0 = 0.0, VMo1 = 0.0, VSh0 = 0.0, VSh1 = 0.0, RG0s = 0.0, RG1s = 0.0;
+    double ALE0 = 0.0, ALE2 = 0.0;
+    const double g0 = P.grav[0], g1 = P.grav[2];
+    for (int q = 0; q < ncbf; ++q) {
+        const BTab& B = bt[(size_t)q * P.cap + i];
+        const DFace& S = sides[B.face];
+        const double xpi = B.loc[0], ypi = B.loc[1];
+        const double IntWdS = B.bi[0], IntWdV = B.bi[2], IntWd1s0 = B.bi[5], IntWd3s0 = B.bi[6];
+        const double n0 = S.T[6], n1 = S.T[8];
+        double GravN = 0.0;
+        GravN = GravN + P.grav[0] * n0;
+        GravN = GravN + P.grav[2] * n1;
+        const double sv0 = S.btype == 1 ? S.vel[0] : 0.0, sv1 = S.btype == 1 ? S.vel[2] : 0.0;
+        const double Dv0 = 2.0 * (v_i[0] - sv0), Dv1 = 2.0 * (v_i[2] - sv1);
+        const double DvelN = Dv0 * n0 + Dv1 * n1;
+        const double distpi = fmax(ypi, distpimin);
+        double pressib = pressi;
+        const double pressibmin = 10.0 * ro0 * distpimin;
+        if (DvelN <= 0.0) {
+            if (pressib < pressibmin) pressib = pressibmin;
+        }
+        const double PressB = pressib - ro0 * GravN * distpi;
+        const double Qii = IntWdS * (pressib + PressB) / roi;
+        double RG0 = 0.0, RG1 = 0.0;
+        RG0 = RG0 + S.RN[0] * g0;
+        RG0 = RG0 + S.RN[1] * g1;
+        RG0 = RG0 * IntWdV;
+        RG1 = RG1 + S.RN[2] * g0;
+        RG1 = RG1 + S.RN[3] * g1;
+        RG1 = RG1 * IntWdV;
+        RG0s = RG0s + RG0;
+        RG1s = RG1s + RG1;
+        ALE0 = ALE0 - n0 * Qii;
+        ALE2 = ALE2 - n1 * Qii;
+        if (xpi >= 0.0 && xpi <= S.len) {
+            double slip = 0.0, cinv = 0.0;
+            if (S.mode == 1) {
+                slip = S.slip;
+                cinv = P.kin_visc;
+            }
+            const double Monvisc = P.alfamon * P.celerita * P.h;
+            const double TN = Monvisc * DvelN * IntWd3s0;
+            double SVforce = 0.0;
+            if (lam_prev == 1 || S.lam == 0) SVforce = 2.0 * cinv * slip * IntWd1s0;
+            VMo0 = VMo0 + TN * n0;
+            VMo1 = VMo1 + TN * n1;
+            VSh0 = VSh0 + (2.0 * v_i[0]) * SVforce;
+            VSh1 = VSh1 + (2.0 * v_i[2]) * SVforce;
+        }
+    }
+    tp[0] = tp[0] - RG0s - ALE0;
+    tp[2] = tp[2] - RG1s - ALE2;
+    td[0] = td[0] - VMo0;
+    td[2] = td[2] - VMo1;
+    tv[0] = tv[0] - VSh0;
+    tv[2] = tv[2] - VSh1;
+}
+
+// AddElasticBoundaryReaction_2D.f90:66-99 over the selected sides
+__device__ __forceinline__ void elastic_2d(const DP& P, const DFace* __restrict__ sides,
+                                           const BTab* __restrict__ bt, int i, int ncbf,
+                                           const double v_i[3], double br[3]) {
+    const double h = P.h;
+    const double ypimin = 0.25 * P.dx;
+    const double celer02 = P.eps / P.den0;
+    for (int q = 0; q < ncbf; ++q) {
+        const BTab& B = bt[(size_t)q * P.cap + i];
+        const DFace& S = sides[B.face];
+        const double xpi = B.loc[0], ypi = B.loc[1];
+        if (ypi < ypimin && xpi > 0.0 && xpi < S.len) {
+            double vin = 0.0;
+            for (int k = 0; k < 3; ++k) vin = vin + v_i[k] * S.T[6 + k];
+            if (vin < 0.0) {
+                const double nr = -1.0 * celer02 * log((h + ypi - ypimin) / h) / h;
+                for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * S.T[6 + d];
+            }
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------
 // momentum: RHS_momentum_equation.f90:72-179, inter_EqMoto.f90:108-283,
 // viscomon.f90:54-71, viscomorris.f90:60-71, AddElasticBoundaryReaction_3D.f90:
@@ -1562,7 +1923,12 @@ __global__ __launch_bounds__(256) void k_momentum(
             double tp[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0}, tv[3] = {0.0, 0.0, 0.0};
             const int ncbf = P.semi ? bn[i] : 0;
             const double v_i[3] = {vi.x, vi.y, vi.z};
-            if (ncbf > 0) sa_momentum(P, faces, bt, i, ncbf, v_i, rpi, lam_in[i], tp, td, tv, iscal);
+            if (ncbf > 0) {
+                if (P.dim == 2)
+                    sa_momentum_2d(P, faces, bt, i, ncbf, v_i, rpi, lam_in[i], tp, td, tv);
+                else
+                    sa_momentum(P, faces, bt, i, ncbf, v_i, rpi, lam_in[i], tp, td, tv, iscal);
+            }
             const int nn = nnb[i];
             double rvs[3];
             if constexpr (!F32) {
@@ -1672,7 +2038,11 @@ __global__ __launch_bounds__(256) void k_momentum(
             }
             lam_out[i] = lam;
             double a[3];
-            if (ncbf > 0) {
+            if (ncbf > 0 && P.dim == 2) {
+                double br[3] = {0.0, 0.0, 0.0};
+                elastic_2d(P, faces, bt, i, ncbf, v_i, br);
+                for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + P.grav[d] + br[d];
+            } else if (ncbf > 0) {
                 double br[3] = {0.0, 0.0, 0.0};
                 const double zimin = 0.25 * P.dx;
                 const double celer02 = P.eps / P.den0;
@@ -1779,7 +2149,22 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                     v[0] = s0; v[1] = s1; v[2] = s2;
                 }
                 const int ncbf = P.semi ? bn[i] : 0;
-                for (int q = 0; q < ncbf; ++q) {
+                // velocity_smoothing_SA_SPH_2D.f90:64-118 (fixe/tapi sides)
+                for (int q = 0; q < ncbf && P.dim == 2; ++q) {
+                    const BTab& B = bt[(size_t)q * P.cap + i];
+                    const DFace& S = faces[B.face];
+                    const double s0 = S.T[0], s1 = S.T[2], n0 = S.T[6], n1 = S.T[8];
+                    const double DVG0 = 2.0 * (S.tvel[0] - vi.x), DVG1 = 2.0 * (S.tvel[2] - vi.z);
+                    const double DVL0 = s0 * DVG0 + s1 * DVG1;
+                    const double DVL1 = n0 * DVG0 + n1 * DVG1;
+                    const double IntWdV = B.bi[2];
+                    const double slip = S.mode == 1 ? S.slip : 0.0;
+                    const double BCL0 = DVL0 * IntWdV * slip;
+                    const double BCL1 = DVL1 * IntWdV;
+                    v[0] = v[0] + (s0 * BCL0 + n0 * BCL1);
+                    v[2] = v[2] + (s1 * BCL0 + n1 * BCL1);
+                }
+                for (int q = 0; q < ncbf && P.dim == 3; ++q) {
                     const BTab& B = bt[(size_t)q * P.cap + i];
                     const DFace& F = faces[B.face];
                     if (F.btype == 3 || F.btype == 4 || F.btype == 5) {
@@ -1848,7 +2233,27 @@ __global__ __launch_bounds__(256) void k_continuity(
             const double rhoi = rpi.x;
             double dd = 0.0;
             const int ncbf = P.semi ? bn[i] : 0;
-            if (ncbf > 0) {
+            if (ncbf > 0 && P.dim == 2) {
+                // SASPH_continuity.f90:126-169 with AddBoundaryContribution_to_CE2D
+                // .f90:74-150 (fixe/tapi sides): grad_u_SA(1) and grad_w_SA(3)
+                double gu0 = 0.0, gw2 = 0.0;
+                for (int q = 0; q < ncbf; ++q) {
+                    const BTab& B = bt[(size_t)q * P.cap + i];
+                    const DFace& S = faces[B.face];
+                    const double IntWdS = B.bi[0];
+                    double dv0, dv1;
+                    if (S.btype == 0) {
+                        dv0 = 2.0 * (-vi.x);
+                        dv1 = 2.0 * (-vi.z);
+                    } else {
+                        dv0 = 2.0 * (S.vel[0] - vi.x);
+                        dv1 = 2.0 * (S.vel[2] - vi.z);
+                    }
+                    gu0 = gu0 - dv0 * S.T[6] * IntWdS;
+                    gw2 = gw2 - dv1 * S.T[8] * IntWdS;
+                }
+                dd = dd - rhoi * (gu0 + 0.0 + gw2);
+            } else if (ncbf > 0) {
                 double gu = 0.0, gv = 0.0, gw = 0.0;
                 const double v_i[3] = {vi.x, vi.y, vi.z};
                 for (int q = 0; q < ncbf; ++q) {
@@ -1937,7 +2342,7 @@ __global__ __launch_bounds__(256) void k_continuity(
 // ----------------------------------------------------------------------------
 template <bool F32, int CH>
 __global__ __launch_bounds__(256) void k_psmooth(
-    DP P, const int* __restrict__ bn, const BTab* __restrict__ bt,
+    DP P, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
     const double4* __restrict__ xm, const double2* __restrict__ rp, const float4* __restrict__ xmf,
     const float2* __restrict__ rpf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
     double* __restrict__ vpres_out, double2* __restrict__ rp_out, float2* __restrict__ rpf_out,
@@ -1999,16 +2404,40 @@ __global__ __launch_bounds__(256) void k_psmooth(
                     app = s_a;
                 }
                 const int ncbf = P.semi ? bn[i] : 0;
-                for (int q = 0; q < ncbf; ++q) {
-                    const BTab& B = bt[(size_t)q * P.cap + i];
-                    if (B.loc[2] > 0.0) app = app + B.bi[1];
+                if (P.dim == 2) {
+                    // PressureSmoothing_2D.f90:124-214, SmoothVersion "b", fixe/tapi
+                    // sides; vpres is the smoothed pressure itself
+                    double VFT = 0.0;
+                    for (int q = 0; q < ncbf; ++q) {
+                        const BTab& B = bt[(size_t)q * P.cap + i];
+                        const DFace& S = faces[B.face];
+                        const double mf0 = S.T[0] * P.grav[0] + S.T[2] * P.grav[2];
+                        const double mf1 = S.T[6] * P.grav[0] + S.T[8] * P.grav[2];
+                        VFT = VFT + B.bi[1];
+                        somp = somp + P.den0 * (mf0 * B.bi[3] + mf1 * B.bi[4]);
+                    }
+                    app = app + VFT + 0.0 + 0.0 + 0.0;
+                    const double pz = (pi - pi) * 0.0;
+                    diffp = pi + tp1 * (somp + pz + pz + (p0i - pi) * (1.0 - app));
+                } else {
+                    for (int q = 0; q < ncbf; ++q) {
+                        const BTab& B = bt[(size_t)q * P.cap + i];
+                        if (B.loc[2] > 0.0) app = app + B.bi[1];
+                    }
+                    if (app < 0.95) diffp = somp + (p0i - pi) * (1.0 - app);
+                    else diffp = somp / app;
                 }
-                if (app < 0.95) diffp = somp + (p0i - pi) * (1.0 - app);
-                else diffp = somp / app;
+            } else if (P.dim == 2) {
+                diffp = rpi.y;  // no list (never: self is listed): pressure unchanged
             }
             vpres_out[i] = diffp;
             double p2 = rpi.y, rho2 = rpi.x;
-            if (P.tetap > 0.0) {
+            if (P.dim == 2) {  // PressureSmoothing_2D.f90:217-231: p = vpres, rho = EoS^-1(p)
+                if (P.tetap > 0.0) {
+                    p2 = diffp;
+                    rho2 = P.den0 * (1.0 + (p2 - P.prif) / P.eps);
+                }
+            } else if (P.tetap > 0.0) {
                 p2 = rpi.y + tp1 * diffp;
                 rho2 = P.den0 * (1.0 + (p2 - P.prif) / P.eps);
             }

@@ -8,7 +8,9 @@
 !   sph_hip_open(prec,ctx)  packs Domain, Grid, Med(1), input_any_t and the
 !                           SA-SPH frontier tables (BoundaryFace, BFaceList,
 !                           Tratto, Partz, GCBFPointers, GCBFVector, kerneltab,
-!                           BoundIntegralTab) and uploads pg(1:nag)
+!                           BoundIntegralTab; in the -DSPACE_2D build the
+!                           boundary sides, BoundarySide, and
+!                           BoundIntegralTab2D) and uploads pg(1:nag)
 !   sph_hip_pull(ctx)       device state -> pg(1:nag), nag, dt, dt_average,
 !                           simulation_time (at output steps)
 !   sph_hip_check(rc,where) stops like diagnostic() on a non-zero status
@@ -25,6 +27,9 @@ use sphera_hip_c
 use Static_allocation_module
 use Hybrid_allocation_module
 use Dynamic_allocation_module
+#ifdef SPACE_2D
+use SA_SPH_module, only: BoundIntegralTab2D
+#endif
 implicit none
 private
 public :: sph_hip_open, sph_hip_pull, sph_hip_check, sph_hip_close
@@ -36,6 +41,9 @@ real(c_double),allocatable,target,save :: fT(:), fLX(:), fGX(:), farea(:), fvel(
 real(c_double),allocatable,target,save :: ftvel(:), fslip(:), fkt(:), fbit(:)
 integer(c_int32_t),allocatable,target,save :: fnodes(:), fbtype(:), fmode(:), flam(:)
 integer(c_int32_t),allocatable,target,save :: gcount(:), gstart(:), gvec(:)
+! 2D: boundary sides
+real(c_double),allocatable,target,save :: sRN(:), sv1(:), slen(:), sang(:)
+integer(c_int32_t),allocatable,target,save :: sprev(:)
 
 contains
 
@@ -74,7 +82,11 @@ type(sph_params_t),intent(out) :: p
 p%precision = prec
 p%capacity = max(nag,1)
 p%nmaxpartj = NMAXPARTJ
+#ifdef SPACE_2D
+p%maxclose = MAXCLOSEBOUNDSIDES
+#else
 p%maxclose = input_any_t%MAXCLOSEBOUNDFACES
+#endif
 p%ncd(1:3) = Grid%ncd(1:3)
 p%extr_lo(1:3) = Grid%extr(1:3,1)
 p%dcd(1:3) = Grid%dcd(1:3)
@@ -105,6 +117,57 @@ p%chunk_c = 0
 p%chunk_p = 0
 end subroutine pack_params
 
+#ifdef SPACE_2D
+! Boundary sides in BoundarySide order (DefineBoundarySideGeometry2D.f90),
+! with their Tratto data and BoundIntegralTab2D.
+subroutine pack_sides(sd)
+type(sph_sides2d_t),intent(out) :: sd
+integer(4) :: k,ns,st,r,c
+ns = NumBSides
+allocate(fT(9*max(ns,1)),sRN(9*max(ns,1)),sv1(3*max(ns,1)),slen(max(ns,1)),sang(max(ns,1)))
+allocate(fvel(3*max(ns,1)),ftvel(3*max(ns,1)),fslip(max(ns,1)),fbtype(max(ns,1)))
+allocate(sprev(max(ns,1)),fmode(max(ns,1)),flam(max(ns,1)),fbit(22*6))
+do k=0,ns-1
+   st = BoundarySide(k+1)%stretch
+   do c=1,3
+      do r=1,3
+         fT(9*k+3*(c-1)+r) = BoundarySide(k+1)%T(r,c)
+         sRN(9*k+3*(c-1)+r) = BoundarySide(k+1)%RN(r,c)
+      enddo
+   enddo
+   sv1(3*k+1:3*k+3) = Vertice(1:3,BoundarySide(k+1)%vertex(1))
+   slen(k+1) = BoundarySide(k+1)%length
+   sang(k+1) = BoundarySide(k+1)%angle
+   sprev(k+1) = BoundarySide(k+1)%previous_side
+   fvel(3*k+1:3*k+3) = BoundarySide(k+1)%velocity(1:3)
+   ftvel(3*k+1:3*k+3) = Tratto(st)%velocity(1:3)
+   fbtype(k+1) = btype_code(Tratto(st)%tipo)
+   fmode(k+1) = Partz(Tratto(st)%zone)%slip_coefficient_mode
+   fslip(k+1) = Partz(Tratto(st)%zone)%BC_shear_stress_input
+   flam(k+1) = 0
+   if (Tratto(st)%laminar_no_slip_check) flam(k+1) = 1
+enddo
+do r=1,22
+   do c=0,5
+      fbit(6*(r-1)+c+1) = BoundIntegralTab2D(r,c)
+   enddo
+enddo
+sd%ns = ns
+sd%T = c_loc(fT)
+sd%RN = c_loc(sRN)
+sd%v1 = c_loc(sv1)
+sd%length = c_loc(slen)
+sd%angle = c_loc(sang)
+sd%velocity = c_loc(fvel)
+sd%tvel = c_loc(ftvel)
+sd%slip_coeff = c_loc(fslip)
+sd%btype = c_loc(fbtype)
+sd%previous = c_loc(sprev)
+sd%slip_mode = c_loc(fmode)
+sd%lam_check = c_loc(flam)
+sd%bit2d = c_loc(fbit)
+end subroutine pack_sides
+#else
 ! Frontier faces in BFaceList order; GCBFVector entries (BoundaryFace
 ! indices) become 0-based positions in that order.
 subroutine pack_faces(f)
@@ -189,11 +252,15 @@ f%ktdelta = ktdelta
 f%bitab = c_loc(fbit)
 deallocate(pos)
 end subroutine pack_faces
+#endif
 
 subroutine free_tables
-deallocate(fT,fLX,fGX,farea,fvel,ftvel,fslip,fnodes,fbtype,fmode,flam)
+if (allocated(fLX)) deallocate(fLX,fGX,farea,fnodes)
+if (allocated(sRN)) deallocate(sRN,sv1,slen,sang,sprev)
+deallocate(fT,fvel,ftvel,fslip,fbtype,fmode,flam)
 if (allocated(gcount)) deallocate(gcount,gstart,gvec)
-if (allocated(fkt)) deallocate(fkt,fbit)
+if (allocated(fkt)) deallocate(fkt)
+if (allocated(fbit)) deallocate(fbit)
 end subroutine free_tables
 
 ! Creates the device context for the current case and uploads pg(1:nag).
@@ -201,18 +268,32 @@ subroutine sph_hip_open(prec,ctx)
 integer(c_int32_t),intent(in) :: prec
 type(c_ptr),intent(out) :: ctx
 type(sph_params_t) :: p
+#ifdef SPACE_2D
+type(sph_sides2d_t) :: sd
+#else
 type(sph_faces_t) :: f
+#endif
 real(c_double),allocatable :: xyz(:),uvw(:),rho(:),pp(:),m(:)
 integer(4) :: i
+#ifdef SPACE_2D
+if ((Domain%tipo/="semi").or.(Domain%time_split/=1).or.(NMedium/=1).or.           &
+   Domain%NormFix.or.(NumOpenSides>0).or.(SourceSide/=0).or.dt_alfa_Mon) then
+#else
 if ((Domain%tipo/="semi").or.(Domain%time_split/=1).or.(NMedium/=1).or.           &
    Domain%NormFix.or.(NumOpenFaces>0).or.(SourceFace/=0).or.dt_alfa_Mon) then
+#endif
    write(*,*) "sphera_hip: case outside the device scope (semi, leapfrog, one ",   &
       "medium, fixed/tapis frontiers)"
    stop 1
 endif
 call pack_params(prec,p)
+#ifdef SPACE_2D
+call pack_sides(sd)
+call sph_hip_check(sph_ctx_create2d(p,sd,ctx),"sph_ctx_create2d")
+#else
 call pack_faces(f)
 call sph_hip_check(sph_ctx_create(p,f,ctx),"sph_ctx_create")
+#endif
 allocate(xyz(3*max(nag,1)),uvw(3*max(nag,1)),rho(max(nag,1)),pp(max(nag,1)),m(max(nag,1)))
 do i=1,nag
    xyz(3*i-2:3*i) = pg(i)%coord(1:3)

@@ -34,11 +34,25 @@ type,bind(C) :: sph_faces_t
    type(c_ptr) :: bitab
 end type sph_faces_t
 
+! 2D boundary sides (sph_sides2d of sphera_hip.h)
+type,bind(C) :: sph_sides2d_t
+   integer(c_int32_t) :: ns
+   type(c_ptr) :: T, RN, v1, length, angle, velocity, tvel, slip_coeff
+   type(c_ptr) :: btype, previous, slip_mode, lam_check
+   type(c_ptr) :: bit2d
+end type sph_sides2d_t
+
 interface
    integer(c_int) function sph_ctx_create(p,f,ctx) bind(C,name="sph_ctx_create")
       import :: c_int, c_ptr, sph_params_t, sph_faces_t
       type(sph_params_t),intent(in) :: p
       type(sph_faces_t),intent(in) :: f
+      type(c_ptr),intent(out) :: ctx
+   end function
+   integer(c_int) function sph_ctx_create2d(p,sd,ctx) bind(C,name="sph_ctx_create2d")
+      import :: c_int, c_ptr, sph_params_t, sph_sides2d_t
+      type(sph_params_t),intent(in) :: p
+      type(sph_sides2d_t),intent(in) :: sd
       type(c_ptr),intent(out) :: ctx
    end function
    integer(c_int) function sph_ctx_destroy(ctx) bind(C,name="sph_ctx_destroy")

@@ -23,7 +23,7 @@ PD = C.POINTER(C.c_double)
 PI = C.POINTER(C.c_int32)
 
 EXPORTS = [
-    "sph_ctx_create", "sph_ctx_destroy", "sph_state_upload", "sph_particle_count",
+    "sph_ctx_create", "sph_ctx_create2d", "sph_ctx_destroy", "sph_state_upload", "sph_particle_count",
     "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_global_quantities",
     "sph_monitor_lines", "sph_free_surface", "sph_get_neighbour_lists", "sph_get_boundary_table", "sph_dt", "sph_momentum",
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
@@ -59,6 +59,13 @@ class SphFaces(C.Structure):
                 ("bitab", PD)]
 
 
+class SphSides2D(C.Structure):
+    _fields_ = [("ns", C.c_int32), ("T", PD), ("RN", PD), ("v1", PD), ("length", PD),
+                ("angle", PD), ("velocity", PD), ("tvel", PD), ("slip_coeff", PD),
+                ("btype", PI), ("previous", PI), ("slip_mode", PI), ("lam_check", PI),
+                ("bit2d", PD)]
+
+
 _lib = None
 
 
@@ -71,6 +78,7 @@ def load():
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
     L = C.CDLL(str(LIB_PATH))
     L.sph_ctx_create.argtypes = [C.POINTER(SphParams), C.POINTER(SphFaces), C.POINTER(C.c_void_p)]
+    L.sph_ctx_create2d.argtypes = [C.POINTER(SphParams), C.POINTER(SphSides2D), C.POINTER(C.c_void_p)]
     L.sph_ctx_destroy.argtypes = [C.c_void_p]
     L.sph_state_upload.argtypes = [C.c_void_p, C.c_int32, PD, PD, PD, PD, PD]
     L.sph_particle_count.argtypes = [C.c_void_p, PI]
@@ -139,13 +147,33 @@ def faces_struct(tb):
     return s, k
 
 
+def sides_struct(tb):
+    """Pack CaseTables2D boundary sides in the C layout; returns (struct, keepalive)."""
+    S = tb.sides
+    k = dict(T=np.ascontiguousarray(S.T.transpose(0, 2, 1).reshape(-1)),
+             RN=np.ascontiguousarray(S.RN.transpose(0, 2, 1).reshape(-1)),
+             v1=np.ascontiguousarray(S.v1.reshape(-1)), L=np.ascontiguousarray(S.length, np.float64),
+             ang=np.ascontiguousarray(S.angle, np.float64),
+             vel=np.ascontiguousarray(S.velocity.reshape(-1)),
+             tvel=np.ascontiguousarray(S.tratto_velocity.reshape(-1)),
+             slip=np.ascontiguousarray(S.slip_coeff, np.float64),
+             bt=np.ascontiguousarray(S.btype, np.int32), prev=np.ascontiguousarray(S.previous, np.int32),
+             mode=np.ascontiguousarray(S.slip_mode, np.int32),
+             lam=np.ascontiguousarray(S.laminar_check, np.int32),
+             bit=np.ascontiguousarray(tb.bit2d.reshape(-1), np.float64))
+    s = SphSides2D(len(S.length), _pd(k["T"]), _pd(k["RN"]), _pd(k["v1"]), _pd(k["L"]), _pd(k["ang"]),
+                   _pd(k["vel"]), _pd(k["tvel"]), _pd(k["slip"]), _pi(k["bt"]), _pi(k["prev"]),
+                   _pi(k["mode"]), _pi(k["lam"]), _pd(k["bit"]))
+    return s, k
+
+
 def params_struct(tb, precision=FP64, capacity=None, semi=True, tune=None):
     s, g, case = tb.scalars, tb.grid, tb.case
     p = SphParams()
     p.precision = precision
     p.capacity = int(capacity or len(tb.particles["coord"]))
     p.nmaxpartj = s["nmaxpartj"]
-    p.maxclose = case.maxcloseboundfaces
+    p.maxclose = 4 if getattr(tb, "dim", 3) == 2 else case.maxcloseboundfaces
     p.ncd[:] = [int(v) for v in g.ncd]
     p.extr_lo[:] = [float(v) for v in g.extr_lo]
     p.dcd[:] = [float(v) for v in g.dcd]
@@ -192,10 +220,15 @@ class SphContext:
         self.tb = tb
         self.precision = precision
         self.params = params_struct(tb, precision, capacity, semi, tune)
-        fs, self._keep = faces_struct(tb)
         h = C.c_void_p()
-        self._check(self.L.sph_ctx_create(C.byref(self.params), C.byref(fs), C.byref(h)),
-                    "sph_ctx_create")
+        if getattr(tb, "dim", 3) == 2:
+            ss, self._keep = sides_struct(tb)
+            self._check(self.L.sph_ctx_create2d(C.byref(self.params), C.byref(ss), C.byref(h)),
+                        "sph_ctx_create2d")
+        else:
+            fs, self._keep = faces_struct(tb)
+            self._check(self.L.sph_ctx_create(C.byref(self.params), C.byref(fs), C.byref(h)),
+                        "sph_ctx_create")
         self.h = h
 
     def _check(self, rc, where):

@@ -45,3 +45,17 @@ def test_dropin_harness_fails_loudly_without_device(tmp_path):
     case = DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
     with pytest.raises(RuntimeError, match="sph_ctx_create failed, status +1"):
         run_harness(case, 2, lists=False, workdir=tmp_path, exe=HARNESS_HIP, timeout=60)
+
+
+def test_dropin_harness_2d_fails_loudly_without_device(tmp_path):
+    """The same for the reference's 2D build with the bridge (sph_ctx_create2d)."""
+    import torch
+    from oracle.decks import HARNESS2D_HIP, run_harness
+    from sphera_amd.case2d import DamBreak2D
+    if not HARNESS2D_HIP.exists():
+        pytest.skip("2D drop-in harness not built")
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a device is present")
+    case = DamBreak2D(tank=(0.6, 0.4), column=(0.2, 0.1), dx=0.01, name="t2d")
+    with pytest.raises(RuntimeError, match="sph_ctx_create2d failed, status +1"):
+        run_harness(case, 2, lists=False, workdir=tmp_path, exe=HARNESS2D_HIP, timeout=60)

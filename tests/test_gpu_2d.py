@@ -1,0 +1,134 @@
+"""2D form on the device (sph_ctx_create2d) vs the FP64 oracle, whose 2D path
+is pinned bit for bit to the reference's -DSPACE_2D build
+(tests/test_oracle_2d.py).
+
+Bars: neighbour lists identical as ordered lists; boundary-table sides and
+local coordinates identical; boundary integrals and fields within
+FP64_REL_TOL (the corner integrals of BoundaryVolumeIntegrals2D and
+ComputeVolumeIntegral_WdV2D call sin, cos, tan and atan2, which the device
+evaluates with its own math library); FP32-mixed within FP32_TOL of the FP64
+oracle.
+"""
+import numpy as np
+import pytest
+
+from sphera_amd import case2d
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
+FP64_REL_TOL = 1e-11
+FP32_TOL = {"coord": 1e-6, "vel": 3e-3, "acc": 3e-3, "dens": 1e-6, "pres": 3e-3,
+            "dden": 3e-3, "vpres": 3e-3}
+SMALL = dict(tank=(0.6, 0.4), column=(0.3, 0.2), dx=0.01, name="t2d")
+
+
+def _jitter2d(coord, dx, seed=20260303, amp=0.01):
+    rng = np.random.default_rng(seed)
+    out = coord.copy()
+    j = rng.uniform(-amp, amp, (len(coord), 2)) * dx
+    out[:, 0] += j[:, 0]
+    out[:, 2] += j[:, 1]
+    return out
+
+
+def _pair(tb, precision, coord=None):
+    from oracle.pyoracle import Oracle
+    from sphera_amd import runtime as rt
+    o = Oracle(tb, coord=coord)
+    o.preloop()
+    d = rt.SphContext(tb, precision=precision)
+    d.upload(coord=coord)
+    d.preloop()
+    return d, o
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _lists_equal(d, o):
+    dc, di = d.lists()
+    oc, oi = o.lists()
+    assert np.array_equal(dc, oc), "neighbour counts differ"
+    for i in range(len(dc)):
+        assert np.array_equal(di[i, :dc[i]], oi[i, :oc[i]]), f"list of particle {i} differs"
+
+
+def _btab_close(d, o):
+    dn, dside, dloc, dbi, _ = d.btab()
+    on, oside, oloc, obi, _ = o.btab()
+    assert np.array_equal(dn, on), "selected side counts differ"
+    m = np.arange(dside.shape[1])[None, :] < dn[:, None]
+    assert np.array_equal(dside[m], oside[:, :dside.shape[1]][m])
+    assert np.array_equal(dloc[m], oloc[:, :dside.shape[1]][m])
+    err = np.abs(dbi[m] - obi[:, :dside.shape[1]][m]).max(initial=0.0)
+    scale = np.abs(obi[:, :dside.shape[1]][m]).max(initial=1.0)
+    assert err <= 1e-13 * scale, f"boundary integrals differ by {err:.3e}"
+    return int(dn.sum())
+
+
+def test_fp64_2d_lists_boundary_table_and_steps_match_oracle():
+    from sphera_amd import runtime as rt
+    tb = case2d.build(case2d.DamBreak2D(**SMALL))
+    coord = _jitter2d(tb.particles["coord"], tb.case.dx)
+    print(f"[gpu] 2d n={len(coord)} fp64", flush=True)
+    d, o = _pair(tb, rt.FP64, coord)
+    _lists_equal(d, o)
+    assert _btab_close(d, o) > 50
+    assert d.state()["dt"] == o.state()["dt"]
+    for k in (1, 4, 10):
+        d.step(k)
+        o.step(k)
+        ds, os_ = d.state(), o.state()
+        assert np.array_equal(ds["ids"], np.arange(len(ds["ids"])))
+        for f in FIELDS:
+            e = _rel(ds[f], os_[f])
+            assert e <= FP64_REL_TOL, f"{f} rel err {e:.3e} after {k} more steps"
+        _lists_equal(d, o)
+        _btab_close(d, o)
+    d.close()
+
+
+def test_fp32_mixed_2d_within_tolerance():
+    from sphera_amd import runtime as rt
+    tb = case2d.build(case2d.DamBreak2D(**SMALL))
+    coord = _jitter2d(tb.particles["coord"], tb.case.dx)
+    print(f"[gpu] 2d n={len(coord)} fp32-mixed", flush=True)
+    d, o = _pair(tb, rt.FP32_MIXED, coord)
+    _lists_equal(d, o)
+    d.step(20)
+    o.step(20)
+    ds, os_ = d.state(), o.state()
+    for f in FIELDS:
+        e = _rel(ds[f], os_[f])
+        assert e <= FP32_TOL[f], f"{f} rel err {e:.3e}"
+    assert np.all(ds["coord"][:, 1] == 0.0) and np.all(ds["vel"][:, 1] == 0.0)
+    d.close()
+
+
+def test_full_2d_dam_break_fp64_and_graph():
+    """3.22 x 1.8 m tank, 1.0 x 0.5 m column, dx = 0.01 m (5,000 particles):
+    200 steps eagerly and as a replayed step graph against the oracle."""
+    from sphera_amd import runtime as rt
+    tb = case2d.build(case2d.DamBreak2D())
+    print(f"[gpu] 2d full n={len(tb.particles['coord'])} fp64 200 steps", flush=True)
+    d, o = _pair(tb, rt.FP64)
+    g = rt.SphContext(tb, precision=rt.FP64)
+    g.upload()
+    g.preloop()
+    d.step(200)
+    o.step(200)
+    for _ in range(2):
+        g.step_graph(100)
+    g.sync()
+    ds, gs, os_ = d.state(), g.state(), o.state()
+    for f in FIELDS:
+        e = _rel(ds[f], os_[f])
+        assert e <= 1e-9, f"{f} rel err {e:.3e} after 200 steps"
+        assert np.array_equal(ds[f], gs[f]), f"graph replay differs in {f}"
+    # the front has left the column and nothing crossed a wall
+    assert ds["coord"][:, 0].max() > 0.995
+    assert ds["coord"][:, 0].min() > 0.0 and ds["coord"][:, 2].min() > 0.0
+    d.close()
+    g.close()

@@ -11,7 +11,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from oracle.decks import HARNESS_HIP, read_dump, run_harness
+from oracle.decks import HARNESS2D_HIP, HARNESS_HIP, read_dump, run_harness
 from sphera_amd.case import DamBreak3D
 from tests.test_gpu_parity import FP32_TOL
 
@@ -85,3 +85,28 @@ def test_bridge_free_surface_equals_reference(tmp_path):
     assert np.allclose(t, g["time"], rtol=1e-6, atol=1e-12)
     assert np.allclose(pel, g["pelolib"], rtol=1e-6, atol=1e-12)
     assert np.allclose(zl, g["zlft"], rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.skipif(not HARNESS2D_HIP.exists(), reason="2D drop-in harness not built")
+def test_bridge_2d_fp64_matches_reference(tmp_path):
+    """The reference's 2D build (-DSPACE_2D) driving the device through the
+    same bridge (sph_ctx_create2d with its BoundarySide tables): fields after
+    10 steps against the 2D reference output (tests/golden/ref_2d.npz) within
+    the 2D FP64 tolerance of test_gpu_2d.py (the device's trigonometric
+    functions in the corner integrals)."""
+    from sphera_amd import case2d
+    from tests.golden.make_golden import CASE2D
+    from tests.test_gpu_2d import FP64_REL_TOL
+    g = np.load(GOLD / "ref_2d.npz")
+    case = case2d.DamBreak2D(**CASE2D)
+    print("[bridge] 2d steps=10", flush=True)
+    hdr, steps = read_dump(run_harness(case, nsteps=10, every=10, lists=False,
+                                       ic_coord=g["ic_coord"], workdir=tmp_path,
+                                       exe=HARNESS2D_HIP, timeout=120,
+                                       extra_env={"SPH_HIP_PRECISION": "0"}))
+    st = {r["it"]: r for r in steps}
+    for f in FIELDS + ("var",):
+        ref = g["s10_" + f]
+        err = np.abs(st[10][f] - ref).max() / max(np.abs(ref).max(), 1e-300)
+        assert err <= FP64_REL_TOL, (f, err)
+    assert abs(st[10]["dt"] - float(g["s10_dt"])) <= 1e-12 * float(g["s10_dt"])
