@@ -51,13 +51,19 @@ class Request:
     done_steps: int = 0
 
 
-def ensemble_case(seed, size_range=(0.2, 0.6), dx_range=(0.008, 0.02)):
-    """Dam break of request `seed`: tank width, depth and height ~U(size_range)
-    m, water column (width/2, depth, 0.8 height), dx ~U(dx_range) m."""
+ENSEMBLE_TANK = (1.6, 0.6, 0.6)
+
+
+def ensemble_case(seed, size_range=(0.2, 0.6), dx_range=(0.008, 0.02), tank=ENSEMBLE_TANK):
+    """Dam break of request `seed` in the fixed 1.6 x 0.6 x 0.6 m tank: water
+    column width, depth and height ~U(size_range) m each (clipped to the
+    tank), dx ~U(dx_range) m; from 1k particles (0.2 m cube at dx = 0.02 m)
+    to 420k (0.6 m cube at dx = 0.008 m)."""
     r = np.random.default_rng(seed)
     W, D, H = r.uniform(*size_range, 3)
     dx = float(r.uniform(*dx_range))
-    return DamBreak3D(tank=(W, D, H), column=(0.5 * W, D, 0.8 * H), dx=dx, name=f"req{seed}")
+    col = (min(W, tank[0]), min(D, tank[1]), min(H, tank[2]))
+    return DamBreak3D(tank=tuple(tank), column=col, dx=dx, name=f"req{seed}")
 
 
 def make_ensemble(count, steps, rate, seed0=0, arrival_seed=12345, **kw):
