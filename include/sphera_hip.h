@@ -186,6 +186,34 @@ int sph_device_memory(unsigned long long* free_bytes, unsigned long long* total_
 /* busy = 1 while work enqueued on the context's stream is running, else 0. */
 int sph_query(sph_ctx* ctx, int32_t* busy);
 
+/* ---- segmented batches --------------------------------------------------- *
+ * No counterpart in the reference (one case per process there).  A batch
+ * context holds nseg independent solves (dam breaks of uneven particle
+ * counts, each with its own grid, h, tables and dt) concatenated in one set of
+ * device arrays: every kernel of a step is launched once over all segments,
+ * the workgroups finding their segment from per-block maps, so one step (or
+ * one graph replay, sph_step_graph) advances every solve.  Each segment's
+ * arithmetic is the one-solve context's, so its results equal those of the
+ * same solve alone.  p[s] and f[s] describe segment s as for sph_ctx_create;
+ * all share precision and wall treatment; launch tunables come from p[0].
+ * The phase entry points, sph_preloop and the sph_step family act on every
+ * segment.  Not available on a batch: sph_global_quantities, the free-surface
+ * monitor and the debug dumps (status 2). */
+int sph_batch_create(int32_t nseg, const sph_params* p, const sph_faces* const* f, sph_ctx** out);
+/* Device bytes a batch context of these segments would hold. */
+int sph_batch_bytes(int32_t nseg, const sph_params* p, unsigned long long* bytes);
+/* Per-segment state: as sph_state_upload / sph_state_download /
+ * sph_get_scalars / sph_particle_count for segment seg.  A batch is loaded by
+ * one sph_segment_upload per segment; the call that completes the set orders
+ * all segments on their grids (OrdGrid1) and leaves the context ready for
+ * sph_preloop. */
+int sph_segment_upload(sph_ctx* ctx, int32_t seg, int32_t n, const double* coord, const double* vel,
+                       const double* dens, const double* pres, const double* mass);
+int sph_segment_download(sph_ctx* ctx, int32_t seg, double* coord, double* vel, double* acc,
+                         double* dens, double* pres, double* dden, double* vpres, int32_t* ids);
+int sph_segment_scalars(sph_ctx* ctx, int32_t seg, double out[5]);
+int sph_segment_count(sph_ctx* ctx, int32_t seg, int32_t* n);
+
 /* Launch shape in use: wg_size, list entries prefetched per step of the
  * FP32 list walks, grid of the particle kernels. */
 int sph_get_launch(sph_ctx* ctx, int32_t out[3]);

@@ -45,7 +45,16 @@ struct Timer {
 }  // namespace
 
 struct sph_ctx {
-    DP P{};
+    DP P{};  // a one-solve context's constants; segment 0's (with the context capacity) otherwise
+    // segmented context: nseg solves in one set of arrays (see Seg)
+    int nseg = 1;
+    std::vector<DP> segs;
+    Seg S{};
+    DP* d_P = nullptr;
+    int *d_pblk = nullptr, *d_kblk = nullptr, *d_rseg = nullptr, *d_adj = nullptr;
+    int nkeys = 1, nrows = 1;  // cell keys (cells + out-of-grid bucket per segment), cell rows
+    int nb_stride_max = 1;     // largest NMAXPARTJ of the segments (list capacity)
+    std::vector<int> staged;   // segments uploaded since the last commit
     int precision = SPH_FP64;
     hipStream_t stream = nullptr;
     // faces / tables
@@ -171,44 +180,58 @@ void scan_ints(sph_ctx* c, const int* in, int* out, int* bs, int m, int* total) 
     k_scan_add<<<nblk(m), 256, 0, c->stream>>>(out, bs, m);
 }
 
+// Launches kernel K in its one-solve (<false>) or segmented (<true>) form.
+#define SEGL(K, G, B, ...)                                                                 \
+    do {                                                                                   \
+        if (c->nseg > 1) K<true><<<(G), (B), 0, c->stream>>>(c->P, c->S, __VA_ARGS__);      \
+        else K<false><<<(G), (B), 0, c->stream>>>(c->P, c->S, __VA_ARGS__);                 \
+    } while (0)
+
+__global__ void k_fill_strided(int* p, int stride, int v, int m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) p[(size_t)i * stride] = v;
+}
+
 // cell sort + block descriptors (OrdGrid1)
 int run_sort(sph_ctx* c) {
-    const DP& P = c->P;
-    const int m = P.nmax + 1;
-    const int cap = P.cap;
+    const int m = c->nkeys;
+    const int cap = c->P.cap;
+    const int ns = c->nseg;
     {
         KTime t(c, "sort_count");
         k_fill_int<<<nblk(m), 256, 0, c->stream>>>(c->cnt, 0, m);
-        k_fill_int<<<1, 64, 0, c->stream>>>(c->iscal + IS_NRM, 0, 1);
-        k_cell_count<<<nblk(cap), 256, 0, c->stream>>>(P, c->xm[1], c->vel[2], c->key, c->slot, c->cnt,
-                                                       c->idslot, c->rmids, c->iscal);
-        k_renumber<<<1, 64, 0, c->stream>>>(c->rmids, c->idslot, c->vel[2], c->iscal);
+        k_fill_strided<<<nblk(ns, 64), 64, 0, c->stream>>>(c->iscal + IS_NRM, IS_COUNT, 0, ns);
+        SEGL(k_cell_count, nblk(cap), 256, c->xm[1], c->vel[2], c->key, c->slot, c->cnt, c->idslot, c->rmids,
+             c->iscal);
+        SEGL(k_renumber, nblk(ns, 64), 64, c->rmids, c->idslot, c->vel[2], c->iscal);
     }
     {
         KTime t(c, "sort_scan");
         scan_ints(c, c->cnt, c->start, c->bsum, m, nullptr);
+        if (ns > 1) {  // each segment's slots start at its base
+            k_seg_adj<<<nblk(ns, 64), 64, 0, c->stream>>>(c->S, c->start, c->d_adj);
+            k_seg_add<<<nblk(m), 256, 0, c->stream>>>(c->S, c->start, c->d_adj, m);
+        }
     }
     {
         KTime t(c, "sort_scatter");
-        k_scatter<<<nblk(cap), 256, 0, c->stream>>>(c->key, c->slot, c->start, c->vel[2], c->perm, c->pid,
-                                                    c->iscal);
-        k_cell_sort<<<(P.nmax + 255) / 256, 256, 0, c->stream>>>(P, c->start, c->perm, c->pid);
+        SEGL(k_scatter, nblk(cap), 256, c->key, c->slot, c->start, c->vel[2], c->perm, c->pid, c->iscal);
+        SEGL(k_cell_sort, ns > 1 ? m / 256 : (c->P.nmax + 255) / 256, 256, c->start, c->perm, c->pid);
     }
     {
         KTime t(c, "sort_gather");
-        k_gather<<<nblk(cap), 256, 0, c->stream>>>(P, c->perm, c->key, c->xm[1], c->vel[2], c->rp[0],
-                                                   c->lam[0], c->acc[1], c->xm[0], c->vel[0], c->rp[1],
-                                                   c->lam[1], c->acc[0], c->cell, c->xmf, XFo(c, 0), XFo(c, 1), XFo(c, 2), VFo(c, 0),
-                                                   c->rpf[1], c->start, c->iscal);
+        SEGL(k_gather, nblk(cap), 256, c->perm, c->key, c->xm[1], c->vel[2], c->rp[0], c->lam[0], c->acc[1],
+             c->xm[0], c->vel[0], c->rp[1], c->lam[1], c->acc[0], c->cell, c->xmf, XFo(c, 0), XFo(c, 1),
+             XFo(c, 2), VFo(c, 0), c->rpf[1], c->start, c->iscal);
     }
     return 0;
 }
 
 int run_neighbours(sph_ctx* c) {
     KTime t(c, "neighbours");
-    k_nb_rowblocks<<<1, 1024, 0, c->stream>>>(c->P, c->start, c->rowblk);
-    k_neighbours<<<c->nb_grid, NB_WG, 0, c->stream>>>(c->P, c->xm[0], c->xmf, c->cell, c->start,
-                                                     c->rowblk, c->nbr, c->nnb, c->iscal, c->pairs);
+    SEGL(k_nb_rowblocks, 1, 1024, c->start, c->rowblk);
+    SEGL(k_neighbours, c->nb_grid, NB_WG, c->xm[0], c->xmf, c->cell, c->start, c->rowblk, c->nbr, c->nnb,
+         c->iscal, c->pairs);
     return 0;
 }
 
@@ -226,104 +249,74 @@ int run_btab(sph_ctx* c) {
     {
         KTime t(c, "boundary_find");
         k_fill_int<<<1, 64, 0, c->stream>>>(c->nwork, 0, 1);
-        k_boundary_find<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->d_faces, c->d_gnear, c->d_gcount,
-                                                                c->d_gstart, c->d_gvec, c->xm[0], c->bn, c->bt,
-                                                                c->work, c->nwork, c->iscal);
+        SEGL(k_boundary_find, nblk(c->P.cap), 256, c->d_faces, c->d_gnear, c->d_gcount, c->d_gstart, c->d_gvec,
+             c->xm[0], c->bn, c->bt, c->work, c->nwork, c->iscal);
     }
     KTime t(c, "boundary_int");
     if (c->P.f32)  // eight lanes per pair
-        k_boundary_int_f32<<<4 * c->bt_grid, 256, 0, c->stream>>>(c->P, c->d_faces, c->d_btf, c->bt, c->work,
-                                                                  c->nwork);
+        SEGL(k_boundary_int_f32, 4 * c->bt_grid, 256, c->d_faces, c->d_btf, c->bt, c->work, c->nwork);
     else
-        k_boundary_int<<<c->bt_grid, 256, 0, c->stream>>>(c->P, c->d_faces, c->d_kt, c->d_bit, c->bt, c->work,
-                                                          c->nwork);
+        SEGL(k_boundary_int, c->bt_grid, 256, c->d_faces, c->d_kt, c->d_bit, c->bt, c->work, c->nwork);
     return 0;
 }
 
 int run_dt(sph_ctx* c, int advance, bool reduce) {
     KTime t(c, "dt");
-    if (reduce)
-        k_dtmin<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->vel[0], c->dtmin, c->iscal);
-    k_dt_begin<<<1, 64, 0, c->stream>>>(c->P, c->scal, c->dtmin, c->iscal, advance);
+    if (reduce) SEGL(k_dtmin, nblk(c->P.cap), 256, c->vel[0], c->dtmin, c->iscal);
+    SEGL(k_dt_begin, nblk(c->nseg, 64), 64, c->scal, c->dtmin, c->iscal, advance);
     return 0;
 }
 
+// The pair passes: FP32-mixed with the list-walk chunk of the pass (4 or 8;
+// segmented contexts use 8), or FP64; one-solve or segmented form.
+#define PAIRL(K, CHV, ...)                                                                          \
+    do {                                                                                            \
+        if (c->P.f32) {                                                                             \
+            if (c->nseg > 1) K<true, 8, true><<<c->grid, c->wg, 0, c->stream>>>(c->P, c->S, __VA_ARGS__); \
+            else if ((CHV) == 4) K<true, 4, false><<<c->grid, c->wg, 0, c->stream>>>(c->P, c->S, __VA_ARGS__); \
+            else K<true, 8, false><<<c->grid, c->wg, 0, c->stream>>>(c->P, c->S, __VA_ARGS__);        \
+        } else {                                                                                    \
+            if (c->nseg > 1) K<false, 4, true><<<c->grid, c->wg, 0, c->stream>>>(c->P, c->S, __VA_ARGS__); \
+            else K<false, 4, false><<<c->grid, c->wg, 0, c->stream>>>(c->P, c->S, __VA_ARGS__);      \
+        }                                                                                           \
+    } while (0)
+
 int run_momentum(sph_ctx* c) {
     KTime t(c, "momentum");
-    if (c->P.f32)
-        {
-            auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[0], XF(c, 0), VF(c, 0), c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
-            c->vel[1], VFo(c, 1), c->scal, c->iscal); };
-            if (c->chunk == 4) L(k_momentum<true, 4>);
-            else L(k_momentum<true, 8>);
-        }
-    else
-        k_momentum<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[0], XF(c, 0), VF(c, 0), c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1],
-            c->vel[1], VFo(c, 1), c->scal, c->iscal);
+    PAIRL(k_momentum, c->chunk, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0], c->rp[0], XF(c, 0), VF(c, 0),
+          c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1], c->vel[1], VFo(c, 1), c->scal, c->iscal);
     return 0;
 }
 
 int run_vsmooth(sph_ctx* c) {
     KTime t(c, "velocity_smoothing");
-    if (c->P.f32)
-        {
-            auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
-            c->rp[0], XF(c, 1), VF(c, 1), c->nbr, c->nnb, c->vel[2], VFo(c, 2), c->xm[1], c->scal,
-            c->dtmin, c->iscal); };
-            if (c->chunk_v == 4) L(k_vsmooth<true, 4>);
-            else L(k_vsmooth<true, 8>);
-        }
-    else
-        k_vsmooth<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1],
-            c->rp[0], XF(c, 1), VF(c, 1), c->nbr, c->nnb, c->vel[2], VFo(c, 2), c->xm[1], c->scal,
-            c->dtmin, c->iscal);
+    PAIRL(k_vsmooth, c->chunk_v, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1], c->rp[0], XF(c, 1), VF(c, 1),
+          c->nbr, c->nnb, c->vel[2], VFo(c, 2), c->xm[1], c->scal, c->dtmin, c->iscal);
     return 0;
 }
 
 int run_continuity(sph_ctx* c) {
     KTime t(c, "continuity");
-    if (c->P.f32)
-        {
-            auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[1], XF(c, 0), VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal); };
-            if (c->chunk_c == 4) L(k_continuity<true, 4>);
-            else L(k_continuity<true, 8>);
-        }
-    else
-        k_continuity<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
-            c->rp[1], XF(c, 0), VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
+    PAIRL(k_continuity, c->chunk_c, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0], c->rp[1], XF(c, 0),
+          VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
     return 0;
 }
 
 int run_psmooth(sph_ctx* c) {
     KTime t(c, "pressure_smoothing");
-    if (c->P.f32)
-        {
-            auto L = [&](auto kern) { kern<<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
-            c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal); };
-            if (c->chunk_p == 4) L(k_psmooth<true, 4>);
-            else L(k_psmooth<true, 8>);
-        }
-    else
-        k_psmooth<false, 4><<<c->grid, c->wg, 0, c->stream>>>(
-            c->P, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf,
-            c->rpf[2], c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal);
+    PAIRL(k_psmooth, c->chunk_p, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf, c->rpf[2], c->nbr,
+          c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal);
     return 0;
 }
 
 int check_flags(sph_ctx* c) {
-    int is[IS_COUNT];
-    HIPC(hipMemcpyAsync(is, c->iscal, sizeof(is), hipMemcpyDeviceToHost, c->stream));
+    std::vector<int> all((size_t)IS_COUNT * c->nseg);
+    HIPC(hipMemcpyAsync(all.data(), c->iscal, sizeof(int) * all.size(), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
+    int is[IS_COUNT] = {0};
+    for (int s = 0; s < c->nseg; ++s)
+        for (int k : {(int)IS_UNSUPPORTED, (int)IS_RM_OVERFLOW, (int)IS_BT_OVERFLOW})
+            is[k] |= all[(size_t)s * IS_COUNT + k];
     if (is[IS_UNSUPPORTED])
         return fail(95, "particle close to a boundary type the device path does not implement "
                         "(sour/velo/flow)");
@@ -349,6 +342,7 @@ sph_ctx::~sph_ctx() {
         if (p) hipFree(p);
     };
     F(d_faces); F(d_gnear); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit); F(d_btf);
+    F(d_P); F(d_pblk); F(d_kblk); F(d_rseg); F(d_adj);
     for (auto p : xm) F(p);
     for (auto p : vel) F(p);
     for (auto p : acc) F(p);
@@ -368,7 +362,8 @@ sph_ctx::~sph_ctx() {
 
 int sph_ctx::alloc() {
     const size_t N = P.cap;
-    const size_t m = P.nmax + 1;
+    const size_t m = nkeys;
+    const size_t ns = nseg;
     for (auto& p : xm) HIPC(da(&p, N));
     for (auto& p : vel) HIPC(da(&p, N));
     for (auto& p : acc) HIPC(da(&p, N));
@@ -389,16 +384,16 @@ int sph_ctx::alloc() {
     HIPC(da(&cnt, m));
     HIPC(da(&start, m + 1));
     HIPC(da(&bsum, (m + 1023) / 1024 + 1));
-    HIPC(da(&rowblk, (size_t)P.ncd[1] * P.ncd[2] + 1));
+    HIPC(da(&rowblk, (size_t)nrows + 1));
     HIPC(da(&idslot, N));
-    HIPC(da(&rmids, RM_CAP));
-    HIPC(da(&nbr, N * (size_t)nb_stride(P.nmaxpartj)));
+    HIPC(da(&rmids, RM_CAP * ns));
+    HIPC(da(&nbr, N * (size_t)nb_stride(nb_stride_max)));
     HIPC(da(&nnb, N));
     HIPC(da(&bn, N));
     HIPC(da(&bt, P.semi ? N * SPH_MB : 1));
-    HIPC(da(&scal, SC_COUNT));
-    HIPC(da(&iscal, IS_COUNT));
-    HIPC(da(&dtmin, 1));
+    HIPC(da(&scal, SC_COUNT * ns));
+    HIPC(da(&iscal, IS_COUNT * ns));
+    HIPC(da(&dtmin, ns));
     HIPC(da(&pairs, 1));
     if (dry) return 0;
     HIPC(hipMemset(pairs, 0, sizeof(unsigned long long)));
@@ -410,66 +405,28 @@ int sph_ctx::alloc() {
     HIPC(hipMemset(vpres, 0, sizeof(double) * N));
     HIPC(hipMemset(nnb, 0, sizeof(int) * N));
     HIPC(hipMemset(bn, 0, sizeof(int) * N));
-    HIPC(hipMemset(scal, 0, sizeof(double) * SC_COUNT));
-    HIPC(hipMemset(iscal, 0, sizeof(int) * IS_COUNT));
+    HIPC(hipMemset(scal, 0, sizeof(double) * SC_COUNT * ns));
+    HIPC(hipMemset(iscal, 0, sizeof(int) * IS_COUNT * ns));
     return 0;
 }
 
-extern "C" {
+namespace {
 
-const char* sph_last_error(void) { return g_err.c_str(); }
-
-int sph_ctx_bytes(const sph_params* p, unsigned long long* bytes) {
-    if (!p || !bytes) return fail(1, "null argument");
-    if (p->capacity <= 0 || p->nmaxpartj <= 0) return fail(2, "invalid capacity or nmaxpartj");
-    sph_ctx c;
-    c.dry = true;
-    c.P.cap = p->capacity;
-    c.P.nmaxpartj = p->nmaxpartj;
-    c.P.nmax = p->ncd[0] * p->ncd[1] * p->ncd[2];
-    for (int d = 0; d < 3; ++d) c.P.ncd[d] = p->ncd[d];
-    c.P.f32 = p->precision == SPH_FP32_MIXED;
-    c.P.semi = p->semi;
-    c.alloc();
-    *bytes = c.bytes;
-    return 0;
-}
-
-int sph_device_memory(unsigned long long* free_bytes, unsigned long long* total_bytes) {
-    if (!free_bytes || !total_bytes) return fail(1, "null argument");
-    size_t f = 0, t = 0;
-    HIPC(hipMemGetInfo(&f, &t));
-    *free_bytes = f;
-    *total_bytes = t;
-    return 0;
-}
-
-int sph_query(sph_ctx* c, int32_t* busy) {
-    if (!c || !busy) return fail(1, "null argument");
-    const hipError_t e = hipStreamQuery(c->stream);
-    if (e == hipErrorNotReady) {
-        *busy = 1;
-        return 0;
-    }
-    if (e != hipSuccess) return fail(100, std::string("hipStreamQuery: ") + hipGetErrorString(e));
-    *busy = 0;
-    return 0;
-}
-
-int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
-    if (!p || !out) return fail(1, "null argument");
+// Validates one solve's parameters and fills its constants (placement fields
+// zero).  The reference's values: ReadInput.f90:198-205 (h, 2h, eta, eta2),
+// Gest_Input.f90:266-289 (kernel constants).
+int fill_dp(const sph_params* p, const sph_faces* f, DP& P, bool dry = false) {
     if (p->capacity <= 0 || p->capacity > (int)NB_MASK) return fail(2, "capacity out of range");
     if (p->nmaxpartj <= 0) return fail(2, "nmaxpartj must be positive");
     for (int d = 0; d < 3; ++d)
         if (p->ncd[d] <= 0 || !(p->dcd[d] > 0.0)) return fail(2, "invalid grid");
     if (p->precision != SPH_FP64 && p->precision != SPH_FP32_MIXED) return fail(2, "invalid precision");
-    if (p->semi && (!f || f->nf <= 0)) return fail(2, "SA-SPH requested without frontier faces");
+    if (p->semi && !dry && (!f || f->nf <= 0)) return fail(2, "SA-SPH requested without frontier faces");
     const long long nmax = (long long)p->ncd[0] * p->ncd[1] * p->ncd[2];
     if (nmax >= (1LL << 30)) return fail(2, "grid too large");
-    sph_ctx* c = new sph_ctx();
-    DP& P = c->P;
-    c->precision = p->precision;
+    P = DP{};
     P.cap = p->capacity;
+    P.scap = p->capacity;
     P.nmaxpartj = p->nmaxpartj;
     P.nmax = (int)nmax;
     P.semi = p->semi ? 1 : 0;
@@ -518,118 +475,267 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
     P.f_alfamon = (float)P.alfamon;
     P.f_celerita = (float)P.celerita;
     P.f_kin_visc = (float)P.kin_visc;
-    int rc;
-    if (hipStreamCreate(&c->stream) != hipSuccess) {
-        delete c;
-        return fail(101, "hipStreamCreate failed");
-    }
-    // launch shape of the particle kernels (one lane per particle)
-    {
-        int dev = 0, ncu = 256;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        c->wg = p->wg_size > 0 ? p->wg_size : 64;
-        if (c->wg < 64 || c->wg > 256 || c->wg % 64) {
-            delete c;
-            return fail(2, "wg_size must be 64, 128, 192 or 256");
+    return 0;
+}
+
+inline int round256(long long v) { return (int)((v + 255) / 256 * 256); }
+
+// Places nseg solves in one context: slots, keys and rows of segment s follow
+// those of segment s-1 (slots and keys in whole 256-blocks).  One solve keeps
+// its own sizes.  Fills c->P (segment 0 with the context capacity), c->segs,
+// nkeys, nrows, nseg and the list stride; allocates nothing.
+int layout_segments(sph_ctx* c, int nseg, const sph_params* p, const sph_faces* const* f = nullptr,
+                    bool dry = false) {
+    c->nseg = nseg;
+    c->segs.assign(nseg, DP{});
+    long long slots = 0, keys = 0, rows = 0, faces = 0;
+    int nmaxpartj = 0;
+    for (int s = 0; s < nseg; ++s) {
+        const sph_faces* fs = f ? f[s] : nullptr;
+        if (int rc = fill_dp(&p[s], fs, c->segs[s], dry)) return rc;
+        DP& P = c->segs[s];
+        if (p[s].precision != p[0].precision || (p[s].semi != 0) != (p[0].semi != 0))
+            return fail(2, "the segments of a batch share precision and wall treatment");
+        P.seg = s;
+        P.base = (int)slots;
+        P.koff = (int)keys;
+        P.roff = (int)rows;
+        P.foff = (int)faces;
+        if (nseg == 1) {
+            slots = P.cap;
+            keys = (long long)P.nmax + 1;
+        } else {
+            slots += round256(P.scap);
+            keys += round256((long long)P.nmax + 1);
         }
-        c->grid = (P.cap + c->wg - 1) / c->wg;
-        c->chunk = p->chunk > 0 ? p->chunk : 8;
-        c->chunk_v = p->chunk_v > 0 ? p->chunk_v : c->chunk;
-        c->chunk_c = p->chunk_c > 0 ? p->chunk_c : c->chunk;
-        c->chunk_p = p->chunk_p > 0 ? p->chunk_p : c->chunk;
-        auto bad = [](int v) { return v != 4 && v != 8; };
-        if (bad(c->chunk) || bad(c->chunk_v) || bad(c->chunk_c) || bad(c->chunk_p)) {
-            delete c;
-            return fail(2, "chunk must be 4 or 8");
-        }
-        c->bt_grid = 4 * ncu;
-        // persistent neighbour-search workgroups: as many as fit a CU (LDS
-        // tile), times the CUs, a multiple of the 8 XCDs
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_neighbours, NB_WG, 0) != hipSuccess ||
-            per_cu <= 0)
-            per_cu = 3;
-        c->nb_grid = (per_cu * ncu + 7) / 8 * 8;
+        rows += (long long)P.ncd[1] * P.ncd[2];
+        faces += P.nf;
+        nmaxpartj = std::max(nmaxpartj, P.nmaxpartj);
+        if (slots > (long long)NB_MASK || keys >= (1LL << 30))
+            return fail(2, "batch too large for one context");
     }
-    if ((rc = c->alloc())) {
+    for (auto& P : c->segs) P.cap = (int)slots;
+    c->P = c->segs[0];
+    c->nkeys = (int)keys;
+    c->nrows = (int)rows;
+    c->nb_stride_max = nmaxpartj;
+    return 0;
+}
+
+// Launch shape of the particle kernels and the persistent grids.
+int setup_launch(sph_ctx* c, const sph_params* p) {
+    if (hipStreamCreate(&c->stream) != hipSuccess) return fail(101, "hipStreamCreate failed");
+    int dev = 0, ncu = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    c->wg = p->wg_size > 0 ? p->wg_size : 64;
+    if (c->wg < 64 || c->wg > 256 || c->wg % 64) return fail(2, "wg_size must be 64, 128, 192 or 256");
+    c->grid = (c->P.cap + c->wg - 1) / c->wg;
+    c->chunk = p->chunk > 0 ? p->chunk : 8;
+    c->chunk_v = p->chunk_v > 0 ? p->chunk_v : c->chunk;
+    c->chunk_c = p->chunk_c > 0 ? p->chunk_c : c->chunk;
+    c->chunk_p = p->chunk_p > 0 ? p->chunk_p : c->chunk;
+    auto bad = [](int v) { return v != 4 && v != 8; };
+    if (bad(c->chunk) || bad(c->chunk_v) || bad(c->chunk_c) || bad(c->chunk_p))
+        return fail(2, "chunk must be 4 or 8");
+    if (c->nseg > 1)  // the segmented pair passes are built for whole-group walks
+        c->chunk = c->chunk_v = c->chunk_c = c->chunk_p = 8;
+    c->bt_grid = 4 * ncu;
+    // persistent neighbour-search workgroups: as many as fit a CU (LDS
+    // tile), times the CUs, a multiple of the 8 XCDs
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_neighbours<false>, NB_WG, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 3;
+    c->nb_grid = (per_cu * ncu + 7) / 8 * 8;
+    return 0;
+}
+
+// The frontier tables of every segment, concatenated: faces (indices offset
+// by foff), per-key incidence (count, 1-based start into the face-index
+// vector, faces in the 27-cell neighbourhood), kernel and direction tables
+// and the FP32 quadrature table, one per segment.
+int upload_tables(sph_ctx* c, const sph_faces* const* f) {
+    const int nseg = c->nseg;
+    long long nf = 0, nv = 0;
+    for (int s = 0; s < nseg; ++s) {
+        const DP& P = c->segs[s];
+        if (!f || !f[s] || f[s]->nf <= 0) return 0;  // no frontier tables (no SA-SPH walls)
+        (void)P;
+        nf += f[s]->nf;
+        for (int i = 0; i < P.nmax; ++i) nv += f[s]->gcbf_count[i];
+    }
+    std::vector<DFace> hf;
+    hf.reserve(nf);
+    std::vector<int> gcount(c->nkeys, 0), gstart(c->nkeys, 0), gnear(c->nkeys, 0), gvec;
+    gvec.reserve(nv);
+    std::vector<double> kt, bit;
+    std::vector<BtF32> btf(nseg);
+    for (int s = 0; s < nseg; ++s) {
+        const sph_faces* fs = f[s];
+        const DP& P = c->segs[s];
+        for (int q = 0; q < fs->nf; ++q) {
+            DFace F;
+            std::memset(&F, 0, sizeof(F));
+            std::memcpy(F.T, fs->T + 9 * q, sizeof(F.T));
+            std::memcpy(F.LX, fs->LX + 8 * q, sizeof(F.LX));
+            std::memcpy(F.GX, fs->GX + 12 * q, sizeof(F.GX));
+            F.area = fs->area[q];
+            F.slip = fs->slip_coeff[q];
+            for (int d = 0; d < 3; ++d) {
+                F.vel[d] = fs->velocity[3 * q + d];
+                F.tvel[d] = fs->tvel[3 * q + d];
+            }
+            F.nodes = fs->nodes[q];
+            F.btype = fs->btype[q];
+            F.mode = fs->slip_mode[q];
+            F.lam = fs->lam_check[q];
+            if (F.nodes != 3 && F.nodes != 4) return fail(2, "SA-SPH faces must be triangles or quadrilaterals");
+            dface_point_coeffs(F);
+            hf.push_back(F);
+        }
+        long long sv = 0;
+        for (int i = 0; i < P.nmax; ++i) sv += fs->gcbf_count[i];
+        for (long long q = 0; q < sv; ++q)
+            if (fs->gcbf_vec[q] < 0 || fs->gcbf_vec[q] >= fs->nf) return fail(2, "gcbf_vec entry out of range");
+        const int voff = (int)gvec.size();
+        for (long long q = 0; q < sv; ++q) gvec.push_back(fs->gcbf_vec[q] + P.foff);
+        const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
+        for (int i = 0; i < P.nmax; ++i) {
+            gcount[P.koff + i] = fs->gcbf_count[i];
+            gstart[P.koff + i] = fs->gcbf_start[i] + voff;
+        }
+        // particles of cells with no face in reach skip the face search
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < nj; ++j)
+                for (int i = 0; i < ni; ++i) {
+                    int sum = 0;
+                    for (int dk = -1; dk <= 1; ++dk)
+                        for (int dj = -1; dj <= 1; ++dj)
+                            for (int di = -1; di <= 1; ++di) {
+                                const int ii = i + di, jj = j + dj, kk = k + dk;
+                                if (ii < 0 || ii >= ni || jj < 0 || jj >= nj || kk < 0 || kk >= nk) continue;
+                                sum += fs->gcbf_count[((size_t)kk * nj + jj) * ni + ii];
+                            }
+                    gnear[P.koff + ((size_t)k * nj + j) * ni + i] = sum;
+                }
+        kt.insert(kt.end(), fs->kerneltab, fs->kerneltab + 41 * 5);
+        bit.insert(bit.end(), fs->bitab, fs->bitab + 256 * 7);
+        build_btf32(fs->bitab, fs->kerneltab, fs->ktdelta, P.h, btf[s]);
+    }
+    if (gvec.empty()) gvec.push_back(0);
+    auto up = [&](auto** d, const auto& h) {
+        using T = typename std::remove_reference<decltype(h[0])>::type;
+        return hipMalloc((void**)d, sizeof(T) * h.size()) == hipSuccess &&
+               hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = up(&c->d_faces, hf) && up(&c->d_gcount, gcount) && up(&c->d_gstart, gstart) &&
+              up(&c->d_gnear, gnear) && up(&c->d_gvec, gvec) && up(&c->d_kt, kt) && up(&c->d_bit, bit) &&
+              up(&c->d_btf, btf);
+    if (!ok) return fail(102, "device allocation/copy of frontier tables failed");
+    return 0;
+}
+
+// Segment maps of a segmented context: per-segment constants, segment of
+// every 256-slot block, 256-key block and cell row.
+int upload_segment_maps(sph_ctx* c) {
+    if (c->nseg == 1) return 0;
+    std::vector<int> pblk(c->P.cap / 256), kblk(c->nkeys / 256), rseg(c->nrows);
+    for (int s = 0; s < c->nseg; ++s) {
+        const DP& P = c->segs[s];
+        const int sb = (s + 1 < c->nseg ? c->segs[s + 1].base : c->P.cap);
+        for (int b = P.base / 256; b < sb / 256; ++b) pblk[b] = s;
+        const int ke = (s + 1 < c->nseg ? c->segs[s + 1].koff : c->nkeys);
+        for (int b = P.koff / 256; b < ke / 256; ++b) kblk[b] = s;
+        for (int r = 0; r < P.ncd[1] * P.ncd[2]; ++r) rseg[P.roff + r] = s;
+    }
+    bool ok = hipMalloc(&c->d_P, sizeof(DP) * c->nseg) == hipSuccess &&
+              hipMemcpy(c->d_P, c->segs.data(), sizeof(DP) * c->nseg, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMalloc(&c->d_pblk, sizeof(int) * pblk.size()) == hipSuccess &&
+              hipMemcpy(c->d_pblk, pblk.data(), sizeof(int) * pblk.size(), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMalloc(&c->d_kblk, sizeof(int) * kblk.size()) == hipSuccess &&
+              hipMemcpy(c->d_kblk, kblk.data(), sizeof(int) * kblk.size(), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMalloc(&c->d_rseg, sizeof(int) * rseg.size()) == hipSuccess &&
+              hipMemcpy(c->d_rseg, rseg.data(), sizeof(int) * rseg.size(), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMalloc(&c->d_adj, sizeof(int) * c->nseg) == hipSuccess;
+    if (!ok) return fail(102, "device allocation/copy of the segment maps failed");
+    c->S = Seg{c->d_P, c->d_pblk, c->d_kblk, c->d_rseg, c->nseg, c->nrows};
+    return 0;
+}
+
+int create_ctx(int nseg, const sph_params* p, const sph_faces* const* f, sph_ctx** out) {
+    sph_ctx* c = new sph_ctx();
+    int rc = layout_segments(c, nseg, p, f);
+    if (!rc) c->precision = p[0].precision;
+    if (!rc) rc = setup_launch(c, &p[0]);
+    if (!rc) rc = c->alloc();
+    if (!rc) rc = upload_tables(c, f);
+    if (!rc) rc = upload_segment_maps(c);
+    if (rc) {
         delete c;
         return rc;
     }
-    if (f && f->nf > 0) {
-        std::vector<DFace> hf(f->nf);
-        for (int q = 0; q < f->nf; ++q) {
-            DFace& F = hf[q];
-            std::memcpy(F.T, f->T + 9 * q, sizeof(F.T));
-            std::memcpy(F.LX, f->LX + 8 * q, sizeof(F.LX));
-            std::memcpy(F.GX, f->GX + 12 * q, sizeof(F.GX));
-            F.area = f->area[q];
-            F.slip = f->slip_coeff[q];
-            for (int d = 0; d < 3; ++d) {
-                F.vel[d] = f->velocity[3 * q + d];
-                F.tvel[d] = f->tvel[3 * q + d];
-            }
-            F.nodes = f->nodes[q];
-            F.btype = f->btype[q];
-            F.mode = f->slip_mode[q];
-            F.lam = f->lam_check[q];
-            if (F.nodes != 3 && F.nodes != 4) {
-                delete c;
-                return fail(2, "SA-SPH faces must be triangles or quadrilaterals");
-            }
-            dface_point_coeffs(F);
-        }
-        long long nv = 0;
-        for (int i = 0; i < P.nmax; ++i) nv += f->gcbf_count[i];
-        bool ok = hipMalloc(&c->d_faces, sizeof(DFace) * f->nf) == hipSuccess &&
-                  hipMalloc(&c->d_gcount, sizeof(int) * P.nmax) == hipSuccess &&
-                  hipMalloc(&c->d_gstart, sizeof(int) * P.nmax) == hipSuccess &&
-                  hipMalloc(&c->d_gvec, sizeof(int) * (nv ? nv : 1)) == hipSuccess &&
-                  hipMalloc(&c->d_kt, sizeof(double) * 41 * 5) == hipSuccess &&
-                  hipMalloc(&c->d_bit, sizeof(double) * 256 * 7) == hipSuccess;
-        ok = ok && hipMemcpy(c->d_faces, hf.data(), sizeof(DFace) * f->nf, hipMemcpyHostToDevice) == hipSuccess;
-        ok = ok && hipMemcpy(c->d_gcount, f->gcbf_count, sizeof(int) * P.nmax, hipMemcpyHostToDevice) == hipSuccess;
-        {
-            // particles of cells with no face in reach skip the face search
-            std::vector<int> near(P.nmax, 0);
-            const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
-            for (int k = 0; k < nk; ++k)
-                for (int j = 0; j < nj; ++j)
-                    for (int i = 0; i < ni; ++i) {
-                        int sum = 0;
-                        for (int dk = -1; dk <= 1; ++dk)
-                            for (int dj = -1; dj <= 1; ++dj)
-                                for (int di = -1; di <= 1; ++di) {
-                                    const int ii = i + di, jj = j + dj, kk = k + dk;
-                                    if (ii < 0 || ii >= ni || jj < 0 || jj >= nj || kk < 0 || kk >= nk) continue;
-                                    sum += f->gcbf_count[((size_t)kk * nj + jj) * ni + ii];
-                                }
-                        near[((size_t)k * nj + j) * ni + i] = sum;
-                    }
-            ok = ok && hipMalloc(&c->d_gnear, sizeof(int) * P.nmax) == hipSuccess &&
-                 hipMemcpy(c->d_gnear, near.data(), sizeof(int) * P.nmax, hipMemcpyHostToDevice) == hipSuccess;
-        }
-        ok = ok && hipMemcpy(c->d_gstart, f->gcbf_start, sizeof(int) * P.nmax, hipMemcpyHostToDevice) == hipSuccess;
-        if (nv) ok = ok && hipMemcpy(c->d_gvec, f->gcbf_vec, sizeof(int) * nv, hipMemcpyHostToDevice) == hipSuccess;
-        ok = ok && hipMemcpy(c->d_kt, f->kerneltab, sizeof(double) * 41 * 5, hipMemcpyHostToDevice) == hipSuccess;
-        ok = ok && hipMemcpy(c->d_bit, f->bitab, sizeof(double) * 256 * 7, hipMemcpyHostToDevice) == hipSuccess;
-        {
-            BtF32 T;
-            build_btf32(f->bitab, f->kerneltab, f->ktdelta, P.h, T);
-            ok = ok && hipMalloc(&c->d_btf, sizeof(BtF32)) == hipSuccess &&
-                 hipMemcpy(c->d_btf, &T, sizeof(BtF32), hipMemcpyHostToDevice) == hipSuccess;
-        }
-        if (!ok) {
-            delete c;
-            return fail(102, "device allocation/copy of frontier tables failed");
-        }
-        for (long long q = 0; q < nv; ++q)
-            if (f->gcbf_vec[q] < 0 || f->gcbf_vec[q] >= f->nf) {
-                delete c;
-                return fail(2, "gcbf_vec entry out of range");
-            }
-    }
     *out = c;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sph_last_error(void) { return g_err.c_str(); }
+
+int sph_ctx_bytes(const sph_params* p, unsigned long long* bytes) {
+    if (!p || !bytes) return fail(1, "null argument");
+    if (p->capacity <= 0 || p->nmaxpartj <= 0) return fail(2, "invalid capacity or nmaxpartj");
+    sph_ctx c;
+    c.dry = true;
+    if (int rc = layout_segments(&c, 1, p, nullptr, true)) return rc;
+    c.alloc();
+    *bytes = c.bytes;
+    return 0;
+}
+
+int sph_device_memory(unsigned long long* free_bytes, unsigned long long* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(1, "null argument");
+    size_t f = 0, t = 0;
+    HIPC(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return 0;
+}
+
+int sph_query(sph_ctx* c, int32_t* busy) {
+    if (!c || !busy) return fail(1, "null argument");
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipErrorNotReady) {
+        *busy = 1;
+        return 0;
+    }
+    if (e != hipSuccess) return fail(100, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+    *busy = 0;
+    return 0;
+}
+
+int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out) {
+    if (!p || !out) return fail(1, "null argument");
+    return create_ctx(1, p, f ? &f : nullptr, out);
+}
+
+int sph_batch_create(int32_t nseg, const sph_params* p, const sph_faces* const* f, sph_ctx** out) {
+    if (!p || !out) return fail(1, "null argument");
+    if (nseg < 1) return fail(2, "a batch holds at least one segment");
+    return create_ctx(nseg, p, f, out);
+}
+
+int sph_batch_bytes(int32_t nseg, const sph_params* p, unsigned long long* bytes) {
+    if (!p || !bytes) return fail(1, "null argument");
+    if (nseg < 1) return fail(2, "a batch holds at least one segment");
+    sph_ctx c;
+    c.dry = true;
+    if (int rc = layout_segments(&c, nseg, p, nullptr, true)) return rc;
+    c.alloc();
+    *bytes = c.bytes;
     return 0;
 }
 
@@ -677,8 +783,17 @@ int sph_ctx_create2d(const sph_params* p, const sph_sides2d* sd, sph_ctx** out) 
                 return fail(2, "previous side out of range");
             }
         }
-        bool ok = hipMalloc(&c->d_faces, sizeof(DFace) * P.nf) == hipSuccess &&
-                  hipMalloc(&c->d_bit, sizeof(double) * 22 * 6) == hipSuccess;
+        // the context was created without SA-SPH faces, so its boundary
+        // table holds one record: give it the SPH_MB slots per particle that
+        // k_boundary_2d writes
+        bool ok = true;
+        if (P.semi) {
+            hipFree(c->bt);
+            c->bt = nullptr;
+            ok = hipMalloc(&c->bt, sizeof(BTab) * (size_t)P.cap * SPH_MB) == hipSuccess;
+        }
+        ok = ok && hipMalloc(&c->d_faces, sizeof(DFace) * P.nf) == hipSuccess &&
+             hipMalloc(&c->d_bit, sizeof(double) * 22 * 6) == hipSuccess;
         ok = ok && hipMemcpy(c->d_faces, hs.data(), sizeof(DFace) * P.nf, hipMemcpyHostToDevice) == hipSuccess;
         ok = ok && hipMemcpy(c->d_bit, sd->bit2d, sizeof(double) * 22 * 6, hipMemcpyHostToDevice) == hipSuccess;
         if (!ok) {
@@ -700,8 +815,17 @@ int sph_ctx_destroy(sph_ctx* c) {
 
 int sph_state_upload(sph_ctx* c, int32_t n, const double* coord, const double* vel,
                      const double* dens, const double* pres, const double* mass) {
+    if (!c) return fail(1, "null argument");
+    if (c->nseg > 1) return fail(2, "a segmented context is loaded with sph_segment_upload");
+    return sph_segment_upload(c, 0, n, coord, vel, dens, pres, mass);
+}
+
+int sph_segment_upload(sph_ctx* c, int32_t seg, int32_t n, const double* coord, const double* vel,
+                       const double* dens, const double* pres, const double* mass) {
     if (!c || !coord || !vel || !dens || !pres || !mass) return fail(1, "null argument");
-    if (n < 0 || n > c->P.cap) return fail(3, "particle count exceeds the context capacity");
+    if (seg < 0 || seg >= c->nseg) return fail(3, "segment index out of range");
+    const DP& P = c->nseg > 1 ? c->segs[seg] : c->P;
+    if (n < 0 || n > P.scap) return fail(3, "particle count exceeds the segment capacity");
     std::vector<double4> hx(n), hv(n);
     std::vector<double2> hr(n);
     for (int i = 0; i < n; ++i) {
@@ -713,13 +837,28 @@ int sph_state_upload(sph_ctx* c, int32_t n, const double* coord, const double* v
     is[IS_N] = n;
     is[IS_ONGOING] = -999;
     double sc[SC_COUNT] = {0};
-    HIPC(hipMemcpyAsync(c->xm[1], hx.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
-    HIPC(hipMemcpyAsync(c->vel[2], hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
-    HIPC(hipMemcpyAsync(c->rp[0], hr.data(), sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
-    HIPC(hipMemsetAsync(c->lam[0], 0, sizeof(int) * c->P.cap, c->stream));
-    HIPC(hipMemsetAsync(c->acc[1], 0, sizeof(double4) * c->P.cap, c->stream));
-    HIPC(hipMemcpyAsync(c->iscal, is, sizeof(is), hipMemcpyHostToDevice, c->stream));
-    HIPC(hipMemcpyAsync(c->scal, sc, sizeof(sc), hipMemcpyHostToDevice, c->stream));
+    const size_t b = P.base;
+    if (c->staged.empty()) {  // first segment of a new upload: clear the whole context
+        HIPC(hipMemsetAsync(c->lam[0], 0, sizeof(int) * c->P.cap, c->stream));
+        HIPC(hipMemsetAsync(c->acc[1], 0, sizeof(double4) * c->P.cap, c->stream));
+        HIPC(hipMemsetAsync(c->iscal, 0, sizeof(int) * IS_COUNT * c->nseg, c->stream));
+        HIPC(hipMemsetAsync(c->scal, 0, sizeof(double) * SC_COUNT * c->nseg, c->stream));
+        c->staged.assign(c->nseg, 0);
+        c->phase = PH_EMPTY;
+    }
+    if (n) {
+        HIPC(hipMemcpyAsync(c->xm[1] + b, hx.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
+        HIPC(hipMemcpyAsync(c->vel[2] + b, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
+        HIPC(hipMemcpyAsync(c->rp[0] + b, hr.data(), sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPC(hipMemcpyAsync(c->iscal + (size_t)seg * IS_COUNT, is, sizeof(is), hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->scal + (size_t)seg * SC_COUNT, sc, sizeof(sc), hipMemcpyHostToDevice, c->stream));
+    // the staging buffers go out of scope: wait for the copies
+    HIPC(hipStreamSynchronize(c->stream));
+    c->staged[seg] = 1;
+    for (int s = 0; s < c->nseg; ++s)
+        if (!c->staged[s]) return 0;  // the other segments come in later calls
+    c->staged.clear();
     int rc = run_sort(c);
     if (rc) return rc;
     // the sort leaves (rho,p) in the buffer the continuity pass reads; the
@@ -735,29 +874,42 @@ int sph_state_upload(sph_ctx* c, int32_t n, const double* coord, const double* v
     return 0;
 }
 
-int sph_particle_count(sph_ctx* c, int32_t* n) {
+int sph_particle_count(sph_ctx* c, int32_t* n) { return sph_segment_count(c, 0, n); }
+
+int sph_segment_count(sph_ctx* c, int32_t seg, int32_t* n) {
     if (!c || !n) return fail(1, "null argument");
-    HIPC(hipMemcpy(n, c->iscal + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    if (seg < 0 || seg >= c->nseg) return fail(3, "segment index out of range");
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpy(n, c->iscal + (size_t)seg * IS_COUNT + IS_N, sizeof(int), hipMemcpyDeviceToHost));
     return 0;
 }
 
 int sph_state_download(sph_ctx* c, double* coord, double* vel, double* acc, double* dens,
                        double* pres, double* dden, double* vpres, int32_t* ids) {
+    return sph_segment_download(c, 0, coord, vel, acc, dens, pres, dden, vpres, ids);
+}
+
+int sph_segment_download(sph_ctx* c, int32_t seg, double* coord, double* vel, double* acc, double* dens,
+                         double* pres, double* dden, double* vpres, int32_t* ids) {
     if (!c) return fail(1, "null argument");
+    if (seg < 0 || seg >= c->nseg) return fail(3, "segment index out of range");
     if (c->phase != PH_SORTED && c->phase != PH_READY && c->phase != PH_PS)
         return fail(121, "state download is only defined at a time-step boundary");
     HIPC(hipStreamSynchronize(c->stream));
     int n;
-    HIPC(hipMemcpy(&n, c->iscal + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(&n, c->iscal + (size_t)seg * IS_COUNT + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    const size_t b = c->nseg > 1 ? c->segs[seg].base : 0;
     std::vector<double4> hx(n), hv(n), ha(n);
     std::vector<double2> hr(n);
     std::vector<double> hd(n), hp(n);
-    HIPC(hipMemcpy(hx.data(), c->xm[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hv.data(), c->vel[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(ha.data(), c->acc[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hr.data(), c->rp[0], sizeof(double2) * n, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hd.data(), c->dden, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hp.data(), c->vpres, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (n) {
+        HIPC(hipMemcpy(hx.data(), c->xm[0] + b, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hv.data(), c->vel[0] + b, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(ha.data(), c->acc[0] + b, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hr.data(), c->rp[0] + b, sizeof(double2) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hd.data(), c->dden + b, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hp.data(), c->vpres + b, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int a, int b) { return hv[a].w < hv[b].w; });
@@ -775,20 +927,24 @@ int sph_state_download(sph_ctx* c, double* coord, double* vel, double* acc, doub
     return 0;
 }
 
-int sph_get_scalars(sph_ctx* c, double out[5]) {
+int sph_get_scalars(sph_ctx* c, double out[5]) { return sph_segment_scalars(c, 0, out); }
+
+int sph_segment_scalars(sph_ctx* c, int32_t seg, double out[5]) {
     if (!c || !out) return fail(1, "null argument");
+    if (seg < 0 || seg >= c->nseg) return fail(3, "segment index out of range");
     HIPC(hipStreamSynchronize(c->stream));
     double sc[SC_COUNT];
     int is[IS_COUNT];
-    HIPC(hipMemcpy(sc, c->scal, sizeof(sc), hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(is, c->iscal, sizeof(is), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(sc, c->scal + (size_t)seg * SC_COUNT, sizeof(sc), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(is, c->iscal + (size_t)seg * IS_COUNT, sizeof(is), hipMemcpyDeviceToHost));
     out[0] = sc[SC_DT];
     out[1] = sc[SC_DTAVG];
     out[2] = sc[SC_TIME];
     out[3] = is[IS_IT];
     const int n = is[IS_N];
+    const size_t b = c->nseg > 1 ? c->segs[seg].base : 0;
     std::vector<int> nn(n);
-    HIPC(hipMemcpy(nn.data(), c->nnb, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (n) HIPC(hipMemcpy(nn.data(), c->nnb + b, sizeof(int) * n, hipMemcpyDeviceToHost));
     long long pc = 0;
     for (int i = 0; i < n; ++i) pc += nn[i] - 1;
     out[4] = (double)pc;
@@ -797,6 +953,7 @@ int sph_get_scalars(sph_ctx* c, double out[5]) {
 
 int sph_global_quantities(sph_ctx* c, double out[8]) {
     if (!c || !out) return fail(1, "null argument");
+    if (c->nseg > 1) return fail(2, "not available on a segmented context");
     if (c->phase != PH_SORTED && c->phase != PH_READY && c->phase != PH_PS)
         return fail(121, "global quantities are taken at a time-step boundary");
     const int nb = 2 * c->bt_grid / 4;  // two workgroups per CU
@@ -844,6 +1001,7 @@ int sph_monitor_lines(sph_ctx* c, int32_t nlines, const int32_t* icont, int32_t 
 
 int sph_free_surface(sph_ctx* c, double* pelolib, double* zlft) {
     if (!c || ((!pelolib || !zlft) && c->mon_nlines)) return fail(1, "null argument");
+    if (c->nseg > 1) return fail(2, "not available on a segmented context");
     if (c->phase != PH_SORTED && c->phase != PH_READY && c->phase != PH_PS)
         return fail(121, "the free surface is taken at a time-step boundary");
     const int nl = c->mon_nlines;
@@ -870,6 +1028,7 @@ int sph_pair_counter(sph_ctx* c, unsigned long long* total) {
 
 int sph_get_neighbour_lists(sph_ctx* c, int32_t* count, int32_t* ids, int32_t stride) {
     if (!c || !count) return fail(1, "null argument");
+    if (c->nseg > 1) return fail(2, "not available on a segmented context");
     if (stride < c->P.nmaxpartj && ids) return fail(3, "stride smaller than nmaxpartj");
     HIPC(hipStreamSynchronize(c->stream));
     int n;
@@ -900,6 +1059,7 @@ int sph_get_neighbour_lists(sph_ctx* c, int32_t* count, int32_t* ids, int32_t st
 int sph_get_boundary_table(sph_ctx* c, int32_t* ncbf, int32_t* face, double* loc, double* bi,
                            double* ig, int32_t stride) {
     if (!c || !ncbf || !face || !loc || !bi || !ig) return fail(1, "null argument");
+    if (c->nseg > 1) return fail(2, "not available on a segmented context");
     if (stride < SPH_MB) return fail(3, "stride smaller than the boundary-table slots");
     HIPC(hipStreamSynchronize(c->stream));
     int n;
@@ -934,7 +1094,7 @@ int sph_get_boundary_table(sph_ctx* c, int32_t* ncbf, int32_t* face, double* loc
 int sph_dt(sph_ctx* c) {
     if (!c) return fail(1, "null argument");
     if (int rc = expect(c, PH_READY, "sph_dt")) return rc;
-    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8, c->stream));
+    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8 * c->nseg, c->stream));
     run_dt(c, 1, true);
     c->phase = PH_DT;
     return 0;
@@ -1015,11 +1175,11 @@ int sph_preloop(sph_ctx* c) {
     if (c->phase != PH_SORTED) return fail(120, "sph_preloop needs a freshly uploaded state");
     run_neighbours(c);
     run_btab(c);
-    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8, c->stream));
+    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8 * c->nseg, c->stream));
     run_dt(c, 0, true);
     // the first step's time_step_duration sees the same velocities again
-    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8, c->stream));
-    k_dtmin<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->vel[0], c->dtmin, c->iscal);
+    HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8 * c->nseg, c->stream));
+    SEGL(k_dtmin, nblk(c->P.cap), 256, c->vel[0], c->dtmin, c->iscal);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(c->stream));
     collect_timing(c);

@@ -48,6 +48,29 @@ struct DP {
     // 3, or 2 for the x-z plane form (-DSPACE_2D build of the reference): one
     // cell row in y (every particle in row j = 1), boundary sides for faces
     int dim;
+    // Placement of this solve in a segmented context (several independent
+    // solves concatenated in one set of arrays; all zero for a context that
+    // holds one solve).  cap is the capacity of the whole context, so list
+    // and boundary-table strides are the same for every segment.
+    int seg;   // segment index
+    int base;  // first sorted slot of the segment (a multiple of 256)
+    int scap;  // slots reserved for the segment
+    int koff;  // first cell key: keys koff .. koff+nmax-1 are the cells, koff+nmax the
+               // particles out of the grid (a multiple of 256)
+    int roff;  // first cell row (fixed j, k) of the segment in the context's row list
+    int foff;  // first frontier face of the segment in the context's face array
+};
+
+// Segment maps of a segmented context, passed by value to the kernels that
+// take a segmented form.  Slots, keys and rows are partitioned in whole
+// 256-blocks (slots, keys) or whole rows, so a workgroup of a particle or key
+// kernel lies in one segment.
+struct Seg {
+    const DP* P;        // per-segment constants
+    const int* pblk;    // segment of each 256-slot block
+    const int* kblk;    // segment of each 256-key block
+    const int* rseg;    // segment of each cell row
+    int nseg, nrows;
 };
 
 // one SA-SPH frontier face (BoundaryFace + its Tratto/Partz data)

@@ -196,7 +196,10 @@ __device__ __forceinline__ bool d_point_internal(const DFace& F, double px0, dou
 // ----------------------------------------------------------------------------
 constexpr int HIST_W = 512;
 
-__global__ __launch_bounds__(256) void k_cell_count(DP P, const double4* __restrict__ xm,
+// Segmented contexts (SG): the workgroup's segment comes from the slot-block
+// map; slots, keys, ids and the removal list are the segment's.
+template <bool SG>
+__global__ __launch_bounds__(256) void k_cell_count(DP P0, Seg S, const double4* __restrict__ xm,
                                                     const double4* __restrict__ vel,
                                                     int* __restrict__ key, int* __restrict__ slot,
                                                     int* __restrict__ cnt, int* __restrict__ idslot,
@@ -205,23 +208,27 @@ __global__ __launch_bounds__(256) void k_cell_count(DP P, const double4* __restr
     __shared__ int base[HIST_W];
     __shared__ int cmin;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(blockIdx.x * blockDim.x) >> 8]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    iscal += sg * IS_COUNT;
+    rmids += sg * RM_CAP;
     const int n = iscal[IS_N];
     for (int b = threadIdx.x; b < HIST_W; b += blockDim.x) hist[b] = 0;
     if (threadIdx.x == 0) cmin = 0x7fffffff;
     __syncthreads();
     int k = -1;
-    if (i < n) {
+    if (i - P.base < n) {
         const double4 x = xm[i];
         const int c = d_particle_cell(P, x.x, x.y, x.z);
         const int id = (int)vel[i].w;
-        idslot[id] = i;
+        idslot[P.base + id] = i;
         if (c <= 0) {  // out of the grid: removed (OrdGrid1.f90:71-82)
-            k = P.nmax;
+            k = P.koff + P.nmax;
             atomicAdd(&iscal[IS_REMOVED], 1);
             const int q = atomicAdd(&iscal[IS_NRM], 1);
             if (q < RM_CAP) rmids[q] = id;
         } else {
-            k = c - 1;
+            k = P.koff + c - 1;
             atomicMin(&cmin, k);
         }
         key[i] = k;
@@ -231,7 +238,7 @@ __global__ __launch_bounds__(256) void k_cell_count(DP P, const double4* __restr
     int loc = -1, rank = 0;
     if (k >= 0) {
         const int d = k - c0;
-        if (k < P.nmax && d < HIST_W) {
+        if (k < P.koff + P.nmax && d < HIST_W) {
             loc = d;
             rank = atomicAdd(&hist[d], 1);
         } else {
@@ -286,9 +293,15 @@ __global__ __launch_bounds__(256) void k_scan_local(const int* __restrict__ in, 
 // ids r in ascending order while r < nag: the last particle moves to r unless
 // it is itself removed (then it is dropped and the next last is tried).  One
 // lane; removals are rare (particles leaving a closed tank).
-__global__ void k_renumber(int* __restrict__ rmids, const int* __restrict__ idslot,
+// Segmented contexts: one lane per segment.
+template <bool SG>
+__global__ void k_renumber(DP P0, Seg S, int* __restrict__ rmids, const int* __restrict__ idslot,
                            double4* __restrict__ vel, int* iscal) {
-    if (threadIdx.x != 0) return;
+    const int sg = SG ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : 0;
+    if (SG ? sg >= S.nseg : threadIdx.x != 0) return;
+    const int pbase = SG ? S.P[sg].base : P0.base;
+    iscal += sg * IS_COUNT;
+    rmids += sg * RM_CAP;
     const int m = iscal[IS_NRM];
     if (m == 0) return;
     if (m > RM_CAP) {
@@ -318,7 +331,7 @@ __global__ void k_renumber(int* __restrict__ rmids, const int* __restrict__ idsl
                 else hi = mid;
             }
             if (lo < m && rmids[lo] == last) continue;
-            vel[idslot[last]].w = (double)r;
+            vel[idslot[pbase + last]].w = (double)r;
             break;
         }
     }
@@ -353,11 +366,26 @@ __global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, 
     if (i < m) out[i] += bsum[i >> 10];
 }
 
-__global__ void k_scatter(const int* __restrict__ key, const int* __restrict__ slot,
+// Segmented contexts: the scan over the whole key space gives every key its
+// start counted from slot 0; a segment's slots begin at its base instead.
+// adj[s] = base_s - start[koff_s], added to every key of segment s.
+__global__ void k_seg_adj(Seg S, const int* __restrict__ start, int* __restrict__ adj) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S.nseg) adj[s] = S.P[s].base - start[S.P[s].koff];
+}
+__global__ void k_seg_add(Seg S, int* __restrict__ start, const int* __restrict__ adj, int m) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) start[k] += adj[S.kblk[k >> 8]];
+}
+
+template <bool SG>
+__global__ void k_scatter(DP P0, Seg S, const int* __restrict__ key, const int* __restrict__ slot,
                           const int* __restrict__ start, const double4* __restrict__ vel,
                           int* __restrict__ perm, int* __restrict__ pid, const int* iscal) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= iscal[IS_N]) return;
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(blockIdx.x * blockDim.x) >> 8]) : 0;
+    const int pbase = SG ? S.P[sg].base : P0.base;
+    if (i - pbase >= iscal[sg * IS_COUNT + IS_N]) return;
     const int d = start[key[i]] + slot[i];
     perm[d] = i;
     pid[d] = (int)vel[i].w;
@@ -368,12 +396,16 @@ __global__ void k_scatter(const int* __restrict__ key, const int* __restrict__ s
 // workgroup sorts the segments of 256 consecutive cells in LDS.
 constexpr int CSORT_CAP = 6144;
 
-__global__ __launch_bounds__(256) void k_cell_sort(DP P, const int* __restrict__ start,
+template <bool SG>
+__global__ __launch_bounds__(256) void k_cell_sort(DP P0, Seg S, const int* __restrict__ start,
                                                    int* __restrict__ perm, int* __restrict__ pid) {
     __shared__ int sid[CSORT_CAP];
     __shared__ int sperm[CSORT_CAP];
     const int c0 = blockIdx.x * 256;
-    const int c1 = min(c0 + 256, P.nmax);
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.kblk[blockIdx.x]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    const int c1 = min(c0 + 256, P.koff + P.nmax);
+    if (c0 >= c1) return;  // a block of the out-of-grid key or of the padding
     const int g0 = start[c0], g1 = start[c1];
     const int len = g1 - g0;
     const int c = c0 + threadIdx.x;
@@ -417,7 +449,8 @@ __global__ __launch_bounds__(256) void k_cell_sort(DP P, const int* __restrict__
 
 // permutes the state into the new order and writes the FP32 cell-relative
 // positions (used by the neighbour test in both precisions)
-__global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restrict__ key,
+template <bool SG>
+__global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* __restrict__ key,
                          const double4* __restrict__ xm_s, const double4* __restrict__ vel_s,
                          const double2* __restrict__ rp_s, const int* __restrict__ lam_s,
                          const double4* __restrict__ acc_s, double4* __restrict__ xm_d,
@@ -427,12 +460,15 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
                          F4Out velf,
                          float2* __restrict__ rpf, const int* __restrict__ start, int* iscal) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nin = start[P.nmax];  // particles still inside the grid
-    if (p == 0) {
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(blockIdx.x * blockDim.x) >> 8]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    iscal += sg * IS_COUNT;
+    const int nin = start[P.koff + P.nmax] - P.base;  // particles still inside the grid
+    if (p == P.base) {
         iscal[IS_NPREV] = iscal[IS_N];
         iscal[IS_N] = nin;
     }
-    if (p >= nin) return;
+    if (p - P.base >= nin) return;
     const int s = perm[p];
     const double4 x = xm_s[s];
     const double4 v = vel_s[s];
@@ -442,7 +478,7 @@ __global__ void k_gather(DP P, const int* __restrict__ perm, const int* __restri
     rp_d[p] = r;
     lam_d[p] = lam_s[s];
     acc_d[p] = acc_s[s];
-    const int c = key[s] + 1;
+    const int c = key[s] - P.koff + 1;
     cell_d[p] = c;
     int ci, cj, ck;
     d_cell_indices(P, c, ci, cj, ck);
@@ -558,8 +594,8 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
                 if (kk < 1 || kk > nk) continue;
                 const int cn = ((kk - 1) * nj + (jj - 1)) * ni + ii;
                 const int code = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-                nb_cell(P, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT, start[cn - 1],
-                        start[cn], 0, load, lo, hi, w, mj);
+                nb_cell(P, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT, start[P.koff + cn - 1],
+                        start[P.koff + cn], 0, load, lo, hi, w, mj);
             }
         }
     }
@@ -580,18 +616,26 @@ constexpr int NB_MAXC = 48;
 constexpr int NB_TILE = 3072;
 
 // rowblk[r] = first block of row r (exclusive scan of ceil(count_r / NB_WG)),
-// rowblk[R] = number of blocks.  One workgroup.
-__global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P, const int* __restrict__ start,
+// rowblk[R] = number of blocks.  One workgroup.  Segmented contexts: the rows
+// of all segments, each row read with its segment's grid.
+template <bool SG>
+__global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P0, Seg S, const int* __restrict__ start,
                                                        int* __restrict__ rowblk) {
     __shared__ int wsum[16];
     __shared__ int carry;
-    const int ni = P.ncd[0];
-    const int R = P.ncd[1] * P.ncd[2];
+    const int R = SG ? S.nrows : P0.ncd[1] * P0.ncd[2];
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < R; base += 1024) {
         const int r = base + threadIdx.x;
-        const int v = r < R ? (start[(r + 1) * ni] - start[r * ni] + NB_WG - 1) / NB_WG : 0;
+        int v = 0;
+        if (r < R) {
+            const DP& P = SG ? S.P[S.rseg[r]] : P0;
+            const int ni = P.ncd[0];
+            const int rl = r - P.roff;
+            const int k0 = P.koff + rl * ni;
+            v = (start[k0 + ni] - start[k0] + NB_WG - 1) / NB_WG;
+        }
         const int incl = wave_incl_scan(v);
         const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
         if (lane == 63) wsum[wid] = incl;
@@ -607,7 +651,8 @@ __global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P, const int* __restri
     if (threadIdx.x == 0) rowblk[R] = carry;
 }
 
-__global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __restrict__ xm,
+template <bool SG>
+__global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double4* __restrict__ xm,
                                                       const float4* __restrict__ xmf,
                                                       const int* __restrict__ cell,
                                                       const int* __restrict__ start,
@@ -620,18 +665,14 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
     __shared__ int seg_g[9];             // sorted index of a segment's first particle
     __shared__ int seg_l[9];             // its LDS offset
     __shared__ int seg_n[9];
-    __shared__ int s_row, s_ia, s_ib, s_p0, s_p1, s_global;
-    stencil_offsets(P);
-    const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
-    const int R = nj * nk;
+    __shared__ int s_row, s_ia, s_ib, s_p0, s_p1, s_global, s_seg;
+    if (!SG) stencil_offsets(P0);
+    int cur_seg = -1;
+    const int R = SG ? S.nrows : P0.ncd[1] * P0.ncd[2];
     const int nblocks = rowblk[R];
     const int per_xcd = gridDim.x >> 3;  // gridDim.x is a multiple of 8
     const int xcd = blockIdx.x & 7;
     const int b_lo = (int)((long long)nblocks * xcd / 8), b_hi = (int)((long long)nblocks * (xcd + 1) / 8);
-    const float sq = (float)P.square_doubleh;
-    const float lo = sq * (1.0f - 4.0e-5f), hi = sq * (1.0f + 4.0e-5f);
-    const int mj = P.nmaxpartj;
-    const size_t N = P.cap;
     unsigned long long mypairs = 0;
     for (int blk = b_lo + (blockIdx.x >> 3); blk < b_hi; blk += per_xcd) {
         if (threadIdx.x == 0) {
@@ -643,10 +684,15 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                 else b = m;
             }
             const int r = a;
-            const int p0 = start[r * ni] + (blk - rowblk[r]) * NB_WG;
-            const int p1 = min(p0 + NB_WG, start[(r + 1) * ni]);
-            const int ia = cell[p0] - r * ni, ib = cell[p1 - 1] - r * ni;
-            s_row = r;
+            const int sg0 = SG ? S.rseg[r] : 0;
+            const DP& Q = SG ? S.P[sg0] : P0;
+            const int rl = r - Q.roff;  // the row in its segment's grid
+            const int qi = Q.ncd[0];
+            const int p0 = start[Q.koff + rl * qi] + (blk - rowblk[r]) * NB_WG;
+            const int p1 = min(p0 + NB_WG, start[Q.koff + (rl + 1) * qi]);
+            const int ia = cell[p0] - rl * qi, ib = cell[p1 - 1] - rl * qi;
+            s_seg = sg0;
+            s_row = rl;
             s_ia = ia;
             s_ib = ib;
             s_p0 = p0;
@@ -654,6 +700,17 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
             s_global = ib - ia + 1 > NB_MAXC;
         }
         __syncthreads();
+        const int sg = SG ? __builtin_amdgcn_readfirstlane(s_seg) : 0;
+        const DP& P = SG ? S.P[sg] : P0;
+        if (SG && sg != cur_seg) {  // the stencil offsets of this segment's grid
+            stencil_offsets(P);
+            cur_seg = sg;
+        }
+        const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
+        const float sq = (float)P.square_doubleh;
+        const float lo = sq * (1.0f - 4.0e-5f), hi = sq * (1.0f + 4.0e-5f);
+        const int mj = P.nmaxpartj;
+        const size_t N = P.cap;
         const int r = s_row, ia = s_ia, ib = s_ib, p0 = s_p0, p1 = s_p1;
         const int jc = r % nj + 1, kc = r / nj + 1;
         const int ca = max(ia - 1, 1), cb = min(ib + 1, ni);
@@ -663,7 +720,7 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                 const int jj = jc + t / 3 - 1, kk = kc + t % 3 - 1;
                 int g0 = 0, g1 = 0;
                 if (jj >= 1 && jj <= nj && kk >= 1 && kk <= nk) {
-                    const int rb = ((kk - 1) * nj + (jj - 1)) * ni;
+                    const int rb = P.koff + ((kk - 1) * nj + (jj - 1)) * ni;
                     g0 = start[rb + ca - 1];
                     g1 = start[rb + cb];
                 }
@@ -693,7 +750,7 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                 int v;
                 if (jj < 1 || jj > nj || kk < 1 || kk > nk || ii < ca) v = seg_l[t];
                 else if (ii > cb) v = seg_l[t] + seg_n[t];
-                else v = seg_l[t] + start[((kk - 1) * nj + (jj - 1)) * ni + ii - 1] - seg_g[t];
+                else v = seg_l[t] + start[P.koff + ((kk - 1) * nj + (jj - 1)) * ni + ii - 1] - seg_g[t];
                 bnd[t][u] = v;
             }
             // stage the candidates' FP32 positions, four loads in flight per thread
@@ -745,14 +802,14 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P, const double4* __res
                 const int cnt = min(w.cnt, mj);
                 nnb[p] = cnt;
                 mypairs += (unsigned long long)(cnt - 1);
-                if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
+                if (ov) atomicAdd(&iscal[sg * IS_COUNT + IS_NB_OVERFLOW], 1);
             }
         } else if (p < p1) {
             bool ov;
             const int cnt = nb_particle_global(P, p, xm, xmf, cell, start, nbr, lo, hi, ov);
             nnb[p] = cnt;
             mypairs += (unsigned long long)(cnt - 1);
-            if (ov) atomicAdd(&iscal[IS_NB_OVERFLOW], 1);
+            if (ov) atomicAdd(&iscal[sg * IS_COUNT + IS_NB_OVERFLOW], 1);
         }
         __syncthreads();  // the tile and the block descriptors are reused
     }
@@ -856,7 +913,8 @@ __device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __
 // ComputeBoundaryVolumeIntegrals_P0.f90:76-155 for one pair per thread, so the
 // 256-direction quadrature keeps every lane busy.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_boundary_find(DP P, const DFace* __restrict__ faces,
+template <bool SG>
+__global__ __launch_bounds__(256) void k_boundary_find(DP P0, Seg S, const DFace* __restrict__ faces,
                                                        const int* __restrict__ gnear,
                                                        const int* __restrict__ gcount,
                                                        const int* __restrict__ gstart,
@@ -865,7 +923,14 @@ __global__ __launch_bounds__(256) void k_boundary_find(DP P, const DFace* __rest
                                                        int* __restrict__ bn, BTab* __restrict__ bt,
                                                        int* __restrict__ work, int* nwork, int* iscal) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= iscal[IS_N]) return;
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(blockIdx.x * blockDim.x) >> 8]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    iscal += sg * IS_COUNT;
+    if (i - P.base >= iscal[IS_N]) return;
+    // the per-cell face tables of a segmented context are indexed by key
+    gnear += P.koff;
+    gcount += P.koff;
+    gstart += P.koff;
     const double4 x = xm[i];
     const double xc[3] = {x.x, x.y, x.z};
     const int nc = d_particle_cell(P, x.x, x.y, x.z);
@@ -935,21 +1000,31 @@ __global__ __launch_bounds__(256) void k_boundary_find(DP P, const DFace* __rest
     }
 }
 
-__global__ __launch_bounds__(256) void k_boundary_int(DP P, const DFace* __restrict__ faces,
+// Segmented contexts: a workgroup's pairs may belong to several segments, so
+// each lane reads its segment's constants and tables (kernel and direction
+// tables of every segment, consecutive) from global memory.
+template <bool SG>
+__global__ __launch_bounds__(256) void k_boundary_int(DP P0, Seg S, const DFace* __restrict__ faces,
                                                       const double* __restrict__ kt_g,
                                                       const double* __restrict__ bit_g,
                                                       BTab* __restrict__ bt,
                                                       const int* __restrict__ work,
                                                       const int* nwork) {
-    __shared__ double kt[41 * 5];
-    __shared__ double bit[256 * 7];
-    for (int t = threadIdx.x; t < 41 * 5; t += blockDim.x) kt[t] = kt_g[t];
-    for (int t = threadIdx.x; t < 256 * 7; t += blockDim.x) bit[t] = bit_g[t];
-    __syncthreads();
+    __shared__ double kt_s[SG ? 1 : 41 * 5];
+    __shared__ double bit_s[SG ? 1 : 256 * 7];
+    if (!SG) {
+        for (int t = threadIdx.x; t < 41 * 5; t += blockDim.x) kt_s[t] = kt_g[t];
+        for (int t = threadIdx.x; t < 256 * 7; t += blockDim.x) bit_s[t] = bit_g[t];
+        __syncthreads();
+    }
     const int nw = *nwork;
     for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) {
         const int e = work[w];
         const int i = e & (int)NB_MASK, q = (int)((uint32_t)e >> NB_SHIFT);
+        const int sg = SG ? S.pblk[i >> 8] : 0;
+        const DP& P = SG ? S.P[sg] : P0;
+        const double* kt = SG ? kt_g + (size_t)sg * (41 * 5) : kt_s;
+        const double* bit = SG ? bit_g + (size_t)sg * (256 * 7) : bit_s;
         BTab& rec = bt[(size_t)q * P.cap + i];
         const double loc[3] = {rec.loc[0], rec.loc[1], rec.loc[2]};
         double bi[8], ig[9];
@@ -1322,21 +1397,24 @@ __device__ void bt_pair_f32_dirs(const BtF32& T, const DFace& F, double L0, doub
     }
 }
 
-__global__ __launch_bounds__(256) void k_boundary_int_f32(DP P, const DFace* __restrict__ faces,
+// Segmented contexts: the lanes of a pair read their segment's table (one
+// per segment, consecutive) from global memory.
+template <bool SG>
+__global__ __launch_bounds__(256) void k_boundary_int_f32(DP P0, Seg S, const DFace* __restrict__ faces,
                                                           const BtF32* __restrict__ tab,
                                                           BTab* __restrict__ bt,
                                                           const int* __restrict__ work,
                                                           const int* nwork) {
-    __shared__ BtF32 T;
+    __shared__ BtF32 T_s;
     const int nw = *nwork;
     const int per_block = blockDim.x / BT_RINGS;
     if (blockIdx.x * per_block >= nw) return;  // no pair for this workgroup
-    {
+    if (!SG) {
         const int4* src = reinterpret_cast<const int4*>(tab);
-        int4* dst = reinterpret_cast<int4*>(&T);
+        int4* dst = reinterpret_cast<int4*>(&T_s);
         for (int t = threadIdx.x; t < (int)(sizeof(BtF32) / 16); t += blockDim.x) dst[t] = src[t];
+        __syncthreads();
     }
-    __syncthreads();
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = gt & (BT_RINGS - 1);
     const int stride = (gridDim.x * blockDim.x) / BT_RINGS;
@@ -1344,6 +1422,9 @@ __global__ __launch_bounds__(256) void k_boundary_int_f32(DP P, const DFace* __r
     for (int w = gt / BT_RINGS; w < nw; w += stride) {
         const int e = work[w];
         const int i = e & (int)NB_MASK, q = (int)((uint32_t)e >> NB_SHIFT);
+        const int sg = SG ? S.pblk[i >> 8] : 0;
+        const DP& P = SG ? S.P[sg] : P0;
+        const BtF32& T = SG ? tab[sg] : T_s;
         BTab& rec = bt[(size_t)q * P.cap + i];
         const double L0 = rec.loc[0], L1 = rec.loc[1];
         double L2 = rec.loc[2];
@@ -1621,11 +1702,15 @@ __device__ __forceinline__ void block_min_to(unsigned long long v, unsigned long
     }
 }
 
-__global__ __launch_bounds__(256) void k_dtmin(DP P, const double4* __restrict__ vel,
+template <bool SG>
+__global__ __launch_bounds__(256) void k_dtmin(DP P0, Seg S, const double4* __restrict__ vel,
                                                unsigned long long* dtmin, const int* iscal) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(blockIdx.x * blockDim.x) >> 8]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    dtmin += sg;
     unsigned long long v = 0x7FF0000000000000ull;
-    if (i < iscal[IS_N]) {
+    if (i - P.base < iscal[sg * IS_COUNT + IS_N]) {
         const double4 u = vel[i];
         v = (unsigned long long)__double_as_longlong(d_dt_cfl(P, u.x, u.y, u.z));
     }
@@ -1633,8 +1718,15 @@ __global__ __launch_bounds__(256) void k_dtmin(DP P, const double4* __restrict__
 }
 
 // step prologue (time_step_loop.f90:180-186): dt_previous_step, dt, time
-__global__ void k_dt_begin(DP P, double* scal, unsigned long long* dtmin, int* iscal, int advance) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Segmented contexts: one lane per segment.
+template <bool SG>
+__global__ void k_dt_begin(DP P0, Seg S, double* scal, unsigned long long* dtmin, int* iscal, int advance) {
+    const int sg = SG ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : 0;
+    if (SG ? sg >= S.nseg : (threadIdx.x != 0 || blockIdx.x != 0)) return;
+    const DP& P = SG ? S.P[sg] : P0;
+    scal += sg * SC_COUNT;
+    iscal += sg * IS_COUNT;
+    dtmin += sg;
     if (advance) {
         iscal[IS_IT] += 1;
         iscal[IS_ONGOING] = iscal[IS_IT];
@@ -1902,22 +1994,27 @@ __device__ __forceinline__ void elastic_2d(const DP& P, const DFace* __restrict_
 // viscomon.f90:54-71, viscomorris.f90:60-71, AddElasticBoundaryReaction_3D.f90:
 // 67-98; fused Leapfrog_momentum.f90:66-86
 // ----------------------------------------------------------------------------
-template <bool F32, int CH>
+template <bool F32, int CH, bool SG>
 __global__ __launch_bounds__(256) void k_momentum(
-    DP P, const DFace* __restrict__ faces,
+    DP P0, Seg S, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
     const double4* __restrict__ vel, const double2* __restrict__ rp, F4In xmf,
     F4In velf, const float2* __restrict__ rpf, const int* __restrict__ lam_in,
     int* __restrict__ lam_out, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
     double4* __restrict__ acc_out, double4* __restrict__ vel_out, F4Out velf_out,
     const double* scal, int* iscal) {
+    const int vb = xcd_block(gridDim.x);
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(vb * (int)blockDim.x) >> 8]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    scal += sg * SC_COUNT;
+    iscal += sg * IS_COUNT;
     stencil_offsets(P);
     const double h = P.h;
     const double dtvel = scal[SC_DTVEL];
     const size_t N = P.cap;
     {
-        const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-        if (i < iscal[IS_N]) {
+        const int i = vb * blockDim.x + threadIdx.x;
+        if (i - P.base < iscal[IS_N]) {
             const double4 vi = vel[i];  // var == vel here
             const double2 rpi = rp[i];
             double tp[3] = {0.0, 0.0, 0.0}, td[3] = {0.0, 0.0, 0.0}, tv[3] = {0.0, 0.0, 0.0};
@@ -2081,9 +2178,9 @@ __global__ __launch_bounds__(256) void k_momentum(
 // Leapfrog_trajectories.f90:48-76 and the particle part of the next
 // time_step_duration.  Neighbour density travels in velf.w (FP32).
 // ----------------------------------------------------------------------------
-template <bool F32, int CH>
+template <bool F32, int CH, bool SG>
 __global__ __launch_bounds__(256) void k_vsmooth(
-    DP P, const DFace* __restrict__ faces,
+    DP P0, Seg S, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
     const double4* __restrict__ vel1, const double2* __restrict__ rp, F4In xmf,
     F4In velf1, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
@@ -2092,6 +2189,12 @@ __global__ __launch_bounds__(256) void k_vsmooth(
     using X = typename Rec<F32>::X;
     using V = typename Rec<F32>::V;
     using RT = typename std::conditional<F32, float, double>::type;
+    const int vb = xcd_block(gridDim.x);
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(vb * (int)blockDim.x) >> 8]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    scal += sg * SC_COUNT;
+    iscal += sg * IS_COUNT;
+    dtmin += sg;
     stencil_offsets(P);
     const size_t N = P.cap;
     const bool smooth = P.tetav > 1.0e-9;
@@ -2099,8 +2202,8 @@ __global__ __launch_bounds__(256) void k_vsmooth(
     const double dt = scal[SC_DT];
     unsigned long long dtbits = 0x7FF0000000000000ull;
     {
-        const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-        if (i < iscal[IS_N]) {
+        const int i = vb * blockDim.x + threadIdx.x;
+        if (i - P.base < iscal[IS_N]) {
             const double4 xi = xm[i];
             const double4 vi = vel1[i];
             double v[3] = {0.0, 0.0, 0.0};
@@ -2210,9 +2313,9 @@ __global__ __launch_bounds__(256) void k_vsmooth(
 // .f90:82-169 (FP64), Continuity_Equation.f90:97-235; fused
 // Leapfrog_continuity.f90:51-60 and EoS_fluid_loop.f90:59-70
 // ----------------------------------------------------------------------------
-template <bool F32, int CH>
+template <bool F32, int CH, bool SG>
 __global__ __launch_bounds__(256) void k_continuity(
-    DP P, const DFace* __restrict__ faces,
+    DP P0, Seg S, const DFace* __restrict__ faces,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
     const double4* __restrict__ vel, const double2* __restrict__ rp, F4In xmf,
     F4In velf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
@@ -2221,13 +2324,18 @@ __global__ __launch_bounds__(256) void k_continuity(
     using X = typename Rec<F32>::X;
     using V = typename Rec<F32>::V;
     using RT = typename std::conditional<F32, float, double>::type;
+    const int vb = xcd_block(gridDim.x);
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(vb * (int)blockDim.x) >> 8]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    scal += sg * SC_COUNT;
+    iscal += sg * IS_COUNT;
     stencil_offsets(P);
     const size_t N = P.cap;
     const double dt = scal[SC_DT];
     const bool eos = iscal[IS_ONGOING] > 0;
     {
-        const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-        if (i < iscal[IS_N]) {
+        const int i = vb * blockDim.x + threadIdx.x;
+        if (i - P.base < iscal[IS_N]) {
             const double4 vi = vel[i];
             const double2 rpi = rp[i];
             const double rhoi = rpi.x;
@@ -2340,21 +2448,26 @@ __global__ __launch_bounds__(256) void k_continuity(
 // ----------------------------------------------------------------------------
 // pressure smoothing: PressureSmoothing_3D.f90:80-161 (both loops)
 // ----------------------------------------------------------------------------
-template <bool F32, int CH>
+template <bool F32, int CH, bool SG>
 __global__ __launch_bounds__(256) void k_psmooth(
-    DP P, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
+    DP P0, Seg S, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
     const double4* __restrict__ xm, const double2* __restrict__ rp, const float4* __restrict__ xmf,
     const float2* __restrict__ rpf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
     double* __restrict__ vpres_out, double2* __restrict__ rp_out, float2* __restrict__ rpf_out,
     const double* scal, const int* iscal) {
     using X = typename Rec<F32>::X;
     using RPT = typename Rec<F32>::RP;
+    const int vb = xcd_block(gridDim.x);
+    const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(vb * (int)blockDim.x) >> 8]) : 0;
+    const DP& P = SG ? S.P[sg] : P0;
+    scal += sg * SC_COUNT;
+    iscal += sg * IS_COUNT;
     stencil_offsets(P);
     const size_t N = P.cap;
     const double tp1 = scal[SC_TP1];
     {
-        const int i = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-        if (i < iscal[IS_N]) {
+        const int i = vb * blockDim.x + threadIdx.x;
+        if (i - P.base < iscal[IS_N]) {
             const double2 rpi = rp[i];
             const int nsp = nnb[i];
             double diffp = 0.0;

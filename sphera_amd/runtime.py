@@ -30,6 +30,8 @@ EXPORTS = [
     "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
     "sph_preloop", "sph_step", "sph_step_async", "sph_step_graph", "sph_sync", "sph_get_launch", "sph_set_timing",
     "sph_get_timing", "sph_ctx_bytes", "sph_device_memory", "sph_query",
+    "sph_batch_create", "sph_batch_bytes", "sph_segment_upload", "sph_segment_download",
+    "sph_segment_scalars", "sph_segment_count",
     "sph_last_error",
 ]
 
@@ -101,6 +103,13 @@ def load():
     L.sph_ctx_bytes.argtypes = [C.POINTER(SphParams), C.POINTER(C.c_uint64)]
     L.sph_device_memory.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.sph_query.argtypes = [C.c_void_p, PI]
+    L.sph_batch_create.argtypes = [C.c_int32, C.POINTER(SphParams), C.POINTER(C.POINTER(SphFaces)),
+                                   C.POINTER(C.c_void_p)]
+    L.sph_batch_bytes.argtypes = [C.c_int32, C.POINTER(SphParams), C.POINTER(C.c_uint64)]
+    L.sph_segment_upload.argtypes = [C.c_void_p, C.c_int32, C.c_int32, PD, PD, PD, PD, PD]
+    L.sph_segment_download.argtypes = [C.c_void_p, C.c_int32, PD, PD, PD, PD, PD, PD, PD, PI]
+    L.sph_segment_scalars.argtypes = [C.c_void_p, C.c_int32, PD]
+    L.sph_segment_count.argtypes = [C.c_void_p, C.c_int32, PI]
     L.sph_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "sph_last_error":
@@ -199,6 +208,17 @@ def ctx_bytes(tb, precision=FP64, capacity=None, semi=True):
     rc = L.sph_ctx_bytes(C.byref(p), C.byref(v))
     if rc:
         raise SphError(rc, "sph_ctx_bytes", L.sph_last_error().decode())
+    return int(v.value)
+
+
+def batch_bytes(tbs, precision=FP32_MIXED, semi=True):
+    """Device bytes a batch context holding these cases would hold."""
+    L = load()
+    v = C.c_uint64()
+    ps = (SphParams * len(tbs))(*[params_struct(tb, precision, None, semi) for tb in tbs])
+    rc = L.sph_batch_bytes(len(tbs), ps, C.byref(v))
+    if rc:
+        raise SphError(rc, "sph_batch_bytes", L.sph_last_error().decode())
     return int(v.value)
 
 
@@ -369,3 +389,71 @@ class SphContext:
         k = self.L.sph_get_timing(self.h, names, 4096, _pd(ms), _pi(calls), 64)
         nm = names.value.decode().split(";")[:k]
         return {nm[i]: (float(ms[i]), int(calls[i])) for i in range(k)}
+
+
+class SphBatch(SphContext):
+    """Several independent 3D solves concatenated in one context (segments):
+    each kernel of a step is launched once over all of them.  Segment k is
+    case tbs[k]; state methods take the segment index."""
+
+    def __init__(self, tbs, precision=FP32_MIXED, semi=True, tune=None):
+        self.L = load()
+        self.tbs = list(tbs)
+        self.tb = self.tbs[0]
+        self.precision = precision
+        n = len(self.tbs)
+        self.params_all = (SphParams * n)(*[params_struct(tb, precision, None, semi, tune)
+                                            for tb in self.tbs])
+        self.params = self.params_all[0]
+        packed = [faces_struct(tb) for tb in self.tbs]
+        self._keep = packed
+        self._faces = [f for f, _ in packed]
+        fptrs = (C.POINTER(SphFaces) * n)(*[C.pointer(f) for f in self._faces])
+        h = C.c_void_p()
+        self._check(self.L.sph_batch_create(n, self.params_all, fptrs, C.byref(h)), "sph_batch_create")
+        self.h = h
+        self.nseg = n
+
+    def upload(self, coords=None):
+        """Uploads every segment (initial state of each case; coords[k]
+        replaces case k's positions when given)."""
+        for k, tb in enumerate(self.tbs):
+            P = tb.particles
+            c = P["coord"] if coords is None or coords[k] is None else coords[k]
+            a = [np.ascontiguousarray(v, np.float64)
+                 for v in (c, P["vel"], P["dens"], P["pres"], P["mass"])]
+            self._check(self.L.sph_segment_upload(self.h, k, len(a[0]), *[_pd(x) for x in a]),
+                        "sph_segment_upload")
+
+    def count(self, k):
+        v = np.zeros(1, np.int32)
+        self._check(self.L.sph_segment_count(self.h, k, _pi(v)), "sph_segment_count")
+        return int(v[0])
+
+    @property
+    def n(self):
+        return sum(self.count(k) for k in range(self.nseg))
+
+    def seg_scalars(self, k):
+        sc = np.zeros(5)
+        self._check(self.L.sph_segment_scalars(self.h, k, _pd(sc)), "sph_segment_scalars")
+        return dict(dt=sc[0], dt_average=sc[1], time=sc[2], it=int(sc[3]), pairs=int(sc[4]))
+
+    def seg_state(self, k):
+        n = self.count(k)
+        out = {q: np.zeros((n, 3)) for q in ("coord", "vel", "acc")}
+        out.update({q: np.zeros(n) for q in ("dens", "pres", "dden", "vpres")})
+        ids = np.zeros(n, np.int32)
+        self._check(self.L.sph_segment_download(self.h, k, *[_pd(out[q]) for q in (
+            "coord", "vel", "acc", "dens", "pres", "dden", "vpres")], _pi(ids)), "sph_segment_download")
+        out["ids"] = ids
+        out["var"] = out["vel"]
+        out.update(self.seg_scalars(k))
+        out["nag"] = n
+        return out
+
+    def state(self):
+        return self.seg_state(0)
+
+    def scalars(self):
+        return self.seg_scalars(0)

@@ -59,11 +59,17 @@ def _btab_close(d, o):
     dn, dside, dloc, dbi, _ = d.btab()
     on, oside, oloc, obi, _ = o.btab()
     assert np.array_equal(dn, on), "selected side counts differ"
-    m = np.arange(dside.shape[1])[None, :] < dn[:, None]
-    assert np.array_equal(dside[m], oside[:, :dside.shape[1]][m])
-    assert np.array_equal(dloc[m], oloc[:, :dside.shape[1]][m])
-    err = np.abs(dbi[m] - obi[:, :dside.shape[1]][m]).max(initial=0.0)
-    scale = np.abs(obi[:, :dside.shape[1]][m]).max(initial=1.0)
+    # the two tables may be read with different slot strides; compare the
+    # slots both hold (the selected count is at most 2 in 2D)
+    w = min(dside.shape[1], oside.shape[1])
+    assert int(dn.max(initial=0)) <= w
+    dside, dloc, dbi = dside[:, :w], dloc[:, :w], dbi[:, :w]
+    oside, oloc, obi = oside[:, :w], oloc[:, :w], obi[:, :w]
+    m = np.arange(w)[None, :] < dn[:, None]
+    assert np.array_equal(dside[m], oside[m])
+    assert np.array_equal(dloc[m], oloc[m])
+    err = np.abs(dbi[m] - obi[m]).max(initial=0.0)
+    scale = np.abs(obi[m]).max(initial=1.0)
     assert err <= 1e-13 * scale, f"boundary integrals differ by {err:.3e}"
     return int(dn.sum())
 
