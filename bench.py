@@ -191,6 +191,9 @@ def main():
     ap.add_argument("--ens-rate", type=float, default=200.0,
                     help="ensemble: Poisson arrival rate, requests per second")
     ap.add_argument("--ens-active", type=int, default=16, help="ensemble: concurrent requests")
+    ap.add_argument("--ens-batch", type=int, default=8,
+                    help="ensemble: requests per segmented batch (0 skips the batched run)")
+    ap.add_argument("--ens-batches", type=int, default=4, help="ensemble: concurrent segmented batches")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -253,16 +256,29 @@ def main():
         log(f"[bench] ensemble: {len(reqs)} requests x {args.ens_steps} steps, "
             f"{min(r.n for r in reqs)}-{max(r.n for r in reqs)} particles, rate {args.ens_rate}/s")
         prec = rt.FP32_MIXED if fp32 else rt.FP64
-        ens = serve.EnsembleServer(precision=prec, max_active=args.ens_active).run(reqs)
-        ens = {"requests": ens["requests"], "steps": args.ens_steps, "arrival_rate_per_s": args.ens_rate,
-               "max_active": args.ens_active,
-               "particles_min": min(ens["particles"]), "particles_max": max(ens["particles"]),
-               "p50_latency_s": ens["p50_latency_s"], "p99_latency_s": ens["p99_latency_s"],
-               "mean_latency_s": ens["mean_latency_s"], "p99_queue_s": ens["p99_queue_s"],
-               "makespan_s": ens["makespan_s"],
-               "pair_interactions_per_s": ens["pair_interactions_per_s"]}
+        def summary(out, **extra):
+            return dict(requests=out["requests"], steps=args.ens_steps, arrival_rate_per_s=args.ens_rate,
+                        **extra, particles_min=min(out["particles"]), particles_max=max(out["particles"]),
+                        p50_latency_s=out["p50_latency_s"], p99_latency_s=out["p99_latency_s"],
+                        mean_latency_s=out["mean_latency_s"], p99_queue_s=out["p99_queue_s"],
+                        makespan_s=out["makespan_s"], pair_interactions_per_s=out["pair_interactions_per_s"])
+
+        ens = summary(serve.EnsembleServer(precision=prec, max_active=args.ens_active).run(reqs),
+                      mode="one context per request", max_active=args.ens_active)
         log(f"[bench] ensemble: p50 {ens['p50_latency_s'] * 1e3:.1f} ms, p99 "
             f"{ens['p99_latency_s'] * 1e3:.1f} ms, {ens['pair_interactions_per_s'] / 1e9:.2f} G pairs/s")
+        if args.ens_batch > 0:
+            for r in reqs:
+                r.t_admit = r.t_done = float("nan")
+                r.pairs, r.state, r.ctx, r.done_steps = 0, None, None, 0
+            out = serve.BatchServer(precision=prec, batch=args.ens_batch,
+                                    max_batches=args.ens_batches).run(reqs)
+            ens["batched"] = summary(out, mode="segmented batches", batch=args.ens_batch,
+                                     max_batches=args.ens_batches)
+            b = ens["batched"]
+            log(f"[bench] ensemble (segmented batches of {args.ens_batch}): p50 "
+                f"{b['p50_latency_s'] * 1e3:.1f} ms, p99 {b['p99_latency_s'] * 1e3:.1f} ms, "
+                f"{b['pair_interactions_per_s'] / 1e9:.2f} G pairs/s")
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"[bench] cpu baseline (up to {args.cpu_steps} steps / {args.cpu_seconds} s)")

@@ -19,6 +19,14 @@ requests overlap on the GPU:
   end (releasing device memory synchronises the device, which would stall the
   streams of the running requests).
 
+With batch > 1 (BatchServer) admitted requests are instead grouped into
+segmented batches: up to `batch` waiting requests of equal step count join one
+batch context (sph_batch_create), whose every kernel is launched once over all
+of its segments, and whose step is one replayed HIP graph.  Several batches
+run at once on their own streams, and a batch is formed from the requests
+waiting whenever a batch slot and memory are free, so requests still join
+while others run.
+
 Latency of a request = completion time - arrival time (time to solution);
 the run reports p50/p99 latency and the aggregate pair-interactions/s.
 """
@@ -187,3 +195,115 @@ def run_alone(req, precision=rt.FP32_MIXED):
         return d.state()
     finally:
         d.close()
+
+
+@dataclass
+class _Batch:
+    reqs: list
+    ctx: object
+    steps: int
+    done_steps: int = 0
+
+
+class BatchServer:
+    """Serves requests in segmented batches (one context per batch of up to
+    `batch` requests); at most `max_batches` batches run at once."""
+
+    def __init__(self, precision=rt.FP32_MIXED, batch=8, max_batches=4, chunk=25, reserve_frac=0.05,
+                 graphs=True):
+        self.precision = precision
+        self.batch = batch
+        self.max_batches = max_batches
+        self.chunk = chunk
+        self.reserve_frac = reserve_frac
+        self.graphs = graphs
+
+    def _form(self, waiting, parked):
+        """Takes the next batch from the head of `waiting`: up to `batch`
+        requests with the head's step count.  If it does not fit the free
+        memory, finished batches are released first, then the batch shrinks.
+        Returns the list, or [] when not even one request fits."""
+        head = waiting[0]
+        group = [r for r in list(waiting)[: 4 * self.batch] if r.steps == head.steps][: self.batch]
+
+        def fits(g):
+            free, total = rt.device_memory()
+            return free - rt.batch_bytes([r.tb for r in g], self.precision) >= self.reserve_frac * total
+
+        if not fits(group) and parked:
+            for b in parked:
+                b.ctx.close()
+            parked.clear()
+        while group and not fits(group):
+            group = group[:-1]
+        return group
+
+    def run(self, requests):
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = deque()
+        active, finished, parked = [], [], []
+        pairs_total = 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        while pending or waiting or active:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.popleft())
+            while waiting and len(active) < self.max_batches:
+                group = self._form(waiting, parked)
+                if not group:
+                    if not active:
+                        raise MemoryError(f"request {waiting[0].rid} does not fit the free device memory")
+                    break
+                for r in group:
+                    waiting.remove(r)
+                    r.t_admit = now()
+                ctx = rt.SphBatch([r.tb for r in group], precision=self.precision)
+                ctx.upload()
+                ctx.preloop()
+                active.append(_Batch(group, ctx, group[0].steps))
+            progressed = False
+            for b in list(active):
+                if b.ctx.busy():
+                    continue
+                progressed = True
+                if b.done_steps < b.steps:
+                    k = min(self.chunk, b.steps - b.done_steps)
+                    if self.graphs:
+                        for _ in range(k):
+                            b.ctx.step_graph(1)
+                    else:
+                        b.ctx.step_async(k)
+                    b.done_steps += k
+                else:
+                    b.ctx.sync()
+                    pairs_total += b.ctx.pair_counter()
+                    tf = now()
+                    for k, r in enumerate(b.reqs):
+                        if r.keep_state:
+                            r.state = b.ctx.seg_state(k)
+                        r.t_done = tf
+                        finished.append(r)
+                    active.remove(b)
+                    parked.append(b)
+            if not progressed:
+                time.sleep(20e-6)
+        makespan = now()
+        for b in parked:
+            b.ctx.close()
+        lat = np.array([r.t_done - r.arrival for r in finished])
+        wait = np.array([r.t_admit - r.arrival for r in finished])
+        return {
+            "requests": len(finished),
+            "batch": self.batch,
+            "max_batches": self.max_batches,
+            "makespan_s": makespan,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()),
+            "p50_queue_s": float(np.percentile(wait, 50)),
+            "p99_queue_s": float(np.percentile(wait, 99)),
+            "pair_interactions": int(pairs_total),
+            "pair_interactions_per_s": pairs_total / makespan if makespan > 0 else 0.0,
+            "particles": [int(r.n) for r in finished],
+        }

@@ -124,3 +124,70 @@ def test_ensemble_cases_span_sizes():
     assert min(ns) > 0 and max(ns) > 4 * min(ns)
     arr = [r.arrival for r in reqs]
     assert arr[0] == 0.0 and all(np.diff(arr) >= 0)
+
+
+class FakeBatch(FakeCtx):
+    """A batch context over several requests' cases (one pool charge)."""
+
+    def __init__(self, tbs, precision=None):
+        self.tbs = list(tbs)
+        self.need = sum(FakeCtx.bytes_for(tb) for tb in self.tbs)
+        FakeCtx.pool["free"] -= self.need
+        self.pending = 0
+        self.steps = 0
+        FakeCtx.log.append(("batch", tuple(tb.rid for tb in self.tbs)))
+
+    def pair_counter(self):
+        return self.steps * sum(tb.n for tb in self.tbs)
+
+    def seg_state(self, k):
+        return {"rid": self.tbs[k].rid}
+
+    def close(self):
+        FakeCtx.pool["free"] += self.need
+        FakeCtx.log.append(("close", tuple(tb.rid for tb in self.tbs)))
+
+
+@pytest.fixture
+def fake_batches(fake_device, monkeypatch):
+    monkeypatch.setattr(serve.rt, "SphBatch", FakeBatch)
+    monkeypatch.setattr(serve.rt, "batch_bytes",
+                        lambda tbs, prec: sum(FakeCtx.bytes_for(tb) for tb in tbs))
+    return fake_device
+
+
+def test_batches_group_fifo_and_equal_steps(fake_batches):
+    reqs = _reqs([1000, 2000, 3000, 4000, 5000])
+    reqs[2].steps = 7  # a different step count: a batch of its own, in its FIFO turn
+    for r in reqs:
+        r.keep_state = True
+    out = serve.BatchServer(batch=2, max_batches=1, chunk=3, reserve_frac=0.0).run(reqs)
+    assert out["requests"] == 5
+    batches = [ids for ev, ids in fake_batches.log if ev == "batch"]
+    assert batches == [(0, 1), (2,), (3, 4)]
+    assert out["pair_interactions"] == sum(r.steps * r.n for r in reqs)
+    assert all(r.state == {"rid": r.rid} for r in reqs)
+    assert fake_batches.pool["free"] == 10_000_000
+
+
+def test_batch_shrinks_to_fit_memory(fake_batches):
+    # 10 MB device; 3 x 4 MB requests do not fit one batch, two do; the rest
+    # waits (FIFO) until the first batch is released
+    reqs = _reqs([4000, 4000, 4000, 1000])
+    serve.BatchServer(batch=3, max_batches=2, chunk=5, reserve_frac=0.0).run(reqs)
+    batches = [ids for ev, ids in fake_batches.log if ev == "batch"]
+    assert batches == [(0, 1), (2, 3)]
+    assert reqs[2].t_admit >= reqs[0].t_done
+    assert fake_batches.pool["free"] == 10_000_000
+
+
+def test_batches_run_at_once(fake_batches):
+    """A later batch starts while an earlier one still runs (continuous
+    batching across batches)."""
+    reqs = _reqs([1000, 1000, 1000])
+    reqs[0].steps = 60
+    reqs[1].steps = reqs[2].steps = 3
+    serve.BatchServer(batch=2, max_batches=2, chunk=2, reserve_frac=0.0).run(reqs)
+    batches = [ids for ev, ids in fake_batches.log if ev == "batch"]
+    assert batches == [(0,), (1, 2)]
+    assert reqs[1].t_admit < reqs[0].t_done and reqs[1].t_done < reqs[0].t_done
