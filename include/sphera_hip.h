@@ -53,6 +53,9 @@ typedef struct sph_params {
                             momentum pass, and the others where theirs is 0 */
     int32_t chunk_v, chunk_c, chunk_p; /* the same for velocity smoothing, continuity,
                                           pressure smoothing (0 = chunk) */
+    int32_t nb_shape;    /* neighbour-search workgroup size x LDS candidate tile:
+                            0 = 256 x 3072 (default), 1 = 128 x 1536, 2 = 256 x 4096,
+                            3 = 256 x 4096 with the tile in FP16 (lists unchanged) */
 } sph_params;
 
 /* SA-SPH frontier faces in BFaceList order and the grid-cell/face incidence

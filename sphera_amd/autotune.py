@@ -6,13 +6,21 @@ entries gathered per step: fewer registers, more waves in flight, or more
 loads in flight per lane).  For each workgroup size two scratch contexts run
 a few steps with every pass at chunk 4 and at chunk 8, with per-kernel HIP
 event timing; each pass takes its faster chunk, and the workgroup size with
-the smallest resulting step wins.  The result is the ``tune`` dict that
+the smallest resulting step wins.  The neighbour search's launch shape
+(workgroup size x LDS candidate tile, ``nb_shape``) is then chosen the same
+way, from its kernel time in each shape.  The result is the ``tune`` dict that
 ``SphContext`` / ``sph_params`` accept.
+
+The grid's cell size is not a tunable: it is the reference's (CreaGrid), and
+it fixes the neighbour-list order the device reproduces.
 """
 from __future__ import annotations
 
 import time
 
+# neighbour-search launch shapes (workgroup size, LDS tile): 256 x 3072,
+# 128 x 1536, 256 x 4096, 256 x 4096 with an FP16 tile
+NB_SHAPES = (0, 1, 2, 3)
 PASSES = {"momentum": "chunk", "velocity_smoothing": "chunk_v", "continuity": "chunk_c",
           "pressure_smoothing": "chunk_p"}
 
@@ -60,4 +68,12 @@ def autotune(tb, coord, precision, steps=4, warmup=2, log=None):
             log(f"[autotune] precision={precision} {tune} -> {est * 1e3:.3f} ms/step")
         if est < best_s:
             best, best_s = tune, est
+    nb_ms = {}
+    for shape in NB_SHAPES:
+        t = dict(best, nb_shape=shape)
+        _, ms = _run(tb, coord, precision, t, steps, warmup)
+        nb_ms[shape] = ms.get("neighbours", float("inf"))
+        if log:
+            log(f"[autotune] nb_shape={shape}: neighbours {nb_ms[shape]:.4f} ms")
+    best = dict(best, nb_shape=min(nb_ms, key=nb_ms.get))
     return best, results

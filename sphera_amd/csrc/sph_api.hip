@@ -77,6 +77,7 @@ struct sph_ctx {
     int* rowblk = nullptr;                  // first neighbour-search block of each cell row
     int *idslot = nullptr, *rmids = nullptr;  // id -> slot, ids removed by a sort
     int nb_grid = 8;                        // persistent workgroups of the neighbour search
+    int nb_shape = 0;                       // its launch shape (index into NB_SHAPES)
     int wg = 256, grid = 1, bt_grid = 1024;  // launch shape of the particle kernels
     int chunk = 8;                            // list entries gathered per step of the FP32 walks
     int chunk_v = 8, chunk_c = 8, chunk_p = 8;  // the same for smoothing, continuity, pressure smoothing
@@ -227,11 +228,29 @@ int run_sort(sph_ctx* c) {
     return 0;
 }
 
+// the tiled search in launch shape `shape` of NB_SHAPES
+#define NB_KERNEL(SG, k) k_neighbours<SG, NB_SHAPES[k].wg, NB_SHAPES[k].tile, NB_SHAPES[k].h16>
+
+template <bool SG>
+void launch_nb(sph_ctx* c) {
+    const int wg = NB_SHAPES[c->nb_shape].wg;
+    auto go = [&](auto kern) {
+        kern<<<c->nb_grid, wg, 0, c->stream>>>(c->P, c->S, c->xm[0], c->xmf, c->cell, c->start, c->rowblk,
+                                               c->nbr, c->nnb, c->iscal, c->pairs);
+    };
+    switch (c->nb_shape) {
+        case 1: go(NB_KERNEL(SG, 1)); break;
+        case 2: go(NB_KERNEL(SG, 2)); break;
+        case 3: go(NB_KERNEL(SG, 3)); break;
+        default: go(NB_KERNEL(SG, 0)); break;
+    }
+}
+
 int run_neighbours(sph_ctx* c) {
     KTime t(c, "neighbours");
-    SEGL(k_nb_rowblocks, 1, 1024, c->start, c->rowblk);
-    SEGL(k_neighbours, c->nb_grid, NB_WG, c->xm[0], c->xmf, c->cell, c->start, c->rowblk, c->nbr, c->nnb,
-         c->iscal, c->pairs);
+    SEGL(k_nb_rowblocks, 1, 1024, c->start, c->rowblk, NB_SHAPES[c->nb_shape].wg);
+    if (c->nseg > 1) launch_nb<true>(c);
+    else launch_nb<false>(c);
     return 0;
 }
 
@@ -543,10 +562,22 @@ int setup_launch(sph_ctx* c, const sph_params* p) {
     c->bt_grid = 4 * ncu;
     // persistent neighbour-search workgroups: as many as fit a CU (LDS
     // tile), times the CUs, a multiple of the 8 XCDs
+    c->nb_shape = p->nb_shape;
+    if (c->nb_shape < 0 || c->nb_shape >= NB_NSHAPES) return fail(2, "nb_shape must be 0, 1, 2 or 3");
+    if (NB_SHAPES[c->nb_shape].h16)  // FP16 keeps its relative precision for cells of 1e-4..1e4 m
+        for (const auto& Q : c->segs)
+            for (int d = 0; d < 3; ++d)
+                if (!(Q.dcd[d] >= 1.0e-4 && Q.dcd[d] <= 1.0e4)) c->nb_shape = 2;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_neighbours<false>, NB_WG, 0) != hipSuccess ||
-        per_cu <= 0)
-        per_cu = 3;
+    hipError_t oe;
+    const int nwg = NB_SHAPES[c->nb_shape].wg;
+    switch (c->nb_shape) {
+        case 1: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 1), nwg, 0); break;
+        case 2: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 2), nwg, 0); break;
+        case 3: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 3), nwg, 0); break;
+        default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 0), nwg, 0); break;
+    }
+    if (oe != hipSuccess || per_cu <= 0) per_cu = 3;
     c->nb_grid = (per_cu * ncu + 7) / 8 * 8;
     return 0;
 }

@@ -17,6 +17,7 @@
 // FP64 kernels are compiled with -ffp-contract=off and evaluate every
 // expression in the order of the cited reference line, so sums over a list in
 // the reference's order reproduce the reference's values.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -611,16 +612,24 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
 // order out of LDS.  A block over more than NB_MAXC cells or with more than
 // NB_TILE candidates takes the global path above.  Persistent workgroups take
 // the blocks of one contiguous range per XCD.
+// Launch shapes of the tiled search (workgroup size, candidate tile, FP16
+// tile): the default 256 x 3072 FP32 (36 KB of LDS, 4 workgroups per CU) and
+// three the autotuner may pick instead.
 constexpr int NB_WG = 256;
 constexpr int NB_MAXC = 48;
 constexpr int NB_TILE = 3072;
+struct NbShape { int wg, tile; bool h16; };
+constexpr int NB_NSHAPES = 4;
+constexpr NbShape NB_SHAPES[NB_NSHAPES] = {
+    {256, 3072, false}, {128, 1536, false}, {256, 4096, false}, {256, 4096, true}};
 
-// rowblk[r] = first block of row r (exclusive scan of ceil(count_r / NB_WG)),
+// rowblk[r] = first block of row r (exclusive scan of ceil(count_r / bw), bw
+// the search's workgroup size),
 // rowblk[R] = number of blocks.  One workgroup.  Segmented contexts: the rows
 // of all segments, each row read with its segment's grid.
 template <bool SG>
 __global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P0, Seg S, const int* __restrict__ start,
-                                                       int* __restrict__ rowblk) {
+                                                       int* __restrict__ rowblk, int bw) {
     __shared__ int wsum[16];
     __shared__ int carry;
     const int R = SG ? S.nrows : P0.ncd[1] * P0.ncd[2];
@@ -634,7 +643,7 @@ __global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P0, Seg S, const int* 
             const int ni = P.ncd[0];
             const int rl = r - P.roff;
             const int k0 = P.koff + rl * ni;
-            v = (start[k0 + ni] - start[k0] + NB_WG - 1) / NB_WG;
+            v = (start[k0 + ni] - start[k0] + bw - 1) / bw;
         }
         const int incl = wave_incl_scan(v);
         const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -651,16 +660,23 @@ __global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P0, Seg S, const int* 
     if (threadIdx.x == 0) rowblk[R] = carry;
 }
 
-template <bool SG>
-__global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double4* __restrict__ xm,
+// H16: the tile holds the candidates' cell-relative positions in FP16 (6
+// bytes each: more workgroups per CU).  A coordinate in [0, dcd] rounds by at
+// most 2^-11 dcd, so the FP32 r^2 from it is within ~2e-3 relative of the
+// exact one near (2h)^2; the FP64 guard band is widened to 5e-3 to keep the
+// lists exactly the reference's.
+template <bool SG, int WG = NB_WG, int TILE = NB_TILE, bool H16 = false>
+__global__ __launch_bounds__(WG) void k_neighbours(DP P0, Seg S, const double4* __restrict__ xm,
                                                       const float4* __restrict__ xmf,
                                                       const int* __restrict__ cell,
                                                       const int* __restrict__ start,
                                                       const int* __restrict__ rowblk,
                                                       uint32_t* __restrict__ nbr, int* __restrict__ nnb,
                                                       int* iscal, unsigned long long* pairs) {
-    __shared__ float2 tile_xy[NB_TILE];  // staged candidates: (x, y) and z, 12 bytes each
-    __shared__ float tile_z[NB_TILE];
+    using TXY = typename std::conditional<H16, __half2, float2>::type;
+    using TZ = typename std::conditional<H16, __half, float>::type;
+    __shared__ TXY tile_xy[TILE];  // staged candidates: (x, y) and z, 12 (FP16: 6) bytes each
+    __shared__ TZ tile_z[TILE];
     __shared__ int bnd[9][NB_MAXC + 4];  // LDS offset of the start of cell ia-1+u of row t
     __shared__ int seg_g[9];             // sorted index of a segment's first particle
     __shared__ int seg_l[9];             // its LDS offset
@@ -688,8 +704,8 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double
             const DP& Q = SG ? S.P[sg0] : P0;
             const int rl = r - Q.roff;  // the row in its segment's grid
             const int qi = Q.ncd[0];
-            const int p0 = start[Q.koff + rl * qi] + (blk - rowblk[r]) * NB_WG;
-            const int p1 = min(p0 + NB_WG, start[Q.koff + (rl + 1) * qi]);
+            const int p0 = start[Q.koff + rl * qi] + (blk - rowblk[r]) * WG;
+            const int p1 = min(p0 + WG, start[Q.koff + (rl + 1) * qi]);
             const int ia = cell[p0] - rl * qi, ib = cell[p1 - 1] - rl * qi;
             s_seg = sg0;
             s_row = rl;
@@ -709,6 +725,8 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double
         const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
         const float sq = (float)P.square_doubleh;
         const float lo = sq * (1.0f - 4.0e-5f), hi = sq * (1.0f + 4.0e-5f);
+        const float band = H16 ? 5.0e-3f : 4.0e-5f;  // guard band of the tiled test
+        const float tlo = sq * (1.0f - band), thi = sq * (1.0f + band);
         const int mj = P.nmaxpartj;
         const size_t N = P.cap;
         const int r = s_row, ia = s_ia, ib = s_ib, p0 = s_p0, p1 = s_p1;
@@ -734,7 +752,7 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double
                     seg_l[t] = off;
                     off += seg_n[t];
                 }
-                s_global = off > NB_TILE;
+                s_global = off > TILE;
             }
             __syncthreads();
         }
@@ -743,7 +761,7 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double
             // cell boundaries inside the tile: u = 0 .. ib-ia+3 is the start of cell
             // ia-1+u (cells ia-1 .. ib+1, and the end of ib+1)
             const int nu = ib - ia + 4;
-            for (int x = threadIdx.x; x < 9 * nu; x += NB_WG) {
+            for (int x = threadIdx.x; x < 9 * nu; x += WG) {
                 const int t = x / nu, u = x % nu;
                 const int jj = jc + t / 3 - 1, kk = kc + t % 3 - 1;
                 const int ii = ia - 1 + u;
@@ -756,12 +774,12 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double
             // stage the candidates' FP32 positions, four loads in flight per thread
             {
                 const int total = seg_l[8] + seg_n[8];
-                for (int e0 = threadIdx.x; e0 < total; e0 += 4 * NB_WG) {
+                for (int e0 = threadIdx.x; e0 < total; e0 += 4 * WG) {
                     float4 v[4];
                     int dst[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const int e = e0 + u * NB_WG;
+                        const int e = e0 + u * WG;
                         dst[u] = -1;
                         if (e < total) {
                             int t = 0;
@@ -774,8 +792,13 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
                         if (dst[u] >= 0) {
-                            tile_xy[dst[u]] = make_float2(v[u].x, v[u].y);
-                            tile_z[dst[u]] = v[u].z;
+                            if constexpr (H16) {
+                                tile_xy[dst[u]] = __floats2half2_rn(v[u].x, v[u].y);
+                                tile_z[dst[u]] = __float2half_rn(v[u].z);
+                            } else {
+                                tile_xy[dst[u]] = make_float2(v[u].x, v[u].y);
+                                tile_z[dst[u]] = v[u].z;
+                            }
                         }
                 }
             }
@@ -786,8 +809,13 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double
                 const double4 xi64 = xm[p];
                 NbOut w(nbr, N, p);
                 auto load = [&](int x) {
-                    const float2 a = tile_xy[x];
-                    return make_float4(a.x, a.y, tile_z[x], 0.0f);
+                    if constexpr (H16) {
+                        const float2 a = __half22float2(tile_xy[x]);
+                        return make_float4(a.x, a.y, __half2float(tile_z[x]), 0.0f);
+                    } else {
+                        const float2 a = tile_xy[x];
+                        return make_float4(a.x, a.y, tile_z[x], 0.0f);
+                    }
                 };
                 for (int dy = 0; dy < 3; ++dy)
                     for (int dx = 0; dx < 3; ++dx)
@@ -796,7 +824,7 @@ __global__ __launch_bounds__(NB_WG) void k_neighbours(DP P0, Seg S, const double
                             const int code = dx + 3 * dy + 9 * dz;
                             nb_cell(P, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT,
                                     bnd[t][ucell + dx - 1], bnd[t][ucell + dx], seg_g[t] - seg_l[t], load,
-                                    lo, hi, w, mj);
+                                    tlo, thi, w, mj);
                         }
                 const bool ov = w.cnt > mj;
                 const int cnt = min(w.cnt, mj);

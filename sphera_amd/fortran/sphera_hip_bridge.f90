@@ -115,6 +115,7 @@ p%chunk = 0
 p%chunk_v = 0
 p%chunk_c = 0
 p%chunk_p = 0
+p%nb_shape = 0
 end subroutine pack_params
 
 #ifdef SPACE_2D

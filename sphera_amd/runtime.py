@@ -50,7 +50,7 @@ class SphParams(C.Structure):
                 ("vsc_coeff", C.c_double), ("pesodt", C.c_double), ("tetap", C.c_double),
                 ("tetav", C.c_double), ("semi", C.c_int32), ("wg_size", C.c_int32),
                 ("chunk", C.c_int32), ("chunk_v", C.c_int32), ("chunk_c", C.c_int32),
-                ("chunk_p", C.c_int32)]
+                ("chunk_p", C.c_int32), ("nb_shape", C.c_int32)]
 
 
 class SphFaces(C.Structure):

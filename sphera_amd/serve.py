@@ -307,3 +307,57 @@ class BatchServer:
             "pair_interactions_per_s": pairs_total / makespan if makespan > 0 else 0.0,
             "particles": [int(r.n) for r in finished],
         }
+
+
+def _reset(reqs):
+    for r in reqs:
+        r.t_admit = r.t_done = float("nan")
+        r.pairs, r.state, r.ctx, r.done_steps = 0, None, None, 0
+
+
+def main(argv=None):
+    """Ensemble sweep: the same seeded requests served at each arrival rate,
+    in segmented batches and/or one context per request; one JSON line per
+    run on stdout, progress on stderr."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--rates", default="4,8,16",
+                    help="Poisson arrival rates, requests per second (0: all at once)")
+    ap.add_argument("--modes", default="batched,contexts")
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--max-batches", type=int, default=4)
+    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--precision", choices=("fp32_mixed", "fp64"), default="fp32_mixed")
+    a = ap.parse_args(argv)
+    prec = rt.FP32_MIXED if a.precision == "fp32_mixed" else rt.FP64
+    t = time.perf_counter()
+    base = make_ensemble(a.requests, a.steps, 1.0)
+    print(f"[serve] {a.requests} cases built in {time.perf_counter() - t:.1f} s, "
+          f"{min(r.n for r in base)}-{max(r.n for r in base)} particles", file=sys.stderr, flush=True)
+    unit_gaps = np.diff([r.arrival for r in base], prepend=0.0)  # exponential(1) gaps
+    for rate in [float(x) for x in a.rates.split(",")]:
+        for mode in a.modes.split(","):
+            _reset(base)
+            t_arr = np.cumsum(unit_gaps / rate) if rate > 0 else np.zeros(len(base))
+            for r, ta in zip(base, t_arr):
+                r.arrival = float(ta)
+            if mode == "batched":
+                srv = BatchServer(precision=prec, batch=a.batch, max_batches=a.max_batches)
+            else:
+                srv = EnsembleServer(precision=prec, max_active=a.max_active)
+            out = srv.run(base)
+            ps = out.pop("particles")
+            line = dict(mode=mode, rate_per_s=rate, steps=a.steps, precision=a.precision,
+                        particles_min=min(ps), particles_max=max(ps), **out)
+            print(f"[serve] {mode} rate {rate}/s: p50 {out['p50_latency_s']:.2f} s, p99 "
+                  f"{out['p99_latency_s']:.2f} s, {out['pair_interactions_per_s'] / 1e9:.2f} G pairs/s",
+                  file=sys.stderr, flush=True)
+            print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -193,6 +193,13 @@ def test_lists_match_oracle_on_clouds(kind):
         d.preloop()
         _lists_equal(d, o)
         d.close()
+    # the other launch shapes of the tiled search (workgroup x LDS tile)
+    for shape in (1, 2, 3):
+        d = rt.SphContext(tb, precision=rt.FP32_MIXED, capacity=n, tune=dict(nb_shape=shape))
+        d.upload(coord=coord, **kw)
+        d.preloop()
+        _lists_equal(d, o)
+        d.close()
 
 
 def test_removal_renumbering_matches_oracle(tiny_case):
@@ -317,3 +324,31 @@ def _device_btab_at_start(tb, coord):
         return d.btab()
     finally:
         d.close()
+
+
+def test_full_size_dam_break_matches_oracle():
+    """The bench workload itself (576,000 particles, jittered lattice, seed
+    20260101): after 5 steps the FP64 device fields, dt and pair count equal
+    the oracle's (FP64_REL_TOL), and the FP32-mixed fields are within
+    FP32_TOL of them."""
+    from sphera_amd.case import DamBreak3D
+    tb = build(DamBreak3D(tank=(1.6, 0.6, 0.6), column=(0.4, 0.6, 0.3), dx=0.005, name="dam_break_3D"))
+    coord = _jittered(tb)
+    print(f"[gpu] full-size n={len(coord)}", flush=True)
+    o = _oracle(tb, coord)
+    o.step(5)
+    os_ = o.state()
+    rt = _rt()
+    for prec in (rt.FP64, rt.FP32_MIXED):
+        d = _device(tb, prec, coord)
+        d.step(5)
+        ds = d.state()
+        d.close()
+        assert ds["nag"] == len(coord)
+        assert ds["pairs"] == o.pair_count()
+        for f in FIELDS:
+            r = _rel(ds[f], os_[f])
+            tol = FP64_REL_TOL if prec == rt.FP64 else FP32_TOL[f]
+            assert r <= tol, f"precision {prec}: {f} rel diff {r:.3e}"
+        if prec == rt.FP64:
+            assert ds["dt"] == os_["dt"]
