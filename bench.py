@@ -183,6 +183,8 @@ def main():
     ap.add_argument("--wg", type=int, default=0,
                     help="fixed workgroup size of the particle kernels (skips the autotuner)")
     ap.add_argument("--chunk", type=int, default=0, help="fixed FP32 list-walk chunk (4 or 8)")
+    ap.add_argument("--nb-shape", type=int, default=0,
+                    help="fixed neighbour-search launch shape (sph_params.nb_shape), with --wg")
     ap.add_argument("--only", choices=("fp32_mixed", "fp64"), default=None,
                     help="time one precision only (profiling runs)")
     ap.add_argument("--ens-requests", type=int, default=64,
@@ -211,7 +213,7 @@ def main():
     for name, prec in (("fp32_mixed", rt.FP32_MIXED), ("fp64", rt.FP64)):
         if args.only and name != args.only:
             continue
-        tune = dict(wg_size=args.wg, chunk=args.chunk) if args.wg else None
+        tune = dict(wg_size=args.wg, chunk=args.chunk, nb_shape=args.nb_shape) if args.wg else None
         if not args.no_autotune and not args.wg:
             from sphera_amd.autotune import autotune
             tune, _ = autotune(tb, coord, prec, log=log)

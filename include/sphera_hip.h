@@ -54,8 +54,10 @@ typedef struct sph_params {
     int32_t chunk_v, chunk_c, chunk_p; /* the same for velocity smoothing, continuity,
                                           pressure smoothing (0 = chunk) */
     int32_t nb_shape;    /* neighbour-search workgroup size x LDS candidate tile:
-                            0 = 256 x 3072 (default), 1 = 128 x 1536, 2 = 256 x 4096,
-                            3 = 256 x 4096 with the tile in FP16 (lists unchanged) */
+                            0 = 256 x 4096 in FP16 (default), 1 = 256 x 3072 in FP32,
+                            2 = 128 x 1536 in FP32, 3 = 256 x 2816 in FP16 with 64
+                            registers (8 waves per SIMD).  The lists are the same
+                            in every shape (FP64 decides near the cut-off). */
 } sph_params;
 
 /* SA-SPH frontier faces in BFaceList order and the grid-cell/face incidence

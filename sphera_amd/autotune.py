@@ -18,8 +18,8 @@ from __future__ import annotations
 
 import time
 
-# neighbour-search launch shapes (workgroup size, LDS tile): 256 x 3072,
-# 128 x 1536, 256 x 4096, 256 x 4096 with an FP16 tile
+# neighbour-search launch shapes (workgroup size x LDS tile): 256 x 4096 FP16,
+# 256 x 3072 FP32, 128 x 1536 FP32, 256 x 2816 FP16 held to 64 registers
 NB_SHAPES = (0, 1, 2, 3)
 PASSES = {"momentum": "chunk", "velocity_smoothing": "chunk_v", "continuity": "chunk_c",
           "pressure_smoothing": "chunk_p"}

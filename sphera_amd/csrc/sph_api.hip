@@ -229,7 +229,8 @@ int run_sort(sph_ctx* c) {
 }
 
 // the tiled search in launch shape `shape` of NB_SHAPES
-#define NB_KERNEL(SG, k) k_neighbours<SG, NB_SHAPES[k].wg, NB_SHAPES[k].tile, NB_SHAPES[k].h16>
+#define NB_KERNEL(SG, k) \
+    k_neighbours<SG, NB_SHAPES[k].wg, NB_SHAPES[k].tile, NB_SHAPES[k].h16, NB_SHAPES[k].minb>
 
 template <bool SG>
 void launch_nb(sph_ctx* c) {
@@ -567,7 +568,7 @@ int setup_launch(sph_ctx* c, const sph_params* p) {
     if (NB_SHAPES[c->nb_shape].h16)  // FP16 keeps its relative precision for cells of 1e-4..1e4 m
         for (const auto& Q : c->segs)
             for (int d = 0; d < 3; ++d)
-                if (!(Q.dcd[d] >= 1.0e-4 && Q.dcd[d] <= 1.0e4)) c->nb_shape = 2;
+                if (!(Q.dcd[d] >= 1.0e-4 && Q.dcd[d] <= 1.0e4)) c->nb_shape = 1;
     int per_cu = 0;
     hipError_t oe;
     const int nwg = NB_SHAPES[c->nb_shape].wg;

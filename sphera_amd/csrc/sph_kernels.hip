@@ -613,15 +613,17 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
 // NB_TILE candidates takes the global path above.  Persistent workgroups take
 // the blocks of one contiguous range per XCD.
 // Launch shapes of the tiled search (workgroup size, candidate tile, FP16
-// tile): the default 256 x 3072 FP32 (36 KB of LDS, 4 workgroups per CU) and
-// three the autotuner may pick instead.
+// tile, register bound): the default 256 x 4096 FP16 (26 KB of LDS, 6 waves
+// per SIMD) and three the autotuner may pick instead: 256 x 3072 FP32 (36 KB,
+// 4 waves), 128 x 1536 FP32, and 256 x 2816 FP16 held to 64 registers for 8
+// waves per SIMD.
 constexpr int NB_WG = 256;
 constexpr int NB_MAXC = 48;
 constexpr int NB_TILE = 3072;
-struct NbShape { int wg, tile; bool h16; };
+struct NbShape { int wg, tile; bool h16; int minb; };  // minb: workgroups per CU the registers must allow
 constexpr int NB_NSHAPES = 4;
 constexpr NbShape NB_SHAPES[NB_NSHAPES] = {
-    {256, 3072, false}, {128, 1536, false}, {256, 4096, false}, {256, 4096, true}};
+    {256, 4096, true, 1}, {256, 3072, false, 1}, {128, 1536, false, 1}, {256, 2816, true, 8}};
 
 // rowblk[r] = first block of row r (exclusive scan of ceil(count_r / bw), bw
 // the search's workgroup size),
@@ -665,8 +667,8 @@ __global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P0, Seg S, const int* 
 // most 2^-11 dcd, so the FP32 r^2 from it is within ~2e-3 relative of the
 // exact one near (2h)^2; the FP64 guard band is widened to 5e-3 to keep the
 // lists exactly the reference's.
-template <bool SG, int WG = NB_WG, int TILE = NB_TILE, bool H16 = false>
-__global__ __launch_bounds__(WG) void k_neighbours(DP P0, Seg S, const double4* __restrict__ xm,
+template <bool SG, int WG = NB_WG, int TILE = NB_TILE, bool H16 = false, int MINB = 1>
+__global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const double4* __restrict__ xm,
                                                       const float4* __restrict__ xmf,
                                                       const int* __restrict__ cell,
                                                       const int* __restrict__ start,
