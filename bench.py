@@ -152,6 +152,35 @@ def cpu_baseline(tb, coord, max_steps, budget_s):
                        f"{steps} whole time steps after 1 untimed ({(t1 - t0):.1f} s)")
 
 
+def cpu_baseline_reference(case, coord, pairs_per_step, n1=1, n2=5):
+    """The reference itself (its flang OpenMP build driven by the dump harness
+    in oracle/_ref, same jittered positions): two runs of n1 and n2 time steps,
+    the difference of their wall times timing n2 - n1 whole steps without the
+    pre-processing both share.  pairs_per_step: the list pairs of one step of
+    this case (the reference's lists are the device's).  None if the harness
+    is not built."""
+    import tempfile
+    from oracle.decks import HARNESS, run_harness
+    if not HARNESS.exists():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
+    times = []
+    for n in (n1, n2):
+        with tempfile.TemporaryDirectory(prefix="sph_refbase_") as wd:
+            t0 = time.perf_counter()
+            run_harness(case, n, every=n, lists=False, ic_coord=coord, threads=threads, workdir=wd)
+            times.append(time.perf_counter() - t0)
+    dt = times[1] - times[0]
+    if dt <= 0:
+        return None
+    return dict(value=pairs_per_step * (n2 - n1) / dt, unit="pair-interactions/s", cores=threads,
+                kind="reference",
+                sample=f"the reference (flang -O2 OpenMP build, oracle/_ref/harness3d.x), full "
+                       f"{len(coord)}-particle case: runs of {n1} and {n2} time steps took "
+                       f"{times[0]:.1f} s and {times[1]:.1f} s; {n2 - n1} steps in {dt:.1f} s at the "
+                       f"device's {pairs_per_step:.4g} list pairs per step")
+
+
 def copy_probe(mib):
     """A plain device copy of known size (mib MiB read, mib MiB written), run
     before the benchmark so that rocprofv3 counter runs of this command can
@@ -177,6 +206,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="CPU baseline sample length (whole steps until this many seconds)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ref-cpu", action="store_true",
+                    help="time only the oracle's CPU restatement, not the reference's own build")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every kernel of a step instead of replaying the step's HIP graph")
@@ -281,11 +312,17 @@ def main():
             log(f"[bench] ensemble (segmented batches of {args.ens_batch}): p50 "
                 f"{b['p50_latency_s'] * 1e3:.1f} ms, p99 {b['p99_latency_s'] * 1e3:.1f} ms, "
                 f"{b['pair_interactions_per_s'] / 1e9:.2f} G pairs/s")
-    cpu = None
+    cpu = cpu_port = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"[bench] cpu baseline (up to {args.cpu_steps} steps / {args.cpu_seconds} s)")
-        cpu = cpu_baseline(tb, coord, args.cpu_steps, args.cpu_seconds)
-        log(f"[bench] cpu baseline: {cpu['value'] / 1e6:.2f} M pairs/s ({cpu['sample']})")
+        cpu_port = cpu_baseline(tb, coord, args.cpu_steps, args.cpu_seconds)
+        log(f"[bench] cpu baseline: {cpu_port['value'] / 1e6:.2f} M pairs/s ({cpu_port['sample']})")
+        if not args.no_ref_cpu:
+            cpu = cpu_baseline_reference(case, coord, pairs_per_step)
+            if cpu:
+                log(f"[bench] reference cpu baseline: {cpu['value'] / 1e6:.2f} M pairs/s ({cpu['sample']})")
+        if cpu is None:
+            cpu, cpu_port = cpu_port, None
     out_line = {
         "metric": "pair-interactions/s mixed vs FP64 (3D dam break); p50/p99 ensemble latency",
         "value": value,
@@ -315,6 +352,7 @@ def main():
         "kernels": per_k,
         "ensemble": ens,
         "cpu_baseline": cpu,
+        "cpu_baseline_port": cpu_port,
     }
     print(json.dumps(out_line), flush=True)
 

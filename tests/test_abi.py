@@ -59,3 +59,22 @@ def test_dropin_harness_2d_fails_loudly_without_device(tmp_path):
     case = DamBreak2D(tank=(0.6, 0.4), column=(0.2, 0.1), dx=0.01, name="t2d")
     with pytest.raises(RuntimeError, match="sph_ctx_create2d failed, status +1"):
         run_harness(case, 2, lists=False, workdir=tmp_path, exe=HARNESS2D_HIP, timeout=60)
+
+
+def test_batch_bytes_layout():
+    """Sizing of segmented batches (no device call): a batch of one segment
+    is sized as a one-solve context; a batch of several holds at least the sum
+    of its segments' particle arrays (slots rounded to whole 256-blocks)."""
+    from sphera_amd import runtime as rt
+    if not rt.LIB_PATH.exists():
+        pytest.skip("device library not built")
+    from sphera_amd.case import DamBreak3D, build
+    a = build(DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="a"))
+    b = build(DamBreak3D(tank=(0.4, 0.2, 0.3), column=(0.12, 0.2, 0.15), dx=0.012, name="b"))
+    for prec in (rt.FP64, rt.FP32_MIXED):
+        one = rt.batch_bytes([a], prec)
+        assert one == rt.ctx_bytes(a, prec)
+        both = rt.batch_bytes([a, b], prec)
+        # the scalar blocks and the list-stride rounding are shared; everything
+        # per particle and per cell is each segment's own
+        assert both >= rt.ctx_bytes(a, prec) + rt.ctx_bytes(b, prec) - 4096
