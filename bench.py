@@ -152,7 +152,7 @@ def cpu_baseline(tb, coord, max_steps, budget_s):
                        f"{steps} whole time steps after 1 untimed ({(t1 - t0):.1f} s)")
 
 
-def cpu_baseline_reference(case, coord, pairs_per_step, n1=1, n2=5):
+def cpu_baseline_reference(case, coord, pairs_per_step, n1=1, n2=25):
     """The reference itself (its flang OpenMP build driven by the dump harness
     in oracle/_ref, same jittered positions): two runs of n1 and n2 time steps,
     the difference of their wall times timing n2 - n1 whole steps without the
