@@ -92,3 +92,40 @@ def test_batch_of_one_is_the_plain_context():
     ref, _ = _alone(tb, rt.FP32_MIXED, coord, 5)
     for f in FIELDS:
         assert np.array_equal(got[f], ref[f]), f
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_batch_segment_removal_equals_alone(precision):
+    """A segment whose initial state has particles outside its grid (the
+    reference's removal fixture) loses them as in OrdGrid1 (swap with the
+    last, per segment) while the other segments are untouched: every segment
+    still equals its solve alone, ids dense."""
+    from pathlib import Path
+
+    from sphera_amd import runtime as rt
+    prec = rt.FP64 if precision == "fp64" else rt.FP32_MIXED
+    g = np.load(Path(__file__).resolve().parent / "golden" / "ref_removal.npz")
+    tiny = build(DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny"))
+    other = _cases()[1]
+    ic = g["ic_coord"]
+    # the removal case in the middle, so that segments before and after it are checked
+    tbs = [other, tiny, other]
+    coords = [_jitter(other, 11), ic, _jitter(other, 12)]
+    print(f"[gpu] batch removal {precision} n={[len(c) for c in coords]}", flush=True)
+
+    assert len(ic) == len(tiny.particles["coord"])  # the fixture moves particles of this case
+    b = rt.SphBatch(tbs, precision=prec)
+    try:
+        b.upload(coords)
+        b.preloop()
+        b.step(6)
+        got = [b.seg_state(k) for k in range(3)]
+    finally:
+        b.close()
+    for k, (tb, c) in enumerate(zip(tbs, coords)):
+        ref, _ = _alone(tb, prec, c, 6)
+        assert got[k]["nag"] == ref["nag"]
+        assert np.array_equal(got[k]["ids"], np.arange(ref["nag"]))
+        for f in FIELDS:
+            assert np.array_equal(got[k][f], ref[f]), (k, f)
+    assert got[1]["nag"] == len(ic) - 6
