@@ -324,8 +324,10 @@ int run_continuity(sph_ctx* c) {
 
 int run_psmooth(sph_ctx* c) {
     KTime t(c, "pressure_smoothing");
+    // w of the velocity half of FP32 record buffer 0 (read by the next momentum pass)
+    float* velw = c->P.f32 ? reinterpret_cast<float*>(c->rec[0] + 1) + 3 : nullptr;
     PAIRL(k_psmooth, c->chunk_p, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf, c->rpf[2], c->nbr,
-          c->nnb, c->vpres, c->rp[0], c->rpf[0], c->scal, c->iscal);
+          c->nnb, c->vpres, c->rp[0], c->rpf[0], velw, c->scal, c->iscal);
     return 0;
 }
 
@@ -495,6 +497,9 @@ int fill_dp(const sph_params* p, const sph_faces* f, DP& P, bool dry = false) {
     P.f_alfamon = (float)P.alfamon;
     P.f_celerita = (float)P.celerita;
     P.f_kin_visc = (float)P.kin_visc;
+    P.f_den0 = (float)P.den0;
+    P.f_eosk = (float)(P.eps / P.den0);
+    P.f_prif = (float)P.prif;
     return 0;
 }
 

@@ -45,6 +45,9 @@ struct DP {
     // FP32-mixed copies
     float f_h, f_unosuh, f_unosusquareh, f_eta, f_eta2, f_ke_coef, f_kacl_coef, f_coefke;
     float f_dcd[3], f_alfamon, f_celerita, f_kin_visc;
+    // the density slot of the FP32 velocity records holds rho - den0; the
+    // linear EoS p = eps/den0 (rho - den0) + prif gives the pressure from it
+    float f_den0, f_eosk, f_prif;
     // 3, or 2 for the x-z plane form (-DSPACE_2D build of the reference): one
     // cell row in y (every particle in row j = 1), boundary sides for faces
     int dim;

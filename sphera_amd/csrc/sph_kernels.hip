@@ -492,7 +492,7 @@ __global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* 
         xmf0[p] = xr;
         xmf1[p] = xr;
         xmf2[p] = xr;
-        velf[p] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)r.x);
+        velf[p] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)(r.x - P.den0));
         rpf[p] = make_float2((float)r.x, (float)r.y);
     }
 }
@@ -2122,22 +2122,24 @@ __global__ __launch_bounds__(256) void k_momentum(
                 const float nu = P.f_kin_visc;
                 const float atil = 4.0f * (nu * nu);
                 const bool morris = P.kin_visc > 1.0e-12;
-                using RM = NRec<float4, float4, float2>;
+                // neighbour records: (x, m) and (v, rho - den0); the pressure
+                // follows from the linear EoS, so one 32-byte run per neighbour
+                using RM = NRec2<float4, float4>;
                 auto fetch = [&](uint32_t e) {
                     const int j = (int)(e & NB_MASK);
-                    return RM{xmf[j], velf[j], rpf[j]};
+                    return RM{xmf[j], velf[j]};
                 };
                 walk_list<CH, RM>(nbr, N, i, nn, fetch, [&](uint32_t e, const RM& q) {
                     if ((int)(e & NB_MASK) == i) return;
                     const float4 xj = q.x;
-                    const float4 vj = q.v;
-                    const float2 rpj = q.r;
+                    const float4 vj = q.r;
                     float R0, R1, R2;
                     rag32(xf, xj, e, R0, R1, R2);
                     const float rij = R0 * R0 + R1 * R1 + R2 * R2;
                     K32 K;
                     kvals32(P, rij, true, true, false, K);
-                    const float rhoj = rpj.x, pj = rpj.y, mj = xj.w;
+                    const float rhoj = P.f_den0 + vj.w, pj = __builtin_fmaf(P.f_eosk, vj.w, P.f_prif);
+                    const float mj = xj.w;
                     const float dv0 = vj.x - vf.x, dv1 = vj.y - vf.y, dv2 = vj.z - vf.z;
                     const float cp = -mj * (pri + pj * __builtin_amdgcn_rcpf(rhoj * rhoj)) * K.K3;
                     s_p0 += cp * R0; s_p1 += cp * R1; s_p2 += cp * R2;
@@ -2196,7 +2198,7 @@ __global__ __launch_bounds__(256) void k_momentum(
             const double4 vn =
                 make_double4(vi.x + dtvel * a[0], vi.y + dtvel * a[1], vi.z + dtvel * a[2], vi.w);
             vel_out[i] = vn;
-            if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, (float)rpi.x);
+            if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, (float)(rpi.x - P.den0));
         }
         __syncthreads();
     }
@@ -2274,7 +2276,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                         rag32(xf, xj, e, R0, R1, R2);
                         K32 K;
                         kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
-                        const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(vj.w);
+                        const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(P.f_den0 + vj.w);
                         s0 += (vj.x - vf.x) * pes;
                         s1 += (vj.y - vf.y) * pes;
                         s2 += (vj.z - vf.z) * pes;
@@ -2327,7 +2329,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
             double4 vn = vi;
             if (smooth) vn = make_double4(vi.x + tv1 * v[0], vi.y + tv1 * v[1], vi.z + tv1 * v[2], vi.w);
             vel_out[i] = vn;
-            if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, (float)rp[i].x);
+            if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, (float)(rp[i].x - P.den0));
             xm_out[i] = make_double4(xi.x + dt * vn.x, xi.y + dt * vn.y, xi.z + dt * vn.z, xi.w);
             const unsigned long long b =
                 (unsigned long long)__double_as_longlong(d_dt_cfl(P, vn.x, vn.y, vn.z));
@@ -2459,7 +2461,7 @@ __global__ __launch_bounds__(256) void k_continuity(
                     rag32(xf, xj, e, R0, R1, R2);
                     K32 K;
                     kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, true, false, false, K);
-                    s += xj.w * __builtin_amdgcn_rcpf(vj.w) * K.K1 *
+                    s += xj.w * __builtin_amdgcn_rcpf(P.f_den0 + vj.w) * K.K1 *
                          ((vj.x - vf.x) * R0 + (vj.y - vf.y) * R1 + (vj.z - vf.z) * R2);
                                 });
                 inner = rhoi * (double)s;
@@ -2484,6 +2486,7 @@ __global__ __launch_bounds__(256) void k_psmooth(
     const double4* __restrict__ xm, const double2* __restrict__ rp, const float4* __restrict__ xmf,
     const float2* __restrict__ rpf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
     double* __restrict__ vpres_out, double2* __restrict__ rp_out, float2* __restrict__ rpf_out,
+    float* __restrict__ velw_out,
     const double* scal, const int* iscal) {
     using X = typename Rec<F32>::X;
     using RPT = typename Rec<F32>::RP;
@@ -2585,7 +2588,12 @@ __global__ __launch_bounds__(256) void k_psmooth(
                 rho2 = P.den0 * (1.0 + (p2 - P.prif) / P.eps);
             }
             rp_out[i] = make_double2(rho2, p2);
-            if (F32) rpf_out[i] = make_float2((float)rho2, (float)p2);
+            if (F32) {
+                rpf_out[i] = make_float2((float)rho2, (float)p2);
+                // the density slot of the velocity record the next momentum
+                // pass reads (element 8 i: 32-byte records, w of the second half)
+                velw_out[8 * (size_t)i] = (float)(rho2 - P.den0);
+            }
         }
         __syncthreads();
     }
