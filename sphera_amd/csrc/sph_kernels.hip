@@ -194,6 +194,9 @@ __device__ __forceinline__ bool d_point_internal(const DFace& F, double px0, dou
 // almost in cell order (they moved less than a cell since the last sort), so
 // a workgroup's 256 particles fall in a narrow window of cells: it counts
 // them in an LDS histogram and issues one global atomic per occupied cell.
+// Inside the window a particle's rank among the workgroup's particles of its
+// cell follows the thread order, so particles that stay in their cell keep
+// their previous (ascending-id) order and k_cell_sort finds its runs sorted.
 // ----------------------------------------------------------------------------
 constexpr int HIST_W = 512;
 
@@ -205,8 +208,8 @@ __global__ __launch_bounds__(256) void k_cell_count(DP P0, Seg S, const double4*
                                                     int* __restrict__ key, int* __restrict__ slot,
                                                     int* __restrict__ cnt, int* __restrict__ idslot,
                                                     int* __restrict__ rmids, int* iscal) {
-    __shared__ int hist[HIST_W];
     __shared__ int base[HIST_W];
+    __shared__ int wcnt[4][HIST_W];  // particles of each window cell per wave
     __shared__ int cmin;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(blockIdx.x * blockDim.x) >> 8]) : 0;
@@ -214,7 +217,8 @@ __global__ __launch_bounds__(256) void k_cell_count(DP P0, Seg S, const double4*
     iscal += sg * IS_COUNT;
     rmids += sg * RM_CAP;
     const int n = iscal[IS_N];
-    for (int b = threadIdx.x; b < HIST_W; b += blockDim.x) hist[b] = 0;
+    for (int b = threadIdx.x; b < HIST_W; b += blockDim.x)
+        for (int w = 0; w < 4; ++w) wcnt[w][b] = 0;
     if (threadIdx.x == 0) cmin = 0x7fffffff;
     __syncthreads();
     int k = -1;
@@ -239,16 +243,29 @@ __global__ __launch_bounds__(256) void k_cell_count(DP P0, Seg S, const double4*
     int loc = -1, rank = 0;
     if (k >= 0) {
         const int d = k - c0;
-        if (k < P.koff + P.nmax && d < HIST_W) {
-            loc = d;
-            rank = atomicAdd(&hist[d], 1);
-        } else {
-            slot[i] = atomicAdd(&cnt[k], 1);
-        }
+        if (k < P.koff + P.nmax && d < HIST_W) loc = d;
+        else slot[i] = atomicAdd(&cnt[k], 1);
+    }
+    // rank within the wave among the lanes of the same window cell, one
+    // ballot per distinct cell of the wave
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(loc >= 0);
+    while (todo) {
+        const int lead = __ffsll((long long)todo) - 1;
+        const int dk = __shfl(loc, lead, 64);
+        const unsigned long long m = __ballot(loc == dk);
+        if (loc == dk) rank = __popcll(m & below);
+        if (lane == lead) wcnt[wid][dk] = __popcll(m);
+        todo &= ~m;
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < HIST_W; b += blockDim.x)
-        if (hist[b] > 0) base[b] = atomicAdd(&cnt[c0 + b], hist[b]);
+    if (loc >= 0)
+        for (int w = 0; w < wid; ++w) rank += wcnt[w][loc];
+    for (int b = threadIdx.x; b < HIST_W; b += blockDim.x) {
+        const int tot = wcnt[0][b] + wcnt[1][b] + wcnt[2][b] + wcnt[3][b];
+        if (tot > 0) base[b] = atomicAdd(&cnt[c0 + b], tot);
+    }
     __syncthreads();
     if (loc >= 0) slot[i] = base[loc] + rank;
 }
