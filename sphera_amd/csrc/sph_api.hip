@@ -68,6 +68,7 @@ struct sph_ctx {
     double2* rp[3] = {};
     int* lam[2] = {};
     float4* xmf = nullptr;  // FP32 cell-relative positions (x, y, z, mass)
+    float4* xrf = nullptr;  // FP32 (x, y, z, rho - den0) after continuity, for the pressure smoothing
     float4* rec[3] = {};  // FP32 record buffers: (position, velocity) float4 pairs per particle
     float2* rpf[3] = {};
     double *dden = nullptr, *vpres = nullptr;
@@ -318,7 +319,7 @@ int run_vsmooth(sph_ctx* c) {
 int run_continuity(sph_ctx* c) {
     KTime t(c, "continuity");
     PAIRL(k_continuity, c->chunk_c, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0], c->rp[1], XF(c, 0),
-          VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->scal, c->iscal);
+          VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->xrf, c->scal, c->iscal);
     return 0;
 }
 
@@ -326,8 +327,8 @@ int run_psmooth(sph_ctx* c) {
     KTime t(c, "pressure_smoothing");
     // w of the velocity half of FP32 record buffer 0 (read by the next momentum pass)
     float* velw = c->P.f32 ? reinterpret_cast<float*>(c->rec[0] + 1) + 3 : nullptr;
-    PAIRL(k_psmooth, c->chunk_p, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf, c->rpf[2], c->nbr,
-          c->nnb, c->vpres, c->rp[0], c->rpf[0], velw, c->scal, c->iscal);
+    PAIRL(k_psmooth, c->chunk_p, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf, c->rpf[2], c->xrf,
+          c->nbr, c->nnb, c->vpres, c->rp[0], c->rpf[0], velw, c->scal, c->iscal);
     return 0;
 }
 
@@ -370,7 +371,7 @@ sph_ctx::~sph_ctx() {
     for (auto p : acc) F(p);
     for (auto p : rp) F(p);
     for (auto p : lam) F(p);
-    F(xmf);
+    F(xmf); F(xrf);
     for (auto p : rec) F(p);
     for (auto p : rpf) F(p);
     F(dden); F(vpres); F(rowblk); F(idslot); F(rmids); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
@@ -392,6 +393,7 @@ int sph_ctx::alloc() {
     for (auto& p : rp) HIPC(da(&p, N));
     for (auto& p : lam) HIPC(da(&p, N));
     HIPC(da(&xmf, N));
+    HIPC(da(&xrf, P.f32 ? N : 1));
     for (auto& r : rec) HIPC(da(&r, P.f32 ? 2 * N : 1));
     for (auto& p : rpf) HIPC(da(&p, P.f32 ? N : 1));
     HIPC(da(&dden, N));
@@ -875,6 +877,26 @@ int sph_segment_upload(sph_ctx* c, int32_t seg, int32_t n, const double* coord, 
     is[IS_ONGOING] = -999;
     double sc[SC_COUNT] = {0};
     const size_t b = P.base;
+    // one mass for the whole solve: the pressure smoothing's one-record form
+    bool um = n > 0;
+    for (int i = 1; i < n && um; ++i) um = mass[i] == mass[0];
+    const int umass = um ? 1 : 0;
+    const float f_mass = um ? (float)mass[0] : 0.0f;
+    if (umass != P.umass || f_mass != P.f_mass) {
+        DP& Q = c->nseg > 1 ? c->segs[seg] : c->P;
+        Q.umass = umass;
+        Q.f_mass = f_mass;
+        if (c->nseg > 1) {
+            if (seg == 0) {
+                c->P.umass = umass;
+                c->P.f_mass = f_mass;
+            }
+            HIPC(hipMemcpy(c->d_P + seg, &Q, sizeof(DP), hipMemcpyHostToDevice));
+        }
+        // captured graphs hold the constants they were captured with
+        for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
+        c->graphs.clear();
+    }
     if (c->staged.empty()) {  // first segment of a new upload: clear the whole context
         HIPC(hipMemsetAsync(c->lam[0], 0, sizeof(int) * c->P.cap, c->stream));
         HIPC(hipMemsetAsync(c->acc[1], 0, sizeof(double4) * c->P.cap, c->stream));

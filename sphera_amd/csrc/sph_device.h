@@ -48,6 +48,11 @@ struct DP {
     // the density slot of the FP32 velocity records holds rho - den0; the
     // linear EoS p = eps/den0 (rho - den0) + prif gives the pressure from it
     float f_den0, f_eosk, f_prif;
+    // every particle of the solve has the mass f_mass (one medium, one dx:
+    // SetParticleParameters.f90:56-57): the pressure smoothing then reads one
+    // (x, y, z, rho - den0) record per neighbour
+    int umass;
+    float f_mass;
     // 3, or 2 for the x-z plane form (-DSPACE_2D build of the reference): one
     // cell row in y (every particle in row j = 1), boundary sides for faces
     int dim;

@@ -2369,6 +2369,7 @@ __global__ __launch_bounds__(256) void k_continuity(
     const double4* __restrict__ vel, const double2* __restrict__ rp, F4In xmf,
     F4In velf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
     double* __restrict__ dden_out, double2* __restrict__ rp_out, float2* __restrict__ rpf_out,
+    float4* __restrict__ xrf_out,
     const double* scal, int* iscal) {
     using X = typename Rec<F32>::X;
     using V = typename Rec<F32>::V;
@@ -2488,7 +2489,12 @@ __global__ __launch_bounds__(256) void k_continuity(
             const double rho1 = rhoi + dt * dden;
             const double p1 = eos ? P.eps / P.den0 * (rho1 - P.den0) + P.prif : rpi.y;
             rp_out[i] = make_double2(rho1, p1);
-            if (F32) rpf_out[i] = make_float2((float)rho1, (float)p1);
+            if (F32) {
+                rpf_out[i] = make_float2((float)rho1, (float)p1);
+                // (cell-relative position, rho - den0) for the pressure smoothing
+                const float4 xo = xmf[i];
+                xrf_out[i] = make_float4(xo.x, xo.y, xo.z, (float)(rho1 - P.den0));
+            }
         }
         __syncthreads();
     }
@@ -2501,7 +2507,8 @@ template <bool F32, int CH, bool SG>
 __global__ __launch_bounds__(256) void k_psmooth(
     DP P0, Seg S, const DFace* __restrict__ faces, const int* __restrict__ bn, const BTab* __restrict__ bt,
     const double4* __restrict__ xm, const double2* __restrict__ rp, const float4* __restrict__ xmf,
-    const float2* __restrict__ rpf, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
+    const float2* __restrict__ rpf, const float4* __restrict__ xrf, const uint32_t* __restrict__ nbr,
+    const int* __restrict__ nnb,
     double* __restrict__ vpres_out, double2* __restrict__ rp_out, float2* __restrict__ rpf_out,
     float* __restrict__ velw_out,
     const double* scal, const int* iscal) {
@@ -2544,25 +2551,42 @@ __global__ __launch_bounds__(256) void k_psmooth(
                         somp = somp + (rpj.y - pi) * pes;
                                         });
                 } else {
-                                        const float4 xf = xmf[i];
+                    const float4 xf = xmf[i];
                     const float pif = rpf[i].y;
                     float s_a = 0.f, s_p = 0.f;
-                    using RP2 = NRec2<float4, float2>;
-                    auto fetch = [&](uint32_t e) {
-                        const int j = (int)(e & NB_MASK);
+                    if (P.umass) {
+                        // one (x, y, z, rho - den0) record per neighbour: m_j is
+                        // the solve's mass, p_j - p_i the linear EoS of the
+                        // density difference
+                        const float wi = xrf[i].w;
+                        auto fetch = [&](uint32_t e) { return xrf[(int)(e & NB_MASK)]; };
+                        walk_list<CH, float4>(nbr, N, i, nsp, fetch, [&](uint32_t e, const float4& xj) {
+                            float R0, R1, R2;
+                            rag32(xf, xj, e, R0, R1, R2);
+                            K32 K;
+                            kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                            const float pes = P.f_mass * K.W * __builtin_amdgcn_rcpf(P.f_den0 + xj.w);
+                            s_a += pes;
+                            s_p += P.f_eosk * (xj.w - wi) * pes;
+                        });
+                    } else {
+                        using RP2 = NRec2<float4, float2>;
+                        auto fetch = [&](uint32_t e) {
+                            const int j = (int)(e & NB_MASK);
                             return RP2{xmf[j], rpf[j]};
-                    };
-                    walk_list<CH, RP2>(nbr, N, i, nsp, fetch, [&](uint32_t e, const RP2& q) {
-                        const float4 xj = q.x;
-                        const float2 rpj = q.r;
-                        float R0, R1, R2;
-                        rag32(xf, xj, e, R0, R1, R2);
-                        K32 K;
-                        kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
-                        const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(rpj.x);
-                        s_a += pes;
-                        s_p += (rpj.y - pif) * pes;
-                                        });
+                        };
+                        walk_list<CH, RP2>(nbr, N, i, nsp, fetch, [&](uint32_t e, const RP2& q) {
+                            const float4 xj = q.x;
+                            const float2 rpj = q.r;
+                            float R0, R1, R2;
+                            rag32(xf, xj, e, R0, R1, R2);
+                            K32 K;
+                            kvals32(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                            const float pes = xj.w * K.W * __builtin_amdgcn_rcpf(rpj.x);
+                            s_a += pes;
+                            s_p += (rpj.y - pif) * pes;
+                        });
+                    }
                     somp = s_p;
                     app = s_a;
                 }
