@@ -59,8 +59,9 @@ struct sph_ctx {
     hipStream_t stream = nullptr;
     // faces / tables
     DFace* d_faces = nullptr;
+    // per cell key: the frontier faces listed in its 27-cell neighbourhood,
+    // each once (count, start into d_gvec)
     int *d_gcount = nullptr, *d_gstart = nullptr, *d_gvec = nullptr;
-    int* d_gnear = nullptr;  // per cell: frontier faces listed in its 27-cell neighbourhood
     double *d_kt = nullptr, *d_bit = nullptr;
     BtF32* d_btf = nullptr;  // the quadrature table in the FP32-mixed layout
     // state
@@ -270,7 +271,7 @@ int run_btab(sph_ctx* c) {
     {
         KTime t(c, "boundary_find");
         k_fill_int<<<1, 64, 0, c->stream>>>(c->nwork, 0, 1);
-        SEGL(k_boundary_find, nblk(c->P.cap), 256, c->d_faces, c->d_gnear, c->d_gcount, c->d_gstart, c->d_gvec,
+        SEGL(k_boundary_find, nblk(c->P.cap), 256, c->d_faces, c->d_gcount, c->d_gstart, c->d_gvec,
              c->xm[0], c->bn, c->bt, c->work, c->nwork, c->iscal);
     }
     KTime t(c, "boundary_int");
@@ -364,7 +365,7 @@ sph_ctx::~sph_ctx() {
     auto F = [](void* p) {
         if (p) hipFree(p);
     };
-    F(d_faces); F(d_gnear); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit); F(d_btf);
+    F(d_faces); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit); F(d_btf);
     F(d_P); F(d_pblk); F(d_kblk); F(d_rseg); F(d_adj);
     for (auto p : xm) F(p);
     for (auto p : vel) F(p);
@@ -606,7 +607,8 @@ int upload_tables(sph_ctx* c, const sph_faces* const* f) {
     }
     std::vector<DFace> hf;
     hf.reserve(nf);
-    std::vector<int> gcount(c->nkeys, 0), gstart(c->nkeys, 0), gnear(c->nkeys, 0), gvec;
+    // per key: count and 0-based start of the cell's near-face list in gvec
+    std::vector<int> gcount(c->nkeys, 0), gstart(c->nkeys, 0), gvec;
     gvec.reserve(nv);
     std::vector<double> kt, bit;
     std::vector<BtF32> btf(nseg);
@@ -637,26 +639,32 @@ int upload_tables(sph_ctx* c, const sph_faces* const* f) {
         for (int i = 0; i < P.nmax; ++i) sv += fs->gcbf_count[i];
         for (long long q = 0; q < sv; ++q)
             if (fs->gcbf_vec[q] < 0 || fs->gcbf_vec[q] >= fs->nf) return fail(2, "gcbf_vec entry out of range");
-        const int voff = (int)gvec.size();
-        for (long long q = 0; q < sv; ++q) gvec.push_back(fs->gcbf_vec[q] + P.foff);
+        // per cell: the frontier faces listed in its 27-cell neighbourhood, each
+        // once, in the order FindCloseBoundaryFaces3D.f90 meets them (cells by
+        // di, dj, dk, faces in GCBFVector order), so the particle's table keeps
+        // the reference's face order
         const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
-        for (int i = 0; i < P.nmax; ++i) {
-            gcount[P.koff + i] = fs->gcbf_count[i];
-            gstart[P.koff + i] = fs->gcbf_start[i] + voff;
-        }
-        // particles of cells with no face in reach skip the face search
+        std::vector<int> seen;
         for (int k = 0; k < nk; ++k)
             for (int j = 0; j < nj; ++j)
                 for (int i = 0; i < ni; ++i) {
-                    int sum = 0;
-                    for (int dk = -1; dk <= 1; ++dk)
+                    seen.clear();
+                    for (int di = -1; di <= 1; ++di)
                         for (int dj = -1; dj <= 1; ++dj)
-                            for (int di = -1; di <= 1; ++di) {
+                            for (int dk = -1; dk <= 1; ++dk) {
                                 const int ii = i + di, jj = j + dj, kk = k + dk;
                                 if (ii < 0 || ii >= ni || jj < 0 || jj >= nj || kk < 0 || kk >= nk) continue;
-                                sum += fs->gcbf_count[((size_t)kk * nj + jj) * ni + ii];
+                                const size_t cn = ((size_t)kk * nj + jj) * ni + ii;
+                                const int ini = fs->gcbf_start[cn] - 1;
+                                for (int q = ini; q < ini + fs->gcbf_count[cn]; ++q) {
+                                    const int fq = fs->gcbf_vec[q] + P.foff;
+                                    if (std::find(seen.begin(), seen.end(), fq) == seen.end()) seen.push_back(fq);
+                                }
                             }
-                    gnear[P.koff + ((size_t)k * nj + j) * ni + i] = sum;
+                    const size_t key = P.koff + ((size_t)k * nj + j) * ni + i;
+                    gcount[key] = (int)seen.size();
+                    gstart[key] = (int)gvec.size();
+                    gvec.insert(gvec.end(), seen.begin(), seen.end());
                 }
         kt.insert(kt.end(), fs->kerneltab, fs->kerneltab + 41 * 5);
         bit.insert(bit.end(), fs->bitab, fs->bitab + 256 * 7);
@@ -669,7 +677,7 @@ int upload_tables(sph_ctx* c, const sph_faces* const* f) {
                hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) == hipSuccess;
     };
     bool ok = up(&c->d_faces, hf) && up(&c->d_gcount, gcount) && up(&c->d_gstart, gstart) &&
-              up(&c->d_gnear, gnear) && up(&c->d_gvec, gvec) && up(&c->d_kt, kt) && up(&c->d_bit, bit) &&
+              up(&c->d_gvec, gvec) && up(&c->d_kt, kt) && up(&c->d_bit, bit) &&
               up(&c->d_btf, btf);
     if (!ok) return fail(102, "device allocation/copy of frontier tables failed");
     return 0;

@@ -962,10 +962,9 @@ __device__ void d_volume_integrals(const DP& P, const DFace& F, const double* __
 // ----------------------------------------------------------------------------
 template <bool SG>
 __global__ __launch_bounds__(256) void k_boundary_find(DP P0, Seg S, const DFace* __restrict__ faces,
-                                                       const int* __restrict__ gnear,
-                                                       const int* __restrict__ gcount,
-                                                       const int* __restrict__ gstart,
-                                                       const int* __restrict__ gvec,
+                                                       const int* __restrict__ ncount,
+                                                       const int* __restrict__ nstart,
+                                                       const int* __restrict__ nvec,
                                                        const double4* __restrict__ xm,
                                                        int* __restrict__ bn, BTab* __restrict__ bt,
                                                        int* __restrict__ work, int* nwork, int* iscal) {
@@ -974,60 +973,45 @@ __global__ __launch_bounds__(256) void k_boundary_find(DP P0, Seg S, const DFace
     const DP& P = SG ? S.P[sg] : P0;
     iscal += sg * IS_COUNT;
     if (i - P.base >= iscal[IS_N]) return;
-    // the per-cell face tables of a segmented context are indexed by key
-    gnear += P.koff;
-    gcount += P.koff;
-    gstart += P.koff;
     const double4 x = xm[i];
     const double xc[3] = {x.x, x.y, x.z};
     const int nc = d_particle_cell(P, x.x, x.y, x.z);
-    if (nc <= 0 || gnear[nc - 1] == 0) {  // no frontier face listed within reach
+    // the faces listed in the 27 cells around the particle's cell, each once
+    // and in the order the reference's cell walk meets them (a face accepted
+    // once is skipped when met again; one rejected is rejected again)
+    const int nf = nc > 0 ? ncount[P.koff + nc - 1] : 0;
+    if (nf == 0) {  // no frontier face listed within reach
         bn[i] = 0;
         return;
     }
     int ncbf = 0;
     int clob[SPH_MB];
     double lz[SPH_MB];
-    if (nc > 0) {
-        int ic, jc, kc;
-        d_cell_indices(P, nc, ic, jc, kc);
-        for (int di = -1; di <= 1; ++di)
-            for (int dj = -1; dj <= 1; ++dj)
-                for (int dk = -1; dk <= 1; ++dk) {
-                    const int cn = d_cell_number(P, ic + di, jc + dj, kc + dk);
-                    if (cn == 0) continue;
-                    const int nfc = gcount[cn - 1];
-                    if (nfc <= 0) continue;
-                    const int ini = gstart[cn - 1] - 1;
-                    for (int fl = ini; fl < ini + nfc; ++fl) {
-                        const int f = gvec[fl];
-                        bool there = false;
-                        for (int q = 0; q < ncbf; ++q) there |= (clob[q] == f);
-                        if (there) continue;
-                        const DFace& F = faces[f];
-                        const double pinmin = (F.btype == 3) ? -P.doubleh : 0.0;
-                        const double* ref = &F.GX[3 * (F.nodes - 1)];
-                        double PX[3];
-                        for (int sdi = 0; sdi < 3; ++sdi) {
-                            double s = 0.0;
-                            for (int sdj = 0; sdj < 3; ++sdj) s = s + F.T[3 * sdi + sdj] * (xc[sdj] - ref[sdj]);
-                            PX[sdi] = s;
-                        }
-                        const double pin = PX[2];
-                        if (pin >= pinmin && pin < P.doubleh) {
-                            const bool need_in = (F.btype == 3 || F.btype == 4 || F.btype == 5);
-                            if (!need_in || d_point_internal(F, PX[0], PX[1])) {
-                                if (ncbf >= SPH_MB) {
-                                    atomicAdd(&iscal[IS_BT_OVERFLOW], 1);
-                                    continue;
-                                }
-                                clob[ncbf] = f;
-                                lz[ncbf] = pin;
-                                ++ncbf;
-                            }
-                        }
-                    }
+    const int ini = nstart[P.koff + nc - 1];
+    for (int fl = ini; fl < ini + nf; ++fl) {
+        const int f = nvec[fl];
+        const DFace& F = faces[f];
+        const double pinmin = (F.btype == 3) ? -P.doubleh : 0.0;
+        const double* ref = &F.GX[3 * (F.nodes - 1)];
+        double PX[3];
+        for (int sdi = 0; sdi < 3; ++sdi) {
+            double s = 0.0;
+            for (int sdj = 0; sdj < 3; ++sdj) s = s + F.T[3 * sdi + sdj] * (xc[sdj] - ref[sdj]);
+            PX[sdi] = s;
+        }
+        const double pin = PX[2];
+        if (pin >= pinmin && pin < P.doubleh) {
+            const bool need_in = (F.btype == 3 || F.btype == 4 || F.btype == 5);
+            if (!need_in || d_point_internal(F, PX[0], PX[1])) {
+                if (ncbf >= SPH_MB) {
+                    atomicAdd(&iscal[IS_BT_OVERFLOW], 1);
+                    continue;
                 }
+                clob[ncbf] = f;
+                lz[ncbf] = pin;
+                ++ncbf;
+            }
+        }
     }
     bn[i] = ncbf;
     if (ncbf == 0) return;
