@@ -70,7 +70,9 @@ struct sph_ctx {
     int* lam[2] = {};
     float4* xmf = nullptr;  // FP32 cell-relative positions (x, y, z, mass)
     float4* xrf = nullptr;  // FP32 (x, y, z, rho - den0) after continuity, for the pressure smoothing
-    float4* rec[3] = {};  // FP32 record buffers: (position, velocity) float4 pairs per particle
+    // FP32 record buffers, (position, velocity) float4 pairs per particle:
+    // [0] read by momentum and continuity, [1] by velocity smoothing
+    float4* rec[2] = {};
     float2* rpf[3] = {};
     double *dden = nullptr, *vpres = nullptr;
     int *cell = nullptr, *key = nullptr, *slot = nullptr, *cnt = nullptr, *start = nullptr,
@@ -225,7 +227,7 @@ int run_sort(sph_ctx* c) {
         KTime t(c, "sort_gather");
         SEGL(k_gather, nblk(cap), 256, c->perm, c->key, c->xm[1], c->vel[2], c->rp[0], c->lam[0], c->acc[1],
              c->xm[0], c->vel[0], c->rp[1], c->lam[1], c->acc[0], c->cell, c->xmf, XFo(c, 0), XFo(c, 1),
-             XFo(c, 2), VFo(c, 0), c->rpf[1], c->start, c->iscal);
+             VFo(c, 0), c->rpf[1], c->start, c->iscal);
     }
     return 0;
 }
@@ -313,7 +315,7 @@ int run_momentum(sph_ctx* c) {
 int run_vsmooth(sph_ctx* c) {
     KTime t(c, "velocity_smoothing");
     PAIRL(k_vsmooth, c->chunk_v, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1], c->rp[0], XF(c, 1), VF(c, 1),
-          c->nbr, c->nnb, c->vel[2], VFo(c, 2), c->xm[1], c->scal, c->dtmin, c->iscal);
+          c->nbr, c->nnb, c->vel[2], c->xm[1], c->scal, c->dtmin, c->iscal);
     return 0;
 }
 

@@ -474,7 +474,7 @@ __global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* 
                          const double4* __restrict__ acc_s, double4* __restrict__ xm_d,
                          double4* __restrict__ vel_d, double2* __restrict__ rp_d,
                          int* __restrict__ lam_d, double4* __restrict__ acc_d,
-                         int* __restrict__ cell_d, float4* __restrict__ xmf, F4Out xmf0, F4Out xmf1, F4Out xmf2,
+                         int* __restrict__ cell_d, float4* __restrict__ xmf, F4Out xmf0, F4Out xmf1,
                          F4Out velf,
                          float2* __restrict__ rpf, const int* __restrict__ start, int* iscal) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -505,10 +505,9 @@ __global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* 
     const double oz = P.extr_lo[2] + P.dcd[2] * (double)(ck - 1);
     const float4 xr = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
     xmf[p] = xr;
-    if (P.f32) {  // the position half of every FP32 record buffer
+    if (P.f32) {  // the position half of both FP32 record buffers
         xmf0[p] = xr;
         xmf1[p] = xr;
-        xmf2[p] = xr;
         velf[p] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)(r.x - P.den0));
         rpf[p] = make_float2((float)r.x, (float)r.y);
     }
@@ -2217,7 +2216,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
     const double4* __restrict__ vel1, const double2* __restrict__ rp, F4In xmf,
     F4In velf1, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
-    double4* __restrict__ vel_out, F4Out velf_out, double4* __restrict__ xm_out,
+    double4* __restrict__ vel_out, double4* __restrict__ xm_out,
     const double* scal, unsigned long long* dtmin, int* iscal) {
     using X = typename Rec<F32>::X;
     using V = typename Rec<F32>::V;
@@ -2330,7 +2329,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
             double4 vn = vi;
             if (smooth) vn = make_double4(vi.x + tv1 * v[0], vi.y + tv1 * v[1], vi.z + tv1 * v[2], vi.w);
             vel_out[i] = vn;
-            if (F32) velf_out[i] = make_float4((float)vn.x, (float)vn.y, (float)vn.z, (float)(rp[i].x - P.den0));
+
             xm_out[i] = make_double4(xi.x + dt * vn.x, xi.y + dt * vn.y, xi.z + dt * vn.z, xi.w);
             const unsigned long long b =
                 (unsigned long long)__double_as_longlong(d_dt_cfl(P, vn.x, vn.y, vn.z));
