@@ -7,7 +7,7 @@ both against the FP64 oracle after the same steps, and times the reference's
 own 2D build (flang OpenMP, through the dump harness) on the same case on the
 host cores.  One JSON line on stdout.
 
-    python -m sphera_amd.bench2d [--steps 1000]
+    python bench2d.py [--steps 1000]
 """
 from __future__ import annotations
 
@@ -20,8 +20,14 @@ import time
 
 import numpy as np
 
-from . import case2d
-from . import runtime as rt
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+from sphera_amd import case2d  # noqa: E402
+from sphera_amd import runtime as rt  # noqa: E402
 
 FIELDS = ("coord", "vel", "dens", "pres")
 

@@ -58,7 +58,24 @@ typedef struct sph_params {
                             2 = 128 x 1536 in FP32, 3 = 256 x 2816 in FP16 with 64
                             registers (8 waves per SIMD).  The lists are the same
                             in every shape (FP64 decides near the cut-off). */
+    int32_t options;     /* SPH_OPT_* flags of the deck (below) */
 } sph_params;
+
+/* Deck options the hot path branches on (input_any_t of
+ * src/Modules/Hybrid_allocation_module.f90:88-113,931; NumBEdges of
+ * Static_allocation_module.f90:220).  SPH_OPT_DENSITY_THRESHOLDS is built: the
+ * 0.9/1.1 den0 clamp of Leapfrog_continuity.f90:61-68.  The others are not
+ * built on the device: sph_ctx_create / sph_ctx_create2d / sph_batch_create
+ * refuse a context that sets them with status 95 (and so does a frontier with
+ * slip_coefficient_mode = 2, the SA-SPH wall function), rather than running
+ * the case without them. */
+enum {
+    SPH_OPT_DENSITY_THRESHOLDS = 1, /* density_thresholds == 1 */
+    SPH_OPT_ALE3 = 2,               /* ALE3 (ALE3_term_momentum.f90, Leapfrog_continuity.f90:70-86) */
+    SPH_OPT_C1 = 4,                 /* C1_BE or C1_divu (renormalisation, CalcVarLength.f90:285-324) */
+    SPH_OPT_KTGF = 8,               /* KTGF_config > 0 (mixture viscosity, erosion) */
+    SPH_OPT_CONVEX_EDGES = 16       /* NumBEdges > 0 (AddElasticBoundaryReaction_3D.f90:106-152) */
+};
 
 /* SA-SPH frontier faces in BFaceList order and the grid-cell/face incidence
  * (BoundaryFace, Tratto, Partz, GCBFPointers, GCBFVector, kerneltab,

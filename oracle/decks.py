@@ -32,7 +32,7 @@ def _f(x):
     return repr(float(x))
 
 
-def deck_text(case, itmax=1000000000, tmax=1000.0):
+def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
     s = case.scalars()
     v = case.vertices()
     lines = ["10.0.0", "##### TITLE #####", case.name, "##### END TITLE #####",
@@ -59,13 +59,15 @@ def deck_text(case, itmax=1000000000, tmax=1000.0):
               f"{_f(case.teta_p)} {_f(case.teta_v)} A",
               f"1.1 {_f(case.coefnmaxpartj)} 0",
               ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.",
-              f"1000000 {int(case.maxcloseboundfaces)} 100 100000000", "0",
+              f"1000000 {int(case.maxcloseboundfaces)} 100 100000000",
+              str(int(getattr(case, "density_thresholds", 0))),
               "##### end RUN PARAMETERS #####",
               "##### general physical properties #####",
               f"{_f(case.grav[0])} {_f(case.grav[1])} {_f(case.grav[2])}", _f(case.p_ref),
               "##### end general physical properties #####",
               "##### restart #####", "##### end restart #####",
-              "##### output regulation #####", "results time 100000.",
+              "##### output regulation #####",
+              f"results step {int(results_step)}" if results_step else "results time 100000.",
               "restart time 100000.", "print partial 100000000", "control time 100000.",
               "level time 100000. medium 1", "depth dt_out 100000.",
               "##### end output regulation #####",
@@ -113,7 +115,8 @@ def deck_text_2d(case, itmax=1000000000, tmax=1000.0):
               f"{_f(case.cfl)} {_f(case.vsc_coeff)} 1 1 {_f(case.pesodt)} .false.",
               f"{_f(case.teta_p)} {_f(case.teta_v)} A",
               f"1.1 {_f(case.coefnmaxpartj)} 0",
-              ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.", "0",
+              ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.",
+              str(int(getattr(case, "density_thresholds", 0))),
               "##### end RUN PARAMETERS #####",
               "##### general physical properties #####",
               f"{_f(case.grav[0])} {_f(case.grav[2])}", _f(case.p_ref),
@@ -134,13 +137,13 @@ def deck_text_2d(case, itmax=1000000000, tmax=1000.0):
     return "\n".join(lines) + "\n"
 
 
-def write_case_dir(case, root, itmax=1000000000):
+def write_case_dir(case, root, itmax=1000000000, results_step=None):
     """Lay out a run directory the reference can start from: it concatenates
     ./input/*/*.inp into <case>.inp (src/Main_algorithm/sphera.f90:222)."""
     root = Path(root)
     (root / "input" / "00_deck").mkdir(parents=True, exist_ok=True)
     text = deck_text_2d(case, itmax=itmax) if getattr(case, "dim", 3) == 2 else \
-        deck_text(case, itmax=itmax)
+        deck_text(case, itmax=itmax, results_step=results_step)
     (root / "input" / "00_deck" / "deck.inp").write_text(text)
     (root / "output").mkdir(exist_ok=True)
     (root / "restart").mkdir(exist_ok=True)
@@ -148,7 +151,7 @@ def write_case_dir(case, root, itmax=1000000000):
 
 
 def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
-                workdir=None, timeout=3600, exe=None, extra_env=None):
+                workdir=None, timeout=3600, exe=None, extra_env=None, ic_vel=None):
     """Run the reference harness (or ``exe``, e.g. the device drop-in harness
     HARNESS_HIP); returns the path of its dump file."""
     exe = Path(exe or (HARNESS2D if getattr(case, "dim", 3) == 2 else HARNESS))
@@ -170,12 +173,77 @@ def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
             fh.write(struct.pack("<i", len(ic_coord)))
             fh.write(np.ascontiguousarray(ic_coord, dtype="<f8").tobytes())
         env["SPH_HARNESS_IC"] = str(icf)
+    if ic_vel is not None:
+        ivf = wd / "icv.bin"
+        with open(ivf, "wb") as fh:
+            fh.write(struct.pack("<i", len(ic_vel)))
+            fh.write(np.ascontiguousarray(ic_vel, dtype="<f8").tobytes())
+        env["SPH_HARNESS_ICV"] = str(ivf)
     res = subprocess.run([str(exe), case.name], cwd=wd, env=env, capture_output=True,
                          text=True, timeout=timeout)
     if res.returncode != 0 or not (wd / "dump.bin").exists():
         raise RuntimeError(f"reference harness failed ({res.returncode}):\n{res.stdout[-3000:]}"
                            f"\n{res.stderr[-3000:]}")
     return wd / "dump.bin"
+
+
+def run_reference(case, itmax, results_step, workdir=None, threads=None, timeout=3600):
+    """Run the unmodified reference program (sphera3d.x, its own
+    time_step_loop) on the case's deck for itmax steps, writing results every
+    results_step steps; returns the run directory (the results are in
+    <case>.ris there)."""
+    if not REFERENCE.exists():
+        raise FileNotFoundError(f"{REFERENCE} not built (make -f oracle/Makefile.ref)")
+    wd = Path(workdir or tempfile.mkdtemp(prefix="sph_ris_")).resolve()
+    write_case_dir(case, wd, itmax=itmax, results_step=results_step)
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    res = subprocess.run([str(REFERENCE), case.name], cwd=wd, env=env, capture_output=True,
+                         text=True, timeout=timeout)
+    if res.returncode != 0 or not (wd / f"{case.name}.ris").exists():
+        raise RuntimeError(f"reference failed ({res.returncode}):\n{res.stdout[-3000:]}"
+                           f"\n{res.stderr[-3000:]}")
+    return wd
+
+
+def fortran_records(path):
+    """Records of a Fortran sequential unformatted file (4-byte little-endian
+    length markers before and after each record)."""
+    b = Path(path).read_bytes()
+    o, out = 0, []
+    while o < len(b):
+        (n,) = struct.unpack_from("<i", b, o)
+        rec = b[o + 4:o + 4 + n]
+        (m,) = struct.unpack_from("<i", b, o + 4 + n)
+        if m != n:
+            raise ValueError("bad record markers")
+        out.append(rec)
+        o += 8 + n
+    return out
+
+
+def read_ris(path):
+    """The visualisation records (restartcode = 0) of a reference results
+    file (Memo_Results.f90:279-287): per record it, time, dt, nag and the
+    per-particle coord (n, 3), vel (n, 3), pres, dens, mass, kin_visc, imed,
+    icol.  Full-state records (restartcode = 1, the first and last step) are
+    skipped."""
+    recs = fortran_records(path)
+    out = []
+    for k, r in enumerate(recs):
+        if len(r) != 36 or k + 1 >= len(recs):
+            continue
+        it, t, dt, nag, code, _ = struct.unpack("<iddiid", r)
+        if code != 0 or len(recs[k + 1]) != nag * (10 * 8 + 2 * 4):
+            continue
+        d = recs[k + 1]
+        f = np.frombuffer(d, "<f8", 10 * nag).reshape(10, nag)
+        i = np.frombuffer(d, "<i4", 2 * nag, offset=80 * nag).reshape(2, nag)
+        out.append(dict(it=it, time=t, dt=dt, nag=nag, coord=f[0:3].T.copy(), vel=f[3:6].T.copy(),
+                        pres=f[6].copy(), dens=f[7].copy(), mass=f[8].copy(),
+                        kin_visc=f[9].copy(), imed=i[0].copy(), icol=i[1].copy()))
+    return out
 
 
 def read_pelo(workdir, case):

@@ -26,7 +26,7 @@ class OrParams(C.Structure):
                 ("eps", C.c_double), ("celerita", C.c_double), ("alfamon", C.c_double),
                 ("kin_visc", C.c_double), ("mumx", C.c_double), ("cfl", C.c_double),
                 ("vsc_coeff", C.c_double), ("pesodt", C.c_double), ("tetap", C.c_double),
-                ("tetav", C.c_double), ("semi", C.c_int32)]
+                ("tetav", C.c_double), ("semi", C.c_int32), ("density_thresholds", C.c_int32)]
 
 
 class OrFaces(C.Structure):
@@ -66,6 +66,8 @@ def lib():
         L.or_step.argtypes = [C.c_void_p, C.c_int]
         L.or_pair_count.argtypes = [C.c_void_p]
         L.or_pair_count.restype = C.c_int64
+        L.or_branch_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.or_branch_counts.restype = None
         L.or_set_state.argtypes = [C.c_void_p, C.c_int, PD, PD, PD, PD, PD, PD, PI, C.c_double,
                                    C.c_double, C.c_double, C.c_int]
         L.or_get_state.argtypes = [C.c_void_p] + [PD] * 8
@@ -103,6 +105,7 @@ def params_from_tables(tb, n, semi=True):
     p.cfl, p.vsc_coeff, p.pesodt = case.cfl, case.vsc_coeff, case.pesodt
     p.tetap, p.tetav = case.teta_p, case.teta_v
     p.semi = 1 if semi else 0
+    p.density_thresholds = int(getattr(case, "density_thresholds", 0))
     return p
 
 
@@ -238,3 +241,10 @@ class Oracle:
 
     def pair_count(self):
         return int(lib().or_pair_count(self.ctx))
+
+    def branch_counts(self):
+        """How often the rare branches ran: elastic wall reactions applied,
+        zpmin clamps of the SA-SPH volume integrals, density_thresholds clamps."""
+        v = (C.c_int64 * 3)()
+        lib().or_branch_counts(self.ctx, v)
+        return dict(elastic=int(v[0]), zpmin=int(v[1]), density_clamp=int(v[2]))

@@ -18,6 +18,9 @@
 !   SPH_HARNESS_IC     optional file: int32 n, then n*3 float64 coordinates
 !                      (x,y,z per particle, pg order).  They replace the
 !                      generated positions before the pre-loop (jittered ICs).
+!   SPH_HARNESS_ICV    optional file: int32 n, then n*3 float64 velocities
+!                      (pg order); they replace the initial vel and var
+!                      (near-wall cases that approach a frontier at once).
 !   SPH_HARNESS_LISTS  0 to omit neighbour lists / boundary tables from dumps
 !   SPH_HARNESS_PELO   1: at every dumped step also run the reference's
 !                      free-surface monitor (CalcVarp, then calc_pelo, as
@@ -70,6 +73,26 @@ if (len_trim(aux)>0) read(aux,*) lists
 pelo = 0
 call get_environment_variable("SPH_HARNESS_PELO",aux)
 if (len_trim(aux)>0) read(aux,*) pelo
+! Optional replacement of the particle velocities (pg order)
+call get_environment_variable("SPH_HARNESS_ICV",aux)
+if (len_trim(aux)>0) then
+   open(newunit=u,file=trim(aux),access="stream",form="unformatted",          &
+      status="old",iostat=ios)
+   if (ios/=0) stop 3
+   read(u) n_ic
+   if (n_ic/=nag) then
+      write(*,*) "harness: ICV file has",n_ic," particles, deck has",nag
+      stop 3
+   endif
+   allocate(xyz(3*nag))
+   read(u) xyz
+   close(u)
+   do npi=1,nag
+      pg(npi)%vel(1:3) = xyz(3*npi-2:3*npi)
+      pg(npi)%var(1:3) = pg(npi)%vel(1:3)
+   enddo
+   deallocate(xyz)
+endif
 ! Optional replacement of the particle positions (same pg order)
 call get_environment_variable("SPH_HARNESS_IC",aux)
 if (len_trim(aux)>0) then

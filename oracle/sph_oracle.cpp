@@ -5,6 +5,7 @@
 // Particle order, loop order and the left-to-right evaluation of every FP
 // expression follow the cited reference lines.  Indices are 0-based here; the
 // reference's 1-based cell ids are kept as such (0 = outside the grid).
+#include <atomic>
 #include "sph_oracle.h"
 
 #include <algorithm>
@@ -53,6 +54,10 @@ struct Ctx {
     int on_going = -999;
     int overflow = 0;
     bool lists_built = false;
+    // how often the rare branches ran (checker counters, read by or_branch_counts):
+    // elastic wall reactions applied, zpmin clamps of the volume integrals,
+    // density_thresholds clamps
+    mutable std::atomic<long long> n_elastic{0}, n_zpmin{0}, n_dclamp{0};
 };
 
 inline double Tm(const Ctx& c, int f, int r, int col) { return c.T[9 * f + col * 3 + r]; }
@@ -147,7 +152,10 @@ void volume_integrals(const Ctx& c, int f, const double locx[3], double bi[8], d
     const double zpmin = 0.05 * c.p.h;
     double LocPi[3] = {locx[0], locx[1], locx[2]};
     if (LocPi[2] < 0.0) return;
-    if (LocPi[2] < zpmin) LocPi[2] = zpmin;
+    if (LocPi[2] < zpmin) {
+        LocPi[2] = zpmin;
+        c.n_zpmin.fetch_add(1, std::memory_order_relaxed);
+    }
     double robpre = 2.1;
     for (int ipk = 0; ipk < 256; ++ipk) {
         double LocPj[3];
@@ -1145,6 +1153,7 @@ int or_momentum(void* ctx) {
                     if (vin < 0.0) {
                         const double nr = -1.0 * celer02 * std::log((h + ypi - ypimin) / h) / h;
                         for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * ST(c, sd, d, 2);
+                        c.n_elastic.fetch_add(1, std::memory_order_relaxed);
                     }
                 }
             }
@@ -1167,6 +1176,7 @@ int or_momentum(void* ctx) {
                     if (vin < 0.0) {
                         const double nr = -1.0 * celer02 * std::log((h + zi - zimin) / h) / h;
                         for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * Tm(c, f, d, 2);
+                        c.n_elastic.fetch_add(1, std::memory_order_relaxed);
                     }
                 }
             }
@@ -1374,11 +1384,31 @@ int or_continuity(void* ctx) {
     return bad ? 94 : 0;
 }
 
-// Leapfrog_continuity.f90:51-60
+// Leapfrog_continuity.f90:51-68 (semi: no division by the Shepard
+// coefficient; the density_thresholds clamp when the deck sets it)
 int or_leapfrog_cont(void* ctx) {
     Ctx& c = *(Ctx*)ctx;
-    for (int i = 0; i < c.n; ++i) c.dens[i] = c.dens[i] + c.dt * c.dden[i];
+    for (int i = 0; i < c.n; ++i) {
+        c.dens[i] = c.dens[i] + c.dt * c.dden[i];
+        if (c.p.density_thresholds == 1) {
+            if (c.dens[i] < (0.9 * c.p.den0)) {
+                c.dens[i] = 0.9 * c.p.den0;
+                c.n_dclamp.fetch_add(1, std::memory_order_relaxed);
+            }
+            if (c.dens[i] > (1.1 * c.p.den0)) {
+                c.dens[i] = 1.1 * c.p.den0;
+                c.n_dclamp.fetch_add(1, std::memory_order_relaxed);
+            }
+        }
+    }
     return 0;
+}
+
+void or_branch_counts(void* ctx, int64_t* out3) {
+    const Ctx& c = *(const Ctx*)ctx;
+    out3[0] = c.n_elastic.load();
+    out3[1] = c.n_zpmin.load();
+    out3[2] = c.n_dclamp.load();
 }
 
 // EoS_fluid_loop.f90:59-70, EoS_barotropic_linear.f90:49-55

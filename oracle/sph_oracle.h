@@ -27,6 +27,7 @@ typedef struct or_params {
     double den0, eps, celerita, alfamon, kin_visc, mumx;
     double cfl, vsc_coeff, pesodt, tetap, tetav;
     int32_t semi;           /* 1: SA-SPH walls active */
+    int32_t density_thresholds; /* input_any_t%density_thresholds (1: 0.9/1.1 den0 clamp) */
 } or_params;
 
 /* nf faces: T[9*f] column-major (T(r,c) at [c*3+r]), LX[8*f] (4 nodes x 2),
@@ -90,6 +91,11 @@ int or_psmooth(void* ctx);          /* PressureSmoothing_3D (2D: _2D) */
 int or_preloop(void* ctx);          /* lists, boundary table, first dt */
 int or_step(void* ctx, int nsteps); /* whole leapfrog time steps */
 int64_t or_pair_count(void* ctx);   /* sum_i (nPartIntorno(i)-1) of the current lists */
+/* how often the rare branches ran since or_create: [0] elastic wall reactions
+ * applied (AddElasticBoundaryReaction_3D/2D, vin < 0 below 0.25 dx), [1] zpmin
+ * clamps (ComputeBoundaryVolumeIntegrals_P0.f90:84), [2] density_thresholds
+ * clamps (Leapfrog_continuity.f90:61-68) */
+void or_branch_counts(void* ctx, int64_t* out3);
 int or_overflow(void* ctx);         /* particles whose list was truncated */
 
 /* calc_pelo: free-surface position (pelolib) and lower-fluid-top height (zlft)

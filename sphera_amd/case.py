@@ -52,6 +52,7 @@ class DamBreak3D:
     coefnmaxpartj: float = 2.0
     maxcloseboundfaces: int = 100
     slip_coeff: float = 0.0       # slip_coefficient_mode=1 input value
+    density_thresholds: int = 0   # 1: Leapfrog_continuity's 0.9/1.1 rho0 clamp
     name: str = "dam_break_3D"
     # monitoring lines of the deck's "control lines" section:
     # (label, first point (x,y,z), second point, number of points)

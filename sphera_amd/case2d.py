@@ -42,6 +42,7 @@ class DamBreak2D:
     teta_v: float = 0.2
     coefnmaxpartj: float = 2.0
     slip_coeff: float = 0.0
+    density_thresholds: int = 0      # 1: Leapfrog_continuity's 0.9/1.1 rho0 clamp
     name: str = "dam_break_2D"
     control_lines: tuple = ()
     dim: int = 2

@@ -439,6 +439,32 @@ int sph_ctx::alloc() {
 
 namespace {
 
+// Deck options the device does not build: status 95 names the first one set
+// (the reference would run them; a device run without them would differ).
+int check_options(int32_t opt) {
+    static const struct { int32_t bit; const char* what; } NOT_BUILT[] = {
+        {SPH_OPT_ALE3, "ALE3 (ALE3_term_momentum.f90, Leapfrog_continuity.f90:70-86)"},
+        {SPH_OPT_C1, "C1_BE / C1_divu renormalisation (CalcVarLength.f90:285-324)"},
+        {SPH_OPT_KTGF, "KTGF_config > 0 (mixture viscosity and erosion)"},
+        {SPH_OPT_CONVEX_EDGES, "convex frontier edges (AddElasticBoundaryReaction_3D.f90:106-152)"},
+    };
+    const int32_t known = SPH_OPT_DENSITY_THRESHOLDS | SPH_OPT_ALE3 | SPH_OPT_C1 | SPH_OPT_KTGF |
+                          SPH_OPT_CONVEX_EDGES;
+    if (opt & ~known) return fail(2, "unknown option bits " + std::to_string(opt & ~known));
+    for (const auto& o : NOT_BUILT)
+        if (opt & o.bit) return fail(95, std::string("not built on the device: ") + o.what);
+    return 0;
+}
+
+// slip_coefficient_mode = 2 on a fixed or tapis frontier selects the SA-SPH
+// wall function (AddBoundaryContributions_to_ME3D.f90:207-230), not built.
+int check_slip_mode(int mode, int btype) {
+    if (mode == 2 && (btype == 0 || btype == 1))
+        return fail(95, "not built on the device: slip_coefficient_mode = 2 (SA-SPH wall function, "
+                        "wall_function_for_SASPH.f90)");
+    return 0;
+}
+
 // Validates one solve's parameters and fills its constants (placement fields
 // zero).  The reference's values: ReadInput.f90:198-205 (h, 2h, eta, eta2),
 // Gest_Input.f90:266-289 (kernel constants).
@@ -451,6 +477,10 @@ int fill_dp(const sph_params* p, const sph_faces* f, DP& P, bool dry = false) {
     if (p->semi && !dry && (!f || f->nf <= 0)) return fail(2, "SA-SPH requested without frontier faces");
     const long long nmax = (long long)p->ncd[0] * p->ncd[1] * p->ncd[2];
     if (nmax >= (1LL << 30)) return fail(2, "grid too large");
+    if (int rc = check_options(p->options)) return rc;
+    if (f && f->nf > 0 && (!f->slip_mode || !f->btype)) return fail(1, "null frontier table");
+    for (int q = 0; f && q < f->nf; ++q)
+        if (int rc = check_slip_mode(f->slip_mode[q], f->btype[q])) return rc;
     P = DP{};
     P.cap = p->capacity;
     P.scap = p->capacity;
@@ -458,6 +488,7 @@ int fill_dp(const sph_params* p, const sph_faces* f, DP& P, bool dry = false) {
     P.nmax = (int)nmax;
     P.semi = p->semi ? 1 : 0;
     P.f32 = p->precision == SPH_FP32_MIXED;
+    P.dthr = (p->options & SPH_OPT_DENSITY_THRESHOLDS) ? 1 : 0;
     P.nf = f ? f->nf : 0;
     P.dim = 3;
     for (int d = 0; d < 3; ++d) {
@@ -792,6 +823,9 @@ int sph_ctx_create2d(const sph_params* p, const sph_sides2d* sd, sph_ctx** out) 
     if (!p || !out) return fail(1, "null argument");
     if (p->ncd[1] != 1) return fail(2, "a 2D grid has one cell row in y (ncd[1] = 1)");
     if (p->semi && (!sd || sd->ns <= 0)) return fail(2, "SA-SPH requested without boundary sides");
+    if (int rc = check_options(p->options)) return rc;
+    for (int k = 0; sd && k < sd->ns; ++k)
+        if (int rc = check_slip_mode(sd->slip_mode[k], sd->btype[k])) return rc;
     sph_params q = *p;
     q.semi = 0;
     int rc = sph_ctx_create(&q, nullptr, out);
@@ -831,6 +865,7 @@ int sph_ctx_create2d(const sph_params* p, const sph_sides2d* sd, sph_ctx** out) 
                 *out = nullptr;
                 return fail(2, "previous side out of range");
             }
+
         }
         // the context was created without SA-SPH faces, so its boundary
         // table holds one record: give it the SPH_MB slots per particle that

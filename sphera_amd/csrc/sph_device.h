@@ -53,6 +53,9 @@ struct DP {
     // (x, y, z, rho - den0) record per neighbour
     int umass;
     float f_mass;
+    // density_thresholds == 1: Leapfrog_continuity.f90:61-68 clamps the
+    // integrated density to [0.9, 1.1] den0
+    int dthr;
     // 3, or 2 for the x-z plane form (-DSPACE_2D build of the reference): one
     // cell row in y (every particle in row j = 1), boundary sides for faces
     int dim;

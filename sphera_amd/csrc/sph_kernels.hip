@@ -2343,7 +2343,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
 // ----------------------------------------------------------------------------
 // continuity: SASPH_continuity.f90:89-169 with AddBoundaryContribution_to_CE3D
 // .f90:82-169 (FP64), Continuity_Equation.f90:97-235; fused
-// Leapfrog_continuity.f90:51-60 and EoS_fluid_loop.f90:59-70
+// Leapfrog_continuity.f90:51-68 (with the density_thresholds clamp) and EoS_fluid_loop.f90:59-70
 // ----------------------------------------------------------------------------
 template <bool F32, int CH, bool SG>
 __global__ __launch_bounds__(256) void k_continuity(
@@ -2469,7 +2469,11 @@ __global__ __launch_bounds__(256) void k_continuity(
             }
             const double dden = dd - inner;
             dden_out[i] = dden;
-            const double rho1 = rhoi + dt * dden;
+            double rho1 = rhoi + dt * dden;
+            if (P.dthr) {  // Leapfrog_continuity.f90:61-68
+                if (rho1 < 0.9 * P.den0) rho1 = 0.9 * P.den0;
+                if (rho1 > 1.1 * P.den0) rho1 = 1.1 * P.den0;
+            }
             const double p1 = eos ? P.eps / P.den0 * (rho1 - P.den0) + P.prif : rpi.y;
             rp_out[i] = make_double2(rho1, p1);
             if (F32) {

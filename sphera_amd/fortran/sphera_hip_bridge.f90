@@ -116,6 +116,16 @@ p%chunk_v = 0
 p%chunk_c = 0
 p%chunk_p = 0
 p%nb_shape = 0
+! Deck options the hot path branches on (SPH_OPT_* of sphera_hip.h): the
+! density clamp is built; the device refuses the others with status 95
+p%options = 0
+if (input_any_t%density_thresholds==1) p%options = ior(p%options,1)
+if (input_any_t%ALE3) p%options = ior(p%options,2)
+if (input_any_t%C1_BE.or.input_any_t%C1_divu) p%options = ior(p%options,4)
+if (Granular_flows_options%KTGF_config>0) p%options = ior(p%options,8)
+#ifdef SPACE_3D
+if (NumBEdges>0) p%options = ior(p%options,16)
+#endif
 end subroutine pack_params
 
 #ifdef SPACE_2D

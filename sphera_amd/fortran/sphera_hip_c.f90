@@ -23,6 +23,7 @@ type,bind(C) :: sph_params_t
    integer(c_int32_t) :: chunk
    integer(c_int32_t) :: chunk_v, chunk_c, chunk_p
    integer(c_int32_t) :: nb_shape
+   integer(c_int32_t) :: options
 end type sph_params_t
 
 type,bind(C) :: sph_faces_t

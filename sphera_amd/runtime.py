@@ -19,6 +19,8 @@ LIB_PATH = Path(os.environ.get("SPHERA_HIP_LIB") or
                 Path(__file__).resolve().parent / "_build" / "libsphera_hip.so")
 
 FP64, FP32_MIXED = 0, 1
+# deck options (SPH_OPT_* of sphera_hip.h)
+OPT_DENSITY_THRESHOLDS, OPT_ALE3, OPT_C1, OPT_KTGF, OPT_CONVEX_EDGES = 1, 2, 4, 8, 16
 PD = C.POINTER(C.c_double)
 PI = C.POINTER(C.c_int32)
 
@@ -50,7 +52,7 @@ class SphParams(C.Structure):
                 ("vsc_coeff", C.c_double), ("pesodt", C.c_double), ("tetap", C.c_double),
                 ("tetav", C.c_double), ("semi", C.c_int32), ("wg_size", C.c_int32),
                 ("chunk", C.c_int32), ("chunk_v", C.c_int32), ("chunk_c", C.c_int32),
-                ("chunk_p", C.c_int32), ("nb_shape", C.c_int32)]
+                ("chunk_p", C.c_int32), ("nb_shape", C.c_int32), ("options", C.c_int32)]
 
 
 class SphFaces(C.Structure):
@@ -195,6 +197,7 @@ def params_struct(tb, precision=FP64, capacity=None, semi=True, tune=None):
     p.cfl, p.vsc_coeff, p.pesodt = case.cfl, case.vsc_coeff, case.pesodt
     p.tetap, p.tetav = case.teta_p, case.teta_v
     p.semi = 1 if semi else 0
+    p.options = OPT_DENSITY_THRESHOLDS if int(getattr(case, "density_thresholds", 0)) == 1 else 0
     for k, v in (tune or {}).items():
         setattr(p, k, int(v))
     return p

@@ -5,7 +5,8 @@ from /root/reference/src) on project-authored decks.
 
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
-With names (tiny, small_jitter, removal, pelo, 2d) only those fixtures are written.
+With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp)
+only those fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -25,6 +26,19 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         seed 20260303; boundary sides of the header; state,
                         neighbour lists and 2D boundary table after the
                         pre-loop and after 10 steps
+  ref_nearwall.npz      800 particles; the layers next to the floor and to the
+                        x = 0 wall moved to 0.02-0.22 dx from it and given
+                        velocities into it (seed 20261203): the elastic wall
+                        reaction (below 0.25 dx, approaching) and the zpmin
+                        clamp of the volume integrals (below 0.05 h) run;
+                        state, lists and boundary table after 0, 4 and 8 steps
+  ref_nearwall_2d.npz   the same for the 2D build (600 particles, floor and
+                        x = 0 side), after 0, 4 and 8 steps
+  ref_dclamp.npz        1440 particles with density_thresholds = 1 and a
+                        velocity field converging on the left half of the
+                        column and diverging on the right (150/s, jittered
+                        lattice): the 0.9/1.1 rho0 density clamp runs; state
+                        after 0, 3 and 6 steps
 """
 from __future__ import annotations
 
@@ -195,8 +209,80 @@ def make_2d():
     np.savez_compressed(HERE / "ref_2d.npz", **out)
 
 
+def nearwall_ic(coord, dx, seed=20261203, dim=3):
+    """Particles of the layer at the floor (z < dx) and of the layer at the x = 0
+    wall (x < dx) moved to U(0.02, 0.22) dx from the wall, with velocities
+    U(0, 0.05) m/s into it."""
+    rng = np.random.default_rng(seed)
+    c = coord.copy()
+    v = np.zeros_like(c)
+    bot = c[:, 2] < dx
+    side = c[:, 0] < dx
+    c[bot, 2] = rng.uniform(0.02, 0.22, bot.sum()) * dx
+    c[side, 0] = rng.uniform(0.02, 0.22, side.sum()) * dx
+    v[bot, 2] = -rng.uniform(0.0, 0.05, bot.sum())
+    v[side, 0] = -rng.uniform(0.0, 0.05, side.sum())
+    return c, v
+
+
+def dclamp_vel(coord, rate=150.0):
+    """Converging on the left half of the column (x below its mean), diverging
+    on the right: |div v| = 3 rate."""
+    ctr = coord.mean(axis=0)
+    d = coord - ctr
+    sgn = np.where(d[:, 0] < 0.0, -1.0, 1.0)[:, None]
+    return sgn * rate * d
+
+
+def _pack_steps(steps, out, lists):
+    for st in steps:
+        pre = f"s{st['it']}_"
+        out[pre + "nag"] = st["nag"]
+        pack_step(pre, st, out, lists=lists)
+
+
+def make_nearwall():
+    """3D near-wall: elastic reaction and zpmin clamp"""
+    case = DamBreak3D(**TINY)
+    tb = build(case)
+    ic, iv = nearwall_ic(tb.particles["coord"], case.dx)
+    hdr, steps = read_dump(run_harness(case, nsteps=8, every=4, ic_coord=ic, ic_vel=iv))
+    out = {"ic_coord": ic, "ic_vel": iv}
+    _pack_steps(steps, out, lists=True)
+    np.savez_compressed(HERE / "ref_nearwall.npz", **out)
+
+
+def make_nearwall_2d():
+    """2D near-wall: elastic reaction of the 2D build"""
+    case = case2d.DamBreak2D(**CASE2D)
+    tb = case2d.build(case)
+    ic, iv = nearwall_ic(tb.particles["coord"], case.dx, dim=2)
+    hdr, steps = read_dump(run_harness(case, nsteps=8, every=4, lists=False, ic_coord=ic, ic_vel=iv))
+    out = {"ic_coord": ic, "ic_vel": iv}
+    for st in steps:
+        pre = f"s{st['it']}_"
+        for k in ("it", "time", "dt", "dt_average", "nag"):
+            out[pre + k] = st[k]
+        for f in FIELDS + ("var", "laminar_flag"):
+            out[pre + f] = st[f]
+    np.savez_compressed(HERE / "ref_nearwall_2d.npz", **out)
+
+
+def make_dclamp():
+    """density_thresholds = 1: the density clamp"""
+    case = DamBreak3D(**SMALL, density_thresholds=1)
+    tb = build(case)
+    ic = jitter(tb.particles["coord"], case.dx)
+    iv = dclamp_vel(ic)
+    hdr, steps = read_dump(run_harness(case, nsteps=6, every=3, lists=False, ic_coord=ic, ic_vel=iv))
+    out = {"ic_coord": ic, "ic_vel": iv}
+    _pack_steps(steps, out, lists=False)
+    np.savez_compressed(HERE / "ref_dclamp.npz", **out)
+
+
 MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_removal,
-          "pelo": make_pelo, "2d": make_2d}
+          "pelo": make_pelo, "2d": make_2d, "nearwall": make_nearwall,
+          "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp}
 
 
 def main():

@@ -2,9 +2,12 @@
 flang-built reference, tests/test_oracle_reference.py).
 
 Bars: neighbour lists identical (as ordered lists, hence also as sorted sets);
-plain FP64 fields identical to the oracle (max |diff| == 0 is asserted where
-the arithmetic is the reference's own; see FP64_REL_TOL); FP32-mixed fields
-within the per-field relative tolerances in FP32_TOL.
+plain FP64 fields, dt and pair counts identical to the oracle (np.array_equal);
+FP32-mixed fields within FP32_TOL of the FP64 oracle, as the largest
+per-particle difference relative to the field's largest |value|.
+The one exception to FP64 equality is the elastic wall reaction, whose log()
+is the device's, not glibc's (the reference's and the oracle's): the
+near-wall test bounds that difference instead (FP64_LOG_TOL).
 """
 import numpy as np
 import pytest
@@ -14,14 +17,13 @@ from sphera_amd.case import DamBreak3D, build
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
-# FP64 device vs oracle.  Both evaluate the same expressions in the same order
-# with separately rounded IEEE operations; the only libm call on the path is
-# log() in the elastic wall reaction, hence a tolerance instead of equality.
-FP64_REL_TOL = 1e-13
-# FP32-mixed vs FP64 oracle after the stated number of steps, relative to the
-# field's scale (max |value| over particles; dden relative to rho0/step-size)
-FP32_TOL = {"coord": 1e-6, "vel": 2e-3, "acc": 2e-3, "dens": 1e-6, "pres": 2e-3,
-            "dden": 2e-3, "vpres": 1e-2}
+# FP64 device vs oracle where the elastic wall reaction runs: its log() is
+# the device library's (correctly rounded in most cases, as is glibc's, but
+# the two differ in the last bit for a fraction of a percent of arguments)
+FP64_LOG_TOL = 1e-12
+# FP32-mixed vs FP64 oracle: the largest per-particle |difference| relative
+# to the field's largest |value|
+FP32_TOL = {f: 1e-4 for f in FIELDS}
 
 
 def _rt():
@@ -81,13 +83,12 @@ def test_fp64_lists_and_steps_match_oracle(which, tiny_case, small_case):
         assert np.array_equal(ds["ids"], np.arange(len(ds["ids"])))
         assert ds["dt"] == os_["dt"], (ds["dt"], os_["dt"])
         for f in FIELDS:
-            r = _rel(ds[f], os_[f])
-            assert r <= FP64_REL_TOL, f"{f} rel diff {r:.3e} after it={ds['it']}"
+            assert np.array_equal(ds[f], os_[f]), f"{f} rel diff {_rel(ds[f], os_[f]):.3e} after it={ds['it']}"
         _lists_equal(d, o)
 
 
-def test_fp64_bitexact_summary(small_case):
-    """Records how many fields are bit-identical after 20 steps (printed)."""
+def test_fp64_bitexact_20_steps(small_case):
+    """Every FP64 field, dt and time bit-identical to the oracle after 20 steps."""
     tb = build(small_case)
     coord = _jittered(tb)
     o = _oracle(tb, coord)
@@ -97,7 +98,8 @@ def test_fp64_bitexact_summary(small_case):
     ds, os_ = d.state(), o.state()
     exact = {f: bool(np.array_equal(ds[f], os_[f])) for f in FIELDS}
     print("[gpu] fp64 bit-identical after 20 steps:", exact, flush=True)
-    assert all(_rel(ds[f], os_[f]) <= FP64_REL_TOL for f in FIELDS)
+    assert all(exact.values()), exact
+    assert ds["dt"] == os_["dt"] and ds["time"] == os_["time"]
 
 
 def test_fp32_mixed_within_tolerance(small_case):
@@ -329,8 +331,8 @@ def _device_btab_at_start(tb, coord):
 def test_full_size_dam_break_matches_oracle():
     """The bench workload itself (576,000 particles, jittered lattice, seed
     20260101): after 5 steps the FP64 device fields, dt and pair count equal
-    the oracle's (FP64_REL_TOL), and the FP32-mixed fields are within
-    FP32_TOL of them."""
+    the oracle's bit for bit, and the FP32-mixed fields are within FP32_TOL
+    of them."""
     from sphera_amd.case import DamBreak3D
     tb = build(DamBreak3D(tank=(1.6, 0.6, 0.6), column=(0.4, 0.6, 0.3), dx=0.005, name="dam_break_3D"))
     coord = _jittered(tb)
@@ -347,8 +349,148 @@ def test_full_size_dam_break_matches_oracle():
         assert ds["nag"] == len(coord)
         assert ds["pairs"] == o.pair_count()
         for f in FIELDS:
-            r = _rel(ds[f], os_[f])
-            tol = FP64_REL_TOL if prec == rt.FP64 else FP32_TOL[f]
-            assert r <= tol, f"precision {prec}: {f} rel diff {r:.3e}"
+            if prec == rt.FP64:
+                assert np.array_equal(ds[f], os_[f]), f"FP64 {f} rel diff {_rel(ds[f], os_[f]):.3e}"
+            else:
+                r = _rel(ds[f], os_[f])
+                assert r <= FP32_TOL[f], f"FP32-mixed {f} rel diff {r:.3e}"
         if prec == rt.FP64:
             assert ds["dt"] == os_["dt"]
+
+
+GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+def _gold_device(tb, g, precision):
+    rt = _rt()
+    d = rt.SphContext(tb, precision=precision)
+    d.upload(coord=g["ic_coord"], vel=g["ic_vel"])
+    d.preloop()
+    return d
+
+
+def test_nearwall_fixture():
+    """The reference's near-wall fixture (tests/golden/ref_nearwall.npz:
+    particles 0.02-0.22 dx from the floor and the x = 0 wall, moving into
+    them), where the elastic reaction (AddElasticBoundaryReaction_3D.f90:
+    67-104) and the zpmin clamp (ComputeBoundaryVolumeIntegrals_P0.f90:84,
+    143) run.  Device FP64: lists and the boundary table (with the clamped
+    integrals) bit-identical at the pre-loop; fields after 4 and 8 steps
+    bit-identical except through the elastic term's log(), bounded by
+    FP64_LOG_TOL (the number of bit-identical fields is printed).  FP32-mixed
+    within FP32_TOL of the reference."""
+    rt = _rt()
+    g = np.load(GOLD / "ref_nearwall.npz")
+    tb = build(DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny"))
+    print(f"[gpu] nearwall n={len(g['ic_coord'])}", flush=True)
+    d = _gold_device(tb, g, rt.FP64)
+    dc, di = d.lists()
+    off = np.concatenate([[0], np.cumsum(g["s0_nbr_count"])])
+    assert np.array_equal(dc, g["s0_nbr_count"])
+    for i in range(len(dc)):
+        assert np.array_equal(di[i, :dc[i]] + 1, g["s0_nbr"][off[i]:off[i + 1]]), i
+    nc, face, loc, bi, ig = d.btab()
+    assert np.array_equal(nc, g["s0_ncbf"])
+    m = np.arange(face.shape[1])[None, :] < nc[:, None]
+    assert np.array_equal(tb.faces.face_id[face[m]], g["s0_bt_face"])
+    for x, name in ((loc, "loc"), (bi, "bi"), (ig, "ig")):
+        assert np.array_equal(x[m], g["s0_bt_" + name]), name
+    ds = d.state()
+    for f in FIELDS:
+        assert np.array_equal(ds[f], g["s0_" + f]), f
+    f32 = _gold_device(tb, g, rt.FP32_MIXED)
+    for it in (4, 8):
+        d.step(4)
+        f32.step(4)
+        ds, fs = d.state(), f32.state()
+        assert ds["nag"] == g[f"s{it}_nag"]
+        exact = {f: bool(np.array_equal(ds[f], g[f"s{it}_" + f])) for f in FIELDS}
+        e64 = {f: _rel(ds[f], g[f"s{it}_" + f]) for f in FIELDS}
+        e32 = {f: _rel(fs[f], g[f"s{it}_" + f]) for f in FIELDS}
+        print(f"[gpu] nearwall it={it} fp64 bit-identical {exact}; fp64 max rel "
+              f"{max(e64.values()):.2e}; fp32 {{{', '.join(f'{k}: {v:.1e}' for k, v in e32.items())}}}",
+              flush=True)
+        for f in FIELDS:
+            assert e64[f] <= FP64_LOG_TOL, (it, f, e64[f])
+            assert e32[f] <= FP32_TOL[f], (it, f, e32[f])
+
+
+def test_density_clamp_fixture():
+    """density_thresholds = 1 (Leapfrog_continuity.f90:61-68) on the
+    reference's fixture (tests/golden/ref_dclamp.npz: converging and
+    diverging halves, the clamp runs at both bounds): FP64 bit-identical to
+    the reference after 3 and 6 steps, FP32-mixed within FP32_TOL."""
+    rt = _rt()
+    g = np.load(GOLD / "ref_dclamp.npz")
+    case = DamBreak3D(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), dx=0.01, name="small",
+                      density_thresholds=1)
+    tb = build(case)
+    print(f"[gpu] density clamp n={len(g['ic_coord'])}", flush=True)
+    d = _gold_device(tb, g, rt.FP64)
+    f32 = _gold_device(tb, g, rt.FP32_MIXED)
+    for it in (3, 6):
+        d.step(3)
+        f32.step(3)
+        ds, fs = d.state(), f32.state()
+        assert ds["dt"] == g[f"s{it}_dt"]
+        for f in FIELDS:
+            assert np.array_equal(ds[f], g[f"s{it}_" + f]), (it, f, _rel(ds[f], g[f"s{it}_" + f]))
+        e32 = {f: _rel(fs[f], g[f"s{it}_" + f]) for f in FIELDS}
+        print(f"[gpu] density clamp it={it} fp32 {e32}", flush=True)
+        for f in FIELDS:
+            assert e32[f] <= FP32_TOL[f], (it, f, e32[f])
+
+
+def test_particle_masses_differ():
+    """Per-particle masses (1% jitter): the pressure smoothing's two-record
+    form (the solve has no single mass).  FP64 bit-identical to the oracle
+    after 10 steps, FP32-mixed within FP32_TOL."""
+    from oracle.pyoracle import Oracle
+    rt = _rt()
+    tb = build(DamBreak3D(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), dx=0.01, name="small"))
+    coord = _jittered(tb)
+    rng = np.random.default_rng(11)
+    mass = tb.particles["mass"] * (1.0 + rng.uniform(-0.01, 0.01, len(coord)))
+    print(f"[gpu] masses n={len(coord)}", flush=True)
+    o = Oracle(tb, coord=coord, mass=mass)
+    o.preloop()
+    o.step(10)
+    os_ = o.state()
+    for prec in (rt.FP64, rt.FP32_MIXED):
+        d = rt.SphContext(tb, precision=prec)
+        d.upload(coord=coord, mass=mass)
+        d.preloop()
+        d.step(10)
+        ds = d.state()
+        d.close()
+        for f in FIELDS:
+            if prec == rt.FP64:
+                assert np.array_equal(ds[f], os_[f]), (f, _rel(ds[f], os_[f]))
+            else:
+                assert _rel(ds[f], os_[f]) <= FP32_TOL[f], (f, _rel(ds[f], os_[f]))
+
+
+@pytest.mark.timeout(900)
+def test_full_size_300_steps():
+    """The bench workload (576,000 particles) against the oracle at steps 100,
+    200 and 300: FP64 fields, dt and time bit-identical, FP32-mixed within
+    FP32_TOL (errors printed per field)."""
+    rt = _rt()
+    tb = build(DamBreak3D(tank=(1.6, 0.6, 0.6), column=(0.4, 0.6, 0.3), dx=0.005, name="dam_break_3D"))
+    coord = _jittered(tb)
+    print(f"[gpu] full-size 300 steps n={len(coord)}", flush=True)
+    o = _oracle(tb, coord)
+    d64 = _device(tb, rt.FP64, coord)
+    d32 = _device(tb, rt.FP32_MIXED, coord)
+    for it in (100, 200, 300):
+        o.step(100)
+        d64.step(100)
+        d32.step(100)
+        os_, ds, fs = o.state(), d64.state(), d32.state()
+        exact = {f: bool(np.array_equal(ds[f], os_[f])) for f in FIELDS}
+        e32 = {f: _rel(fs[f], os_[f]) for f in FIELDS}
+        print(f"[gpu] full-size it={it} fp64 bit-identical {exact} dt {ds['dt'] == os_['dt']}; fp32 max rel "
+              f"{{{', '.join(f'{k}: {v:.2e}' for k, v in e32.items())}}}", flush=True)
+        assert all(exact.values()) and ds["dt"] == os_["dt"] and ds["time"] == os_["time"], exact
+        for f in FIELDS:
+            assert e32[f] <= FP32_TOL[f], (it, f, e32[f])

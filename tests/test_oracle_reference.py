@@ -219,3 +219,86 @@ def test_live_free_surface(tmp_path):
             o.step(4)
         pel, zl = o.free_surface(icont, pts)
         assert _g14_close(pel, pel_r[k]) and _g14_close(zl, zl_r[k]), k
+
+
+def _oracle_steps(o, g, its, lists=False, tb=None):
+    prev = 0
+    for it in its:
+        if it:
+            o.step(it - prev)
+            prev = it
+        st = o.state()
+        assert st["nag"] == g[f"s{it}_nag"], it
+        assert st["dt"] == g[f"s{it}_dt"] and st["time"] == g[f"s{it}_time"], it
+        for f in FIELDS:
+            assert np.array_equal(st[f], g[f"s{it}_" + f]), (it, f)
+        if lists:
+            _check_lists(o, g[f"s{it}_nbr_count"], g[f"s{it}_nbr"])
+            _check_btab(o, tb, g[f"s{it}_ncbf"], g[f"s{it}_bt_face"], g[f"s{it}_bt_loc"],
+                        g[f"s{it}_bt_bi"], g[f"s{it}_bt_ig"])
+
+
+def test_golden_nearwall_bit_identical():
+    """Particles 0.02-0.22 dx from the floor and the x = 0 wall, moving into
+    them: the elastic reaction of AddElasticBoundaryReaction_3D.f90:67-104
+    and the zpmin clamp of ComputeBoundaryVolumeIntegrals_P0.f90:84,143 run
+    (the oracle's counters say how often), and lists, boundary table and
+    fields equal the reference's after 0, 4 and 8 steps."""
+    g = np.load(GOLD / "ref_nearwall.npz")
+    tb = build(_tiny())
+    o = Oracle(tb, coord=g["ic_coord"], vel=g["ic_vel"])
+    o.preloop()
+    _oracle_steps(o, g, (0, 4, 8), lists=True, tb=tb)
+    n = o.branch_counts()
+    assert n["elastic"] > 50 and n["zpmin"] > 50, n
+
+
+def test_golden_nearwall_2d_bit_identical():
+    """The 2D build with particles close to the floor and the x = 0 side,
+    moving into them: AddElasticBoundaryReaction_2D.f90 runs."""
+    from sphera_amd.case2d import DamBreak2D, build as build2d
+    g = np.load(GOLD / "ref_nearwall_2d.npz")
+    tb = build2d(DamBreak2D(tank=(0.6, 0.4), column=(0.3, 0.2), dx=0.01, name="t2d"))
+    o = Oracle(tb, coord=g["ic_coord"], vel=g["ic_vel"])
+    o.preloop()
+    _oracle_steps(o, g, (0, 4, 8))
+    assert o.branch_counts()["elastic"] > 20, o.branch_counts()
+
+
+def test_golden_density_clamp_bit_identical():
+    """density_thresholds = 1 with a converging and a diverging half: the
+    clamp of Leapfrog_continuity.f90:61-68 runs and the fields equal the
+    reference's after 3 and 6 steps."""
+    g = np.load(GOLD / "ref_dclamp.npz")
+    case = _small()
+    case.density_thresholds = 1
+    tb = build(case)
+    o = Oracle(tb, coord=g["ic_coord"], vel=g["ic_vel"])
+    o.preloop()
+    _oracle_steps(o, g, (0, 3, 6))
+    assert o.branch_counts()["density_clamp"] > 50, o.branch_counts()
+
+
+REFERENCE = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "sphera3d.x"
+
+
+@pytest.mark.skipif(not (HARNESS.exists() and REFERENCE.exists()), reason="reference not built here")
+def test_harness_equals_unmodified_reference_program(tmp_path):
+    """The dump harness (a project-written time_step_loop linked with the
+    reference's objects) against the reference program as shipped: the
+    results file that sphera3d.x writes every 5 steps (Memo_Results.f90:
+    279-287, restartcode 0) holds the same positions, velocities, pressures,
+    densities, times and dt, bit for bit, as the harness's dump."""
+    from oracle.decks import read_dump, read_ris, run_harness, run_reference
+    case = DamBreak3D(tank=(0.5, 0.2, 0.3), column=(0.15, 0.2, 0.15), dx=0.01, name="ris")
+    wd = run_reference(case, itmax=20, results_step=5, workdir=tmp_path / "ref")
+    ris = read_ris(wd / "ris.ris")
+    assert [r["it"] for r in ris] == [5, 10, 15, 20]
+    hdr, steps = read_dump(run_harness(case, nsteps=20, every=5, lists=False,
+                                       workdir=tmp_path / "harness"))
+    by_it = {s["it"]: s for s in steps}
+    for r in ris:
+        s = by_it[r["it"]]
+        assert r["time"] == s["time"] and r["dt"] == s["dt"], r["it"]
+        for f in ("coord", "vel", "pres", "dens", "mass"):
+            assert np.array_equal(r[f], s[f]), (r["it"], f)
