@@ -141,6 +141,17 @@ int sph_get_scalars(sph_ctx* ctx, double out[5]);
  * momentum (3), angular momentum about the origin (3), volume sum m/rho;
  * compensated FP64 sums (exact sum of the per-particle terms, rounded once). */
 int sph_global_quantities(sph_ctx* ctx, double out[8]);
+/* sph_global_quantities without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-workgroup partial sums live in one fixed device array
+ * of the library (2048 workgroups x 16 doubles + 8: 262,208 bytes per GPU,
+ * held until the process ends) instead of a per-call allocation; the
+ * results equal sph_global_quantities' bit for bit.  Status 3 if the device
+ * needs more than 2048 partial-sum workgroups.  Every call on one GPU uses
+ * that one array, so calls that overlap on one GPU (from several host
+ * threads) race; the call blocks until its results are copied. */
+int sph_global_quantities_noalloc(sph_ctx* ctx, double out[8]);
 /* Monitoring lines of the free-surface monitor, as ReadInputControlLines.f90
  * :97-124 stores them: icont[2*i], icont[2*i+1] = control_lines(i+1)%Icont
  * (1-based first and last point), points[3*j..] = control_points(j+1)%coord.

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -103,6 +104,7 @@ struct sph_ctx {
     std::vector<Timer> timers;
     std::vector<hipEvent_t> ev;
     int n_upload = 0;
+    int device = 0;  // the GPU current at creation (its stream's device)
 
     ~sph_ctx();
     int alloc();
@@ -588,6 +590,7 @@ int setup_launch(sph_ctx* c, const sph_params* p) {
     if (hipStreamCreate(&c->stream) != hipSuccess) return fail(101, "hipStreamCreate failed");
     int dev = 0, ncu = 256;
     hipGetDevice(&dev);
+    c->device = dev;
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     c->wg = p->wg_size > 0 ? p->wg_size : 64;
     if (c->wg < 64 || c->wg > 256 || c->wg % 64) return fail(2, "wg_size must be 64, 128, 192 or 256");
@@ -756,6 +759,38 @@ int create_ctx(int nseg, const sph_params* p, const sph_faces* const* f, sph_ctx
         return rc;
     }
     *out = c;
+    return 0;
+}
+
+}  // namespace
+
+// Partial sums of sph_global_quantities_noalloc: GQ_SCRATCH_BLOCKS
+// workgroups' double-double sums (2 x GQ_N doubles each), then the GQ_N
+// results.  One copy per GPU, held until the process ends (262,208 bytes).
+// Host code never passes this name to a kernel or a copy: it uses the
+// device address scratch_on() looked up for the device.
+constexpr int GQ_SCRATCH_BLOCKS = 2048;
+static __device__ __align__(256) double scratch[2 * GQ_N * GQ_SCRATCH_BLOCKS + GQ_N];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on GPU dev, which the caller has made current:
+// looked up once per device with hipGetSymbolAddress, then reused.
+int scratch_on(int dev, double** out) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(2, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(100 + (int)e, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
     return 0;
 }
 
@@ -1073,6 +1108,39 @@ int sph_global_quantities(sph_ctx* c, double out[8]) {
     hipFree(dout);
     if (e != hipSuccess) return fail(100, std::string("global quantities: ") + hipGetErrorString(e));
     return 0;
+}
+
+int sph_global_quantities_noalloc(sph_ctx* c, double out[8]) {
+    if (!c || !out) return fail(1, "null argument");
+    if (c->nseg > 1) return fail(2, "not available on a segmented context");
+    if (c->phase != PH_SORTED && c->phase != PH_READY && c->phase != PH_PS)
+        return fail(121, "global quantities are taken at a time-step boundary");
+    // the workgroup partition of sph_global_quantities, so the double-double
+    // partial sums and the rounded results are the same bit for bit
+    const int nb = 2 * c->bt_grid / 4;
+    if (nb > GQ_SCRATCH_BLOCKS)
+        return fail(3, "the global sums need " + std::to_string(nb) + " partial-sum workgroups; the "
+                       "library's scratch holds " + std::to_string(GQ_SCRATCH_BLOCKS));
+    int prev = 0;
+    HIPC(hipGetDevice(&prev));
+    if (prev != c->device) HIPC(hipSetDevice(c->device));
+    double* base = nullptr;
+    int rc = scratch_on(c->device, &base);
+    if (!rc) {
+        // every partial slot the final pass reads (b < nb) is written by the
+        // first pass of this call, so nothing has to start at zero
+        double* part = base;
+        double* dout = base + 2 * GQ_N * GQ_SCRATCH_BLOCKS;
+        k_global_partial<<<nb, 256, 0, c->stream>>>(c->xm[0], c->vel[0], c->rp[0], part, c->iscal);
+        k_global_final<<<1, 64, 0, c->stream>>>(part, nb, dout);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, dout, sizeof(double) * GQ_N, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(100 + (int)e, std::string("global quantities: ") + hipGetErrorString(e));
+    }
+    if (prev != c->device) hipSetDevice(prev);
+    return rc;
 }
 
 int sph_monitor_lines(sph_ctx* c, int32_t nlines, const int32_t* icont, int32_t npoints,

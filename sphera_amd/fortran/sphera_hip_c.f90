@@ -172,6 +172,13 @@ interface
       type(c_ptr),value :: ctx
       real(c_double),intent(out) :: out(8)
    end function
+   ! the same sums with no device allocation (a fixed library array per GPU)
+   integer(c_int) function sph_global_quantities_noalloc(ctx,out)                      &
+      bind(C,name="sph_global_quantities_noalloc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr),value :: ctx
+      real(c_double),intent(out) :: out(8)
+   end function
    integer(c_int) function sph_monitor_lines(ctx,nlines,icont,npoints,points)          &
       bind(C,name="sph_monitor_lines")
       import :: c_int, c_int32_t, c_ptr, c_double

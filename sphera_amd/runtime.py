@@ -27,6 +27,7 @@ PI = C.POINTER(C.c_int32)
 EXPORTS = [
     "sph_ctx_create", "sph_ctx_create2d", "sph_ctx_destroy", "sph_state_upload", "sph_particle_count",
     "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_global_quantities",
+    "sph_global_quantities_noalloc",
     "sph_monitor_lines", "sph_free_surface", "sph_get_neighbour_lists", "sph_get_boundary_table", "sph_dt", "sph_momentum",
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
     "sph_boundary_tab", "sph_continuity", "sph_leapfrog_cont", "sph_eos", "sph_pressure_smoothing",
@@ -92,6 +93,7 @@ def load():
     L.sph_get_boundary_table.argtypes = [C.c_void_p, PI, PI, PD, PD, PD, C.c_int32]
     L.sph_pair_counter.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.sph_global_quantities.argtypes = [C.c_void_p, PD]
+    L.sph_global_quantities_noalloc.argtypes = [C.c_void_p, PD]
     L.sph_monitor_lines.argtypes = [C.c_void_p, C.c_int32, PI, C.c_int32, PD]
     L.sph_free_surface.argtypes = [C.c_void_p, PD, PD]
     for ph in PHASES + ["preloop", "sync"]:
@@ -350,10 +352,13 @@ class SphContext:
     def sync(self):
         return self.phase("sync")
 
-    def global_quantities(self):
-        """mass, linear momentum (3), angular momentum (3), volume."""
+    def global_quantities(self, noalloc=False):
+        """mass, linear momentum (3), angular momentum (3), volume
+        (noalloc: through sph_global_quantities_noalloc, which allocates no
+        device memory)."""
         v = np.zeros(8)
-        self._check(self.L.sph_global_quantities(self.h, _pd(v)), "sph_global_quantities")
+        name = "sph_global_quantities_noalloc" if noalloc else "sph_global_quantities"
+        self._check(getattr(self.L, name)(self.h, _pd(v)), name)
         return dict(mass=v[0], lin_mom=v[1:4].copy(), ang_mom=v[4:7].copy(), volume=v[7])
 
     def monitor_lines(self, icont, points):
