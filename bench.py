@@ -10,6 +10,13 @@ smoothing), inputs resident in HBM.  value = pair interactions / s of the
 FP32-mixed path, sum_steps sum_i (nPartIntorno(i)-1) / t; the plain FP64 path
 is timed beside it.
 
+The served ensemble ("ensemble" in the output line): 512 seeded dam-break
+requests (seeds 0-511; column sides and height U[0.2, 0.6] m, dx U[0.008,
+0.02] m, 2k-324k particles) of 500 FP32-mixed steps each, Poisson arrivals at
+16 and 32 requests/s, each served as segmented batches and with one context
+per request; p50/p99 time to solution and aggregate pair-interactions/s per
+run, the batched form at the lower rate as the headline.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -218,11 +225,12 @@ def main():
                     help="fixed neighbour-search launch shape (sph_params.nb_shape), with --wg")
     ap.add_argument("--only", choices=("fp32_mixed", "fp64"), default=None,
                     help="time one precision only (profiling runs)")
-    ap.add_argument("--ens-requests", type=int, default=64,
+    ap.add_argument("--ens-requests", type=int, default=512,
                     help="ensemble: number of requests (0 skips the ensemble)")
-    ap.add_argument("--ens-steps", type=int, default=100, help="ensemble: time steps per request")
-    ap.add_argument("--ens-rate", type=float, default=200.0,
-                    help="ensemble: Poisson arrival rate, requests per second")
+    ap.add_argument("--ens-steps", type=int, default=500, help="ensemble: time steps per request")
+    ap.add_argument("--ens-rates", default="16,32",
+                    help="ensemble: Poisson arrival rates, requests per second (one run of each "
+                         "serving form per rate)")
     ap.add_argument("--ens-active", type=int, default=16, help="ensemble: concurrent requests")
     ap.add_argument("--ens-batch", type=int, default=8,
                     help="ensemble: requests per segmented batch (0 skips the batched run)")
@@ -285,33 +293,48 @@ def main():
     ens = None
     if args.ens_requests > 0:
         from sphera_amd import serve
-        reqs = serve.make_ensemble(args.ens_requests, args.ens_steps, args.ens_rate)
+        t_b = time.perf_counter()
+        reqs = serve.make_ensemble(args.ens_requests, args.ens_steps, 1.0)
+        unit_gaps = np.diff([r.arrival for r in reqs], prepend=0.0)  # exponential(1) gaps
         log(f"[bench] ensemble: {len(reqs)} requests x {args.ens_steps} steps, "
-            f"{min(r.n for r in reqs)}-{max(r.n for r in reqs)} particles, rate {args.ens_rate}/s")
+            f"{min(r.n for r in reqs)}-{max(r.n for r in reqs)} particles, built in "
+            f"{time.perf_counter() - t_b:.1f} s")
         prec = rt.FP32_MIXED if fp32 else rt.FP64
-        def summary(out, **extra):
-            return dict(requests=out["requests"], steps=args.ens_steps, arrival_rate_per_s=args.ens_rate,
-                        **extra, particles_min=min(out["particles"]), particles_max=max(out["particles"]),
-                        p50_latency_s=out["p50_latency_s"], p99_latency_s=out["p99_latency_s"],
-                        mean_latency_s=out["mean_latency_s"], p99_queue_s=out["p99_queue_s"],
-                        makespan_s=out["makespan_s"], pair_interactions_per_s=out["pair_interactions_per_s"])
-
-        ctxs = summary(serve.EnsembleServer(precision=prec, max_active=args.ens_active).run(reqs),
-                       mode="one context per request", max_active=args.ens_active)
-        log(f"[bench] ensemble (one context per request): p50 {ctxs['p50_latency_s'] * 1e3:.1f} ms, p99 "
-            f"{ctxs['p99_latency_s'] * 1e3:.1f} ms, {ctxs['pair_interactions_per_s'] / 1e9:.2f} G pairs/s")
-        ens = ctxs
-        if args.ens_batch > 0:  # the served form: segmented batches; the other form beside it
-            for r in reqs:
-                r.t_admit = r.t_done = float("nan")
-                r.pairs, r.state, r.ctx, r.done_steps = 0, None, None, 0
-            out = serve.BatchServer(precision=prec, batch=args.ens_batch,
-                                    max_batches=args.ens_batches).run(reqs)
-            ens = summary(out, mode="segmented batches", batch=args.ens_batch, max_batches=args.ens_batches)
-            ens["one_context_per_request"] = ctxs
-            log(f"[bench] ensemble (segmented batches of {args.ens_batch}): p50 "
-                f"{ens['p50_latency_s'] * 1e3:.1f} ms, p99 {ens['p99_latency_s'] * 1e3:.1f} ms, "
-                f"{ens['pair_interactions_per_s'] / 1e9:.2f} G pairs/s")
+        runs = []
+        for rate in [float(x) for x in args.ens_rates.split(",")]:
+            forms = [("segmented batches", lambda: serve.BatchServer(
+                precision=prec, batch=args.ens_batch, max_batches=args.ens_batches))] \
+                if args.ens_batch > 0 else []
+            forms.append(("one context per request",
+                          lambda: serve.EnsembleServer(precision=prec, max_active=args.ens_active)))
+            for mode, make in forms:
+                for r in reqs:
+                    r.t_admit = r.t_done = float("nan")
+                    r.pairs, r.state, r.ctx, r.done_steps = 0, None, None, 0
+                t_arr = np.cumsum(unit_gaps / rate)
+                for r, ta in zip(reqs, t_arr):
+                    r.arrival = float(ta)
+                out = make().run(reqs)
+                run = dict(mode=mode, arrival_rate_per_s=rate, p50_latency_s=out["p50_latency_s"],
+                           p99_latency_s=out["p99_latency_s"], mean_latency_s=out["mean_latency_s"],
+                           p99_queue_s=out["p99_queue_s"], makespan_s=out["makespan_s"],
+                           pair_interactions_per_s=out["pair_interactions_per_s"])
+                if mode == "segmented batches":
+                    run.update(batch=args.ens_batch, max_batches=args.ens_batches)
+                else:
+                    run.update(max_active=args.ens_active)
+                runs.append(run)
+                log(f"[bench] ensemble {mode} at {rate}/s: p50 {run['p50_latency_s']:.3f} s, p99 "
+                    f"{run['p99_latency_s']:.3f} s, {run['pair_interactions_per_s'] / 1e9:.2f} G pairs/s")
+        head = runs[0]  # the served form at the first (lowest) rate
+        ens = dict(workload=f"{len(reqs)} seeded dam-break requests (seeds 0-{len(reqs) - 1}) in the "
+                            "1.6x0.6x0.6 m tank, column sides and height U[0.2,0.6] m, dx U[0.008,0.02] m, "
+                            f"{args.ens_steps} steps each, Poisson arrivals",
+                   requests=len(reqs), steps=args.ens_steps,
+                   particles_min=min(r.n for r in reqs), particles_max=max(r.n for r in reqs),
+                   mode=head["mode"], arrival_rate_per_s=head["arrival_rate_per_s"],
+                   p50_latency_s=head["p50_latency_s"], p99_latency_s=head["p99_latency_s"],
+                   pair_interactions_per_s=head["pair_interactions_per_s"], runs=runs)
     cpu = cpu_port = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"[bench] cpu baseline (up to {args.cpu_steps} steps / {args.cpu_seconds} s)")

@@ -67,6 +67,10 @@ def lib():
         L.or_pair_count.argtypes = [C.c_void_p]
         L.or_pair_count.restype = C.c_int64
         L.or_branch_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.or_compare_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
+        L.or_compare_lists.restype = C.c_int64
+        L.or_set_lists_only.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_lists_only.restype = None
         L.or_branch_counts.restype = None
         L.or_set_state.argtypes = [C.c_void_p, C.c_int, PD, PD, PD, PD, PD, PD, PI, C.c_double,
                                    C.c_double, C.c_double, C.c_int]
@@ -218,6 +222,12 @@ class Oracle:
             raise RuntimeError("lists not built")
         return cnt, ids
 
+    def first_list_mismatch(self, count, ids):
+        """First particle whose list differs from the oracle's (-1: none)."""
+        count = np.ascontiguousarray(count, np.int32)
+        ids = np.ascontiguousarray(ids, np.int32)
+        return int(lib().or_compare_lists(self.ctx, _pi(count), _pi(ids), ids.shape[1]))
+
     def btab(self):
         n = self.n
         mc = int(self.p.maxclose)
@@ -241,6 +251,12 @@ class Oracle:
 
     def pair_count(self):
         return int(lib().or_pair_count(self.ctx))
+
+    def lists_only(self):
+        """Build the neighbour lists alone (no rag / PartKernel) on the next
+        list build: list checks of large clouds; the passes cannot run."""
+        lib().or_set_lists_only(self.ctx, 1)
+        return self.call("calcvarlength")
 
     def branch_counts(self):
         """How often the rare branches ran: elastic wall reactions applied,

@@ -6,6 +6,7 @@
 // expression follow the cited reference lines.  Indices are 0-based here; the
 // reference's 1-based cell ids are kept as such (0 = outside the grid).
 #include <atomic>
+#include <cstdint>
 #include "sph_oracle.h"
 
 #include <algorithm>
@@ -58,6 +59,9 @@ struct Ctx {
     // elastic wall reactions applied, zpmin clamps of the volume integrals,
     // density_thresholds clamps
     mutable std::atomic<long long> n_elastic{0}, n_zpmin{0}, n_dclamp{0};
+    // lists only: CalcVarLength stores the neighbour lists without rag and
+    // PartKernel (list checks of large clouds; the passes cannot run)
+    bool lists_only = false;
 };
 
 inline double Tm(const Ctx& c, int f, int r, int col) { return c.T[9 * f + col * 3 + r]; }
@@ -713,6 +717,21 @@ int or_get_lists(void* ctx, int32_t* count, int32_t* ids, int32_t stride) {
     return 0;
 }
 
+int64_t or_compare_lists(void* ctx, const int32_t* count, const int32_t* ids, int32_t stride) {
+    Ctx& c = *(Ctx*)ctx;
+    if (!c.lists_built) return -2;
+    const int mj = c.p.nmaxpartj;
+    int64_t first = INT64_MAX;
+#pragma omp parallel for schedule(static) reduction(min : first)
+    for (int i = 0; i < c.n; ++i) {
+        bool same = count[i] == c.nnb[i];
+        for (int k = 0; same && k < c.nnb[i]; ++k)
+            same = ids[(size_t)i * stride + k] == c.nbr[(size_t)i * mj + k];
+        if (!same && i < first) first = i;
+    }
+    return first == INT64_MAX ? -1 : first;
+}
+
 int or_get_btab(void* ctx, int32_t* ncbf, int32_t* face, double* loc, double* bi, double* ig) {
     Ctx& c = *(Ctx*)ctx;
     const int mc = c.p.maxclose;
@@ -802,8 +821,9 @@ int or_calcvarlength(void* ctx) {
     const int n = c.n;
     const int mj = c.p.nmaxpartj;
     c.nnb.assign(n, 0);
-    if ((int)c.nbr.size() < n * mj) {
-        c.nbr.resize((size_t)n * mj);
+    const bool lo = c.lists_only;
+    if (c.nbr.size() < (size_t)n * mj) c.nbr.resize((size_t)n * mj);
+    if (!lo && c.rag.size() < (size_t)n * mj * 3) {
         c.rag.resize((size_t)n * mj * 3);
         c.pk.resize((size_t)n * mj * 4);
     }
@@ -836,6 +856,7 @@ int or_calcvarlength(void* ctx) {
                         cnt = cnt + 1;
                         const size_t s = (size_t)npi * mj + (cnt - 1);
                         c.nbr[s] = npj;
+                        if (lo) continue;
                         const double rij2 = rij;
                         rij = std::sqrt(rij);
                         const double q = rij / c.p.h;
@@ -1403,6 +1424,8 @@ int or_leapfrog_cont(void* ctx) {
     }
     return 0;
 }
+
+void or_set_lists_only(void* ctx, int on) { ((Ctx*)ctx)->lists_only = on != 0; }
 
 void or_branch_counts(void* ctx, int64_t* out3) {
     const Ctx& c = *(const Ctx*)ctx;

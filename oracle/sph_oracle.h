@@ -73,6 +73,9 @@ void or_get_state(void* ctx, double* coord, double* vel, double* var, double* ac
                   double* pres, double* dden, double* vpres);
 void or_get_scalars(void* ctx, double* out4); /* dt, dt_average, time, it */
 int or_get_lists(void* ctx, int32_t* count, int32_t* ids, int32_t stride); /* ids 0-based */
+/* first particle whose list (count, then entries in order) differs from the
+ * oracle's, -1 if none (-2: lists not built); ids are 0-based, stride >= count */
+int64_t or_compare_lists(void* ctx, const int32_t* count, const int32_t* ids, int32_t stride);
 int or_get_btab(void* ctx, int32_t* ncbf, int32_t* face, double* loc, double* bi, double* ig);
 
 /* phases (time_step_loop.f90 order); each returns 0 or a diagnostic code */
@@ -96,6 +99,10 @@ int64_t or_pair_count(void* ctx);   /* sum_i (nPartIntorno(i)-1) of the current 
  * clamps (ComputeBoundaryVolumeIntegrals_P0.f90:84), [2] density_thresholds
  * clamps (Leapfrog_continuity.f90:61-68) */
 void or_branch_counts(void* ctx, int64_t* out3);
+/* on != 0: or_calcvarlength builds the neighbour lists only (no rag or
+ * PartKernel, about 1/15 of the memory), for list checks of large inputs;
+ * the pair passes must not run afterwards */
+void or_set_lists_only(void* ctx, int on);
 int or_overflow(void* ctx);         /* particles whose list was truncated */
 
 /* calc_pelo: free-surface position (pelolib) and lower-fluid-top height (zlft)

@@ -523,20 +523,81 @@ __global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* 
 // |x - corner| <= dcd, so the FP32 separation is within ~5e-7 dcd per axis and
 // the FP32 r^2 within ~1.5e-6 relative of (2h)^2 near the cut-off.)
 // ----------------------------------------------------------------------------
-// List writer of one lane: the next entry's address advances by one inside a
-// group of NB_G entries and jumps to the particle's next group after it.
+// List writer of one lane.  SPH_NB_BUF = 4 (default) or 8: the entries are
+// collected in a shift register of that many registers and stored as one
+// 16-byte (two for 8) vector store per full run, so the lane's list reaches
+// L2 in whole 16/32-byte pieces instead of one 4-byte store per entry; flush()
+// stores the last, partial run (its unused tail slots are past the count and
+// never read).  A run is stored only while it lies inside the particle's
+// list capacity (nb_stride(NMAXPARTJ)); the count keeps running, so the first
+// NMAXPARTJ entries are the stored ones, as in the reference.  SPH_NB_BUF = 0:
+// one 4-byte store per entry.
+#ifndef SPH_NB_BUF
+#define SPH_NB_BUF 4
+#endif
+#if SPH_NB_BUF == 0
 struct NbOut {
     uint32_t* wp;
     int cnt;
     size_t jump;  // N*NB_G - (NB_G-1): from the last entry of a group to the first of the next
-    __device__ __forceinline__ NbOut(uint32_t* nbr, size_t N, int p)
+    __device__ __forceinline__ NbOut(uint32_t* nbr, size_t N, int p, int)
         : wp(nbr + nb_index(N, p, 0)), cnt(0), jump(N * NB_G - (NB_G - 1)) {}
     __device__ __forceinline__ void push(uint32_t e, int mj) {
         if (cnt < mj) *wp = e;
         ++cnt;
         wp += ((cnt & (NB_G - 1)) == 0) ? jump : (size_t)1;
     }
+    __device__ __forceinline__ void flush() {}
 };
+#else
+static_assert(SPH_NB_BUF == 4 || SPH_NB_BUF == 8, "SPH_NB_BUF is 0, 4 or 8");
+static_assert(NB_G == 8, "the buffered list writer stores runs inside 8-entry groups");
+struct NbOut {
+    static constexpr int B = SPH_NB_BUF;
+    uint32_t* wp;  // first slot of the current run
+    int cnt;
+    int lim;       // list capacity: runs ending past it are not stored
+    size_t jump;   // from the last run of a group to the first slot of the particle's next group
+    uint32_t b[B]; // b[B-1] is the newest entry
+    __device__ __forceinline__ NbOut(uint32_t* nbr, size_t N, int p, int stride)
+        : wp(nbr + nb_index(N, p, 0)), cnt(0), lim(stride), jump(N * NB_G - (NB_G - B)) {
+#pragma unroll
+        for (int u = 0; u < B; ++u) b[u] = 0u;
+    }
+    __device__ __forceinline__ void store_run() {
+        uint4* q = reinterpret_cast<uint4*>(wp);
+        q[0] = make_uint4(b[0], b[1], b[2], b[3]);
+        if constexpr (B == 8) q[1] = make_uint4(b[4], b[5], b[6], b[7]);
+    }
+    __device__ __forceinline__ void push(uint32_t e, int) {
+#pragma unroll
+        for (int u = 0; u + 1 < B; ++u) b[u] = b[u + 1];
+        b[B - 1] = e;
+        ++cnt;
+        if ((cnt & (B - 1)) == 0) {
+            if (cnt <= lim) store_run();
+            wp += ((cnt & (NB_G - 1)) == 0) ? jump : (size_t)B;
+        }
+    }
+    // the r = cnt % B entries since the last run, moved to the run's first slots
+    __device__ __forceinline__ void flush() {
+        const int r = cnt & (B - 1);
+        if (r == 0 || cnt > lim) return;
+        uint32_t o[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            uint32_t v = 0u;
+#pragma unroll
+            for (int s = 1; s < B; ++s)  // slot u holds b[B - r + u] when u < r
+                if (r == s && u < s) v = b[B - s + u];
+            o[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) b[u] = o[u];
+        store_run();
+    }
+};
+#endif
 
 // The neighbour test of one candidate and the list append: FP32 on
 // cell-relative positions, and the reference's FP64 expression on the FP64
@@ -598,7 +659,7 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
     d_cell_indices(P, cell[p], ic, jc, kc);
     const float4 xi = xmf[p];
     const double4 xi64 = xm[p];
-    NbOut w(nbr, N, p);
+    NbOut w(nbr, N, p, nb_stride(mj));
     auto load = [&](int x) { return xmf[x]; };
     for (int dy = -1; dy <= 1; ++dy) {
         const int jj = jc + dy;
@@ -616,6 +677,7 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
             }
         }
     }
+    w.flush();
     ov = w.cnt > mj;
     return min(w.cnt, mj);
 }
@@ -825,7 +887,7 @@ __global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const dou
                 const int ucell = cell[p] - r * ni - ia + 1;  // u index of the own cell
                 const float4 xi = xmf[p];
                 const double4 xi64 = xm[p];
-                NbOut w(nbr, N, p);
+                NbOut w(nbr, N, p, nb_stride(mj));
                 auto load = [&](int x) {
                     if constexpr (H16) {
                         const float2 a = __half22float2(tile_xy[x]);
@@ -844,6 +906,7 @@ __global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const dou
                                     bnd[t][ucell + dx - 1], bnd[t][ucell + dx], seg_g[t] - seg_l[t], load,
                                     tlo, thi, w, mj);
                         }
+                w.flush();
                 const bool ov = w.cnt > mj;
                 const int cnt = min(w.cnt, mj);
                 nnb[p] = cnt;

@@ -138,3 +138,26 @@ def test_neighbour_truncation_like_reference(k):
     from oracle.pyoracle import lib
     # particles that met a candidate once full (the reference warns for each)
     assert lib().or_overflow(o.ctx) >= int(np.sum(cnt_full > 10 * k)) > 0
+
+
+def test_lists_only_mode_and_list_comparison():
+    """The oracle's lists-only build gives the lists of the full
+    CalcVarLength, and the list comparison finds a planted difference (one
+    entry, one count) at the first particle it touches."""
+    import numpy as np
+    from oracle.pyoracle import Oracle
+    from sphera_amd.microbench import cloud
+    tb, st = cloud(20_000)
+    full = Oracle(tb, semi=False, **st)
+    full.preloop()
+    lo = Oracle(tb, semi=False, **st)
+    lo.lists_only()
+    c, ids = full.lists()
+    c2, ids2 = lo.lists()
+    assert np.array_equal(c, c2) and np.array_equal(ids, ids2)
+    assert lo.first_list_mismatch(c, ids) == -1
+    ids[17, 2] += 1
+    assert lo.first_list_mismatch(c, ids) == 17
+    ids[17, 2] -= 1
+    c[3] -= 1
+    assert lo.first_list_mismatch(c, ids) == 3
