@@ -56,15 +56,17 @@ typedef struct sph_params {
     int32_t nb_shape;    /* neighbour-search workgroup size x LDS candidate tile:
                             0 = 256 x 4096 in FP16 (default), 1 = 256 x 3072 in FP32,
                             2 = 128 x 1536 in FP32, 3 = 256 x 2816 in FP16 with 64
-                            registers (8 waves per SIMD).  The lists are the same
-                            in every shape (FP64 decides near the cut-off). */
+                            registers (8 waves per SIMD), 4 = 256 x 4096 in 16-bit
+                            fixed point over the cell.  The lists are the same in
+                            every shape (FP64 decides near the cut-off). */
     int32_t options;     /* SPH_OPT_* flags of the deck (below) */
 } sph_params;
 
 /* Deck options the hot path branches on (input_any_t of
  * src/Modules/Hybrid_allocation_module.f90:88-113,931; NumBEdges of
- * Static_allocation_module.f90:220).  SPH_OPT_DENSITY_THRESHOLDS is built: the
- * 0.9/1.1 den0 clamp of Leapfrog_continuity.f90:61-68.  The others are not
+ * Static_allocation_module.f90:220; Domain%time_split and RKscheme).
+ * SPH_OPT_DENSITY_THRESHOLDS is built: the 0.9/1.1 den0 clamp of
+ * Leapfrog_continuity.f90:61-68; so is SPH_OPT_RK1.  The others are not
  * built on the device: sph_ctx_create / sph_ctx_create2d / sph_batch_create
  * refuse a context that sets them with status 95 (and so does a frontier with
  * slip_coefficient_mode = 2, the SA-SPH wall function), rather than running
@@ -74,7 +76,11 @@ enum {
     SPH_OPT_ALE3 = 2,               /* ALE3 (ALE3_term_momentum.f90, Leapfrog_continuity.f90:70-86) */
     SPH_OPT_C1 = 4,                 /* C1_BE or C1_divu (renormalisation, CalcVarLength.f90:285-324) */
     SPH_OPT_KTGF = 8,               /* KTGF_config > 0 (mixture viscosity, erosion) */
-    SPH_OPT_CONVEX_EDGES = 16       /* NumBEdges > 0 (AddElasticBoundaryReaction_3D.f90:106-152) */
+    SPH_OPT_CONVEX_EDGES = 16,      /* NumBEdges > 0 (AddElasticBoundaryReaction_3D.f90:106-152) */
+    SPH_OPT_RK1 = 32,               /* time_split = 0, RKscheme = 1: explicit Euler, built
+                                       (time_step_loop.f90:190-489, Euler.f90:61-241); such a
+                                       context steps with sph_step / sph_step_graph only */
+    SPH_OPT_RK2 = 64                /* time_split = 0, RKscheme = 2 (Heun.f90) */
 };
 
 /* SA-SPH frontier faces in BFaceList order and the grid-cell/face incidence

@@ -55,7 +55,8 @@ def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
               "##### medium #####", "liquid", "1", f"{_f(case.rho0)} {_f(case.bulk)}",
               f"{_f(case.alfa_mon)} 0.", _f(case.mu), "##### end medium #####",
               "##### RUN PARAMETERS #####", f"{_f(tmax)} {int(itmax)}",
-              f"{_f(case.cfl)} {_f(case.vsc_coeff)} 1 1 {_f(case.pesodt)} .false.",
+              f"{_f(case.cfl)} {_f(case.vsc_coeff)} {int(getattr(case, 'time_split', 1))} 1 "
+              f"{_f(case.pesodt)} .false.",
               f"{_f(case.teta_p)} {_f(case.teta_v)} A",
               f"1.1 {_f(case.coefnmaxpartj)} 0",
               ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.",
@@ -85,7 +86,7 @@ def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
     return "\n".join(lines) + "\n"
 
 
-def deck_text_2d(case, itmax=1000000000, tmax=1000.0):
+def deck_text_2d(case, itmax=1000000000, tmax=1000.0, results_step=None):
     """v10 deck of a DamBreak2D for the reference's 2D build: vertices (x, z),
     a LINES section instead of FACES, gravity and monitor points with two
     components, and none of the 3D-only lines (the CLC flag of
@@ -112,7 +113,8 @@ def deck_text_2d(case, itmax=1000000000, tmax=1000.0):
               "##### medium #####", "liquid", "1", f"{_f(case.rho0)} {_f(case.bulk)}",
               f"{_f(case.alfa_mon)} 0.", _f(case.mu), "##### end medium #####",
               "##### RUN PARAMETERS #####", f"{_f(tmax)} {int(itmax)}",
-              f"{_f(case.cfl)} {_f(case.vsc_coeff)} 1 1 {_f(case.pesodt)} .false.",
+              f"{_f(case.cfl)} {_f(case.vsc_coeff)} {int(getattr(case, 'time_split', 1))} 1 "
+              f"{_f(case.pesodt)} .false.",
               f"{_f(case.teta_p)} {_f(case.teta_v)} A",
               f"1.1 {_f(case.coefnmaxpartj)} 0",
               ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.",
@@ -122,7 +124,8 @@ def deck_text_2d(case, itmax=1000000000, tmax=1000.0):
               f"{_f(case.grav[0])} {_f(case.grav[2])}", _f(case.p_ref),
               "##### end general physical properties #####",
               "##### restart #####", "##### end restart #####",
-              "##### output regulation #####", "results time 100000.",
+              "##### output regulation #####",
+              f"results step {int(results_step)}" if results_step else "results time 100000.",
               "restart time 100000.", "print partial 100000000", "control time 100000.",
               "level time 100000. medium 1", "depth dt_out 100000.",
               "##### end output regulation #####",
@@ -142,7 +145,7 @@ def write_case_dir(case, root, itmax=1000000000, results_step=None):
     ./input/*/*.inp into <case>.inp (src/Main_algorithm/sphera.f90:222)."""
     root = Path(root)
     (root / "input" / "00_deck").mkdir(parents=True, exist_ok=True)
-    text = deck_text_2d(case, itmax=itmax) if getattr(case, "dim", 3) == 2 else \
+    text = deck_text_2d(case, itmax=itmax, results_step=results_step) if getattr(case, "dim", 3) == 2 else \
         deck_text(case, itmax=itmax, results_step=results_step)
     (root / "input" / "00_deck" / "deck.inp").write_text(text)
     (root / "output").mkdir(exist_ok=True)
@@ -188,18 +191,19 @@ def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
 
 
 def run_reference(case, itmax, results_step, workdir=None, threads=None, timeout=3600):
-    """Run the unmodified reference program (sphera3d.x, its own
-    time_step_loop) on the case's deck for itmax steps, writing results every
+    """Run the unmodified reference program (sphera3d.x, or sphera2d.x for a
+    2D case; its own time_step_loop) on the case's deck for itmax steps, writing results every
     results_step steps; returns the run directory (the results are in
     <case>.ris there)."""
-    if not REFERENCE.exists():
-        raise FileNotFoundError(f"{REFERENCE} not built (make -f oracle/Makefile.ref)")
+    exe = REFERENCE2D if getattr(case, "dim", 3) == 2 else REFERENCE
+    if not exe.exists():
+        raise FileNotFoundError(f"{exe} not built (make -f oracle/Makefile.ref)")
     wd = Path(workdir or tempfile.mkdtemp(prefix="sph_ris_")).resolve()
     write_case_dir(case, wd, itmax=itmax, results_step=results_step)
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
-    res = subprocess.run([str(REFERENCE), case.name], cwd=wd, env=env, capture_output=True,
+    res = subprocess.run([str(exe), case.name], cwd=wd, env=env, capture_output=True,
                          text=True, timeout=timeout)
     if res.returncode != 0 or not (wd / f"{case.name}.ris").exists():
         raise RuntimeError(f"reference failed ({res.returncode}):\n{res.stdout[-3000:]}"

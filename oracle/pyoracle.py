@@ -26,7 +26,8 @@ class OrParams(C.Structure):
                 ("eps", C.c_double), ("celerita", C.c_double), ("alfamon", C.c_double),
                 ("kin_visc", C.c_double), ("mumx", C.c_double), ("cfl", C.c_double),
                 ("vsc_coeff", C.c_double), ("pesodt", C.c_double), ("tetap", C.c_double),
-                ("tetav", C.c_double), ("semi", C.c_int32), ("density_thresholds", C.c_int32)]
+                ("tetav", C.c_double), ("semi", C.c_int32), ("density_thresholds", C.c_int32),
+                ("time_split", C.c_int32)]
 
 
 class OrFaces(C.Structure):
@@ -110,6 +111,7 @@ def params_from_tables(tb, n, semi=True):
     p.tetap, p.tetav = case.teta_p, case.teta_v
     p.semi = 1 if semi else 0
     p.density_thresholds = int(getattr(case, "density_thresholds", 0))
+    p.time_split = int(getattr(case, "time_split", 1))
     return p
 
 

@@ -8,7 +8,13 @@
 ! same per-step call sequence for the branch the dam-break decks use
 ! (Domain%tipo=="semi", time_split=1 leapfrog, no bodies, no KTGF; 3D, or 2D
 ! when compiled with -DSPACE_2D) and writes the FP64 state the hot path
-! produces to a binary stream file.
+! produces to a binary stream file.  With time_split=0 and RKscheme=1 it runs
+! the explicit RK1 branch of time_step_loop.f90:175-489 instead, where the
+! reference's time_integration.f90:54 calls the 3D Euler(BC_zmax_flag) with no
+! argument (its BC_zmax_flag then reads an undefined address and the 3D
+! program stops with a segmentation fault when TetaV > 0); this harness calls
+! Euler with the loop's BC_zmax_flag (.false.), then EoS_fluid_loop, which is
+! the rest of time_integration.
 !
 ! Environment:
 !   SPH_HARNESS_OUT    output file (required)
@@ -52,10 +58,13 @@ SourceFace = 0
 #endif
 it_eff = it_start
 BC_zmax_flag = .false.
-if ((Domain%tipo/="semi").or.(Domain%time_split/=1)) then
-   write(*,*) "harness: only tipo=semi with leapfrog (time_split=1) is handled"
+if ((Domain%tipo/="semi").or.((Domain%time_split/=1).and.                    &
+   ((Domain%time_split/=0).or.(Domain%RKscheme/=1)))) then
+   write(*,*) "harness: only tipo=semi with leapfrog (time_split=1) or RK1 ",   &
+      "(time_split=0, RKscheme=1) is handled"
    stop 2
 endif
+if (Domain%time_split==0) Domain%time_stage = 1
 call get_environment_variable("SPH_HARNESS_OUT",fout)
 if (len_trim(fout)==0) then
    write(*,*) "harness: SPH_HARNESS_OUT not set"
@@ -144,6 +153,59 @@ do while (it<nsteps)
    dt_previous_step = dt
    if (nag>0) call time_step_duration
    simulation_time = simulation_time + dt
+   if (Domain%time_split==0) then
+! Explicit RK1 (time_step_loop.f90:190-489 with time_stage = RKscheme = 1)
+      call CalcVarLength
+#ifdef SPACE_2D
+      call ComputeBoundaryDataTab_2D
+#else
+      call ComputeBoundaryDataTab_3D
+#endif
+      call fluid_particle_imposed_kinematics
+      call liquid_particle_ID_array
+      call RHS_momentum_equation
+#ifdef SPACE_3D
+      Ncbf_Max = 0
+!$omp parallel do default(none) shared(nag,pg,Domain,Ncbf_Max) private(npi)
+#else
+!$omp parallel do default(none) shared(nag,pg,Domain) private(npi)
+#endif
+      do npi=1,nag
+         pg(npi)%dden = 0.d0
+         pg(npi)%dden_ALE12 = 0.d0
+         pg(npi)%dden_ALE12_frozen = 0.d0
+         pg(npi)%koddens = 0
+#ifdef SPACE_3D
+         if (pg(npi)%state=="flu") call SASPH_continuity(npi,Ncbf_Max)
+#else
+         if (pg(npi)%state=="flu") call SASPH_continuity(npi)
+#endif
+         call Continuity_Equation(npi)
+      enddo
+!$omp end parallel do
+#ifdef SPACE_3D
+      if (Ncbf_Max>input_any_t%MAXCLOSEBOUNDFACES) then
+         write(*,*) "harness: MAXCLOSEBOUNDFACES exceeded",Ncbf_Max
+         stop 4
+      endif
+      call Euler(BC_zmax_flag)
+#else
+      call Euler
+#endif
+      call EoS_fluid_loop
+#ifdef SPACE_2D
+      if (input_any_t%TetaP>zero) call PressureSmoothing_2D
+#else
+      if (input_any_t%TetaP>zero) call PressureSmoothing_3D
+      call BC_zmax_anyt
+#endif
+      call OrdGrid1
+      call liquid_particle_ID_array
+      dtvel = dt
+      if (mod(it,every)==0) call write_step(u,it,lists)
+      if (simulation_time>=input_any_t%tmax) exit
+      cycle
+   endif
    call fluid_particle_imposed_kinematics
    call RHS_momentum_equation
    call Leapfrog_momentum(dt_previous_step,dtvel)

@@ -1520,9 +1520,47 @@ int or_preloop(void* ctx) {
     return rc;
 }
 
+// time_step_loop.f90:175-489 with time_split=0, RKscheme=1 (one stage): lists
+// and boundary table at the start of the stage (:192-212), RHS_momentum_equation,
+// the continuity loop (:335-362), time_integration (time_integration.f90:52-65:
+// Euler, then EoS_fluid_loop), PressureSmoothing, and OrdGrid1 at the end of
+// the stage (:425-458).  Euler.f90:61-241 for semi: vel += dt acc
+// (kodvel = 0), velocity_smoothing and var = vel + TetaV1 var, vel = var
+// (:85-115), coord += dt var (:122-139), dens += dt dden (:231-241).  The
+// pressure smoothing reads the kernel values and boundary table of the
+// stage's start, as the reference's PartKernel(4) and BoundaryDataTab are.
+static int step_rk1(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    c.it = c.it + 1;
+    c.on_going = c.it;
+    if (c.n > 0) or_dt(ctx);
+    c.time = c.time + c.dt;
+    or_calcvarlength(ctx);
+    int rc = or_boundary_tab(ctx);
+    if (rc) return rc;
+    rc = or_momentum(ctx);
+    if (rc) return rc;
+    rc = or_continuity(ctx);
+    if (rc) return rc;
+    for (int i = 0; i < c.n; ++i)
+        for (int d = 0; d < 3; ++d) c.vel[3 * i + d] = c.vel[3 * i + d] + c.dt * c.acc[3 * i + d];
+    or_vsmooth(ctx);
+    or_leapfrog_traj(ctx);
+    for (int i = 0; i < c.n; ++i) c.dens[i] = c.dens[i] + c.dt * c.dden[i];
+    or_eos(ctx);
+    or_psmooth(ctx);
+    or_ordgrid(ctx);
+    return 0;
+}
+
 // time_step_loop.f90:175-489 (time_split=1, semi, no open/inlet/zmax zones)
 int or_step(void* ctx, int nsteps) {
     Ctx& c = *(Ctx*)ctx;
+    if (c.p.time_split == 0) {
+        for (int s = 0; s < nsteps; ++s)
+            if (int rc = step_rk1(ctx)) return rc;
+        return 0;
+    }
     for (int s = 0; s < nsteps; ++s) {
         c.it = c.it + 1;
         c.on_going = c.it;

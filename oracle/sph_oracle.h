@@ -28,6 +28,7 @@ typedef struct or_params {
     double cfl, vsc_coeff, pesodt, tetap, tetav;
     int32_t semi;           /* 1: SA-SPH walls active */
     int32_t density_thresholds; /* input_any_t%density_thresholds (1: 0.9/1.1 den0 clamp) */
+    int32_t time_split;     /* Domain%time_split: 1 leapfrog, 0 explicit RK1 (RKscheme 1) */
 } or_params;
 
 /* nf faces: T[9*f] column-major (T(r,c) at [c*3+r]), LX[8*f] (4 nodes x 2),
@@ -92,7 +93,7 @@ int or_leapfrog_cont(void* ctx);
 int or_eos(void* ctx);
 int or_psmooth(void* ctx);          /* PressureSmoothing_3D (2D: _2D) */
 int or_preloop(void* ctx);          /* lists, boundary table, first dt */
-int or_step(void* ctx, int nsteps); /* whole leapfrog time steps */
+int or_step(void* ctx, int nsteps); /* whole time steps (leapfrog, or RK1 when time_split = 0) */
 int64_t or_pair_count(void* ctx);   /* sum_i (nPartIntorno(i)-1) of the current lists */
 /* how often the rare branches ran since or_create: [0] elastic wall reactions
  * applied (AddElasticBoundaryReaction_3D/2D, vin < 0 below 0.25 dx), [1] zpmin

@@ -19,8 +19,9 @@ from __future__ import annotations
 import time
 
 # neighbour-search launch shapes (workgroup size x LDS tile): 256 x 4096 FP16,
-# 256 x 3072 FP32, 128 x 1536 FP32, 256 x 2816 FP16 held to 64 registers
-NB_SHAPES = (0, 1, 2, 3)
+# 256 x 3072 FP32, 128 x 1536 FP32, 256 x 2816 FP16 held to 64 registers,
+# 256 x 4096 in 16-bit fixed point over the cell
+NB_SHAPES = (0, 1, 2, 3, 4)
 PASSES = {"momentum": "chunk", "velocity_smoothing": "chunk_v", "continuity": "chunk_c",
           "pressure_smoothing": "chunk_p"}
 

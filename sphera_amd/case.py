@@ -53,6 +53,7 @@ class DamBreak3D:
     maxcloseboundfaces: int = 100
     slip_coeff: float = 0.0       # slip_coefficient_mode=1 input value
     density_thresholds: int = 0   # 1: Leapfrog_continuity's 0.9/1.1 rho0 clamp
+    time_split: int = 1           # 1: leapfrog; 0: explicit RK (RKscheme 1, Euler)
     name: str = "dam_break_3D"
     # monitoring lines of the deck's "control lines" section:
     # (label, first point (x,y,z), second point, number of points)

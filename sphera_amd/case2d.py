@@ -43,6 +43,7 @@ class DamBreak2D:
     coefnmaxpartj: float = 2.0
     slip_coeff: float = 0.0
     density_thresholds: int = 0      # 1: Leapfrog_continuity's 0.9/1.1 rho0 clamp
+    time_split: int = 1           # 1: leapfrog; 0: explicit RK (RKscheme 1, Euler)
     name: str = "dam_break_2D"
     control_lines: tuple = ()
     dim: int = 2

@@ -76,6 +76,8 @@ struct sph_ctx {
     float4* rec[2] = {};
     float2* rpf[3] = {};
     double *dden = nullptr, *vpres = nullptr;
+    // explicit RK1: dden and vpres in the order after the step's closing sort
+    double *dden2 = nullptr, *vpres2 = nullptr;
     int *cell = nullptr, *key = nullptr, *slot = nullptr, *cnt = nullptr, *start = nullptr,
         *perm = nullptr, *pid = nullptr, *bsum = nullptr;
     int *work = nullptr, *nwork = nullptr;  // (particle, face) pairs of the boundary table
@@ -199,6 +201,11 @@ __global__ void k_fill_strided(int* p, int stride, int v, int m) {
     if (i < m) p[(size_t)i * stride] = v;
 }
 
+// Buffer of the current (rho, p) at a step boundary: the pressure smoothing
+// writes rp[0]; the leapfrog step's sort comes before it, the RK1 step's sort
+// after it (into rp[1]).
+inline int cur_rp(const sph_ctx* c) { return c->P.rk1 ? 1 : 0; }
+
 // cell sort + block descriptors (OrdGrid1)
 int run_sort(sph_ctx* c) {
     const int m = c->nkeys;
@@ -229,14 +236,15 @@ int run_sort(sph_ctx* c) {
         KTime t(c, "sort_gather");
         SEGL(k_gather, nblk(cap), 256, c->perm, c->key, c->xm[1], c->vel[2], c->rp[0], c->lam[0], c->acc[1],
              c->xm[0], c->vel[0], c->rp[1], c->lam[1], c->acc[0], c->cell, c->xmf, XFo(c, 0), XFo(c, 1),
-             VFo(c, 0), c->rpf[1], c->start, c->iscal);
+             VFo(c, 0), c->rpf[1], c->start, c->iscal, c->P.rk1 ? c->dden : nullptr, c->dden2,
+             c->P.rk1 ? c->vpres : nullptr, c->vpres2);
     }
     return 0;
 }
 
 // the tiled search in launch shape `shape` of NB_SHAPES
 #define NB_KERNEL(SG, k) \
-    k_neighbours<SG, NB_SHAPES[k].wg, NB_SHAPES[k].tile, NB_SHAPES[k].h16, NB_SHAPES[k].minb>
+    k_neighbours<SG, NB_SHAPES[k].wg, NB_SHAPES[k].tile, NB_SHAPES[k].tk, NB_SHAPES[k].minb>
 
 template <bool SG>
 void launch_nb(sph_ctx* c) {
@@ -249,6 +257,7 @@ void launch_nb(sph_ctx* c) {
         case 1: go(NB_KERNEL(SG, 1)); break;
         case 2: go(NB_KERNEL(SG, 2)); break;
         case 3: go(NB_KERNEL(SG, 3)); break;
+        case 4: go(NB_KERNEL(SG, 4)); break;
         default: go(NB_KERNEL(SG, 0)); break;
     }
 }
@@ -309,15 +318,16 @@ int run_dt(sph_ctx* c, int advance, bool reduce) {
 
 int run_momentum(sph_ctx* c) {
     KTime t(c, "momentum");
-    PAIRL(k_momentum, c->chunk, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0], c->rp[0], XF(c, 0), VF(c, 0),
-          c->rpf[0], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1], c->vel[1], VFo(c, 1), c->scal, c->iscal);
+    PAIRL(k_momentum, c->chunk, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0], c->rp[cur_rp(c)], XF(c, 0),
+          VF(c, 0), c->rpf[cur_rp(c)], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1], c->vel[1], VFo(c, 1), c->scal, c->iscal);
     return 0;
 }
 
 int run_vsmooth(sph_ctx* c) {
     KTime t(c, "velocity_smoothing");
-    PAIRL(k_vsmooth, c->chunk_v, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1], c->rp[0], XF(c, 1), VF(c, 1),
-          c->nbr, c->nnb, c->vel[2], c->xm[1], c->scal, c->dtmin, c->iscal);
+    PAIRL(k_vsmooth, c->chunk_v, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1], c->rp[cur_rp(c)], XF(c, 1),
+          VF(c, 1), c->nbr, c->nnb, c->vel[2], c->xm[1], c->scal, c->dtmin, c->iscal, c->dden, c->rp[2],
+          c->rpf[2], c->xrf);
     return 0;
 }
 
@@ -358,6 +368,9 @@ int check_flags(sph_ctx* c) {
 }
 
 int expect(sph_ctx* c, Phase need, const char* what) {
+    if (c->P.rk1)
+        return fail(2, std::string(what) + ": the phase entry points follow the leapfrog order; an "
+                                           "explicit RK1 context steps with sph_step / sph_step_graph");
     if (c->phase != need)
         return fail(120, std::string(what) + " called out of the time_step_loop order");
     return 0;
@@ -379,7 +392,7 @@ sph_ctx::~sph_ctx() {
     F(xmf); F(xrf);
     for (auto p : rec) F(p);
     for (auto p : rpf) F(p);
-    F(dden); F(vpres); F(rowblk); F(idslot); F(rmids); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
+    F(dden); F(vpres); F(dden2); F(vpres2); F(rowblk); F(idslot); F(rmids); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     F(mon_icont); F(mon_pts); F(mon_out);
@@ -403,6 +416,8 @@ int sph_ctx::alloc() {
     for (auto& p : rpf) HIPC(da(&p, P.f32 ? N : 1));
     HIPC(da(&dden, N));
     HIPC(da(&vpres, N));
+    HIPC(da(&dden2, P.rk1 ? N : 1));
+    HIPC(da(&vpres2, P.rk1 ? N : 1));
     HIPC(da(&cell, N));
     HIPC(da(&key, N));
     HIPC(da(&slot, N));
@@ -432,6 +447,10 @@ int sph_ctx::alloc() {
     HIPC(hipMemset(lam[1], 0, sizeof(int) * N));
     HIPC(hipMemset(dden, 0, sizeof(double) * N));
     HIPC(hipMemset(vpres, 0, sizeof(double) * N));
+    if (P.rk1) {
+        HIPC(hipMemset(dden2, 0, sizeof(double) * N));
+        HIPC(hipMemset(vpres2, 0, sizeof(double) * N));
+    }
     HIPC(hipMemset(nnb, 0, sizeof(int) * N));
     HIPC(hipMemset(bn, 0, sizeof(int) * N));
     HIPC(hipMemset(scal, 0, sizeof(double) * SC_COUNT * ns));
@@ -449,9 +468,11 @@ int check_options(int32_t opt) {
         {SPH_OPT_C1, "C1_BE / C1_divu renormalisation (CalcVarLength.f90:285-324)"},
         {SPH_OPT_KTGF, "KTGF_config > 0 (mixture viscosity and erosion)"},
         {SPH_OPT_CONVEX_EDGES, "convex frontier edges (AddElasticBoundaryReaction_3D.f90:106-152)"},
+        {SPH_OPT_RK2, "the RK2 (Heun) time integration (Heun.f90)"},
     };
     const int32_t known = SPH_OPT_DENSITY_THRESHOLDS | SPH_OPT_ALE3 | SPH_OPT_C1 | SPH_OPT_KTGF |
-                          SPH_OPT_CONVEX_EDGES;
+                          SPH_OPT_CONVEX_EDGES | SPH_OPT_RK1 | SPH_OPT_RK2;
+    if ((opt & SPH_OPT_RK1) && (opt & SPH_OPT_RK2)) return fail(2, "RK1 and RK2 are exclusive");
     if (opt & ~known) return fail(2, "unknown option bits " + std::to_string(opt & ~known));
     for (const auto& o : NOT_BUILT)
         if (opt & o.bit) return fail(95, std::string("not built on the device: ") + o.what);
@@ -491,6 +512,7 @@ int fill_dp(const sph_params* p, const sph_faces* f, DP& P, bool dry = false) {
     P.semi = p->semi ? 1 : 0;
     P.f32 = p->precision == SPH_FP32_MIXED;
     P.dthr = (p->options & SPH_OPT_DENSITY_THRESHOLDS) ? 1 : 0;
+    P.rk1 = (p->options & SPH_OPT_RK1) ? 1 : 0;
     P.nf = f ? f->nf : 0;
     P.dim = 3;
     for (int d = 0; d < 3; ++d) {
@@ -557,8 +579,9 @@ int layout_segments(sph_ctx* c, int nseg, const sph_params* p, const sph_faces* 
         const sph_faces* fs = f ? f[s] : nullptr;
         if (int rc = fill_dp(&p[s], fs, c->segs[s], dry)) return rc;
         DP& P = c->segs[s];
-        if (p[s].precision != p[0].precision || (p[s].semi != 0) != (p[0].semi != 0))
-            return fail(2, "the segments of a batch share precision and wall treatment");
+        if (p[s].precision != p[0].precision || (p[s].semi != 0) != (p[0].semi != 0) ||
+            ((p[s].options ^ p[0].options) & SPH_OPT_RK1))
+            return fail(2, "the segments of a batch share precision, wall treatment and time integration");
         P.seg = s;
         P.base = (int)slots;
         P.koff = (int)keys;
@@ -608,8 +631,8 @@ int setup_launch(sph_ctx* c, const sph_params* p) {
     // persistent neighbour-search workgroups: as many as fit a CU (LDS
     // tile), times the CUs, a multiple of the 8 XCDs
     c->nb_shape = p->nb_shape;
-    if (c->nb_shape < 0 || c->nb_shape >= NB_NSHAPES) return fail(2, "nb_shape must be 0, 1, 2 or 3");
-    if (NB_SHAPES[c->nb_shape].h16)  // FP16 keeps its relative precision for cells of 1e-4..1e4 m
+    if (c->nb_shape < 0 || c->nb_shape >= NB_NSHAPES) return fail(2, "nb_shape must be 0 to 4");
+    if (NB_SHAPES[c->nb_shape].tk == NB_T16)  // FP16 keeps its relative precision for cells of 1e-4..1e4 m
         for (const auto& Q : c->segs)
             for (int d = 0; d < 3; ++d)
                 if (!(Q.dcd[d] >= 1.0e-4 && Q.dcd[d] <= 1.0e4)) c->nb_shape = 1;
@@ -620,6 +643,7 @@ int setup_launch(sph_ctx* c, const sph_params* p) {
         case 1: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 1), nwg, 0); break;
         case 2: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 2), nwg, 0); break;
         case 3: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 3), nwg, 0); break;
+        case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 4), nwg, 0); break;
         default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, NB_KERNEL(false, 0), nwg, 0); break;
     }
     if (oe != hipSuccess || per_cu <= 0) per_cu = 3;
@@ -1045,9 +1069,11 @@ int sph_segment_download(sph_ctx* c, int32_t seg, double* coord, double* vel, do
         HIPC(hipMemcpy(hx.data(), c->xm[0] + b, sizeof(double4) * n, hipMemcpyDeviceToHost));
         HIPC(hipMemcpy(hv.data(), c->vel[0] + b, sizeof(double4) * n, hipMemcpyDeviceToHost));
         HIPC(hipMemcpy(ha.data(), c->acc[0] + b, sizeof(double4) * n, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(hr.data(), c->rp[0] + b, sizeof(double2) * n, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(hd.data(), c->dden + b, sizeof(double) * n, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(hp.data(), c->vpres + b, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hr.data(), c->rp[cur_rp(c)] + b, sizeof(double2) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hd.data(), (c->P.rk1 ? c->dden2 : c->dden) + b, sizeof(double) * n,
+                       hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hp.data(), (c->P.rk1 ? c->vpres2 : c->vpres) + b, sizeof(double) * n,
+                       hipMemcpyDeviceToHost));
     }
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
@@ -1100,7 +1126,7 @@ int sph_global_quantities(sph_ctx* c, double out[8]) {
     double* dout = nullptr;
     HIPC(hipMalloc(&part, sizeof(double) * 2 * GQ_N * nb));
     HIPC(hipMalloc(&dout, sizeof(double) * GQ_N));
-    k_global_partial<<<nb, 256, 0, c->stream>>>(c->xm[0], c->vel[0], c->rp[0], part, c->iscal);
+    k_global_partial<<<nb, 256, 0, c->stream>>>(c->xm[0], c->vel[0], c->rp[cur_rp(c)], part, c->iscal);
     k_global_final<<<1, 64, 0, c->stream>>>(part, nb, dout);
     hipError_t e = hipMemcpyAsync(out, dout, sizeof(double) * GQ_N, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1131,7 +1157,7 @@ int sph_global_quantities_noalloc(sph_ctx* c, double out[8]) {
         // first pass of this call, so nothing has to start at zero
         double* part = base;
         double* dout = base + 2 * GQ_N * GQ_SCRATCH_BLOCKS;
-        k_global_partial<<<nb, 256, 0, c->stream>>>(c->xm[0], c->vel[0], c->rp[0], part, c->iscal);
+        k_global_partial<<<nb, 256, 0, c->stream>>>(c->xm[0], c->vel[0], c->rp[cur_rp(c)], part, c->iscal);
         k_global_final<<<1, 64, 0, c->stream>>>(part, nb, dout);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
@@ -1303,6 +1329,7 @@ int sph_sort(sph_ctx* c) {
 }
 int sph_neighbours(sph_ctx* c) {
     if (!c) return fail(1, "null argument");
+    if (c->P.rk1) return expect(c, c->phase, "sph_neighbours");
     if (c->phase != PH_SORT && c->phase != PH_SORTED)
         return fail(120, "sph_neighbours called out of the time_step_loop order");
     run_neighbours(c);
@@ -1311,6 +1338,7 @@ int sph_neighbours(sph_ctx* c) {
 }
 int sph_boundary_tab(sph_ctx* c) {
     if (!c) return fail(1, "null argument");
+    if (c->P.rk1) return expect(c, c->phase, "sph_boundary_tab");
     if (c->phase != PH_NB && c->phase != PH_SORTED)
         return fail(120, "sph_boundary_tab called out of the time_step_loop order");
     run_btab(c);
@@ -1360,6 +1388,25 @@ int sph_preloop(sph_ctx* c) {
 }
 
 static void enqueue_steps(sph_ctx* c, int nsteps) {
+    if (c->P.rk1) {
+        // time_step_loop.f90:175-489 with time_split = 0, RKscheme = 1: the
+        // lists and boundary table of a stage are built by the previous step's
+        // closing sort (nothing moves between that OrdGrid1 and the next
+        // stage's CalcVarLength); momentum and continuity on the stage's
+        // state, Euler's velocity, smoothing, position and density updates
+        // with EoS, pressure smoothing, then OrdGrid1
+        for (int s = 0; s < nsteps; ++s) {
+            run_dt(c, 1, false);
+            run_momentum(c);
+            run_continuity(c);
+            run_vsmooth(c);
+            run_psmooth(c);
+            run_sort(c);
+            run_neighbours(c);
+            run_btab(c);
+        }
+        return;
+    }
     for (int s = 0; s < nsteps; ++s) {
         run_dt(c, 1, false);
         run_momentum(c);

@@ -56,6 +56,8 @@ struct DP {
     // density_thresholds == 1: Leapfrog_continuity.f90:61-68 clamps the
     // integrated density to [0.9, 1.1] den0
     int dthr;
+    // time_split = 0, RKscheme = 1: the explicit RK1 (Euler) step order
+    int rk1;
     // 3, or 2 for the x-z plane form (-DSPACE_2D build of the reference): one
     // cell row in y (every particle in row j = 1), boundary sides for faces
     int dim;

@@ -476,7 +476,9 @@ __global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* 
                          int* __restrict__ lam_d, double4* __restrict__ acc_d,
                          int* __restrict__ cell_d, float4* __restrict__ xmf, F4Out xmf0, F4Out xmf1,
                          F4Out velf,
-                         float2* __restrict__ rpf, const int* __restrict__ start, int* iscal) {
+                         float2* __restrict__ rpf, const int* __restrict__ start, int* iscal,
+                         const double* __restrict__ dd_s, double* __restrict__ dd_d,
+                         const double* __restrict__ vp_s, double* __restrict__ vp_d) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(blockIdx.x * blockDim.x) >> 8]) : 0;
     const DP& P = SG ? S.P[sg] : P0;
@@ -496,6 +498,10 @@ __global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* 
     rp_d[p] = r;
     lam_d[p] = lam_s[s];
     acc_d[p] = acc_s[s];
+    if (dd_s) {  // explicit RK1: dden and vpres of the step end in the new order
+        dd_d[p] = dd_s[s];
+        vp_d[p] = vp_s[s];
+    }
     const int c = key[s] - P.koff + 1;
     cell_d[p] = c;
     int ci, cj, ck;
@@ -608,12 +614,22 @@ struct NbOut {
 // exact one, far inside the 4e-5 band.  Every accepted candidate is counted;
 // the first NMAXPARTJ are stored (the reference keeps the first NMAXPARTJ of
 // its list, CalcVarLength.f90:200-207), so the loops have no early exit.
+// Q: xj holds the candidate's 16-bit fixed-point coordinates (0..65535 over
+// the cell), decoded with the per-axis step sc in one fused multiply-add.
+template <bool Q = false>
 __device__ __forceinline__ void nb_test(const DP& P, const double4* __restrict__ xm, double4 xi64,
                                         float4 xo, float4 xj, int m, uint32_t ctag, float lo, float hi,
-                                        NbOut& w, int mj) {
-    const float r0 = xo.x - xj.x;
-    const float r1 = xo.y - xj.y;
-    const float r2 = xo.z - xj.z;
+                                        NbOut& w, int mj, float4 sc = float4{}) {
+    float r0, r1, r2;
+    if constexpr (Q) {
+        r0 = __builtin_fmaf(xj.x, -sc.x, xo.x);
+        r1 = __builtin_fmaf(xj.y, -sc.y, xo.y);
+        r2 = __builtin_fmaf(xj.z, -sc.z, xo.z);
+    } else {
+        r0 = xo.x - xj.x;
+        r1 = xo.y - xj.y;
+        r2 = xo.z - xj.z;
+    }
     const float r2f = __builtin_fmaf(r0, r0, __builtin_fmaf(r1, r1, r2 * r2));
     if (r2f <= hi) {
         bool acc = r2f < lo;
@@ -631,20 +647,21 @@ __device__ __forceinline__ void nb_test(const DP& P, const double4* __restrict__
 
 // Candidates [b,e) of one stencil cell, four loads ahead: load(x) gives the
 // FP32 position of slot x, gl + x its sorted index.
-template <class Load>
+template <bool Q = false, class Load>
 __device__ __forceinline__ void nb_cell(const DP& P, const double4* __restrict__ xm, double4 xi64,
                                         float4 xi, float4 o, uint32_t ctag, int b, int e, int gl,
-                                        Load&& load, float lo, float hi, NbOut& w, int mj) {
+                                        Load&& load, float lo, float hi, NbOut& w, int mj,
+                                        float4 sc = float4{}) {
     const float4 xo = make_float4(xi.x - o.x, xi.y - o.y, xi.z - o.z, 0.0f);
     int x = b;
     for (; x + 4 <= e; x += 4) {
         const float4 q0 = load(x), q1 = load(x + 1), q2 = load(x + 2), q3 = load(x + 3);
-        nb_test(P, xm, xi64, xo, q0, gl + x, ctag, lo, hi, w, mj);
-        nb_test(P, xm, xi64, xo, q1, gl + x + 1, ctag, lo, hi, w, mj);
-        nb_test(P, xm, xi64, xo, q2, gl + x + 2, ctag, lo, hi, w, mj);
-        nb_test(P, xm, xi64, xo, q3, gl + x + 3, ctag, lo, hi, w, mj);
+        nb_test<Q>(P, xm, xi64, xo, q0, gl + x, ctag, lo, hi, w, mj, sc);
+        nb_test<Q>(P, xm, xi64, xo, q1, gl + x + 1, ctag, lo, hi, w, mj, sc);
+        nb_test<Q>(P, xm, xi64, xo, q2, gl + x + 2, ctag, lo, hi, w, mj, sc);
+        nb_test<Q>(P, xm, xi64, xo, q3, gl + x + 3, ctag, lo, hi, w, mj, sc);
     }
-    for (; x < e; ++x) nb_test(P, xm, xi64, xo, load(x), gl + x, ctag, lo, hi, w, mj);
+    for (; x < e; ++x) nb_test<Q>(P, xm, xi64, xo, load(x), gl + x, ctag, lo, hi, w, mj, sc);
 }
 
 // List of particle p from global memory (candidates gathered through L1/L2).
@@ -692,16 +709,21 @@ __device__ int nb_particle_global(const DP& P, int p, const double4* __restrict_
 // the blocks of one contiguous range per XCD.
 // Launch shapes of the tiled search (workgroup size, candidate tile, FP16
 // tile, register bound): the default 256 x 4096 FP16 (26 KB of LDS, 6 waves
-// per SIMD) and three the autotuner may pick instead: 256 x 3072 FP32 (36 KB,
+// per SIMD, registers held to 80 so that 6 fit) and three the autotuner may
+// pick instead: 256 x 3072 FP32 (36 KB,
 // 4 waves), 128 x 1536 FP32, and 256 x 2816 FP16 held to 64 registers for 8
 // waves per SIMD.
 constexpr int NB_WG = 256;
 constexpr int NB_MAXC = 48;
 constexpr int NB_TILE = 3072;
-struct NbShape { int wg, tile; bool h16; int minb; };  // minb: workgroups per CU the registers must allow
-constexpr int NB_NSHAPES = 4;
-constexpr NbShape NB_SHAPES[NB_NSHAPES] = {
-    {256, 4096, true, 1}, {256, 3072, false, 1}, {128, 1536, false, 1}, {256, 2816, true, 8}};
+// tk: tile coordinates in FP32 (NB_T32), FP16 (NB_T16) or 16-bit fixed point
+// over the cell (NB_TQ16); minb: workgroups per CU the registers must allow
+enum { NB_T32 = 0, NB_T16 = 1, NB_TQ16 = 2 };
+struct NbShape { int wg, tile; int tk; int minb; };
+constexpr int NB_NSHAPES = 5;
+constexpr NbShape NB_SHAPES[NB_NSHAPES] = {{256, 4096, NB_T16, 6},  {256, 3072, NB_T32, 1},
+                                           {128, 1536, NB_T32, 1},  {256, 2816, NB_T16, 8},
+                                           {256, 4096, NB_TQ16, 6}};
 
 // rowblk[r] = first block of row r (exclusive scan of ceil(count_r / bw), bw
 // the search's workgroup size),
@@ -740,12 +762,16 @@ __global__ __launch_bounds__(1024) void k_nb_rowblocks(DP P0, Seg S, const int* 
     if (threadIdx.x == 0) rowblk[R] = carry;
 }
 
-// H16: the tile holds the candidates' cell-relative positions in FP16 (6
-// bytes each: more workgroups per CU).  A coordinate in [0, dcd] rounds by at
-// most 2^-11 dcd, so the FP32 r^2 from it is within ~2e-3 relative of the
+// TK = NB_T16: the tile holds the candidates' cell-relative positions in FP16
+// (6 bytes each: more workgroups per CU).  A coordinate in [0, dcd] rounds by
+// at most 2^-11 dcd, so the FP32 r^2 from it is within ~2e-3 relative of the
 // exact one near (2h)^2; the FP64 guard band is widened to 5e-3 to keep the
-// lists exactly the reference's.
-template <bool SG, int WG = NB_WG, int TILE = NB_TILE, bool H16 = false, int MINB = 1>
+// lists exactly the reference's.  TK = NB_TQ16: 16-bit fixed point over the
+// cell instead (q = rint(x 65535 / dcd), also 6 bytes): a coordinate rounds
+// by at most 2^-17 dcd, the FP32 r^2 is within ~2.8e-5 (dcd / 2h) relative of
+// the exact one near (2h)^2 (dcd / 2h < 1.5 on the reference's grid), and the
+// guard band is 2e-4, so the FP64 test runs 25 times less often.
+template <bool SG, int WG = NB_WG, int TILE = NB_TILE, int TK = NB_T32, int MINB = 1>
 __global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const double4* __restrict__ xm,
                                                       const float4* __restrict__ xmf,
                                                       const int* __restrict__ cell,
@@ -753,8 +779,11 @@ __global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const dou
                                                       const int* __restrict__ rowblk,
                                                       uint32_t* __restrict__ nbr, int* __restrict__ nnb,
                                                       int* iscal, unsigned long long* pairs) {
-    using TXY = typename std::conditional<H16, __half2, float2>::type;
-    using TZ = typename std::conditional<H16, __half, float>::type;
+    using TXY = typename std::conditional<TK == NB_T16, __half2,
+                                          typename std::conditional<TK == NB_TQ16, uint32_t, float2>::type>::type;
+    using TZ = typename std::conditional<TK == NB_T16, __half,
+                                         typename std::conditional<TK == NB_TQ16, uint16_t, float>::type>::type;
+    constexpr bool Q = TK == NB_TQ16;
     __shared__ TXY tile_xy[TILE];  // staged candidates: (x, y) and z, 12 (FP16: 6) bytes each
     __shared__ TZ tile_z[TILE];
     __shared__ int bnd[9][NB_MAXC + 4];  // LDS offset of the start of cell ia-1+u of row t
@@ -805,7 +834,11 @@ __global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const dou
         const int ni = P.ncd[0], nj = P.ncd[1], nk = P.ncd[2];
         const float sq = (float)P.square_doubleh;
         const float lo = sq * (1.0f - 4.0e-5f), hi = sq * (1.0f + 4.0e-5f);
-        const float band = H16 ? 5.0e-3f : 4.0e-5f;  // guard band of the tiled test
+        const float band = TK == NB_T16 ? 5.0e-3f : (Q ? 2.0e-4f : 4.0e-5f);  // guard band of the tiled test
+        // fixed-point step per axis (NB_TQ16) and its inverse for the staging
+        const float4 qs = make_float4(P.f_dcd[0] * (1.0f / 65535.0f), P.f_dcd[1] * (1.0f / 65535.0f),
+                                      P.f_dcd[2] * (1.0f / 65535.0f), 0.0f);
+        const float4 qi = make_float4(65535.0f / P.f_dcd[0], 65535.0f / P.f_dcd[1], 65535.0f / P.f_dcd[2], 0.0f);
         const float tlo = sq * (1.0f - band), thi = sq * (1.0f + band);
         const int mj = P.nmaxpartj;
         const size_t N = P.cap;
@@ -872,9 +905,15 @@ __global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const dou
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
                         if (dst[u] >= 0) {
-                            if constexpr (H16) {
+                            if constexpr (TK == NB_T16) {
                                 tile_xy[dst[u]] = __floats2half2_rn(v[u].x, v[u].y);
                                 tile_z[dst[u]] = __float2half_rn(v[u].z);
+                            } else if constexpr (Q) {
+                                auto q16 = [](float a) {  // 0 <= a <= 65535 (+ rounding)
+                                    return (uint32_t)__float2uint_rn(fminf(fmaxf(a, 0.0f), 65535.0f));
+                                };
+                                tile_xy[dst[u]] = q16(v[u].x * qi.x) | (q16(v[u].y * qi.y) << 16);
+                                tile_z[dst[u]] = (uint16_t)q16(v[u].z * qi.z);
                             } else {
                                 tile_xy[dst[u]] = make_float2(v[u].x, v[u].y);
                                 tile_z[dst[u]] = v[u].z;
@@ -889,9 +928,12 @@ __global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const dou
                 const double4 xi64 = xm[p];
                 NbOut w(nbr, N, p, nb_stride(mj));
                 auto load = [&](int x) {
-                    if constexpr (H16) {
+                    if constexpr (TK == NB_T16) {
                         const float2 a = __half22float2(tile_xy[x]);
                         return make_float4(a.x, a.y, __half2float(tile_z[x]), 0.0f);
+                    } else if constexpr (Q) {
+                        const uint32_t a = tile_xy[x];
+                        return make_float4((float)(a & 0xFFFFu), (float)(a >> 16), (float)tile_z[x], 0.0f);
                     } else {
                         const float2 a = tile_xy[x];
                         return make_float4(a.x, a.y, tile_z[x], 0.0f);
@@ -902,9 +944,9 @@ __global__ __launch_bounds__(WG, MINB) void k_neighbours(DP P0, Seg S, const dou
                         for (int dz = 0; dz < 3; ++dz) {
                             const int t = 3 * dy + dz;
                             const int code = dx + 3 * dy + 9 * dz;
-                            nb_cell(P, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT,
-                                    bnd[t][ucell + dx - 1], bnd[t][ucell + dx], seg_g[t] - seg_l[t], load,
-                                    tlo, thi, w, mj);
+                            nb_cell<Q>(P, xm, xi64, xi, g_soff[code], (uint32_t)code << NB_SHIFT,
+                                       bnd[t][ucell + dx - 1], bnd[t][ucell + dx], seg_g[t] - seg_l[t], load,
+                                       tlo, thi, w, mj, qs);
                         }
                 w.flush();
                 const bool ov = w.cnt > mj;
@@ -2103,7 +2145,9 @@ __global__ __launch_bounds__(256) void k_momentum(
     iscal += sg * IS_COUNT;
     stencil_offsets(P);
     const double h = P.h;
-    const double dtvel = scal[SC_DTVEL];
+    // Leapfrog_momentum.f90:66-86 integrates with dtvel; the explicit RK1
+    // branch (Euler.f90:85-93) with dt
+    const double dtvel = P.rk1 ? scal[SC_DT] : scal[SC_DTVEL];
     const size_t N = P.cap;
     {
         const int i = vb * blockDim.x + threadIdx.x;
@@ -2257,7 +2301,7 @@ __global__ __launch_bounds__(256) void k_momentum(
                 for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + P.grav[d];
             }
             acc_out[i] = make_double4(a[0], a[1], a[2], 0.0);
-            // Leapfrog_momentum: vel = vel + dtvel*acc (kodvel == 0)
+            // Leapfrog_momentum (RK1: Euler): vel = vel + dtvel*acc (kodvel == 0)
             const double4 vn =
                 make_double4(vi.x + dtvel * a[0], vi.y + dtvel * a[1], vi.z + dtvel * a[2], vi.w);
             vel_out[i] = vn;
@@ -2271,7 +2315,9 @@ __global__ __launch_bounds__(256) void k_momentum(
 // velocity smoothing: velocity_smoothing.f90:74-133, velocity_smoothing_SA_SPH
 // _3D.f90:64-122 (FP64), velocity_smoothing_2.f90:59-103; fused
 // Leapfrog_trajectories.f90:48-76 and the particle part of the next
-// time_step_duration.  Neighbour density travels in velf.w (FP32).
+// time_step_duration.  Neighbour density travels in velf.w (FP32).  Explicit
+// RK1: the same smoothing, update and positions (Euler.f90:94-139), then the
+// density update and EoS of Euler.f90:231-241 and time_integration.f90:52-65.
 // ----------------------------------------------------------------------------
 template <bool F32, int CH, bool SG>
 __global__ __launch_bounds__(256) void k_vsmooth(
@@ -2280,7 +2326,8 @@ __global__ __launch_bounds__(256) void k_vsmooth(
     const double4* __restrict__ vel1, const double2* __restrict__ rp, F4In xmf,
     F4In velf1, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
     double4* __restrict__ vel_out, double4* __restrict__ xm_out,
-    const double* scal, unsigned long long* dtmin, int* iscal) {
+    const double* scal, unsigned long long* dtmin, int* iscal, const double* __restrict__ dden_in,
+    double2* __restrict__ rp_out, float2* __restrict__ rpf_out, float4* __restrict__ xrf_out) {
     using X = typename Rec<F32>::X;
     using V = typename Rec<F32>::V;
     using RT = typename std::conditional<F32, float, double>::type;
@@ -2289,6 +2336,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
     const DP& P = SG ? S.P[sg] : P0;
     scal += sg * SC_COUNT;
     iscal += sg * IS_COUNT;
+    const bool eos = iscal[IS_ONGOING] > 0;
     dtmin += sg;
     stencil_offsets(P);
     const size_t N = P.cap;
@@ -2394,6 +2442,21 @@ __global__ __launch_bounds__(256) void k_vsmooth(
             vel_out[i] = vn;
 
             xm_out[i] = make_double4(xi.x + dt * vn.x, xi.y + dt * vn.y, xi.z + dt * vn.z, xi.w);
+            if (P.rk1) {
+                // explicit RK1: Euler.f90:231-241 (dens = dens + dt dden) and
+                // EoS_fluid_loop.f90:59-70 (time_integration.f90:52-65); the
+                // pressure smoothing that follows reads these with the
+                // stage's kernel values, hence the old cell-relative position
+                const double2 rpi = rp[i];
+                const double rho1 = rpi.x + dt * dden_in[i];
+                const double p1 = eos ? P.eps / P.den0 * (rho1 - P.den0) + P.prif : rpi.y;
+                rp_out[i] = make_double2(rho1, p1);
+                if (F32) {
+                    rpf_out[i] = make_float2((float)rho1, (float)p1);
+                    const float4 xo = xmf[i];
+                    xrf_out[i] = make_float4(xo.x, xo.y, xo.z, (float)(rho1 - P.den0));
+                }
+            }
             const unsigned long long b =
                 (unsigned long long)__double_as_longlong(d_dt_cfl(P, vn.x, vn.y, vn.z));
             dtbits = b < dtbits ? b : dtbits;
@@ -2532,18 +2595,22 @@ __global__ __launch_bounds__(256) void k_continuity(
             }
             const double dden = dd - inner;
             dden_out[i] = dden;
-            double rho1 = rhoi + dt * dden;
-            if (P.dthr) {  // Leapfrog_continuity.f90:61-68
-                if (rho1 < 0.9 * P.den0) rho1 = 0.9 * P.den0;
-                if (rho1 > 1.1 * P.den0) rho1 = 1.1 * P.den0;
-            }
-            const double p1 = eos ? P.eps / P.den0 * (rho1 - P.den0) + P.prif : rpi.y;
-            rp_out[i] = make_double2(rho1, p1);
-            if (F32) {
-                rpf_out[i] = make_float2((float)rho1, (float)p1);
-                // (cell-relative position, rho - den0) for the pressure smoothing
-                const float4 xo = xmf[i];
-                xrf_out[i] = make_float4(xo.x, xo.y, xo.z, (float)(rho1 - P.den0));
+            // RK1: the density is integrated after the velocities and positions
+            // (Euler.f90:231-241, in k_vsmooth)
+            if (!P.rk1) {
+                double rho1 = rhoi + dt * dden;
+                if (P.dthr) {  // Leapfrog_continuity.f90:61-68
+                    if (rho1 < 0.9 * P.den0) rho1 = 0.9 * P.den0;
+                    if (rho1 > 1.1 * P.den0) rho1 = 1.1 * P.den0;
+                }
+                const double p1 = eos ? P.eps / P.den0 * (rho1 - P.den0) + P.prif : rpi.y;
+                rp_out[i] = make_double2(rho1, p1);
+                if (F32) {
+                    rpf_out[i] = make_float2((float)rho1, (float)p1);
+                    // (cell-relative position, rho - den0) for the pressure smoothing
+                    const float4 xo = xmf[i];
+                    xrf_out[i] = make_float4(xo.x, xo.y, xo.z, (float)(rho1 - P.den0));
+                }
             }
         }
         __syncthreads();

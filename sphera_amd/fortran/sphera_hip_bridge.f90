@@ -126,6 +126,9 @@ if (Granular_flows_options%KTGF_config>0) p%options = ior(p%options,8)
 #ifdef SPACE_3D
 if (NumBEdges>0) p%options = ior(p%options,16)
 #endif
+! explicit RK (time_split=0): RK1 is built, RK2 (Heun) is refused by the device
+if ((Domain%time_split==0).and.(Domain%RKscheme==1)) p%options = ior(p%options,32)
+if ((Domain%time_split==0).and.(Domain%RKscheme/=1)) p%options = ior(p%options,64)
 end subroutine pack_params
 
 #ifdef SPACE_2D
@@ -287,14 +290,16 @@ type(sph_faces_t) :: f
 real(c_double),allocatable :: xyz(:),uvw(:),rho(:),pp(:),m(:)
 integer(4) :: i
 #ifdef SPACE_2D
-if ((Domain%tipo/="semi").or.(Domain%time_split/=1).or.(NMedium/=1).or.           &
-   Domain%NormFix.or.(NumOpenSides>0).or.(SourceSide/=0).or.dt_alfa_Mon) then
+if ((Domain%tipo/="semi").or.(Domain%time_split<0).or.(Domain%time_split>1).or.    &
+   (NMedium/=1).or.Domain%NormFix.or.(NumOpenSides>0).or.(SourceSide/=0).or.       &
+   dt_alfa_Mon) then
 #else
-if ((Domain%tipo/="semi").or.(Domain%time_split/=1).or.(NMedium/=1).or.           &
-   Domain%NormFix.or.(NumOpenFaces>0).or.(SourceFace/=0).or.dt_alfa_Mon) then
+if ((Domain%tipo/="semi").or.(Domain%time_split<0).or.(Domain%time_split>1).or.    &
+   (NMedium/=1).or.Domain%NormFix.or.(NumOpenFaces>0).or.(SourceFace/=0).or.       &
+   dt_alfa_Mon) then
 #endif
-   write(*,*) "sphera_hip: case outside the device scope (semi, leapfrog, one ",   &
-      "medium, fixed/tapis frontiers)"
+   write(*,*) "sphera_hip: case outside the device scope (semi, leapfrog or ",     &
+      "explicit RK, one medium, fixed/tapis frontiers)"
    stop 1
 endif
 call pack_params(prec,p)

@@ -20,7 +20,8 @@ LIB_PATH = Path(os.environ.get("SPHERA_HIP_LIB") or
 
 FP64, FP32_MIXED = 0, 1
 # deck options (SPH_OPT_* of sphera_hip.h)
-OPT_DENSITY_THRESHOLDS, OPT_ALE3, OPT_C1, OPT_KTGF, OPT_CONVEX_EDGES = 1, 2, 4, 8, 16
+OPT_DENSITY_THRESHOLDS, OPT_ALE3, OPT_C1, OPT_KTGF, OPT_CONVEX_EDGES, OPT_RK1, OPT_RK2 = \
+    1, 2, 4, 8, 16, 32, 64
 PD = C.POINTER(C.c_double)
 PI = C.POINTER(C.c_int32)
 
@@ -200,6 +201,8 @@ def params_struct(tb, precision=FP64, capacity=None, semi=True, tune=None):
     p.tetap, p.tetav = case.teta_p, case.teta_v
     p.semi = 1 if semi else 0
     p.options = OPT_DENSITY_THRESHOLDS if int(getattr(case, "density_thresholds", 0)) == 1 else 0
+    if int(getattr(case, "time_split", 1)) == 0:
+        p.options |= OPT_RK1
     for k, v in (tune or {}).items():
         setattr(p, k, int(v))
     return p

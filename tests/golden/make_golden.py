@@ -5,8 +5,8 @@ from /root/reference/src) on project-authored decks.
 
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
-With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp)
-only those fixtures are written.
+With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp,
+rk1) only those fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -34,6 +34,9 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         state, lists and boundary table after 0, 4 and 8 steps
   ref_nearwall_2d.npz   the same for the 2D build (600 particles, floor and
                         x = 0 side), after 0, 4 and 8 steps
+  ref_rk1.npz           1440 particles, jittered lattice (seed 20260101), the
+                        explicit RK1 branch (time_split = 0, RKscheme = 1:
+                        Euler); state after 0, 6 and 12 steps
   ref_dclamp.npz        1440 particles with density_thresholds = 1 and a
                         velocity field converging on the left half of the
                         column and diverging on the right (150/s, jittered
@@ -280,9 +283,20 @@ def make_dclamp():
     np.savez_compressed(HERE / "ref_dclamp.npz", **out)
 
 
+def make_rk1():
+    """explicit RK1 (Euler) time integration"""
+    case = DamBreak3D(**SMALL, time_split=0)
+    tb = build(case)
+    ic = jitter(tb.particles["coord"], case.dx)
+    hdr, steps = read_dump(run_harness(case, nsteps=12, every=6, lists=False, ic_coord=ic))
+    out = {"ic_coord": ic}
+    _pack_steps(steps, out, lists=False)
+    np.savez_compressed(HERE / "ref_rk1.npz", **out)
+
+
 MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_removal,
           "pelo": make_pelo, "2d": make_2d, "nearwall": make_nearwall,
-          "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp}
+          "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp, "rk1": make_rk1}
 
 
 def main():

@@ -196,7 +196,7 @@ def test_lists_match_oracle_on_clouds(kind):
         _lists_equal(d, o)
         d.close()
     # the other launch shapes of the tiled search (workgroup x LDS tile)
-    for shape in (1, 2, 3):
+    for shape in (1, 2, 3, 4):
         d = rt.SphContext(tb, precision=rt.FP32_MIXED, capacity=n, tune=dict(nb_shape=shape))
         d.upload(coord=coord, **kw)
         d.preloop()

@@ -81,3 +81,12 @@ def test_params_carry_density_thresholds():
                       density_thresholds=1)
     assert rt.params_struct(build(case)).options == rt.OPT_DENSITY_THRESHOLDS
     assert rt.params_struct(_tiny()).options == 0
+
+
+def test_rk2_refused_rk1_accepted_in_params():
+    rc, msg = _create(_tiny(), options=rt.OPT_RK2)
+    assert rc == 95 and "Heun" in msg, (rc, msg)
+    rc, msg = _create(_tiny(), options=rt.OPT_RK1 | rt.OPT_RK2)
+    assert rc == 2
+    case = DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny", time_split=0)
+    assert rt.params_struct(build(case)).options == rt.OPT_RK1

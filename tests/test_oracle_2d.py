@@ -117,3 +117,35 @@ def test_oracle2d_full_dam_break_vs_live_reference():
             assert np.array_equal(mine[f], st[f]), (st["it"], f)
         assert mine["dt"] == st["dt"]
     assert done == 60
+
+
+REFERENCE2D = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "sphera2d.x"
+
+
+@pytest.mark.skipif(not (HARNESS2D.exists() and REFERENCE2D.exists()), reason="2D reference not built here")
+@pytest.mark.parametrize("time_split", [1, 0])
+def test_2d_harness_and_oracle_equal_unmodified_reference_program(time_split, tmp_path):
+    """The reference's 2D program as shipped (its own time_step_loop, leapfrog
+    or explicit RK1 with TetaV = 0.2; in 2D its time_integration calls Euler
+    with the matching interface) writes results every 5 steps: the harness's
+    dump and the oracle equal them bit for bit (positions, velocities,
+    pressures, densities, time, dt)."""
+    from oracle.decks import read_dump, read_ris, run_harness, run_reference
+    case = case2d.DamBreak2D(**CASE2D, time_split=time_split)
+    wd = run_reference(case, itmax=20, results_step=5, workdir=tmp_path / "ref")
+    ris = read_ris(wd / "t2d.ris")
+    assert [r["it"] for r in ris] == [5, 10, 15, 20]
+    hdr, steps = read_dump(run_harness(case, nsteps=20, every=5, lists=False,
+                                       workdir=tmp_path / "harness"))
+    by_it = {s["it"]: s for s in steps}
+    o = Oracle(case2d.build(case))
+    o.preloop()
+    prev = 0
+    for r in ris:
+        s = by_it[r["it"]]
+        o.step(r["it"] - prev)
+        prev = r["it"]
+        st = o.state()
+        assert r["time"] == s["time"] == st["time"] and r["dt"] == s["dt"] == st["dt"], r["it"]
+        for f in ("coord", "vel", "pres", "dens"):
+            assert np.array_equal(r[f], s[f]) and np.array_equal(r[f], st[f]), (r["it"], f)

@@ -302,3 +302,40 @@ def test_harness_equals_unmodified_reference_program(tmp_path):
         assert r["time"] == s["time"] and r["dt"] == s["dt"], r["it"]
         for f in ("coord", "vel", "pres", "dens", "mass"):
             assert np.array_equal(r[f], s[f]), (r["it"], f)
+
+
+def test_golden_rk1_bit_identical():
+    """The explicit RK1 branch (time_split = 0, RKscheme = 1: Euler.f90:61-241
+    then EoS_fluid_loop, with the lists, boundary table and kernel values of
+    the stage's start in the pressure smoothing) against the reference's
+    fixture: fields, dt and time bit-identical after 6 and 12 steps."""
+    g = np.load(GOLD / "ref_rk1.npz")
+    case = _small()
+    case.time_split = 0
+    tb = build(case)
+    o = Oracle(tb, coord=g["ic_coord"])
+    o.preloop()
+    _oracle_steps(o, g, (0, 6, 12))
+
+
+@pytest.mark.skipif(not (HARNESS.exists() and REFERENCE.exists()), reason="reference not built here")
+def test_rk1_harness_equals_unmodified_reference_program(tmp_path):
+    """The harness's RK1 branch against the reference program's own loop.  The
+    3D program reaches Euler(BC_zmax_flag) through time_integration's call
+    without the argument (time_integration.f90:54) and faults where
+    velocity_smoothing reads it, i.e. when TetaV > 0; with TetaV = 0 it runs,
+    and its results file equals the harness's dump bit for bit."""
+    from oracle.decks import read_dump, read_ris, run_harness, run_reference
+    case = DamBreak3D(tank=(0.5, 0.2, 0.3), column=(0.15, 0.2, 0.15), dx=0.01, name="rk",
+                      time_split=0, teta_v=0.0)
+    wd = run_reference(case, itmax=20, results_step=5, workdir=tmp_path / "ref")
+    ris = read_ris(wd / "rk.ris")
+    assert [r["it"] for r in ris] == [5, 10, 15, 20]
+    hdr, steps = read_dump(run_harness(case, nsteps=20, every=5, lists=False,
+                                       workdir=tmp_path / "harness"))
+    by_it = {s["it"]: s for s in steps}
+    for r in ris:
+        s = by_it[r["it"]]
+        assert r["time"] == s["time"] and r["dt"] == s["dt"], r["it"]
+        for f in ("coord", "vel", "pres", "dens", "mass"):
+            assert np.array_equal(r[f], s[f]), (r["it"], f)
