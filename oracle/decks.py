@@ -28,125 +28,18 @@ HARNESS2D_HIP = REF_DIR / "harness2d_hip.x"
 REFERENCE2D = REF_DIR / "sphera2d.x"
 
 
-def _f(x):
-    return repr(float(x))
+from sphera_amd.deck import deck_text, deck_text_2d  # noqa: E402,F401  (the v10 deck writer)
 
 
-def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
-    s = case.scalars()
-    v = case.vertices()
-    lines = ["10.0.0", "##### TITLE #####", case.name, "##### END TITLE #####",
-             "##### DOMAIN #####", "semi", f"{_f(case.dx)} {_f(case.trunc)}", ".false.",
-             "##### END DOMAIN #####", "##### VERTICES #####"]
-    for i, p in enumerate(v, start=1):
-        lines.append(f"{i} {_f(p[0])} {_f(p[1])} {_f(p[2])}")
-    lines += ["##### END VERTICES #####", "##### FACES #####"]
-    for i, r in enumerate(case.faces(), start=1):
-        lines.append(f"{i} {r[0]} {r[1]} {r[2]} {r[3]} -1 -1 {r[4]}")
-    lines += ["##### END FACES #####", "##### BOUNDARIES #####",
-              "Boundary", "1", "fixed", "1 .false.", _f(case.slip_coeff), "111111", "1 8",
-              "Reservoir", "2", "perimeter", "1", "uniform 009EA8",
-              "std 0. 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[2])} 0.", "1 0 .false.",
-              "##### END BOUNDARIES #####",
-              "##### DBSPH #####", "1. .false. 0.5 0 .false.", ".false. .true.", "0 0",
-              "0 .false.", "0 0 6", "##### END DBSPH #####",
-              "##### BED LOAD TRANSPORT #####", "0 1", "1 0. 0.",
-              "##### end BED LOAD TRANSPORT #####",
-              "##### medium #####", "liquid", "1", f"{_f(case.rho0)} {_f(case.bulk)}",
-              f"{_f(case.alfa_mon)} 0.", _f(case.mu), "##### end medium #####",
-              "##### RUN PARAMETERS #####", f"{_f(tmax)} {int(itmax)}",
-              f"{_f(case.cfl)} {_f(case.vsc_coeff)} {int(getattr(case, 'time_split', 1))} 1 "
-              f"{_f(case.pesodt)} .false.",
-              f"{_f(case.teta_p)} {_f(case.teta_v)} A",
-              f"1.1 {_f(case.coefnmaxpartj)} 0",
-              ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.",
-              f"1000000 {int(case.maxcloseboundfaces)} 100 100000000",
-              str(int(getattr(case, "density_thresholds", 0))),
-              "##### end RUN PARAMETERS #####",
-              "##### general physical properties #####",
-              f"{_f(case.grav[0])} {_f(case.grav[1])} {_f(case.grav[2])}", _f(case.p_ref),
-              "##### end general physical properties #####",
-              "##### restart #####", "##### end restart #####",
-              "##### output regulation #####",
-              f"results step {int(results_step)}" if results_step else "results time 100000.",
-              "restart time 100000.", "print partial 100000000", "control time 100000.",
-              "level time 100000. medium 1", "depth dt_out 100000.",
-              "##### end output regulation #####",
-              "##### draw options #####", "vtkconverter any 100000.",
-              "##### end draw options #####",
-              "##### control points #####", "##### end control points #####",
-              "##### control lines #####"]
-    for label, first, second, ndiv in case.control_lines:
-        lines += [str(label)[:8], " ".join(_f(v) for v in first), " ".join(_f(v) for v in second),
-                  str(int(ndiv))]
-    lines += ["##### end control lines #####",
-              "##### section flow rate #####", "0 999. 1", "##### end section flow rate #####",
-              "##### substations #####", "0 999.", "##### end substations #####"]
-    del s
-    return "\n".join(lines) + "\n"
-
-
-def deck_text_2d(case, itmax=1000000000, tmax=1000.0, results_step=None):
-    """v10 deck of a DamBreak2D for the reference's 2D build: vertices (x, z),
-    a LINES section instead of FACES, gravity and monitor points with two
-    components, and none of the 3D-only lines (the CLC flag of
-    ReadInputDomain.f90:109-113, MAXCLOSEBOUNDFACES of
-    ReadInputRunParameters.f90:70-76)."""
-    v = case.vertices()
-    lines = ["10.0.0", "##### TITLE #####", case.name, "##### END TITLE #####",
-             "##### DOMAIN #####", "semi", f"{_f(case.dx)} {_f(case.trunc)}",
-             "##### END DOMAIN #####", "##### VERTICES #####"]
-    for i, p in enumerate(v, start=1):
-        lines.append(f"{i} {_f(p[0])} {_f(p[1])}")
-    lines += ["##### END VERTICES #####", "##### LINES #####"]
-    for lid, vids in sorted(case.lines().items()):
-        lines.append(f"{lid} " + " ".join(str(i) for i in vids))
-    lines += ["##### END LINES #####", "##### BOUNDARIES #####",
-              "Boundary", "1", "fixed", "1 .false.", _f(case.slip_coeff), "111111",
-              "Reservoir", "2", "perimeter", "1", "uniform 009EA8",
-              "std 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[1])}", "1 0 .false.",
-              "##### END BOUNDARIES #####",
-              "##### DBSPH #####", "1. .false. 0.5 0 .false.", ".false. .true.", "0 0",
-              "0 .false.", "0 0 6", "##### END DBSPH #####",
-              "##### BED LOAD TRANSPORT #####", "0 1", "1 0. 0.",
-              "##### end BED LOAD TRANSPORT #####",
-              "##### medium #####", "liquid", "1", f"{_f(case.rho0)} {_f(case.bulk)}",
-              f"{_f(case.alfa_mon)} 0.", _f(case.mu), "##### end medium #####",
-              "##### RUN PARAMETERS #####", f"{_f(tmax)} {int(itmax)}",
-              f"{_f(case.cfl)} {_f(case.vsc_coeff)} {int(getattr(case, 'time_split', 1))} 1 "
-              f"{_f(case.pesodt)} .false.",
-              f"{_f(case.teta_p)} {_f(case.teta_v)} A",
-              f"1.1 {_f(case.coefnmaxpartj)} 0",
-              ".false. .false. .false. .false. .false. 0.614 .false. 50. .false.",
-              str(int(getattr(case, "density_thresholds", 0))),
-              "##### end RUN PARAMETERS #####",
-              "##### general physical properties #####",
-              f"{_f(case.grav[0])} {_f(case.grav[2])}", _f(case.p_ref),
-              "##### end general physical properties #####",
-              "##### restart #####", "##### end restart #####",
-              "##### output regulation #####",
-              f"results step {int(results_step)}" if results_step else "results time 100000.",
-              "restart time 100000.", "print partial 100000000", "control time 100000.",
-              "level time 100000. medium 1", "depth dt_out 100000.",
-              "##### end output regulation #####",
-              "##### draw options #####", "vtkconverter any 100000.",
-              "##### end draw options #####",
-              "##### control points #####", "##### end control points #####",
-              "##### control lines #####"]
-    for label, first, second, ndiv in case.control_lines:
-        lines += [str(label)[:8], " ".join(_f(v) for v in first), " ".join(_f(v) for v in second),
-                  str(int(ndiv))]
-    lines += ["##### end control lines #####"]
-    return "\n".join(lines) + "\n"
-
-
-def write_case_dir(case, root, itmax=1000000000, results_step=None):
+def write_case_dir(case, root, itmax=1000000000, results_step=None, text=None):
     """Lay out a run directory the reference can start from: it concatenates
-    ./input/*/*.inp into <case>.inp (src/Main_algorithm/sphera.f90:222)."""
+    ./input/*/*.inp into <case>.inp (src/Main_algorithm/sphera.f90:222).
+    text: a deck to run instead of the one written from `case`."""
     root = Path(root)
     (root / "input" / "00_deck").mkdir(parents=True, exist_ok=True)
-    text = deck_text_2d(case, itmax=itmax, results_step=results_step) if getattr(case, "dim", 3) == 2 else \
-        deck_text(case, itmax=itmax, results_step=results_step)
+    if text is None:
+        text = deck_text_2d(case, itmax=itmax, results_step=results_step) if getattr(case, "dim", 3) == 2 \
+            else deck_text(case, itmax=itmax, results_step=results_step)
     (root / "input" / "00_deck" / "deck.inp").write_text(text)
     (root / "output").mkdir(exist_ok=True)
     (root / "restart").mkdir(exist_ok=True)
@@ -154,14 +47,14 @@ def write_case_dir(case, root, itmax=1000000000, results_step=None):
 
 
 def run_harness(case, nsteps, every=1, lists=True, ic_coord=None, threads=None,
-                workdir=None, timeout=3600, exe=None, extra_env=None, ic_vel=None):
+                workdir=None, timeout=3600, exe=None, extra_env=None, ic_vel=None, text=None):
     """Run the reference harness (or ``exe``, e.g. the device drop-in harness
     HARNESS_HIP); returns the path of its dump file."""
     exe = Path(exe or (HARNESS2D if getattr(case, "dim", 3) == 2 else HARNESS))
     if not exe.exists():
         raise FileNotFoundError(f"{exe} not built (make -f oracle/Makefile.ref)")
     wd = Path(workdir or tempfile.mkdtemp(prefix="sph_ref_")).resolve()
-    write_case_dir(case, wd)
+    write_case_dir(case, wd, text=text)
     env = dict(os.environ)
     env["SPH_HARNESS_OUT"] = str(wd / "dump.bin")
     env["SPH_HARNESS_STEPS"] = str(int(nsteps))
@@ -211,43 +104,8 @@ def run_reference(case, itmax, results_step, workdir=None, threads=None, timeout
     return wd
 
 
-def fortran_records(path):
-    """Records of a Fortran sequential unformatted file (4-byte little-endian
-    length markers before and after each record)."""
-    b = Path(path).read_bytes()
-    o, out = 0, []
-    while o < len(b):
-        (n,) = struct.unpack_from("<i", b, o)
-        rec = b[o + 4:o + 4 + n]
-        (m,) = struct.unpack_from("<i", b, o + 4 + n)
-        if m != n:
-            raise ValueError("bad record markers")
-        out.append(rec)
-        o += 8 + n
-    return out
-
-
-def read_ris(path):
-    """The visualisation records (restartcode = 0) of a reference results
-    file (Memo_Results.f90:279-287): per record it, time, dt, nag and the
-    per-particle coord (n, 3), vel (n, 3), pres, dens, mass, kin_visc, imed,
-    icol.  Full-state records (restartcode = 1, the first and last step) are
-    skipped."""
-    recs = fortran_records(path)
-    out = []
-    for k, r in enumerate(recs):
-        if len(r) != 36 or k + 1 >= len(recs):
-            continue
-        it, t, dt, nag, code, _ = struct.unpack("<iddiid", r)
-        if code != 0 or len(recs[k + 1]) != nag * (10 * 8 + 2 * 4):
-            continue
-        d = recs[k + 1]
-        f = np.frombuffer(d, "<f8", 10 * nag).reshape(10, nag)
-        i = np.frombuffer(d, "<i4", 2 * nag, offset=80 * nag).reshape(2, nag)
-        out.append(dict(it=it, time=t, dt=dt, nag=nag, coord=f[0:3].T.copy(), vel=f[3:6].T.copy(),
-                        pres=f[6].copy(), dens=f[7].copy(), mass=f[8].copy(),
-                        kin_visc=f[9].copy(), imed=i[0].copy(), icol=i[1].copy()))
-    return out
+from sphera_amd.results import read_results as read_ris  # noqa: E402,F401
+from sphera_amd.results import records as fortran_records  # noqa: E402,F401
 
 
 def read_pelo(workdir, case):

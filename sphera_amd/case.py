@@ -109,6 +109,16 @@ class DamBreak3D:
     def zone_types(self):
         return {1: "fixe", 2: "peri"}
 
+    def fluid_zone_box(self):
+        """Lower and upper corner of the perimeter zone the lattice fills
+        (the column box: vertices 9-16)."""
+        col = self.vertices()[8:16]
+        return col.min(0), col.max(0)
+
+    def free_surface_height(self):
+        """The "qp" height of the perimeter zone's hydrostatic pressure."""
+        return float(self.column[2])
+
 
 # ---------------------------------------------------------------------------
 # Grid: CreaGrid.f90:58-74
@@ -394,8 +404,7 @@ def initial_particles(case: DamBreak3D):
     s = case.scalars()
     vert = case.vertices()
     xmin_all = vert.min(0)
-    col = vert[8:16]
-    zmin, zmax = col.min(0), col.max(0)
+    zmin, zmax = case.fluid_zone_box()
     npps, xres = [], []
     for i in range(3):
         nbefore = fortran_nint((zmin[i] - xmin_all[i]) / case.dx)
@@ -412,7 +421,7 @@ def initial_particles(case: DamBreak3D):
     gmod = math.sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2])
     coshor = -g[2] / gmod
     senhor = g[0] / gmod
-    zq = float(case.column[2])
+    zq = case.free_surface_height()
     affond = (zq - coord[:, 2]) * coshor + coord[:, 0] * senhor
     pres = (affond * case.rho0 * gmod) + case.p_ref
     dens = case.rho0 * (1.0 + (pres - case.p_ref) / case.bulk)
