@@ -141,6 +141,10 @@ int sph_particle_count(sph_ctx* ctx, int32_t* n);
  * returned row (rows are sorted by it). */
 int sph_state_download(sph_ctx* ctx, double* coord, double* vel, double* acc, double* dens,
                        double* pres, double* dden, double* vpres, int32_t* ids);
+/* Laminar flags (pg%laminar_flag, set by the momentum pass,
+ * inter_EqMoto.f90:269-283) in pg order, at a time-step boundary; the
+ * reference writes them to its Paraview files. */
+int sph_get_laminar(sph_ctx* ctx, int32_t* lam);
 /* dt, dt_average, simulation_time, it, pair count of the current lists */
 int sph_get_scalars(sph_ctx* ctx, double out[5]);
 /* Fluid global quantities (fluid_global_quantities.f90:66-72): mass, linear

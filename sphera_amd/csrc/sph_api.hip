@@ -1092,6 +1092,27 @@ int sph_segment_download(sph_ctx* c, int32_t seg, double* coord, double* vel, do
     return 0;
 }
 
+int sph_get_laminar(sph_ctx* c, int32_t* lam) {
+    if (!c || !lam) return fail(1, "null argument");
+    if (c->nseg > 1) return fail(2, "not available on a segmented context");
+    if (c->phase != PH_SORTED && c->phase != PH_READY && c->phase != PH_PS)
+        return fail(121, "the laminar flags are taken at a time-step boundary");
+    HIPC(hipStreamSynchronize(c->stream));
+    int n;
+    HIPC(hipMemcpy(&n, c->iscal + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> hl(n);
+    std::vector<double4> hv(n);
+    if (n) {
+        // the momentum pass writes lam[0]; every sort gathers it into lam[1]
+        // (in the leapfrog step before the next momentum pass reads it, in
+        // the RK1 step at its end), so lam[1] holds the current flags
+        HIPC(hipMemcpy(hl.data(), c->lam[1], sizeof(int) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hv.data(), c->vel[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
+    }
+    for (int s = 0; s < n; ++s) lam[(int)hv[s].w] = hl[s];
+    return 0;
+}
+
 int sph_get_scalars(sph_ctx* c, double out[5]) { return sph_segment_scalars(c, 0, out); }
 
 int sph_segment_scalars(sph_ctx* c, int32_t seg, double out[5]) {

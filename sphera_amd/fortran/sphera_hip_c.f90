@@ -81,6 +81,11 @@ interface
       real(c_double),intent(out) :: dden(*), vpres(*)
       integer(c_int32_t),intent(out) :: ids(*)
    end function
+   integer(c_int) function sph_get_laminar(ctx,lam) bind(C,name="sph_get_laminar")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr),value :: ctx
+      integer(c_int32_t),intent(out) :: lam(*)
+   end function
    integer(c_int) function sph_get_scalars(ctx,out) bind(C,name="sph_get_scalars")
       import :: c_int, c_ptr, c_double
       type(c_ptr),value :: ctx

@@ -27,7 +27,7 @@ PI = C.POINTER(C.c_int32)
 
 EXPORTS = [
     "sph_ctx_create", "sph_ctx_create2d", "sph_ctx_destroy", "sph_state_upload", "sph_particle_count",
-    "sph_state_download", "sph_get_scalars", "sph_pair_counter", "sph_global_quantities",
+    "sph_state_download", "sph_get_scalars", "sph_get_laminar", "sph_pair_counter", "sph_global_quantities",
     "sph_global_quantities_noalloc",
     "sph_monitor_lines", "sph_free_surface", "sph_get_neighbour_lists", "sph_get_boundary_table", "sph_dt", "sph_momentum",
     "sph_leapfrog_mom", "sph_velocity_smoothing", "sph_leapfrog_traj", "sph_sort", "sph_neighbours",
@@ -90,6 +90,7 @@ def load():
     L.sph_particle_count.argtypes = [C.c_void_p, PI]
     L.sph_state_download.argtypes = [C.c_void_p, PD, PD, PD, PD, PD, PD, PD, PI]
     L.sph_get_scalars.argtypes = [C.c_void_p, PD]
+    L.sph_get_laminar.argtypes = [C.c_void_p, PI]
     L.sph_get_neighbour_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
     L.sph_get_boundary_table.argtypes = [C.c_void_p, PI, PI, PD, PD, PD, C.c_int32]
     L.sph_pair_counter.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
@@ -303,6 +304,12 @@ class SphContext:
         out.update(sc)
         out["nag"] = n
         return out
+
+    def laminar(self):
+        """Laminar flags in pg order (pg%laminar_flag)."""
+        v = np.zeros(self.n, np.int32)
+        self._check(self.L.sph_get_laminar(self.h, _pi(v)), "sph_get_laminar")
+        return v
 
     def scalars(self):
         sc = np.zeros(5)

@@ -6,7 +6,7 @@ from /root/reference/src) on project-authored decks.
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
 With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp,
-rk1) only those fixtures are written.
+rk1, viscous) only those fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -34,6 +34,10 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         state, lists and boundary table after 0, 4 and 8 steps
   ref_nearwall_2d.npz   the same for the 2D build (600 particles, floor and
                         x = 0 side), after 0, 4 and 8 steps
+  ref_viscous.npz       800 particles, dynamic viscosity 20 Pa s and a shear
+                        flow u = 3 z: the laminar flag of inter_EqMoto.f90:
+                        269-283 is set for hundreds of particles and feeds the
+                        wall shear term; state after 0, 3 and 6 steps
   ref_rk1.npz           1440 particles, jittered lattice (seed 20260101), the
                         explicit RK1 branch (time_split = 0, RKscheme = 1:
                         Euler); state after 0, 6 and 12 steps
@@ -283,6 +287,19 @@ def make_dclamp():
     np.savez_compressed(HERE / "ref_dclamp.npz", **out)
 
 
+def make_viscous():
+    """viscous shear flow: laminar flags"""
+    case = DamBreak3D(**TINY, mu=20.0)
+    tb = build(case)
+    c = tb.particles["coord"]
+    iv = np.zeros_like(c)
+    iv[:, 0] = 3.0 * c[:, 2]
+    hdr, steps = read_dump(run_harness(case, nsteps=6, every=3, lists=False, ic_vel=iv))
+    out = {"ic_vel": iv}
+    _pack_steps(steps, out, lists=False)
+    np.savez_compressed(HERE / "ref_viscous.npz", **out)
+
+
 def make_rk1():
     """explicit RK1 (Euler) time integration"""
     case = DamBreak3D(**SMALL, time_split=0)
@@ -296,7 +313,8 @@ def make_rk1():
 
 MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_removal,
           "pelo": make_pelo, "2d": make_2d, "nearwall": make_nearwall,
-          "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp, "rk1": make_rk1}
+          "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp, "rk1": make_rk1,
+          "viscous": make_viscous}
 
 
 def main():

@@ -494,3 +494,42 @@ def test_full_size_300_steps():
         assert all(exact.values()) and ds["dt"] == os_["dt"] and ds["time"] == os_["time"], exact
         for f in FIELDS:
             assert e32[f] <= FP32_TOL[f], (it, f, e32[f])
+
+
+def test_laminar_flags_and_output_files(tmp_path):
+    """A viscous shear flow (fixture ref_viscous.npz: 20 Pa s, u = 3 z, the
+    laminar flag set for hundreds of particles): FP64 fields equal the
+    reference's after 3 and 6 steps, the laminar flags the device keeps
+    (sph_get_laminar) equal the reference's, and the results and Paraview
+    files written from the device state equal the ones written from the
+    reference's state (which equal the unmodified reference program's own
+    files, tests/test_io_formats.py)."""
+    import io
+    from sphera_amd import results
+    g = np.load(GOLD / "ref_viscous.npz")
+    tb = build(DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny", mu=20.0))
+    rt = _rt()
+    d = rt.SphContext(tb, precision=rt.FP64)
+    d.upload(vel=g["ic_vel"])
+    d.preloop()
+    for it in (3, 6):
+        d.step(3)
+        st = d.state()
+        lam = d.laminar()
+        for f in FIELDS:
+            assert np.array_equal(st[f], g[f"s{it}_" + f]), (it, f)
+        assert np.array_equal(lam, g[f"s{it}_laminar_flag"]), it
+    d.close()
+    assert int(lam.sum()) > 100
+    ref = {k: g["s6_" + k] for k in ("coord", "vel", "pres", "dens")}
+    ref["mass"] = st["mass"] = tb.particles["mass"]
+    kin = tb.scalars["kin_visc"]
+    for writer in ("ris", "vtu"):
+        a, b = (io.BytesIO(), io.BytesIO()) if writer == "ris" else (io.StringIO(), io.StringIO())
+        if writer == "ris":
+            results.write_results(a, 6, st["time"], st["dt"], st, kin)
+            results.write_results(b, 6, float(g["s6_time"]), float(g["s6_dt"]), ref, kin)
+        else:
+            results.write_vtu(a, st, kin, lam)
+            results.write_vtu(b, ref, kin, g["s6_laminar_flag"])
+        assert a.getvalue() == b.getvalue(), writer

@@ -152,3 +152,21 @@ def test_results_records_equal_the_reference_program(tmp_path):
         results.write_results(fh, it, s["time"], s["dt"], s, kin_visc=kin)
         mine = results.records_from_bytes(fh.getvalue())
         assert mine[0] == recs[k] and mine[1] == recs[k + 1], it
+
+
+@pytest.mark.skipif(not ((REF / "harness3d.x").exists() and (REF / "sphera3d.x").exists()),
+                    reason="reference not built here")
+def test_vtu_equals_the_reference_program(tmp_path):
+    """The unmodified reference program writes the particle file at the start
+    (block 0) and after its last step (block 1; its loop runs itmax + 1
+    steps): the files written here from the harness's state at those steps
+    are the same text."""
+    from oracle.decks import read_dump, run_harness, run_reference
+    case = DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="vt")
+    wd = run_reference(case, itmax=4, results_step=2, workdir=tmp_path / "ref")
+    hdr, steps = read_dump(run_harness(case, nsteps=5, every=5, lists=False, workdir=tmp_path / "h"))
+    assert [s["it"] for s in steps] == [0, 5]
+    for blk, st in enumerate(steps):
+        fh = io.StringIO()
+        results.write_vtu(fh, st, case.scalars()["kin_visc"], st["laminar_flag"])
+        assert fh.getvalue() == (wd / f"VTKConverter_vt_block_{blk}.vtu").read_text(), blk

@@ -339,3 +339,17 @@ def test_rk1_harness_equals_unmodified_reference_program(tmp_path):
         assert r["time"] == s["time"] and r["dt"] == s["dt"], r["it"]
         for f in ("coord", "vel", "pres", "dens", "mass"):
             assert np.array_equal(r[f], s[f]), (r["it"], f)
+
+
+def test_golden_viscous_laminar_flags_bit_identical():
+    """A viscous shear flow (20 Pa s, u = 3 z) in which the laminar flag of
+    inter_EqMoto.f90:269-283 is set for hundreds of particles: fields, dt and
+    time equal the reference's after 3 and 6 steps."""
+    g = np.load(GOLD / "ref_viscous.npz")
+    case = _tiny()
+    case.mu = 20.0
+    tb = build(case)
+    o = Oracle(tb, vel=g["ic_vel"])
+    o.preloop()
+    _oracle_steps(o, g, (0, 3, 6))
+    assert int(g["s6_laminar_flag"].sum()) > 100
