@@ -68,9 +68,10 @@ typedef struct sph_params {
  * SPH_OPT_DENSITY_THRESHOLDS is built: the 0.9/1.1 den0 clamp of
  * Leapfrog_continuity.f90:61-68; so is SPH_OPT_RK1.  The others are not
  * built on the device: sph_ctx_create / sph_ctx_create2d / sph_batch_create
- * refuse a context that sets them with status 95 (and so does a frontier with
- * slip_coefficient_mode = 2, the SA-SPH wall function), rather than running
- * the case without them. */
+ * refuse a context that sets them with status 95, rather than running the
+ * case without them.  A frontier's slip_mode is 0, 1 or 2 (2: the SA-SPH
+ * wall function of wall_function_for_SASPH.f90, built); other values are
+ * refused with status 2. */
 enum {
     SPH_OPT_DENSITY_THRESHOLDS = 1, /* density_thresholds == 1 */
     SPH_OPT_ALE3 = 2,               /* ALE3 (ALE3_term_momentum.f90, Leapfrog_continuity.f90:70-86) */

@@ -262,7 +262,9 @@ class Oracle:
 
     def branch_counts(self):
         """How often the rare branches ran: elastic wall reactions applied,
-        zpmin clamps of the SA-SPH volume integrals, density_thresholds clamps."""
-        v = (C.c_int64 * 3)()
+        zpmin clamps of the SA-SPH volume integrals, density_thresholds clamps,
+        wall-function interactions below z0 e and on the logarithmic branch."""
+        v = (C.c_int64 * 5)()
         lib().or_branch_counts(self.ctx, v)
-        return dict(elastic=int(v[0]), zpmin=int(v[1]), density_clamp=int(v[2]))
+        return dict(elastic=int(v[0]), zpmin=int(v[1]), density_clamp=int(v[2]),
+                    wall_fn_near=int(v[3]), wall_fn_log=int(v[4]))

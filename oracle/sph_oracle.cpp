@@ -57,8 +57,8 @@ struct Ctx {
     bool lists_built = false;
     // how often the rare branches ran (checker counters, read by or_branch_counts):
     // elastic wall reactions applied, zpmin clamps of the volume integrals,
-    // density_thresholds clamps
-    mutable std::atomic<long long> n_elastic{0}, n_zpmin{0}, n_dclamp{0};
+    // density_thresholds clamps, wall-function interactions below z0 e and above
+    mutable std::atomic<long long> n_elastic{0}, n_zpmin{0}, n_dclamp{0}, n_wf_near{0}, n_wf_log{0};
     // lists only: CalcVarLength stores the neighbour lists without rag and
     // PartKernel (list checks of large clouds; the passes cannot run)
     bool lists_only = false;
@@ -294,6 +294,35 @@ const double DELTAX_BIT = 0.1;                   // SA_SPH_module.f90:28
 
 inline double ST(const Ctx& c, int s, int r, int col) { return c.sT[9 * s + col * 3 + r]; }
 inline double SRN(const Ctx& c, int s, int r, int col) { return c.sRN[9 * s + col * 3 + r]; }
+
+// wall_function_for_SASPH.f90:56-80 (no land-cover map): slip coefficient and
+// mixing-length viscosity of one particle-frontier pair at distance r; d50 is
+// ten times the frontier's roughness length (the callers pass
+// BC_shear_stress_input * 10, AddBoundaryContributions_to_ME3D.f90:222-224).
+// Nepero_number and k_v: Static_allocation_module.f90:51,53.
+// Returns 0 outside the wall layer, 1 below z0 e, 2 on the logarithmic branch.
+int wall_function(double u_t, double d50, double r, double dx, double& slip, double& nut) {
+    const double e = 2.718281828, kv2 = 0.41 * 0.41;
+    slip = 0.0;
+    nut = 1.0e-15;
+    if (r <= 0.5 * dx) {
+        const double z0 = d50 / 10.0;
+        if (r < z0 * e) {
+            slip = 1.0;
+            nut = kv2 * u_t * z0 * e;
+            return 1;
+        }
+        slip = 1.0 / std::log(r / z0);
+        nut = kv2 * u_t * r / std::log(r / z0);
+        return 2;
+    }
+    return 0;
+}
+
+void count_wall_function(const Ctx& c, int branch) {
+    if (branch == 1) c.n_wf_near.fetch_add(1, std::memory_order_relaxed);
+    if (branch == 2) c.n_wf_log.fetch_add(1, std::memory_order_relaxed);
+}
 
 // w.f90:48-57
 double w_fn(double r, double h, double coef) {
@@ -1013,6 +1042,13 @@ int or_momentum(void* ctx) {
                     if (c.sslipmode[sd] == 1) {
                         slip = c.sslip[sd];
                         cinv = c.p.kin_visc;
+                    } else if (c.sslipmode[sd] == 2) {
+                        // AddBoundaryContributions_to_ME2D.f90:314-326
+                        double ut[3];
+                        for (int k = 0; k < 3; ++k)
+                            ut[k] = (vi[k] - c.sfvel[3 * sd + k]) - (0.5 * DvelN * ST(c, sd, k, 2));
+                        const double u = std::sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+                        count_wall_function(c, wall_function(u, c.sslip[sd] * 10.0, ypi, c.p.dx, slip, cinv));
                     }
                     const double Monvisc = c.p.alfamon * c.p.celerita * h;
                     const double TN = Monvisc * DvelN * IntWd3s0;
@@ -1087,6 +1123,12 @@ int or_momentum(void* ctx) {
                 if (c.slip_mode[f] == 1) {
                     slip = c.slip_coeff[f];
                     cinv = c.p.kin_visc;
+                } else if (c.slip_mode[f] == 2) {
+                    // AddBoundaryContributions_to_ME3D.f90:207-227
+                    double ut[3];
+                    for (int d = 0; d < 3; ++d) ut[d] = 0.5 * (dvij[d] - dvn * Tm(c, f, d, 2));
+                    const double u = std::sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+                    count_wall_function(c, wall_function(u, c.slip_coeff[f] * 10.0, L[2], c.p.dx, slip, cinv));
                 }
                 if (cinv > 0.0) {
                     if (c.laminar[npi] == 1 || c.lam_check[f] == 0) {
@@ -1250,7 +1292,16 @@ int or_vsmooth(void* ctx) {
                 const double DVL1 = nnn[0] * DVG[0] + nnn[1] * DVG[1];
                 const double IntWdV = c.bbi[8 * sl + 2];
                 double slip = 0.0;
-                if (c.sslipmode[sd] == 1) slip = c.sslip[sd];
+                if (c.sslipmode[sd] == 1) {
+                    slip = c.sslip[sd];
+                } else if (c.sslipmode[sd] == 2) {
+                    // velocity_smoothing_SA_SPH_2D.f90:93-106
+                    double ut[3], nut;
+                    for (int k = 0; k < 3; ++k)
+                        ut[k] = (c.stvel[3 * sd + k] - vel_i[k]) - (0.5 * DVL1 * ST(c, sd, k, 2));
+                    const double u = std::sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+                    wall_function(u, c.sslip[sd] * 10.0, c.bloc[3 * sl + 1], c.p.dx, slip, nut);
+                }
                 const double BCL0 = DVL0 * IntWdV * slip;
                 const double BCL1 = DVL1 * IntWdV;
                 v[0] = v[0] + (sss[0] * BCL0 + nnn[0] * BCL1);
@@ -1274,7 +1325,15 @@ int or_vsmooth(void* ctx) {
                 const double IntWdV = c.bbi[8 * s + 1];
                 if (bt == 0 || bt == 1) {
                     double slip = 0.0;
-                    if (c.slip_mode[f] == 1) slip = c.slip_coeff[f];
+                    if (c.slip_mode[f] == 1) {
+                        slip = c.slip_coeff[f];
+                    } else if (c.slip_mode[f] == 2) {
+                        // velocity_smoothing_SA_SPH_3D.f90:89-110
+                        double ut[3], nut;
+                        for (int d = 0; d < 3; ++d) ut[d] = 0.5 * (DVG[d] - DVL[2] * Tm(c, f, d, 2));
+                        const double u = std::sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+                        wall_function(u, c.slip_coeff[f] * 10.0, c.bloc[3 * s + 2], c.p.dx, slip, nut);
+                    }
                     double BCL[3];
                     BCL[0] = DVL[0] * IntWdV * slip;
                     BCL[1] = DVL[1] * IntWdV * slip;
@@ -1427,11 +1486,13 @@ int or_leapfrog_cont(void* ctx) {
 
 void or_set_lists_only(void* ctx, int on) { ((Ctx*)ctx)->lists_only = on != 0; }
 
-void or_branch_counts(void* ctx, int64_t* out3) {
+void or_branch_counts(void* ctx, int64_t* out5) {
     const Ctx& c = *(const Ctx*)ctx;
-    out3[0] = c.n_elastic.load();
-    out3[1] = c.n_zpmin.load();
-    out3[2] = c.n_dclamp.load();
+    out5[0] = c.n_elastic.load();
+    out5[1] = c.n_zpmin.load();
+    out5[2] = c.n_dclamp.load();
+    out5[3] = c.n_wf_near.load();
+    out5[4] = c.n_wf_log.load();
 }
 
 // EoS_fluid_loop.f90:59-70, EoS_barotropic_linear.f90:49-55

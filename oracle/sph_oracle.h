@@ -98,8 +98,10 @@ int64_t or_pair_count(void* ctx);   /* sum_i (nPartIntorno(i)-1) of the current 
 /* how often the rare branches ran since or_create: [0] elastic wall reactions
  * applied (AddElasticBoundaryReaction_3D/2D, vin < 0 below 0.25 dx), [1] zpmin
  * clamps (ComputeBoundaryVolumeIntegrals_P0.f90:84), [2] density_thresholds
- * clamps (Leapfrog_continuity.f90:61-68) */
-void or_branch_counts(void* ctx, int64_t* out3);
+ * clamps (Leapfrog_continuity.f90:61-68), [3] and [4] wall-function
+ * interactions of the momentum equation below z0 e and on the logarithmic
+ * branch (wall_function_for_SASPH.f90:69-78) */
+void or_branch_counts(void* ctx, int64_t* out5);
 /* on != 0: or_calcvarlength builds the neighbour lists only (no rag or
  * PartKernel, about 1/15 of the memory), for list checks of large inputs;
  * the pair passes must not run afterwards */

@@ -52,6 +52,8 @@ class DamBreak3D:
     coefnmaxpartj: float = 2.0
     maxcloseboundfaces: int = 100
     slip_coeff: float = 0.0       # slip_coefficient_mode=1 input value
+    slip_mode: int = 1            # 1: slip coefficient from input; 2: SA-SPH wall function
+    roughness: tuple = (0.0, 0.0)  # slip_coefficient_mode=2 inputs (eps_sigma_z, H_sgr)
     density_thresholds: int = 0   # 1: Leapfrog_continuity's 0.9/1.1 rho0 clamp
     time_split: int = 1           # 1: leapfrog; 0: explicit RK (RKscheme 1, Euler)
     name: str = "dam_break_3D"
@@ -64,6 +66,15 @@ class DamBreak3D:
             g = math.sqrt(sum(c * c for c in self.grav))
             c0 = 10.0 * math.sqrt(g * self.column[2])
             self.bulk = self.rho0 * c0 * c0
+
+    @property
+    def frontier_slip_input(self):
+        """The frontier zone's BC_shear_stress_input: the slip coefficient
+        (mode 1) or the roughness length 2 eps/5 + H/5 (mode 2,
+        ReadInputBoundaries.f90:207-208)."""
+        if self.slip_mode == 2:
+            return 2.0 * self.roughness[0] / 5.0 + self.roughness[1] / 5.0
+        return self.slip_coeff if self.slip_mode == 1 else 0.0
 
     # -- derived scalars (ReadInput.f90:198-205, Gest_Input.f90:266-289) ----
     @property
@@ -230,7 +241,8 @@ def make_faces(case: DamBreak3D) -> Faces:
         GX=np.array([r[6] for r in recs]).reshape(nf, 4, 3),
         LX=np.array([r[7] for r in recs]).reshape(nf, 4, 2),
         velocity=np.zeros((nf, 3)), tratto_velocity=np.zeros((nf, 3)),
-        slip_mode=np.ones(nf, np.int32), slip_coeff=np.full(nf, case.slip_coeff),
+        slip_mode=np.full(nf, case.slip_mode, np.int32),
+        slip_coeff=np.full(nf, case.frontier_slip_input),
         laminar_check=np.zeros(nf, np.int32))
 
 

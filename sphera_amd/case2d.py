@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .case import BT_FIXE, BT_PERI, BT_TAPI, _BT_NAMES, Grid, fortran_nint, make_grid
+from .case import BT_FIXE, BT_PERI, BT_TAPI, _BT_NAMES, DamBreak3D, Grid, fortran_nint, make_grid
 
 PIGRECO = 3.14159265358979
 MAXCLOSEBOUNDSIDES = 4   # Static_allocation_module.f90:38
@@ -42,6 +42,8 @@ class DamBreak2D:
     teta_v: float = 0.2
     coefnmaxpartj: float = 2.0
     slip_coeff: float = 0.0
+    slip_mode: int = 1            # 1: slip coefficient from input; 2: SA-SPH wall function
+    roughness: tuple = (0.0, 0.0)  # slip_coefficient_mode=2 inputs (eps_sigma_z, H_sgr)
     density_thresholds: int = 0      # 1: Leapfrog_continuity's 0.9/1.1 rho0 clamp
     time_split: int = 1           # 1: leapfrog; 0: explicit RK (RKscheme 1, Euler)
     name: str = "dam_break_2D"
@@ -57,6 +59,8 @@ class DamBreak2D:
     @property
     def h(self):
         return self.dx * self.trunc
+
+    frontier_slip_input = property(DamBreak3D.frontier_slip_input.fget)
 
     def scalars(self):
         """ReadInput.f90:198-205 and Gest_Input.f90:266-289 (2D branches)."""
@@ -165,7 +169,8 @@ def make_sides(case: DamBreak2D) -> Sides:
         T=np.array([r["T"] for r in recs]).reshape(ns, 3, 3),
         RN=np.array([r["RN"] for r in recs]).reshape(ns, 3, 3),
         velocity=np.zeros((ns, 3)), tratto_velocity=np.zeros((ns, 3)),
-        slip_mode=np.ones(ns, np.int32), slip_coeff=np.full(ns, case.slip_coeff),
+        slip_mode=np.full(ns, case.slip_mode, np.int32),
+        slip_coeff=np.full(ns, case.frontier_slip_input),
         laminar_check=np.zeros(ns, np.int32))
 
 

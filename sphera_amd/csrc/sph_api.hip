@@ -479,12 +479,13 @@ int check_options(int32_t opt) {
     return 0;
 }
 
-// slip_coefficient_mode = 2 on a fixed or tapis frontier selects the SA-SPH
-// wall function (AddBoundaryContributions_to_ME3D.f90:207-230), not built.
+// slip_coefficient_mode of a fixed or tapis frontier: 0 (no wall shear), 1
+// (slip coefficient from input) or 2 (SA-SPH wall function,
+// AddBoundaryContributions_to_ME3D.f90:207-230); ReadInputBoundaries.f90:
+// 186-211 reads no other value.
 int check_slip_mode(int mode, int btype) {
-    if (mode == 2 && (btype == 0 || btype == 1))
-        return fail(95, "not built on the device: slip_coefficient_mode = 2 (SA-SPH wall function, "
-                        "wall_function_for_SASPH.f90)");
+    if ((btype == 0 || btype == 1) && (mode < 0 || mode > 2))
+        return fail(2, "slip_coefficient_mode must be 0, 1 or 2, got " + std::to_string(mode));
     return 0;
 }
 

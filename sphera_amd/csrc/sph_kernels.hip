@@ -1968,6 +1968,28 @@ struct NRec2 {
     RP r;
 };
 
+// wall_function_for_SASPH.f90:56-80 (no land-cover map), FP64: slip
+// coefficient and mixing-length viscosity of one particle-frontier pair at
+// distance r; d50 is ten times the frontier's roughness length.  Nepero_number
+// and k_v: Static_allocation_module.f90:51,53.
+__device__ __forceinline__ void wall_function(double u_t, double d50, double r, double dx,
+                                              double& slip, double& nut) {
+    const double e = 2.718281828, kv2 = 0.41 * 0.41;
+    slip = 0.0;
+    nut = 1.0e-15;
+    if (r <= 0.5 * dx) {
+        const double z0 = d50 / 10.0;
+        if (r < z0 * e) {
+            slip = 1.0;
+            nut = kv2 * u_t * z0 * e;
+        } else {
+            const double L = log(r / z0);
+            slip = 1.0 / L;
+            nut = kv2 * u_t * r / L;
+        }
+    }
+}
+
 // SA-SPH terms of the momentum equation (AddBoundaryContributions_to_ME3D.f90:
 // 118-336), FP64; tp/td/tv start at zero
 __device__ __forceinline__ void sa_momentum(const DP& P, const DFace* __restrict__ faces,
@@ -2023,6 +2045,12 @@ __device__ __forceinline__ void sa_momentum(const DP& P, const DFace* __restrict
         if (F.mode == 1) {
             slip = F.slip;
             cinv = P.kin_visc;
+        } else if (F.mode == 2) {
+            // AddBoundaryContributions_to_ME3D.f90:207-227
+            double ut[3];
+            for (int d = 0; d < 3; ++d) ut[d] = 0.5 * (dvij[d] - dvn * F.T[6 + d]);
+            wall_function(sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]), F.slip * 10.0,
+                          B.loc[2], P.dx, slip, cinv);
         }
         if (cinv > 0.0 && (lam_prev == 1 || F.lam == 0)) {
             const double cm = cinv * B.bi[2] * slip;
@@ -2083,6 +2111,12 @@ __device__ __forceinline__ void sa_momentum_2d(const DP& P, const DFace* __restr
             if (S.mode == 1) {
                 slip = S.slip;
                 cinv = P.kin_visc;
+            } else if (S.mode == 2) {
+                // AddBoundaryContributions_to_ME2D.f90:314-326
+                double ut[3];
+                for (int k = 0; k < 3; ++k) ut[k] = (v_i[k] - S.vel[k]) - (0.5 * DvelN * S.T[6 + k]);
+                wall_function(sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]), S.slip * 10.0, ypi,
+                              P.dx, slip, cinv);
             }
             const double Monvisc = P.alfamon * P.celerita * P.h;
             const double TN = Monvisc * DvelN * IntWd3s0;
@@ -2404,7 +2438,16 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                     const double DVL0 = s0 * DVG0 + s1 * DVG1;
                     const double DVL1 = n0 * DVG0 + n1 * DVG1;
                     const double IntWdV = B.bi[2];
-                    const double slip = S.mode == 1 ? S.slip : 0.0;
+                    double slip = S.mode == 1 ? S.slip : 0.0;
+                    if (S.mode == 2) {
+                        // velocity_smoothing_SA_SPH_2D.f90:93-106
+                        const double u0 = (S.tvel[0] - vi.x) - (0.5 * DVL1 * S.T[6]);
+                        const double u1 = (S.tvel[1] - vi.y) - (0.5 * DVL1 * S.T[7]);
+                        const double u2 = (S.tvel[2] - vi.z) - (0.5 * DVL1 * S.T[8]);
+                        double nut;
+                        wall_function(sqrt(u0 * u0 + u1 * u1 + u2 * u2), S.slip * 10.0, B.loc[1], P.dx,
+                                      slip, nut);
+                    }
                     const double BCL0 = DVL0 * IntWdV * slip;
                     const double BCL1 = DVL1 * IntWdV;
                     v[0] = v[0] + (s0 * BCL0 + n0 * BCL1);
@@ -2427,7 +2470,14 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                         DVL[ii] = a;
                     }
                     const double IntWdV = B.bi[1];
-                    const double slip = F.mode == 1 ? F.slip : 0.0;
+                    double slip = F.mode == 1 ? F.slip : 0.0;
+                    if (F.mode == 2) {
+                        // velocity_smoothing_SA_SPH_3D.f90:89-110
+                        double ut[3], nut;
+                        for (int d = 0; d < 3; ++d) ut[d] = 0.5 * (DVG[d] - DVL[2] * F.T[6 + d]);
+                        wall_function(sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]), F.slip * 10.0,
+                                      B.loc[2], P.dx, slip, nut);
+                    }
                     const double BCL[3] = {DVL[0] * IntWdV * slip, DVL[1] * IntWdV * slip,
                                            DVL[2] * IntWdV};
                     for (int ii = 0; ii < 3; ++ii) {

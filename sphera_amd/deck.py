@@ -29,6 +29,19 @@ def _f(x):
     return repr(float(x))
 
 
+def _fixed_zone(case):
+    """Lines of a ``fixed`` frontier zone after its type (ReadInputBoundaries.f90:
+    183-208): slip_coefficient_mode and laminar_no_slip_check, then the slip
+    coefficient (mode 1) or eps_sigma_z and H_sgr (mode 2)."""
+    mode = int(getattr(case, "slip_mode", 1))
+    out = [f"{mode} .false."]
+    if mode == 1:
+        out.append(_f(case.slip_coeff))
+    elif mode == 2:
+        out.append(f"{_f(case.roughness[0])} {_f(case.roughness[1])}")
+    return out
+
+
 def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
     s = case.scalars()
     v = case.vertices()
@@ -41,7 +54,7 @@ def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
     for i, r in enumerate(case.faces(), start=1):
         lines.append(f"{i} {r[0]} {r[1]} {r[2]} {r[3]} -1 -1 {r[4]}")
     lines += ["##### END FACES #####", "##### BOUNDARIES #####",
-              "Boundary", "1", "fixed", "1 .false.", _f(case.slip_coeff), "111111", "1 8",
+              "Boundary", "1", "fixed", *_fixed_zone(case), "111111", "1 8",
               "Reservoir", "2", "perimeter", "1", "uniform 009EA8",
               "std 0. 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[2])} 0.", "1 0 .false.",
               "##### END BOUNDARIES #####",
@@ -99,7 +112,7 @@ def deck_text_2d(case, itmax=1000000000, tmax=1000.0, results_step=None):
     for lid, vids in sorted(case.lines().items()):
         lines.append(f"{lid} " + " ".join(str(i) for i in vids))
     lines += ["##### END LINES #####", "##### BOUNDARIES #####",
-              "Boundary", "1", "fixed", "1 .false.", _f(case.slip_coeff), "111111",
+              "Boundary", "1", "fixed", *_fixed_zone(case), "111111",
               "Reservoir", "2", "perimeter", "1", "uniform 009EA8",
               "std 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[1])}", "1 0 .false.",
               "##### END BOUNDARIES #####",
@@ -210,6 +223,8 @@ class DeckCase:
     density_thresholds: int
     time_split: int
     pool: float
+    slip_mode: int
+    roughness: tuple
     verts: np.ndarray = field(repr=False)
     face_rows: list = field(repr=False)
     zones: dict = field(repr=False)
@@ -234,6 +249,13 @@ class DeckCase:
         s["kin_visc"] = self.mu / self.rho0
         s["pvolume"] = self.dx * self.dx * self.dx
         return s
+
+    @property
+    def frontier_slip_input(self):
+        """BC_shear_stress_input of the frontier zone (mode 2: 2 eps/5 + H/5)."""
+        if self.slip_mode == 2:
+            return 2.0 * self.roughness[0] / 5.0 + self.roughness[1] / 5.0
+        return self.slip_coeff if self.slip_mode == 1 else 0.0
 
     def vertices(self):
         return self.verts
@@ -287,17 +309,21 @@ def parse_deck(text, name=None):
     # BOUNDARIES (ReadInputBoundaries.f90): name, zone, type, then per type
     b = sec["boundaries"]
     zones, k = {}, 0
-    slip, pool, fluid_zone = 0.0, None, None
+    slip, mode, rough, pool, fluid_zone = 0.0, 1, (0.0, 0.0), None, None
     while k < len(b):
         zone = _int(b[k + 1])
         kind = b[k + 2].lower()
         if kind == "fixed":
             mode = _int(b[k + 3].split()[0])
-            if mode != 1:
-                raise ValueError("only slip_coefficient_mode 1 frontiers are read")
-            slip = _num(b[k + 4].split()[0])
+            if mode not in (0, 1, 2):
+                raise ValueError("slip_coefficient_mode must be 0, 1 or 2")
+            nval = 0
+            if mode == 1:
+                slip, nval = _num(b[k + 4].split()[0]), 1
+            elif mode == 2:
+                rough, nval = tuple(_num(v) for v in b[k + 4].split()[:2]), 1
             zones[zone] = "fixe"
-            k += 7
+            k += 6 + nval
         elif kind == "perimeter":
             q = b[k + 6].split()
             if q[0].lower() != "qp":
@@ -345,5 +371,5 @@ def parse_deck(text, name=None):
                     alfa_mon=alfa_mon, mu=mu, grav=grav, p_ref=p_ref, cfl=cfl, vsc_coeff=vsc,
                     pesodt=pesodt, teta_p=tetap, teta_v=tetav, coefnmaxpartj=coefj,
                     maxcloseboundfaces=maxclose, slip_coeff=slip, density_thresholds=dthr,
-                    time_split=time_split, pool=pool, verts=verts, face_rows=rows, zones=zones,
+                    time_split=time_split, pool=pool, slip_mode=mode, roughness=rough, verts=verts, face_rows=rows, zones=zones,
                     fluid_zone=fluid_zone, control_lines=tuple(lines))

@@ -6,7 +6,7 @@ from /root/reference/src) on project-authored decks.
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
 With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp,
-rk1, viscous) only those fixtures are written.
+rk1, viscous, wallfn, wallfn_2d) only those fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -38,6 +38,16 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         flow u = 3 z: the laminar flag of inter_EqMoto.f90:
                         269-283 is set for hundreds of particles and feeds the
                         wall shear term; state after 0, 3 and 6 steps
+  ref_wallfn.npz        800 particles, jittered lattice with the near-wall
+                        layers of ref_nearwall.npz and a tangential velocity
+                        (v_y = 0.2 m/s, v_x += 2 z); the frontier zone has
+                        slip_coefficient_mode = 2 with eps_sigma_z = 0.0005,
+                        H_sgr = 0.0015 (roughness length 0.0005 m): the SA-SPH
+                        wall function on both of its branches, in the momentum
+                        equation and the velocity smoothing; state after 0, 4
+                        and 8 steps
+  ref_wallfn_2d.npz     the same for the 2D build (600 particles, v_x = 0.2 +
+                        2 z, v_z += 0.2)
   ref_rk1.npz           1440 particles, jittered lattice (seed 20260101), the
                         explicit RK1 branch (time_split = 0, RKscheme = 1:
                         Euler); state after 0, 6 and 12 steps
@@ -275,6 +285,40 @@ def make_nearwall_2d():
     np.savez_compressed(HERE / "ref_nearwall_2d.npz", **out)
 
 
+WALLFN = dict(slip_mode=2, roughness=(0.0005, 0.0015))
+
+
+def make_wallfn():
+    """slip_coefficient_mode = 2: the SA-SPH wall function"""
+    case = DamBreak3D(**TINY, **WALLFN)
+    tb = build(case)
+    ic, iv = nearwall_ic(jitter(tb.particles["coord"], case.dx), case.dx)
+    iv[:, 1] += 0.2
+    iv[:, 0] += 2.0 * ic[:, 2]
+    hdr, steps = read_dump(run_harness(case, nsteps=8, every=4, lists=False, ic_coord=ic, ic_vel=iv))
+    out = {"ic_coord": ic, "ic_vel": iv}
+    _pack_steps(steps, out, lists=False)
+    np.savez_compressed(HERE / "ref_wallfn.npz", **out)
+
+
+def make_wallfn_2d():
+    """2D wall function"""
+    case = case2d.DamBreak2D(**CASE2D, **WALLFN)
+    tb = case2d.build(case)
+    ic, iv = nearwall_ic(jitter2d(tb.particles["coord"], case.dx), case.dx, dim=2)
+    iv[:, 0] += 0.2 + 2.0 * ic[:, 2]
+    iv[:, 2] += 0.2
+    hdr, steps = read_dump(run_harness(case, nsteps=8, every=4, lists=False, ic_coord=ic, ic_vel=iv))
+    out = {"ic_coord": ic, "ic_vel": iv}
+    for st in steps:
+        pre = f"s{st['it']}_"
+        for k in ("it", "time", "dt", "dt_average", "nag"):
+            out[pre + k] = st[k]
+        for f in FIELDS + ("var", "laminar_flag"):
+            out[pre + f] = st[f]
+    np.savez_compressed(HERE / "ref_wallfn_2d.npz", **out)
+
+
 def make_dclamp():
     """density_thresholds = 1: the density clamp"""
     case = DamBreak3D(**SMALL, density_thresholds=1)
@@ -314,7 +358,7 @@ def make_rk1():
 MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_removal,
           "pelo": make_pelo, "2d": make_2d, "nearwall": make_nearwall,
           "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp, "rk1": make_rk1,
-          "viscous": make_viscous}
+          "viscous": make_viscous, "wallfn": make_wallfn, "wallfn_2d": make_wallfn_2d}
 
 
 def main():

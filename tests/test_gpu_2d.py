@@ -172,3 +172,35 @@ def test_nearwall_2d_fixture():
                 assert errs[f] <= tol, (it, prec, f, errs[f])
     for d in ctx.values():
         d.close()
+
+
+def test_wall_function_2d_fixture():
+    """slip_coefficient_mode = 2 in the 2D build (tests/golden/
+    ref_wallfn_2d.npz): the wall function of AddBoundaryContributions_to_ME2D.
+    f90:314-326 and velocity_smoothing_SA_SPH_2D.f90:93-106.  Device FP64
+    within FP64_REL_TOL of the reference after 4 and 8 steps, FP32-mixed
+    within FP32_TOL."""
+    from pathlib import Path
+    from sphera_amd import runtime as rt
+    g = np.load(Path(__file__).resolve().parent / "golden" / "ref_wallfn_2d.npz")
+    tb = case2d.build(case2d.DamBreak2D(**SMALL, slip_mode=2, roughness=(0.0005, 0.0015)))
+    print(f"[gpu] 2d wall function n={len(g['ic_coord'])}", flush=True)
+    ctx = {}
+    for prec in (rt.FP64, rt.FP32_MIXED):
+        d = rt.SphContext(tb, precision=prec)
+        d.upload(coord=g["ic_coord"], vel=g["ic_vel"])
+        d.preloop()
+        ctx[prec] = d
+    for it in (4, 8):
+        for prec, d in ctx.items():
+            d.step(4)
+            ds = d.state()
+            errs = {f: _rel(ds[f], g[f"s{it}_" + f]) for f in FIELDS}
+            exact = sum(bool(np.array_equal(ds[f], g[f"s{it}_" + f])) for f in FIELDS)
+            print(f"[gpu] 2d wall function it={it} precision={prec} bit-identical fields {exact}/7 "
+                  f"max rel {max(errs.values()):.2e}", flush=True)
+            tol = FP64_REL_TOL if prec == rt.FP64 else FP32_TOL["vel"]
+            for f in FIELDS:
+                assert errs[f] <= tol, (it, prec, f, errs[f])
+    for d in ctx.values():
+        d.close()

@@ -415,6 +415,39 @@ def test_nearwall_fixture():
             assert e32[f] <= FP32_TOL[f], (it, f, e32[f])
 
 
+def test_wall_function_fixture():
+    """slip_coefficient_mode = 2 on the reference's fixture
+    (tests/golden/ref_wallfn.npz: roughness length 0.0005 m, particles
+    0.02-0.22 dx from two walls, tangential velocity), where the SA-SPH wall
+    function (wall_function_for_SASPH.f90) sets the slip coefficient and the
+    turbulent viscosity of the wall terms in the momentum equation and the
+    velocity smoothing.  Device FP64 within FP64_LOG_TOL of the reference
+    after 4 and 8 steps (its log() is the device library's), FP32-mixed within
+    FP32_TOL."""
+    rt = _rt()
+    g = np.load(GOLD / "ref_wallfn.npz")
+    tb = build(DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny",
+                          slip_mode=2, roughness=(0.0005, 0.0015)))
+    print(f"[gpu] wall function n={len(g['ic_coord'])}", flush=True)
+    d, f32 = _gold_device(tb, g, rt.FP64), _gold_device(tb, g, rt.FP32_MIXED)
+    for it in (4, 8):
+        d.step(4)
+        f32.step(4)
+        ds, fs = d.state(), f32.state()
+        assert ds["nag"] == g[f"s{it}_nag"]
+        exact = sum(bool(np.array_equal(ds[f], g[f"s{it}_" + f])) for f in FIELDS)
+        e64 = {f: _rel(ds[f], g[f"s{it}_" + f]) for f in FIELDS}
+        e32 = {f: _rel(fs[f], g[f"s{it}_" + f]) for f in FIELDS}
+        print(f"[gpu] wall function it={it} fp64 bit-identical fields {exact}/7, max rel "
+              f"{max(e64.values()):.2e}; fp32 {{{', '.join(f'{k}: {v:.1e}' for k, v in e32.items())}}}",
+              flush=True)
+        for f in FIELDS:
+            assert e64[f] <= FP64_LOG_TOL, (it, f, e64[f])
+            assert e32[f] <= FP32_TOL[f], (it, f, e32[f])
+    d.close()
+    f32.close()
+
+
 def test_density_clamp_fixture():
     """density_thresholds = 1 (Leapfrog_continuity.f90:61-68) on the
     reference's fixture (tests/golden/ref_dclamp.npz: converging and

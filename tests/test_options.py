@@ -1,11 +1,12 @@
 """Deck options the device does not build are refused, not run without them.
 
 sph_ctx_create / sph_ctx_create2d / sph_batch_create return status 95 for
-ALE3, C1_BE / C1_divu, KTGF_config > 0, convex frontier edges (NumBEdges > 0)
-and the SA-SPH wall function (slip_coefficient_mode = 2).  The checks run
-before any device call, so these tests run without a GPU.  The density clamp
-(density_thresholds = 1) is built and is accepted; its parity tests are with
-the other parity tests."""
+ALE3, C1_BE / C1_divu, KTGF_config > 0 and convex frontier edges
+(NumBEdges > 0), and status 2 for a slip_coefficient_mode outside 0-2.  The
+checks run before any device call, so these tests run without a GPU.  The
+density clamp (density_thresholds = 1) and the SA-SPH wall function
+(slip_coefficient_mode = 2) are built and are accepted; their parity tests are
+with the other parity tests."""
 import ctypes as C
 
 import numpy as np
@@ -47,22 +48,22 @@ def test_unknown_option_bits_rejected():
     assert rc == 2 and "unknown option" in msg
 
 
-def test_wall_function_refused_3d():
+def test_unknown_slip_mode_rejected_3d():
     def patch(fs, keep):
-        keep["mode"][:] = 2
+        keep["mode"][:] = 3
     rc, msg = _create(_tiny(), faces_patch=patch)
-    assert rc == 95 and "slip_coefficient_mode = 2" in msg, (rc, msg)
+    assert rc == 2 and "slip_coefficient_mode must be 0, 1 or 2" in msg, (rc, msg)
 
 
-def test_wall_function_refused_2d():
+def test_unknown_slip_mode_rejected_2d():
     L = rt.load()
     tb = build2d(DamBreak2D(tank=(0.6, 0.4), column=(0.2, 0.1), dx=0.01, name="t2d"))
     p = rt.params_struct(tb)
     ss, keep = rt.sides_struct(tb)
-    keep["mode"][:] = 2
+    keep["mode"][:] = -1
     h = C.c_void_p()
     rc = L.sph_ctx_create2d(C.byref(p), C.byref(ss), C.byref(h))
-    assert rc == 95 and "slip_coefficient_mode = 2" in L.sph_last_error().decode()
+    assert rc == 2 and "slip_coefficient_mode must be" in L.sph_last_error().decode()
 
 
 def test_batch_refuses_unbuilt_option():
