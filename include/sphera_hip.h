@@ -66,7 +66,8 @@ typedef struct sph_params {
  * src/Modules/Hybrid_allocation_module.f90:88-113,931; NumBEdges of
  * Static_allocation_module.f90:220; Domain%time_split and RKscheme).
  * SPH_OPT_DENSITY_THRESHOLDS is built: the 0.9/1.1 den0 clamp of
- * Leapfrog_continuity.f90:61-68; so is SPH_OPT_RK1.  The others are not
+ * Leapfrog_continuity.f90:61-68; so are SPH_OPT_RK1 and
+ * SPH_OPT_CONVEX_EDGES (3D contexts only).  The others are not
  * built on the device: sph_ctx_create / sph_ctx_create2d / sph_batch_create
  * refuse a context that sets them with status 95, rather than running the
  * case without them.  A frontier's slip_mode is 0, 1 or 2 (2: the SA-SPH
@@ -77,7 +78,8 @@ enum {
     SPH_OPT_ALE3 = 2,               /* ALE3 (ALE3_term_momentum.f90, Leapfrog_continuity.f90:70-86) */
     SPH_OPT_C1 = 4,                 /* C1_BE or C1_divu (renormalisation, CalcVarLength.f90:285-324) */
     SPH_OPT_KTGF = 8,               /* KTGF_config > 0 (mixture viscosity, erosion) */
-    SPH_OPT_CONVEX_EDGES = 16,      /* NumBEdges > 0 (AddElasticBoundaryReaction_3D.f90:106-152) */
+    SPH_OPT_CONVEX_EDGES = 16,      /* NumBEdges > 0 (AddElasticBoundaryReaction_3D.f90:106-152),
+                                       built: set exactly when sph_faces.nedges > 0 */
     SPH_OPT_RK1 = 32,               /* time_split = 0, RKscheme = 1: explicit Euler, built
                                        (time_step_loop.f90:190-489, Euler.f90:61-241); such a
                                        context steps with sph_step / sph_step_graph only */
@@ -98,6 +100,14 @@ typedef struct sph_faces {
     const double *kerneltab; /* 41 x 5 row-major */
     double ktdelta;
     const double *bitab;     /* 256 x 7 row-major */
+    /* convex edges (BoundaryConvexEdge of FindBoundaryConvexEdges3D.f90, in
+     * its order; NumBEdges of them): edge_face 2 per edge (0-based face
+     * indices), edge_gx the first node (x,y,z), edge_comp the second node
+     * minus the first, edge_len its length.  nedges = 0: none (the pointers
+     * may be null). */
+    int32_t nedges;
+    const int32_t *edge_face;
+    const double *edge_gx, *edge_comp, *edge_len;
 } sph_faces;
 
 typedef struct sph_ctx sph_ctx;

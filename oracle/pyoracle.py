@@ -35,7 +35,20 @@ class OrFaces(C.Structure):
                 ("velocity", PD), ("tvel", PD), ("slip_coeff", PD), ("nodes", PI),
                 ("btype", PI), ("slip_mode", PI), ("lam_check", PI), ("gcbf_count", PI),
                 ("gcbf_start", PI), ("gcbf_vec", PI), ("kerneltab", PD), ("ktdelta", C.c_double),
-                ("bitab", PD)]
+                ("bitab", PD), ("nedges", C.c_int32), ("edge_face", PI), ("edge_gx", PD),
+                ("edge_comp", PD), ("edge_len", PD)]
+
+
+def _edge_arrays(tb, k):
+    """The convex-edge arrays of the case tables (at least one element each)
+    into k; returns the number of edges."""
+    E = getattr(tb, "edges", None)
+    ne = len(E) if E is not None else 0
+    k["ef"] = np.ascontiguousarray(E.face.reshape(-1) if ne else np.zeros(2), np.int32)
+    k["egx"] = np.ascontiguousarray(E.gx.reshape(-1) if ne else np.zeros(3), np.float64)
+    k["ecomp"] = np.ascontiguousarray(E.component.reshape(-1) if ne else np.zeros(3), np.float64)
+    k["elen"] = np.ascontiguousarray(E.length if ne else np.zeros(1), np.float64)
+    return ne
 
 
 class OrSides2D(C.Structure):
@@ -167,10 +180,12 @@ class Oracle:
             kt=np.ascontiguousarray(tb.kerneltab.reshape(-1)),
             bit=np.ascontiguousarray(tb.bitab.reshape(-1)))
         k = self._keep
+        ne = _edge_arrays(tb, k)
         f = OrFaces(len(F.face_id), _pd(k["T"]), _pd(k["LX"]), _pd(k["GX"]), _pd(k["area"]),
                     _pd(k["vel"]), _pd(k["tvel"]), _pd(k["slip"]), _pi(k["nodes"]), _pi(k["btype"]),
                     _pi(k["mode"]), _pi(k["lam"]), _pi(k["gc"]), _pi(k["gs"]), _pi(k["gv"]),
-                    _pd(k["kt"]), tb.ktdelta, _pd(k["bit"]))
+                    _pd(k["kt"]), tb.ktdelta, _pd(k["bit"]), ne, _pi(k["ef"]), _pd(k["egx"]),
+                    _pd(k["ecomp"]), _pd(k["elen"]))
         self.ctx = L.or_create(C.byref(self.p), C.byref(f), _pd(self.coord), _pd(vel), _pd(dens),
                                _pd(pres), _pd(mass))
 
@@ -263,8 +278,9 @@ class Oracle:
     def branch_counts(self):
         """How often the rare branches ran: elastic wall reactions applied,
         zpmin clamps of the SA-SPH volume integrals, density_thresholds clamps,
-        wall-function interactions below z0 e and on the logarithmic branch."""
-        v = (C.c_int64 * 5)()
+        wall-function interactions below z0 e and on the logarithmic branch,
+        convex-edge reactions applied."""
+        v = (C.c_int64 * 6)()
         lib().or_branch_counts(self.ctx, v)
         return dict(elastic=int(v[0]), zpmin=int(v[1]), density_clamp=int(v[2]),
-                    wall_fn_near=int(v[3]), wall_fn_log=int(v[4]))
+                    wall_fn_near=int(v[3]), wall_fn_log=int(v[4]), convex_edge=int(v[5]))

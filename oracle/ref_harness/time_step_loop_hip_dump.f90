@@ -15,6 +15,7 @@
 !   SPH_HARNESS_STEPS  number of time steps (default input_any_t%itmax)
 !   SPH_HARNESS_EVERY  dump every K steps (default 1)
 !   SPH_HARNESS_IC     optional int32 n + n*3 float64 positions (pg order)
+!   SPH_HARNESS_ICV    optional int32 n + n*3 float64 velocities (pg order)
 !   SPH_HIP_PRECISION  0 FP64 (default), 1 FP32-mixed
 !   SPH_HIP_PHASES     1: call the per-phase entry points in the reference's
 !                      order instead of sph_step
@@ -65,6 +66,22 @@ if (len_trim(aux)>0) read(aux,*) phases
 pelo = 0
 call get_environment_variable("SPH_HARNESS_PELO",aux)
 if (len_trim(aux)>0) read(aux,*) pelo
+call get_environment_variable("SPH_HARNESS_ICV",aux)
+if (len_trim(aux)>0) then
+   open(newunit=u,file=trim(aux),access="stream",form="unformatted",          &
+      status="old",iostat=ios)
+   if (ios/=0) stop 3
+   read(u) n_ic
+   if (n_ic/=nag) stop 3
+   allocate(xyz(3*nag))
+   read(u) xyz
+   close(u)
+   do npi=1,nag
+      pg(npi)%vel(1:3) = xyz(3*npi-2:3*npi)
+      pg(npi)%var(1:3) = pg(npi)%vel(1:3)
+   enddo
+   deallocate(xyz)
+endif
 call get_environment_variable("SPH_HARNESS_IC",aux)
 if (len_trim(aux)>0) then
    open(newunit=u,file=trim(aux),access="stream",form="unformatted",          &

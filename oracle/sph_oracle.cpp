@@ -26,6 +26,9 @@ struct Ctx {
     int nf = 0;
     std::vector<double> T, LX, GX, area, fvel, tvel, slip_coeff;
     std::vector<int> nodes, btype, slip_mode, lam_check;
+    // convex frontier edges (BoundaryConvexEdge)
+    std::vector<int> edge_face;
+    std::vector<double> edge_gx, edge_comp, edge_len;
     std::vector<int> gcbf_count, gcbf_start, gcbf_vec;
     double kt[41][5];
     double ktdelta = 0.0;
@@ -58,7 +61,8 @@ struct Ctx {
     // how often the rare branches ran (checker counters, read by or_branch_counts):
     // elastic wall reactions applied, zpmin clamps of the volume integrals,
     // density_thresholds clamps, wall-function interactions below z0 e and above
-    mutable std::atomic<long long> n_elastic{0}, n_zpmin{0}, n_dclamp{0}, n_wf_near{0}, n_wf_log{0};
+    mutable std::atomic<long long> n_elastic{0}, n_zpmin{0}, n_dclamp{0}, n_wf_near{0}, n_wf_log{0},
+        n_edge{0};
     // lists only: CalcVarLength stores the neighbour lists without rag and
     // PartKernel (list checks of large clouds; the passes cannot run)
     bool lists_only = false;
@@ -634,6 +638,11 @@ void* or_create(const or_params* p, const or_faces* f, const double* coord, cons
         std::memcpy(c->kt, f->kerneltab, sizeof(c->kt));
         c->ktdelta = f->ktdelta;
         std::memcpy(c->bit, f->bitab, sizeof(c->bit));
+        const int ne = f->nedges > 0 ? f->nedges : 0;
+        c->edge_face.assign(f->edge_face, f->edge_face + 2 * ne);
+        c->edge_gx.assign(f->edge_gx, f->edge_gx + 3 * ne);
+        c->edge_comp.assign(f->edge_comp, f->edge_comp + 3 * ne);
+        c->edge_len.assign(f->edge_len, f->edge_len + ne);
     } else {
         c->gcbf_count.assign(c->nmax, 0);
         c->gcbf_start.assign(c->nmax, 0);
@@ -1222,10 +1231,11 @@ int or_momentum(void* ctx) {
             }
             for (int d = 0; d < 3; ++d) a[d] = tp[d] + td[d] + tv[d] + c.p.grav[d] + br[d];
         } else if (ncbf > 0) {
-            // AddElasticBoundaryReaction_3D (faces part; no convex edges)
+            // AddElasticBoundaryReaction_3D.f90:72-104 (faces)
             double br[3] = {0, 0, 0};
             const double zimin = 0.25 * c.p.dx;
             const double celer02 = c.p.eps / c.p.den0;
+            std::vector<char> rea(ncbf, 0);  // ReaFace
             for (int q = 0; q < ncbf; ++q) {
                 const size_t s = (size_t)npi * mc + q;
                 const int f = c.bface[s];
@@ -1240,6 +1250,44 @@ int or_momentum(void* ctx) {
                         const double nr = -1.0 * celer02 * std::log((h + zi - zimin) / h) / h;
                         for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * Tm(c, f, d, 2);
                         c.n_elastic.fetch_add(1, std::memory_order_relaxed);
+                    }
+                    rea[q] = 1;
+                }
+            }
+            // AddElasticBoundaryReaction_3D.f90:106-152 (convex edges in
+            // order; the subroutine returns at the first edge whose two faces
+            // are not both among the particle's non-reacting close faces)
+            const int nedge = (int)c.edge_len.size();
+            for (int ne = 0; ne < nedge; ++ne) {
+                int nclose = 0;
+                for (int q = 0; q < ncbf; ++q) {
+                    const int f = c.bface[(size_t)npi * mc + q];
+                    if (f == c.edge_face[2 * ne] && !rea[q]) nclose += 1;
+                    else if (f == c.edge_face[2 * ne + 1] && !rea[q]) nclose += 1;
+                }
+                if (nclose != 2) break;
+                const double* X = &c.coord[3 * npi];
+                const double* G = &c.edge_gx[3 * ne];
+                const double* K = &c.edge_comp[3 * ne];
+                double scaprod = 0.0;
+                for (int sd = 0; sd < 3; ++sd) scaprod = scaprod + (X[sd] - G[sd]) * K[sd];
+                const double edgelen2 = c.edge_len[ne] * c.edge_len[ne];
+                const double tau = scaprod / edgelen2;
+                if (tau >= 0.0 && tau <= 1.0) {
+                    double XQ[3], QP[3], d2 = 0.0;
+                    for (int sd = 0; sd < 3; ++sd) XQ[sd] = G[sd] + K[sd] * tau;
+                    for (int sd = 0; sd < 3; ++sd) QP[sd] = X[sd] - XQ[sd];
+                    for (int sd = 0; sd < 3; ++sd) d2 = d2 + QP[sd] * QP[sd];
+                    const double zi = std::sqrt(d2);
+                    if (zi < zimin) {
+                        double cd[3], vin = 0.0;
+                        for (int sd = 0; sd < 3; ++sd) cd[sd] = QP[sd] / zi;
+                        for (int sd = 0; sd < 3; ++sd) vin = vin + vi[sd] * cd[sd];
+                        if (vin < 0.0) {
+                            const double nr = -1.0 * celer02 * std::log(zi / zimin) / zimin;
+                            for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * cd[d];
+                            c.n_edge.fetch_add(1, std::memory_order_relaxed);
+                        }
                     }
                 }
             }
@@ -1486,13 +1534,14 @@ int or_leapfrog_cont(void* ctx) {
 
 void or_set_lists_only(void* ctx, int on) { ((Ctx*)ctx)->lists_only = on != 0; }
 
-void or_branch_counts(void* ctx, int64_t* out5) {
+void or_branch_counts(void* ctx, int64_t* out6) {
     const Ctx& c = *(const Ctx*)ctx;
-    out5[0] = c.n_elastic.load();
-    out5[1] = c.n_zpmin.load();
-    out5[2] = c.n_dclamp.load();
-    out5[3] = c.n_wf_near.load();
-    out5[4] = c.n_wf_log.load();
+    out6[0] = c.n_elastic.load();
+    out6[1] = c.n_zpmin.load();
+    out6[2] = c.n_dclamp.load();
+    out6[3] = c.n_wf_near.load();
+    out6[4] = c.n_wf_log.load();
+    out6[5] = c.n_edge.load();
 }
 
 // EoS_fluid_loop.f90:59-70, EoS_barotropic_linear.f90:49-55

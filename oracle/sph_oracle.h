@@ -43,6 +43,11 @@ typedef struct or_faces {
     const double *kerneltab;  /* 41 x 5, row-major [row*5+col] */
     double ktdelta;
     const double *bitab;      /* 256 x 7, row-major */
+    /* convex edges (FindBoundaryConvexEdges3D.f90 order): 2 face indices
+     * (0-based) per edge, first node (x,y,z), second minus first, length */
+    int32_t nedges;
+    const int32_t *edge_face;
+    const double *edge_gx, *edge_comp, *edge_len;
 } or_faces;
 
 void* or_create(const or_params* p, const or_faces* f, const double* coord, const double* vel,
@@ -100,8 +105,9 @@ int64_t or_pair_count(void* ctx);   /* sum_i (nPartIntorno(i)-1) of the current 
  * clamps (ComputeBoundaryVolumeIntegrals_P0.f90:84), [2] density_thresholds
  * clamps (Leapfrog_continuity.f90:61-68), [3] and [4] wall-function
  * interactions of the momentum equation below z0 e and on the logarithmic
- * branch (wall_function_for_SASPH.f90:69-78) */
-void or_branch_counts(void* ctx, int64_t* out5);
+ * branch (wall_function_for_SASPH.f90:69-78), [5] convex-edge reactions
+ * applied (AddElasticBoundaryReaction_3D.f90:106-152) */
+void or_branch_counts(void* ctx, int64_t* out6);
 /* on != 0: or_calcvarlength builds the neighbour lists only (no rag or
  * PartKernel, about 1/15 of the memory), for list checks of large inputs;
  * the pair passes must not run afterwards */

@@ -56,6 +56,10 @@ class DamBreak3D:
     roughness: tuple = (0.0, 0.0)  # slip_coefficient_mode=2 inputs (eps_sigma_z, H_sgr)
     density_thresholds: int = 0   # 1: Leapfrog_continuity's 0.9/1.1 rho0 clamp
     time_split: int = 1           # 1: leapfrog; 0: explicit RK (RKscheme 1, Euler)
+    # (x0, x1, z_top): a block on the floor across the tank's width, x0 < x < x1,
+    # whose upstream, top and downstream faces join the frontier zone (two
+    # convex edges, FindBoundaryConvexEdges3D.f90); None: no block
+    obstacle: tuple | None = None
     name: str = "dam_break_3D"
     # monitoring lines of the deck's "control lines" section:
     # (label, first point (x,y,z), second point, number of points)
@@ -96,12 +100,17 @@ class DamBreak3D:
 
     # -- geometry -----------------------------------------------------------
     def vertices(self):
-        """Vertex table (1-based ids): 1-8 tank corners, 9-16 column corners.
-        Corner order per box: bottom (x0y0, x1y0, x1y1, x0y1), then top."""
-        def box(x1, y1, z1):
-            return [(0.0, 0.0, 0.0), (x1, 0.0, 0.0), (x1, y1, 0.0), (0.0, y1, 0.0),
-                    (0.0, 0.0, z1), (x1, 0.0, z1), (x1, y1, z1), (0.0, y1, z1)]
-        return np.array(box(*self.tank) + box(*self.column), dtype=np.float64)
+        """Vertex table (1-based ids): 1-8 tank corners, 9-16 column corners,
+        17-24 the block's corners if there is one.  Corner order per box:
+        bottom (x0y0, x1y0, x1y1, x0y1), then top."""
+        def box(x1, y1, z1, x0=0.0):
+            return [(x0, 0.0, 0.0), (x1, 0.0, 0.0), (x1, y1, 0.0), (x0, y1, 0.0),
+                    (x0, 0.0, z1), (x1, 0.0, z1), (x1, y1, z1), (x0, y1, z1)]
+        v = box(*self.tank) + box(*self.column)
+        if self.obstacle is not None:
+            x0, x1, zt = self.obstacle
+            v += box(x1, self.tank[1], zt, x0)
+        return np.array(v, dtype=np.float64)
 
     def faces(self):
         """Face table rows (n1,n2,n3,n4,zone).  Node order makes the normal
@@ -115,6 +124,11 @@ class DamBreak3D:
             return f
         rows = [fc + (1,) for fc in box_faces(0, False)]
         rows += [fc + (2,) for fc in box_faces(8, True)]
+        if self.obstacle is not None:
+            # the block's upstream (normal -x), top (+z) and downstream (+x) faces
+            o = 16
+            rows += [(o + 5, o + 8, o + 4, o + 1, 1), (o + 5, o + 6, o + 7, o + 8, 1),
+                     (o + 3, o + 7, o + 6, o + 2, 1)]
         return rows
 
     def zone_types(self):
@@ -177,6 +191,7 @@ class Faces:
     """SA-SPH frontier faces in BFaceList order (faces of non-peri zones)."""
     face_id: np.ndarray      # reference face index (1-based)
     nodes: np.ndarray        # 3 or 4
+    node_ids: np.ndarray     # (nf,4) vertex ids (1-based) of the face's nodes
     btype: np.ndarray        # BT_* code of the face's zone
     zone: np.ndarray
     area: np.ndarray
@@ -229,11 +244,12 @@ def make_faces(case: DamBreak3D) -> Faces:
                 continue
             gx = np.array([vert[n - 1] for n in row[:4]])
             T, lx, area = _local_system(gx)
-            recs.append((fi, 4, _BT_NAMES[ztype[zone]], zone, area, T, gx, lx))
+            recs.append((fi, 4, _BT_NAMES[ztype[zone]], zone, area, T, gx, lx, row[:4]))
     nf = len(recs)
     return Faces(
         face_id=np.array([r[0] for r in recs], np.int32),
         nodes=np.array([r[1] for r in recs], np.int32),
+        node_ids=np.array([r[8] for r in recs], np.int32).reshape(nf, 4),
         btype=np.array([r[2] for r in recs], np.int32),
         zone=np.array([r[3] for r in recs], np.int32),
         area=np.array([r[4] for r in recs]),
@@ -244,6 +260,73 @@ def make_faces(case: DamBreak3D) -> Faces:
         slip_mode=np.full(nf, case.slip_mode, np.int32),
         slip_coeff=np.full(nf, case.frontier_slip_input),
         laminar_check=np.zeros(nf, np.int32))
+
+
+@dataclass
+class ConvexEdges:
+    """BoundaryConvexEdge (Hybrid_allocation_module.f90) of the frontier faces:
+    per edge the two faces (0-based indices into Faces), the first node's
+    position, the vector to the second node and its length."""
+    face: np.ndarray       # (ne,2) int32
+    gx: np.ndarray         # (ne,3)
+    component: np.ndarray  # (ne,3)
+    length: np.ndarray     # (ne,)
+
+    def __len__(self):
+        return len(self.length)
+
+
+def find_convex_edges(faces: Faces, verts) -> ConvexEdges:
+    """FindBoundaryConvexEdges3D.f90:58-145 over the fixe/tapi faces in
+    BFaceList order.  For face nf and each later face nf1 sharing an edge
+    (two node ids) the node of nf1 after the shared edge is tested against
+    nf's plane (normal T(:,3), measured from nf's last node): below it, the
+    edge is convex.  At most one edge is kept per face nf (the loops over nf1
+    end at the first convex edge found)."""
+    nxt = {3: (1, 2, 0), 4: (1, 2, 3, 0)}   # iseg: the node after node i (0-based)
+    ok = [int(b) in (BT_FIXE, BT_TAPI) for b in faces.btype]
+    fl, gx, comp, ln = [], [], [], []
+    nfc = len(faces.face_id)
+    for a in range(nfc - 1):
+        if not ok[a]:
+            continue
+        na = int(faces.nodes[a])
+        ida = faces.node_ids[a]
+        for b in range(a + 1, nfc):
+            if not ok[b]:
+                continue
+            nb = int(faces.nodes[b])
+            idb = faces.node_ids[b]
+            found = False
+            for i in range(na):
+                nodi, nodj = int(ida[i]), int(ida[nxt[na][i]])
+                for ii in range(nb):
+                    jj = nxt[nb][ii]
+                    nodii, nodjj = int(idb[ii]), int(idb[jj])
+                    if (nodi == nodii and nodj == nodjj) or (nodi == nodjj and nodj == nodii):
+                        nodkk = int(idb[nxt[nb][jj]])
+                        ref = faces.GX[a][na - 1]
+                        z = 0.0
+                        for sd in range(3):
+                            z = z + faces.T[a][sd, 2] * (verts[nodkk - 1][sd] - ref[sd])
+                        if z < 0.0:
+                            fl.append((a, b))
+                            gx.append([float(verts[nodi - 1][sd]) for sd in range(3)])
+                            d = [verts[nodj - 1][sd] - verts[nodi - 1][sd] for sd in range(3)]
+                            l2 = 0.0
+                            for sd in range(3):
+                                l2 = l2 + d[sd] * d[sd]
+                            comp.append(d)
+                            ln.append(math.sqrt(l2))
+                            found = True
+                            break
+                if found:
+                    break
+            if found:
+                break
+    ne = len(ln)
+    return ConvexEdges(np.array(fl, np.int32).reshape(ne, 2), np.array(gx, np.float64).reshape(ne, 3),
+                       np.array(comp, np.float64).reshape(ne, 3), np.array(ln, np.float64))
 
 
 def grid_face_incidence(g: Grid, faces: Faces, maxclose: int):
@@ -480,6 +563,7 @@ class CaseTables:
     ktdelta: float
     bitab: np.ndarray
     particles: dict = field(default_factory=dict)
+    edges: "ConvexEdges" = None
 
 
 def build(case: DamBreak3D, particles=True, incidence=True) -> CaseTables:
@@ -499,4 +583,5 @@ def build(case: DamBreak3D, particles=True, incidence=True) -> CaseTables:
     bit = boundary_integral_tab(s["doubleh"])
     parts = initial_particles(case) if particles else {
         k: np.zeros((0, 3) if k in ("coord", "vel") else 0) for k in ("coord", "vel", "pres", "dens", "mass")}
-    return CaseTables(case, s, g, f, cnt, st, vec, kt, ktd, bit, parts)
+    return CaseTables(case, s, g, f, cnt, st, vec, kt, ktd, bit, parts,
+                      find_convex_edges(f, case.vertices()))

@@ -60,6 +60,7 @@ struct sph_ctx {
     hipStream_t stream = nullptr;
     // faces / tables
     DFace* d_faces = nullptr;
+    DEdge* d_edges = nullptr;  // convex frontier edges of every segment
     // per cell key: the frontier faces listed in its 27-cell neighbourhood,
     // each once (count, start into d_gvec)
     int *d_gcount = nullptr, *d_gstart = nullptr, *d_gvec = nullptr;
@@ -318,7 +319,7 @@ int run_dt(sph_ctx* c, int advance, bool reduce) {
 
 int run_momentum(sph_ctx* c) {
     KTime t(c, "momentum");
-    PAIRL(k_momentum, c->chunk, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0], c->rp[cur_rp(c)], XF(c, 0),
+    PAIRL(k_momentum, c->chunk, c->d_faces, c->d_edges, c->bn, c->bt, c->xm[0], c->vel[0], c->rp[cur_rp(c)], XF(c, 0),
           VF(c, 0), c->rpf[cur_rp(c)], c->lam[1], c->lam[0], c->nbr, c->nnb, c->acc[1], c->vel[1], VFo(c, 1), c->scal, c->iscal);
     return 0;
 }
@@ -382,7 +383,7 @@ sph_ctx::~sph_ctx() {
     auto F = [](void* p) {
         if (p) hipFree(p);
     };
-    F(d_faces); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit); F(d_btf);
+    F(d_faces); F(d_edges); F(d_gcount); F(d_gstart); F(d_gvec); F(d_kt); F(d_bit); F(d_btf);
     F(d_P); F(d_pblk); F(d_kblk); F(d_rseg); F(d_adj);
     for (auto p : xm) F(p);
     for (auto p : vel) F(p);
@@ -467,7 +468,6 @@ int check_options(int32_t opt) {
         {SPH_OPT_ALE3, "ALE3 (ALE3_term_momentum.f90, Leapfrog_continuity.f90:70-86)"},
         {SPH_OPT_C1, "C1_BE / C1_divu renormalisation (CalcVarLength.f90:285-324)"},
         {SPH_OPT_KTGF, "KTGF_config > 0 (mixture viscosity and erosion)"},
-        {SPH_OPT_CONVEX_EDGES, "convex frontier edges (AddElasticBoundaryReaction_3D.f90:106-152)"},
         {SPH_OPT_RK2, "the RK2 (Heun) time integration (Heun.f90)"},
     };
     const int32_t known = SPH_OPT_DENSITY_THRESHOLDS | SPH_OPT_ALE3 | SPH_OPT_C1 | SPH_OPT_KTGF |
@@ -505,6 +505,17 @@ int fill_dp(const sph_params* p, const sph_faces* f, DP& P, bool dry = false) {
     if (f && f->nf > 0 && (!f->slip_mode || !f->btype)) return fail(1, "null frontier table");
     for (int q = 0; f && q < f->nf; ++q)
         if (int rc = check_slip_mode(f->slip_mode[q], f->btype[q])) return rc;
+    // convex edges: SPH_OPT_CONVEX_EDGES says NumBEdges > 0, the faces carry them
+    const int ne = (f && p->semi) ? f->nedges : 0;
+    if (f && f->nedges < 0) return fail(2, "negative number of convex edges");
+    if (((p->options & SPH_OPT_CONVEX_EDGES) != 0) != (ne > 0))
+        return fail(2, "SPH_OPT_CONVEX_EDGES is set exactly when the frontier faces carry convex edges");
+    if (ne > 0 && (!f->edge_face || !f->edge_gx || !f->edge_comp || !f->edge_len))
+        return fail(1, "null convex-edge table");
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < 2; ++k)
+            if (f->edge_face[2 * e + k] < 0 || f->edge_face[2 * e + k] >= f->nf)
+                return fail(2, "convex edge face index out of range");
     P = DP{};
     P.cap = p->capacity;
     P.scap = p->capacity;
@@ -515,6 +526,7 @@ int fill_dp(const sph_params* p, const sph_faces* f, DP& P, bool dry = false) {
     P.dthr = (p->options & SPH_OPT_DENSITY_THRESHOLDS) ? 1 : 0;
     P.rk1 = (p->options & SPH_OPT_RK1) ? 1 : 0;
     P.nf = f ? f->nf : 0;
+    P.ne = ne;
     P.dim = 3;
     for (int d = 0; d < 3; ++d) {
         P.ncd[d] = p->ncd[d];
@@ -574,7 +586,7 @@ int layout_segments(sph_ctx* c, int nseg, const sph_params* p, const sph_faces* 
                     bool dry = false) {
     c->nseg = nseg;
     c->segs.assign(nseg, DP{});
-    long long slots = 0, keys = 0, rows = 0, faces = 0;
+    long long slots = 0, keys = 0, rows = 0, faces = 0, edges = 0;
     int nmaxpartj = 0;
     for (int s = 0; s < nseg; ++s) {
         const sph_faces* fs = f ? f[s] : nullptr;
@@ -588,6 +600,7 @@ int layout_segments(sph_ctx* c, int nseg, const sph_params* p, const sph_faces* 
         P.koff = (int)keys;
         P.roff = (int)rows;
         P.foff = (int)faces;
+        P.eoff = (int)edges;
         if (nseg == 1) {
             slots = P.cap;
             keys = (long long)P.nmax + 1;
@@ -597,6 +610,7 @@ int layout_segments(sph_ctx* c, int nseg, const sph_params* p, const sph_faces* 
         }
         rows += (long long)P.ncd[1] * P.ncd[2];
         faces += P.nf;
+        edges += P.ne;
         nmaxpartj = std::max(nmaxpartj, P.nmaxpartj);
         if (slots > (long long)NB_MASK || keys >= (1LL << 30))
             return fail(2, "batch too large for one context");
@@ -668,6 +682,7 @@ int upload_tables(sph_ctx* c, const sph_faces* const* f) {
     }
     std::vector<DFace> hf;
     hf.reserve(nf);
+    std::vector<DEdge> he;
     // per key: count and 0-based start of the cell's near-face list in gvec
     std::vector<int> gcount(c->nkeys, 0), gstart(c->nkeys, 0), gvec;
     gvec.reserve(nv);
@@ -727,11 +742,24 @@ int upload_tables(sph_ctx* c, const sph_faces* const* f) {
                     gstart[key] = (int)gvec.size();
                     gvec.insert(gvec.end(), seen.begin(), seen.end());
                 }
+        for (int e = 0; e < P.ne; ++e) {
+            DEdge E;
+            std::memset(&E, 0, sizeof(E));
+            for (int d = 0; d < 3; ++d) {
+                E.gx[d] = fs->edge_gx[3 * e + d];
+                E.comp[d] = fs->edge_comp[3 * e + d];
+            }
+            E.len = fs->edge_len[e];
+            E.face[0] = fs->edge_face[2 * e] + P.foff;
+            E.face[1] = fs->edge_face[2 * e + 1] + P.foff;
+            he.push_back(E);
+        }
         kt.insert(kt.end(), fs->kerneltab, fs->kerneltab + 41 * 5);
         bit.insert(bit.end(), fs->bitab, fs->bitab + 256 * 7);
         build_btf32(fs->bitab, fs->kerneltab, fs->ktdelta, P.h, btf[s]);
     }
     if (gvec.empty()) gvec.push_back(0);
+    if (he.empty()) he.push_back(DEdge{});
     auto up = [&](auto** d, const auto& h) {
         using T = typename std::remove_reference<decltype(h[0])>::type;
         return hipMalloc((void**)d, sizeof(T) * h.size()) == hipSuccess &&
@@ -739,7 +767,7 @@ int upload_tables(sph_ctx* c, const sph_faces* const* f) {
     };
     bool ok = up(&c->d_faces, hf) && up(&c->d_gcount, gcount) && up(&c->d_gstart, gstart) &&
               up(&c->d_gvec, gvec) && up(&c->d_kt, kt) && up(&c->d_bit, bit) &&
-              up(&c->d_btf, btf);
+              up(&c->d_btf, btf) && up(&c->d_edges, he);
     if (!ok) return fail(102, "device allocation/copy of frontier tables failed");
     return 0;
 }
@@ -884,6 +912,7 @@ int sph_ctx_create2d(const sph_params* p, const sph_sides2d* sd, sph_ctx** out) 
     if (p->ncd[1] != 1) return fail(2, "a 2D grid has one cell row in y (ncd[1] = 1)");
     if (p->semi && (!sd || sd->ns <= 0)) return fail(2, "SA-SPH requested without boundary sides");
     if (int rc = check_options(p->options)) return rc;
+    if (p->options & SPH_OPT_CONVEX_EDGES) return fail(2, "convex edges are a 3D frontier feature");
     for (int k = 0; sd && k < sd->ns; ++k)
         if (int rc = check_slip_mode(sd->slip_mode[k], sd->btype[k])) return rc;
     sph_params q = *p;

@@ -72,6 +72,8 @@ struct DP {
                // particles out of the grid (a multiple of 256)
     int roff;  // first cell row (fixed j, k) of the segment in the context's row list
     int foff;  // first frontier face of the segment in the context's face array
+    int eoff;  // first convex edge of the segment in the context's edge array
+    int ne;    // its convex edges (NumBEdges)
 };
 
 // Segment maps of a segmented context, passed by value to the kernels that
@@ -84,6 +86,14 @@ struct Seg {
     const int* kblk;    // segment of each 256-key block
     const int* rseg;    // segment of each cell row
     int nseg, nrows;
+};
+
+// one convex frontier edge (BoundaryConvexEdge, FindBoundaryConvexEdges3D.f90):
+// its faces (indices into the context's face array), first node, the vector
+// to the second node and its length
+struct DEdge {
+    double gx[3], comp[3], len;
+    int face[2];
 };
 
 // one SA-SPH frontier face (BoundaryFace + its Tratto/Partz data)

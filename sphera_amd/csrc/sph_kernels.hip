@@ -2165,7 +2165,7 @@ __device__ __forceinline__ void elastic_2d(const DP& P, const DFace* __restrict_
 // ----------------------------------------------------------------------------
 template <bool F32, int CH, bool SG>
 __global__ __launch_bounds__(256) void k_momentum(
-    DP P0, Seg S, const DFace* __restrict__ faces,
+    DP P0, Seg S, const DFace* __restrict__ faces, const DEdge* __restrict__ edges,
     const int* __restrict__ bn, const BTab* __restrict__ bt, const double4* __restrict__ xm,
     const double4* __restrict__ vel, const double2* __restrict__ rp, F4In xmf,
     F4In velf, const float2* __restrict__ rpf, const int* __restrict__ lam_in,
@@ -2316,6 +2316,10 @@ __global__ __launch_bounds__(256) void k_momentum(
                 double br[3] = {0.0, 0.0, 0.0};
                 const double zimin = 0.25 * P.dx;
                 const double celer02 = P.eps / P.den0;
+                // AddElasticBoundaryReaction_3D.f90:72-104 (faces); rea: the
+                // slots whose face reacted (ReaFace), for the edges below
+                static_assert(SPH_MB <= 32, "one ReaFace bit per boundary-table slot");
+                unsigned rea = 0u;
                 for (int q = 0; q < ncbf; ++q) {
                     const BTab& B = bt[(size_t)q * P.cap + i];
                     const DFace& F = faces[B.face];
@@ -2327,6 +2331,41 @@ __global__ __launch_bounds__(256) void k_momentum(
                         if (vin < 0.0) {
                             const double nr = -1.0 * celer02 * log((h + zi - zimin) / h) / h;
                             for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * F.T[6 + d];
+                        }
+                        rea |= 1u << q;
+                    }
+                }
+                // AddElasticBoundaryReaction_3D.f90:106-152 (convex edges, in
+                // order; the reference returns at the first edge whose two
+                // faces are not both among the particle's non-reacting faces)
+                for (int ne = 0; ne < P.ne; ++ne) {
+                    const DEdge& E = edges[P.eoff + ne];
+                    int nclose = 0;
+                    for (int q = 0; q < ncbf; ++q) {
+                        const int f = bt[(size_t)q * P.cap + i].face;
+                        const bool rq = (rea >> q) & 1u;
+                        if (f == E.face[0] && !rq) ++nclose;
+                        else if (f == E.face[1] && !rq) ++nclose;
+                    }
+                    if (nclose != 2) break;
+                    const double4 xi = xm[i];
+                    const double x[3] = {xi.x, xi.y, xi.z};
+                    double scaprod = 0.0;
+                    for (int sd = 0; sd < 3; ++sd) scaprod = scaprod + (x[sd] - E.gx[sd]) * E.comp[sd];
+                    const double tau = scaprod / (E.len * E.len);
+                    if (tau >= 0.0 && tau <= 1.0) {
+                        double QP[3], d2 = 0.0;
+                        for (int sd = 0; sd < 3; ++sd) QP[sd] = x[sd] - (E.gx[sd] + E.comp[sd] * tau);
+                        for (int sd = 0; sd < 3; ++sd) d2 = d2 + QP[sd] * QP[sd];
+                        const double zi = sqrt(d2);
+                        if (zi < zimin) {
+                            double cd[3], vin = 0.0;
+                            for (int sd = 0; sd < 3; ++sd) cd[sd] = QP[sd] / zi;
+                            for (int sd = 0; sd < 3; ++sd) vin = vin + v_i[sd] * cd[sd];
+                            if (vin < 0.0) {
+                                const double nr = -1.0 * celer02 * log(zi / zimin) / zimin;
+                                for (int d = 0; d < 3; ++d) br[d] = br[d] + nr * cd[d];
+                            }
                         }
                     }
                 }

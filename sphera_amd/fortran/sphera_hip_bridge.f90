@@ -41,6 +41,9 @@ real(c_double),allocatable,target,save :: fT(:), fLX(:), fGX(:), farea(:), fvel(
 real(c_double),allocatable,target,save :: ftvel(:), fslip(:), fkt(:), fbit(:)
 integer(c_int32_t),allocatable,target,save :: fnodes(:), fbtype(:), fmode(:), flam(:)
 integer(c_int32_t),allocatable,target,save :: gcount(:), gstart(:), gvec(:)
+! convex frontier edges (BoundaryConvexEdge, in the reference's order)
+real(c_double),allocatable,target,save :: egx(:), ecomp(:), elen(:)
+integer(c_int32_t),allocatable,target,save :: eface(:)
 ! 2D: boundary sides
 real(c_double),allocatable,target,save :: sRN(:), sv1(:), slen(:), sang(:)
 integer(c_int32_t),allocatable,target,save :: sprev(:)
@@ -117,7 +120,8 @@ p%chunk_c = 0
 p%chunk_p = 0
 p%nb_shape = 0
 ! Deck options the hot path branches on (SPH_OPT_* of sphera_hip.h): the
-! density clamp is built; the device refuses the others with status 95
+! density clamp, RK1 and convex edges are built; the device refuses ALE3,
+! C1, KTGF and RK2 with status 95
 p%options = 0
 if (input_any_t%density_thresholds==1) p%options = ior(p%options,1)
 if (input_any_t%ALE3) p%options = ior(p%options,2)
@@ -264,6 +268,25 @@ f%gcbf_vec = c_loc(gvec)
 f%kerneltab = c_loc(fkt)
 f%ktdelta = ktdelta
 f%bitab = c_loc(fbit)
+! FindBoundaryConvexEdges3D's edges, faces as 0-based positions
+allocate(eface(2*max(NumBEdges,1)),egx(3*max(NumBEdges,1)),ecomp(3*max(NumBEdges,1)))
+allocate(elen(max(NumBEdges,1)))
+eface(:) = 0
+egx(:) = 0.d0
+ecomp(:) = 0.d0
+elen(:) = 0.d0
+do k=1,NumBEdges
+   eface(2*k-1) = pos(BoundaryConvexEdge(k)%face(1))
+   eface(2*k) = pos(BoundaryConvexEdge(k)%face(2))
+   egx(3*k-2:3*k) = BoundaryConvexEdge(k)%node(1)%GX(1:3)
+   ecomp(3*k-2:3*k) = BoundaryConvexEdge(k)%component(1:3)
+   elen(k) = BoundaryConvexEdge(k)%length
+enddo
+f%nedges = NumBEdges
+f%edge_face = c_loc(eface)
+f%edge_gx = c_loc(egx)
+f%edge_comp = c_loc(ecomp)
+f%edge_len = c_loc(elen)
 deallocate(pos)
 end subroutine pack_faces
 #endif
@@ -275,6 +298,7 @@ deallocate(fT,fvel,ftvel,fslip,fbtype,fmode,flam)
 if (allocated(gcount)) deallocate(gcount,gstart,gvec)
 if (allocated(fkt)) deallocate(fkt)
 if (allocated(fbit)) deallocate(fbit)
+if (allocated(eface)) deallocate(eface,egx,ecomp,elen)
 end subroutine free_tables
 
 ! Creates the device context for the current case and uploads pg(1:nag).

@@ -34,6 +34,8 @@ type,bind(C) :: sph_faces_t
    type(c_ptr) :: kerneltab
    real(c_double) :: ktdelta
    type(c_ptr) :: bitab
+   integer(c_int32_t) :: nedges
+   type(c_ptr) :: edge_face, edge_gx, edge_comp, edge_len
 end type sph_faces_t
 
 ! 2D boundary sides (sph_sides2d of sphera_hip.h)

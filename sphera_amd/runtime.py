@@ -62,7 +62,8 @@ class SphFaces(C.Structure):
                 ("velocity", PD), ("tvel", PD), ("slip_coeff", PD), ("nodes", PI),
                 ("btype", PI), ("slip_mode", PI), ("lam_check", PI), ("gcbf_count", PI),
                 ("gcbf_start", PI), ("gcbf_vec", PI), ("kerneltab", PD), ("ktdelta", C.c_double),
-                ("bitab", PD)]
+                ("bitab", PD), ("nedges", C.c_int32), ("edge_face", PI), ("edge_gx", PD),
+                ("edge_comp", PD), ("edge_len", PD)]
 
 
 class SphSides2D(C.Structure):
@@ -155,11 +156,25 @@ def faces_struct(tb):
         gs=np.ascontiguousarray(tb.gcbf_start, np.int32),
         gv=np.ascontiguousarray(tb.gcbf_vector if len(tb.gcbf_vector) else np.zeros(1), np.int32),
         kt=np.ascontiguousarray(tb.kerneltab.reshape(-1)), bit=np.ascontiguousarray(tb.bitab.reshape(-1)))
+    ne = edge_arrays(tb, k)
     s = SphFaces(len(F.face_id), _pd(k["T"]), _pd(k["LX"]), _pd(k["GX"]), _pd(k["area"]),
                  _pd(k["vel"]), _pd(k["tvel"]), _pd(k["slip"]), _pi(k["nodes"]), _pi(k["btype"]),
                  _pi(k["mode"]), _pi(k["lam"]), _pi(k["gc"]), _pi(k["gs"]), _pi(k["gv"]),
-                 _pd(k["kt"]), tb.ktdelta, _pd(k["bit"]))
+                 _pd(k["kt"]), tb.ktdelta, _pd(k["bit"]), ne, _pi(k["ef"]), _pd(k["egx"]),
+                 _pd(k["ecomp"]), _pd(k["elen"]))
     return s, k
+
+
+def edge_arrays(tb, k):
+    """Adds the convex-edge arrays of CaseTables (contiguous, at least one
+    element each) to the keepalive dict k; returns the number of edges."""
+    E = getattr(tb, "edges", None)
+    ne = len(E) if E is not None else 0
+    k["ef"] = np.ascontiguousarray(E.face.reshape(-1) if ne else np.zeros(2), np.int32)
+    k["egx"] = np.ascontiguousarray(E.gx.reshape(-1) if ne else np.zeros(3), np.float64)
+    k["ecomp"] = np.ascontiguousarray(E.component.reshape(-1) if ne else np.zeros(3), np.float64)
+    k["elen"] = np.ascontiguousarray(E.length if ne else np.zeros(1), np.float64)
+    return ne
 
 
 def sides_struct(tb):
@@ -204,6 +219,8 @@ def params_struct(tb, precision=FP64, capacity=None, semi=True, tune=None):
     p.options = OPT_DENSITY_THRESHOLDS if int(getattr(case, "density_thresholds", 0)) == 1 else 0
     if int(getattr(case, "time_split", 1)) == 0:
         p.options |= OPT_RK1
+    if semi and getattr(tb, "edges", None) is not None and len(tb.edges):
+        p.options |= OPT_CONVEX_EDGES
     for k, v in (tune or {}).items():
         setattr(p, k, int(v))
     return p

@@ -6,7 +6,7 @@ from /root/reference/src) on project-authored decks.
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
 With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp,
-rk1, viscous, wallfn, wallfn_2d) only those fixtures are written.
+rk1, viscous, wallfn, wallfn_2d, edges) only those fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -48,6 +48,18 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         and 8 steps
   ref_wallfn_2d.npz     the same for the 2D build (600 particles, v_x = 0.2 +
                         2 z, v_z += 0.2)
+  ref_edges.npz         800 particles; a block on the floor across the tank
+                        (0.15 < x < 0.2 m, top at z = 0.04 m) adds three
+                        frontier faces and two convex edges
+                        (FindBoundaryConvexEdges3D); the top layer of the
+                        column (80 particles) is moved next to the block, in
+                        four groups of 20 moving into it: beyond the first
+                        edge (within 0.16 dx of it, outside both faces), in
+                        front of the upstream face, above the top face and
+                        beyond the second edge; convex-edge and face
+                        reactions run; state after 0, 4 and 8 steps (one
+                        OpenMP thread: the reference's edge order is the
+                        threads' arrival order)
   ref_rk1.npz           1440 particles, jittered lattice (seed 20260101), the
                         explicit RK1 branch (time_split = 0, RKscheme = 1:
                         Euler); state after 0, 6 and 12 steps
@@ -285,6 +297,59 @@ def make_nearwall_2d():
     np.savez_compressed(HERE / "ref_nearwall_2d.npz", **out)
 
 
+OBSTACLE = (0.15, 0.2, 0.04)
+
+
+def obstacle_ic(coord, dx, seed=20261204):
+    """The column's top layer (80 particles) moved next to the block x0 < x <
+    x1, z < zt, in four groups of 20 spread over y, each moving into the block
+    at U(0.02, 0.08) m/s per component: beyond the edge (x0, zt) at
+    U(0.03, 0.16) dx from both planes, in front of the upstream face
+    (z < zt), above the top face (x0 < x < x1), beyond the edge (x1, zt)."""
+    rng = np.random.default_rng(seed)
+    x0, x1, zt = OBSTACLE
+    c = coord.copy()
+    v = np.zeros_like(c)
+    top = np.flatnonzero(c[:, 2] > c[:, 2].max() - 0.5 * dx)
+    assert len(top) == 80, len(top)
+    a = rng.uniform(0.03, 0.16, (80, 2)) * dx
+    sp = rng.uniform(0.02, 0.08, (80, 2))
+    y = np.tile(0.0025 + 0.005 * np.arange(20), 4) + rng.uniform(-0.001, 0.001, 80)
+    for g in range(4):
+        idx = top[20 * g:20 * (g + 1)]
+        s = slice(20 * g, 20 * (g + 1))
+        if g == 0:    # beyond the first edge, moving towards it
+            c[idx] = np.stack([x0 - a[s, 0], y[s], zt + a[s, 1]], 1)
+            v[idx] = np.stack([sp[s, 0], 0 * y[s], -sp[s, 1]], 1)
+        elif g == 1:  # in front of the upstream face
+            c[idx] = np.stack([x0 - a[s, 0], y[s], rng.uniform(0.3, 0.8, 20) * zt], 1)
+            v[idx] = np.stack([sp[s, 0], 0 * y[s], 0 * y[s]], 1)
+        elif g == 2:  # above the top face
+            c[idx] = np.stack([rng.uniform(x0 + 0.1 * (x1 - x0), x1 - 0.1 * (x1 - x0), 20), y[s],
+                               zt + a[s, 1]], 1)
+            v[idx] = np.stack([0 * y[s], 0 * y[s], -sp[s, 1]], 1)
+        else:         # beyond the second edge (the reference returns at the first edge)
+            c[idx] = np.stack([x1 + a[s, 0], y[s], zt + a[s, 1]], 1)
+            v[idx] = np.stack([-sp[s, 0], 0 * y[s], -sp[s, 1]], 1)
+    return c, v
+
+
+def make_edges():
+    """a block on the floor: convex-edge elastic reaction"""
+    case = DamBreak3D(**TINY, obstacle=OBSTACLE)
+    tb = build(case)
+    ic, iv = obstacle_ic(tb.particles["coord"], case.dx)
+    # one thread: FindBoundaryConvexEdges3D.f90:69-146 appends the edges it
+    # finds from an OpenMP loop in thread-arrival order, and the reaction
+    # loop returns at the first edge that does not apply, so with several
+    # threads the reacting particles change from run to run
+    hdr, steps = read_dump(run_harness(case, nsteps=8, every=4, lists=False, ic_coord=ic, ic_vel=iv,
+                                       threads=1))
+    out = {"ic_coord": ic, "ic_vel": iv}
+    _pack_steps(steps, out, lists=False)
+    np.savez_compressed(HERE / "ref_edges.npz", **out)
+
+
 WALLFN = dict(slip_mode=2, roughness=(0.0005, 0.0015))
 
 
@@ -358,7 +423,8 @@ def make_rk1():
 MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_removal,
           "pelo": make_pelo, "2d": make_2d, "nearwall": make_nearwall,
           "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp, "rk1": make_rk1,
-          "viscous": make_viscous, "wallfn": make_wallfn, "wallfn_2d": make_wallfn_2d}
+          "viscous": make_viscous, "wallfn": make_wallfn, "wallfn_2d": make_wallfn_2d,
+          "edges": make_edges}
 
 
 def main():

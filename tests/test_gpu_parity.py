@@ -448,6 +448,37 @@ def test_wall_function_fixture():
     f32.close()
 
 
+def test_convex_edges_fixture():
+    """A block on the floor with two convex frontier edges (tests/golden/
+    ref_edges.npz): the edge reaction of AddElasticBoundaryReaction_3D.f90:
+    106-152 runs for the particles beyond the first edge.  Device FP64 within
+    FP64_LOG_TOL of the reference after 4 and 8 steps (the reactions' log()
+    is the device library's), FP32-mixed within FP32_TOL."""
+    rt = _rt()
+    g = np.load(GOLD / "ref_edges.npz")
+    tb = build(DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny",
+                          obstacle=(0.15, 0.2, 0.04)))
+    assert len(tb.edges) == 2
+    print(f"[gpu] convex edges n={len(g['ic_coord'])}", flush=True)
+    d, f32 = _gold_device(tb, g, rt.FP64), _gold_device(tb, g, rt.FP32_MIXED)
+    for it in (4, 8):
+        d.step(4)
+        f32.step(4)
+        ds, fs = d.state(), f32.state()
+        assert ds["nag"] == g[f"s{it}_nag"]
+        exact = sum(bool(np.array_equal(ds[f], g[f"s{it}_" + f])) for f in FIELDS)
+        e64 = {f: _rel(ds[f], g[f"s{it}_" + f]) for f in FIELDS}
+        e32 = {f: _rel(fs[f], g[f"s{it}_" + f]) for f in FIELDS}
+        print(f"[gpu] convex edges it={it} fp64 bit-identical fields {exact}/7, max rel "
+              f"{max(e64.values()):.2e}; fp32 {{{', '.join(f'{k}: {v:.1e}' for k, v in e32.items())}}}",
+              flush=True)
+        for f in FIELDS:
+            assert e64[f] <= FP64_LOG_TOL, (it, f, e64[f])
+            assert e32[f] <= FP32_TOL[f], (it, f, e32[f])
+    d.close()
+    f32.close()
+
+
 def test_density_clamp_fixture():
     """density_thresholds = 1 (Leapfrog_continuity.f90:61-68) on the
     reference's fixture (tests/golden/ref_dclamp.npz: converging and

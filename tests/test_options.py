@@ -1,12 +1,13 @@
 """Deck options the device does not build are refused, not run without them.
 
 sph_ctx_create / sph_ctx_create2d / sph_batch_create return status 95 for
-ALE3, C1_BE / C1_divu, KTGF_config > 0 and convex frontier edges
-(NumBEdges > 0), and status 2 for a slip_coefficient_mode outside 0-2.  The
-checks run before any device call, so these tests run without a GPU.  The
-density clamp (density_thresholds = 1) and the SA-SPH wall function
-(slip_coefficient_mode = 2) are built and are accepted; their parity tests are
-with the other parity tests."""
+ALE3, C1_BE / C1_divu and KTGF_config > 0, and status 2 for a
+slip_coefficient_mode outside 0-2 or a convex-edge option that disagrees with
+the edge table.  The checks run before any device call, so these tests run
+without a GPU.  The density clamp (density_thresholds = 1), the SA-SPH wall
+function (slip_coefficient_mode = 2) and convex frontier edges (NumBEdges > 0)
+are built and are accepted; their parity tests are with the other parity
+tests."""
 import ctypes as C
 
 import numpy as np
@@ -36,11 +37,36 @@ def _create(tb, options=0, faces_patch=None):
 
 
 @pytest.mark.parametrize("opt,what", [(rt.OPT_ALE3, "ALE3"), (rt.OPT_C1, "C1_BE"),
-                                      (rt.OPT_KTGF, "KTGF_config"),
-                                      (rt.OPT_CONVEX_EDGES, "convex frontier edges")])
+                                      (rt.OPT_KTGF, "KTGF_config")])
 def test_unbuilt_option_refused(opt, what):
     rc, msg = _create(_tiny(), options=opt | rt.OPT_DENSITY_THRESHOLDS)
     assert rc == 95 and what in msg, (rc, msg)
+
+
+def test_convex_edge_option_matches_the_edge_table():
+    """SPH_OPT_CONVEX_EDGES (NumBEdges > 0) is built; it must agree with the
+    convex edges the frontier faces carry, and a 2D context has none."""
+    tb = _tiny()
+    rc, msg = _create(tb, options=rt.OPT_CONVEX_EDGES)
+    assert rc == 2 and "SPH_OPT_CONVEX_EDGES" in msg, (rc, msg)
+    case = DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny",
+                      obstacle=(0.15, 0.2, 0.04))
+    tbe = build(case)
+    assert rt.params_struct(tbe).options & rt.OPT_CONVEX_EDGES
+    rc, msg = _create(tbe, options=0)
+    assert rc == 2 and "SPH_OPT_CONVEX_EDGES" in msg, (rc, msg)
+
+    def bad_face(fs, keep):
+        keep["ef"][:] = 99
+    rc, msg = _create(tbe, options=rt.OPT_CONVEX_EDGES, faces_patch=bad_face)
+    assert rc == 2 and "out of range" in msg, (rc, msg)
+    L = rt.load()
+    t2 = build2d(DamBreak2D(tank=(0.6, 0.4), column=(0.2, 0.1), dx=0.01, name="t2d"))
+    p = rt.params_struct(t2)
+    p.options = rt.OPT_CONVEX_EDGES
+    ss, keep = rt.sides_struct(t2)
+    h = C.c_void_p()
+    assert L.sph_ctx_create2d(C.byref(p), C.byref(ss), C.byref(h)) == 2
 
 
 def test_unknown_option_bits_rejected():

@@ -393,3 +393,28 @@ def test_golden_wall_function_bit_identical(dim):
     o0.step(8)
     assert not np.array_equal(o0.state()["vel"], g["s8_vel"])
     assert o0.branch_counts()["wall_fn_near"] == 0
+
+
+def test_golden_convex_edges_bit_identical():
+    """A block on the floor adds two convex frontier edges
+    (FindBoundaryConvexEdges3D, restated in sphera_amd.case.find_convex_edges);
+    particles beyond the first edge moving into it get the edge reaction of
+    AddElasticBoundaryReaction_3D.f90:106-152, those beyond the second edge
+    none (the subroutine returns at the first edge that does not apply), and
+    lists, fields, dt and time equal the reference's (one OpenMP thread) after
+    0, 4 and 8 steps."""
+    g = np.load(GOLD / "ref_edges.npz")
+    case = _tiny()
+    case.obstacle = (0.15, 0.2, 0.04)
+    tb = build(case)
+    e = tb.edges
+    assert len(e) == 2
+    assert [tuple(tb.faces.face_id[f] for f in pair) for pair in e.face] == [(12, 13), (13, 14)]
+    assert np.array_equal(e.gx, [[0.15, 0.0, 0.04], [0.2, 0.0, 0.04]])
+    assert np.array_equal(e.component, [[0.0, 0.1, 0.0]] * 2) and np.array_equal(e.length, [0.1, 0.1])
+    assert len(build(_tiny()).edges) == 0  # the tank alone: every edge concave
+    o = Oracle(tb, coord=g["ic_coord"], vel=g["ic_vel"])
+    o.preloop()
+    _oracle_steps(o, g, (0, 4, 8))
+    n = o.branch_counts()
+    assert n["convex_edge"] >= 15 and n["elastic"] > 20, n
