@@ -73,9 +73,12 @@ def lib():
             getattr(L, name).restype = None
         for name in ("or_nag", "or_ordgrid", "or_calcvarlength", "or_boundary_tab", "or_dt",
                      "or_momentum", "or_vsmooth", "or_leapfrog_traj", "or_continuity",
-                     "or_leapfrog_cont", "or_eos", "or_psmooth", "or_preloop", "or_overflow"):
+                     "or_leapfrog_cont", "or_eos", "or_psmooth", "or_preloop", "or_overflow",
+                     "or_cancel_outgone"):
             getattr(L, name).argtypes = [C.c_void_p]
             getattr(L, name).restype = C.c_int
+        L.or_outgone_count.argtypes = [C.c_void_p]
+        L.or_outgone_count.restype = C.c_longlong
         L.or_leapfrog_mom.argtypes = [C.c_void_p, C.c_double]
         L.or_step.argtypes = [C.c_void_p, C.c_int]
         L.or_pair_count.argtypes = [C.c_void_p]
@@ -274,6 +277,10 @@ class Oracle:
         list build: list checks of large clouds; the passes cannot run."""
         lib().or_set_lists_only(self.ctx, 1)
         return self.call("calcvarlength")
+
+    def outgone_count(self):
+        """Particles removed through open faces so far (CancelOutgoneParticles_3D)."""
+        return int(lib().or_outgone_count(self.ctx))
 
     def branch_counts(self):
         """How often the rare branches ran: elastic wall reactions applied,

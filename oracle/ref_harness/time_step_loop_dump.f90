@@ -195,8 +195,10 @@ do while (it<nsteps)
       call EoS_fluid_loop
 #ifdef SPACE_2D
       if (input_any_t%TetaP>zero) call PressureSmoothing_2D
+      if (NumOpenSides>0) call CancelOutgoneParticles_2D
 #else
       if (input_any_t%TetaP>zero) call PressureSmoothing_3D
+      if (NumOpenFaces>0) call CancelOutgoneParticles_3D
       call BC_zmax_anyt
 #endif
       call OrdGrid1

@@ -39,6 +39,7 @@ struct Ctx {
     // state, pg order
     int n;
     std::vector<double> coord, vel, var, acc, dens, pres, dden, vpres, mass;
+    std::vector<double> coord_old;  // CoordOld: the position before the last trajectory update
     std::vector<int> cella, kodvel, koddens, laminar;
     std::vector<int> id;  // not used by the physics; tracks swap-with-last removals
     // lists
@@ -63,6 +64,7 @@ struct Ctx {
     // density_thresholds clamps, wall-function interactions below z0 e and above
     mutable std::atomic<long long> n_elastic{0}, n_zpmin{0}, n_dclamp{0}, n_wf_near{0}, n_wf_log{0},
         n_edge{0};
+    long long n_outgone = 0;  // particles removed through open faces (OpCount)
     // lists only: CalcVarLength stores the neighbour lists without rag and
     // PartKernel (list checks of large clouds; the passes cannot run)
     bool lists_only = false;
@@ -1415,11 +1417,117 @@ int or_vsmooth(void* ctx) {
 // Leapfrog_trajectories.f90:48-76
 int or_leapfrog_traj(void* ctx) {
     Ctx& c = *(Ctx*)ctx;
+    c.coord_old.resize(3 * (size_t)c.n);
     for (int i = 0; i < c.n; ++i) {
         if (c.cella[i] <= 0) continue;
+        for (int d = 0; d < 3; ++d) c.coord_old[3 * i + d] = c.coord[3 * i + d];
         for (int d = 0; d < 3; ++d) c.coord[3 * i + d] = c.coord[3 * i + d] + c.dt * c.vel[3 * i + d];
     }
     return 0;
+}
+
+// line_plane_intersection.f90:39-76 with three_plane_intersection.f90:42-66
+// and Matrix_Inversion_3x3.f90:39-58: the point where the line through P1
+// and P2 meets the plane through A, B and C, as the intersection of two
+// planes through the line (each with a point 999 away from P1) and the
+// plane; false when the 3x3 system's |determinant| is below 1e-36.
+static bool line_plane(const double P1[3], const double P2[3], const double A[3], const double B[3],
+                const double C[3], double X[3]) {
+    const double Q1[3] = {P1[0] - 999.0, P1[1], P1[2]};
+    const double Q2[3] = {P1[0], P1[1] - 999.0, P2[2] == P2[2] ? P1[2] - 999.0 : P1[2]};
+    auto plane_line = [&](const double Q[3], double& a, double& b, double& cc, double& d) {
+        a = (P2[1] - P1[1]) * (Q[2] - P1[2]) - (Q[1] - P1[1]) * (P2[2] - P1[2]);
+        b = -((P2[0] - P1[0]) * (Q[2] - P1[2]) - (Q[0] - P1[0]) * (P2[2] - P1[2]));
+        cc = (P2[0] - P1[0]) * (Q[1] - P1[1]) - (Q[0] - P1[0]) * (P2[1] - P1[1]);
+        d = -(a * P1[0] + b * P1[1] + cc * P1[2]);
+    };
+    double m[3][3], dv[3];
+    plane_line(Q1, m[0][0], m[0][1], m[0][2], dv[0]);
+    plane_line(Q2, m[1][0], m[1][1], m[1][2], dv[1]);
+    m[2][0] = (B[1] - A[1]) * (C[2] - A[2]) - (C[1] - A[1]) * (B[2] - A[2]);
+    m[2][1] = -((B[0] - A[0]) * (C[2] - A[2]) - (C[0] - A[0]) * (B[2] - A[2]));
+    m[2][2] = (B[0] - A[0]) * (C[1] - A[1]) - (C[0] - A[0]) * (B[1] - A[1]);
+    dv[2] = -(m[2][0] * A[0] + m[2][1] * A[1] + m[2][2] * A[2]);
+    const double det = m[0][0] * m[1][1] * m[2][2] + m[1][0] * m[2][1] * m[0][2] +
+                       m[2][0] * m[0][1] * m[1][2] - m[0][0] * m[2][1] * m[1][2] -
+                       m[2][0] * m[1][1] * m[0][2] - m[1][0] * m[0][1] * m[2][2];
+    if (!(std::fabs(det) >= 1.0e-36)) return false;
+    double inv[3][3];
+    inv[0][0] = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    inv[0][1] = m[0][2] * m[2][1] - m[0][1] * m[2][2];
+    inv[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+    inv[1][0] = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    inv[1][1] = m[0][0] * m[2][2] - m[0][2] * m[2][0];
+    inv[1][2] = m[0][2] * m[1][0] - m[0][0] * m[1][2];
+    inv[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    inv[2][1] = m[0][1] * m[2][0] - m[0][0] * m[2][1];
+    inv[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) inv[i][j] = inv[i][j] / det;
+    const double b[3] = {-dv[0], -dv[1], -dv[2]};
+    for (int i = 0; i < 3; ++i) X[i] = inv[i][0] * b[0] + inv[i][1] * b[1] + inv[i][2] * b[2];
+    return true;
+}
+
+// CancelOutgoneParticles_3D.f90:67-160: a particle whose last trajectory step
+// crossed an open face (from its inner to its outer side, through the face)
+// gets cella = -1, and the next OrdGrid1 removes it.  Returns the number of
+// particles marked.
+int or_cancel_outgone(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    int marked = 0;
+    if (c.coord_old.size() < 3 * (size_t)c.n) return 0;
+    for (int f = 0; f < c.nf; ++f) {
+        if (c.btype[f] != 2) continue;
+        const int nodes = c.nodes[f];
+        const double* G1 = &c.GX[12 * f];
+        const double* G2 = &c.GX[12 * f + 3];
+        const double* G3 = &c.GX[12 * f + 6];
+        const double* Gn = &c.GX[12 * f + 3 * (nodes - 1)];
+        const double dx21 = G2[0] - G1[0], dx31 = G3[0] - G1[0];
+        const double dy21 = G2[1] - G1[1], dy31 = G3[1] - G1[1];
+        const double dz21 = G2[2] - G1[2], dz31 = G3[2] - G1[2];
+        double Pp[3][3];  // the face's nodes 1-3 in its local system (from its last node)
+        const double* Gk[3] = {G1, G2, G3};
+        for (int k = 0; k < 3; ++k)
+            for (int sdi = 0; sdi < 3; ++sdi) {
+                double a = 0.0;
+                for (int sdj = 0; sdj < 3; ++sdj) a = a + Tm(c, f, sdj, sdi) * (Gk[k][sdj] - Gn[sdj]);
+                Pp[k][sdi] = a;
+            }
+        auto det_of = [&](const double* X) {
+            const double dx = X[0] - G1[0], dy = X[1] - G1[1], dz = X[2] - G1[2];
+            return dx21 * dy31 * dz + dx31 * dy * dz21 + dx * dy21 * dz31 - dx21 * dy * dz31 -
+                   dx31 * dy21 * dz - dx * dy31 * dz21;
+        };
+        for (int i = 0; i < c.n; ++i) {
+            if (c.cella[i] == 0) continue;
+            const double* Xn = &c.coord[3 * i];
+            const double* Xo = &c.coord_old[3 * i];
+            const double dn = det_of(Xn);
+            if (!(dn <= 0.0)) continue;
+            const double dpo = det_of(Xo);
+            if (std::copysign(1.0, dn) == std::copysign(1.0, dpo)) continue;
+            double Ln[3], Lo[3];
+            for (int sdi = 0; sdi < 3; ++sdi) {
+                double a = 0.0, b = 0.0;
+                for (int sdj = 0; sdj < 3; ++sdj) {
+                    a = a + Tm(c, f, sdj, sdi) * (Xn[sdj] - Gn[sdj]);
+                    b = b + Tm(c, f, sdj, sdi) * (Xo[sdj] - Gn[sdj]);
+                }
+                Ln[sdi] = a;
+                Lo[sdi] = b;
+            }
+            double LX[3];
+            if (!line_plane(Ln, Lo, Pp[0], Pp[1], Pp[2], LX)) continue;
+            if (point_internal(c, f, LX[0], LX[1])) {
+                c.cella[i] = -1;
+                marked += 1;
+            }
+        }
+    }
+    c.n_outgone += marked;
+    return marked;
 }
 
 // time_step_loop.f90:338-360: SASPH_continuity.f90:89-169 with
@@ -1544,6 +1652,8 @@ void or_branch_counts(void* ctx, int64_t* out6) {
     out6[5] = c.n_edge.load();
 }
 
+long long or_outgone_count(void* ctx) { return ((Ctx*)ctx)->n_outgone; }
+
 // EoS_fluid_loop.f90:59-70, EoS_barotropic_linear.f90:49-55
 int or_eos(void* ctx) {
     Ctx& c = *(Ctx*)ctx;
@@ -1659,6 +1769,8 @@ static int step_rk1(void* ctx) {
     for (int i = 0; i < c.n; ++i) c.dens[i] = c.dens[i] + c.dt * c.dden[i];
     or_eos(ctx);
     or_psmooth(ctx);
+    // time_step_loop.f90:417-423: open faces, then OrdGrid1
+    or_cancel_outgone(ctx);
     or_ordgrid(ctx);
     return 0;
 }
@@ -1682,6 +1794,7 @@ int or_step(void* ctx, int nsteps) {
         or_leapfrog_mom(ctx, dt_prev);
         or_vsmooth(ctx);
         or_leapfrog_traj(ctx);
+        or_cancel_outgone(ctx);  // time_step_loop.f90:265-269
         or_ordgrid(ctx);
         or_calcvarlength(ctx);
         rc = or_boundary_tab(ctx);

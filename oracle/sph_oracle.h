@@ -93,6 +93,8 @@ int or_momentum(void* ctx);         /* RHS_momentum_equation */
 int or_leapfrog_mom(void* ctx, double dt_previous);
 int or_vsmooth(void* ctx);          /* velocity_smoothing + velocity_smoothing_2 */
 int or_leapfrog_traj(void* ctx);
+int or_cancel_outgone(void* ctx);  /* CancelOutgoneParticles_3D: marks crossings of open faces */
+long long or_outgone_count(void* ctx); /* particles marked by it since or_create */
 int or_continuity(void* ctx);       /* SASPH_continuity + Continuity_Equation loop */
 int or_leapfrog_cont(void* ctx);
 int or_eos(void* ctx);

@@ -60,6 +60,9 @@ class DamBreak3D:
     # whose upstream, top and downstream faces join the frontier zone (two
     # convex edges, FindBoundaryConvexEdges3D.f90); None: no block
     obstacle: tuple | None = None
+    # True: the tank's downstream wall (x = tank length) is an open face
+    # (zone 3, type "open"): particles leaving through it are removed
+    outlet: bool = False
     name: str = "dam_break_3D"
     # monitoring lines of the deck's "control lines" section:
     # (label, first point (x,y,z), second point, number of points)
@@ -123,6 +126,8 @@ class DamBreak3D:
                 f.append((o + 7, o + 6, o + 5, o + 8))
             return f
         rows = [fc + (1,) for fc in box_faces(0, False)]
+        if self.outlet:  # the x = tank length wall (4th face) opens
+            rows[3] = rows[3][:4] + (3,)
         rows += [fc + (2,) for fc in box_faces(8, True)]
         if self.obstacle is not None:
             # the block's upstream (normal -x), top (+z) and downstream (+x) faces
@@ -132,7 +137,10 @@ class DamBreak3D:
         return rows
 
     def zone_types(self):
-        return {1: "fixe", 2: "peri"}
+        z = {1: "fixe", 2: "peri"}
+        if self.outlet:
+            z[3] = "open"
+        return z
 
     def fluid_zone_box(self):
         """Lower and upper corner of the perimeter zone the lattice fills

@@ -79,6 +79,7 @@ struct sph_ctx {
     double *dden = nullptr, *vpres = nullptr;
     // explicit RK1: dden and vpres in the order after the step's closing sort
     double *dden2 = nullptr, *vpres2 = nullptr;
+    int* gone = nullptr;  // 1: the last trajectory step left through an open face
     int *cell = nullptr, *key = nullptr, *slot = nullptr, *cnt = nullptr, *start = nullptr,
         *perm = nullptr, *pid = nullptr, *bsum = nullptr;
     int *work = nullptr, *nwork = nullptr;  // (particle, face) pairs of the boundary table
@@ -217,7 +218,7 @@ int run_sort(sph_ctx* c) {
         k_fill_int<<<nblk(m), 256, 0, c->stream>>>(c->cnt, 0, m);
         k_fill_strided<<<nblk(ns, 64), 64, 0, c->stream>>>(c->iscal + IS_NRM, IS_COUNT, 0, ns);
         SEGL(k_cell_count, nblk(cap), 256, c->xm[1], c->vel[2], c->key, c->slot, c->cnt, c->idslot, c->rmids,
-             c->iscal);
+             c->iscal, c->gone);
         SEGL(k_renumber, nblk(ns, 64), 64, c->rmids, c->idslot, c->vel[2], c->iscal);
     }
     {
@@ -328,7 +329,7 @@ int run_vsmooth(sph_ctx* c) {
     KTime t(c, "velocity_smoothing");
     PAIRL(k_vsmooth, c->chunk_v, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[1], c->rp[cur_rp(c)], XF(c, 1),
           VF(c, 1), c->nbr, c->nnb, c->vel[2], c->xm[1], c->scal, c->dtmin, c->iscal, c->dden, c->rp[2],
-          c->rpf[2], c->xrf);
+          c->rpf[2], c->xrf, c->gone);
     return 0;
 }
 
@@ -393,7 +394,7 @@ sph_ctx::~sph_ctx() {
     F(xmf); F(xrf);
     for (auto p : rec) F(p);
     for (auto p : rpf) F(p);
-    F(dden); F(vpres); F(dden2); F(vpres2); F(rowblk); F(idslot); F(rmids); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
+    F(dden); F(vpres); F(dden2); F(vpres2); F(gone); F(rowblk); F(idslot); F(rmids); F(cell); F(key); F(slot); F(cnt); F(start); F(perm); F(pid); F(bsum);
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     F(mon_icont); F(mon_pts); F(mon_out);
@@ -419,6 +420,7 @@ int sph_ctx::alloc() {
     HIPC(da(&vpres, N));
     HIPC(da(&dden2, P.rk1 ? N : 1));
     HIPC(da(&vpres2, P.rk1 ? N : 1));
+    HIPC(da(&gone, N));
     HIPC(da(&cell, N));
     HIPC(da(&key, N));
     HIPC(da(&slot, N));
@@ -453,6 +455,7 @@ int sph_ctx::alloc() {
         HIPC(hipMemset(vpres2, 0, sizeof(double) * N));
     }
     HIPC(hipMemset(nnb, 0, sizeof(int) * N));
+    HIPC(hipMemset(gone, 0, sizeof(int) * N));
     HIPC(hipMemset(bn, 0, sizeof(int) * N));
     HIPC(hipMemset(scal, 0, sizeof(double) * SC_COUNT * ns));
     HIPC(hipMemset(iscal, 0, sizeof(int) * IS_COUNT * ns));
@@ -527,6 +530,8 @@ int fill_dp(const sph_params* p, const sph_faces* f, DP& P, bool dry = false) {
     P.rk1 = (p->options & SPH_OPT_RK1) ? 1 : 0;
     P.nf = f ? f->nf : 0;
     P.ne = ne;
+    P.nopen = 0;  // open faces (CancelOutgoneParticles_3D)
+    for (int q = 0; p->semi && f && q < f->nf; ++q) P.nopen += f->btype[q] == 2;
     P.dim = 3;
     for (int d = 0; d < 3; ++d) {
         P.ncd[d] = p->ncd[d];

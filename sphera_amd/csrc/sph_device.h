@@ -74,6 +74,7 @@ struct DP {
     int foff;  // first frontier face of the segment in the context's face array
     int eoff;  // first convex edge of the segment in the context's edge array
     int ne;    // its convex edges (NumBEdges)
+    int nopen; // its open faces (NumOpenFaces)
 };
 
 // Segment maps of a segmented context, passed by value to the kernels that

@@ -207,7 +207,8 @@ __global__ __launch_bounds__(256) void k_cell_count(DP P0, Seg S, const double4*
                                                     const double4* __restrict__ vel,
                                                     int* __restrict__ key, int* __restrict__ slot,
                                                     int* __restrict__ cnt, int* __restrict__ idslot,
-                                                    int* __restrict__ rmids, int* iscal) {
+                                                    int* __restrict__ rmids, int* iscal,
+                                                    int* __restrict__ gone) {
     __shared__ int base[HIST_W];
     __shared__ int wcnt[4][HIST_W];  // particles of each window cell per wave
     __shared__ int cmin;
@@ -227,7 +228,10 @@ __global__ __launch_bounds__(256) void k_cell_count(DP P0, Seg S, const double4*
         const int c = d_particle_cell(P, x.x, x.y, x.z);
         const int id = (int)vel[i].w;
         idslot[P.base + id] = i;
-        if (c <= 0) {  // out of the grid: removed (OrdGrid1.f90:71-82)
+        // the last trajectory step left through an open face: cella = -1 (read once)
+        const bool out = gone[i] != 0;
+        if (out) gone[i] = 0;
+        if (c <= 0 || out) {  // out of the grid or cella <= 0: removed (OrdGrid1.f90:66-82)
             k = P.koff + P.nmax;
             atomicAdd(&iscal[IS_REMOVED], 1);
             const int q = atomicAdd(&iscal[IS_NRM], 1);
@@ -1990,6 +1994,91 @@ __device__ __forceinline__ void wall_function(double u_t, double d50, double r, 
     }
 }
 
+// line_plane_intersection.f90:39-76 with three_plane_intersection.f90:42-66
+// and Matrix_Inversion_3x3.f90:39-58, FP64: the point where the line through
+// P1 and P2 meets the plane through A, B and C (two planes through the line,
+// each with a point 999 away from P1, and the plane); false when the 3x3
+// system's |determinant| is below 1e-36.
+__device__ bool d_line_plane(const double P1[3], const double P2[3], const double A[3], const double B[3],
+                             const double C[3], double X[3]) {
+    const double Q1[3] = {P1[0] - 999.0, P1[1], P1[2]};
+    const double Q2[3] = {P1[0], P1[1] - 999.0, P2[2] == P2[2] ? P1[2] - 999.0 : P1[2]};
+    double m[3][3], dv[3];
+    for (int r = 0; r < 2; ++r) {
+        const double* Q = r == 0 ? Q1 : Q2;
+        m[r][0] = (P2[1] - P1[1]) * (Q[2] - P1[2]) - (Q[1] - P1[1]) * (P2[2] - P1[2]);
+        m[r][1] = -((P2[0] - P1[0]) * (Q[2] - P1[2]) - (Q[0] - P1[0]) * (P2[2] - P1[2]));
+        m[r][2] = (P2[0] - P1[0]) * (Q[1] - P1[1]) - (Q[0] - P1[0]) * (P2[1] - P1[1]);
+        dv[r] = -(m[r][0] * P1[0] + m[r][1] * P1[1] + m[r][2] * P1[2]);
+    }
+    m[2][0] = (B[1] - A[1]) * (C[2] - A[2]) - (C[1] - A[1]) * (B[2] - A[2]);
+    m[2][1] = -((B[0] - A[0]) * (C[2] - A[2]) - (C[0] - A[0]) * (B[2] - A[2]));
+    m[2][2] = (B[0] - A[0]) * (C[1] - A[1]) - (C[0] - A[0]) * (B[1] - A[1]);
+    dv[2] = -(m[2][0] * A[0] + m[2][1] * A[1] + m[2][2] * A[2]);
+    const double det = m[0][0] * m[1][1] * m[2][2] + m[1][0] * m[2][1] * m[0][2] +
+                       m[2][0] * m[0][1] * m[1][2] - m[0][0] * m[2][1] * m[1][2] -
+                       m[2][0] * m[1][1] * m[0][2] - m[1][0] * m[0][1] * m[2][2];
+    if (!(fabs(det) >= 1.0e-36)) return false;
+    double inv[3][3];
+    inv[0][0] = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    inv[0][1] = m[0][2] * m[2][1] - m[0][1] * m[2][2];
+    inv[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+    inv[1][0] = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    inv[1][1] = m[0][0] * m[2][2] - m[0][2] * m[2][0];
+    inv[1][2] = m[0][2] * m[1][0] - m[0][0] * m[1][2];
+    inv[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    inv[2][1] = m[0][1] * m[2][0] - m[0][0] * m[2][1];
+    inv[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) inv[i][j] = inv[i][j] / det;
+    const double b[3] = {-dv[0], -dv[1], -dv[2]};
+    for (int i = 0; i < 3; ++i) X[i] = inv[i][0] * b[0] + inv[i][1] * b[1] + inv[i][2] * b[2];
+    return true;
+}
+
+// CancelOutgoneParticles_3D.f90:67-158 for one open face, FP64: the step from
+// xo to xn ends on the face's outer side, started on its inner side, and
+// the segment crosses the face's plane inside the face.
+__device__ bool open_face_crossed(const DFace& F, const double xo[3], const double xn[3]) {
+    const int nodes = F.nodes;
+    const double* G1 = &F.GX[0];
+    const double* G2 = &F.GX[3];
+    const double* G3 = &F.GX[6];
+    const double* Gn = &F.GX[3 * (nodes - 1)];
+    const double dx21 = G2[0] - G1[0], dx31 = G3[0] - G1[0];
+    const double dy21 = G2[1] - G1[1], dy31 = G3[1] - G1[1];
+    const double dz21 = G2[2] - G1[2], dz31 = G3[2] - G1[2];
+    auto det_of = [&](const double* X) {
+        const double dx = X[0] - G1[0], dy = X[1] - G1[1], dz = X[2] - G1[2];
+        return dx21 * dy31 * dz + dx31 * dy * dz21 + dx * dy21 * dz31 - dx21 * dy * dz31 - dx31 * dy21 * dz -
+               dx * dy31 * dz21;
+    };
+    const double dn = det_of(xn);
+    if (!(dn <= 0.0)) return false;
+    const double dpo = det_of(xo);
+    if (copysign(1.0, dn) == copysign(1.0, dpo)) return false;
+    // local coordinates (T(:,i) . (X - node(nodes))) of the face's nodes 1-3 and of both ends
+    double Pp[3][3], Ln[3], Lo[3];
+    const double* Gk[3] = {G1, G2, G3};
+    for (int sdi = 0; sdi < 3; ++sdi) {
+        for (int k = 0; k < 3; ++k) {
+            double a = 0.0;
+            for (int sdj = 0; sdj < 3; ++sdj) a = a + F.T[3 * sdi + sdj] * (Gk[k][sdj] - Gn[sdj]);
+            Pp[k][sdi] = a;
+        }
+        double a = 0.0, b = 0.0;
+        for (int sdj = 0; sdj < 3; ++sdj) {
+            a = a + F.T[3 * sdi + sdj] * (xn[sdj] - Gn[sdj]);
+            b = b + F.T[3 * sdi + sdj] * (xo[sdj] - Gn[sdj]);
+        }
+        Ln[sdi] = a;
+        Lo[sdi] = b;
+    }
+    double LX[3];
+    if (!d_line_plane(Ln, Lo, Pp[0], Pp[1], Pp[2], LX)) return false;
+    return d_point_internal(pic_of(F), LX[0], LX[1]);
+}
+
 // SA-SPH terms of the momentum equation (AddBoundaryContributions_to_ME3D.f90:
 // 118-336), FP64; tp/td/tv start at zero
 __device__ __forceinline__ void sa_momentum(const DP& P, const DFace* __restrict__ faces,
@@ -2400,7 +2489,8 @@ __global__ __launch_bounds__(256) void k_vsmooth(
     F4In velf1, const uint32_t* __restrict__ nbr, const int* __restrict__ nnb,
     double4* __restrict__ vel_out, double4* __restrict__ xm_out,
     const double* scal, unsigned long long* dtmin, int* iscal, const double* __restrict__ dden_in,
-    double2* __restrict__ rp_out, float2* __restrict__ rpf_out, float4* __restrict__ xrf_out) {
+    double2* __restrict__ rp_out, float2* __restrict__ rpf_out, float4* __restrict__ xrf_out,
+    int* __restrict__ gone) {
     using X = typename Rec<F32>::X;
     using V = typename Rec<F32>::V;
     using RT = typename std::conditional<F32, float, double>::type;
@@ -2530,7 +2620,20 @@ __global__ __launch_bounds__(256) void k_vsmooth(
             if (smooth) vn = make_double4(vi.x + tv1 * v[0], vi.y + tv1 * v[1], vi.z + tv1 * v[2], vi.w);
             vel_out[i] = vn;
 
-            xm_out[i] = make_double4(xi.x + dt * vn.x, xi.y + dt * vn.y, xi.z + dt * vn.z, xi.w);
+            const double xn[3] = {xi.x + dt * vn.x, xi.y + dt * vn.y, xi.z + dt * vn.z};
+            xm_out[i] = make_double4(xn[0], xn[1], xn[2], xi.w);
+            // the next time_step_duration runs over the particles the coming
+            // OrdGrid1 keeps: not those out of the grid, nor those that left
+            // through an open face
+            bool stays = d_particle_cell(P, xn[0], xn[1], xn[2]) > 0;
+            if (P.nopen > 0) {  // CancelOutgoneParticles_3D on the step just made
+                const double xo[3] = {xi.x, xi.y, xi.z};
+                int g = 0;
+                for (int f = P.foff; f < P.foff + P.nf; ++f)
+                    if (faces[f].btype == 2 && open_face_crossed(faces[f], xo, xn)) g = 1;
+                gone[i] = g;
+                stays = stays && !g;
+            }
             if (P.rk1) {
                 // explicit RK1: Euler.f90:231-241 (dens = dens + dt dden) and
                 // EoS_fluid_loop.f90:59-70 (time_integration.f90:52-65); the
@@ -2546,9 +2649,11 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                     xrf_out[i] = make_float4(xo.x, xo.y, xo.z, (float)(rho1 - P.den0));
                 }
             }
-            const unsigned long long b =
-                (unsigned long long)__double_as_longlong(d_dt_cfl(P, vn.x, vn.y, vn.z));
-            dtbits = b < dtbits ? b : dtbits;
+            if (stays) {
+                const unsigned long long b =
+                    (unsigned long long)__double_as_longlong(d_dt_cfl(P, vn.x, vn.y, vn.z));
+                dtbits = b < dtbits ? b : dtbits;
+            }
         }
         __syncthreads();
     }

@@ -57,6 +57,7 @@ def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
               "Boundary", "1", "fixed", *_fixed_zone(case), "111111", "1 8",
               "Reservoir", "2", "perimeter", "1", "uniform 009EA8",
               "std 0. 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[2])} 0.", "1 0 .false.",
+              *(["Outlet", "3", "open", "FF0000"] if getattr(case, "outlet", False) else []),
               "##### END BOUNDARIES #####",
               "##### DBSPH #####", "1. .false. 0.5 0 .false.", ".false. .true.", "0 0",
               "0 .false.", "0 0 6", "##### END DBSPH #####",
@@ -233,6 +234,11 @@ class DeckCase:
     dim: int = 3
 
     @property
+    def outlet(self):
+        """True when an open zone holds frontier faces."""
+        return "open" in self.zones.values()
+
+    @property
     def h(self):
         return self.dx * self.trunc
 
@@ -332,6 +338,9 @@ def parse_deck(text, name=None):
             zones[zone] = "peri"
             fluid_zone = zone
             k += 8
+        elif kind == "open":  # name, zone, type, colour (ReadInputBoundaries.f90:172-182)
+            zones[zone] = "open"
+            k += 4
         else:
             raise ValueError(f"boundary type {kind!r} is not read")
     if fluid_zone is None:

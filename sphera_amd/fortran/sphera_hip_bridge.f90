@@ -19,8 +19,9 @@
 !                           .zlft records (calc_pelo.f90:132-133)
 !
 ! Scope the device implements (checked by sph_hip_open): Domain%tipo="semi",
-! leapfrog (time_split=1), one medium, no NormFix, no open/source faces, no
-! dt_alfa_Mon, fluid particles only.
+! leapfrog (time_split=1) or explicit RK1, one medium, no NormFix, no source
+! faces, open faces of type "open" only (3D), no dt_alfa_Mon, fluid
+! particles only.
 module sphera_hip_bridge
 use iso_c_binding
 use sphera_hip_c
@@ -58,6 +59,18 @@ write(*,'(a,a,a,i5,a,a)') "sphera_hip: ",where," failed, status ",rc,": ",      
    sph_error_message()
 stop 1
 end subroutine sph_hip_check
+
+#ifdef SPACE_3D
+! True when a face of OpenFace (Gest_Input.f90:617-629) is of type "leve",
+! "crit", "velo" or "flow": only "open" ones are on the device.
+logical function other_open_faces()
+integer(4) :: k
+other_open_faces = .false.
+do k=1,NumOpenFaces
+   if (Tratto(BoundaryFace(OpenFace(k))%stretch)%tipo/="open") other_open_faces = .true.
+enddo
+end function other_open_faces
+#endif
 
 integer(c_int32_t) function btype_code(tipo)
 character(len=4),intent(in) :: tipo
@@ -319,11 +332,11 @@ if ((Domain%tipo/="semi").or.(Domain%time_split<0).or.(Domain%time_split>1).or. 
    dt_alfa_Mon) then
 #else
 if ((Domain%tipo/="semi").or.(Domain%time_split<0).or.(Domain%time_split>1).or.    &
-   (NMedium/=1).or.Domain%NormFix.or.(NumOpenFaces>0).or.(SourceFace/=0).or.       &
-   dt_alfa_Mon) then
+   (NMedium/=1).or.Domain%NormFix.or.(SourceFace/=0).or.dt_alfa_Mon.or.            &
+   other_open_faces()) then
 #endif
    write(*,*) "sphera_hip: case outside the device scope (semi, leapfrog or ",     &
-      "explicit RK, one medium, fixed/tapis frontiers)"
+      "explicit RK, one medium, fixed/tapis/open frontiers)"
    stop 1
 endif
 call pack_params(prec,p)

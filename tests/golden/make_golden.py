@@ -6,7 +6,7 @@ from /root/reference/src) on project-authored decks.
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
 With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp,
-rk1, viscous, wallfn, wallfn_2d, edges) only those fixtures are written.
+rk1, viscous, wallfn, wallfn_2d, edges, outlet) only those fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -60,6 +60,20 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         reactions run; state after 0, 4 and 8 steps (one
                         OpenMP thread: the reference's edge order is the
                         threads' arrival order)
+  ref_outlet.npz        800 particles; the tank's x = 0.3 m wall is an open
+                        face (zone "open"); the jittered column shifted to
+                        0.224 < x < 0.3 m and moving towards it at
+                        U(2, 4) m/s (seed 20261205), the 10 top particles
+                        next to it lifted above the wall's top edge, five
+                        particles just below the grid's top moving up:
+                        particles crossing the face are removed
+                        (CancelOutgoneParticles_3D, then
+                        OrdGrid1's swap-with-last), those crossing its
+                        plane above it stay, those leaving the grid are
+                        removed; state (with the reference's
+                        particle order) after 0, 8 and 16 steps,
+                        leapfrog, and the same with explicit RK1
+                        (ref_outlet_rk1.npz)
   ref_rk1.npz           1440 particles, jittered lattice (seed 20260101), the
                         explicit RK1 branch (time_split = 0, RKscheme = 1:
                         Euler); state after 0, 6 and 12 steps
@@ -350,6 +364,43 @@ def make_edges():
     np.savez_compressed(HERE / "ref_edges.npz", **out)
 
 
+def outlet_ic(coord, dx, seed=20261205):
+    """The jittered column shifted by 0.224 m towards the open x = 0.3 m wall,
+    moving into it at U(2, 4) m/s, with U(-0.5, 0.5) m/s vertically."""
+    rng = np.random.default_rng(seed)
+    c = jitter(coord, dx)
+    c[:, 0] += 0.224
+    v = np.zeros_like(c)
+    v[:, 0] = rng.uniform(2.0, 4.0, len(c))
+    v[:, 2] = rng.uniform(-0.5, 0.5, len(c))
+    # the top particles of the layer next to the wall lifted above the tank
+    # (0.205-0.215 m, the open face ends at 0.2 m): they cross the face's
+    # plane outside the face and stay
+    zt = c[:, 2].max() - 0.5 * dx
+    top = np.flatnonzero((c[:, 0] > c[:, 0].max() - 0.5 * dx) & (c[:, 2] > zt))
+    up = np.flatnonzero((c[:, 0] < c[:, 0].min() + 0.5 * dx) & (c[:, 2] > zt))[:5]
+    c[top, 2] = rng.uniform(0.205, 0.215, len(top))
+    v[top, 2] = 0.0
+    # five particles of the column's top layer, at its upstream end, put just
+    # below the grid's top (z = 0.2261 m) and moving up: they leave the grid
+    # during the run (OrdGrid1 removes them; the next dt excludes them)
+    c[up, 2] = rng.uniform(0.2250, 0.2258, len(up))
+    v[up] = [0.0, 0.0, 3.0]
+    return c, v
+
+
+def make_outlet():
+    """an open face: particles leaving through it are removed"""
+    for ts, name in ((1, "ref_outlet.npz"), (0, "ref_outlet_rk1.npz")):
+        case = DamBreak3D(**TINY, outlet=True, time_split=ts)
+        tb = build(case)
+        ic, iv = outlet_ic(tb.particles["coord"], case.dx)
+        hdr, steps = read_dump(run_harness(case, nsteps=16, every=8, lists=False, ic_coord=ic, ic_vel=iv))
+        out = {"ic_coord": ic, "ic_vel": iv}
+        _pack_steps(steps, out, lists=False)
+        np.savez_compressed(HERE / name, **out)
+
+
 WALLFN = dict(slip_mode=2, roughness=(0.0005, 0.0015))
 
 
@@ -424,7 +475,7 @@ MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_
           "pelo": make_pelo, "2d": make_2d, "nearwall": make_nearwall,
           "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp, "rk1": make_rk1,
           "viscous": make_viscous, "wallfn": make_wallfn, "wallfn_2d": make_wallfn_2d,
-          "edges": make_edges}
+          "edges": make_edges, "outlet": make_outlet}
 
 
 def main():

@@ -112,24 +112,29 @@ def test_bridge_2d_fp64_matches_reference(tmp_path):
     assert abs(st[10]["dt"] - float(g["s10_dt"])) <= 1e-12 * float(g["s10_dt"])
 
 
-def test_bridge_convex_edges_and_wall_function(tmp_path):
+def test_bridge_convex_edges_wall_function_open_face(tmp_path):
     """Decks with a block on the floor (two convex edges, passed as the
-    reference's BoundaryConvexEdge table) and with slip_coefficient_mode = 2:
-    the device run through the bridge stays within 1e-12 of the reference's
-    fixtures (the wall reactions' and the wall function's log() are the device
-    library's).  One OpenMP thread: the reference orders the convex edges by
-    thread arrival."""
+    reference's BoundaryConvexEdge table), with slip_coefficient_mode = 2 and
+    with an open face (the particles leaving through it are removed, in the
+    reference's order): the device run through the bridge keeps the
+    reference's particle count and stays within 1e-12 of its fixtures (the
+    wall reactions' and the wall function's log() are the device library's).
+    One OpenMP thread: the reference orders the convex edges by thread
+    arrival."""
     from tests.golden.make_golden import OBSTACLE, TINY, WALLFN
-    for name, case in (("ref_edges.npz", DamBreak3D(**TINY, obstacle=OBSTACLE)),
-                       ("ref_wallfn.npz", DamBreak3D(**TINY, **WALLFN))):
+    for name, case, its in (("ref_edges.npz", DamBreak3D(**TINY, obstacle=OBSTACLE), (4, 8)),
+                            ("ref_wallfn.npz", DamBreak3D(**TINY, **WALLFN), (4, 8)),
+                            ("ref_outlet.npz", DamBreak3D(**TINY, outlet=True), (8, 16))):
         g = np.load(GOLD / name)
-        print(f"[bridge] {name} steps=8", flush=True)
-        hdr, steps = read_dump(run_harness(case, nsteps=8, every=4, lists=False, ic_coord=g["ic_coord"],
-                                           ic_vel=g["ic_vel"], workdir=tmp_path / name[:-4],
-                                           exe=HARNESS_HIP, timeout=120, threads=1,
+        print(f"[bridge] {name} steps={its[1]}", flush=True)
+        hdr, steps = read_dump(run_harness(case, nsteps=its[1], every=its[0], lists=False,
+                                           ic_coord=g["ic_coord"], ic_vel=g["ic_vel"],
+                                           workdir=tmp_path / name[:-4], exe=HARNESS_HIP, timeout=120,
+                                           threads=1,
                                            extra_env={"SPH_HIP_PRECISION": "0", "SPH_HIP_PHASES": "0"}))
         st = {r["it"]: r for r in steps}
-        for it in (4, 8):
+        for it in its:
+            assert st[it]["nag"] == g[f"s{it}_nag"], (name, it)
             for f in FIELDS:
                 ref = g[f"s{it}_" + f]
                 err = np.abs(st[it][f] - ref).max() / max(np.abs(ref).max(), 1e-300)

@@ -479,6 +479,36 @@ def test_convex_edges_fixture():
     f32.close()
 
 
+@pytest.mark.parametrize("ts,name", [(1, "ref_outlet.npz"), (0, "ref_outlet_rk1.npz")])
+def test_open_face_fixture(ts, name):
+    """An open face (tests/golden/ref_outlet*.npz): the particles whose step
+    crosses it are removed in the reference's order.  Device FP64 equal to
+    the reference (count, order, every field) after 8 and 16 steps, leapfrog
+    and explicit RK1; FP32-mixed removes the same particles and is within
+    FP32_TOL."""
+    rt = _rt()
+    g = np.load(GOLD / name)
+    tb = build(DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny",
+                          outlet=True, time_split=ts))
+    print(f"[gpu] open face {name} n={len(g['ic_coord'])}", flush=True)
+    d, f32 = _gold_device(tb, g, rt.FP64), _gold_device(tb, g, rt.FP32_MIXED)
+    for it in (8, 16):
+        d.step(8)
+        f32.step(8)
+        ds, fs = d.state(), f32.state()
+        assert ds["nag"] == fs["nag"] == g[f"s{it}_nag"], (ds["nag"], fs["nag"])
+        assert ds["dt"] == g[f"s{it}_dt"]
+        for f in FIELDS:
+            assert np.array_equal(ds[f], g[f"s{it}_" + f]), (it, f)
+        e32 = {f: _rel(fs[f], g[f"s{it}_" + f]) for f in FIELDS}
+        print(f"[gpu] open face it={it} n={ds['nag']} fp64 bit-identical; fp32 "
+              f"{{{', '.join(f'{k}: {v:.1e}' for k, v in e32.items())}}}", flush=True)
+        for f in FIELDS:
+            assert e32[f] <= FP32_TOL[f], (it, f, e32[f])
+    d.close()
+    f32.close()
+
+
 def test_density_clamp_fixture():
     """density_thresholds = 1 (Leapfrog_continuity.f90:61-68) on the
     reference's fixture (tests/golden/ref_dclamp.npz: converging and

@@ -25,7 +25,7 @@ TUTORIAL = Path("/root/reference/input/tutorials/release_v_10_0_0/42_dam_break_3
 
 @pytest.mark.parametrize("kw", [dict(), dict(density_thresholds=1), dict(time_split=0),
                                 dict(slip_mode=2, roughness=(0.0005, 0.0015)), dict(slip_mode=0),
-                                dict(obstacle=(0.15, 0.2, 0.04)),
+                                dict(obstacle=(0.15, 0.2, 0.04)), dict(outlet=True),
                                 dict(tank=(0.5, 0.3, 0.3), column=(0.2, 0.3, 0.15), dx=0.012)])
 def test_deck_round_trip_builds_the_same_tables(kw):
     args = dict(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="rt",

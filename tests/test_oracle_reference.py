@@ -418,3 +418,27 @@ def test_golden_convex_edges_bit_identical():
     _oracle_steps(o, g, (0, 4, 8))
     n = o.branch_counts()
     assert n["convex_edge"] >= 15 and n["elastic"] > 20, n
+
+
+@pytest.mark.parametrize("ts,name", [(1, "ref_outlet.npz"), (0, "ref_outlet_rk1.npz")])
+def test_golden_open_face_bit_identical(ts, name):
+    """The tank's x = 0.3 m wall is an open face: the particles whose step
+    crosses it are removed (CancelOutgoneParticles_3D marks them, OrdGrid1
+    removes them by swap-with-last), the ten that cross its plane above the
+    wall stay, and five leave the grid through its top during the run;
+    count, particle order, dt and every field equal the reference's after 8
+    and 16 steps, leapfrog and explicit RK1.  Without the open face the same
+    input keeps all but those five."""
+    g = np.load(GOLD / name)
+    case = _tiny()
+    case.outlet, case.time_split = True, ts
+    o = Oracle(build(case), coord=g["ic_coord"], vel=g["ic_vel"])
+    o.preloop()
+    _oracle_steps(o, g, (0, 8, 16))
+    assert o.outgone_count() == 90 and int(g["s16_nag"]) == 705  # and 5 left the grid
+    assert int((g["s16_coord"][:, 0] > 0.3).sum()) == 10
+    case.outlet = False
+    o0 = Oracle(build(case), coord=g["ic_coord"], vel=g["ic_vel"])
+    o0.preloop()
+    o0.step(16)
+    assert o0.state()["nag"] == 795 and o0.outgone_count() == 0
