@@ -90,8 +90,10 @@ enum {
  * (BoundaryFace, Tratto, Partz, GCBFPointers, GCBFVector, kerneltab,
  * BoundIntegralTab).  T is column-major per face (T(r,c) at [9f+3c+r]);
  * LX: 4 nodes x (x,y); GX: 4 nodes x (x,y,z); btype: 0 fixe, 1 tapi, 2 open,
- * 3 sour, 4 velo, 5 flow.  gcbf_start is 1-based into gcbf_vec, which holds
- * 0-based face indices. */
+ * 3 sour, 4 velo, 5 flow.  A particle whose trajectory step crosses an open
+ * face is removed at the next sort (CancelOutgoneParticles_3D.f90, then
+ * OrdGrid1's swap-with-last).  gcbf_start is 1-based into gcbf_vec, which
+ * holds 0-based face indices. */
 typedef struct sph_faces {
     int32_t nf;
     const double *T, *LX, *GX, *area, *velocity, *tvel, *slip_coeff;
