@@ -133,12 +133,18 @@ int sph_ctx_create(const sph_params* p, const sph_faces* f, sph_ctx** out);
  * AddBoundaryContributions_to_ME2D, AddElasticBoundaryReaction_2D,
  * velocity_smoothing_SA_SPH_2D, AddBoundaryContribution_to_CE2D and
  * PressureSmoothing_2D.  Replaces Gest_Input.f90:292-293 (side geometry
- * computed by the caller). */
+ * computed by the caller).  v2 is the second vertex (x, 0, z) of each side:
+ * read for open sides (btype 2), whose particles the next sort removes when
+ * their step crosses the side (CancelOutgoneParticles_2D.f90:60-104,
+ * OpenSide of Gest_Input.f90:600-613); it may be NULL when no side is open
+ * (status 2 otherwise).  Open sides enter the boundary table as
+ * SelectCloseBoundarySides2D.f90:99-107 selects them and add no wall term. */
 typedef struct sph_sides2d {
     int32_t ns;
     const double *T, *RN, *v1, *length, *angle, *velocity, *tvel, *slip_coeff;
     const int32_t *btype, *previous, *slip_mode, *lam_check;
     const double *bit2d;
+    const double *v2;
 } sph_sides2d;
 int sph_ctx_create2d(const sph_params* p, const sph_sides2d* s, sph_ctx** out);
 int sph_ctx_destroy(sph_ctx* ctx);

@@ -55,7 +55,7 @@ class OrSides2D(C.Structure):
     _fields_ = [("ns", C.c_int32), ("T", PD), ("RN", PD), ("v1", PD), ("length", PD),
                 ("angle", PD), ("velocity", PD), ("tvel", PD), ("slip_coeff", PD),
                 ("btype", PI), ("previous", PI), ("slip_mode", PI), ("lam_check", PI),
-                ("bit2d", PD)]
+                ("bit2d", PD), ("v2", PD)]
 
 
 def lib():
@@ -150,8 +150,8 @@ class Oracle:
             self._keep = dict(
                 T=np.ascontiguousarray(S.T.transpose(0, 2, 1).reshape(-1)),
                 RN=np.ascontiguousarray(S.RN.transpose(0, 2, 1).reshape(-1)),
-                v1=np.ascontiguousarray(S.v1.reshape(-1)), L=np.ascontiguousarray(S.length),
-                ang=np.ascontiguousarray(S.angle), vel=np.ascontiguousarray(S.velocity.reshape(-1)),
+                v1=np.ascontiguousarray(S.v1.reshape(-1)), v2=np.ascontiguousarray(S.v2.reshape(-1)),
+                L=np.ascontiguousarray(S.length), ang=np.ascontiguousarray(S.angle), vel=np.ascontiguousarray(S.velocity.reshape(-1)),
                 tvel=np.ascontiguousarray(S.tratto_velocity.reshape(-1)),
                 slip=np.ascontiguousarray(S.slip_coeff), bt=np.ascontiguousarray(S.btype, np.int32),
                 prev=np.ascontiguousarray(S.previous, np.int32),
@@ -162,9 +162,11 @@ class Oracle:
             sd = OrSides2D(len(S.length), _pd(k["T"]), _pd(k["RN"]), _pd(k["v1"]), _pd(k["L"]),
                            _pd(k["ang"]), _pd(k["vel"]), _pd(k["tvel"]), _pd(k["slip"]),
                            _pi(k["bt"]), _pi(k["prev"]), _pi(k["mode"]), _pi(k["lam"]),
-                           _pd(k["bit"]))
+                           _pd(k["bit"]), _pd(k["v2"]))
             self.ctx = L.or_create2d(C.byref(self.p), C.byref(sd), _pd(self.coord), _pd(vel),
                                      _pd(dens), _pd(pres), _pd(mass))
+            if not self.ctx:
+                raise ValueError("or_create2d refused the sides (an open side without v2)")
             return
         F = tb.faces
         self._keep = dict(

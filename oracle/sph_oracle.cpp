@@ -51,7 +51,7 @@ struct Ctx {
     // 2D (x-z plane): boundary sides instead of faces
     int dim = 3;
     int ns = 0;
-    std::vector<double> sT, sRN, sv1, slen, sang, stvel, sfvel, sslip;
+    std::vector<double> sT, sRN, sv1, sv2, slen, sang, stvel, sfvel, sslip;
     std::vector<int> sbtype, sprev, sslipmode, slam;
     double bit2d[22][6];
     double dt = 0.0, dt_average = 0.0, time = 0.0;
@@ -536,7 +536,7 @@ void boundary_tab_particle_2d(Ctx& c, int npi) {
     }
     for (int q = 0; q < ncbs; ++q) {
         const int bt = c.sbtype[clo[q]];
-        if (bt != 0 && bt != 1) { c.bn[npi] = -95; return; }
+        if (bt != 0 && bt != 1 && bt != 2) { c.bn[npi] = -95; return; }
     }
     if (ncbs == 2) {
         const int a = clo[0], b = clo[1];
@@ -545,12 +545,15 @@ void boundary_tab_particle_2d(Ctx& c, int npi) {
         else if (c.sprev[b] == a + 1)
             loc[0] = c.slen[a] - loc[0];
     }
-    // SelectCloseBoundarySides2D: fixe/tapi sides with y > 0
+    // SelectCloseBoundarySides2D.f90:66-120: fixe/tapi sides with y > 0; open
+    // sides with y > 0 and 0 < x < side length
     int nint = 0;
     int iside[4];
     double iloc[8];
     for (int q = 0; q < ncbs; ++q) {
-        if (loc[2 * q + 1] > 0.0) {
+        bool keep = loc[2 * q + 1] > 0.0;
+        if (c.sbtype[clo[q]] == 2) keep = keep && loc[2 * q] > 0.0 && loc[2 * q] < c.slen[clo[q]];
+        if (keep) {
             iside[nint] = clo[q];
             iloc[2 * nint] = loc[2 * q];
             iloc[2 * nint + 1] = loc[2 * q + 1];
@@ -681,6 +684,7 @@ void* or_create2d(const or_params* p, const or_sides2d* sd, const double* coord,
         c->sT.assign(sd->T, sd->T + 9 * ns);
         c->sRN.assign(sd->RN, sd->RN + 9 * ns);
         c->sv1.assign(sd->v1, sd->v1 + 3 * ns);
+        if (sd->v2) c->sv2.assign(sd->v2, sd->v2 + 3 * ns);
         c->slen.assign(sd->length, sd->length + ns);
         c->sang.assign(sd->angle, sd->angle + ns);
         c->sfvel.assign(sd->velocity, sd->velocity + 3 * ns);
@@ -691,6 +695,11 @@ void* or_create2d(const or_params* p, const or_sides2d* sd, const double* coord,
         c->sslipmode.assign(sd->slip_mode, sd->slip_mode + ns);
         c->slam.assign(sd->lam_check, sd->lam_check + ns);
         std::memcpy(c->bit2d, sd->bit2d, sizeof(c->bit2d));
+        for (int k = 0; k < ns; ++k)
+            if (sd->btype[k] == 2 && !sd->v2) {  // open sides need their second vertex
+                or_destroy(c);
+                return nullptr;
+            }
     }
     // cells again with j = 1 (ParticleCellNumber.f90:59-61), then OrdGrid1
     for (int i = 0; i < c->n; ++i) c->cella[i] = particle_cell(*c, &c->coord[3 * i]);
@@ -1018,6 +1027,7 @@ int or_momentum(void* ctx) {
             for (int q = 0; q < ncbf; ++q) {
                 const size_t sl = (size_t)npi * mc + q;
                 const int sd = c.bface[sl];
+                if (c.sbtype[sd] == 2) continue;  // AddBoundaryContributions_to_ME2D.f90:124
                 const double xpi = c.bloc[3 * sl], ypi = c.bloc[3 * sl + 1];
                 const double* BI = &c.bbi[8 * sl];
                 const double IntWdS = BI[0], IntWdV = BI[2], IntWd1s0 = BI[5], IntWd3s0 = BI[6];
@@ -1220,6 +1230,7 @@ int or_momentum(void* ctx) {
             for (int q = 0; q < ncbf; ++q) {
                 const size_t sl = (size_t)npi * mc + q;
                 const int sd = c.bface[sl];
+                if (c.sbtype[sd] != 0 && c.sbtype[sd] != 1) continue;
                 const double xpi = c.bloc[3 * sl], ypi = c.bloc[3 * sl + 1];
                 if (ypi < ypimin && xpi > 0.0 && xpi < c.slen[sd]) {
                     double vin = 0.0;
@@ -1334,6 +1345,7 @@ int or_vsmooth(void* ctx) {
                 // velocity_smoothing_SA_SPH_2D.f90:64-118 (fixe/tapi sides)
                 const size_t sl = (size_t)npi * mc + q;
                 const int sd = c.bface[sl];
+                if (c.sbtype[sd] != 0 && c.sbtype[sd] != 1) continue;
                 const double sss[2] = {ST(c, sd, 0, 0), ST(c, sd, 2, 0)};
                 const double nnn[2] = {ST(c, sd, 0, 2), ST(c, sd, 2, 2)};
                 const double DVG[2] = {2.0 * (c.stvel[3 * sd] - vel_i[0]),
@@ -1526,6 +1538,29 @@ int or_cancel_outgone(void* ctx) {
             }
         }
     }
+    // CancelOutgoneParticles_2D.f90:60-104: the same over the open sides (x-z
+    // plane): the step's end lies on the outer side of the side's line
+    // (determinant <= 0) and its start did not, and the side's two vertices
+    // lie on opposite sides of the step's line
+    for (int s = 0; s < c.ns; ++s) {
+        if (c.sbtype[s] != 2) continue;
+        const double o1x = c.sv1[3 * s], o1z = c.sv1[3 * s + 2];
+        const double o2x = c.sv2[3 * s], o2z = c.sv2[3 * s + 2];
+        for (int i = 0; i < c.n; ++i) {
+            if (c.cella[i] == 0) continue;
+            const double nx = c.coord[3 * i], nz = c.coord[3 * i + 2];
+            const double ox = c.coord_old[3 * i], oz = c.coord_old[3 * i + 2];
+            const double dn = (nz - o1z) * (o2x - o1x) - (nx - o1x) * (o2z - o1z);
+            if (!(dn <= 0.0)) continue;
+            const double dold = (oz - o1z) * (o2x - o1x) - (ox - o1x) * (o2z - o1z);
+            if (std::copysign(1.0, dn) == std::copysign(1.0, dold)) continue;
+            const double d1 = (o1z - oz) * (nx - ox) - (o1x - ox) * (nz - oz);
+            const double d2 = (o2z - oz) * (nx - ox) - (o2x - ox) * (nz - oz);
+            if (std::copysign(1.0, d1) == std::copysign(1.0, d2)) continue;
+            c.cella[i] = -1;
+            marked += 1;
+        }
+    }
     c.n_outgone += marked;
     return marked;
 }
@@ -1551,6 +1586,7 @@ int or_continuity(void* ctx) {
             for (int q = 0; q < ncbf; ++q) {
                 const size_t sl = (size_t)npi * mc + q;
                 const int sd = c.bface[sl];
+                if (c.sbtype[sd] != 0 && c.sbtype[sd] != 1) continue;
                 const double IntWdS = c.bbi[8 * sl];
                 double dvel[2];
                 if (c.sbtype[sd] == 0) {
@@ -1692,6 +1728,7 @@ int or_psmooth(void* ctx) {
                 for (int q = 0; q < ncbf; ++q) {
                     const size_t sl = (size_t)npi * mc + q;
                     const int sd = c.bface[sl];
+                    if (c.sbtype[sd] != 0 && c.sbtype[sd] != 1) continue;
                     const double sss[2] = {ST(c, sd, 0, 0), ST(c, sd, 2, 0)};
                     const double nnn[2] = {ST(c, sd, 0, 2), ST(c, sd, 2, 2)};
                     const double mf0 = sss[0] * c.p.grav[0] + sss[1] * c.p.grav[2];

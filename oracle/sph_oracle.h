@@ -58,12 +58,15 @@ void* or_create(const or_params* p, const or_faces* f, const double* coord, cons
  * (6 = peri), previous = previous_side (1-based, 0 = none), velocity =
  * BoundarySide%velocity, tvel = Tratto%velocity, bit2d = BoundIntegralTab2D
  * (22 x 6, row-major).  The particles lie in the x-z plane (y = 0); the
- * grid has ncd[1] = 1; maxclose must be MAXCLOSEBOUNDSIDES (4). */
+ * grid has ncd[1] = 1; maxclose must be MAXCLOSEBOUNDSIDES (4).  v2 = second
+ * vertex (x, 0, z); read for open sides (btype 2) only, may be NULL when no
+ * side is open. */
 typedef struct or_sides2d {
     int32_t ns;
     const double *T, *RN, *v1, *length, *angle, *velocity, *tvel, *slip_coeff;
     const int32_t *btype, *previous, *slip_mode, *lam_check;
     const double *bit2d;
+    const double *v2;
 } or_sides2d;
 
 void* or_create2d(const or_params* p, const or_sides2d* s, const double* coord, const double* vel,

@@ -46,6 +46,7 @@ class DamBreak2D:
     roughness: tuple = (0.0, 0.0)  # slip_coefficient_mode=2 inputs (eps_sigma_z, H_sgr)
     density_thresholds: int = 0      # 1: Leapfrog_continuity's 0.9/1.1 rho0 clamp
     time_split: int = 1           # 1: leapfrog; 0: explicit RK (RKscheme 1, Euler)
+    outlet: bool = False          # the right wall (x = W) an "open" side of zone 3
     name: str = "dam_break_2D"
     control_lines: tuple = ()
     dim: int = 2
@@ -86,11 +87,18 @@ class DamBreak2D:
                          (0.0, ch), (0.0, 0.0), (cw, 0.0), (cw, ch)], dtype=np.float64)
 
     def lines(self):
-        """LINES section: line id (= zone id) -> vertex ids."""
+        """LINES section: line id (= zone id) -> vertex ids.  With an outlet
+        the right wall (vertices 3-4) is line 3, an open side whose particles
+        leave the domain when they cross it (CancelOutgoneParticles_2D.f90)."""
+        if self.outlet:
+            return {1: [1, 2, 3], 2: [5, 6, 7, 8, 5], 3: [3, 4]}
         return {1: [1, 2, 3, 4], 2: [5, 6, 7, 8, 5]}
 
     def zone_types(self):
-        return {1: "fixe", 2: "peri"}
+        z = {1: "fixe", 2: "peri"}
+        if self.outlet:
+            z[3] = "open"
+        return z
 
 
 @dataclass

@@ -918,8 +918,12 @@ int sph_ctx_create2d(const sph_params* p, const sph_sides2d* sd, sph_ctx** out) 
     if (p->semi && (!sd || sd->ns <= 0)) return fail(2, "SA-SPH requested without boundary sides");
     if (int rc = check_options(p->options)) return rc;
     if (p->options & SPH_OPT_CONVEX_EDGES) return fail(2, "convex edges are a 3D frontier feature");
-    for (int k = 0; sd && k < sd->ns; ++k)
+    int nopen = 0;
+    for (int k = 0; sd && k < sd->ns; ++k) {
         if (int rc = check_slip_mode(sd->slip_mode[k], sd->btype[k])) return rc;
+        if (sd->btype[k] == 2) nopen += 1;
+    }
+    if (nopen > 0 && !sd->v2) return fail(2, "open sides need their second vertex (sides.v2)");
     sph_params q = *p;
     q.semi = 0;
     int rc = sph_ctx_create(&q, nullptr, out);
@@ -929,6 +933,7 @@ int sph_ctx_create2d(const sph_params* p, const sph_sides2d* sd, sph_ctx** out) 
     P.dim = 2;
     P.semi = p->semi ? 1 : 0;
     P.nf = sd ? sd->ns : 0;
+    P.nopen = P.semi ? nopen : 0;  // open sides (CancelOutgoneParticles_2D)
     if (P.nf > 0) {
         std::vector<DFace> hs(P.nf);
         for (int k = 0; k < P.nf; ++k) {
@@ -942,6 +947,10 @@ int sph_ctx_create2d(const sph_params* p, const sph_sides2d* sd, sph_ctx** out) 
             S.RN[3] = RN[8];
             S.v1[0] = sd->v1[3 * k];
             S.v1[1] = sd->v1[3 * k + 2];
+            if (sd->btype[k] == 2) {
+                S.v2[0] = sd->v2[3 * k];
+                S.v2[1] = sd->v2[3 * k + 2];
+            }
             S.len = sd->length[k];
             S.angle = sd->angle[k];
             S.prev = sd->previous[k];

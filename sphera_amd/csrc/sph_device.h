@@ -114,10 +114,11 @@ struct DFace {
     // 2D context: the array holds the boundary sides (BoundarySide of
     // DefineBoundarySideGeometry2D.f90) and these fields: RN(1,1), RN(1,3),
     // RN(3,1), RN(3,3) of BoundaryMassForceMatrix2D, the first vertex (x, z),
-    // length, angle to the previous side and previous_side (1-based, 0 none).
+    // length, angle to the previous side and previous_side (1-based, 0 none),
+    // the second vertex (x, z) of an open side (CancelOutgoneParticles_2D).
     // T, vel, tvel, slip, btype, mode and lam keep their meaning.
     double RN[4];
-    double v1[2];
+    double v1[2], v2[2];
     double len, angle;
     int prev;
 };

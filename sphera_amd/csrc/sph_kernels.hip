@@ -1295,7 +1295,7 @@ __global__ __launch_bounds__(256) void k_boundary_2d(DP P, const DFace* __restri
     }
     for (int q = 0; q < ncbs && !bad; ++q) {
         const int b = sides[clo[q]].btype;
-        if (b != 0 && b != 1) {
+        if (b != 0 && b != 1 && b != 2) {
             atomicAdd(&iscal[IS_UNSUPPORTED], 1);
             bad = true;
         }
@@ -1312,10 +1312,14 @@ __global__ __launch_bounds__(256) void k_boundary_2d(DP P, const DFace* __restri
         else if (sides[b].prev == a + 1)
             loc[0] = sides[a].len - loc[0];
     }
+    // SelectCloseBoundarySides2D.f90:66-120: fixe/tapi sides with y > 0; open
+    // sides with y > 0 and 0 < x < side length
     int iside[4];
     double iloc[8];
     for (int q = 0; q < ncbs; ++q) {
-        if (loc[2 * q + 1] > 0.0) {
+        bool keep = loc[2 * q + 1] > 0.0;
+        if (sides[clo[q]].btype == 2) keep = keep && loc[2 * q] > 0.0 && loc[2 * q] < sides[clo[q]].len;
+        if (keep) {
             iside[nint] = clo[q];
             iloc[2 * nint] = loc[2 * q];
             iloc[2 * nint + 1] = loc[2 * q + 1];
@@ -2079,6 +2083,21 @@ __device__ bool open_face_crossed(const DFace& F, const double xo[3], const doub
     return d_point_internal(pic_of(F), LX[0], LX[1]);
 }
 
+// CancelOutgoneParticles_2D.f90:80-101 for one open side (x-z plane), FP64:
+// the step from xo to xn ends on the outer side of the side's line
+// (determinant <= 0), started on the other, and the side's two vertices lie
+// on opposite sides of the step's line.
+__device__ bool open_side_crossed(const DFace& S, const double xo[3], const double xn[3]) {
+    const double o1x = S.v1[0], o1z = S.v1[1], o2x = S.v2[0], o2z = S.v2[1];
+    const double dn = (xn[2] - o1z) * (o2x - o1x) - (xn[0] - o1x) * (o2z - o1z);
+    if (!(dn <= 0.0)) return false;
+    const double dold = (xo[2] - o1z) * (o2x - o1x) - (xo[0] - o1x) * (o2z - o1z);
+    if (copysign(1.0, dn) == copysign(1.0, dold)) return false;
+    const double d1 = (o1z - xo[2]) * (xn[0] - xo[0]) - (o1x - xo[0]) * (xn[2] - xo[2]);
+    const double d2 = (o2z - xo[2]) * (xn[0] - xo[0]) - (o2x - xo[0]) * (xn[2] - xo[2]);
+    return copysign(1.0, d1) != copysign(1.0, d2);
+}
+
 // SA-SPH terms of the momentum equation (AddBoundaryContributions_to_ME3D.f90:
 // 118-336), FP64; tp/td/tv start at zero
 __device__ __forceinline__ void sa_momentum(const DP& P, const DFace* __restrict__ faces,
@@ -2167,6 +2186,7 @@ __device__ __forceinline__ void sa_momentum_2d(const DP& P, const DFace* __restr
     for (int q = 0; q < ncbf; ++q) {
         const BTab& B = bt[(size_t)q * P.cap + i];
         const DFace& S = sides[B.face];
+        if (S.btype == 2) continue;  // AddBoundaryContributions_to_ME2D.f90:124
         const double xpi = B.loc[0], ypi = B.loc[1];
         const double IntWdS = B.bi[0], IntWdV = B.bi[2], IntWd1s0 = B.bi[5], IntWd3s0 = B.bi[6];
         const double n0 = S.T[6], n1 = S.T[8];
@@ -2235,6 +2255,7 @@ __device__ __forceinline__ void elastic_2d(const DP& P, const DFace* __restrict_
     for (int q = 0; q < ncbf; ++q) {
         const BTab& B = bt[(size_t)q * P.cap + i];
         const DFace& S = sides[B.face];
+        if (S.btype != 0 && S.btype != 1) continue;  // AddElasticBoundaryReaction_2D.f90:75
         const double xpi = B.loc[0], ypi = B.loc[1];
         if (ypi < ypimin && xpi > 0.0 && xpi < S.len) {
             double vin = 0.0;
@@ -2562,6 +2583,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                 for (int q = 0; q < ncbf && P.dim == 2; ++q) {
                     const BTab& B = bt[(size_t)q * P.cap + i];
                     const DFace& S = faces[B.face];
+                    if (S.btype != 0 && S.btype != 1) continue;
                     const double s0 = S.T[0], s1 = S.T[2], n0 = S.T[6], n1 = S.T[8];
                     const double DVG0 = 2.0 * (S.tvel[0] - vi.x), DVG1 = 2.0 * (S.tvel[2] - vi.z);
                     const double DVL0 = s0 * DVG0 + s1 * DVG1;
@@ -2626,11 +2648,13 @@ __global__ __launch_bounds__(256) void k_vsmooth(
             // OrdGrid1 keeps: not those out of the grid, nor those that left
             // through an open face
             bool stays = d_particle_cell(P, xn[0], xn[1], xn[2]) > 0;
-            if (P.nopen > 0) {  // CancelOutgoneParticles_3D on the step just made
+            if (P.nopen > 0) {  // CancelOutgoneParticles_3D / _2D on the step just made
                 const double xo[3] = {xi.x, xi.y, xi.z};
                 int g = 0;
                 for (int f = P.foff; f < P.foff + P.nf; ++f)
-                    if (faces[f].btype == 2 && open_face_crossed(faces[f], xo, xn)) g = 1;
+                    if (faces[f].btype == 2 &&
+                        (P.dim == 2 ? open_side_crossed(faces[f], xo, xn) : open_face_crossed(faces[f], xo, xn)))
+                        g = 1;
                 gone[i] = g;
                 stays = stays && !g;
             }
@@ -2701,6 +2725,7 @@ __global__ __launch_bounds__(256) void k_continuity(
                 for (int q = 0; q < ncbf; ++q) {
                     const BTab& B = bt[(size_t)q * P.cap + i];
                     const DFace& S = faces[B.face];
+                    if (S.btype != 0 && S.btype != 1) continue;
                     const double IntWdS = B.bi[0];
                     double dv0, dv1;
                     if (S.btype == 0) {
@@ -2909,6 +2934,7 @@ __global__ __launch_bounds__(256) void k_psmooth(
                     for (int q = 0; q < ncbf; ++q) {
                         const BTab& B = bt[(size_t)q * P.cap + i];
                         const DFace& S = faces[B.face];
+                        if (S.btype != 0 && S.btype != 1) continue;
                         const double mf0 = S.T[0] * P.grav[0] + S.T[2] * P.grav[2];
                         const double mf1 = S.T[6] * P.grav[0] + S.T[8] * P.grav[2];
                         VFT = VFT + B.bi[1];

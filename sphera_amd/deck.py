@@ -116,6 +116,7 @@ def deck_text_2d(case, itmax=1000000000, tmax=1000.0, results_step=None):
               "Boundary", "1", "fixed", *_fixed_zone(case), "111111",
               "Reservoir", "2", "perimeter", "1", "uniform 009EA8",
               "std 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[1])}", "1 0 .false.",
+              *(["Outlet", "3", "open", "FF0000"] if getattr(case, "outlet", False) else []),
               "##### END BOUNDARIES #####",
               "##### DBSPH #####", "1. .false. 0.5 0 .false.", ".false. .true.", "0 0",
               "0 .false.", "0 0 6", "##### END DBSPH #####",

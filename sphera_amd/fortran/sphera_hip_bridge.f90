@@ -20,8 +20,8 @@
 !
 ! Scope the device implements (checked by sph_hip_open): Domain%tipo="semi",
 ! leapfrog (time_split=1) or explicit RK1, one medium, no NormFix, no source
-! faces, open faces of type "open" only (3D), no dt_alfa_Mon, fluid
-! particles only.
+! faces or sides, open faces (3D) and open sides (2D) of type "open" only, no
+! dt_alfa_Mon, fluid particles only.
 module sphera_hip_bridge
 use iso_c_binding
 use sphera_hip_c
@@ -46,7 +46,7 @@ integer(c_int32_t),allocatable,target,save :: gcount(:), gstart(:), gvec(:)
 real(c_double),allocatable,target,save :: egx(:), ecomp(:), elen(:)
 integer(c_int32_t),allocatable,target,save :: eface(:)
 ! 2D: boundary sides
-real(c_double),allocatable,target,save :: sRN(:), sv1(:), slen(:), sang(:)
+real(c_double),allocatable,target,save :: sRN(:), sv1(:), sv2(:), slen(:), sang(:)
 integer(c_int32_t),allocatable,target,save :: sprev(:)
 
 contains
@@ -70,6 +70,16 @@ do k=1,NumOpenFaces
    if (Tratto(BoundaryFace(OpenFace(k))%stretch)%tipo/="open") other_open_faces = .true.
 enddo
 end function other_open_faces
+#else
+! True when a side of OpenSide (Gest_Input.f90:600-613) is of type "leve",
+! "crit", "velo" or "flow": only "open" ones are on the device.
+logical function other_open_sides()
+integer(4) :: k
+other_open_sides = .false.
+do k=1,NumOpenSides
+   if (BoundarySide(OpenSide(k))%tipo/="open") other_open_sides = .true.
+enddo
+end function other_open_sides
 #endif
 
 integer(c_int32_t) function btype_code(tipo)
@@ -155,7 +165,8 @@ subroutine pack_sides(sd)
 type(sph_sides2d_t),intent(out) :: sd
 integer(4) :: k,ns,st,r,c
 ns = NumBSides
-allocate(fT(9*max(ns,1)),sRN(9*max(ns,1)),sv1(3*max(ns,1)),slen(max(ns,1)),sang(max(ns,1)))
+allocate(fT(9*max(ns,1)),sRN(9*max(ns,1)),sv1(3*max(ns,1)),sv2(3*max(ns,1)),slen(max(ns,1)))
+allocate(sang(max(ns,1)))
 allocate(fvel(3*max(ns,1)),ftvel(3*max(ns,1)),fslip(max(ns,1)),fbtype(max(ns,1)))
 allocate(sprev(max(ns,1)),fmode(max(ns,1)),flam(max(ns,1)),fbit(22*6))
 do k=0,ns-1
@@ -167,6 +178,7 @@ do k=0,ns-1
       enddo
    enddo
    sv1(3*k+1:3*k+3) = Vertice(1:3,BoundarySide(k+1)%vertex(1))
+   sv2(3*k+1:3*k+3) = Vertice(1:3,BoundarySide(k+1)%vertex(2))
    slen(k+1) = BoundarySide(k+1)%length
    sang(k+1) = BoundarySide(k+1)%angle
    sprev(k+1) = BoundarySide(k+1)%previous_side
@@ -197,6 +209,7 @@ sd%previous = c_loc(sprev)
 sd%slip_mode = c_loc(fmode)
 sd%lam_check = c_loc(flam)
 sd%bit2d = c_loc(fbit)
+sd%v2 = c_loc(sv2)
 end subroutine pack_sides
 #else
 ! Frontier faces in BFaceList order; GCBFVector entries (BoundaryFace
@@ -306,7 +319,7 @@ end subroutine pack_faces
 
 subroutine free_tables
 if (allocated(fLX)) deallocate(fLX,fGX,farea,fnodes)
-if (allocated(sRN)) deallocate(sRN,sv1,slen,sang,sprev)
+if (allocated(sRN)) deallocate(sRN,sv1,sv2,slen,sang,sprev)
 deallocate(fT,fvel,ftvel,fslip,fbtype,fmode,flam)
 if (allocated(gcount)) deallocate(gcount,gstart,gvec)
 if (allocated(fkt)) deallocate(fkt)
@@ -328,8 +341,8 @@ real(c_double),allocatable :: xyz(:),uvw(:),rho(:),pp(:),m(:)
 integer(4) :: i
 #ifdef SPACE_2D
 if ((Domain%tipo/="semi").or.(Domain%time_split<0).or.(Domain%time_split>1).or.    &
-   (NMedium/=1).or.Domain%NormFix.or.(NumOpenSides>0).or.(SourceSide/=0).or.       &
-   dt_alfa_Mon) then
+   (NMedium/=1).or.Domain%NormFix.or.(SourceSide/=0).or.dt_alfa_Mon.or.             &
+   other_open_sides()) then
 #else
 if ((Domain%tipo/="semi").or.(Domain%time_split<0).or.(Domain%time_split>1).or.    &
    (NMedium/=1).or.Domain%NormFix.or.(SourceFace/=0).or.dt_alfa_Mon.or.            &

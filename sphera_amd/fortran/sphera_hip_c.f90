@@ -44,6 +44,7 @@ type,bind(C) :: sph_sides2d_t
    type(c_ptr) :: T, RN, v1, length, angle, velocity, tvel, slip_coeff
    type(c_ptr) :: btype, previous, slip_mode, lam_check
    type(c_ptr) :: bit2d
+   type(c_ptr) :: v2
 end type sph_sides2d_t
 
 interface

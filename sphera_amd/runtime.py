@@ -70,7 +70,7 @@ class SphSides2D(C.Structure):
     _fields_ = [("ns", C.c_int32), ("T", PD), ("RN", PD), ("v1", PD), ("length", PD),
                 ("angle", PD), ("velocity", PD), ("tvel", PD), ("slip_coeff", PD),
                 ("btype", PI), ("previous", PI), ("slip_mode", PI), ("lam_check", PI),
-                ("bit2d", PD)]
+                ("bit2d", PD), ("v2", PD)]
 
 
 _lib = None
@@ -182,7 +182,8 @@ def sides_struct(tb):
     S = tb.sides
     k = dict(T=np.ascontiguousarray(S.T.transpose(0, 2, 1).reshape(-1)),
              RN=np.ascontiguousarray(S.RN.transpose(0, 2, 1).reshape(-1)),
-             v1=np.ascontiguousarray(S.v1.reshape(-1)), L=np.ascontiguousarray(S.length, np.float64),
+             v1=np.ascontiguousarray(S.v1.reshape(-1)), v2=np.ascontiguousarray(S.v2.reshape(-1)),
+             L=np.ascontiguousarray(S.length, np.float64),
              ang=np.ascontiguousarray(S.angle, np.float64),
              vel=np.ascontiguousarray(S.velocity.reshape(-1)),
              tvel=np.ascontiguousarray(S.tratto_velocity.reshape(-1)),
@@ -193,7 +194,7 @@ def sides_struct(tb):
              bit=np.ascontiguousarray(tb.bit2d.reshape(-1), np.float64))
     s = SphSides2D(len(S.length), _pd(k["T"]), _pd(k["RN"]), _pd(k["v1"]), _pd(k["L"]), _pd(k["ang"]),
                    _pd(k["vel"]), _pd(k["tvel"]), _pd(k["slip"]), _pi(k["bt"]), _pi(k["prev"]),
-                   _pi(k["mode"]), _pi(k["lam"]), _pd(k["bit"]))
+                   _pi(k["mode"]), _pi(k["lam"]), _pd(k["bit"]), _pd(k["v2"]))
     return s, k
 
 

@@ -6,7 +6,8 @@ from /root/reference/src) on project-authored decks.
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
 With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp,
-rk1, viscous, wallfn, wallfn_2d, edges, outlet) only those fixtures are written.
+rk1, viscous, wallfn, wallfn_2d, edges, outlet, outlet_2d) only those fixtures are
+written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -74,6 +75,14 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         particle order) after 0, 8 and 16 steps,
                         leapfrog, and the same with explicit RK1
                         (ref_outlet_rk1.npz)
+  ref_outlet_2d.npz     the 2D build (600 particles) with the tank's x = 0.6 m
+                        side open (zone 3 "open"): the jittered column
+                        shifted by 0.3035 m and moving towards it at
+                        U(2, 4) m/s (seed 20261206), the top four next to it
+                        lifted above the side's end, three just below the
+                        grid's top moving up (CancelOutgoneParticles_2D);
+                        state after 0, 8 and 16 steps, leapfrog, and the
+                        same with explicit RK1 (ref_outlet_2d_rk1.npz)
   ref_rk1.npz           1440 particles, jittered lattice (seed 20260101), the
                         explicit RK1 branch (time_split = 0, RKscheme = 1:
                         Euler); state after 0, 6 and 12 steps
@@ -401,6 +410,48 @@ def make_outlet():
         np.savez_compressed(HERE / name, **out)
 
 
+def outlet_2d_ic(coord, dx, seed=20261206):
+    """The jittered 2D column shifted by 0.3035 m towards the open x = 0.6 m
+    side, moving into it at U(2, 4) m/s, with U(-0.5, 0.5) m/s vertically."""
+    rng = np.random.default_rng(seed)
+    c = jitter2d(coord, dx)
+    c[:, 0] += 0.3035
+    v = np.zeros_like(c)
+    v[:, 0] = rng.uniform(2.0, 4.0, len(c))
+    v[:, 2] = rng.uniform(-0.5, 0.5, len(c))
+    # the top four particles of the column next to the side lifted above the
+    # tank (0.405-0.415 m, the open side ends at 0.4 m): they cross the
+    # side's line beyond its end and stay
+    right = np.flatnonzero(c[:, 0] > c[:, 0].max() - 0.5 * dx)
+    top = right[np.argsort(c[right, 2])[-4:]]
+    left = np.flatnonzero(c[:, 0] < c[:, 0].min() + 0.5 * dx)
+    up = left[np.argsort(c[left, 2])[-3:]]
+    c[top, 2] = rng.uniform(0.405, 0.415, len(top))
+    v[top, 2] = 0.0
+    # three of the upstream column's top put just below the grid's top
+    # (z = 0.4261 m) and moving up: they leave the grid during the run
+    c[up, 2] = rng.uniform(0.4250, 0.4258, len(up))
+    v[up] = [0.0, 0.0, 3.0]
+    return c, v
+
+
+def make_outlet_2d():
+    """2D open side: particles leaving through it are removed"""
+    for ts, name in ((1, "ref_outlet_2d.npz"), (0, "ref_outlet_2d_rk1.npz")):
+        case = case2d.DamBreak2D(**CASE2D, outlet=True, time_split=ts)
+        tb = case2d.build(case)
+        ic, iv = outlet_2d_ic(tb.particles["coord"], case.dx)
+        hdr, steps = read_dump(run_harness(case, nsteps=16, every=8, lists=False, ic_coord=ic, ic_vel=iv))
+        out = {"ic_coord": ic, "ic_vel": iv}
+        for st in steps:
+            pre = f"s{st['it']}_"
+            for k in ("it", "time", "dt", "dt_average", "nag"):
+                out[pre + k] = st[k]
+            for f in FIELDS + ("var", "laminar_flag"):
+                out[pre + f] = st[f]
+        np.savez_compressed(HERE / name, **out)
+
+
 WALLFN = dict(slip_mode=2, roughness=(0.0005, 0.0015))
 
 
@@ -475,7 +526,7 @@ MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_
           "pelo": make_pelo, "2d": make_2d, "nearwall": make_nearwall,
           "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp, "rk1": make_rk1,
           "viscous": make_viscous, "wallfn": make_wallfn, "wallfn_2d": make_wallfn_2d,
-          "edges": make_edges, "outlet": make_outlet}
+          "edges": make_edges, "outlet": make_outlet, "outlet_2d": make_outlet_2d}
 
 
 def main():

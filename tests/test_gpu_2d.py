@@ -204,3 +204,45 @@ def test_wall_function_2d_fixture():
                 assert errs[f] <= tol, (it, prec, f, errs[f])
     for d in ctx.values():
         d.close()
+
+
+@pytest.mark.parametrize("ts,name", [(1, "ref_outlet_2d.npz"), (0, "ref_outlet_2d_rk1.npz")])
+def test_open_side_2d_fixture(ts, name):
+    """An open side in the 2D build (tests/golden/ref_outlet_2d*.npz, the
+    x = 0.6 m side): it enters the boundary table of the particles next to
+    it as in the oracle (SelectCloseBoundarySides2D.f90:99-107: sides and
+    local coordinates identical), adds no wall term, and the particles whose
+    step crosses it are removed (CancelOutgoneParticles_2D.f90).  Count and
+    particle order equal the reference's after 8 and 16 steps, leapfrog and
+    explicit RK1; FP64 fields within FP64_REL_TOL; FP32-mixed removes the
+    same particles and is within FP32_TOL."""
+    from pathlib import Path
+    from oracle.pyoracle import Oracle
+    from sphera_amd import runtime as rt
+    g = np.load(Path(__file__).resolve().parent / "golden" / name)
+    tb = case2d.build(case2d.DamBreak2D(**SMALL, outlet=True, time_split=ts))
+    print(f"[gpu] 2d open side {name} n={len(g['ic_coord'])}", flush=True)
+    o = Oracle(tb, coord=g["ic_coord"], vel=g["ic_vel"])
+    o.preloop()
+    ctx = {}
+    for prec in (rt.FP64, rt.FP32_MIXED):
+        d = rt.SphContext(tb, precision=prec)
+        d.upload(coord=g["ic_coord"], vel=g["ic_vel"])
+        d.preloop()
+        ctx[prec] = d
+    _btab_close(ctx[rt.FP64], o)
+    dn, dside, _, _, _ = ctx[rt.FP64].btab()
+    assert ((dside == 6) & (np.arange(dside.shape[1]) < dn[:, None])).any(axis=1).sum() == 56
+    for it in (8, 16):
+        for prec, d in ctx.items():
+            d.step(8)
+            ds = d.state()
+            assert ds["nag"] == g[f"s{it}_nag"], (it, prec, ds["nag"])
+            errs = {f: _rel(ds[f], g[f"s{it}_" + f]) for f in FIELDS}
+            print(f"[gpu] 2d open side it={it} precision={prec} n={ds['nag']} "
+                  f"max rel {max(errs.values()):.2e}", flush=True)
+            tol = FP64_REL_TOL if prec == rt.FP64 else FP32_TOL["vel"]
+            for f in FIELDS:
+                assert errs[f] <= tol, (it, prec, f, errs[f])
+    for d in ctx.values():
+        d.close()

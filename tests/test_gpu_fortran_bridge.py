@@ -139,3 +139,29 @@ def test_bridge_convex_edges_wall_function_open_face(tmp_path):
                 ref = g[f"s{it}_" + f]
                 err = np.abs(st[it][f] - ref).max() / max(np.abs(ref).max(), 1e-300)
                 assert err <= 1e-12, (name, it, f, err)
+
+
+@pytest.mark.skipif(not HARNESS2D_HIP.exists(), reason="2D drop-in harness not built")
+def test_bridge_2d_open_side(tmp_path):
+    """The reference's 2D build with an open side (tests/golden/
+    ref_outlet_2d.npz) driving the device through the bridge: the bridge
+    passes the open side with its second vertex, the device removes the
+    particles that cross it, and count and fields after 8 and 16 steps match
+    the reference's within the 2D FP64 tolerance of test_gpu_2d.py."""
+    from sphera_amd import case2d
+    from tests.golden.make_golden import CASE2D
+    from tests.test_gpu_2d import FP64_REL_TOL
+    g = np.load(GOLD / "ref_outlet_2d.npz")
+    case = case2d.DamBreak2D(**CASE2D, outlet=True)
+    print("[bridge] 2d open side steps=16", flush=True)
+    hdr, steps = read_dump(run_harness(case, nsteps=16, every=8, lists=False,
+                                       ic_coord=g["ic_coord"], ic_vel=g["ic_vel"],
+                                       workdir=tmp_path, exe=HARNESS2D_HIP, timeout=120,
+                                       extra_env={"SPH_HIP_PRECISION": "0"}))
+    st = {r["it"]: r for r in steps}
+    for it in (8, 16):
+        assert st[it]["nag"] == g[f"s{it}_nag"], it
+        for f in FIELDS + ("var",):
+            ref = g[f"s{it}_" + f]
+            err = np.abs(st[it][f] - ref).max() / max(np.abs(ref).max(), 1e-300)
+            assert err <= FP64_REL_TOL, (it, f, err)

@@ -442,3 +442,35 @@ def test_golden_open_face_bit_identical(ts, name):
     o0.preloop()
     o0.step(16)
     assert o0.state()["nag"] == 795 and o0.outgone_count() == 0
+
+
+@pytest.mark.parametrize("ts,name", [(1, "ref_outlet_2d.npz"), (0, "ref_outlet_2d_rk1.npz")])
+def test_golden_open_side_2d_bit_identical(ts, name):
+    """The 2D build with the tank's x = 0.6 m side open: the particles whose
+    step crosses it are removed (CancelOutgoneParticles_2D), the four that
+    cross its line above its end stay, three leave the grid through its top.
+    The open side enters the boundary table of the 56 particles next to it
+    (SelectCloseBoundarySides2D.f90:99-107), nine of them beside the floor,
+    whose integrals then take the two-side form; it adds no wall term.
+    Count, particle order, dt and every field equal the reference's after 8
+    and 16 steps, leapfrog and explicit RK1.  With a fixed wall in its place
+    the same input keeps all but the three."""
+    from sphera_amd.case2d import DamBreak2D, build as build2d
+    g = np.load(GOLD / name)
+    case = DamBreak2D(tank=(0.6, 0.4), column=(0.3, 0.2), dx=0.01, name="t2d", outlet=True,
+                      time_split=ts)
+    tb = build2d(case)
+    assert list(tb.sides.btype) == [0, 0, 6, 6, 6, 6, 2]
+    o = Oracle(tb, coord=g["ic_coord"], vel=g["ic_vel"])
+    o.preloop()
+    nc, face, _, _, _ = o.btab()
+    on_open = ((face == 6) & (np.arange(face.shape[1]) < nc[:, None])).any(axis=1)
+    assert on_open.sum() == 56 and (on_open & (nc == 2)).sum() == 9
+    _oracle_steps(o, g, (0, 8, 16))
+    assert o.outgone_count() == 16 and int(g["s16_nag"]) == 581  # and 3 left the grid
+    assert int((g["s16_coord"][:, 0] > 0.6).sum()) == 4
+    case.outlet = False
+    o0 = Oracle(build2d(case), coord=g["ic_coord"], vel=g["ic_vel"])
+    o0.preloop()
+    o0.step(16)
+    assert o0.state()["nag"] == 597 and o0.outgone_count() == 0
