@@ -96,6 +96,12 @@ def lib():
         L.or_get_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
         L.or_get_btab.argtypes = [C.c_void_p, PI, PI, PD, PD, PD]
         L.or_free_surface.argtypes = [C.c_void_p, C.c_int, PI, C.c_int, PD, PD, PD]
+        L.or_set_dbsph.argtypes = [C.c_void_p, C.c_int] + [PD] * 6 + [PI]
+        L.or_set_dbsph.restype = C.c_int
+        L.or_get_dbsph.argtypes = [C.c_void_p] + [PD] * 6 + [PI, PD, PD, PI]
+        L.or_get_dbsph.restype = None
+        L.or_get_fw_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
+        L.or_get_fw_lists.restype = C.c_int
         _lib = L
     return _lib
 
@@ -143,6 +149,9 @@ class Oracle:
         dens = np.ascontiguousarray(P["dens"] if dens is None else dens, np.float64)
         pres = np.ascontiguousarray(P["pres"] if pres is None else pres, np.float64)
         mass = np.ascontiguousarray(P["mass"] if mass is None else mass, np.float64)
+        db = getattr(tb.case, "dbsph", None) if getattr(tb, "dim", 3) == 3 else None
+        if db is not None:  # DB-SPH walls instead of SA-SPH frontiers
+            semi = False
         self.tb = tb
         self.p = params_from_tables(tb, n, semi)
         if getattr(tb, "dim", 3) == 2:
@@ -193,6 +202,13 @@ class Oracle:
                     _pd(k["ecomp"]), _pd(k["elen"]))
         self.ctx = L.or_create(C.byref(self.p), C.byref(f), _pd(self.coord), _pd(vel), _pd(dens),
                                _pd(pres), _pd(mass))
+        if db is not None:
+            W = tb.walls
+            self._wk = [np.ascontiguousarray(a, np.float64).reshape(-1)
+                        for a in (W.coord, W.normal, W.vel, W.weight, W.volume, W.mass)]
+            self._wflags = np.array([db.gamma_limiter, db.fs_allowed, db.negative_wall_p_allowed], np.int32)
+            if L.or_set_dbsph(self.ctx, len(W), *[_pd(a) for a in self._wk], _pi(self._wflags)):
+                raise ValueError("a DB-SPH wall element lies outside the grid")
 
     def __del__(self):
         if getattr(self, "ctx", None) and _lib is not None:
@@ -249,6 +265,31 @@ class Oracle:
         count = np.ascontiguousarray(count, np.int32)
         ids = np.ascontiguousarray(ids, np.int32)
         return int(lib().or_compare_lists(self.ctx, _pi(count), _pi(ids), ids.shape[1]))
+
+    def dbsph_state(self):
+        """DB-SPH state: per particle gamma, sigma, rho_sph (rhoSPH_new), dshep,
+        drho (n,3), dvel (n,3,3) = dvel(i,j), fs; per wall element wall_dens,
+        wall_pres, wall_wet."""
+        n, nw = self.n, len(self.tb.walls)
+        o = {k: np.zeros(n) for k in ("gamma", "sigma", "rho_sph", "dshep")}
+        o["drho"] = np.zeros((n, 3))
+        o["dvel"] = np.zeros((n, 3, 3))
+        o["fs"] = np.zeros(n, np.int32)
+        o["wall_dens"], o["wall_pres"] = np.zeros(nw), np.zeros(nw)
+        o["wall_wet"] = np.zeros(nw, np.int32)
+        lib().or_get_dbsph(self.ctx, *[_pd(o[k]) for k in ("gamma", "sigma", "rho_sph", "dshep", "drho",
+                                                           "dvel")], _pi(o["fs"]), _pd(o["wall_dens"]),
+                           _pd(o["wall_pres"]), _pi(o["wall_wet"]))
+        return o
+
+    def fw_lists(self):
+        """Fluid-wall lists: counts and 0-based wall ids in the reference's order."""
+        n = self.n
+        stride = int(self.p.nmaxpartj)
+        cnt = np.zeros(n, np.int32)
+        ids = np.zeros((n, stride), np.int32)
+        lib().or_get_fw_lists(self.ctx, _pi(cnt), _pi(ids), stride)
+        return cnt, ids
 
     def btab(self):
         n = self.n

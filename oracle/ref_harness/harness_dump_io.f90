@@ -20,7 +20,11 @@ write(u) 1213417555                                  ! 'SPHH'
 #ifdef SPACE_2D
 write(u) 2, nag, NMAXPARTJ, Grid%nmax, Grid%ncd(1:3)
 #else
-write(u) 1, nag, NMAXPARTJ, Grid%nmax, Grid%ncd(1:3)
+if (Domain%tipo=="bsph") then
+   write(u) 3, nag, NMAXPARTJ, Grid%nmax, Grid%ncd(1:3)
+   else
+      write(u) 1, nag, NMAXPARTJ, Grid%nmax, Grid%ncd(1:3)
+endif
 #endif
 write(u) Grid%extr(1:3,1), Grid%extr(1:3,2), Grid%dcd(1:3)
 write(u) Domain%h, Domain%dx, Domain%coefke, Domain%coefkacl, Domain%prif,  &
@@ -50,6 +54,22 @@ do kf=1,NumBSides
       Partz(Tratto(st)%zone)%BC_shear_stress_input, ic
 enddo
 #else
+if (Domain%tipo=="bsph") then
+! DB-SPH: the wall elements (DBSPH_IC_surface_elements.f90,
+! semi_particle_volumes.f90) and the DBSPH section's parameters
+   write(u) ic, NMAXPARTJ, DBSPH%n_w, 0
+   write(u) DBSPH%dx_dxw, DBSPH%k_w
+   write(u) DBSPH%slip_ID, merge(1,0,DBSPH%MUSCL_boundary_flag),              &
+      merge(1,0,DBSPH%Gamma_limiter_flag),                                    &
+      merge(1,0,DBSPH%negative_wall_p_allowed), merge(1,0,DBSPH%FS_allowed),  &
+      DBSPH%n_inlet, DBSPH%n_outlet
+   do kf=1,DBSPH%n_w
+      write(u) pg_w(kf)%coord(1:3), pg_w(kf)%normal(1:3), pg_w(kf)%vel(1:3),  &
+         pg_w(kf)%weight, pg_w(kf)%volume, pg_w(kf)%mass, pg_w(kf)%k_d
+      write(u) pg_w(kf)%adjacent_faces(1:3), pg_w(kf)%cella
+   enddo
+   return
+endif
 write(u) ic, input_any_t%MAXCLOSEBOUNDFACES, NumFacce, NumBEdges
 do kf=1,NumFacce
    nf = BFaceList(kf)
@@ -89,12 +109,35 @@ write(u) (pg(i)%dens,i=1,nag), (pg(i)%pres,i=1,nag), (pg(i)%dden,i=1,nag),  &
    (pg(i)%vpres,i=1,nag), (pg(i)%mass,i=1,nag)
 write(u) (pg(i)%cella,i=1,nag), (pg(i)%kodvel,i=1,nag),                     &
    (pg(i)%koddens,i=1,nag), (pg(i)%laminar_flag,i=1,nag)
+#ifdef SPACE_3D
+if (Domain%tipo=="bsph") then
+! DB-SPH per-particle and wall-element state
+   write(u) (pg(i)%Gamma,i=1,nag), (pg(i)%sigma,i=1,nag),                     &
+      (pg(i)%rhoSPH_new,i=1,nag), (pg(i)%dShep,i=1,nag)
+   write(u) (pg(i)%drho(1:3),i=1,nag)
+   write(u) (pg(i)%dvel(1:3,1:3),i=1,nag)
+   write(u) (pg(i)%FS,i=1,nag)
+   write(u) (pg_w(i)%dens,i=1,DBSPH%n_w), (pg_w(i)%pres,i=1,DBSPH%n_w)
+   write(u) (pg_w(i)%wet,i=1,DBSPH%n_w)
+endif
+#endif
 write(u) lists
 if (lists==0) return
 write(u) nPartIntorno(1:nag)
 do i=1,nag
    write(u) PartIntorno((i-1)*NMAXPARTJ+1:(i-1)*NMAXPARTJ+nPartIntorno(i))
 enddo
+#ifdef SPACE_3D
+if (Domain%tipo=="bsph") then
+! the fluid-wall lists (CalcVarLength.f90:462-540)
+   write(u) nPartIntorno_fw(1:nag)
+   do i=1,nag
+      write(u) PartIntorno_fw((i-1)*NMAXPARTJ+1:(i-1)*NMAXPARTJ+              &
+         nPartIntorno_fw(i))
+   enddo
+   return
+endif
+#endif
 #ifdef SPACE_2D
 ! 2D: Ncbs and IntNcbs per particle, then the IntNcbs records
 ! (ComputeBoundaryDataTab_2D.f90 fills IntNcbs records)

@@ -14,7 +14,13 @@
 ! argument (its BC_zmax_flag then reads an undefined address and the 3D
 ! program stops with a segmentation fault when TetaV > 0); this harness calls
 ! Euler with the loop's BC_zmax_flag (.false.), then EoS_fluid_loop, which is
-! the rest of time_integration.
+! the rest of time_integration.  With Domain%tipo=="bsph" (DB-SPH walls, 3D,
+! RK1) it runs the DB-SPH pre-loop of time_step_loop.f90:82-135 (CalcVarLength
+! with on_going_time_step = -2, removing_DBSPH_fictitious_reservoirs,
+! OrdGrid1, CalcVarLength with -1, Gradients_to_MUSCL and BC_wall_elements)
+! and the DB-SPH stage (no stage-start CalcVarLength; momentum, continuity,
+! Euler with its second OrdGrid1 and CalcVarLength, EoS_fluid_loop, the
+! pressure smoothing, then Gradients_to_MUSCL and BC_wall_elements, :470-473).
 !
 ! Environment:
 !   SPH_HARNESS_OUT    output file (required)
@@ -58,10 +64,12 @@ SourceFace = 0
 #endif
 it_eff = it_start
 BC_zmax_flag = .false.
-if ((Domain%tipo/="semi").or.((Domain%time_split/=1).and.                    &
-   ((Domain%time_split/=0).or.(Domain%RKscheme/=1)))) then
+if (((Domain%tipo/="semi").and.(Domain%tipo/="bsph")).or.                   &
+   ((Domain%time_split/=1).and.                                               &
+   ((Domain%time_split/=0).or.(Domain%RKscheme/=1))).or.                      &
+   ((Domain%tipo=="bsph").and.(Domain%time_split/=0))) then
    write(*,*) "harness: only tipo=semi with leapfrog (time_split=1) or RK1 ",   &
-      "(time_split=0, RKscheme=1) is handled"
+      "(time_split=0, RKscheme=1), or tipo=bsph with RK1, is handled"
    stop 2
 endif
 if (Domain%time_split==0) Domain%time_stage = 1
@@ -126,6 +134,60 @@ endif
 open(newunit=u,file=trim(fout),access="stream",form="unformatted",            &
    status="replace")
 call write_header(u)
+#ifdef SPACE_3D
+if (Domain%tipo=="bsph") then
+! DB-SPH pre-loop (time_step_loop.f90:82-135)
+   call liquid_particle_ID_array
+   call inlet_sections
+   on_going_time_step = -2
+   call CalcVarLength
+   on_going_time_step = it_start
+   call removing_DBSPH_fictitious_reservoirs
+   call OrdGrid1
+   call liquid_particle_ID_array
+   on_going_time_step = -1
+   call CalcVarLength
+   on_going_time_step = it_start
+   if ((nag>0).and.(DBSPH%n_w>0)) then
+      call Gradients_to_MUSCL
+      call BC_wall_elements
+   endif
+   if (it_start==0) call time_step_duration
+   it = it_start
+   call write_step(u,it,lists)
+   do while (it<nsteps)
+      it = it + 1
+      on_going_time_step = it
+      it_eff = it
+      dt_previous_step = dt
+      if (nag>0) call time_step_duration
+      simulation_time = simulation_time + dt
+      call liquid_particle_ID_array
+      call RHS_momentum_equation
+!$omp parallel do default(none) shared(nag,pg) private(npi)
+      do npi=1,nag
+         pg(npi)%dden = 0.d0
+         pg(npi)%dden_ALE12 = 0.d0
+         pg(npi)%dden_ALE12_frozen = 0.d0
+         pg(npi)%koddens = 0
+         call Continuity_Equation(npi)
+      enddo
+!$omp end parallel do
+      call Euler(BC_zmax_flag)
+      call EoS_fluid_loop
+      if (input_any_t%TetaP>zero) call PressureSmoothing_3D
+      if ((nag>0).and.(DBSPH%n_w>0)) then
+         call Gradients_to_MUSCL
+         call BC_wall_elements
+      endif
+      dtvel = dt
+      if (mod(it,every)==0) call write_step(u,it,lists)
+      if (simulation_time>=input_any_t%tmax) exit
+   enddo
+   close(u)
+   return
+endif
+#endif
 ! Pre-loop (time_step_loop.f90:82-173 for the semi branch)
 call liquid_particle_ID_array
 call inlet_sections

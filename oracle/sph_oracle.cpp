@@ -68,6 +68,17 @@ struct Ctx {
     // lists only: CalcVarLength stores the neighbour lists without rag and
     // PartKernel (list checks of large clouds; the passes cannot run)
     bool lists_only = false;
+    // DB-SPH walls (Domain%tipo = "bsph"): wall elements pg_w(1:n_w), their
+    // cell lists (Icont_w, NPartOrd_w), the particles' DB-SPH state and the
+    // fluid-wall lists (nPartIntorno_fw, PartIntorno_fw, rag_fw, kernel_fw)
+    bool dbsph = false;
+    int nw = 0;
+    std::vector<double> wcoord, wnormal, wvel, wweight, wvolume, wmass, wdens, wpres;
+    std::vector<int> wwet, icont_w, npartord_w;
+    int gamma_limiter = 0, fs_allowed = 1, neg_wall_p = 0;
+    std::vector<double> gamma, sigma, rhosph, rhosph_old, dshep, drho, dvelg;
+    std::vector<int> fs, nfw, fwl;
+    std::vector<double> rag_fw, kfw;
 };
 
 inline double Tm(const Ctx& c, int f, int r, int col) { return c.T[9 * f + col * 3 + r]; }
@@ -822,6 +833,16 @@ int or_ordgrid(void* ctx) {
         c.koddens[dst] = c.koddens[src];
         c.laminar[dst] = c.laminar[src];
         c.id[dst] = c.id[src];
+        if (c.dbsph) {  // the whole pg record moves (OrdGrid1.f90:73-79)
+            c.gamma[dst] = c.gamma[src];
+            c.sigma[dst] = c.sigma[src];
+            c.rhosph[dst] = c.rhosph[src];
+            c.rhosph_old[dst] = c.rhosph_old[src];
+            c.dshep[dst] = c.dshep[src];
+            c.fs[dst] = c.fs[src];
+            for (int d = 0; d < 3; ++d) c.drho[3 * dst + d] = c.drho[3 * src + d];
+            for (int d = 0; d < 9; ++d) c.dvelg[9 * dst + d] = c.dvelg[9 * src + d];
+        }
     };
     while (c.n > 0 && npi < c.n) {
         npi = npi + 1;
@@ -851,6 +872,16 @@ int or_ordgrid(void* ctx) {
     c.koddens.resize(c.n);
     c.laminar.resize(c.n);
     c.id.resize(c.n);
+    if (c.dbsph) {
+        c.gamma.resize(c.n);
+        c.sigma.resize(c.n);
+        c.rhosph.resize(c.n);
+        c.rhosph_old.resize(c.n);
+        c.dshep.resize(c.n);
+        c.fs.resize(c.n);
+        c.drho.resize(3 * (size_t)c.n);
+        c.dvelg.resize(9 * (size_t)c.n);
+    }
     for (int i = nmax; i >= 1; --i) c.icont[i] = c.icont[i + 1] - c.icont[i];
     for (int i = 1; i <= nmax + 1; ++i) c.icont[i] = c.icont[i] + 1;
     c.npartord.assign(c.n + 1, 0);
@@ -862,6 +893,178 @@ int or_ordgrid(void* ctx) {
     }
     c.lists_built = false;
     return 0;
+}
+
+// CalcVarLength.f90:462-540: the wall elements of cell ncelj, visited after the
+// cell's fluid particles (the fluid-wall list, kernel_fw, dShep, the wet
+// flags, and the wall parts of sigma and rhoSPH_new).  A slot at exactly
+// r = 2h keeps kernel values of zero here; the reference leaves that slot's
+// previous contents.
+static void dbsph_cell_walls(Ctx& c, int npi, int ncelj, bool& ov) {
+    const int mj = c.p.nmaxpartj;
+    const double ke_coef = c.p.coefke / c.p.h;
+    const double* xi = &c.coord[3 * npi];
+    const double* vi = &c.var[3 * npi];
+    for (int fw = c.icont_w[ncelj]; fw <= c.icont_w[ncelj + 1] - 1; ++fw) {
+        const int npj = c.npartord_w[fw];
+        const double* xw = &c.wcoord[3 * npj];
+        const double r0 = xi[0] - xw[0], r1 = xi[1] - xw[1], r2 = xi[2] - xw[2];
+        const double rij = r0 * r0 + r1 * r1 + r2 * r2;
+        if (rij > c.square_doubleh) continue;
+        if (c.nfw[npi] >= mj) {  // the reference stops (diagnostic 10/9)
+            ov = true;
+            continue;
+        }
+        const size_t s = (size_t)npi * mj + c.nfw[npi];
+        c.nfw[npi] = c.nfw[npi] + 1;
+        c.fwl[s] = npj;
+        const double rij2 = rij;
+        const double q = std::sqrt(rij) / c.p.h;
+        const double q2 = rij2 / c.squareh;
+        const int iq = (int)q;
+        c.rag_fw[3 * s] = r0;
+        c.rag_fw[3 * s + 1] = r1;
+        c.rag_fw[3 * s + 2] = r2;
+        c.kfw[2 * s] = c.kfw[2 * s + 1] = 0.0;
+        if (iq >= 2) continue;
+        double wu = 0.666666667 + q2 * (q * 0.5 - 1.0);
+        if (iq > 0) wu = (2.0 - q) * (2.0 - q) * (2.0 - q) * 0.166666666667;
+        const double k1 = wu * c.p.coefke;
+        c.kfw[2 * s] = k1;
+        const double* nw = &c.wnormal[3 * npj];
+        const double* vw = &c.wvel[3 * npj];
+        c.dshep[npi] = c.dshep[npi] + k1 * c.wweight[npj] *
+                                          (nw[0] * (vw[0] - vi[0]) + nw[1] * (vw[1] - vi[1]) +
+                                           nw[2] * (vw[2] - vi[2]));
+        if ((q * c.p.h) <= (1.3 * c.p.dx + c.wweight[npj] / 2.0))
+            __atomic_store_n(&c.wwet[npj], 1, __ATOMIC_RELAXED);
+        const double denom = 1.0 / (std::sqrt(rij) + c.p.eta);
+        double gm = -2.0 * q + 1.5 * q2;
+        if (iq > 0) gm = -gm + q2 - 2.0;
+        c.kfw[2 * s + 1] = gm * ke_coef * denom;
+        c.sigma[npi] = c.sigma[npi] + c.wvolume[npj] * k1;
+        c.rhosph[npi] = c.rhosph[npi] + c.wmass[npj] * k1;  // NMedium = 1
+    }
+}
+
+// CalcVarLength.f90:842-877: Gamma set from sigma on the pre-loop's first
+// call (on_going_time_step = -2); afterwards the free-surface flag FS where
+// sigma + 0.05 < Gamma, and the density from rhoSPH_new on a particle's
+// first call
+static void dbsph_after_lists(Ctx& c) {
+    for (int npi = 0; npi < c.n; ++npi) {
+        if (c.on_going == -2) {
+            if (c.nfw[npi] == 0) {
+                c.gamma[npi] = 1.0;
+            } else {
+                c.gamma[npi] = c.sigma[npi];
+                if (c.gamma_limiter) c.gamma[npi] = std::min(c.gamma[npi], 1.0);
+            }
+        }
+        if (c.on_going > -2) {
+            const double msg = std::min(c.sigma[npi] + 0.05, c.gamma[npi]);
+            if (c.fs_allowed && msg != c.gamma[npi]) c.fs[npi] = 1;
+            if (c.rhosph_old[npi] == 0.0) {
+                if (c.fs[npi] == 0) c.dens[npi] = c.rhosph[npi] / c.gamma[npi];
+                else c.dens[npi] = c.rhosph[npi] / c.sigma[npi];
+            }
+        }
+    }
+}
+
+// inter_EqMoto.f90:286-322 with slip_ID = 0: the wall elements' pressure
+// terms; the wall shear term (DBSPH_BC_shear_viscosity_term.f90:46-49) and
+// the semi-particle Morris term read the VSL velocity gradients and
+// semi-particle viscosities, which are zero without the VSL model, so they
+// add zeros
+static void dbsph_wall_momentum(Ctx& c, int npi, double tp[3], double tv[3]) {
+    const int mj = c.p.nmaxpartj;
+    const double rhoi = c.dens[npi], pi = c.pres[npi];
+    for (int k = 0; k < c.nfw[npi]; ++k) {
+        const size_t s = (size_t)npi * mj + k;
+        const int npj = c.fwl[s];
+        const double wd = c.wdens[npj];
+        const double A = pi / (rhoi * rhoi) + c.wpres[npj] / (wd * wd);
+        const double* nw = &c.wnormal[3 * npj];
+        const double* R = &c.rag_fw[3 * s];
+        for (int d = 0; d < 3; ++d) {
+            double ap = -wd * c.wweight[npj] * c.kfw[2 * s] * A * nw[d];
+            ap = ap - c.wmass[npj] * A * R[d] * c.kfw[2 * s + 1];
+            tp[d] = tp[d] + ap;
+        }
+    }
+    for (int d = 0; d < 3; ++d) tv[d] = tv[d] + 0.0 + 0.0;
+}
+
+// Gradients_to_MUSCL.f90:47-91 (the density and velocity gradients of the
+// particles with wall neighbours; others keep their previous values) and
+// BC_wall_elements.f90:71-158 (each wall element's density and pressure:
+// the Shepard average, over its fluid neighbours in pg order, of the
+// partial-Riemann state rho* = rho_L + (u_L - u_R) rho_L / c_L, clamped at
+// 10, and p* = EoS(rho*), clamped at -99 kPa)
+static void dbsph_walls_state(Ctx& c) {
+    const int mj = c.p.nmaxpartj;
+    for (int npi = 0; npi < c.n; ++npi) {
+        if (c.nfw[npi] <= 0) continue;
+        double* g = &c.drho[3 * npi];
+        double* V = &c.dvelg[9 * npi];  // V[3 r + k] = dvel(r+1, k+1)
+        for (int d = 0; d < 3; ++d) g[d] = 0.0;
+        for (int d = 0; d < 9; ++d) V[d] = 0.0;
+        const double* vi = &c.var[3 * npi];
+        for (int k = 0; k < c.nnb[npi]; ++k) {
+            const size_t s = (size_t)npi * mj + k;
+            const int npj = c.nbr[s];
+            const double vol = c.mass[npj] / c.dens[npj];
+            const double* R = &c.rag[3 * s];
+            const double K0 = c.pk[4 * s];
+            for (int d = 0; d < 3; ++d) g[d] = g[d] + R[d] * (c.dens[npj] - c.dens[npi]) * K0 * vol;
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d)
+                    V[3 * r + d] = V[3 * r + d] + R[d] * (c.var[3 * npj + r] - vi[r]) * K0 * vol;
+        }
+    }
+    std::vector<double> den(c.nw, 0.0);
+    std::vector<char> neigh(c.nw, 0);
+    for (int w = 0; w < c.nw; ++w) {
+        c.wpres[w] = 0.0;
+        c.wdens[w] = c.p.den0;
+    }
+    for (int npi = 0; npi < c.n; ++npi) {
+        const double* g = &c.drho[3 * npi];
+        const double* V = &c.dvelg[9 * npi];
+        const double* vi = &c.var[3 * npi];
+        for (int k = 0; k < c.nfw[npi]; ++k) {
+            const size_t s = (size_t)npi * mj + k;
+            const int npj = c.fwl[s];
+            if (!neigh[npj]) {
+                c.wdens[npj] = 0.0;
+                neigh[npj] = 1;
+            }
+            const double* R = &c.rag_fw[3 * s];
+            const double* nw = &c.wnormal[3 * npj];
+            const double* vw = &c.wvel[3 * npj];
+            const double rhoL = c.dens[npi] - (g[0] * R[0] + g[1] * R[1] + g[2] * R[2]);
+            double uC[3];
+            for (int r = 0; r < 3; ++r) uC[r] = vi[r] - (V[3 * r] * R[0] + V[3 * r + 1] * R[1] + V[3 * r + 2] * R[2]);
+            const double uL = uC[0] * nw[0] + uC[1] * nw[1] + uC[2] * nw[2];
+            const double uR = vw[0] * nw[0] + vw[1] * nw[1] + vw[2] * nw[2];
+            const double cL = std::sqrt(c.p.eps / rhoL);
+            double rhostar = rhoL + (uL - uR) * rhoL / cL;
+            if (rhostar < 10.0) rhostar = 10.0;
+            double pstar = c.p.eps / c.p.den0 * (rhostar - c.p.den0) + c.p.prif;
+            if (pstar < -99.0e3) pstar = -99.0e3;
+            const double Ww = c.mass[npi] / c.dens[npi] * c.kfw[2 * s];
+            den[npj] = den[npj] + Ww;
+            c.wdens[npj] = c.wdens[npj] + rhostar * Ww;
+            c.wpres[npj] = c.wpres[npj] + pstar * Ww;
+        }
+    }
+    for (int w = 0; w < c.nw; ++w) {
+        if (!neigh[w]) continue;
+        c.wdens[w] = c.wdens[w] / den[w];
+        c.wpres[w] = c.wpres[w] / den[w];
+        if (!c.neg_wall_p && c.wpres[w] < 0.0) c.wpres[w] = 0.0;
+    }
 }
 
 // CalcVarLength.f90:111-269 (fluid-fluid part: lists, rag, PartKernel)
@@ -878,9 +1081,25 @@ int or_calcvarlength(void* ctx) {
     }
     const double ke_coef = c.p.coefke / c.p.h;
     const double kacl_coef = c.p.coefkacl / c.p.h;
+    const bool db = c.dbsph && !lo;
+    if (db) {
+        c.nfw.assign(n, 0);
+        if (c.fwl.size() < (size_t)n * mj) {
+            c.fwl.resize((size_t)n * mj);
+            c.rag_fw.resize((size_t)n * mj * 3);
+            c.kfw.resize((size_t)n * mj * 2);
+        }
+    }
     int overflow = 0;
 #pragma omp parallel for schedule(static) reduction(+ : overflow)
     for (int npi = 0; npi < n; ++npi) {
+        if (db) {  // CalcVarLength.f90:134-140
+            c.rhosph_old[npi] = c.rhosph[npi];
+            c.rhosph[npi] = 0.0;
+            c.dshep[npi] = 0.0;
+            c.sigma[npi] = 0.0;
+            c.fs[npi] = 0;
+        }
         const int nceli = c.cella[npi];
         if (nceli == 0) continue;
         int ig, jg, kg;
@@ -895,6 +1114,7 @@ int or_calcvarlength(void* ctx) {
                     if (ncelj == 0) continue;
                     for (int mm = c.icont[ncelj]; mm <= c.icont[ncelj + 1] - 1; ++mm) {
                         if (cnt >= mj) { ov = true; continue; }
+                        if (db && c.nfw[npi] >= mj) { ov = true; continue; }
                         const int npj = c.npartord[mm];
                         const double* xj = &c.coord[3 * npj];
                         const double r0 = xi[0] - xj[0];
@@ -931,13 +1151,19 @@ int or_calcvarlength(void* ctx) {
                         K[1] = K[0] / (rij2 + c.p.eta2);
                         K[2] = gw * denom;
                         K[3] = wu * c.p.coefke;
+                        if (db) {  // CalcVarLength.f90:330-340 (one medium)
+                            c.sigma[npi] = c.sigma[npi] + c.mass[npj] * K[3] / c.dens[npj];
+                            c.rhosph[npi] = c.rhosph[npi] + c.mass[npj] * K[3];
+                        }
                     }
+                    if (db) dbsph_cell_walls(c, npi, ncelj, ov);
                 }
         c.nnb[npi] = cnt;
         if (ov) overflow += 1;
     }
     c.overflow = overflow;
     c.lists_built = true;
+    if (db) dbsph_after_lists(c);
     return 0;
 }
 
@@ -1181,13 +1407,18 @@ int or_momentum(void* ctx) {
             const double dv[3] = {vj[0] - vel_i[0], vj[1] - vel_i[1], vj[2] - vel_i[2]};
             const double* R = &c.rag[3 * s];
             const double* K = &c.pk[4 * s];
-            double alpha = (pj - pi) / (rhoi * rhoj);
-            for (int d = 0; d < 3; ++d) {
-                const double ap = -amassj * alpha * R[d] * K[2];
-                tpa[d] = tpa[d] + ap;
+            if (c.dbsph) {  // inter_EqMoto.f90:229-235
+                const double alpha = pi / (rhoi * rhoi) + pj / (rhoj * rhoj);
+                for (int d = 0; d < 3; ++d) tpa[d] = tpa[d] + -amassj * alpha * R[d] * K[0];
+            } else {
+                double alpha = (pj - pi) / (rhoi * rhoj);
+                for (int d = 0; d < 3; ++d) {
+                    const double ap = -amassj * alpha * R[d] * K[2];
+                    tpa[d] = tpa[d] + ap;
+                }
+                alpha = (pi * (rhoj / rhoi + 1.0) + pj * (rhoi / rhoj - 1.0)) / (rhoi * rhoj);
+                for (int d = 0; d < 3; ++d) ale[d] = ale[d] - amassj * alpha * R[d] * K[2];
             }
-            alpha = (pi * (rhoj / rhoi + 1.0) + pj * (rhoi / rhoj - 1.0)) / (rhoi * rhoj);
-            for (int d = 0; d < 3; ++d) ale[d] = ale[d] - amassj * alpha * R[d] * K[2];
             // viscomon
             const double vrij = -dv[0] * R[0] - dv[1] * R[1] - dv[2] * R[2];
             const double rhot = rhoi + rhoj;
@@ -1222,6 +1453,11 @@ int or_momentum(void* ctx) {
         }
         c.laminar[npi] = lam;
         double* a = &c.acc[3 * npi];
+        if (c.dbsph) {
+            dbsph_wall_momentum(c, npi, tp, tv);
+            for (int d = 0; d < 3; ++d) a[d] = (tp[d] + td[d] + tv[d]) / c.gamma[npi] + c.p.grav[d];
+            continue;
+        }
         if (ncbf > 0 && c.dim == 2) {
             // AddElasticBoundaryReaction_2D.f90:66-99 over the IntNcbs sides
             double br[3] = {0, 0, 0};
@@ -1742,6 +1978,10 @@ int or_psmooth(void* ctx) {
                 c.vpres[npi] = pi + TetaP1 * (somp + (pi - pi) * 0.0 + (pi - pi) * 0.0 + (p0i - pi) * (1.0 - app));
                 continue;
             }
+            if (c.dbsph) {  // PressureSmoothing_3D.f90:110-112
+                c.vpres[npi] = somp / app;
+                continue;
+            }
             for (int q = 0; q < ncbf; ++q) {
                 const size_t s = (size_t)npi * mc + q;
                 if (c.bloc[3 * s + 2] > 0.0) app = app + c.bbi[8 * s + 1];
@@ -1768,8 +2008,27 @@ int or_psmooth(void* ctx) {
 }
 
 // time_step_loop.f90:82-173 (semi branch)
+// time_step_loop.f90:82-135 for Domain%tipo = "bsph": CalcVarLength with
+// on_going_time_step = -2 (Gamma from sigma), the reservoir removal (none
+// here: no fictitious reservoirs), OrdGrid1, CalcVarLength with -1, then the
+// wall elements' state (Gradients_to_MUSCL, BC_wall_elements)
+static int preloop_dbsph(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    c.on_going = -2;
+    or_calcvarlength(ctx);
+    c.on_going = c.it;
+    or_ordgrid(ctx);
+    c.on_going = -1;
+    or_calcvarlength(ctx);
+    c.on_going = c.it;
+    if (c.n > 0 && c.nw > 0) dbsph_walls_state(c);
+    if (c.it == 0) or_dt(ctx);
+    return c.overflow ? 89 : 0;
+}
+
 int or_preloop(void* ctx) {
     Ctx& c = *(Ctx*)ctx;
+    if (c.dbsph) return preloop_dbsph(ctx);
     c.on_going = -999;
     int rc = or_calcvarlength(ctx);
     if (!rc) rc = or_boundary_tab(ctx);
@@ -1812,9 +2071,59 @@ static int step_rk1(void* ctx) {
     return 0;
 }
 
+// time_step_loop.f90:175-489 for Domain%tipo = "bsph" with RK1: no CalcVarLength
+// at the stage's start (the lists are those of the previous step's end),
+// RHS_momentum_equation, Continuity_Equation, then Euler.f90:61-241 for
+// bsph: vel += dt acc, velocity_smoothing and var, coord += dt var,
+// OrdGrid1, Gamma += dShep dt, CalcVarLength, dden / Gamma and
+// dens = rhoSPH_new / Gamma (/ sigma at the free surface, not below
+// 0.98 rho0); EoS_fluid_loop, PressureSmoothing_3D, and the wall elements'
+// state (:470-473)
+static int step_dbsph(void* ctx) {
+    Ctx& c = *(Ctx*)ctx;
+    c.it = c.it + 1;
+    c.on_going = c.it;
+    if (c.n > 0) or_dt(ctx);
+    c.time = c.time + c.dt;
+    int rc = or_momentum(ctx);
+    if (rc) return rc;
+    rc = or_continuity(ctx);
+    if (rc) return rc;
+    for (int i = 0; i < c.n; ++i)
+        for (int d = 0; d < 3; ++d) c.vel[3 * i + d] = c.vel[3 * i + d] + c.dt * c.acc[3 * i + d];
+    or_vsmooth(ctx);
+    or_leapfrog_traj(ctx);
+    or_ordgrid(ctx);
+    for (int i = 0; i < c.n; ++i) {
+        c.gamma[i] = c.gamma[i] + c.dshep[i] * c.dt;
+        if (c.gamma_limiter && c.gamma[i] >= 1.0) c.gamma[i] = 1.0;
+    }
+    or_calcvarlength(ctx);
+    if (c.overflow) return 89;
+    for (int i = 0; i < c.n; ++i) {
+        if (c.gamma_limiter && c.nfw[i] == 0) c.gamma[i] = 1.0;
+        c.dden[i] = c.dden[i] / c.gamma[i];
+        if (c.fs[i] == 0) {
+            c.dens[i] = c.rhosph[i] / c.gamma[i];
+        } else {
+            c.dens[i] = c.rhosph[i] / c.sigma[i];
+            if (c.dens[i] < (0.98 * c.p.den0)) c.dens[i] = 0.98 * c.p.den0;
+        }
+    }
+    or_eos(ctx);
+    or_psmooth(ctx);
+    if (c.n > 0 && c.nw > 0) dbsph_walls_state(c);
+    return 0;
+}
+
 // time_step_loop.f90:175-489 (time_split=1, semi, no open/inlet/zmax zones)
 int or_step(void* ctx, int nsteps) {
     Ctx& c = *(Ctx*)ctx;
+    if (c.dbsph) {
+        for (int s = 0; s < nsteps; ++s)
+            if (int rc = step_dbsph(ctx)) return rc;
+        return 0;
+    }
     if (c.p.time_split == 0) {
         for (int s = 0; s < nsteps; ++s)
             if (int rc = step_rk1(ctx)) return rc;
@@ -1853,6 +2162,82 @@ int64_t or_pair_count(void* ctx) {
 }
 
 int or_overflow(void* ctx) { return ((Ctx*)ctx)->overflow; }
+
+// DB-SPH walls: the wall elements of DBSPH_IC_surface_elements.f90 and
+// semi_particle_volumes.f90 (see sph_oracle.h); bins them into the grid
+// cells in ascending id (OrdGrid1.f90:108-137) and starts the particles' and
+// the walls' DB-SPH state as the reference's initial values do
+int or_set_dbsph(void* ctx, int nw, const double* coord, const double* normal, const double* vel,
+                 const double* weight, const double* volume, const double* mass, const int32_t* flags) {
+    Ctx& c = *(Ctx*)ctx;
+    c.dbsph = true;
+    c.nw = nw;
+    c.wcoord.assign(coord, coord + 3 * nw);
+    c.wnormal.assign(normal, normal + 3 * nw);
+    c.wvel.assign(vel, vel + 3 * nw);
+    c.wweight.assign(weight, weight + nw);
+    c.wvolume.assign(volume, volume + nw);
+    c.wmass.assign(mass, mass + nw);
+    c.wdens.assign(nw, c.p.den0);
+    c.wpres.assign(nw, c.p.prif);
+    c.wwet.assign(nw, 0);
+    c.gamma_limiter = flags[0];
+    c.fs_allowed = flags[1];
+    c.neg_wall_p = flags[2];
+    const int n = c.n;
+    c.gamma.assign(n, 0.0);
+    c.sigma.assign(n, 0.0);
+    c.rhosph.assign(n, 0.0);
+    c.rhosph_old.assign(n, 0.0);
+    c.dshep.assign(n, 0.0);
+    c.fs.assign(n, 0);
+    c.drho.assign(3 * (size_t)n, 0.0);
+    c.dvelg.assign(9 * (size_t)n, 0.0);
+    c.nfw.assign(n, 0);
+    c.icont_w.assign(c.nmax + 2, 0);
+    std::vector<int> cell(nw), inc(c.nmax + 1, 0);
+    for (int w = 0; w < nw; ++w) {
+        cell[w] = particle_cell(c, &c.wcoord[3 * w]);
+        if (cell[w] <= 0) return 2;  // a wall element outside the grid
+        c.icont_w[cell[w]] += 1;
+        c.icont_w[c.nmax + 1] += 1;
+    }
+    for (int i = c.nmax; i >= 1; --i) c.icont_w[i] = c.icont_w[i + 1] - c.icont_w[i];
+    for (int i = 1; i <= c.nmax + 1; ++i) c.icont_w[i] = c.icont_w[i] + 1;
+    c.npartord_w.assign(nw + 1, 0);
+    for (int w = 0; w < nw; ++w) {
+        c.npartord_w[c.icont_w[cell[w]] + inc[cell[w]]] = w;
+        inc[cell[w]] += 1;
+    }
+    return 0;
+}
+
+void or_get_dbsph(void* ctx, double* gamma, double* sigma, double* rhosph, double* dshep, double* drho,
+                  double* dvel, int32_t* fs, double* wall_dens, double* wall_pres, int32_t* wall_wet) {
+    Ctx& c = *(Ctx*)ctx;
+    const int n = c.n;
+    if (gamma) std::copy(c.gamma.begin(), c.gamma.begin() + n, gamma);
+    if (sigma) std::copy(c.sigma.begin(), c.sigma.begin() + n, sigma);
+    if (rhosph) std::copy(c.rhosph.begin(), c.rhosph.begin() + n, rhosph);
+    if (dshep) std::copy(c.dshep.begin(), c.dshep.begin() + n, dshep);
+    if (drho) std::copy(c.drho.begin(), c.drho.begin() + 3 * (size_t)n, drho);
+    if (dvel) std::copy(c.dvelg.begin(), c.dvelg.begin() + 9 * (size_t)n, dvel);
+    if (fs) std::copy(c.fs.begin(), c.fs.begin() + n, fs);
+    if (wall_dens) std::copy(c.wdens.begin(), c.wdens.end(), wall_dens);
+    if (wall_pres) std::copy(c.wpres.begin(), c.wpres.end(), wall_pres);
+    if (wall_wet) std::copy(c.wwet.begin(), c.wwet.end(), wall_wet);
+}
+
+int or_get_fw_lists(void* ctx, int32_t* count, int32_t* ids, int32_t stride) {
+    Ctx& c = *(Ctx*)ctx;
+    const int mj = c.p.nmaxpartj;
+    for (int i = 0; i < c.n; ++i) {
+        const int k = i < (int)c.nfw.size() ? c.nfw[i] : 0;
+        count[i] = k;
+        for (int q = 0; q < k && q < stride; ++q) ids[(size_t)i * stride + q] = c.fwl[(size_t)i * mj + q];
+    }
+    return 0;
+}
 
 // Free-surface monitor lines: calc_pelo.f90:82-131, with the point cells of
 // CalcVarp.f90:66-72 (ceiling of the offset over the cell size, CellNumber).

@@ -117,7 +117,22 @@ void or_branch_counts(void* ctx, int64_t* out6);
  * PartKernel, about 1/15 of the memory), for list checks of large inputs;
  * the pair passes must not run afterwards */
 void or_set_lists_only(void* ctx, int on);
-int or_overflow(void* ctx);         /* particles whose list was truncated */
+int or_overflow(void* ctx);
+
+/* DB-SPH walls (Domain%tipo = "bsph", 3D, explicit RK1): nw wall elements with
+ * centroid, unit normal into the fluid, velocity (xyz interleaved), area
+ * ("weight"), semi-particle volume and mass; flags = {Gamma_limiter_flag,
+ * FS_allowed, negative_wall_p_allowed}.  Called once after or_create (with
+ * semi = 0 in or_params); or_preloop and or_step then run the DB-SPH branch.
+ * Returns 2 if a wall element lies outside the grid. */
+int or_set_dbsph(void* ctx, int nw, const double* coord, const double* normal, const double* vel,
+                 const double* weight, const double* volume, const double* mass, const int32_t* flags);
+/* DB-SPH state in pg order (any pointer may be NULL): Gamma, sigma, rhoSPH_new,
+ * dShep, drho (3), dvel (3x3 row-major), FS; wall elements' dens, pres, wet. */
+void or_get_dbsph(void* ctx, double* gamma, double* sigma, double* rhosph, double* dshep, double* drho,
+                  double* dvel, int32_t* fs, double* wall_dens, double* wall_pres, int32_t* wall_wet);
+/* fluid-wall lists (0-based wall ids, the reference's order) */
+int or_get_fw_lists(void* ctx, int32_t* count, int32_t* ids, int32_t stride);         /* particles whose list was truncated */
 
 /* calc_pelo: free-surface position (pelolib) and lower-fluid-top height (zlft)
  * of each monitoring line, 3 values per line; icont: 2 per line, 1-based

@@ -63,6 +63,10 @@ class DamBreak3D:
     # True: the tank's downstream wall (x = tank length) is an open face
     # (zone 3, type "open"): particles leaving through it are removed
     outlet: bool = False
+    # DB-SPH walls instead of SA-SPH ones (Domain%tipo = "bsph"): a
+    # sphera_amd.dbsph.DBSPHParams; the tank's surface mesh is
+    # sphera_amd.dbsph.tank_mesh (explicit RK1 only)
+    dbsph: object = None
     name: str = "dam_break_3D"
     # monitoring lines of the deck's "control lines" section:
     # (label, first point (x,y,z), second point, number of points)
@@ -488,9 +492,12 @@ def boundary_integral_tab(doubleh):
 # Initial particles: GeneratePart.f90:230-280, SetParticles.f90:86-152,
 # SetParticleParameters.f90:54-131, SubCalcPreIdro.f90:60-196 ("qp" branch)
 # ---------------------------------------------------------------------------
-def _axis_points(xmin, npps, dx):
+def _axis_points(xmin, npps, dx, offset=None):
+    """SetParticles.f90:77-95: PX = Xmin + offset, then npps steps of dx;
+    the offset is -dx/2, or for DB-SPH walls +dx/4 along x and -dx/4 along
+    y and z."""
     pts = []
-    p = xmin - dx * 0.5
+    p = xmin - dx * 0.5 if offset is None else xmin + offset
     for _ in range(npps):
         p = p + dx
         pts.append(p)
@@ -514,7 +521,11 @@ def initial_particles(case: DamBreak3D):
         xr = xmin_all[i] + nbefore * case.dx
         xres.append(xr)
         npps.append(fortran_nint((zmax[i] - xr) / case.dx))
-    ax = [_axis_points(xres[i], npps[i], case.dx) for i in range(3)]
+    if getattr(case, "dbsph", None) is not None:
+        offs = (0.25 * case.dx, -(0.25 * case.dx), -(0.25 * case.dx))
+    else:
+        offs = (None, None, None)
+    ax = [_axis_points(xres[i], npps[i], case.dx, offs[i]) for i in range(3)]
     X, Y, Z = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
     coord = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
     inside = np.all((coord > zmin) & (coord < zmax), axis=1)
@@ -526,7 +537,12 @@ def initial_particles(case: DamBreak3D):
     senhor = g[0] / gmod
     zq = case.free_surface_height()
     affond = (zq - coord[:, 2]) * coshor + coord[:, 0] * senhor
-    pres = (affond * case.rho0 * gmod) + case.p_ref
+    if getattr(case, "dbsph", None) is not None:
+        # SubCalcPreIdro.f90:184-187 for DB-SPH walls: a zero pressure (signed
+        # as the product it is written as), the density rho0
+        pres = 0.0 * (affond * case.rho0 * gmod) * (1.0 - coord[:, 0] / 0.5925)
+    else:
+        pres = (affond * case.rho0 * gmod) + case.p_ref
     dens = case.rho0 * (1.0 + (pres - case.p_ref) / case.bulk)
     mass = np.full(n, s["pvolume"] * case.rho0)
     return dict(coord=coord, vel=np.zeros((n, 3)), dens=dens, pres=pres, mass=mass)
@@ -572,6 +588,7 @@ class CaseTables:
     bitab: np.ndarray
     particles: dict = field(default_factory=dict)
     edges: "ConvexEdges" = None
+    walls: object = None   # DB-SPH wall elements (sphera_amd.dbsph.WallElements)
 
 
 def build(case: DamBreak3D, particles=True, incidence=True) -> CaseTables:
@@ -591,5 +608,9 @@ def build(case: DamBreak3D, particles=True, incidence=True) -> CaseTables:
     bit = boundary_integral_tab(s["doubleh"])
     parts = initial_particles(case) if particles else {
         k: np.zeros((0, 3) if k in ("coord", "vel") else 0) for k in ("coord", "vel", "pres", "dens", "mass")}
+    walls = None
+    if getattr(case, "dbsph", None) is not None:
+        from .dbsph import tank_mesh, wall_elements
+        walls = wall_elements(*tank_mesh(case.tank, case.dx), g, case.dx, case.rho0, case.dbsph)
     return CaseTables(case, s, g, f, cnt, st, vec, kt, ktd, bit, parts,
-                      find_convex_edges(f, case.vertices()))
+                      find_convex_edges(f, case.vertices()), walls)

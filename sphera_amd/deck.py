@@ -45,8 +45,10 @@ def _fixed_zone(case):
 def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
     s = case.scalars()
     v = case.vertices()
+    db = getattr(case, "dbsph", None)
     lines = ["10.0.0", "##### TITLE #####", case.name, "##### END TITLE #####",
-             "##### DOMAIN #####", "semi", f"{_f(case.dx)} {_f(case.trunc)}", ".false.",
+             "##### DOMAIN #####", "bsph" if db is not None else "semi",
+             f"{_f(case.dx)} {_f(case.trunc)}", ".false.",
              "##### END DOMAIN #####", "##### VERTICES #####"]
     for i, p in enumerate(v, start=1):
         lines.append(f"{i} {_f(p[0])} {_f(p[1])} {_f(p[2])}")
@@ -59,8 +61,10 @@ def deck_text(case, itmax=1000000000, tmax=1000.0, results_step=None):
               "std 0. 0. 0. 0. 0. 0. 0.", f"qp {_f(case.column[2])} 0.", "1 0 .false.",
               *(["Outlet", "3", "open", "FF0000"] if getattr(case, "outlet", False) else []),
               "##### END BOUNDARIES #####",
-              "##### DBSPH #####", "1. .false. 0.5 0 .false.", ".false. .true.", "0 0",
-              "0 .false.", "0 0 6", "##### END DBSPH #####",
+              "##### DBSPH #####",
+              *(db.deck_lines() if db is not None else
+                ["1. .false. 0.5 0 .false.", ".false. .true.", "0 0", "0 .false.", "0 0 6"]),
+              "##### END DBSPH #####",
               "##### BED LOAD TRANSPORT #####", "0 1", "1 0. 0.",
               "##### end BED LOAD TRANSPORT #####",
               "##### medium #####", "liquid", "1", f"{_f(case.rho0)} {_f(case.bulk)}",

@@ -6,8 +6,8 @@ from /root/reference/src) on project-authored decks.
     make -f oracle/Makefile.ref -j8 && python tests/golden/make_golden.py [fixture ...]
 
 With names (tiny, small_jitter, removal, pelo, 2d, nearwall, nearwall_2d, dclamp,
-rk1, viscous, wallfn, wallfn_2d, edges, outlet, outlet_2d) only those fixtures are
-written.
+rk1, viscous, wallfn, wallfn_2d, edges, outlet, outlet_2d, dbsph) only those
+fixtures are written.
 
 Fixtures (numpy .npz, inputs and reference outputs only):
   ref_tiny.npz          800 particles, lattice IC; state + neighbour lists +
@@ -75,6 +75,18 @@ Fixtures (numpy .npz, inputs and reference outputs only):
                         particle order) after 0, 8 and 16 steps,
                         leapfrog, and the same with explicit RK1
                         (ref_outlet_rk1.npz)
+  ref_dbsph.npz         DB-SPH walls (Domain%tipo = "bsph", RK1): the tiny
+                        tank as a surface mesh (sphera_amd.dbsph.tank_mesh:
+                        3,800 triangles), 700 particles on the DB-SPH
+                        lattice, U(+-0.01 dx) jitter, v_x U(0, 1) m/s,
+                        v_z -U(0, 0.5) m/s (seed 20261207); the wall elements
+                        of the header; state, DB-SPH state (Gamma, sigma,
+                        rhoSPH_new, dShep, MUSCL gradients, FS, the wall
+                        elements' density, pressure and wet flags) and the
+                        fluid-wall lists after 0, 10 and 20 steps (fluid
+                        lists after 0); ref_dbsph_opts.npz the same with the
+                        Gamma limiter, negative wall pressures allowed and no
+                        free-surface detection
   ref_outlet_2d.npz     the 2D build (600 particles) with the tank's x = 0.6 m
                         side open (zone 3 "open"): the jittered column
                         shifted by 0.3035 m and moving towards it at
@@ -452,6 +464,45 @@ def make_outlet_2d():
         np.savez_compressed(HERE / name, **out)
 
 
+def dbsph_ic(coord, dx, seed=20261207):
+    """U(+-0.01 dx) jitter; v_x U(0, 1) m/s, v_z -U(0, 0.5) m/s."""
+    rng = np.random.default_rng(seed)
+    c = coord + rng.uniform(-0.01, 0.01, coord.shape) * dx
+    v = np.zeros_like(c)
+    v[:, 0] = rng.uniform(0.0, 1.0, len(c))
+    v[:, 2] = -rng.uniform(0.0, 0.5, len(c))
+    return c, v
+
+
+DBSPH_OPTS = {"ref_dbsph.npz": dict(),
+              "ref_dbsph_opts.npz": dict(gamma_limiter=True, negative_wall_p_allowed=True,
+                                         fs_allowed=False)}
+
+
+def make_dbsph():
+    """DB-SPH walls: a tank mesh, RK1"""
+    from sphera_amd.dbsph import DBSPHParams
+    for name, opts in DBSPH_OPTS.items():
+        case = DamBreak3D(**TINY, time_split=0, dbsph=DBSPHParams(**opts))
+        tb = build(case)
+        ic, iv = dbsph_ic(tb.particles["coord"], case.dx)
+        hdr, steps = read_dump(run_harness(case, nsteps=20, every=10, lists=True, ic_coord=ic, ic_vel=iv))
+        out = {"ic_coord": ic, "ic_vel": iv}
+        for k, v in hdr["walls"].items():
+            out["wall_" + k] = v
+        for st in steps:
+            pre = f"s{st['it']}_"
+            for k in ("it", "time", "dt", "dt_average", "nag"):
+                out[pre + k] = st[k]
+            for f in FIELDS + ("var", "laminar_flag", "gamma", "sigma", "rho_sph", "dshep", "drho", "dvel",
+                               "fs", "wall_dens", "wall_pres", "wall_wet", "fw_count", "fw"):
+                out[pre + f] = st[f]
+            if st["it"] == 0:
+                out[pre + "nbr_count"] = st["nbr_count"]
+                out[pre + "nbr"] = st["nbr"]
+        np.savez_compressed(HERE / name, **out)
+
+
 WALLFN = dict(slip_mode=2, roughness=(0.0005, 0.0015))
 
 
@@ -526,7 +577,8 @@ MAKERS = {"tiny": make_tiny, "small_jitter": make_small_jitter, "removal": make_
           "pelo": make_pelo, "2d": make_2d, "nearwall": make_nearwall,
           "nearwall_2d": make_nearwall_2d, "dclamp": make_dclamp, "rk1": make_rk1,
           "viscous": make_viscous, "wallfn": make_wallfn, "wallfn_2d": make_wallfn_2d,
-          "edges": make_edges, "outlet": make_outlet, "outlet_2d": make_outlet_2d}
+          "edges": make_edges, "outlet": make_outlet, "outlet_2d": make_outlet_2d,
+          "dbsph": make_dbsph}
 
 
 def main():

@@ -474,3 +474,75 @@ def test_golden_open_side_2d_bit_identical(ts, name):
     o0.preloop()
     o0.step(16)
     assert o0.state()["nag"] == 597 and o0.outgone_count() == 0
+
+
+def _dbsph_case(name):
+    from sphera_amd.dbsph import DBSPHParams
+    from tests.golden.make_golden import DBSPH_OPTS
+    return DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny", time_split=0,
+                      dbsph=DBSPHParams(**DBSPH_OPTS[name]))
+
+
+def test_dbsph_wall_elements_equal_reference():
+    """The tank mesh's wall elements (sphera_amd.dbsph: the quadrilaterals
+    split into triangles, centroids, normals, areas, edge adjacency through
+    the grid cells, depth coefficients k_d, semi-particle volumes and masses)
+    equal the reference's pg_w after its own pre-processing
+    (Import_ply_surface_meshes, DBSPH_IC_surface_elements,
+    DBSPH_find_close_faces, semi_particle_volumes) bit for bit.  Independent
+    checks: the areas sum to the tank's wetted surface, the normals are unit
+    vectors into the tank, k_d is 1 on flat interior triangles and above 1
+    at the concave edges and corners."""
+    g = np.load(GOLD / "ref_dbsph.npz")
+    W = build(_dbsph_case("ref_dbsph.npz")).walls
+    for k in ("coord", "normal", "weight", "volume", "mass", "k_d", "adjacent", "cell"):
+        assert np.array_equal(getattr(W, k), g["wall_" + k]), k
+    X, Y, Z = 0.3, 0.1, 0.2
+    assert abs(W.weight.sum() - (X * Y + 2 * X * Z + 2 * Y * Z)) < 1e-12
+    assert np.allclose(np.linalg.norm(W.normal, axis=1), 1.0, atol=1e-15)
+    ctr = np.array([X / 2, Y / 2, Z])
+    assert (np.einsum("ij,ij->i", W.normal, ctr - W.coord) > 0).all()
+    flat = (W.coord[:, 2] > 0.02) & (W.coord[:, 2] < 0.18) & (W.coord[:, 1] > 0.0) & (W.coord[:, 1] < Y) \
+        & (W.coord[:, 0] > 0.02) & (W.coord[:, 0] < 0.28)
+    assert np.allclose(W.k_d[flat & ((W.adjacent > 0).sum(1) == 3)], 1.0, atol=1e-12)
+    edge = (W.coord[:, 2] < 0.004) & ((W.coord[:, 1] < 0.004) | (W.coord[:, 1] > Y - 0.004))
+    assert (W.k_d[edge] > 1.2).all()
+
+
+@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz"])
+def test_golden_dbsph_bit_identical(name):
+    """DB-SPH walls (Domain%tipo = "bsph", RK1): the oracle's restatement of
+    the fluid-wall lists and Shepard sums (CalcVarLength.f90:462-540,
+    842-877), the MUSCL gradients and partial-Riemann wall state
+    (Gradients_to_MUSCL.f90, BC_wall_elements.f90), the DB-SPH momentum
+    terms (inter_EqMoto.f90:229-235,286-322, RHS_momentum_equation.f90:
+    173-177) and Euler's DB-SPH step (Euler.f90:61-241) equals the
+    reference bit for bit after the pre-loop and 10 and 20 steps: particle
+    fields, dt and time, Gamma, sigma, rhoSPH_new, dShep, the gradients, the
+    free-surface flags, the wall elements' density, pressure and wet flags,
+    and the fluid-wall lists in the reference's order (fluid lists at the
+    pre-loop).  The second fixture runs the Gamma limiter, negative wall
+    pressures and no free-surface detection."""
+    g = np.load(GOLD / name)
+    o = Oracle(build(_dbsph_case(name)), coord=g["ic_coord"], vel=g["ic_vel"])
+    o.preloop()
+    _check_lists(o, g["s0_nbr_count"], g["s0_nbr"])
+    prev = 0
+    for it in (0, 10, 20):
+        if it:
+            o.step(it - prev)
+            prev = it
+        st, db = o.state(), o.dbsph_state()
+        assert st["nag"] == g[f"s{it}_nag"] and st["dt"] == g[f"s{it}_dt"] and st["time"] == g[f"s{it}_time"]
+        for f in FIELDS + ("var",):
+            assert np.array_equal(st[f], g[f"s{it}_" + f]), (it, f)
+        for f in ("gamma", "sigma", "rho_sph", "dshep", "drho", "dvel", "fs", "wall_dens", "wall_pres",
+                  "wall_wet"):
+            assert np.array_equal(db[f], g[f"s{it}_" + f]), (it, f)
+        cw, idw = o.fw_lists()
+        assert np.array_equal(cw, g[f"s{it}_fw_count"])
+        assert np.array_equal(np.concatenate([idw[i, :cw[i]] for i in range(len(cw))]) + 1, g[f"s{it}_fw"])
+    if name == "ref_dbsph.npz":
+        assert g["s20_fs"].sum() > 0 and (g["s20_wall_pres"] > 0).sum() > 100
+    else:
+        assert g["s20_gamma"].max() == 1.0 and (g["s20_wall_pres"] < 0).any()
