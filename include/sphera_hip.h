@@ -147,6 +147,40 @@ typedef struct sph_sides2d {
     const double *v2;
 } sph_sides2d;
 int sph_ctx_create2d(const sph_params* p, const sph_sides2d* s, sph_ctx** out);
+
+/* DB-SPH walls (Domain%tipo = "bsph"): the wall elements ("semi-particles")
+ * of the frontier's surface mesh, as the reference's pre-processing leaves
+ * pg_w(1:n_w) (Import_ply_surface_meshes.f90, DBSPH_IC_surface_elements.f90,
+ * semi_particle_volumes.f90; sphera_amd/dbsph.py computes them from a mesh):
+ * centroid, unit normal into the fluid and velocity (xyz interleaved), area
+ * ("weight"), semi-particle volume and mass, and the DBSPH section's
+ * Gamma_limiter_flag, FS_allowed and negative_wall_p_allowed.  Replaces the
+ * pg_w set-up of Gest_Input.f90:416-440,574-594 for the path.
+ *
+ * Called once on a context created with semi = 0, no faces, SPH_OPT_RK1
+ * (time_split = 0, RKscheme = 1) and SPH_FP64, before sph_preloop.  The
+ * context then runs the DB-SPH step: the fluid-wall lists and Shepard sums
+ * of CalcVarLength.f90:462-540,842-877, the DB-SPH momentum terms of
+ * inter_EqMoto.f90:229-235,286-322 and RHS_momentum_equation.f90:173-177,
+ * Euler.f90:61-241's DB-SPH branch (Gamma += dShep dt, density from
+ * rhoSPH_new / Gamma), PressureSmoothing_3D.f90:110-112, and the wall
+ * elements' state (Gradients_to_MUSCL.f90, BC_wall_elements.f90).  Static
+ * walls (no DBSPH kinematics), slip_ID = 0 (no VSL model), no MUSCL boundary
+ * terms, no DB-SPH inlets or outlets, one medium.  Status 95 for a segmented
+ * or FP32-mixed context, 2 for the other preconditions, 87 from a step when
+ * a particle has more than 128 wall elements within 2h or a wall element
+ * more than 192 particles. */
+typedef struct sph_dbsph_walls_t {
+    int32_t nw;
+    const double *coord, *normal, *vel, *weight, *volume, *mass;
+    int32_t gamma_limiter, fs_allowed, negative_wall_p_allowed;
+} sph_dbsph_walls_t;
+int sph_dbsph_walls(sph_ctx* ctx, const sph_dbsph_walls_t* walls);
+/* DB-SPH state at a time-step boundary, particles in pg order (any pointer
+ * may be NULL): Gamma, sigma, rhoSPH_new, dShep, FS; the wall elements'
+ * density, pressure and wet flags (pg_w%dens, %pres, %wet). */
+int sph_dbsph_state(sph_ctx* ctx, double* gamma, double* sigma, double* rhosph, double* dshep, int32_t* fs,
+                    double* wall_dens, double* wall_pres, int32_t* wall_wet);
 int sph_ctx_destroy(sph_ctx* ctx);
 
 /* Particle state in pg order: coord, vel (xyz interleaved), dens, pres, mass.

@@ -12,6 +12,7 @@
 
 #include "../../include/sphera_hip.h"
 #include "sph_kernels.hip"
+#include "sph_dbsph.hip"
 
 using namespace sph;
 
@@ -99,6 +100,16 @@ struct sph_ctx {
     int* iscal = nullptr;
     unsigned long long* dtmin = nullptr;
     unsigned long long* pairs = nullptr;  // running pair-interaction count
+    // DB-SPH walls: the wall-element tables (w0, w1, w2, cell lists), their
+    // density, pressure and wet flags; per particle Gamma and dShep (two
+    // buffers each: the sort gathers [0] into [1]), sigma, rhoSPH_new, FS,
+    // the fluid-wall lists and the MUSCL gradients
+    double4 *d_w0 = nullptr, *d_w1 = nullptr, *d_w2 = nullptr;
+    int *d_wstart = nullptr, *d_wids = nullptr, *d_wcell = nullptr;
+    double *d_wdens = nullptr, *d_wpres = nullptr;
+    int* d_wwet = nullptr;
+    double *gam[2] = {}, *dsh[2] = {}, *sig = nullptr, *rsph = nullptr, *drho = nullptr, *dvelg = nullptr;
+    int *fsf = nullptr, *nfw = nullptr, *fwl = nullptr;
     // monitoring lines of the free-surface monitor
     int mon_nlines = 0, mon_npoints = 0;
     int* mon_icont = nullptr;
@@ -206,7 +217,7 @@ __global__ void k_fill_strided(int* p, int stride, int v, int m) {
 // Buffer of the current (rho, p) at a step boundary: the pressure smoothing
 // writes rp[0]; the leapfrog step's sort comes before it, the RK1 step's sort
 // after it (into rp[1]).
-inline int cur_rp(const sph_ctx* c) { return c->P.rk1 ? 1 : 0; }
+inline int cur_rp(const sph_ctx* c) { return c->P.dbsph ? 0 : (c->P.rk1 ? 1 : 0); }
 
 // cell sort + block descriptors (OrdGrid1)
 int run_sort(sph_ctx* c) {
@@ -239,7 +250,8 @@ int run_sort(sph_ctx* c) {
         SEGL(k_gather, nblk(cap), 256, c->perm, c->key, c->xm[1], c->vel[2], c->rp[0], c->lam[0], c->acc[1],
              c->xm[0], c->vel[0], c->rp[1], c->lam[1], c->acc[0], c->cell, c->xmf, XFo(c, 0), XFo(c, 1),
              VFo(c, 0), c->rpf[1], c->start, c->iscal, c->P.rk1 ? c->dden : nullptr, c->dden2,
-             c->P.rk1 ? c->vpres : nullptr, c->vpres2);
+             c->P.rk1 ? c->vpres : nullptr, c->vpres2, c->P.dbsph ? c->gam[0] : nullptr, c->gam[1],
+             c->dsh[0], c->dsh[1]);
     }
     return 0;
 }
@@ -335,7 +347,8 @@ int run_vsmooth(sph_ctx* c) {
 
 int run_continuity(sph_ctx* c) {
     KTime t(c, "continuity");
-    PAIRL(k_continuity, c->chunk_c, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0], c->rp[1], XF(c, 0),
+    PAIRL(k_continuity, c->chunk_c, c->d_faces, c->bn, c->bt, c->xm[0], c->vel[0],
+          c->P.dbsph ? c->rp[0] : c->rp[1], XF(c, 0),
           VF(c, 0), c->nbr, c->nnb, c->dden, c->rp[2], c->rpf[2], c->xrf, c->scal, c->iscal);
     return 0;
 }
@@ -349,13 +362,66 @@ int run_psmooth(sph_ctx* c) {
     return 0;
 }
 
+// DB-SPH walls (one FP64 solve): CalcVarLength's DB-SPH sums in mode 0/1
+// (pre-loop) or 2 (Euler, after the sort: the state in rp[1], the new density
+// and pressure into rp[2])
+DWall dwall(const sph_ctx* c) { return DWall{c->d_w0, c->d_w1, c->d_w2, c->d_wstart, c->d_wids, c->d_wcell}; }
+
+int run_dbsph_lists(sph_ctx* c, int mode) {
+    KTime t(c, "dbsph_lists");
+    const int cap = c->P.cap;
+    if (mode == 0)
+        k_dbsph_lists<0><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->cell,
+                                                           c->start, c->gam[0], c->dsh[0], c->gam[0], c->dsh[0],
+                                                           c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
+                                                           c->dden2, c->rp[2], c->scal, c->iscal);
+    else if (mode == 1)
+        k_dbsph_lists<1><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->cell,
+                                                           c->start, c->gam[0], c->dsh[0], c->gam[0], c->dsh[0],
+                                                           c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
+                                                           c->dden2, c->rp[2], c->scal, c->iscal);
+    else
+        k_dbsph_lists<2><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[1], c->cell,
+                                                           c->start, c->gam[1], c->dsh[1], c->gam[0], c->dsh[0],
+                                                           c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
+                                                           c->dden2, c->rp[2], c->scal, c->iscal);
+    return 0;
+}
+
+int run_momentum_dbsph(sph_ctx* c) {
+    KTime t(c, "momentum");
+    k_momentum_dbsph<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->nbr,
+                                                            c->nnb, c->nfw, c->fwl, c->gam[0], c->d_wdens,
+                                                            c->d_wpres, c->lam[0], c->acc[1], c->vel[1], c->scal,
+                                                            c->iscal);
+    return 0;
+}
+
+// PressureSmoothing_3D.f90 for bsph on the new lists: rp[2] -> rp[0]
+int run_psmooth_dbsph(sph_ctx* c) {
+    KTime t(c, "pressure_smoothing");
+    PAIRL(k_psmooth, c->chunk_p, c->d_faces, c->bn, c->bt, c->xm[0], c->rp[2], c->xmf, c->rpf[2], c->xrf,
+          c->nbr, c->nnb, c->vpres2, c->rp[0], c->rpf[0], nullptr, c->scal, c->iscal);
+    return 0;
+}
+
+// Gradients_to_MUSCL and BC_wall_elements on the step's final state
+int run_dbsph_walls(sph_ctx* c) {
+    KTime t(c, "dbsph_walls");
+    k_dbsph_muscl<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->xm[0], c->vel[0], c->rp[0], c->nbr, c->nnb,
+                                                         c->nfw, c->drho, c->dvelg, c->iscal);
+    k_dbsph_walls<<<nblk(c->P.nw), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->start,
+                                                        c->drho, c->dvelg, c->d_wdens, c->d_wpres, c->iscal);
+    return 0;
+}
+
 int check_flags(sph_ctx* c) {
     std::vector<int> all((size_t)IS_COUNT * c->nseg);
     HIPC(hipMemcpyAsync(all.data(), c->iscal, sizeof(int) * all.size(), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
     int is[IS_COUNT] = {0};
     for (int s = 0; s < c->nseg; ++s)
-        for (int k : {(int)IS_UNSUPPORTED, (int)IS_RM_OVERFLOW, (int)IS_BT_OVERFLOW})
+        for (int k : {(int)IS_UNSUPPORTED, (int)IS_RM_OVERFLOW, (int)IS_BT_OVERFLOW, (int)IS_DB_OVERFLOW})
             is[k] |= all[(size_t)s * IS_COUNT + k];
     if (is[IS_UNSUPPORTED])
         return fail(95, "particle close to a boundary type the device path does not implement "
@@ -366,6 +432,10 @@ int check_flags(sph_ctx* c) {
     if (is[IS_BT_OVERFLOW])
         return fail(86, "a particle has more close boundary faces than the device table holds (" +
                             std::to_string(SPH_MB) + ")");
+    if (is[IS_DB_OVERFLOW])
+        return fail(87, "a DB-SPH list is full: more than " + std::to_string(SPH_MW) +
+                            " wall elements within 2h of a particle, " + std::to_string(SPH_WF) +
+                            " particles within 2h of a wall element, or NMAXPARTJ fluid neighbours");
     return 0;
 }
 
@@ -398,6 +468,10 @@ sph_ctx::~sph_ctx() {
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     F(mon_icont); F(mon_pts); F(mon_out);
+    F(d_w0); F(d_w1); F(d_w2); F(d_wstart); F(d_wids); F(d_wcell); F(d_wdens); F(d_wpres); F(d_wwet);
+    for (auto p : gam) F(p);
+    for (auto p : dsh) F(p);
+    F(sig); F(rsph); F(drho); F(dvelg); F(fsf); F(nfw); F(fwl);
     for (auto e : ev) hipEventDestroy(e);
     for (auto& g : graphs) hipGraphExecDestroy(g.second);
     if (stream) hipStreamDestroy(stream);
@@ -1000,6 +1074,118 @@ int sph_ctx_destroy(sph_ctx* c) {
     return 0;
 }
 
+int sph_dbsph_walls(sph_ctx* c, const sph_dbsph_walls_t* w) {
+    if (!c || !w) return fail(1, "null argument");
+    if (c->P.dbsph) return fail(2, "the context already has DB-SPH walls");
+    if (c->phase != PH_EMPTY && c->phase != PH_SORTED)
+        return fail(120, "sph_dbsph_walls is called before sph_preloop");
+    if (c->nseg != 1) return fail(95, "DB-SPH walls run in a one-solve context");
+    if (c->P.f32) return fail(95, "DB-SPH walls run in FP64 (precision SPH_FP64)");
+    if (!c->P.rk1) return fail(2, "DB-SPH walls need the explicit RK1 step (SPH_OPT_RK1: time_split = 0)");
+    if (c->P.dim != 3) return fail(2, "DB-SPH walls are a 3D frontier");
+    if (c->P.semi || c->P.nf) return fail(2, "a DB-SPH context has no SA-SPH frontier (semi = 0, no faces)");
+    const int nw = w->nw;
+    if (nw <= 0) return fail(2, "no wall elements");
+    if (!w->coord || !w->normal || !w->vel || !w->weight || !w->volume || !w->mass)
+        return fail(1, "null wall-element array");
+    DP& P = c->P;
+    // wall elements into the grid cells, ascending id per cell (OrdGrid1.f90:108-137)
+    std::vector<int> wcell(nw), count(P.nmax + 1, 0);
+    for (int k = 0; k < nw; ++k) {
+        const double* x = w->coord + 3 * k;
+        const int i = (int)std::ceil((x[0] - P.extr_lo[0]) / P.dcd[0]);
+        const int j = (int)std::ceil((x[1] - P.extr_lo[1]) / P.dcd[1]);
+        const int kk = (int)std::ceil((x[2] - P.extr_lo[2]) / P.dcd[2]);
+        if (i < 1 || i > P.ncd[0] || j < 1 || j > P.ncd[1] || kk < 1 || kk > P.ncd[2])
+            return fail(2, "a wall element lies outside the grid");
+        wcell[k] = ((kk - 1) * P.ncd[1] + (j - 1)) * P.ncd[0] + i;
+        count[wcell[k]] += 1;
+    }
+    std::vector<int> wstart(P.nmax + 1, 0), fill(P.nmax + 1, 0), wids(nw);
+    for (int cc = 1; cc <= P.nmax; ++cc) wstart[cc] = wstart[cc - 1] + count[cc];
+    for (int k = 0; k < nw; ++k) {
+        const int cc = wcell[k];
+        wids[wstart[cc - 1] + fill[cc]] = k;
+        fill[cc] += 1;
+    }
+    std::vector<double4> h0(nw), h1(nw), h2(nw);
+    for (int k = 0; k < nw; ++k) {
+        h0[k] = make_double4(w->coord[3 * k], w->coord[3 * k + 1], w->coord[3 * k + 2], w->weight[k]);
+        h1[k] = make_double4(w->normal[3 * k], w->normal[3 * k + 1], w->normal[3 * k + 2], w->volume[k]);
+        h2[k] = make_double4(w->vel[3 * k], w->vel[3 * k + 1], w->vel[3 * k + 2], w->mass[k]);
+    }
+    const size_t N = P.cap;
+    bool ok = dalloc(&c->d_w0, nw) == hipSuccess && dalloc(&c->d_w1, nw) == hipSuccess &&
+              dalloc(&c->d_w2, nw) == hipSuccess && dalloc(&c->d_wstart, P.nmax + 1) == hipSuccess &&
+              dalloc(&c->d_wids, nw) == hipSuccess && dalloc(&c->d_wcell, nw) == hipSuccess &&
+              dalloc(&c->d_wdens, nw) == hipSuccess && dalloc(&c->d_wpres, nw) == hipSuccess &&
+              dalloc(&c->d_wwet, nw) == hipSuccess;
+    for (auto& p : c->gam) ok = ok && dalloc(&p, N) == hipSuccess;
+    for (auto& p : c->dsh) ok = ok && dalloc(&p, N) == hipSuccess;
+    ok = ok && dalloc(&c->sig, N) == hipSuccess && dalloc(&c->rsph, N) == hipSuccess &&
+         dalloc(&c->drho, 3 * N) == hipSuccess && dalloc(&c->dvelg, 9 * N) == hipSuccess &&
+         dalloc(&c->fsf, N) == hipSuccess && dalloc(&c->nfw, N) == hipSuccess &&
+         dalloc(&c->fwl, N * SPH_MW) == hipSuccess;
+    if (!ok) return fail(102, "device allocation of the DB-SPH arrays failed");
+    // DBSPH_IC_surface_elements.f90: dens = den0, pres = prif, wet = 0; the
+    // particles' DB-SPH sums start at zero
+    std::vector<double> d0(nw, P.den0), p0(nw, P.prif);
+    HIPC(hipMemcpy(c->d_w0, h0.data(), sizeof(double4) * nw, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_w1, h1.data(), sizeof(double4) * nw, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_w2, h2.data(), sizeof(double4) * nw, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_wstart, wstart.data(), sizeof(int) * (P.nmax + 1), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_wids, wids.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_wcell, wcell.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_wdens, d0.data(), sizeof(double) * nw, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_wpres, p0.data(), sizeof(double) * nw, hipMemcpyHostToDevice));
+    HIPC(hipMemset(c->d_wwet, 0, sizeof(int) * nw));
+    for (auto p : c->gam) HIPC(hipMemset(p, 0, sizeof(double) * N));
+    for (auto p : c->dsh) HIPC(hipMemset(p, 0, sizeof(double) * N));
+    HIPC(hipMemset(c->nfw, 0, sizeof(int) * N));
+    HIPC(hipMemset(c->fsf, 0, sizeof(int) * N));
+    P.dbsph = 1;
+    P.nw = nw;
+    P.gamma_lim = w->gamma_limiter ? 1 : 0;
+    P.fs_allowed = w->fs_allowed ? 1 : 0;
+    P.neg_wall_p = w->negative_wall_p_allowed ? 1 : 0;
+    return 0;
+}
+
+int sph_dbsph_state(sph_ctx* c, double* gamma, double* sigma, double* rhosph, double* dshep, int32_t* fs,
+                    double* wall_dens, double* wall_pres, int32_t* wall_wet) {
+    if (!c) return fail(1, "null argument");
+    if (!c->P.dbsph) return fail(2, "the context has no DB-SPH walls");
+    if (c->phase != PH_READY) return fail(121, "the DB-SPH state is defined at a time-step boundary");
+    HIPC(hipStreamSynchronize(c->stream));
+    int n;
+    HIPC(hipMemcpy(&n, c->iscal + IS_N, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double4> hv(n);
+    std::vector<double> hg(n), hs(n), hr(n), hd(n);
+    std::vector<int> hf(n);
+    if (n) {
+        HIPC(hipMemcpy(hv.data(), c->vel[0], sizeof(double4) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hg.data(), c->gam[0], sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hs.data(), c->sig, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hr.data(), c->rsph, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hd.data(), c->dsh[0], sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(hf.data(), c->fsf, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
+    for (int s = 0; s < n; ++s) {  // rows in pg (id) order
+        const int r = (int)hv[s].w;
+        if (r < 0 || r >= n) return fail(103, "particle ids are not 0..n-1");
+        if (gamma) gamma[r] = hg[s];
+        if (sigma) sigma[r] = hs[s];
+        if (rhosph) rhosph[r] = hr[s];
+        if (dshep) dshep[r] = hd[s];
+        if (fs) fs[r] = hf[s];
+    }
+    const int nw = c->P.nw;
+    if (wall_dens) HIPC(hipMemcpy(wall_dens, c->d_wdens, sizeof(double) * nw, hipMemcpyDeviceToHost));
+    if (wall_pres) HIPC(hipMemcpy(wall_pres, c->d_wpres, sizeof(double) * nw, hipMemcpyDeviceToHost));
+    if (wall_wet) HIPC(hipMemcpy(wall_wet, c->d_wwet, sizeof(int) * nw, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int sph_state_upload(sph_ctx* c, int32_t n, const double* coord, const double* vel,
                      const double* dens, const double* pres, const double* mass) {
     if (!c) return fail(1, "null argument");
@@ -1440,6 +1626,15 @@ int sph_preloop(sph_ctx* c) {
     if (c->phase != PH_SORTED) return fail(120, "sph_preloop needs a freshly uploaded state");
     run_neighbours(c);
     run_btab(c);
+    if (c->P.dbsph) {
+        // time_step_loop.f90:82-135 for bsph: CalcVarLength with Gamma from
+        // sigma, then again after OrdGrid1 (nothing has moved since the
+        // upload's sort, so that OrdGrid1 keeps every particle in place), and
+        // the wall elements' state
+        run_dbsph_lists(c, 0);
+        run_dbsph_lists(c, 1);
+        run_dbsph_walls(c);
+    }
     HIPC(hipMemsetAsync(c->dtmin, 0x7F, 8 * c->nseg, c->stream));
     run_dt(c, 0, true);
     // the first step's time_step_duration sees the same velocities again
@@ -1453,6 +1648,24 @@ int sph_preloop(sph_ctx* c) {
 }
 
 static void enqueue_steps(sph_ctx* c, int nsteps) {
+    if (c->P.dbsph) {
+        // time_step_loop.f90:175-489 for bsph with RK1: momentum and continuity
+        // on the lists of the previous step's end, Euler (velocity, smoothing,
+        // positions; OrdGrid1; Gamma, CalcVarLength, density), EoS, pressure
+        // smoothing on the new lists, the wall elements' state
+        for (int s = 0; s < nsteps; ++s) {
+            run_dt(c, 1, false);
+            run_momentum_dbsph(c);
+            run_continuity(c);
+            run_vsmooth(c);
+            run_sort(c);
+            run_neighbours(c);
+            run_dbsph_lists(c, 2);
+            run_psmooth_dbsph(c);
+            run_dbsph_walls(c);
+        }
+        return;
+    }
     if (c->P.rk1) {
         // time_step_loop.f90:175-489 with time_split = 0, RKscheme = 1: the
         // lists and boundary table of a stage are built by the previous step's

@@ -27,7 +27,7 @@ __host__ __device__ inline int nb_stride(int nmaxpartj) { return (nmaxpartj + NB
 enum { SC_DT = 0, SC_DTPREV, SC_DTAVG, SC_TIME, SC_DTVEL, SC_TV1, SC_TP1, SC_COUNT };
 enum {
     IS_N = 0, IS_IT, IS_ONGOING, IS_NB_OVERFLOW, IS_BT_OVERFLOW, IS_REMOVED, IS_UNSUPPORTED,
-    IS_NPREV, IS_NRM, IS_RM_OVERFLOW, IS_COUNT
+    IS_NPREV, IS_NRM, IS_RM_OVERFLOW, IS_DB_OVERFLOW, IS_COUNT
 };
 
 // most particles one sort may remove from the grid (renumbering list)
@@ -75,6 +75,9 @@ struct DP {
     int eoff;  // first convex edge of the segment in the context's edge array
     int ne;    // its convex edges (NumBEdges)
     int nopen; // its open faces (NumOpenFaces)
+    // DB-SPH walls (Domain%tipo = "bsph"): nw wall elements; the DBSPH
+    // section's Gamma_limiter_flag, FS_allowed and negative_wall_p_allowed
+    int dbsph, nw, gamma_lim, fs_allowed, neg_wall_p;
 };
 
 // Segment maps of a segmented context, passed by value to the kernels that

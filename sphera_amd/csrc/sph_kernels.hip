@@ -482,7 +482,9 @@ __global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* 
                          F4Out velf,
                          float2* __restrict__ rpf, const int* __restrict__ start, int* iscal,
                          const double* __restrict__ dd_s, double* __restrict__ dd_d,
-                         const double* __restrict__ vp_s, double* __restrict__ vp_d) {
+                         const double* __restrict__ vp_s, double* __restrict__ vp_d,
+                         const double* __restrict__ g_s, double* __restrict__ g_d,
+                         const double* __restrict__ h_s, double* __restrict__ h_d) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int sg = SG ? __builtin_amdgcn_readfirstlane(S.pblk[(blockIdx.x * blockDim.x) >> 8]) : 0;
     const DP& P = SG ? S.P[sg] : P0;
@@ -505,6 +507,10 @@ __global__ void k_gather(DP P0, Seg S, const int* __restrict__ perm, const int* 
     if (dd_s) {  // explicit RK1: dden and vpres of the step end in the new order
         dd_d[p] = dd_s[s];
         vp_d[p] = vp_s[s];
+    }
+    if (g_s) {  // DB-SPH: Gamma and dShep travel with the particle
+        g_d[p] = g_s[s];
+        h_d[p] = h_s[s];
     }
     const int c = key[s] - P.koff + 1;
     cell_d[p] = c;
@@ -2658,7 +2664,7 @@ __global__ __launch_bounds__(256) void k_vsmooth(
                 gone[i] = g;
                 stays = stays && !g;
             }
-            if (P.rk1) {
+            if (P.rk1 && !P.dbsph) {
                 // explicit RK1: Euler.f90:231-241 (dens = dens + dt dden) and
                 // EoS_fluid_loop.f90:59-70 (time_integration.f90:52-65); the
                 // pressure smoothing that follows reads these with the
@@ -2948,7 +2954,8 @@ __global__ __launch_bounds__(256) void k_psmooth(
                         const BTab& B = bt[(size_t)q * P.cap + i];
                         if (B.loc[2] > 0.0) app = app + B.bi[1];
                     }
-                    if (app < 0.95) diffp = somp + (p0i - pi) * (1.0 - app);
+                    if (P.dbsph) diffp = somp / app;  // PressureSmoothing_3D.f90:110-112
+                    else if (app < 0.95) diffp = somp + (p0i - pi) * (1.0 - app);
                     else diffp = somp / app;
                 }
             } else if (P.dim == 2) {

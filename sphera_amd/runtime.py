@@ -73,6 +73,12 @@ class SphSides2D(C.Structure):
                 ("bit2d", PD), ("v2", PD)]
 
 
+class SphDBSPHWalls(C.Structure):
+    _fields_ = [("nw", C.c_int32), ("coord", PD), ("normal", PD), ("vel", PD), ("weight", PD),
+                ("volume", PD), ("mass", PD), ("gamma_limiter", C.c_int32), ("fs_allowed", C.c_int32),
+                ("negative_wall_p_allowed", C.c_int32)]
+
+
 _lib = None
 
 
@@ -97,6 +103,8 @@ def load():
     L.sph_pair_counter.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.sph_global_quantities.argtypes = [C.c_void_p, PD]
     L.sph_global_quantities_noalloc.argtypes = [C.c_void_p, PD]
+    L.sph_dbsph_walls.argtypes = [C.c_void_p, C.POINTER(SphDBSPHWalls)]
+    L.sph_dbsph_state.argtypes = [C.c_void_p, PD, PD, PD, PD, PI, PD, PD, PI]
     L.sph_monitor_lines.argtypes = [C.c_void_p, C.c_int32, PI, C.c_int32, PD]
     L.sph_free_surface.argtypes = [C.c_void_p, PD, PD]
     for ph in PHASES + ["preloop", "sync"]:
@@ -266,8 +274,21 @@ class SphContext:
         self.L = load()
         self.tb = tb
         self.precision = precision
+        db = getattr(tb.case, "dbsph", None) if getattr(tb, "dim", 3) == 3 else None
+        if db is not None:  # DB-SPH walls instead of SA-SPH frontiers
+            semi = False
         self.params = params_struct(tb, precision, capacity, semi, tune)
         h = C.c_void_p()
+        if db is not None:
+            self._check(self.L.sph_ctx_create(C.byref(self.params), None, C.byref(h)), "sph_ctx_create")
+            self.h = h
+            W = tb.walls
+            self._wkeep = [np.ascontiguousarray(a, np.float64).reshape(-1)
+                           for a in (W.coord, W.normal, W.vel, W.weight, W.volume, W.mass)]
+            ws = SphDBSPHWalls(len(W), *[_pd(a) for a in self._wkeep], int(db.gamma_limiter),
+                               int(db.fs_allowed), int(db.negative_wall_p_allowed))
+            self._check(self.L.sph_dbsph_walls(self.h, C.byref(ws)), "sph_dbsph_walls")
+            return
         if getattr(tb, "dim", 3) == 2:
             ss, self._keep = sides_struct(tb)
             self._check(self.L.sph_ctx_create2d(C.byref(self.params), C.byref(ss), C.byref(h)),
@@ -308,6 +329,19 @@ class SphContext:
         v = np.zeros(1, np.int32)
         self._check(self.L.sph_particle_count(self.h, _pi(v)), "sph_particle_count")
         return int(v[0])
+
+    def dbsph_state(self):
+        """DB-SPH state in pg order: gamma, sigma, rho_sph, dshep, fs; wall_dens,
+        wall_pres, wall_wet per wall element."""
+        n, nw = self.n, len(self.tb.walls)
+        o = {k: np.zeros(n) for k in ("gamma", "sigma", "rho_sph", "dshep")}
+        o["fs"] = np.zeros(n, np.int32)
+        o["wall_dens"], o["wall_pres"] = np.zeros(nw), np.zeros(nw)
+        o["wall_wet"] = np.zeros(nw, np.int32)
+        self._check(self.L.sph_dbsph_state(self.h, *[_pd(o[k]) for k in ("gamma", "sigma", "rho_sph", "dshep")],
+                                           _pi(o["fs"]), _pd(o["wall_dens"]), _pd(o["wall_pres"]),
+                                           _pi(o["wall_wet"])), "sph_dbsph_state")
+        return o
 
     def state(self):
         n = self.n

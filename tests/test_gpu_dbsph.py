@@ -1,0 +1,103 @@
+"""DB-SPH walls on the device (sph_dbsph_walls; Domain%tipo = "bsph", explicit
+RK1, FP64) against the reference's DB-SPH fixtures (tests/golden/ref_dbsph*.npz,
+written from the flang build) and the FP64 oracle, whose DB-SPH restatement
+those fixtures pin bit for bit (tests/test_oracle_reference.py).
+
+Bar: bit-identical particle fields, dt and time, Gamma, sigma, rhoSPH_new,
+dShep, the free-surface flags and the wall elements' density, pressure and
+wet flags.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from sphera_amd.case import DamBreak3D, build
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
+DBF = ("gamma", "sigma", "rho_sph", "dshep", "fs", "wall_dens", "wall_pres", "wall_wet")
+
+
+def _case(fixture="ref_dbsph.npz", **kw):
+    from sphera_amd.dbsph import DBSPHParams
+    from tests.golden.make_golden import DBSPH_OPTS
+    base = dict(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
+    base.update(kw)
+    return DamBreak3D(**base, time_split=0, dbsph=DBSPHParams(**DBSPH_OPTS[fixture]))
+
+
+def _ctx(tb, coord=None, vel=None):
+    from sphera_amd import runtime as rt
+    d = rt.SphContext(tb, precision=rt.FP64)
+    d.upload(coord=coord, vel=vel)
+    d.preloop()
+    return d
+
+
+@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz"])
+def test_dbsph_matches_reference_fixture(name):
+    """The reference's DB-SPH run (700 particles in a 3,800-element tank
+    mesh, jittered and moving): the device equals it bit for bit after the
+    pre-loop and after 10 and 20 steps."""
+    g = np.load(GOLD / name)
+    tb = build(_case(name))
+    print(f"[gpu] dbsph {name} n={len(g['ic_coord'])} walls={len(tb.walls)}", flush=True)
+    d = _ctx(tb, g["ic_coord"], g["ic_vel"])
+    prev = 0
+    for it in (0, 10, 20):
+        if it:
+            d.step(it - prev)
+            prev = it
+        st, db = d.state(), d.dbsph_state()
+        assert st["nag"] == g[f"s{it}_nag"] and st["dt"] == g[f"s{it}_dt"] and st["time"] == g[f"s{it}_time"], it
+        bad = [f for f in FIELDS if not np.array_equal(st[f], g[f"s{it}_" + f])]
+        bad += [f for f in DBF if not np.array_equal(db[f], g[f"s{it}_" + f])]
+        print(f"[gpu] dbsph {name} it={it} differing: {bad}", flush=True)
+        assert not bad, (it, bad)
+    d.close()
+
+
+def test_dbsph_longer_oracle_and_graph():
+    """A larger tank (1,320 particles, 8,800 wall elements) for 60 steps:
+    the device equals the oracle bit for bit, and six replays of a captured
+    10-step graph equal 60 eager steps."""
+    from oracle.pyoracle import Oracle
+    tb = build(_case(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), name="small"))
+    rng = np.random.default_rng(11)
+    c = tb.particles["coord"]
+    coord = c + rng.uniform(-0.01, 0.01, c.shape) * tb.case.dx
+    print(f"[gpu] dbsph small n={len(coord)} walls={len(tb.walls)}", flush=True)
+    o = Oracle(tb, coord=coord)
+    o.preloop()
+    o.step(60)
+    os_, odb = o.state(), o.dbsph_state()
+    a = _ctx(tb, coord)
+    b = _ctx(tb, coord)
+    a.step(60)
+    for _ in range(6):
+        b.step_graph(10)
+    b.sync()
+    sa, sb, da = a.state(), b.state(), a.dbsph_state()
+    for f in FIELDS:
+        assert np.array_equal(sa[f], sb[f]), ("graph", f)
+        assert np.array_equal(sa[f], os_[f]), ("oracle", f)
+    for f in DBF:
+        assert np.array_equal(da[f], odb[f]), ("oracle", f)
+    a.close()
+    b.close()
+
+
+def test_dbsph_preconditions_refused():
+    """FP32-mixed and leapfrog contexts refuse DB-SPH walls (status 95, 2)."""
+    from sphera_amd import runtime as rt
+    tb = build(_case())
+    with pytest.raises(rt.SphError) as e:
+        rt.SphContext(tb, precision=rt.FP32_MIXED)
+    assert e.value.code == 95
+    tb.case.time_split = 1
+    with pytest.raises(rt.SphError) as e:
+        rt.SphContext(tb, precision=rt.FP64)
+    assert e.value.code == 2
