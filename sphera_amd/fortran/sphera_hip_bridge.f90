@@ -21,7 +21,10 @@
 ! Scope the device implements (checked by sph_hip_open): Domain%tipo="semi",
 ! leapfrog (time_split=1) or explicit RK1, one medium, no NormFix, no source
 ! faces or sides, open faces (3D) and open sides (2D) of type "open" only, no
-! dt_alfa_Mon, fluid particles only.
+! dt_alfa_Mon, fluid particles only; or Domain%tipo="bsph" (3D DB-SPH walls,
+! pg_w passed to sph_dbsph_walls) with explicit RK1, FP64, slip_ID = 0, no
+! MUSCL boundary terms, no DB-SPH inlets, outlets, kinematics or fictitious
+! reservoirs.
 module sphera_hip_bridge
 use iso_c_binding
 use sphera_hip_c
@@ -47,6 +50,8 @@ real(c_double),allocatable,target,save :: egx(:), ecomp(:), elen(:)
 integer(c_int32_t),allocatable,target,save :: eface(:)
 ! 2D: boundary sides
 real(c_double),allocatable,target,save :: sRN(:), sv1(:), sv2(:), slen(:), sang(:)
+! DB-SPH: wall elements (pg_w)
+real(c_double),allocatable,target,save :: wx(:), wn(:), wv(:), ww(:), wvol(:), wm(:)
 integer(c_int32_t),allocatable,target,save :: sprev(:)
 
 contains
@@ -61,6 +66,25 @@ stop 1
 end subroutine sph_hip_check
 
 #ifdef SPACE_3D
+! True when the DB-SPH case uses what the device does not build: the VSL
+! wall model (slip_ID > 0), MUSCL boundary terms, inlet or outlet sections,
+! moving walls (kinematics records) or fictitious reservoirs / reservoirs
+! from topography (removed in the pre-loop by
+! removing_DBSPH_fictitious_reservoirs.f90)
+logical function dbsph_outside_scope()
+integer(4) :: k
+dbsph_outside_scope = (DBSPH%slip_ID/=0).or.DBSPH%MUSCL_boundary_flag.or.      &
+   (DBSPH%n_inlet/=0).or.(DBSPH%n_outlet/=0).or.(Domain%time_split/=0).or.      &
+   (Domain%RKscheme/=1)
+if (allocated(DBSPH%n_kinematics_records)) then
+   if (any(DBSPH%n_kinematics_records/=0)) dbsph_outside_scope = .true.
+endif
+do k=1,NPartZone
+   if (Partz(k)%DBSPH_fictitious_reservoir_flag.or.(Partz(k)%IC_source_type==2))&
+      dbsph_outside_scope = .true.
+enddo
+end function dbsph_outside_scope
+
 ! True when a face of OpenFace (Gest_Input.f90:617-629) is of type "leve",
 ! "crit", "velo" or "flow": only "open" ones are on the device.
 logical function other_open_faces()
@@ -317,6 +341,65 @@ deallocate(pos)
 end subroutine pack_faces
 #endif
 
+#ifdef SPACE_3D
+! No frontier faces (a DB-SPH context: the walls come from sph_dbsph_walls)
+subroutine empty_faces(f)
+type(sph_faces_t),intent(out) :: f
+f%nf = 0
+f%T = c_null_ptr
+f%LX = c_null_ptr
+f%GX = c_null_ptr
+f%area = c_null_ptr
+f%velocity = c_null_ptr
+f%tvel = c_null_ptr
+f%slip_coeff = c_null_ptr
+f%nodes = c_null_ptr
+f%btype = c_null_ptr
+f%slip_mode = c_null_ptr
+f%lam_check = c_null_ptr
+f%gcbf_count = c_null_ptr
+f%gcbf_start = c_null_ptr
+f%gcbf_vec = c_null_ptr
+f%kerneltab = c_null_ptr
+f%ktdelta = 0.d0
+f%bitab = c_null_ptr
+f%nedges = 0
+f%edge_face = c_null_ptr
+f%edge_gx = c_null_ptr
+f%edge_comp = c_null_ptr
+f%edge_len = c_null_ptr
+end subroutine empty_faces
+
+! pg_w(1:n_w) (DBSPH_IC_surface_elements.f90, semi_particle_volumes.f90) and
+! the DBSPH section's flags in the C layout
+subroutine pack_walls(w)
+type(sph_dbsph_walls_t),intent(out) :: w
+integer(4) :: k,nw
+nw = DBSPH%n_w
+if (allocated(wx)) deallocate(wx,wn,wv,ww,wvol,wm)
+allocate(wx(3*max(nw,1)),wn(3*max(nw,1)),wv(3*max(nw,1)),ww(max(nw,1)),wvol(max(nw,1)))
+allocate(wm(max(nw,1)))
+do k=1,nw
+   wx(3*k-2:3*k) = pg_w(k)%coord(1:3)
+   wn(3*k-2:3*k) = pg_w(k)%normal(1:3)
+   wv(3*k-2:3*k) = pg_w(k)%vel(1:3)
+   ww(k) = pg_w(k)%weight
+   wvol(k) = pg_w(k)%volume
+   wm(k) = pg_w(k)%mass
+enddo
+w%nw = nw
+w%coord = c_loc(wx)
+w%normal = c_loc(wn)
+w%vel = c_loc(wv)
+w%weight = c_loc(ww)
+w%volume = c_loc(wvol)
+w%mass = c_loc(wm)
+w%gamma_limiter = merge(1,0,DBSPH%Gamma_limiter_flag)
+w%fs_allowed = merge(1,0,DBSPH%FS_allowed)
+w%negative_wall_p_allowed = merge(1,0,DBSPH%negative_wall_p_allowed)
+end subroutine pack_walls
+#endif
+
 subroutine free_tables
 if (allocated(fLX)) deallocate(fLX,fGX,farea,fnodes)
 if (allocated(sRN)) deallocate(sRN,sv1,sv2,slen,sang,sprev)
@@ -336,6 +419,7 @@ type(sph_params_t) :: p
 type(sph_sides2d_t) :: sd
 #else
 type(sph_faces_t) :: f
+type(sph_dbsph_walls_t) :: w
 #endif
 real(c_double),allocatable :: xyz(:),uvw(:),rho(:),pp(:),m(:)
 integer(4) :: i
@@ -344,9 +428,9 @@ if ((Domain%tipo/="semi").or.(Domain%time_split<0).or.(Domain%time_split>1).or. 
    (NMedium/=1).or.Domain%NormFix.or.(SourceSide/=0).or.dt_alfa_Mon.or.             &
    other_open_sides()) then
 #else
-if ((Domain%tipo/="semi").or.(Domain%time_split<0).or.(Domain%time_split>1).or.    &
-   (NMedium/=1).or.Domain%NormFix.or.(SourceFace/=0).or.dt_alfa_Mon.or.            &
-   other_open_faces()) then
+if (((Domain%tipo/="semi").and.(Domain%tipo/="bsph")).or.(Domain%time_split<0).or. &
+   (Domain%time_split>1).or.(NMedium/=1).or.Domain%NormFix.or.(SourceFace/=0).or.  &
+   dt_alfa_Mon.or.other_open_faces()) then
 #endif
    write(*,*) "sphera_hip: case outside the device scope (semi, leapfrog or ",     &
       "explicit RK, one medium, fixed/tapis/open frontiers)"
@@ -357,8 +441,20 @@ call pack_params(prec,p)
 call pack_sides(sd)
 call sph_hip_check(sph_ctx_create2d(p,sd,ctx),"sph_ctx_create2d")
 #else
-call pack_faces(f)
-call sph_hip_check(sph_ctx_create(p,f,ctx),"sph_ctx_create")
+if (Domain%tipo=="bsph") then
+   if (dbsph_outside_scope()) then
+      write(*,*) "sphera_hip: DB-SPH case outside the device scope (RK1, ",         &
+         "slip_ID 0, static walls, no inlets/outlets/MUSCL boundary terms)"
+      stop 1
+   endif
+   call empty_faces(f)
+   call sph_hip_check(sph_ctx_create(p,f,ctx),"sph_ctx_create")
+   call pack_walls(w)
+   call sph_hip_check(sph_dbsph_walls(ctx,w),"sph_dbsph_walls")
+   else
+      call pack_faces(f)
+      call sph_hip_check(sph_ctx_create(p,f,ctx),"sph_ctx_create")
+endif
 #endif
 allocate(xyz(3*max(nag,1)),uvw(3*max(nag,1)),rho(max(nag,1)),pp(max(nag,1)),m(max(nag,1)))
 do i=1,nag

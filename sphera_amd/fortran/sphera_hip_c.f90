@@ -38,6 +38,13 @@ type,bind(C) :: sph_faces_t
    type(c_ptr) :: edge_face, edge_gx, edge_comp, edge_len
 end type sph_faces_t
 
+! DB-SPH wall elements (sph_dbsph_walls_t of sphera_hip.h)
+type,bind(C) :: sph_dbsph_walls_t
+   integer(c_int32_t) :: nw
+   type(c_ptr) :: coord, normal, vel, weight, volume, mass
+   integer(c_int32_t) :: gamma_limiter, fs_allowed, negative_wall_p_allowed
+end type sph_dbsph_walls_t
+
 ! 2D boundary sides (sph_sides2d of sphera_hip.h)
 type,bind(C) :: sph_sides2d_t
    integer(c_int32_t) :: ns
@@ -154,6 +161,11 @@ interface
    integer(c_int) function sph_pressure_smoothing(ctx) bind(C,name="sph_pressure_smoothing")
       import :: c_int, c_ptr
       type(c_ptr),value :: ctx
+   end function
+   integer(c_int) function sph_dbsph_walls(ctx,w) bind(C,name="sph_dbsph_walls")
+      import :: c_int, c_ptr, sph_dbsph_walls_t
+      type(c_ptr),value :: ctx
+      type(sph_dbsph_walls_t),intent(in) :: w
    end function
    integer(c_int) function sph_preloop(ctx) bind(C,name="sph_preloop")
       import :: c_int, c_ptr

@@ -165,3 +165,25 @@ def test_bridge_2d_open_side(tmp_path):
             ref = g[f"s{it}_" + f]
             err = np.abs(st[it][f] - ref).max() / max(np.abs(ref).max(), 1e-300)
             assert err <= FP64_REL_TOL, (it, f, err)
+
+
+def test_bridge_dbsph(tmp_path):
+    """The reference program with DB-SPH walls (Domain%tipo = "bsph", RK1)
+    driving the device through the bridge: the bridge passes pg_w, as the
+    reference's own pre-processing built it from the tank's .ply mesh, to
+    sph_dbsph_walls; count and fields after 10 and 20 steps equal the
+    reference's own run (tests/golden/ref_dbsph.npz) bit for bit."""
+    from sphera_amd.dbsph import DBSPHParams
+    from tests.golden.make_golden import TINY
+    g = np.load(GOLD / "ref_dbsph.npz")
+    case = DamBreak3D(**TINY, time_split=0, dbsph=DBSPHParams())
+    print("[bridge] dbsph steps=20", flush=True)
+    hdr, steps = read_dump(run_harness(case, nsteps=20, every=10, lists=False,
+                                       ic_coord=g["ic_coord"], ic_vel=g["ic_vel"],
+                                       workdir=tmp_path, exe=HARNESS_HIP, timeout=120,
+                                       extra_env={"SPH_HIP_PRECISION": "0"}))
+    st = {r["it"]: r for r in steps}
+    for it in (10, 20):
+        assert st[it]["nag"] == g[f"s{it}_nag"] and st[it]["dt"] == g[f"s{it}_dt"], it
+        for f in FIELDS:
+            assert np.array_equal(st[it][f], g[f"s{it}_" + f]), (it, f)
