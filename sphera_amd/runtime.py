@@ -35,7 +35,7 @@ EXPORTS = [
     "sph_preloop", "sph_step", "sph_step_async", "sph_step_graph", "sph_sync", "sph_get_launch", "sph_set_timing",
     "sph_get_timing", "sph_ctx_bytes", "sph_device_memory", "sph_query",
     "sph_batch_create", "sph_batch_bytes", "sph_segment_upload", "sph_segment_download",
-    "sph_segment_scalars", "sph_segment_count",
+    "sph_segment_scalars", "sph_segment_count", "sph_dbsph_walls", "sph_dbsph_state",
     "sph_last_error",
 ]
 
