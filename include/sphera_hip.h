@@ -169,7 +169,7 @@ int sph_ctx_create2d(const sph_params* p, const sph_sides2d* s, sph_ctx** out);
  * terms, no DB-SPH inlets or outlets, one medium.  Status 95 for a segmented
  * or FP32-mixed context, 2 for the other preconditions, 87 from a step when
  * a particle has more than 128 wall elements within 2h or a wall element
- * more than 192 particles. */
+ * more than 128 particles. */
 typedef struct sph_dbsph_walls_t {
     int32_t nw;
     const double *coord, *normal, *vel, *weight, *volume, *mass;

@@ -372,17 +372,17 @@ int run_dbsph_lists(sph_ctx* c, int mode) {
     const int cap = c->P.cap;
     if (mode == 0)
         k_dbsph_lists<0><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->cell,
-                                                           c->start, c->gam[0], c->dsh[0], c->gam[0], c->dsh[0],
+                                                           c->nbr, c->nnb, c->gam[0], c->dsh[0], c->gam[0], c->dsh[0],
                                                            c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
                                                            c->dden2, c->rp[2], c->scal, c->iscal);
     else if (mode == 1)
         k_dbsph_lists<1><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->cell,
-                                                           c->start, c->gam[0], c->dsh[0], c->gam[0], c->dsh[0],
+                                                           c->nbr, c->nnb, c->gam[0], c->dsh[0], c->gam[0], c->dsh[0],
                                                            c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
                                                            c->dden2, c->rp[2], c->scal, c->iscal);
     else
         k_dbsph_lists<2><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[1], c->cell,
-                                                           c->start, c->gam[1], c->dsh[1], c->gam[0], c->dsh[0],
+                                                           c->nbr, c->nnb, c->gam[1], c->dsh[1], c->gam[0], c->dsh[0],
                                                            c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
                                                            c->dden2, c->rp[2], c->scal, c->iscal);
     return 0;

@@ -20,7 +20,7 @@
 namespace sph {
 
 constexpr int SPH_MW = 128;   // fluid-wall list capacity per particle
-constexpr int SPH_WF = 192;   // fluid neighbours a wall element gathers
+constexpr int SPH_WF = 128;   // fluid neighbours a wall element gathers
 
 // wall-element tables: w0 = (x, y, z, weight), w1 = (normal, volume),
 // w2 = (velocity, mass)
@@ -37,10 +37,15 @@ struct DWall {
 // or 1 without wall neighbours); 1: its second call (Gamma kept, FS set);
 // 2: inside Euler after OrdGrid1 (Gamma += dShep dt before the sums, then
 // dden / Gamma and the density and pressure of the particle)
+// The fluid sums walk the particle's neighbour list, whose entries carry the
+// stencil code (dx+1) + 3(dy+1) + 9(dz+1) of their cell and come in the
+// reference's cell order (y, x, z nest; ascending pg index in a cell): each
+// cell's run of the list, then that cell's wall elements.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_dbsph_lists(
     DP P, DWall W, const double4* __restrict__ xm, const double4* __restrict__ vel,
-    const double2* __restrict__ rp, const int* __restrict__ cell, const int* __restrict__ start,
+    const double2* __restrict__ rp, const int* __restrict__ cell, const uint32_t* __restrict__ nbr,
+    const int* __restrict__ nnb,
     const double* __restrict__ gam_in, const double* __restrict__ dsh_in, double* __restrict__ gam_out,
     double* __restrict__ dsh_out, double* __restrict__ sig_out, double* __restrict__ rsph_out,
     int* __restrict__ fs_out, int* __restrict__ nfw, int* __restrict__ fwl, int* __restrict__ wwet,
@@ -57,31 +62,31 @@ __global__ __launch_bounds__(256) void k_dbsph_lists(
         gamma = gam_in[i];
     }
     double sigma = 0.0, rsph = 0.0, dshep = 0.0;
-    int nf = 0, cnt = 0;
+    int nf = 0;
     bool ov = false;
     int ci, cj, ck;
     d_cell_indices(P, cell[i], ci, cj, ck);
-    for (int jr = cj - 1; jr <= cj + 1; ++jr)
-        for (int ir = ci - 1; ir <= ci + 1; ++ir)
-            for (int kr = ck - 1; kr <= ck + 1; ++kr) {
-                const int c = d_cell_number(P, ir, jr, kr);
-                if (c == 0) continue;
-                // the cell's fluid particles (ascending pg index), then its walls
-                for (int s = start[c - 1]; s < start[c]; ++s) {
-                    if (cnt >= P.nmaxpartj) {  // the reference's list would be full
-                        ov = true;
-                        continue;
-                    }
-                    const double4 xj = xm[s];
+    const size_t N = P.cap;
+    const int nn = nnb[i];
+    int kl = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dz = -1; dz <= 1; ++dz) {
+                // the cell's fluid neighbours, then its wall elements
+                const uint32_t code = (uint32_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
+                while (kl < nn) {
+                    const uint32_t e = nbr[nb_index(N, i, kl)];
+                    if ((e >> NB_SHIFT) != code) break;
+                    const double4 xj = xm[e & NB_MASK];
                     const double R0 = xi.x - xj.x, R1 = xi.y - xj.y, R2 = xi.z - xj.z;
-                    const double rij = R0 * R0 + R1 * R1 + R2 * R2;
-                    if (rij > P.square_doubleh) continue;
-                    cnt = cnt + 1;
                     K64 K;
-                    kvals64(P, rij, false, false, true, K);
-                    sigma = sigma + xj.w * K.W / rp[s].x;
+                    kvals64(P, R0 * R0 + R1 * R1 + R2 * R2, false, false, true, K);
+                    sigma = sigma + xj.w * K.W / rp[e & NB_MASK].x;
                     rsph = rsph + xj.w * K.W;
+                    ++kl;
                 }
+                const int c = d_cell_number(P, ci + dx, cj + dy, ck + dz);
+                if (c == 0) continue;
                 for (int w = W.wstart[c - 1]; w < W.wstart[c]; ++w) {
                     const int id = W.wids[w];
                     const double4 a = W.w0[id];
