@@ -611,6 +611,9 @@ def build(case: DamBreak3D, particles=True, incidence=True) -> CaseTables:
     walls = None
     if getattr(case, "dbsph", None) is not None:
         from .dbsph import tank_mesh, wall_elements
-        walls = wall_elements(*tank_mesh(case.tank, case.dx), g, case.dx, case.rho0, case.dbsph)
+        mesh = getattr(case, "mesh", None)
+        if mesh is None:
+            mesh = tank_mesh(case.tank, case.dx)
+        walls = wall_elements(*mesh, g, case.dx, case.rho0, case.dbsph)
     return CaseTables(case, s, g, f, cnt, st, vec, kt, ktd, bit, parts,
                       find_convex_edges(f, case.vertices()), walls)

@@ -237,6 +237,10 @@ class DeckCase:
     fluid_zone: int = 2
     control_lines: tuple = ()
     dim: int = 3
+    # DB-SPH walls (Domain%tipo = "bsph"): sphera_amd.dbsph.DBSPHParams and
+    # the surface mesh (vertices, faces) of input/07_DBSPH; None for SA-SPH
+    dbsph: object = None
+    mesh: object = field(default=None, repr=False)
 
     @property
     def outlet(self):
@@ -297,8 +301,8 @@ def parse_deck(text, name=None):
     # DOMAIN (ReadInputDomain.f90): tipo; dx trunc; the CLC flag (3D)
     dom = sec["domain"]
     tipo = dom[0].split()[-1].lower()
-    if tipo != "semi":
-        raise ValueError(f"Domain%tipo {tipo!r}: only SA-SPH walls (semi) are read")
+    if tipo not in ("semi", "bsph"):
+        raise ValueError(f"Domain%tipo {tipo!r}: only SA-SPH (semi) and DB-SPH (bsph) walls are read")
     dx, trunc = (_num(t) for t in dom[1].split()[:2])
     # VERTICES (ReadInputVertices.f90): id x y z
     vt = {}
@@ -381,9 +385,45 @@ def parse_deck(text, name=None):
         lines.append((cl[i].split()[0], tuple(_num(v) for v in cl[i + 1].split()[:3]),
                       tuple(_num(v) for v in cl[i + 2].split()[:3]), _int(cl[i + 3].split()[0])))
     title = sec.get("title", ["case"])
-    return DeckCase(name=name or title[0].split()[0], dx=dx, trunc=trunc, rho0=rho0, bulk=bulk,
+    dbsph = None
+    if tipo == "bsph":  # ReadDBSPH.f90:35-99, the subset sphera_amd.dbsph runs
+        from .dbsph import DBSPHParams
+        db = sec.get("dbsph", [])
+        if len(db) < 6:
+            raise ValueError("DBSPH section missing or short")
+        t = db[0].split()
+        if _logical(t[1]) or _int(t[3]) != 0:
+            raise ValueError("DB-SPH MUSCL boundary terms and slip_ID > 0 are not read")
+        n_mon = [_int(v) for v in db[2].split()[:2]]
+        n_kin = _int(db[4].split()[0])
+        n_io = [_int(v) for v in db[5].split()[:2]]
+        if any(n_mon) or n_kin or any(n_io) or _int(db[3].split()[0]) != 1:
+            raise ValueError("DB-SPH monitors, kinematics, inlets/outlets and several meshes are not read")
+        neg, fs = (_logical(v) for v in db[1].split()[:2])
+        dbsph = DBSPHParams(dx_dxw=_num(t[0]), k_w=_num(t[2]), gamma_limiter=_logical(t[4]),
+                            negative_wall_p_allowed=neg, fs_allowed=fs)
+        if time_split != 0:
+            raise ValueError("DB-SPH walls run with explicit RK1 (time_split = 0)")
+    return DeckCase(dbsph=dbsph, name=name or title[0].split()[0], dx=dx, trunc=trunc, rho0=rho0, bulk=bulk,
                     alfa_mon=alfa_mon, mu=mu, grav=grav, p_ref=p_ref, cfl=cfl, vsc_coeff=vsc,
                     pesodt=pesodt, teta_p=tetap, teta_v=tetav, coefnmaxpartj=coefj,
                     maxcloseboundfaces=maxclose, slip_coeff=slip, density_thresholds=dthr,
                     time_split=time_split, pool=pool, slip_mode=mode, roughness=rough, verts=verts, face_rows=rows, zones=zones,
                     fluid_zone=fluid_zone, control_lines=tuple(lines))
+
+
+def read_case_dir(root, name=None):
+    """A case directory as the reference starts from it: the deck of
+    input/00_deck/*.inp, and for DB-SPH walls the surface meshes listed in
+    input/07_DBSPH/surface_mesh_list.txt (one mesh file is read)."""
+    from pathlib import Path
+    root = Path(root)
+    decks = sorted((root / "input").glob("*/*.inp"))
+    if not decks:
+        raise ValueError(f"no deck under {root / 'input'}")
+    case = parse_deck("".join(p.read_text() for p in decks), name=name)
+    if case.dbsph is not None:
+        from .dbsph import read_ply
+        lst = (root / "input" / "07_DBSPH" / "surface_mesh_list.txt").read_text().split()
+        case.mesh = read_ply(root / "input" / "07_DBSPH" / lst[1])
+    return case

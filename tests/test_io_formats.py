@@ -52,6 +52,8 @@ def test_deck_round_trip_builds_the_same_tables(kw):
 def test_parser_refuses_what_it_does_not_read():
     text = deck.deck_text(DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01))
     with pytest.raises(ValueError, match="semi"):
+        deck.parse_deck(text.replace("\nsemi\n", "\nfull\n"))
+    with pytest.raises(ValueError, match="DBSPH"):  # a bsph deck without a surface mesh
         deck.parse_deck(text.replace("\nsemi\n", "\nbsph\n"))
     with pytest.raises(ValueError, match="10.0.0"):
         deck.parse_deck(text.replace("10.0.0", "9.0.0", 1))
@@ -174,3 +176,25 @@ def test_vtu_equals_the_reference_program(tmp_path):
         fh = io.StringIO()
         results.write_vtu(fh, st, case.scalars()["kin_visc"], st["laminar_flag"])
         assert fh.getvalue() == (wd / f"VTKConverter_vt_block_{blk}.vtu").read_text(), blk
+
+
+def test_dbsph_case_directory_round_trip(tmp_path):
+    """A DB-SPH case written as the reference reads it (the bsph deck, the
+    tank's .ply mesh and surface_mesh_list.txt) and read back by
+    deck.read_case_dir builds the same particles, grid and wall elements,
+    with the DBSPH section's flags."""
+    from oracle.decks import write_case_dir
+    from sphera_amd.dbsph import DBSPHParams, read_ply, tank_mesh
+    case = DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="rtdb", time_split=0,
+                      dbsph=DBSPHParams(gamma_limiter=True, fs_allowed=False, negative_wall_p_allowed=True))
+    write_case_dir(case, tmp_path)
+    v, f = read_ply(tmp_path / "input" / "07_DBSPH" / "tank.ply")
+    v0, f0 = tank_mesh(case.tank, case.dx)
+    assert np.array_equal(v, v0) and np.array_equal(np.array(f), f0)
+    d = deck.read_case_dir(tmp_path)
+    assert d.dbsph == case.dbsph and d.time_split == 0
+    a, b = build(case), build(d)
+    for k in ("coord", "dens", "pres", "mass"):
+        assert np.array_equal(a.particles[k], b.particles[k]), k
+    for k in ("coord", "normal", "weight", "volume", "mass", "k_d", "adjacent", "cell"):
+        assert np.array_equal(getattr(a.walls, k), getattr(b.walls, k)), k
