@@ -25,6 +25,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--dx", type=float, default=0.005)
+    ap.add_argument("--rows-json", default=None, help="also write the rows as JSON to this file")
     a = ap.parse_args()
     tb, coord = longrun.make_case(a.dx)
     o = Oracle(tb, coord=coord)
@@ -53,8 +54,8 @@ def main():
         print(f"| {r['it']} | {'yes' if r['fp64_equals_oracle'] else 'NO'} | " +
               " | ".join(f"{r['oracle_' + f + '_max_rel']:.1e} / {r['oracle_' + f + '_rms_rel']:.1e}"
                          for f in longrun.FIELDS) + " |")
-    if Path("gpurun_out").is_dir():
-        with open(Path("gpurun_out") / "longrun_rows.json", "w") as fh:
+    if a.rows_json:
+        with open(Path(a.rows_json), "w") as fh:
             json.dump(rows, fh)
 
 
