@@ -106,6 +106,7 @@ struct sph_ctx {
     // the fluid-wall lists and the MUSCL gradients
     double4 *d_w0 = nullptr, *d_w1 = nullptr, *d_w2 = nullptr;
     int *d_wstart = nullptr, *d_wids = nullptr, *d_wcell = nullptr;
+    double4* d_wpos = nullptr;
     double *d_wdens = nullptr, *d_wpres = nullptr;
     int* d_wwet = nullptr;
     double *gam[2] = {}, *dsh[2] = {}, *sig = nullptr, *rsph = nullptr, *drho = nullptr, *dvelg = nullptr;
@@ -365,23 +366,23 @@ int run_psmooth(sph_ctx* c) {
 // DB-SPH walls (one FP64 solve): CalcVarLength's DB-SPH sums in mode 0/1
 // (pre-loop) or 2 (Euler, after the sort: the state in rp[1], the new density
 // and pressure into rp[2])
-DWall dwall(const sph_ctx* c) { return DWall{c->d_w0, c->d_w1, c->d_w2, c->d_wstart, c->d_wids, c->d_wcell}; }
+DWall dwall(const sph_ctx* c) { return DWall{c->d_w0, c->d_w1, c->d_w2, c->d_wstart, c->d_wids, c->d_wcell, c->d_wpos}; }
 
 int run_dbsph_lists(sph_ctx* c, int mode) {
     KTime t(c, "dbsph_lists");
     const int cap = c->P.cap;
     if (mode == 0)
-        k_dbsph_lists<0><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->cell,
+        k_dbsph_lists<0><<<nblk(DB_G * cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->cell,
                                                            c->nbr, c->nnb, c->gam[0], c->dsh[0], c->gam[0], c->dsh[0],
                                                            c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
                                                            c->dden2, c->rp[2], c->scal, c->iscal);
     else if (mode == 1)
-        k_dbsph_lists<1><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->cell,
+        k_dbsph_lists<1><<<nblk(DB_G * cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->cell,
                                                            c->nbr, c->nnb, c->gam[0], c->dsh[0], c->gam[0], c->dsh[0],
                                                            c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
                                                            c->dden2, c->rp[2], c->scal, c->iscal);
     else
-        k_dbsph_lists<2><<<nblk(cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[1], c->cell,
+        k_dbsph_lists<2><<<nblk(DB_G * cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[1], c->cell,
                                                            c->nbr, c->nnb, c->gam[1], c->dsh[1], c->gam[0], c->dsh[0],
                                                            c->sig, c->rsph, c->fsf, c->nfw, c->fwl, c->d_wwet,
                                                            c->dden2, c->rp[2], c->scal, c->iscal);
@@ -390,7 +391,7 @@ int run_dbsph_lists(sph_ctx* c, int mode) {
 
 int run_momentum_dbsph(sph_ctx* c) {
     KTime t(c, "momentum");
-    k_momentum_dbsph<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->nbr,
+    k_momentum_dbsph<<<nblk(DB_G * c->P.cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->nbr,
                                                             c->nnb, c->nfw, c->fwl, c->gam[0], c->d_wdens,
                                                             c->d_wpres, c->lam[0], c->acc[1], c->vel[1], c->scal,
                                                             c->iscal);
@@ -410,7 +411,7 @@ int run_dbsph_walls(sph_ctx* c) {
     KTime t(c, "dbsph_walls");
     k_dbsph_muscl<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->xm[0], c->vel[0], c->rp[0], c->nbr, c->nnb,
                                                          c->nfw, c->drho, c->dvelg, c->iscal);
-    k_dbsph_walls<<<nblk(c->P.nw), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->start,
+    k_dbsph_walls<<<(c->P.nw + DB_WPB - 1) / DB_WPB, 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->start,
                                                         c->drho, c->dvelg, c->d_wdens, c->d_wpres, c->iscal);
     return 0;
 }
@@ -468,7 +469,7 @@ sph_ctx::~sph_ctx() {
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     F(mon_icont); F(mon_pts); F(mon_out);
-    F(d_w0); F(d_w1); F(d_w2); F(d_wstart); F(d_wids); F(d_wcell); F(d_wdens); F(d_wpres); F(d_wwet);
+    F(d_w0); F(d_w1); F(d_w2); F(d_wstart); F(d_wids); F(d_wcell); F(d_wpos); F(d_wdens); F(d_wpres); F(d_wwet);
     for (auto p : gam) F(p);
     for (auto p : dsh) F(p);
     F(sig); F(rsph); F(drho); F(dvelg); F(fsf); F(nfw); F(fwl);
@@ -1108,6 +1109,11 @@ int sph_dbsph_walls(sph_ctx* c, const sph_dbsph_walls_t* w) {
         wids[wstart[cc - 1] + fill[cc]] = k;
         fill[cc] += 1;
     }
+    std::vector<double4> hpos(nw);
+    for (int w2 = 0; w2 < nw; ++w2) {
+        const double* x = w->coord + 3 * wids[w2];
+        hpos[w2] = make_double4(x[0], x[1], x[2], (double)wids[w2]);
+    }
     std::vector<double4> h0(nw), h1(nw), h2(nw);
     for (int k = 0; k < nw; ++k) {
         h0[k] = make_double4(w->coord[3 * k], w->coord[3 * k + 1], w->coord[3 * k + 2], w->weight[k]);
@@ -1117,7 +1123,7 @@ int sph_dbsph_walls(sph_ctx* c, const sph_dbsph_walls_t* w) {
     const size_t N = P.cap;
     bool ok = dalloc(&c->d_w0, nw) == hipSuccess && dalloc(&c->d_w1, nw) == hipSuccess &&
               dalloc(&c->d_w2, nw) == hipSuccess && dalloc(&c->d_wstart, P.nmax + 1) == hipSuccess &&
-              dalloc(&c->d_wids, nw) == hipSuccess && dalloc(&c->d_wcell, nw) == hipSuccess &&
+              dalloc(&c->d_wids, nw) == hipSuccess && dalloc(&c->d_wcell, nw) == hipSuccess && dalloc(&c->d_wpos, nw) == hipSuccess &&
               dalloc(&c->d_wdens, nw) == hipSuccess && dalloc(&c->d_wpres, nw) == hipSuccess &&
               dalloc(&c->d_wwet, nw) == hipSuccess;
     for (auto& p : c->gam) ok = ok && dalloc(&p, N) == hipSuccess;
@@ -1136,6 +1142,7 @@ int sph_dbsph_walls(sph_ctx* c, const sph_dbsph_walls_t* w) {
     HIPC(hipMemcpy(c->d_wstart, wstart.data(), sizeof(int) * (P.nmax + 1), hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d_wids, wids.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d_wcell, wcell.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_wpos, hpos.data(), sizeof(double4) * nw, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d_wdens, d0.data(), sizeof(double) * nw, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d_wpres, p0.data(), sizeof(double) * nw, hipMemcpyHostToDevice));
     HIPC(hipMemset(c->d_wwet, 0, sizeof(int) * nw));
