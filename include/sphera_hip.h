@@ -174,6 +174,9 @@ typedef struct sph_dbsph_walls_t {
     int32_t nw;
     const double *coord, *normal, *vel, *weight, *volume, *mass;
     int32_t gamma_limiter, fs_allowed, negative_wall_p_allowed;
+    /* DBSPH%MUSCL_boundary_flag: the wall elements' terms in the particles'
+     * MUSCL gradients (Gradients_to_MUSCL_boundary.f90) */
+    int32_t muscl_boundary;
 } sph_dbsph_walls_t;
 int sph_dbsph_walls(sph_ctx* ctx, const sph_dbsph_walls_t* walls);
 /* DB-SPH state at a time-step boundary, particles in pg order (any pointer

@@ -206,7 +206,8 @@ class Oracle:
             W = tb.walls
             self._wk = [np.ascontiguousarray(a, np.float64).reshape(-1)
                         for a in (W.coord, W.normal, W.vel, W.weight, W.volume, W.mass)]
-            self._wflags = np.array([db.gamma_limiter, db.fs_allowed, db.negative_wall_p_allowed], np.int32)
+            self._wflags = np.array([db.gamma_limiter, db.fs_allowed, db.negative_wall_p_allowed,
+                                     db.muscl_boundary], np.int32)
             if L.or_set_dbsph(self.ctx, len(W), *[_pd(a) for a in self._wk], _pi(self._wflags)):
                 raise ValueError("a DB-SPH wall element lies outside the grid")
 

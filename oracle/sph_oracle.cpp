@@ -75,7 +75,7 @@ struct Ctx {
     int nw = 0;
     std::vector<double> wcoord, wnormal, wvel, wweight, wvolume, wmass, wdens, wpres;
     std::vector<int> wwet, icont_w, npartord_w;
-    int gamma_limiter = 0, fs_allowed = 1, neg_wall_p = 0;
+    int gamma_limiter = 0, fs_allowed = 1, neg_wall_p = 0, muscl_boundary = 0;
     std::vector<double> gamma, sigma, rhosph, rhosph_old, dshep, drho, dvelg;
     std::vector<int> fs, nfw, fwl;
     std::vector<double> rag_fw, kfw;
@@ -1021,6 +1021,27 @@ static void dbsph_walls_state(Ctx& c) {
             for (int r = 0; r < 3; ++r)
                 for (int d = 0; d < 3; ++d)
                     V[3 * r + d] = V[3 * r + d] + R[d] * (c.var[3 * npj + r] - vi[r]) * K0 * vol;
+        }
+        if (!c.muscl_boundary) continue;
+        // Gradients_to_MUSCL_boundary.f90:44-86 (one medium): each wall
+        // neighbour's Shepard term along its normal, then its gradient term
+        const double sg = c.sigma[npi];
+        for (int k = 0; k < c.nfw[npi]; ++k) {
+            const size_t s = (size_t)npi * mj + k;
+            const int w = c.fwl[s];
+            const double* nw = &c.wnormal[3 * w];
+            const double* vw = &c.wvel[3 * w];
+            const double* R = &c.rag_fw[3 * s];
+            const double dr = c.wdens[w] - c.dens[npi];
+            const double Ww = c.wweight[w] * c.kfw[2 * s] / sg;
+            for (int d = 0; d < 3; ++d) g[d] = g[d] + dr * nw[d] * Ww;
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d) V[3 * r + d] = V[3 * r + d] + (vw[r] - vi[r]) * nw[d] * Ww;
+            const double vol = c.wmass[w] / c.wdens[w] / sg;
+            for (int d = 0; d < 3; ++d) g[d] = g[d] + R[d] * dr * c.kfw[2 * s + 1] * vol;
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d)
+                    V[3 * r + d] = V[3 * r + d] + R[d] * (vw[r] - vi[r]) * c.kfw[2 * s + 1] * vol;
         }
     }
     std::vector<double> den(c.nw, 0.0);
@@ -2184,6 +2205,7 @@ int or_set_dbsph(void* ctx, int nw, const double* coord, const double* normal, c
     c.gamma_limiter = flags[0];
     c.fs_allowed = flags[1];
     c.neg_wall_p = flags[2];
+    c.muscl_boundary = flags[3];
     const int n = c.n;
     c.gamma.assign(n, 0.0);
     c.sigma.assign(n, 0.0);

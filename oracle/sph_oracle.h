@@ -122,7 +122,7 @@ int or_overflow(void* ctx);
 /* DB-SPH walls (Domain%tipo = "bsph", 3D, explicit RK1): nw wall elements with
  * centroid, unit normal into the fluid, velocity (xyz interleaved), area
  * ("weight"), semi-particle volume and mass; flags = {Gamma_limiter_flag,
- * FS_allowed, negative_wall_p_allowed}.  Called once after or_create (with
+ * FS_allowed, negative_wall_p_allowed, MUSCL_boundary_flag}.  Called once after or_create (with
  * semi = 0 in or_params); or_preloop and or_step then run the DB-SPH branch.
  * Returns 2 if a wall element lies outside the grid. */
 int or_set_dbsph(void* ctx, int nw, const double* coord, const double* normal, const double* vel,

@@ -409,8 +409,9 @@ int run_psmooth_dbsph(sph_ctx* c) {
 // Gradients_to_MUSCL and BC_wall_elements on the step's final state
 int run_dbsph_walls(sph_ctx* c) {
     KTime t(c, "dbsph_walls");
-    k_dbsph_muscl<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, c->xm[0], c->vel[0], c->rp[0], c->nbr, c->nnb,
-                                                         c->nfw, c->drho, c->dvelg, c->iscal);
+    k_dbsph_muscl<<<nblk(c->P.cap), 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->nbr,
+                                                         c->nnb, c->nfw, c->fwl, c->sig, c->d_wdens, c->drho,
+                                                         c->dvelg, c->iscal);
     k_dbsph_walls<<<(c->P.nw + DB_WPB - 1) / DB_WPB, 256, 0, c->stream>>>(c->P, dwall(c), c->xm[0], c->vel[0], c->rp[0], c->start,
                                                         c->drho, c->dvelg, c->d_wdens, c->d_wpres, c->iscal);
     return 0;
@@ -1155,6 +1156,7 @@ int sph_dbsph_walls(sph_ctx* c, const sph_dbsph_walls_t* w) {
     P.gamma_lim = w->gamma_limiter ? 1 : 0;
     P.fs_allowed = w->fs_allowed ? 1 : 0;
     P.neg_wall_p = w->negative_wall_p_allowed ? 1 : 0;
+    P.muscl_bnd = w->muscl_boundary ? 1 : 0;
     return 0;
 }
 

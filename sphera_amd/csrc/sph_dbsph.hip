@@ -9,7 +9,8 @@
 //                     density from rhoSPH_new / Gamma with the EoS
 //   k_momentum_dbsph  RHS_momentum_equation.f90:173-177 with inter_EqMoto.f90
 //                     (DB-SPH pressure term :229-235, wall terms :286-322)
-//   k_dbsph_muscl     Gradients_to_MUSCL.f90:47-91
+//   k_dbsph_muscl     Gradients_to_MUSCL.f90:47-91 (with the wall terms of
+//                     Gradients_to_MUSCL_boundary.f90 when asked for)
 //   k_dbsph_walls     BC_wall_elements.f90:71-158
 //
 // Wall elements do not move.  Their cell lists are built once on the host
@@ -347,11 +348,13 @@ __global__ __launch_bounds__(256) void k_momentum_dbsph(
 // Gradients_to_MUSCL.f90:47-91 for the particles with wall neighbours:
 // drho = sum r (rho_j - rho_i) PartKernel(1) m_j/rho_j and dvel(r, :) the
 // same of var(r); drho[3 i + d], dvel[9 i + 3 r + d]
-__global__ __launch_bounds__(256) void k_dbsph_muscl(DP P, const double4* __restrict__ xm,
+__global__ __launch_bounds__(256) void k_dbsph_muscl(DP P, DWall W, const double4* __restrict__ xm,
                                                      const double4* __restrict__ vel,
                                                      const double2* __restrict__ rp,
                                                      const uint32_t* __restrict__ nbr,
                                                      const int* __restrict__ nnb, const int* __restrict__ nfw,
+                                                     const int* __restrict__ fwl, const double* __restrict__ sig,
+                                                     const double* __restrict__ wdens,
                                                      double* __restrict__ drho, double* __restrict__ dvel,
                                                      const int* iscal) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -376,6 +379,32 @@ __global__ __launch_bounds__(256) void k_dbsph_muscl(DP P, const double4* __rest
         for (int r = 0; r < 3; ++r)
             for (int d = 0; d < 3; ++d) V[3 * r + d] = V[3 * r + d] + R[d] * dv[r] * K.K1 * vol;
     });
+    if (P.muscl_bnd) {  // Gradients_to_MUSCL_boundary.f90:44-86 (one medium)
+        const double sg = sig[i];
+        const int nf = nfw[i];
+        const double v[3] = {vi.x, vi.y, vi.z};
+        for (int k = 0; k < nf; ++k) {
+            const int id = fwl[(size_t)k * P.cap + i];
+            const double4 a = W.w0[id];
+            const double4 b = W.w1[id];
+            const double4 wv = W.w2[id];
+            const double wd = wdens[id];
+            const double R[3] = {xi.x - a.x, xi.y - a.y, xi.z - a.z};
+            K64 K;  // kernel_fw(1) = W, kernel_fw(2) = K1
+            kvals64(P, R[0] * R[0] + R[1] * R[1] + R[2] * R[2], true, false, true, K);
+            const double nw[3] = {b.x, b.y, b.z};
+            const double vw[3] = {wv.x, wv.y, wv.z};
+            const double dr = wd - rhoi;
+            const double Ww = a.w * K.W / sg;
+            for (int d = 0; d < 3; ++d) g[d] = g[d] + dr * nw[d] * Ww;
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d) V[3 * r + d] = V[3 * r + d] + (vw[r] - v[r]) * nw[d] * Ww;
+            const double vol = wv.w / wd / sg;
+            for (int d = 0; d < 3; ++d) g[d] = g[d] + R[d] * dr * K.K1 * vol;
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d) V[3 * r + d] = V[3 * r + d] + R[d] * (vw[r] - v[r]) * K.K1 * vol;
+        }
+    }
     for (int d = 0; d < 3; ++d) drho[3 * (size_t)i + d] = g[d];
     for (int d = 0; d < 9; ++d) dvel[9 * (size_t)i + d] = V[d];
 }

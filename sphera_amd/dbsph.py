@@ -37,16 +37,17 @@ from .case import PIGRECO, Grid, cell_number
 @dataclass
 class DBSPHParams:
     """The DBSPH deck section (ReadDBSPH.f90:35-99) for walls without
-    kinematics, monitors, inlets or outlets."""
+    kinematics, monitors, inlets or outlets, with slip_ID = 0."""
     dx_dxw: float = 1.0                 # fluid spacing / wall-element spacing
     k_w: float = 0.5                    # semi-particle coefficient
     gamma_limiter: bool = False         # Gamma <= 1
     negative_wall_p_allowed: bool = False
     fs_allowed: bool = True             # free-surface detection via sigma
+    muscl_boundary: bool = False        # wall terms in the MUSCL gradients
 
     def deck_lines(self, nfiles=1):
         b = lambda v: ".true." if v else ".false."  # noqa: E731
-        return [f"{float(self.dx_dxw)!r} .false. {float(self.k_w)!r} 0 {b(self.gamma_limiter)}",
+        return [f"{float(self.dx_dxw)!r} {b(self.muscl_boundary)} {float(self.k_w)!r} 0 {b(self.gamma_limiter)}",
                 f"{b(self.negative_wall_p_allowed)} {b(self.fs_allowed)}", "0 0",
                 f"{int(nfiles)} .false.", "0 0. 0. 0.", "0 0 4"]
 

@@ -392,8 +392,8 @@ def parse_deck(text, name=None):
         if len(db) < 6:
             raise ValueError("DBSPH section missing or short")
         t = db[0].split()
-        if _logical(t[1]) or _int(t[3]) != 0:
-            raise ValueError("DB-SPH MUSCL boundary terms and slip_ID > 0 are not read")
+        if _int(t[3]) != 0:
+            raise ValueError("DB-SPH slip_ID > 0 (the VSL wall model) is not read")
         n_mon = [_int(v) for v in db[2].split()[:2]]
         n_kin = _int(db[4].split()[0])
         n_io = [_int(v) for v in db[5].split()[:2]]
@@ -401,7 +401,7 @@ def parse_deck(text, name=None):
             raise ValueError("DB-SPH monitors, kinematics, inlets/outlets and several meshes are not read")
         neg, fs = (_logical(v) for v in db[1].split()[:2])
         dbsph = DBSPHParams(dx_dxw=_num(t[0]), k_w=_num(t[2]), gamma_limiter=_logical(t[4]),
-                            negative_wall_p_allowed=neg, fs_allowed=fs)
+                            negative_wall_p_allowed=neg, fs_allowed=fs, muscl_boundary=_logical(t[1]))
         if time_split != 0:
             raise ValueError("DB-SPH walls run with explicit RK1 (time_split = 0)")
     return DeckCase(dbsph=dbsph, name=name or title[0].split()[0], dx=dx, trunc=trunc, rho0=rho0, bulk=bulk,

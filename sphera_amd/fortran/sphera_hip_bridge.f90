@@ -23,8 +23,7 @@
 ! faces or sides, open faces (3D) and open sides (2D) of type "open" only, no
 ! dt_alfa_Mon, fluid particles only; or Domain%tipo="bsph" (3D DB-SPH walls,
 ! pg_w passed to sph_dbsph_walls) with explicit RK1, FP64, slip_ID = 0, no
-! MUSCL boundary terms, no DB-SPH inlets, outlets, kinematics or fictitious
-! reservoirs.
+! DB-SPH inlets, outlets, kinematics or fictitious reservoirs.
 module sphera_hip_bridge
 use iso_c_binding
 use sphera_hip_c
@@ -67,13 +66,13 @@ end subroutine sph_hip_check
 
 #ifdef SPACE_3D
 ! True when the DB-SPH case uses what the device does not build: the VSL
-! wall model (slip_ID > 0), MUSCL boundary terms, inlet or outlet sections,
+! wall model (slip_ID > 0), inlet or outlet sections,
 ! moving walls (kinematics records) or fictitious reservoirs / reservoirs
 ! from topography (removed in the pre-loop by
 ! removing_DBSPH_fictitious_reservoirs.f90)
 logical function dbsph_outside_scope()
 integer(4) :: k
-dbsph_outside_scope = (DBSPH%slip_ID/=0).or.DBSPH%MUSCL_boundary_flag.or.      &
+dbsph_outside_scope = (DBSPH%slip_ID/=0).or.                                  &
    (DBSPH%n_inlet/=0).or.(DBSPH%n_outlet/=0).or.(Domain%time_split/=0).or.      &
    (Domain%RKscheme/=1)
 if (allocated(DBSPH%n_kinematics_records)) then
@@ -397,6 +396,7 @@ w%mass = c_loc(wm)
 w%gamma_limiter = merge(1,0,DBSPH%Gamma_limiter_flag)
 w%fs_allowed = merge(1,0,DBSPH%FS_allowed)
 w%negative_wall_p_allowed = merge(1,0,DBSPH%negative_wall_p_allowed)
+w%muscl_boundary = merge(1,0,DBSPH%MUSCL_boundary_flag)
 end subroutine pack_walls
 #endif
 
@@ -444,7 +444,7 @@ call sph_hip_check(sph_ctx_create2d(p,sd,ctx),"sph_ctx_create2d")
 if (Domain%tipo=="bsph") then
    if (dbsph_outside_scope()) then
       write(*,*) "sphera_hip: DB-SPH case outside the device scope (RK1, ",         &
-         "slip_ID 0, static walls, no inlets/outlets/MUSCL boundary terms)"
+         "slip_ID 0, static walls, no inlets/outlets)"
       stop 1
    endif
    call empty_faces(f)

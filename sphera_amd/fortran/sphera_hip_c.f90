@@ -43,6 +43,7 @@ type,bind(C) :: sph_dbsph_walls_t
    integer(c_int32_t) :: nw
    type(c_ptr) :: coord, normal, vel, weight, volume, mass
    integer(c_int32_t) :: gamma_limiter, fs_allowed, negative_wall_p_allowed
+   integer(c_int32_t) :: muscl_boundary
 end type sph_dbsph_walls_t
 
 ! 2D boundary sides (sph_sides2d of sphera_hip.h)

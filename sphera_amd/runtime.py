@@ -76,7 +76,7 @@ class SphSides2D(C.Structure):
 class SphDBSPHWalls(C.Structure):
     _fields_ = [("nw", C.c_int32), ("coord", PD), ("normal", PD), ("vel", PD), ("weight", PD),
                 ("volume", PD), ("mass", PD), ("gamma_limiter", C.c_int32), ("fs_allowed", C.c_int32),
-                ("negative_wall_p_allowed", C.c_int32)]
+                ("negative_wall_p_allowed", C.c_int32), ("muscl_boundary", C.c_int32)]
 
 
 _lib = None
@@ -286,7 +286,7 @@ class SphContext:
             self._wkeep = [np.ascontiguousarray(a, np.float64).reshape(-1)
                            for a in (W.coord, W.normal, W.vel, W.weight, W.volume, W.mass)]
             ws = SphDBSPHWalls(len(W), *[_pd(a) for a in self._wkeep], int(db.gamma_limiter),
-                               int(db.fs_allowed), int(db.negative_wall_p_allowed))
+                               int(db.fs_allowed), int(db.negative_wall_p_allowed), int(db.muscl_boundary))
             self._check(self.L.sph_dbsph_walls(self.h, C.byref(ws)), "sph_dbsph_walls")
             return
         if getattr(tb, "dim", 3) == 2:

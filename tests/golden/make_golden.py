@@ -476,13 +476,16 @@ def dbsph_ic(coord, dx, seed=20261207):
 
 DBSPH_OPTS = {"ref_dbsph.npz": dict(),
               "ref_dbsph_opts.npz": dict(gamma_limiter=True, negative_wall_p_allowed=True,
-                                         fs_allowed=False)}
+                                         fs_allowed=False),
+              "ref_dbsph_muscl.npz": dict(muscl_boundary=True)}
 
 
-def make_dbsph():
-    """DB-SPH walls: a tank mesh, RK1"""
+def make_dbsph(names=None):
+    """DB-SPH walls: a tank mesh, RK1 (names: a subset of DBSPH_OPTS)"""
     from sphera_amd.dbsph import DBSPHParams
     for name, opts in DBSPH_OPTS.items():
+        if names is not None and name not in names:
+            continue
         case = DamBreak3D(**TINY, time_split=0, dbsph=DBSPHParams(**opts))
         tb = build(case)
         ic, iv = dbsph_ic(tb.particles["coord"], case.dx)

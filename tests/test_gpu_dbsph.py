@@ -37,7 +37,7 @@ def _ctx(tb, coord=None, vel=None):
     return d
 
 
-@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz"])
+@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz", "ref_dbsph_muscl.npz"])
 def test_dbsph_matches_reference_fixture(name):
     """The reference's DB-SPH run (700 particles in a 3,800-element tank
     mesh, jittered and moving): the device equals it bit for bit after the

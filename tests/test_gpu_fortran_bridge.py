@@ -167,17 +167,19 @@ def test_bridge_2d_open_side(tmp_path):
             assert err <= FP64_REL_TOL, (it, f, err)
 
 
-def test_bridge_dbsph(tmp_path):
+@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_muscl.npz"])
+def test_bridge_dbsph(tmp_path, name):
     """The reference program with DB-SPH walls (Domain%tipo = "bsph", RK1)
     driving the device through the bridge: the bridge passes pg_w, as the
     reference's own pre-processing built it from the tank's .ply mesh, to
     sph_dbsph_walls; count and fields after 10 and 20 steps equal the
-    reference's own run (tests/golden/ref_dbsph.npz) bit for bit."""
+    reference's own run (tests/golden/ref_dbsph*.npz) bit for bit, without
+    and with the MUSCL wall terms."""
     from sphera_amd.dbsph import DBSPHParams
-    from tests.golden.make_golden import TINY
-    g = np.load(GOLD / "ref_dbsph.npz")
-    case = DamBreak3D(**TINY, time_split=0, dbsph=DBSPHParams())
-    print("[bridge] dbsph steps=20", flush=True)
+    from tests.golden.make_golden import DBSPH_OPTS, TINY
+    g = np.load(GOLD / name)
+    case = DamBreak3D(**TINY, time_split=0, dbsph=DBSPHParams(**DBSPH_OPTS[name]))
+    print(f"[bridge] dbsph {name} steps=20", flush=True)
     hdr, steps = read_dump(run_harness(case, nsteps=20, every=10, lists=False,
                                        ic_coord=g["ic_coord"], ic_vel=g["ic_vel"],
                                        workdir=tmp_path, exe=HARNESS_HIP, timeout=120,

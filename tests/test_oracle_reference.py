@@ -509,7 +509,7 @@ def test_dbsph_wall_elements_equal_reference():
     assert (W.k_d[edge] > 1.2).all()
 
 
-@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz"])
+@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz", "ref_dbsph_muscl.npz"])
 def test_golden_dbsph_bit_identical(name):
     """DB-SPH walls (Domain%tipo = "bsph", RK1): the oracle's restatement of
     the fluid-wall lists and Shepard sums (CalcVarLength.f90:462-540,
@@ -522,7 +522,9 @@ def test_golden_dbsph_bit_identical(name):
     free-surface flags, the wall elements' density, pressure and wet flags,
     and the fluid-wall lists in the reference's order (fluid lists at the
     pre-loop).  The second fixture runs the Gamma limiter, negative wall
-    pressures and no free-surface detection."""
+    pressures and no free-surface detection; the third the wall elements'
+    terms in the MUSCL gradients (Gradients_to_MUSCL_boundary.f90), whose
+    gradients and wall pressures differ from the first's at every step."""
     g = np.load(GOLD / name)
     o = Oracle(build(_dbsph_case(name)), coord=g["ic_coord"], vel=g["ic_vel"])
     o.preloop()
@@ -544,5 +546,10 @@ def test_golden_dbsph_bit_identical(name):
         assert np.array_equal(np.concatenate([idw[i, :cw[i]] for i in range(len(cw))]) + 1, g[f"s{it}_fw"])
     if name == "ref_dbsph.npz":
         assert g["s20_fs"].sum() > 0 and (g["s20_wall_pres"] > 0).sum() > 100
-    else:
+    elif name == "ref_dbsph_opts.npz":
         assert g["s20_gamma"].max() == 1.0 and (g["s20_wall_pres"] < 0).any()
+    else:  # the wall terms change the gradients and the wall state
+        p = np.load(GOLD / "ref_dbsph.npz")
+        for it in (10, 20):
+            for f in ("drho", "dvel", "wall_pres"):
+                assert not np.allclose(g[f"s{it}_" + f], p[f"s{it}_" + f], rtol=1e-3, atol=0), (it, f)
