@@ -179,6 +179,15 @@ typedef struct sph_dbsph_walls_t {
     int32_t muscl_boundary;
 } sph_dbsph_walls_t;
 int sph_dbsph_walls(sph_ctx* ctx, const sph_dbsph_walls_t* walls);
+/* Moving DB-SPH walls (DBSPH_kinematics.f90, Euler.f90:152-159): nrec
+ * records (t, vx, vy, vz, wx, wy, wz) of the mesh, the first at t <= 0, and
+ * its rotation centre.  Each step the wall elements take the velocity
+ * omega x (x - centre) + v at the step's time and move by dt times it; they
+ * keep their cells, as the reference bins them once.  nrec = 0: static walls.
+ * After sph_dbsph_walls, before sph_preloop.  Replaces the DBSPH%kinematics
+ * and DBSPH%rotation_centre that ReadDBSPH.f90:115-141 reads. */
+#define SPH_MAX_KIN 4096
+int sph_dbsph_kinematics(sph_ctx* ctx, int32_t nrec, const double* records, const double* centre);
 /* DB-SPH state at a time-step boundary, particles in pg order (any pointer
  * may be NULL): Gamma, sigma, rhoSPH_new, dShep, FS; the wall elements'
  * density, pressure and wet flags (pg_w%dens, %pres, %wet). */

@@ -98,6 +98,8 @@ def lib():
         L.or_free_surface.argtypes = [C.c_void_p, C.c_int, PI, C.c_int, PD, PD, PD]
         L.or_set_dbsph.argtypes = [C.c_void_p, C.c_int] + [PD] * 6 + [PI]
         L.or_set_dbsph.restype = C.c_int
+        L.or_set_dbsph_kinematics.argtypes = [C.c_void_p, C.c_int, PD, PD]
+        L.or_set_dbsph_kinematics.restype = C.c_int
         L.or_get_dbsph.argtypes = [C.c_void_p] + [PD] * 6 + [PI, PD, PD, PI]
         L.or_get_dbsph.restype = None
         L.or_get_fw_lists.argtypes = [C.c_void_p, PI, PI, C.c_int32]
@@ -210,6 +212,11 @@ class Oracle:
                                      db.muscl_boundary], np.int32)
             if L.or_set_dbsph(self.ctx, len(W), *[_pd(a) for a in self._wk], _pi(self._wflags)):
                 raise ValueError("a DB-SPH wall element lies outside the grid")
+            if db.kinematics:
+                self._kin = np.ascontiguousarray(db.kinematics, np.float64).reshape(-1, 7)
+                self._kc = np.ascontiguousarray(db.rotation_centre, np.float64)
+                if L.or_set_dbsph_kinematics(self.ctx, len(self._kin), _pd(self._kin), _pd(self._kc)):
+                    raise ValueError("DB-SPH kinematics: the first record's time exceeds 0")
 
     def __del__(self):
         if getattr(self, "ctx", None) and _lib is not None:

@@ -76,6 +76,8 @@ struct Ctx {
     std::vector<double> wcoord, wnormal, wvel, wweight, wvolume, wmass, wdens, wpres;
     std::vector<int> wwet, icont_w, npartord_w;
     int gamma_limiter = 0, fs_allowed = 1, neg_wall_p = 0, muscl_boundary = 0;
+    std::vector<double> kin;  // kinematics records (t, v, omega) of the one mesh
+    double kin_centre[3] = {0.0, 0.0, 0.0};
     std::vector<double> gamma, sigma, rhosph, rhosph_old, dshep, drho, dvelg;
     std::vector<int> fs, nfw, fwl;
     std::vector<double> rag_fw, kfw;
@@ -2100,6 +2102,55 @@ static int step_rk1(void* ctx) {
 // dens = rhoSPH_new / Gamma (/ sigma at the free surface, not below
 // 0.98 rho0); EoS_fluid_loop, PressureSmoothing_3D, and the wall elements'
 // state (:470-473)
+// DBSPH_kinematics.f90:52-107 and Euler.f90:152-159: the mesh's velocity and
+// angular velocity at simulation_time from its records (the first record at or
+// after it, linearly interpolated from the one before); each wall element's
+// velocity omega x (x - centre) + v; then every element moves by dt vel.  The
+// elements keep their cells (OrdGrid1.f90:108 bins them on the first step only).
+static void dbsph_move_walls(Ctx& c) {
+    const int nrec = (int)(c.kin.size() / 7);
+    if (nrec > 0) {
+        double va[3] = {0.0, 0.0, 0.0}, om[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < nrec; ++j) {
+            const double* r = &c.kin[7 * j];
+            if (r[0] >= c.time) {
+                if (r[0] == c.time) {
+                    for (int d = 0; d < 3; ++d) {
+                        va[d] = r[1 + d];
+                        om[d] = r[4 + d];
+                    }
+                } else {
+                    const double* q = &c.kin[7 * (j - 1)];
+                    for (int d = 0; d < 3; ++d) {
+                        va[d] = q[1 + d] + (r[1 + d] - q[1 + d]) / (r[0] - q[0]) * (c.time - q[0]);
+                        om[d] = q[4 + d] + (r[4 + d] - q[4 + d]) / (r[0] - q[0]) * (c.time - q[0]);
+                    }
+                }
+                break;
+            }
+        }
+        for (int w = 0; w < c.nw; ++w) {
+            double rel[3], v[3];
+            for (int d = 0; d < 3; ++d) rel[d] = c.wcoord[3 * w + d] - c.kin_centre[d];
+            v[0] = om[1] * rel[2] - om[2] * rel[1];
+            v[1] = om[2] * rel[0] - om[0] * rel[2];
+            v[2] = om[0] * rel[1] - om[1] * rel[0];
+            for (int d = 0; d < 3; ++d) c.wvel[3 * w + d] = v[d] + va[d];
+        }
+    }
+    for (int w = 0; w < c.nw; ++w)
+        for (int d = 0; d < 3; ++d) c.wcoord[3 * w + d] = c.wcoord[3 * w + d] + c.dt * c.wvel[3 * w + d];
+}
+
+int or_set_dbsph_kinematics(void* ctx, int nrec, const double* records, const double* centre) {
+    Ctx& c = *(Ctx*)ctx;
+    if (nrec < 0 || (nrec > 0 && (!records || !centre))) return 1;
+    if (nrec > 0 && records[0] > 0.0) return 2;  // the reference would read before its first record
+    c.kin.assign(records, records + 7 * (size_t)nrec);
+    for (int d = 0; d < 3; ++d) c.kin_centre[d] = nrec > 0 ? centre[d] : 0.0;
+    return 0;
+}
+
 static int step_dbsph(void* ctx) {
     Ctx& c = *(Ctx*)ctx;
     c.it = c.it + 1;
@@ -2114,6 +2165,7 @@ static int step_dbsph(void* ctx) {
         for (int d = 0; d < 3; ++d) c.vel[3 * i + d] = c.vel[3 * i + d] + c.dt * c.acc[3 * i + d];
     or_vsmooth(ctx);
     or_leapfrog_traj(ctx);
+    dbsph_move_walls(c);
     or_ordgrid(ctx);
     for (int i = 0; i < c.n; ++i) {
         c.gamma[i] = c.gamma[i] + c.dshep[i] * c.dt;

@@ -127,6 +127,10 @@ int or_overflow(void* ctx);
  * Returns 2 if a wall element lies outside the grid. */
 int or_set_dbsph(void* ctx, int nw, const double* coord, const double* normal, const double* vel,
                  const double* weight, const double* volume, const double* mass, const int32_t* flags);
+/* Moving DB-SPH walls (DBSPH_kinematics.f90): nrec records (t, vx, vy, vz,
+ * wx, wy, wz) of the mesh and its rotation centre; the first record's time
+ * must not exceed 0.  Returns 2 otherwise. */
+int or_set_dbsph_kinematics(void* ctx, int nrec, const double* records, const double* centre);
 /* DB-SPH state in pg order (any pointer may be NULL): Gamma, sigma, rhoSPH_new,
  * dShep, drho (3), dvel (3x3 row-major), FS; wall elements' dens, pres, wet. */
 void or_get_dbsph(void* ctx, double* gamma, double* sigma, double* rhosph, double* dshep, double* drho,

@@ -107,6 +107,7 @@ struct sph_ctx {
     double4 *d_w0 = nullptr, *d_w1 = nullptr, *d_w2 = nullptr;
     int *d_wstart = nullptr, *d_wids = nullptr, *d_wcell = nullptr;
     double4* d_wpos = nullptr;
+    double* d_kin = nullptr;  // wall kinematics records (t, v, omega)
     double *d_wdens = nullptr, *d_wpres = nullptr;
     int* d_wwet = nullptr;
     double *gam[2] = {}, *dsh[2] = {}, *sig = nullptr, *rsph = nullptr, *drho = nullptr, *dvelg = nullptr;
@@ -406,6 +407,14 @@ int run_psmooth_dbsph(sph_ctx* c) {
     return 0;
 }
 
+// the wall elements' velocity and position in Euler, after the particles'
+int run_dbsph_kinematics(sph_ctx* c) {
+    KTime t(c, "dbsph_kinematics");
+    k_dbsph_kinematics<<<nblk(c->P.nw), 256, 0, c->stream>>>(c->P, c->d_kin, c->d_wids, c->d_w0, c->d_w2,
+                                                             c->d_wpos, c->scal);
+    return 0;
+}
+
 // Gradients_to_MUSCL and BC_wall_elements on the step's final state
 int run_dbsph_walls(sph_ctx* c) {
     KTime t(c, "dbsph_walls");
@@ -470,7 +479,7 @@ sph_ctx::~sph_ctx() {
     F(work); F(nwork);
     F(nbr); F(nnb); F(bn); F(bt); F(scal); F(iscal); F(dtmin); F(pairs);
     F(mon_icont); F(mon_pts); F(mon_out);
-    F(d_w0); F(d_w1); F(d_w2); F(d_wstart); F(d_wids); F(d_wcell); F(d_wpos); F(d_wdens); F(d_wpres); F(d_wwet);
+    F(d_w0); F(d_w1); F(d_w2); F(d_wstart); F(d_wids); F(d_wcell); F(d_wpos); F(d_kin); F(d_wdens); F(d_wpres); F(d_wwet);
     for (auto p : gam) F(p);
     for (auto p : dsh) F(p);
     F(sig); F(rsph); F(drho); F(dvelg); F(fsf); F(nfw); F(fwl);
@@ -1160,6 +1169,28 @@ int sph_dbsph_walls(sph_ctx* c, const sph_dbsph_walls_t* w) {
     return 0;
 }
 
+int sph_dbsph_kinematics(sph_ctx* c, int32_t nrec, const double* records, const double* centre) {
+    if (!c) return fail(1, "null argument");
+    if (!c->P.dbsph) return fail(2, "sph_dbsph_kinematics needs DB-SPH walls (sph_dbsph_walls first)");
+    if (c->phase != PH_EMPTY && c->phase != PH_SORTED)
+        return fail(120, "sph_dbsph_kinematics is called before sph_preloop");
+    if (nrec < 0 || nrec > SPH_MAX_KIN) return fail(2, "kinematics records: 0 to 4096");
+    if (nrec > 0 && (!records || !centre)) return fail(1, "null kinematics array");
+    if (nrec > 0 && !(records[0] <= 0.0))
+        return fail(2, "the first kinematics record's time must not exceed 0 (DBSPH_kinematics.f90 reads before it)");
+    if (c->d_kin) {
+        HIPC(hipFree(c->d_kin));
+        c->d_kin = nullptr;
+    }
+    if (nrec > 0) {
+        if (dalloc(&c->d_kin, 7 * (size_t)nrec) != hipSuccess) return fail(102, "device allocation failed");
+        HIPC(hipMemcpy(c->d_kin, records, sizeof(double) * 7 * nrec, hipMemcpyHostToDevice));
+    }
+    c->P.nkin = nrec;
+    for (int d = 0; d < 3; ++d) c->P.kin_c[d] = nrec > 0 ? centre[d] : 0.0;
+    return 0;
+}
+
 int sph_dbsph_state(sph_ctx* c, double* gamma, double* sigma, double* rhosph, double* dshep, int32_t* fs,
                     double* wall_dens, double* wall_pres, int32_t* wall_wet) {
     if (!c) return fail(1, "null argument");
@@ -1667,6 +1698,7 @@ static void enqueue_steps(sph_ctx* c, int nsteps) {
             run_momentum_dbsph(c);
             run_continuity(c);
             run_vsmooth(c);
+            run_dbsph_kinematics(c);
             run_sort(c);
             run_neighbours(c);
             run_dbsph_lists(c, 2);

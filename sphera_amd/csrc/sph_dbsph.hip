@@ -12,6 +12,7 @@
 //   k_dbsph_muscl     Gradients_to_MUSCL.f90:47-91 (with the wall terms of
 //                     Gradients_to_MUSCL_boundary.f90 when asked for)
 //   k_dbsph_walls     BC_wall_elements.f90:71-158
+//   k_dbsph_kinematics DBSPH_kinematics.f90 and the wall move of Euler.f90:152-159
 //
 // Wall elements do not move.  Their cell lists are built once on the host
 // (OrdGrid1.f90:108-137): wstart[c-1] .. wstart[c]-1 index wids, the wall
@@ -407,6 +408,55 @@ __global__ __launch_bounds__(256) void k_dbsph_muscl(DP P, DWall W, const double
     }
     for (int d = 0; d < 3; ++d) drho[3 * (size_t)i + d] = g[d];
     for (int d = 0; d < 9; ++d) dvel[9 * (size_t)i + d] = V[d];
+}
+
+// DBSPH_kinematics.f90:52-107 and Euler.f90:152-159, one thread per sorted
+// wall slot: with kinematics records (t, v, omega), the mesh's velocity and
+// angular velocity at the step's time (the first record at or after it,
+// linearly interpolated from the one before; zero past the last) give each
+// element omega x (x - centre) + v; then every element moves by dt vel.
+// The elements keep their cells: OrdGrid1.f90:108 bins them on the first
+// step only.
+__global__ __launch_bounds__(256) void k_dbsph_kinematics(DP P, const double* __restrict__ kin,
+                                                          const int* __restrict__ wids, double4* w0,
+                                                          double4* w2, double4* wpos, const double* scal) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.nw) return;
+    const int id = wids[s];
+    const double t = scal[SC_TIME], dt = scal[SC_DT];
+    double4 a = w0[id];
+    double4 g = w2[id];
+    if (P.nkin > 0) {
+        double va[3] = {0.0, 0.0, 0.0}, om[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < P.nkin; ++j) {
+            const double* r = &kin[7 * j];
+            if (r[0] >= t) {
+                if (r[0] == t) {
+                    for (int d = 0; d < 3; ++d) {
+                        va[d] = r[1 + d];
+                        om[d] = r[4 + d];
+                    }
+                } else {
+                    const double* q = &kin[7 * (j - 1)];
+                    for (int d = 0; d < 3; ++d) {
+                        va[d] = q[1 + d] + (r[1 + d] - q[1 + d]) / (r[0] - q[0]) * (t - q[0]);
+                        om[d] = q[4 + d] + (r[4 + d] - q[4 + d]) / (r[0] - q[0]) * (t - q[0]);
+                    }
+                }
+                break;
+            }
+        }
+        const double rel[3] = {a.x - P.kin_c[0], a.y - P.kin_c[1], a.z - P.kin_c[2]};
+        g.x = (om[1] * rel[2] - om[2] * rel[1]) + va[0];
+        g.y = (om[2] * rel[0] - om[0] * rel[2]) + va[1];
+        g.z = (om[0] * rel[1] - om[1] * rel[0]) + va[2];
+        w2[id] = g;
+    }
+    a.x = a.x + dt * g.x;
+    a.y = a.y + dt * g.y;
+    a.z = a.z + dt * g.z;
+    w0[id] = a;
+    wpos[s] = make_double4(a.x, a.y, a.z, (double)id);
 }
 
 // BC_wall_elements.f90:71-158, one wavefront per wall element: the fluid

@@ -77,7 +77,8 @@ struct DP {
     int nopen; // its open faces (NumOpenFaces)
     // DB-SPH walls (Domain%tipo = "bsph"): nw wall elements; the DBSPH
     // section's Gamma_limiter_flag, FS_allowed and negative_wall_p_allowed
-    int dbsph, nw, gamma_lim, fs_allowed, neg_wall_p, muscl_bnd;
+    int dbsph, nw, gamma_lim, fs_allowed, neg_wall_p, muscl_bnd, nkin;
+    double kin_c[3];  // rotation centre of the wall kinematics
 };
 
 // Segment maps of a segmented context, passed by value to the kernels that

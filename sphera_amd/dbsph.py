@@ -36,20 +36,26 @@ from .case import PIGRECO, Grid, cell_number
 
 @dataclass
 class DBSPHParams:
-    """The DBSPH deck section (ReadDBSPH.f90:35-99) for walls without
-    kinematics, monitors, inlets or outlets, with slip_ID = 0."""
+    """The DBSPH deck section (ReadDBSPH.f90:35-99) for one surface mesh,
+    without monitors, inlets or outlets, with slip_ID = 0.  kinematics: the
+    mesh's records (t, vx, vy, vz, wx, wy, wz), the first at t <= 0, with
+    its rotation centre (DBSPH_kinematics.f90); empty for static walls."""
     dx_dxw: float = 1.0                 # fluid spacing / wall-element spacing
     k_w: float = 0.5                    # semi-particle coefficient
     gamma_limiter: bool = False         # Gamma <= 1
     negative_wall_p_allowed: bool = False
     fs_allowed: bool = True             # free-surface detection via sigma
     muscl_boundary: bool = False        # wall terms in the MUSCL gradients
+    kinematics: tuple = ()              # ((t, vx, vy, vz, wx, wy, wz), ...)
+    rotation_centre: tuple = (0.0, 0.0, 0.0)
 
     def deck_lines(self, nfiles=1):
         b = lambda v: ".true." if v else ".false."  # noqa: E731
         return [f"{float(self.dx_dxw)!r} {b(self.muscl_boundary)} {float(self.k_w)!r} 0 {b(self.gamma_limiter)}",
                 f"{b(self.negative_wall_p_allowed)} {b(self.fs_allowed)}", "0 0",
-                f"{int(nfiles)} .false.", "0 0. 0. 0.", "0 0 4"]
+                f"{int(nfiles)} .false.",
+                " ".join([str(len(self.kinematics))] + [repr(float(v)) for v in self.rotation_centre])] + \
+            [" ".join(repr(float(v)) for v in r) for r in self.kinematics] + ["0 0 4"]
 
 
 def tank_mesh(tank, dx):

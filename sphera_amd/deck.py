@@ -395,13 +395,19 @@ def parse_deck(text, name=None):
         if _int(t[3]) != 0:
             raise ValueError("DB-SPH slip_ID > 0 (the VSL wall model) is not read")
         n_mon = [_int(v) for v in db[2].split()[:2]]
-        n_kin = _int(db[4].split()[0])
-        n_io = [_int(v) for v in db[5].split()[:2]]
-        if any(n_mon) or n_kin or any(n_io) or _int(db[3].split()[0]) != 1:
-            raise ValueError("DB-SPH monitors, kinematics, inlets/outlets and several meshes are not read")
+        kl = db[4].split()
+        n_kin = _int(kl[0])
+        if len(db) < 6 + n_kin:
+            raise ValueError("DBSPH section short")
+        kin = tuple(tuple(_num(v) for v in db[5 + j].split()[:7]) for j in range(n_kin))
+        n_io = [_int(v) for v in db[5 + n_kin].split()[:2]]
+        if any(n_mon) or any(n_io) or _int(db[3].split()[0]) != 1:
+            raise ValueError("DB-SPH monitors, inlets/outlets and several meshes are not read")
         neg, fs = (_logical(v) for v in db[1].split()[:2])
         dbsph = DBSPHParams(dx_dxw=_num(t[0]), k_w=_num(t[2]), gamma_limiter=_logical(t[4]),
-                            negative_wall_p_allowed=neg, fs_allowed=fs, muscl_boundary=_logical(t[1]))
+                            negative_wall_p_allowed=neg, fs_allowed=fs, muscl_boundary=_logical(t[1]),
+                            kinematics=kin,
+                            rotation_centre=tuple(_num(v) for v in kl[1:4]) if n_kin else (0.0, 0.0, 0.0))
         if time_split != 0:
             raise ValueError("DB-SPH walls run with explicit RK1 (time_split = 0)")
     return DeckCase(dbsph=dbsph, name=name or title[0].split()[0], dx=dx, trunc=trunc, rho0=rho0, bulk=bulk,

@@ -23,7 +23,8 @@
 ! faces or sides, open faces (3D) and open sides (2D) of type "open" only, no
 ! dt_alfa_Mon, fluid particles only; or Domain%tipo="bsph" (3D DB-SPH walls,
 ! pg_w passed to sph_dbsph_walls) with explicit RK1, FP64, slip_ID = 0, no
-! DB-SPH inlets, outlets, kinematics or fictitious reservoirs.
+! DB-SPH inlets, outlets or fictitious reservoirs; kinematics records for one
+! mesh.
 module sphera_hip_bridge
 use iso_c_binding
 use sphera_hip_c
@@ -66,8 +67,8 @@ end subroutine sph_hip_check
 
 #ifdef SPACE_3D
 ! True when the DB-SPH case uses what the device does not build: the VSL
-! wall model (slip_ID > 0), inlet or outlet sections,
-! moving walls (kinematics records) or fictitious reservoirs / reservoirs
+! wall model (slip_ID > 0), inlet or outlet sections, kinematics records
+! with more than one mesh, or fictitious reservoirs / reservoirs
 ! from topography (removed in the pre-loop by
 ! removing_DBSPH_fictitious_reservoirs.f90)
 logical function dbsph_outside_scope()
@@ -76,7 +77,8 @@ dbsph_outside_scope = (DBSPH%slip_ID/=0).or.                                  &
    (DBSPH%n_inlet/=0).or.(DBSPH%n_outlet/=0).or.(Domain%time_split/=0).or.      &
    (Domain%RKscheme/=1)
 if (allocated(DBSPH%n_kinematics_records)) then
-   if (any(DBSPH%n_kinematics_records/=0)) dbsph_outside_scope = .true.
+   if ((DBSPH%surface_mesh_files>1).and.any(DBSPH%n_kinematics_records/=0))  &
+      dbsph_outside_scope = .true.
 endif
 do k=1,NPartZone
    if (Partz(k)%DBSPH_fictitious_reservoir_flag.or.(Partz(k)%IC_source_type==2))&
@@ -398,6 +400,25 @@ w%fs_allowed = merge(1,0,DBSPH%FS_allowed)
 w%negative_wall_p_allowed = merge(1,0,DBSPH%negative_wall_p_allowed)
 w%muscl_boundary = merge(1,0,DBSPH%MUSCL_boundary_flag)
 end subroutine pack_walls
+
+! The mesh's kinematics records (DBSPH%kinematics(1,:,1:7)) and rotation
+! centre to sph_dbsph_kinematics (DBSPH_kinematics.f90 on the device)
+subroutine pass_kinematics(ctx)
+type(c_ptr),intent(in) :: ctx
+integer(4) :: j,nr
+double precision,allocatable :: kr(:)
+double precision :: cr(3)
+nr = 0
+if (allocated(DBSPH%n_kinematics_records)) nr = DBSPH%n_kinematics_records(1)
+if (nr<=0) return
+allocate(kr(7*nr))
+do j=1,nr
+   kr(7*(j-1)+1:7*j) = DBSPH%kinematics(1,j,1:7)
+enddo
+cr(:) = DBSPH%rotation_centre(1,1:3)
+call sph_hip_check(sph_dbsph_kinematics(ctx,int(nr,c_int32_t),kr,cr),"sph_dbsph_kinematics")
+deallocate(kr)
+end subroutine pass_kinematics
 #endif
 
 subroutine free_tables
@@ -451,6 +472,7 @@ if (Domain%tipo=="bsph") then
    call sph_hip_check(sph_ctx_create(p,f,ctx),"sph_ctx_create")
    call pack_walls(w)
    call sph_hip_check(sph_dbsph_walls(ctx,w),"sph_dbsph_walls")
+   call pass_kinematics(ctx)
    else
       call pack_faces(f)
       call sph_hip_check(sph_ctx_create(p,f,ctx),"sph_ctx_create")

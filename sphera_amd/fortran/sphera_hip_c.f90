@@ -168,6 +168,13 @@ interface
       type(c_ptr),value :: ctx
       type(sph_dbsph_walls_t),intent(in) :: w
    end function
+   integer(c_int) function sph_dbsph_kinematics(ctx,nrec,records,centre)                 &
+      bind(C,name="sph_dbsph_kinematics")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr),value :: ctx
+      integer(c_int32_t),value :: nrec
+      real(c_double),intent(in) :: records(*), centre(3)
+   end function
    integer(c_int) function sph_preloop(ctx) bind(C,name="sph_preloop")
       import :: c_int, c_ptr
       type(c_ptr),value :: ctx

@@ -35,7 +35,7 @@ EXPORTS = [
     "sph_preloop", "sph_step", "sph_step_async", "sph_step_graph", "sph_sync", "sph_get_launch", "sph_set_timing",
     "sph_get_timing", "sph_ctx_bytes", "sph_device_memory", "sph_query",
     "sph_batch_create", "sph_batch_bytes", "sph_segment_upload", "sph_segment_download",
-    "sph_segment_scalars", "sph_segment_count", "sph_dbsph_walls", "sph_dbsph_state",
+    "sph_segment_scalars", "sph_segment_count", "sph_dbsph_walls", "sph_dbsph_state", "sph_dbsph_kinematics",
     "sph_last_error",
 ]
 
@@ -104,6 +104,7 @@ def load():
     L.sph_global_quantities.argtypes = [C.c_void_p, PD]
     L.sph_global_quantities_noalloc.argtypes = [C.c_void_p, PD]
     L.sph_dbsph_walls.argtypes = [C.c_void_p, C.POINTER(SphDBSPHWalls)]
+    L.sph_dbsph_kinematics.argtypes = [C.c_void_p, C.c_int32, PD, PD]
     L.sph_dbsph_state.argtypes = [C.c_void_p, PD, PD, PD, PD, PI, PD, PD, PI]
     L.sph_monitor_lines.argtypes = [C.c_void_p, C.c_int32, PI, C.c_int32, PD]
     L.sph_free_surface.argtypes = [C.c_void_p, PD, PD]
@@ -288,6 +289,11 @@ class SphContext:
             ws = SphDBSPHWalls(len(W), *[_pd(a) for a in self._wkeep], int(db.gamma_limiter),
                                int(db.fs_allowed), int(db.negative_wall_p_allowed), int(db.muscl_boundary))
             self._check(self.L.sph_dbsph_walls(self.h, C.byref(ws)), "sph_dbsph_walls")
+            if db.kinematics:
+                self._kin = np.ascontiguousarray(db.kinematics, np.float64).reshape(-1, 7)
+                self._kc = np.ascontiguousarray(db.rotation_centre, np.float64)
+                self._check(self.L.sph_dbsph_kinematics(self.h, len(self._kin), _pd(self._kin), _pd(self._kc)),
+                            "sph_dbsph_kinematics")
             return
         if getattr(tb, "dim", 3) == 2:
             ss, self._keep = sides_struct(tb)

@@ -477,7 +477,14 @@ def dbsph_ic(coord, dx, seed=20261207):
 DBSPH_OPTS = {"ref_dbsph.npz": dict(),
               "ref_dbsph_opts.npz": dict(gamma_limiter=True, negative_wall_p_allowed=True,
                                          fs_allowed=False),
-              "ref_dbsph_muscl.npz": dict(muscl_boundary=True)}
+              "ref_dbsph_muscl.npz": dict(muscl_boundary=True),
+              # the tank moves: a translation ramping up to 0.5 m/s along x and a
+              # rotation about the vertical through the tank's floor centre
+              # ramping up to 4 rad/s (wall speeds up to about 1 m/s)
+              "ref_dbsph_moving.npz": dict(kinematics=((0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0),
+                                                       (0.001, 0.5, 0.0, 0.0, 0.0, 0.0, 4.0),
+                                                       (1.0, 0.5, 0.0, 0.0, 0.0, 0.0, 4.0)),
+                                           rotation_centre=(0.15, 0.05, 0.0))}
 
 
 def make_dbsph(names=None):

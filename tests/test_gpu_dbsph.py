@@ -37,7 +37,8 @@ def _ctx(tb, coord=None, vel=None):
     return d
 
 
-@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz", "ref_dbsph_muscl.npz"])
+@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz", "ref_dbsph_muscl.npz",
+                                  "ref_dbsph_moving.npz"])
 def test_dbsph_matches_reference_fixture(name):
     """The reference's DB-SPH run (700 particles in a 3,800-element tank
     mesh, jittered and moving): the device equals it bit for bit after the
@@ -60,12 +61,13 @@ def test_dbsph_matches_reference_fixture(name):
     d.close()
 
 
-def test_dbsph_longer_oracle_and_graph():
-    """A larger tank (1,320 particles, 8,800 wall elements) for 60 steps:
-    the device equals the oracle bit for bit, and six replays of a captured
-    10-step graph equal 60 eager steps."""
+@pytest.mark.parametrize("fixture", ["ref_dbsph.npz", "ref_dbsph_moving.npz"])
+def test_dbsph_longer_oracle_and_graph(fixture):
+    """A larger tank (1,320 particles, 8,800 wall elements) for 60 steps,
+    static and moving: the device equals the oracle bit for bit, and six
+    replays of a captured 10-step graph equal 60 eager steps."""
     from oracle.pyoracle import Oracle
-    tb = build(_case(tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), name="small"))
+    tb = build(_case(fixture, tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), name="small"))
     rng = np.random.default_rng(11)
     c = tb.particles["coord"]
     coord = c + rng.uniform(-0.01, 0.01, c.shape) * tb.case.dx
@@ -91,9 +93,17 @@ def test_dbsph_longer_oracle_and_graph():
 
 
 def test_dbsph_preconditions_refused():
-    """FP32-mixed and leapfrog contexts refuse DB-SPH walls (status 95, 2)."""
+    """FP32-mixed and leapfrog contexts refuse DB-SPH walls (status 95, 2),
+    and kinematics records that start after t = 0 are refused (status 2)."""
+    import dataclasses
     from sphera_amd import runtime as rt
     tb = build(_case())
+    late = dataclasses.replace(tb.case.dbsph, kinematics=((0.1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0),))
+    tb.case.dbsph = late
+    with pytest.raises(rt.SphError) as e:
+        rt.SphContext(tb, precision=rt.FP64)
+    assert e.value.code == 2
+    tb.case.dbsph = dataclasses.replace(late, kinematics=())
     with pytest.raises(rt.SphError) as e:
         rt.SphContext(tb, precision=rt.FP32_MIXED)
     assert e.value.code == 95

@@ -167,14 +167,14 @@ def test_bridge_2d_open_side(tmp_path):
             assert err <= FP64_REL_TOL, (it, f, err)
 
 
-@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_muscl.npz"])
+@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_muscl.npz", "ref_dbsph_moving.npz"])
 def test_bridge_dbsph(tmp_path, name):
     """The reference program with DB-SPH walls (Domain%tipo = "bsph", RK1)
     driving the device through the bridge: the bridge passes pg_w, as the
     reference's own pre-processing built it from the tank's .ply mesh, to
     sph_dbsph_walls; count and fields after 10 and 20 steps equal the
-    reference's own run (tests/golden/ref_dbsph*.npz) bit for bit, without
-    and with the MUSCL wall terms."""
+    reference's own run (tests/golden/ref_dbsph*.npz) bit for bit: static
+    walls, with the MUSCL wall terms, and a moving tank (kinematics records)."""
     from sphera_amd.dbsph import DBSPHParams
     from tests.golden.make_golden import DBSPH_OPTS, TINY
     g = np.load(GOLD / name)

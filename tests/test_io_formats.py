@@ -187,7 +187,10 @@ def test_dbsph_case_directory_round_trip(tmp_path):
     from sphera_amd.dbsph import DBSPHParams, read_ply, tank_mesh
     case = DamBreak3D(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="rtdb", time_split=0,
                       dbsph=DBSPHParams(gamma_limiter=True, fs_allowed=False, negative_wall_p_allowed=True,
-                                        muscl_boundary=True))
+                                        muscl_boundary=True,
+                                        kinematics=((0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0),
+                                                    (0.5, 0.25, 0.0, -0.125, 0.0, 0.0, 2.0)),
+                                        rotation_centre=(0.15, 0.05, 0.0)))
     write_case_dir(case, tmp_path)
     v, f = read_ply(tmp_path / "input" / "07_DBSPH" / "tank.ply")
     v0, f0 = tank_mesh(case.tank, case.dx)

@@ -509,7 +509,8 @@ def test_dbsph_wall_elements_equal_reference():
     assert (W.k_d[edge] > 1.2).all()
 
 
-@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz", "ref_dbsph_muscl.npz"])
+@pytest.mark.parametrize("name", ["ref_dbsph.npz", "ref_dbsph_opts.npz", "ref_dbsph_muscl.npz",
+                                  "ref_dbsph_moving.npz"])
 def test_golden_dbsph_bit_identical(name):
     """DB-SPH walls (Domain%tipo = "bsph", RK1): the oracle's restatement of
     the fluid-wall lists and Shepard sums (CalcVarLength.f90:462-540,
@@ -524,7 +525,9 @@ def test_golden_dbsph_bit_identical(name):
     pre-loop).  The second fixture runs the Gamma limiter, negative wall
     pressures and no free-surface detection; the third the wall elements'
     terms in the MUSCL gradients (Gradients_to_MUSCL_boundary.f90), whose
-    gradients and wall pressures differ from the first's at every step."""
+    gradients and wall pressures differ from the first's at every step; the
+    fourth a moving tank (DBSPH_kinematics.f90: an interpolated translation
+    and rotation), whose fluid-wall lists differ from the first's."""
     g = np.load(GOLD / name)
     o = Oracle(build(_dbsph_case(name)), coord=g["ic_coord"], vel=g["ic_vel"])
     o.preloop()
@@ -548,8 +551,22 @@ def test_golden_dbsph_bit_identical(name):
         assert g["s20_fs"].sum() > 0 and (g["s20_wall_pres"] > 0).sum() > 100
     elif name == "ref_dbsph_opts.npz":
         assert g["s20_gamma"].max() == 1.0 and (g["s20_wall_pres"] < 0).any()
+    elif name == "ref_dbsph_moving.npz":  # the moving walls change the lists
+        p = np.load(GOLD / "ref_dbsph.npz")
+        for it in (10, 20):
+            assert (g[f"s{it}_fw_count"] != p[f"s{it}_fw_count"]).sum() > 100, it
     else:  # the wall terms change the gradients and the wall state
         p = np.load(GOLD / "ref_dbsph.npz")
         for it in (10, 20):
             for f in ("drho", "dvel", "wall_pres"):
                 assert not np.allclose(g[f"s{it}_" + f], p[f"s{it}_" + f], rtol=1e-3, atol=0), (it, f)
+
+
+def test_dbsph_kinematics_starting_late_refused():
+    """DBSPH_kinematics.f90 reads the record before the first one at or after
+    the current time, so a first record after t = 0 is refused."""
+    import dataclasses
+    case = _dbsph_case("ref_dbsph.npz")
+    case.dbsph = dataclasses.replace(case.dbsph, kinematics=((0.1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0),))
+    with pytest.raises(ValueError, match="first record"):
+        Oracle(build(case))
