@@ -9,7 +9,11 @@ against the FP64 oracle after the same steps, and times the reference's own
 DB-SPH build (flang OpenMP, through the dump harness) on the same case on
 the host cores.  One JSON line on stdout.
 
-    python bench_dbsph.py [--steps 200] [--oracle-steps 20] [--ref-steps 10]
+    python bench_dbsph.py [--steps 200] [--oracle-steps 20] [--ref-steps 10] [--moving]
+
+--moving: the tank moves (kinematics records: a translation ramping up to
+0.5 m/s along x and a rotation ramping up to 1 rad/s about the vertical
+through the floor's centre).
 """
 from __future__ import annotations
 
@@ -34,9 +38,14 @@ from sphera_amd.dbsph import DBSPHParams  # noqa: E402
 FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
 
 
-def _case(dx):
+MOVING = dict(kinematics=((0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0), (0.01, 0.5, 0.0, 0.0, 0.0, 0.0, 1.0),
+                          (100.0, 0.5, 0.0, 0.0, 0.0, 0.0, 1.0)),
+              rotation_centre=(0.8, 0.3, 0.0))
+
+
+def _case(dx, moving=False):
     return DamBreak3D(tank=(1.6, 0.6, 0.6), column=(0.4, 0.6, 0.3), dx=dx, name="dam_break_dbsph",
-                      time_split=0, dbsph=DBSPHParams())
+                      time_split=0, dbsph=DBSPHParams(**(MOVING if moving else {})))
 
 
 def _ctx(tb, coord):
@@ -54,8 +63,9 @@ def main(argv=None):
     ap.add_argument("--ref-steps", type=int, default=10,
                     help="steps of the reference's timed sample (difference of two runs)")
     ap.add_argument("--no-ref", action="store_true")
+    ap.add_argument("--moving", action="store_true", help="moving tank (kinematics records)")
     a = ap.parse_args(argv)
-    case = _case(a.dx)
+    case = _case(a.dx, a.moving)
     t0 = time.perf_counter()
     tb = build(case)
     t_pre = time.perf_counter() - t0
@@ -64,7 +74,8 @@ def main(argv=None):
     coord = c + rng.uniform(-0.01, 0.01, c.shape) * case.dx
     n, nw = len(coord), len(tb.walls)
     print(f"[bench_dbsph] n={n} walls={nw} (wall pre-processing {t_pre:.1f} s)", file=sys.stderr, flush=True)
-    out = dict(workload=f"3D dam break with DB-SPH walls: 1.6 x 0.6 x 0.6 m tank mesh ({nw} wall elements), "
+    out = dict(workload=f"3D dam break with DB-SPH walls: 1.6 x 0.6 x 0.6 m tank mesh ({nw} wall elements"
+                        f"{', moving' if a.moving else ''}), "
                         f"0.4 x 0.6 x 0.3 m column, dx {case.dx} m, {n} particles, explicit RK1",
                dtype="f64", n_particles=n, n_walls=nw, steps=a.steps)
     # parity: the device against the oracle after the same steps
