@@ -21,12 +21,20 @@ FIELDS = ("coord", "vel", "acc", "dens", "pres", "dden", "vpres")
 DBF = ("gamma", "sigma", "rho_sph", "dshep", "fs", "wall_dens", "wall_pres", "wall_wet")
 
 
+# moving walls, the MUSCL wall terms and the Gamma limiter together
+COMBINED = "moving+muscl+gamma_limiter"
+
+
 def _case(fixture="ref_dbsph.npz", **kw):
     from sphera_amd.dbsph import DBSPHParams
     from tests.golden.make_golden import DBSPH_OPTS
     base = dict(tank=(0.3, 0.1, 0.2), column=(0.08, 0.1, 0.1), dx=0.01, name="tiny")
     base.update(kw)
-    return DamBreak3D(**base, time_split=0, dbsph=DBSPHParams(**DBSPH_OPTS[fixture]))
+    if fixture == COMBINED:
+        opts = dict(DBSPH_OPTS["ref_dbsph_moving.npz"], muscl_boundary=True, gamma_limiter=True)
+    else:
+        opts = DBSPH_OPTS[fixture]
+    return DamBreak3D(**base, time_split=0, dbsph=DBSPHParams(**opts))
 
 
 def _ctx(tb, coord=None, vel=None):
@@ -61,11 +69,12 @@ def test_dbsph_matches_reference_fixture(name):
     d.close()
 
 
-@pytest.mark.parametrize("fixture", ["ref_dbsph.npz", "ref_dbsph_moving.npz"])
+@pytest.mark.parametrize("fixture", ["ref_dbsph.npz", "ref_dbsph_moving.npz", COMBINED])
 def test_dbsph_longer_oracle_and_graph(fixture):
     """A larger tank (1,320 particles, 8,800 wall elements) for 60 steps,
-    static and moving: the device equals the oracle bit for bit, and six
-    replays of a captured 10-step graph equal 60 eager steps."""
+    static, moving, and moving with the MUSCL wall terms and the Gamma
+    limiter: the device equals the oracle bit for bit, and six replays of a
+    captured 10-step graph equal 60 eager steps."""
     from oracle.pyoracle import Oracle
     tb = build(_case(fixture, tank=(0.4, 0.2, 0.3), column=(0.12, 0.1, 0.12), name="small"))
     rng = np.random.default_rng(11)
